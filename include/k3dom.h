@@ -1,0 +1,198 @@
+/* C ABI of the MI355X-native K3DOM per-frame dynamic occupancy update.
+ *
+ * One handle owns one map: its device buffers (structure-of-arrays voxel
+ * masses and particles), one HIP stream and the per-frame launch plan.  Every
+ * call is synchronous with respect to the host unless its name ends in
+ * _async.  Handles are not thread-safe; distinct handles may be used from
+ * distinct threads.  Every function returns a k3dom_status; the text of the
+ * last error on a handle is available from k3dom_last_error().
+ *
+ * Each entry point replaces one member of the reference's C++ filter class
+ * (youngjae-min/k3dom, namespace dom, classes DOM_c and DOM).  The reference
+ * has no C ABI of its own: its ROS node links the .cu sources directly.
+ */
+#ifndef K3DOM_H
+#define K3DOM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum k3dom_status {
+    K3DOM_OK = 0,
+    K3DOM_ERR_INVALID_ARG = 1,   /* bad pointer, size or parameter value      */
+    K3DOM_ERR_HIP = 2,           /* a HIP runtime call failed                  */
+    K3DOM_ERR_OUT_OF_MEMORY = 3, /* device or host allocation failed           */
+    K3DOM_ERR_STATE = 4          /* call not valid in the handle's state       */
+} k3dom_status;
+
+/* Filter parameters: field-for-field the reference's dom::DOM_c::Params
+ * (include/common/dom.h:19-52), in the units DOM_c expects (the ROS node
+ * divides the stddevs and velocities by the resolution and converts
+ * map_shift_thresh to voxels, src/k3dom/k3dom_node.cpp:344-368;
+ * k3dom_params_from_yaml() performs that conversion).  `seed` keys the
+ * counter-based random generator (the reference seeds cuRAND with 123456,
+ * src/common/dom.cu:96). */
+typedef struct k3dom_params {
+    float size;
+    float size_z;
+    float mass_scale;
+    float resolution;
+    int32_t particle_count;
+    int32_t new_born_particle_count;
+    float persistence_prob;
+    float stddev_process_noise_position;
+    float stddev_process_noise_velocity;
+    float birth_prob;
+    float stddev_velocity;
+    float init_max_velocity;
+    float particle_min_vel;
+    float sensor_off_x;
+    float sensor_off_y;
+    float sensor_off_z;
+    int32_t map_shift_thresh;
+    float prior_free;
+    float prior_static;
+    float prior_dynamic;
+    float prior_occ;
+    float sigma;
+    float ls;
+    float gamma;
+    float max_range;
+    float meas_occ_stddev;
+    uint64_t seed;
+} k3dom_params;
+
+/* Readback record with the field order, types and size (68 bytes) of the
+ * reference's dom::GridCell (include/common/dom_types.h:12-45). */
+typedef struct k3dom_cell {
+    int32_t start_idx;
+    int32_t end_idx;
+    float mu_A;
+    float mu_UA;
+    float w_A;
+    float w_UA;
+    float mean_x_vel;
+    float mean_y_vel;
+    float mean_z_vel;
+    float likelihood;
+    float p_A;
+    float free_mass;
+    float static_mass;
+    float dynamic_mass;
+    float occ_mass;
+    float pers_mass;
+    float pred_mass;
+} k3dom_cell;
+
+/* Geometry and pose of a map, as returned by the reference's getters
+ * (include/common/dom.h:59-83). */
+typedef struct k3dom_info {
+    int32_t grid_size;
+    int32_t grid_size_z;
+    int32_t grid_cell_count;
+    int32_t particle_count;
+    int32_t new_born_particle_count;
+    int32_t iteration;          /* frames processed by k3dom_update_grid      */
+    float resolution;
+    float center_pos[3];
+    float sensor_pos[3];
+    double updated_time;        /* time of the last frame, -1 before the first */
+} k3dom_info;
+
+typedef struct k3dom_handle k3dom_handle;
+
+/* Fills *out with the reference node's defaults (src/k3dom/k3dom_node.cpp:215-258)
+ * after unit conversion, seed 123456. */
+k3dom_status k3dom_params_default(k3dom_params* out);
+
+/* Reads a method YAML and then a dataset YAML (either path may be NULL),
+ * flat `key: value` pairs as in config/k3dom.yaml and config/complex.yaml, with
+ * the node's defaults for missing keys and its unit conversion
+ * (src/k3dom/k3dom_node.cpp:215-368; rosparam load order of
+ * launch/k3dom_demo.launch:33-36).  Writes the error text into err (capacity
+ * err_len) on failure. */
+k3dom_status k3dom_params_from_yaml(const char* method_yaml, const char* dataset_yaml,
+                                    k3dom_params* out, char* err, int32_t err_len);
+
+/* Replaces dom::DOM::DOM / DOM_c::DOM_c + initialize (src/k3dom/dom.cu:20-23,
+ * src/common/dom.cu:29-66,90-109): allocates the map on the current HIP device
+ * and sets every voxel to the priors. */
+k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out);
+
+/* Replaces DOM_c::~DOM_c (src/common/dom.cu:68-88).  NULL is accepted. */
+k3dom_status k3dom_destroy(k3dom_handle* h);
+
+/* Replaces DOM_c::updatePose (src/common/dom.cu:111-158): sets the sensor
+ * position (global frame, metres) and shifts the map by whole voxels when the
+ * sensor has moved map_shift_thresh voxels or more. */
+k3dom_status k3dom_update_pose(k3dom_handle* h, float x, float y, float z);
+
+/* Replaces DOM::updateGrid (src/k3dom/dom.cu:29-50): one filter frame at time t
+ * (seconds) with n points given relative to the sensor in map-aligned axes
+ * (host arrays).  The first frame (and any frame while the stored time is
+ * <= 0, as in the reference) initialises the particles instead. */
+k3dom_status k3dom_update_grid(k3dom_handle* h, double t, const float* xs, const float* ys,
+                               const float* zs, int32_t n);
+
+/* As k3dom_update_grid with the three point arrays already in device memory
+ * (no host copy of the points). */
+k3dom_status k3dom_update_grid_device(k3dom_handle* h, double t, const float* d_xs,
+                                      const float* d_ys, const float* d_zs, int32_t n);
+
+/* Geometry, pose, time and frame count (include/common/dom.h:59-83). */
+k3dom_status k3dom_get_info(const k3dom_handle* h, k3dom_info* out);
+
+/* Copies every voxel into a host array of grid_cell_count GridCell-compatible
+ * records, in the reference's index order i = x + gs*y + gs*gs*z.  This is the
+ * drop-in for the caller-side cudaMemcpy of grid_cell_array
+ * (src/k3dom/k3dom_node.cpp:131-132,162-163). */
+k3dom_status k3dom_read_cells(k3dom_handle* h, k3dom_cell* out);
+
+/* Structure-of-arrays readback of the voxel masses and mean velocity (any
+ * pointer may be NULL).  Velocities are in voxels per second as in GridCell. */
+k3dom_status k3dom_read_masses(k3dom_handle* h, float* free_mass, float* static_mass,
+                               float* dynamic_mass, float* mean_vx, float* mean_vy, float* mean_vz);
+
+/* Internal arrays of the last frame, for parity checks and diagnostics. */
+typedef enum k3dom_array {
+    K3DOM_ARR_PARTICLE_POS = 0,  /* 3*N float: x[N], y[N], z[N] (current set)     */
+    K3DOM_ARR_PARTICLE_VEL = 1,  /* 3*N float                                      */
+    K3DOM_ARR_PARTICLE_WEIGHT = 2, /* N float                                      */
+    K3DOM_ARR_PARTICLE_KEY = 3,  /* N int32 voxel keys (copied with the particle)  */
+    K3DOM_ARR_SORT_PERM = 4,     /* N int32: sorted position -> predicted index    */
+    K3DOM_ARR_SORTED_KEY = 5,    /* N int32 keys after the stable sort             */
+    K3DOM_ARR_RESAMPLE_IDX = 6,  /* N int32 indices into [persistent ++ newborn]   */
+    K3DOM_ARR_NEWBORN_POS = 7,   /* 3*B float                                      */
+    K3DOM_ARR_NEWBORN_VEL = 8,   /* 3*B float                                      */
+    K3DOM_ARR_NEWBORN_WEIGHT = 9,/* B float (after the static penalty)             */
+    K3DOM_ARR_NEWBORN_KEY = 10,  /* B int32                                        */
+    K3DOM_ARR_BORN_MASS = 11,    /* V float                                        */
+    K3DOM_ARR_DEL_OCC = 12,      /* V float occupied evidence of the last frame    */
+    K3DOM_ARR_DEL_FREE = 13,     /* V float free evidence of the last frame        */
+    K3DOM_ARR_SORTED_WEIGHT = 14 /* N float weights of the sorted set after the    */
+                                 /* persistent update and static penalty            */
+} k3dom_array;
+
+/* Copies internal array `which` into host memory dst of capacity bytes. */
+k3dom_status k3dom_read_array(k3dom_handle* h, int32_t which, void* dst, int64_t capacity);
+
+/* Milliseconds spent in each stage of the last frame, measured with HIP
+ * events on the handle's stream: [0] predict, [1] sort and ranges,
+ * [2] point preparation and candidate lists, [3] kernel inference (BGKI),
+ * [4] mass update, [5] persistent update, [6] newborn, [7] moments,
+ * [8] resampling, [9] whole frame.  Timing is recorded only after
+ * k3dom_set_timing(h, 1). */
+k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable);
+k3dom_status k3dom_stage_times(k3dom_handle* h, float* ms, int32_t count);
+
+/* Text of the last error on h (or of the last failed create when h is NULL). */
+const char* k3dom_last_error(const k3dom_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* K3DOM_H */

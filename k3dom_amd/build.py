@@ -1,0 +1,56 @@
+"""Builds the in-tree shared libraries of the package.
+
+libk3dom.so    the product: gfx950 kernels + C ABI (hipcc --offload-arch=gfx950)
+libk3scene.so  the synthetic box-scene / LiDAR generator used by the benchmark
+               and the tests (host C++)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # bit-reproducible FP32: no contraction, IEEE divide/sqrt, denormals kept
+    "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    # packed FP32 instructions disabled
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib(verbose_asm=False):
+    out = os.path.join(HERE, "libk3dom.so")
+    srcs = [os.path.join(CSRC, "k3_kernels.hip"), os.path.join(CSRC, "k3_api.cpp")]
+    cmd = ["hipcc", *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", *srcs]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_scene():
+    out = os.path.join(HERE, "libk3scene.so")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
+           os.path.join(CSRC, "k3_scene.cpp"), "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all():
+    build_scene()
+    build_lib()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "lib": build_lib, "scene": build_scene}[w]()

@@ -1,0 +1,419 @@
+// Deterministic FP32 arithmetic shared by the gfx950 kernels and the host-side
+// serial restatement of the update step.
+//
+// Every function here uses only IEEE-754 operations that are correctly rounded
+// on both x86-64 (gcc, -ffp-contract=off) and gfx950 (hipcc, -ffp-contract=off,
+// correctly rounded f32 divide/sqrt): +, -, *, /, sqrtf, fmaf, rintf, min/max
+// and integer bit manipulation.  No libm/ocml transcendental is called, so the
+// same inputs give the same bits on the CPU and on the GPU.
+//
+// Reference formulas restated here (youngjae-min/k3dom):
+//   kernel_func / distance_with_margin / BGKI  src/k3dom/kernel/mass_update.cu:24-77
+//   calc_credit / positive_diff / calc_ratio    src/k3dom/kernel/mass_update.cu:12-22
+//   predict_mass / update_mass                  src/k3dom/kernel/mass_update.cu:79-139
+//   separate_newborn_part                       src/k3dom/kernel/mass_update.cu:141-150
+//   particle key                                src/common/kernel/predict.cu:51-55
+// The reference draws its noise from cuRAND XORWOW; this code uses the
+// counter-based Philox4x32-10 generator (Salmon, Moraes, Dror, Shaw,
+// "Parallel random numbers: as easy as 1, 2, 3", SC'11) and the Box-Muller
+// transform (Box & Muller 1958) with the polynomial ln/sin/cos below.
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define K3_HD __host__ __device__ __forceinline__
+#else
+#define K3_HD static inline
+#endif
+
+// ---------------------------------------------------------------- constants
+// sin(pi r) = r (S1 + r^2 (S3 + r^2 (S5 + r^2 (S7 + r^2 S9))))  and
+// cos(pi r) = 1 + r^2 (C2 + ... + r^2 C10): Taylor coefficients pi^k/k!
+// rounded to float.  For |r| <= 1/4 the truncation error is below 2e-9.
+#define K3_S1 3.1415927410125732f
+#define K3_S3 (-5.167712688446045f)
+#define K3_S5 2.550163984298706f
+#define K3_S7 (-0.5992645025253296f)
+#define K3_S9 0.08214588463306427f
+#define K3_C2 (-4.934802055358887f)
+#define K3_C4 4.058712005615234f
+#define K3_C6 (-1.3352627754211426f)
+#define K3_C8 0.23533062636852264f
+#define K3_C10 (-0.025806890800595284f)
+// Cody-Waite split of ln 2: LN2_HI has 16 significant bits so k*LN2_HI is exact.
+#define K3_LN2_HI 0.693145751953125f
+#define K3_LN2_LO 1.428606765330187e-06f
+#define K3_LOG2E 1.4426950216293335f
+#define K3_SQRT2 1.4142135381698608f
+// Kernel-function constants: 1/3 and 1/(2*3.141592) (the reference divides by
+// 2.0f*3.141592f, a truncated pi, at src/k3dom/kernel/mass_update.cu:25).
+#define K3_THIRD 0.3333333432674408f
+#define K3_INV_2PI_T 0.1591549813747406f
+
+// Philox4x32-10 round multipliers and Weyl key increments (Salmon et al. 2011).
+#define K3_PHILOX_M0 0xD2511F53u
+#define K3_PHILOX_M1 0xCD9E8D57u
+#define K3_PHILOX_W0 0x9E3779B9u
+#define K3_PHILOX_W1 0xBB67AE85u
+
+// BGKI sums are two-level: terms are added in ascending point index inside
+// aligned chunks of 2^K3_BGKI_CHUNK_LOG2 points, and the chunk partial sums are
+// added in ascending chunk order.  Both the GPU and the serial restatement use
+// this association, so the sum does not depend on how the work is split.
+#define K3_BGKI_CHUNK_LOG2 10
+
+// Philox stream ids (third counter word) of the random draws of one frame.
+#define K3_STREAM_PREDICT 1u
+#define K3_STREAM_NEWBORN 2u
+#define K3_STREAM_INIT 3u
+#define K3_STREAM_RESAMPLE 4u
+
+K3_HD uint32_t k3_f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
+K3_HD float k3_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+K3_HD double k3_pow2d(int n) { return __builtin_bit_cast(double, (uint64_t)(n + 1023) << 52); }  // n in [-1022,1023]
+
+// ---------------------------------------------------------------- Philox4x32-10
+K3_HD void k3_philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                            uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#ifdef __HIPCC__
+#pragma unroll
+#endif
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)K3_PHILOX_M0 * c0;
+        uint64_t p1 = (uint64_t)K3_PHILOX_M1 * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += K3_PHILOX_W0;
+        k1 += K3_PHILOX_W1;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// 24-bit uniform in [0, 1)
+K3_HD float k3_uniform01(uint32_t r) { return (float)(r >> 8) * 0x1p-24f; }
+// 24-bit uniform in (0, 1]
+K3_HD float k3_uniform01_open0(uint32_t r) { return (float)((r >> 8) + 1u) * 0x1p-24f; }
+
+// ---------------------------------------------------------------- sin/cos(pi t)
+// Exact reduction t = n/2 + r with |r| <= 1/4, then the Taylor polynomials.
+K3_HD void k3_sincospi(float t, float* sn, float* cs) {
+    float n = rintf(t * 2.0f);
+    float r = t - n * 0.5f;
+    float r2 = r * r;
+    float ps = fmaf(r2, K3_S9, K3_S7);
+    ps = fmaf(r2, ps, K3_S5);
+    ps = fmaf(r2, ps, K3_S3);
+    ps = fmaf(r2, ps, K3_S1);
+    ps = ps * r;
+    float pc = fmaf(r2, K3_C10, K3_C8);
+    pc = fmaf(r2, pc, K3_C6);
+    pc = fmaf(r2, pc, K3_C4);
+    pc = fmaf(r2, pc, K3_C2);
+    pc = fmaf(r2, pc, 1.0f);
+    int q = ((int)n) & 3;
+    float s = (q == 0) ? ps : (q == 1) ? pc : (q == 2) ? -ps : -pc;
+    float c = (q == 0) ? pc : (q == 1) ? -ps : (q == 2) ? -pc : ps;
+    *sn = s;
+    *cs = c;
+}
+
+// ---------------------------------------------------------------- ln(x), x > 0
+// x = 2^e m with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m-1)/(m+1),
+// by the atanh series up to s^11 (|s| <= 0.172, truncation < 1e-10).
+K3_HD float k3_ln(float x) {
+    int e = 0;
+    if (x < 0x1p-126f) { x = x * 0x1p24f; e = -24; }
+    uint32_t b = k3_f2u(x);
+    e += (int)((b >> 23) & 0xffu) - 127;
+    float m = k3_u2f((b & 0x7fffffu) | 0x3f800000u);
+    if (m > K3_SQRT2) { m = m * 0.5f; e += 1; }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float s2 = s * s;
+    float p = fmaf(s2, 0.09090909361839294f /*1/11*/, 0.1111111119389534f /*1/9*/);
+    p = fmaf(s2, p, 0.1428571492433548f /*1/7*/);
+    p = fmaf(s2, p, 0.20000000298023224f /*1/5*/);
+    p = fmaf(s2, p, 0.3333333432674408f /*1/3*/);
+    p = p * s2;
+    float two_s = s + s;
+    float lnm = fmaf(two_s, p, two_s);
+    float fe = (float)e;
+    return fmaf(fe, K3_LN2_HI, fmaf(fe, K3_LN2_LO, lnm));
+}
+
+// ---------------------------------------------------------------- exp(x)
+// Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, Taylor polynomial to r^8.
+K3_HD float k3_pow2i(int n) { return k3_u2f((uint32_t)(n + 127) << 23); }  // n in [-126,127]
+K3_HD float k3_exp(float x) {
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return k3_u2f(0x7f800000u);
+    if (x < -103.97208f) return 0.0f;
+    float k = rintf(x * K3_LOG2E);
+    float r = fmaf(-k, K3_LN2_HI, x);
+    r = fmaf(-k, K3_LN2_LO, r);
+    float p = fmaf(r, 2.4801587642822415e-05f /*1/8!*/, 0.0001984127011382952f /*1/7!*/);
+    p = fmaf(r, p, 0.0013888889225199819f /*1/6!*/);
+    p = fmaf(r, p, 0.008333333767950535f /*1/5!*/);
+    p = fmaf(r, p, 0.0416666679084301f /*1/4!*/);
+    p = fmaf(r, p, 0.1666666716337204f /*1/3!*/);
+    p = fmaf(r, p, 0.5f);
+    p = fmaf(r, p, 1.0f);
+    p = fmaf(r, p, 1.0f);
+    int ki = (int)k;
+    int k1 = ki / 2;
+    int k2 = ki - k1;
+    return (p * k3_pow2i(k1)) * k3_pow2i(k2);
+}
+
+// ---------------------------------------------------------------- Box-Muller
+K3_HD void k3_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) {
+    float u1 = k3_uniform01_open0(a);
+    float u2 = k3_uniform01(b);
+    float r = sqrtf(-2.0f * k3_ln(u1));
+    float s, c;
+    k3_sincospi(u2 + u2, &s, &c);
+    *n0 = r * c;
+    *n1 = r * s;
+}
+
+// ---------------------------------------------------------------- kernel inference
+// Sparse kernel of src/k3dom/kernel/mass_update.cu:24-26:
+//   k(d) = max(0, sigma * ((2 + cospi(2d/ls)) (1 - d/ls) / 3 + sinpi(2d/ls) / (2*3.141592)))
+// with q = d * (1/ls) and the two divisions by constants taken as products
+// with the rounded reciprocals.
+K3_HD float k3_kernel_value(float d, float inv_ls, float sigma) {
+    float q = d * inv_ls;
+    float t = q + q;
+    float s, c;
+    k3_sincospi(t, &s, &c);
+    float a = (2.0f + c) * (1.0f - q);
+    float v = sigma * fmaf(a, K3_THIRD, s * K3_INV_2PI_T);
+    return fmaxf(0.0f, v);
+}
+
+// distance_with_margin, src/k3dom/kernel/mass_update.cu:29-31
+K3_HD float k3_margin(float distance, float resolution) {
+    return fmaxf(0.9f * distance, distance - 2.0f * resolution);
+}
+
+// Per-point values of one measurement that every voxel reuses (one 48-byte
+// record per point).  p = (x, y, z) is the point relative to the sensor in
+// map-aligned axes; dm = |p|; mg = margin(dm); e = (mg/dm) p is the end of the
+// free segment; inv_dm = 1/dm (NaN when dm == 0, which disables the free term
+// exactly as the reference's 0/0 does).
+struct K3PointRec {
+    float x, y, z, inv_dm;
+    float dm, mg, ex, ey;
+    float ez, valid, pad0, pad1;
+};
+
+// valid = 0 for points with a non-finite coordinate or |p|^2 overflow: such a
+// point contributes nothing.
+K3_HD K3PointRec k3_point_prep(float x, float y, float z, float resolution) {
+    K3PointRec r;
+    float ss = fmaf(z, z, fmaf(y, y, x * x));
+    bool ok = (x - x == 0.0f) && (y - y == 0.0f) && (z - z == 0.0f) && (ss - ss == 0.0f);
+    float dm = sqrtf(ss);
+    float mg = k3_margin(dm, resolution);
+    float resize = mg / dm;
+    r.x = x; r.y = y; r.z = z;
+    r.inv_dm = (dm > 0.0f) ? 1.0f / dm : k3_u2f(0x7fc00000u);
+    r.dm = dm; r.mg = mg;
+    r.ex = resize * x; r.ey = resize * y; r.ez = resize * z;
+    r.valid = ok ? 1.0f : 0.0f;
+    r.pad0 = 0.0f; r.pad1 = 0.0f;
+    return r;
+}
+
+// Conservative bound for sqrtf(s) < ls: s >= ls2g implies sqrtf(s) >= ls.
+K3_HD float k3_ls2_guard(float ls) {
+    return (float)((double)ls * (double)ls * (1.0 + 0x1p-20));
+}
+
+// Occupied-evidence distance of BGKI (src/k3dom/kernel/mass_update.cu:49):
+// returns d if d < ls, else a negative value.
+K3_HD float k3_occ_distance(float vx, float vy, float vz, const K3PointRec& p, float ls, float ls2g) {
+    float dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
+    float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    if (!(s < ls2g)) return -1.0f;
+    float d = sqrtf(s);
+    return (d < ls) ? d : -1.0f;
+}
+
+// Free-evidence distance of BGKI (src/k3dom/kernel/mass_update.cu:54-68):
+// distance from the voxel centre to the free segment [0, e] of the ray, with
+// the region beyond the measured point excluded.  Returns d if d < ls, else a
+// negative value.  dc = |v|, dc2 = dc*dc.  The perpendicular distance squared
+// is clamped at 0 (the reference can take sqrtf of a rounding-negative value
+// and drop the term).
+K3_HD float k3_free_distance(float vx, float vy, float vz, float dc, float dc2,
+                             const K3PointRec& p, float ls, float ls2g) {
+    float dp = fmaf(vz, p.z, fmaf(vy, p.y, vx * p.x)) * p.inv_dm;
+    float d;
+    if (dp <= 0.0f) {
+        d = dc;
+    } else if (dp < p.mg) {
+        float a = fmaxf(0.0f, fmaf(-dp, dp, dc2));
+        if (!(a < ls2g)) return -1.0f;
+        d = sqrtf(a);
+    } else if (dp < p.dm) {
+        float dx = vx - p.ex, dy = vy - p.ey, dz = vz - p.ez;
+        float a = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (!(a < ls2g)) return -1.0f;
+        d = sqrtf(a);
+    } else {
+        return -1.0f;
+    }
+    return (d < ls) ? d : -1.0f;
+}
+
+// Voxel centre relative to the sensor, src/k3dom/kernel/mass_update.cu:92-99.
+K3_HD void k3_voxel_center(int cell, int gs, int gsz, float res, float sx, float sy, float sz,
+                           float* x, float* y, float* z) {
+    float xi = (float)(cell % gs) + 0.5f;
+    float yi = (float)((cell % (gs * gs)) / gs) + 0.5f;
+    float zi = (float)(cell / (gs * gs)) + 0.5f;
+    *x = (xi - (float)gs / 2.0f) * res - sx;
+    *y = (yi - (float)gs / 2.0f) * res - sy;
+    *z = (zi - (float)gsz / 2.0f) * res - sz;
+}
+
+// ---------------------------------------------------------------- mass rules
+// calc_credit, src/k3dom/kernel/mass_update.cu:12-14 (= tanh(mass/scale))
+K3_HD float k3_credit(float mass, float mass_scale) {
+    return 1.0f - 2.0f / (1.0f + k3_exp(2.0f * mass / mass_scale));
+}
+K3_HD float k3_positive_diff(float a, float b) { return fmaxf(0.0f, a - b); }
+K3_HD float k3_calc_ratio(float numerator, float rest) {
+    return (numerator <= 0.0f || rest < 0.0f) ? 0.0f : (1.0f / (1.0f + rest / numerator));
+}
+
+// predict_mass, src/k3dom/kernel/mass_update.cu:79-84; decay = gamma^dt.
+K3_HD void k3_predict_mass(float* f, float* s, float* d, float dyn_pred, float pred_total, float decay) {
+    *s = *s * decay;
+    *d = fminf(dyn_pred, fmaxf(0.0f, pred_total - *s));
+    *f = pred_total - *d;
+}
+
+// update_mass, src/k3dom/kernel/mass_update.cu:106-137 (the evidence part).
+K3_HD void k3_update_mass(float* f, float* s, float* d, float del_occ, float del_free, float mass_scale) {
+    if (del_free > 0.0f || del_occ > 0.0f) {
+        float m_f = *f, m_s = *s, m_d = *d;
+        float del_sigma = del_free + del_occ;
+        float prime_sigma = m_f + m_s + m_d;
+        float credit = k3_credit(sqrtf(del_sigma * prime_sigma), mass_scale);
+        float lamda_DtoF = credit * k3_positive_diff(del_free, del_occ) / del_sigma * m_d / prime_sigma;
+        float lamda_StoFD = credit * k3_positive_diff(del_free, del_occ) / del_sigma *
+                            k3_positive_diff(m_s + m_d, m_f) / prime_sigma;
+        float lamda_FtoD = credit * k3_positive_diff(del_occ, del_free) / del_sigma *
+                           k3_positive_diff(m_f, m_s + m_d) / prime_sigma;
+        float diff_f = lamda_StoFD / 2.0f * m_s + lamda_DtoF * m_d - lamda_FtoD * m_f;
+        float diff_s = -lamda_StoFD * m_s;
+        float diff_d = lamda_FtoD * m_f + lamda_StoFD / 2.0f * m_s - lamda_DtoF * m_d;
+        m_f += diff_f;
+        m_s += diff_s;
+        m_d += diff_d;
+        credit = k3_credit(m_s + m_d, mass_scale);
+        float beta = (1.0f - credit) + credit * k3_calc_ratio(m_s, m_d);
+        *s = m_s + beta * del_occ;
+        *d = m_d + (1.0f - beta) * del_occ;
+        *f = m_f + del_free;
+    }
+}
+
+// separate_newborn_part, src/k3dom/kernel/mass_update.cu:141-150
+K3_HD float k3_newborn_part(float m_dyn_pred, float m_total_pred, float m_dyn_up, float p_B) {
+    if (m_dyn_pred <= 0.0f) return m_dyn_up;
+    if (m_total_pred <= m_dyn_pred) return 0.0f;
+    return (m_dyn_up * p_B * (m_total_pred - m_dyn_pred)) / (m_dyn_pred + p_B * (m_total_pred - m_dyn_pred));
+}
+
+// ---------------------------------------------------------------- particles
+// Voxel key of a position in voxel-index units, src/common/kernel/predict.cu:51-55.
+// Clamping in float before truncation equals the reference's truncate-then-clamp
+// for every finite input; NaN maps to index 0.
+K3_HD int k3_key(float x, float y, float z, int gs, int gsz) {
+    int ix = (int)fminf(fmaxf(x, 0.0f), (float)(gs - 1));
+    int iy = (int)fminf(fmaxf(y, 0.0f), (float)(gs - 1));
+    int iz = (int)fminf(fmaxf(z, 0.0f), (float)(gsz - 1));
+    return ix + gs * iy + gs * gs * iz;
+}
+
+K3_HD float k3_clampf(float a, float lo, float hi) { return fmaxf(fminf(a, hi), lo); }
+
+// Velocity sampled U[-vmax, vmax) per component with the minimum-speed rescale of
+// src/common/kernel/init_new_particles.cu:168-176 (and :87-95).
+K3_HD void k3_sample_velocity(uint32_t r0, uint32_t r1, uint32_t r2, float vmax, float min_vel,
+                              float* vx, float* vy, float* vz) {
+    float lo = -vmax, span = vmax - (-vmax);
+    float x = lo + span * k3_uniform01(r0);
+    float y = lo + span * k3_uniform01(r1);
+    float z = lo + span * k3_uniform01(r2);
+    float vel_sq = x * x + y * y + z * z;
+    if (vel_sq < min_vel * min_vel && vel_sq != 0.0f) {
+        float rate = min_vel / sqrtf(vel_sq);
+        x *= rate; y *= rate; z *= rate;
+    }
+    *vx = x; *vy = y; *vz = z;
+}
+
+// One particle's prediction step, src/common/kernel/predict.cu:20-56, with the
+// six N(0, sigma) draws taken from Philox counters (i, frame, PREDICT, 0..1).
+struct K3PredictConst {
+    float dt, p_S, sigma_pos, sigma_vel, vclamp;
+    int gs, gsz;
+    uint32_t frame, key0, key1;
+};
+
+K3_HD void k3_predict_particle(uint32_t i, const K3PredictConst& c, float* px, float* py, float* pz,
+                               float* vx, float* vy, float* vz, float* w, int* key) {
+    uint32_t r[8];
+    k3_philox4x32_10(i, c.frame, K3_STREAM_PREDICT, 0u, c.key0, c.key1, r);
+    k3_philox4x32_10(i, c.frame, K3_STREAM_PREDICT, 1u, c.key0, c.key1, r + 4);
+    float n0, n1, n2, n3, n4, n5;
+    k3_box_muller(r[0], r[1], &n0, &n1);
+    k3_box_muller(r[2], r[3], &n2, &n3);
+    k3_box_muller(r[4], r[5], &n4, &n5);
+    float npx = n0 * c.sigma_pos + 0.0f, npy = n1 * c.sigma_pos + 0.0f, npz = n2 * c.sigma_pos + 0.0f;
+    float nvx = n3 * c.sigma_vel + 0.0f, nvy = n4 * c.sigma_vel + 0.0f, nvz = n5 * c.sigma_vel + 0.0f;
+    float x = *px + (*vx * c.dt + npx);
+    float y = *py + (*vy * c.dt + npy);
+    float z = *pz + (*vz * c.dt + npz);
+    float ux = *vx + nvx, uy = *vy + nvy, uz = *vz + nvz;
+    float ww = *w * c.p_S;
+    ux = k3_clampf(ux, -c.vclamp, c.vclamp);
+    uy = k3_clampf(uy, -c.vclamp, c.vclamp);
+    uz = k3_clampf(uz, -c.vclamp, c.vclamp);
+    if ((x >= (float)c.gs || x < 0.0f) || (y >= (float)c.gs || y < 0.0f) || (z >= (float)c.gsz || z < 0.0f))
+        ww = 0.0f;
+    *px = x; *py = y; *pz = z;
+    *vx = ux; *vy = uy; *vz = uz;
+    *w = ww;
+    *key = k3_key(x, y, z, c.gs, c.gsz);
+}
+
+// ---------------------------------------------------------------- fixed-point CDFs
+// Weights become u64 fixed point q = trunc(w * 2^E) (computed exactly in double)
+// with E chosen from the largest weight so that the total stays below 2^62.
+// Integer prefix sums do not depend on the summation order.
+K3_HD int k3_fixed_exponent(float wmax, uint32_t count) {
+    if (!(wmax > 0.0f)) return 0;
+    int lg = 0;
+    while (lg < 32 && ((uint64_t)1 << lg) < (uint64_t)count) ++lg;
+    uint32_t b = k3_f2u(wmax);
+    int e = (int)((b >> 23) & 0xffu) - 127;
+    if (((b >> 23) & 0xffu) == 0) e = -126;  // subnormal max
+    return 61 - lg - e;
+}
+K3_HD uint64_t k3_to_fixed(float w, int E) {
+    if (!(w > 0.0f) || !(w - w == 0.0f)) return 0;
+    double v = (double)w * k3_pow2d(E);
+    return (uint64_t)v;
+}

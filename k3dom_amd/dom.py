@@ -1,0 +1,166 @@
+"""Python mirror of the reference's filter class dom::DOM
+(include/k3dom/dom.h:17-30, include/common/dom.h:16-148) over the C ABI.
+
+Method names follow the reference (updatePose, updateGrid, getGridSize, ...);
+readback replaces the caller-side cudaMemcpy of grid_cell_array
+(src/k3dom/k3dom_node.cpp:131-132) with copyCellsToHost().
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Params, Cell, Info, ARR, check
+
+CELL_DTYPE = np.dtype([
+    ("start_idx", "<i4"), ("end_idx", "<i4"), ("mu_A", "<f4"), ("mu_UA", "<f4"), ("w_A", "<f4"), ("w_UA", "<f4"),
+    ("mean_x_vel", "<f4"), ("mean_y_vel", "<f4"), ("mean_z_vel", "<f4"), ("likelihood", "<f4"), ("p_A", "<f4"),
+    ("free_mass", "<f4"), ("static_mass", "<f4"), ("dynamic_mass", "<f4"), ("occ_mass", "<f4"),
+    ("pers_mass", "<f4"), ("pred_mass", "<f4"),
+])
+assert CELL_DTYPE.itemsize == C.sizeof(Cell) == 68
+
+
+def params_default():
+    p = Params()
+    check(_lib.load().k3dom_params_default(C.byref(p)), None, "k3dom_params_default")
+    return p
+
+
+def params_from_yaml(method_yaml=None, dataset_yaml=None):
+    """YAML -> Params with the ROS node's defaults and unit conversion
+    (src/k3dom/k3dom_node.cpp:215-368)."""
+    p = Params()
+    err = C.create_string_buffer(512)
+    st = _lib.load().k3dom_params_from_yaml(
+        method_yaml.encode() if method_yaml else None, dataset_yaml.encode() if dataset_yaml else None,
+        C.byref(p), err, 512)
+    if st != 0:
+        raise _lib.K3domError(f"k3dom_params_from_yaml: {err.value.decode()}")
+    return p
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class DOM:
+    """One map on the current HIP device."""
+
+    def __init__(self, params):
+        self._lib = _lib.load()
+        self.params = params
+        h = C.c_void_p()
+        check(self._lib.k3dom_create(C.byref(params), C.byref(h)), None, "k3dom_create")
+        self._h = h
+        info = self.info()
+        self.grid_size = info.grid_size
+        self.grid_size_z = info.grid_size_z
+        self.grid_cell_count = info.grid_cell_count
+        self.particle_count = info.particle_count
+        self.new_born_particle_count = info.new_born_particle_count
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.k3dom_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- reference API
+    def updatePose(self, x, y, z):
+        check(self._lib.k3dom_update_pose(self._h, float(x), float(y), float(z)), self._h, "updatePose")
+
+    def updateGrid(self, t, xs, ys, zs):
+        xs = np.ascontiguousarray(xs, dtype=np.float32)
+        ys = np.ascontiguousarray(ys, dtype=np.float32)
+        zs = np.ascontiguousarray(zs, dtype=np.float32)
+        if not (len(xs) == len(ys) == len(zs)):
+            raise ValueError("point arrays differ in length")
+        check(self._lib.k3dom_update_grid(self._h, float(t), _fptr(xs), _fptr(ys), _fptr(zs), len(xs)),
+              self._h, "updateGrid")
+
+    def updateGridDevice(self, t, d_xs, d_ys, d_zs, n):
+        """Points already in device memory (integer device addresses)."""
+        check(self._lib.k3dom_update_grid_device(self._h, float(t), C.c_void_p(d_xs), C.c_void_p(d_ys),
+                                                 C.c_void_p(d_zs), int(n)), self._h, "updateGridDevice")
+
+    def info(self):
+        out = Info()
+        check(self._lib.k3dom_get_info(self._h, C.byref(out)), self._h, "get_info")
+        return out
+
+    def getGridSize(self):
+        return self.grid_size
+
+    def getResolution(self):
+        return self.params.resolution
+
+    def getCenterPosition(self):
+        return tuple(self.info().center_pos)
+
+    def getSensorPosition(self):
+        return tuple(self.info().sensor_pos)
+
+    def getIteration(self):
+        return self.info().iteration
+
+    def indexToCoord(self, i):
+        """indexToCoordX/Y/Z (include/common/dom.h:70-81), FP32 as in the reference."""
+        gs = np.int64(self.grid_size)
+        i = np.asarray(i, dtype=np.int64)
+        res = np.float32(self.params.resolution)
+        c = self.info().center_pos
+        f32 = np.float32
+        x = (f32(i % gs) + f32(0.5)) * res - f32(self.params.size) / f32(2.0) + f32(c[0])
+        y = (f32((i % (gs * gs)) // gs) + f32(0.5)) * res - f32(self.params.size) / f32(2.0) + f32(c[1])
+        z = (f32(i // (gs * gs)) + f32(0.5)) * res - f32(self.params.size_z) / f32(2.0) + f32(c[2])
+        return x, y, z
+
+    def copyCellsToHost(self):
+        """All voxels as a GridCell-compatible structured array (index order of the reference)."""
+        out = np.empty(self.grid_cell_count, dtype=CELL_DTYPE)
+        check(self._lib.k3dom_read_cells(self._h, out.ctypes.data_as(C.POINTER(Cell))), self._h, "read_cells")
+        return out
+
+    # ---------------------------------------------------------------- extras
+    def read_masses(self):
+        V = self.grid_cell_count
+        a = [np.empty(V, np.float32) for _ in range(6)]
+        check(self._lib.k3dom_read_masses(self._h, *[_fptr(x) for x in a]), self._h, "read_masses")
+        return dict(zip(["free", "static", "dynamic", "mean_vx", "mean_vy", "mean_vz"], a))
+
+    def read_array(self, name):
+        which = ARR[name]
+        N, B, V = self.particle_count, self.new_born_particle_count, self.grid_cell_count
+        shape, dt = {
+            "PARTICLE_POS": ((3, N), np.float32), "PARTICLE_VEL": ((3, N), np.float32),
+            "PARTICLE_WEIGHT": ((N,), np.float32), "PARTICLE_KEY": ((N,), np.int32),
+            "SORT_PERM": ((N,), np.int32), "SORTED_KEY": ((N,), np.int32), "RESAMPLE_IDX": ((N,), np.int32),
+            "NEWBORN_POS": ((3, B), np.float32), "NEWBORN_VEL": ((3, B), np.float32),
+            "NEWBORN_WEIGHT": ((B,), np.float32), "NEWBORN_KEY": ((B,), np.int32),
+            "BORN_MASS": ((V,), np.float32), "DEL_OCC": ((V,), np.float32), "DEL_FREE": ((V,), np.float32),
+            "SORTED_WEIGHT": ((N,), np.float32),
+        }[name]
+        out = np.empty(shape, dt)
+        check(self._lib.k3dom_read_array(self._h, which, out.ctypes.data_as(C.c_void_p), out.nbytes), self._h,
+              f"read_array({name})")
+        return out
+
+    def set_timing(self, on=True):
+        check(self._lib.k3dom_set_timing(self._h, 1 if on else 0), self._h, "set_timing")
+
+    def stage_times(self):
+        out = np.zeros(10, np.float32)
+        check(self._lib.k3dom_stage_times(self._h, _fptr(out), 10), self._h, "stage_times")
+        return out

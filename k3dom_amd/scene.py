@@ -1,0 +1,188 @@
+"""Synthetic workloads (seeded box scenes + ray-cast LiDAR) and the filter
+parameters of each benchmark configuration.
+
+Filter parameters follow the K3DOM method configuration of the reference
+(config/k3dom.yaml:22-44: sigma 0.1, ls 0.5 m, mass_scale 2.5, priors 0.001,
+gamma 0.99, p_S 0.99, p_B 0.02, min velocity 0.1 m/s, initial max velocity
+5 m/s, process noise 0.01 m / 0.1 m/s, map shift threshold 1 m) with the unit
+conversion of the node (src/k3dom/k3dom_node.cpp:344-368): velocities and
+stddevs in voxels, map_shift_thresh in voxels.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import Params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SCENE_LIB = os.path.join(HERE, "libk3scene.so")
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("n_static", C.c_int32), ("n_moving", C.c_int32),
+        ("side_min", C.c_float), ("side_max", C.c_float), ("speed_min", C.c_float), ("speed_max", C.c_float),
+        ("xmin", C.c_float), ("xmax", C.c_float), ("ymin", C.c_float), ("ymax", C.c_float),
+        ("ground_z", C.c_float), ("clearance", C.c_float),
+        ("ego_x0", C.c_float), ("ego_y0", C.c_float), ("ego_z0", C.c_float), ("ego_vx", C.c_float),
+        ("ego_vy", C.c_float), ("beams", C.c_int32), ("az_bins", C.c_int32),
+        ("elev_min_deg", C.c_float), ("elev_max_deg", C.c_float), ("max_range", C.c_float),
+        ("range_noise", C.c_float), ("frame_dt", C.c_float), ("t0", C.c_double),
+        ("ground", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+_slib = None
+
+
+def _scene_lib():
+    global _slib
+    if _slib is None:
+        if not os.path.exists(SCENE_LIB):
+            raise RuntimeError(f"{SCENE_LIB} is missing: build it with `python -m k3dom_amd.build scene`")
+        lib = C.CDLL(SCENE_LIB)
+        lib.k3scene_create.restype = C.c_void_p
+        lib.k3scene_create.argtypes = [C.POINTER(SceneDesc)]
+        lib.k3scene_destroy.argtypes = [C.c_void_p]
+        lib.k3scene_frame.restype = C.c_int
+        lib.k3scene_frame.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                      C.POINTER(C.c_double)]
+        lib.k3scene_occupancy.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                          C.c_float, C.c_float, C.c_float, C.c_void_p]
+        _slib = lib
+    return _slib
+
+
+class Scene:
+    def __init__(self, desc):
+        self.desc = desc
+        self._lib = _scene_lib()
+        self._h = self._lib.k3scene_create(C.byref(desc))
+        self.cap = desc.beams * desc.az_bins
+
+    def frame(self, f):
+        """Returns (t, sensor_xyz, xs, ys, zs) of frame f."""
+        xs = np.empty(self.cap, np.float32)
+        ys = np.empty(self.cap, np.float32)
+        zs = np.empty(self.cap, np.float32)
+        pose = (C.c_double * 4)()
+        n = self._lib.k3scene_frame(self._h, f, xs.ctypes.data, ys.ctypes.data, zs.ctypes.data, self.cap, pose)
+        return pose[3], (pose[0], pose[1], pose[2]), xs[:n].copy(), ys[:n].copy(), zs[:n].copy()
+
+    def occupancy(self, f, gs, gsz, res, center, size, size_z):
+        out = np.empty(gs * gs * gsz, np.uint8)
+        self._lib.k3scene_occupancy(self._h, f, gs, gsz, res, center[0], center[1], center[2], size, size_z,
+                                    out.ctypes.data)
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.k3scene_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def grid_size_param(edge, res):
+    """A `size` whose FP32 truncation size/res gives exactly `edge` voxels."""
+    r = np.float32(res)
+    for s in (edge * res, (edge + 0.5) * res):
+        if int(np.float32(s) / r) == edge:
+            return float(np.float32(s))
+    raise ValueError(f"no size for edge {edge} at {res}")
+
+
+def k3dom_params(res, edge_xy, edge_z, particles, newborn, seed=123456, **over):
+    """Filter parameters of the K3DOM method configuration at resolution res."""
+    r = np.float32(res)
+    p = Params()
+    p.size = grid_size_param(edge_xy, res)
+    p.size_z = grid_size_param(edge_z, res)
+    p.mass_scale = 2.5
+    p.resolution = res
+    p.particle_count = particles
+    p.new_born_particle_count = newborn
+    p.persistence_prob = 0.99
+    p.stddev_process_noise_position = _f32(np.float32(0.01) / r)
+    p.stddev_process_noise_velocity = _f32(np.float32(0.1) / r)
+    p.birth_prob = 0.02
+    p.stddev_velocity = _f32(np.float32(4.0) / r)
+    p.init_max_velocity = _f32(np.float32(5.0) / r)
+    p.particle_min_vel = _f32(np.float32(0.1) / r)
+    p.sensor_off_x = p.sensor_off_y = p.sensor_off_z = 0.0
+    p.map_shift_thresh = int(np.float32(1.0) / r)
+    p.prior_free = p.prior_static = p.prior_dynamic = 0.001
+    p.prior_occ = 0.0
+    p.sigma = 0.1
+    p.ls = 0.5
+    p.gamma = 0.99
+    p.max_range = 0.0
+    p.meas_occ_stddev = 0.0
+    p.seed = seed
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+@dataclass
+class Workload:
+    name: str
+    params: Params
+    scene: SceneDesc
+    frames: int
+    extra: dict = field(default_factory=dict)
+
+
+def _desc(**kw):
+    d = SceneDesc()
+    defaults = dict(side_min=0.5, side_max=3.0, speed_min=0.0, clearance=2.0, ego_x0=0.0, ego_y0=0.0,
+                    ego_z0=0.0, ego_vx=0.0, ego_vy=0.0, elev_min_deg=-15.0, elev_max_deg=15.0, frame_dt=0.1,
+                    t0=1.0, ground=1)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(d, k, v)
+    return d
+
+
+def workload(cfg):
+    """The five benchmark configurations (1-5)."""
+    if cfg == 1:
+        # 64x64x16 @0.2 m, 65,536 particles (+10% newborn), 16x1024 LiDAR, static ego
+        p = k3dom_params(0.2, 64, 16, 65536, 6553)
+        s = _desc(seed=1, n_static=20, n_moving=4, speed_max=2.0, xmin=-12, xmax=12, ymin=-12, ymax=12,
+                  ground_z=-1.6, beams=16, az_bins=1024, max_range=20.0, range_noise=0.02)
+        return Workload("plain-small 64x64x16@0.2m N=65536", p, s, 20)
+    if cfg == 2:
+        # 256x256x32 @0.1 m, 2,000,000 particles + 200,000 newborn, 64x2048 LiDAR, ego 10 m/s
+        p = k3dom_params(0.1, 256, 32, 2_000_000, 200_000)
+        s = _desc(seed=2, n_static=300, n_moving=50, speed_max=5.0, xmin=-40, xmax=140, ymin=-40, ymax=40,
+                  ground_z=-1.6, ego_vx=10.0, beams=64, az_bins=2048, max_range=40.0, range_noise=0.03)
+        return Workload("plain-full 256x256x32@0.1m N=2000000", p, s, 100)
+    if cfg == 3:
+        p = k3dom_params(0.2, 64, 16, 65536, 6553, ls=0.3)
+        s = workload(1).scene
+        return Workload("differentiated config-1 scene, ls0=0.3", p, s, 8)
+    if cfg == 5:
+        p = k3dom_params(0.1, 512, 64, 8_000_000, 800_000)
+        s = _desc(seed=5, n_static=1000, n_moving=200, speed_max=15.0, xmin=-60, xmax=300, ymin=-60, ymax=60,
+                  ground_z=-3.2, ego_vx=15.0, beams=128, az_bins=2048, max_range=40.0, range_noise=0.03)
+        return Workload("stress 512x512x64@0.1m N=8000000", p, s, 50)
+    raise ValueError(f"unknown configuration {cfg}")
+
+
+def small_workload(edge=16, edge_z=8, res=0.2, particles=4096, newborn=409, beams=8, az=128, frames=6, seed=11,
+                   moving=2, static=6, ego_vx=0.0, **over):
+    """A tiny scene for fast parity tests."""
+    p = k3dom_params(res, edge, edge_z, particles, newborn, **over)
+    half = edge * res / 2
+    s = _desc(seed=seed, n_static=static, n_moving=moving, speed_max=2.0, xmin=-2 * half, xmax=2 * half,
+              ymin=-2 * half, ymax=2 * half, ground_z=-edge_z * res / 2, clearance=0.6, ego_vx=ego_vx,
+              beams=beams, az_bins=az, max_range=3 * half, range_noise=0.02)
+    return Workload(f"small {edge}x{edge}x{edge_z}@{res} N={particles}", p, s, frames)

@@ -1,0 +1,174 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the serial CPU restatement
+(oracle/oracle.cpp -> oracle/liborc.so).  Imported by tests/, by the smoke
+check and by bench.py's CPU-baseline leg, never by the product."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+_orc = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(LIB):
+            build()
+        from k3dom_amd._lib import Params, Cell, Info  # struct layouts shared with the public header
+        L = C.CDLL(LIB)
+        fp, u32p = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+        sig = {
+            "orc_create": (C.c_void_p, [C.POINTER(Params)]),
+            "orc_destroy": (None, [C.c_void_p]),
+            "orc_set_brute": (None, [C.c_void_p, C.c_int]),
+            "orc_update_pose": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+            "orc_update_grid": (C.c_int, [C.c_void_p, C.c_double, fp, fp, fp, C.c_int]),
+            "orc_get_info": (None, [C.c_void_p, C.POINTER(Info)]),
+            "orc_read_cells": (None, [C.c_void_p, C.POINTER(Cell)]),
+            "orc_read_array": (C.c_int64, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
+            "orc_stage_times": (None, [C.c_void_p, C.POINTER(C.c_double)]),
+            "orc_bgki_pairs": (None, [C.c_void_p, C.POINTER(C.c_uint64)]),
+            "orc_philox": (None, [u32p, u32p, u32p]),
+            "orc_ln": (C.c_float, [C.c_float]),
+            "orc_exp": (C.c_float, [C.c_float]),
+            "orc_sincospi": (None, [C.c_float, fp, fp]),
+            "orc_kernel": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+            "orc_credit": (C.c_float, [C.c_float, C.c_float]),
+            "orc_box_muller": (None, [C.c_uint32, C.c_uint32, fp, fp]),
+            "orc_update_mass": (None, [fp, C.c_float, C.c_float, C.c_float]),
+            "orc_predict_mass": (None, [fp, C.c_float, C.c_float, C.c_float]),
+            "orc_newborn_part": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
+            "orc_key": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]),
+            "orc_bgki": (None, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                fp, fp, fp, C.c_int, C.c_int, fp, fp, C.POINTER(C.c_uint64)]),
+            "orc_bgki64": (None, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                  C.c_float, fp, fp, fp, C.c_int, C.POINTER(C.c_int32), C.c_int,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "orc_mass_update64": (None, [fp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.POINTER(C.c_double)]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _orc = L
+    return _orc
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class OracleDOM:
+    """Serial restatement with the same surface as k3dom_amd.dom.DOM."""
+
+    def __init__(self, params, brute=False):
+        from k3dom_amd._lib import Info
+        self._L = lib()
+        self.params = params
+        self._h = self._L.orc_create(C.byref(params))
+        if not self._h:
+            raise ValueError("invalid parameters")
+        if brute:
+            self._L.orc_set_brute(self._h, 1)
+        info = Info()
+        self._L.orc_get_info(self._h, C.byref(info))
+        self.grid_size, self.grid_size_z = info.grid_size, info.grid_size_z
+        self.grid_cell_count = info.grid_cell_count
+        self.particle_count = info.particle_count
+        self.new_born_particle_count = info.new_born_particle_count
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._L.orc_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def updatePose(self, x, y, z):
+        self._L.orc_update_pose(self._h, float(x), float(y), float(z))
+
+    def updateGrid(self, t, xs, ys, zs):
+        xs = np.ascontiguousarray(xs, np.float32)
+        ys = np.ascontiguousarray(ys, np.float32)
+        zs = np.ascontiguousarray(zs, np.float32)
+        st = self._L.orc_update_grid(self._h, float(t), _fp(xs), _fp(ys), _fp(zs), len(xs))
+        if st != 0:
+            raise ValueError(f"orc_update_grid status {st}")
+
+    def info(self):
+        from k3dom_amd._lib import Info
+        info = Info()
+        self._L.orc_get_info(self._h, C.byref(info))
+        return info
+
+    def copyCellsToHost(self):
+        from k3dom_amd.dom import CELL_DTYPE
+        from k3dom_amd._lib import Cell
+        out = np.empty(self.grid_cell_count, dtype=CELL_DTYPE)
+        self._L.orc_read_cells(self._h, out.ctypes.data_as(C.POINTER(Cell)))
+        return out
+
+    def read_array(self, name):
+        from k3dom_amd._lib import ARR
+        N, B, V = self.particle_count, self.new_born_particle_count, self.grid_cell_count
+        shape, dt = {
+            "PARTICLE_POS": ((3, N), np.float32), "PARTICLE_VEL": ((3, N), np.float32),
+            "PARTICLE_WEIGHT": ((N,), np.float32), "PARTICLE_KEY": ((N,), np.int32),
+            "SORT_PERM": ((N,), np.int32), "SORTED_KEY": ((N,), np.int32), "RESAMPLE_IDX": ((N,), np.int32),
+            "NEWBORN_POS": ((3, B), np.float32), "NEWBORN_VEL": ((3, B), np.float32),
+            "NEWBORN_WEIGHT": ((B,), np.float32), "NEWBORN_KEY": ((B,), np.int32),
+            "BORN_MASS": ((V,), np.float32), "DEL_OCC": ((V,), np.float32), "DEL_FREE": ((V,), np.float32),
+            "SORTED_WEIGHT": ((N,), np.float32),
+        }[name]
+        out = np.empty(shape, dt)
+        n = self._L.orc_read_array(self._h, ARR[name], out.ctypes.data_as(C.c_void_p), out.nbytes)
+        if n < 0:
+            raise ValueError(f"read_array({name}) failed")
+        return out
+
+    def stage_times(self):
+        out = (C.c_double * 10)()
+        self._L.orc_stage_times(self._h, out)
+        return np.array(out[:])
+
+    def bgki_pairs(self):
+        out = (C.c_uint64 * 2)()
+        self._L.orc_bgki_pairs(self._h, out)
+        return int(out[0]), int(out[1])
+
+
+def bgki(gs, gsz, res, sensor_rel, ls, sigma, xs, ys, zs, brute=False):
+    """(del_occ, del_free, (occ_pairs, free_pairs)) of one point set on a grid."""
+    V = gs * gs * gsz
+    o = np.empty(V, np.float32)
+    f = np.empty(V, np.float32)
+    pairs = (C.c_uint64 * 2)()
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.float32)
+    zs = np.ascontiguousarray(zs, np.float32)
+    lib().orc_bgki(gs, gsz, res, sensor_rel[0], sensor_rel[1], sensor_rel[2], ls, sigma, _fp(xs), _fp(ys), _fp(zs),
+                   len(xs), 1 if brute else 0, _fp(o), _fp(f), pairs)
+    return o, f, (int(pairs[0]), int(pairs[1]))
+
+
+def bgki64(gs, gsz, res, sensor_rel, ls, sigma, xs, ys, zs, cells):
+    cells = np.ascontiguousarray(cells, np.int32)
+    o = np.empty(len(cells), np.float64)
+    f = np.empty(len(cells), np.float64)
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.float32)
+    zs = np.ascontiguousarray(zs, np.float32)
+    lib().orc_bgki64(gs, gsz, res, sensor_rel[0], sensor_rel[1], sensor_rel[2], ls, sigma, _fp(xs), _fp(ys),
+                     _fp(zs), len(xs), cells.ctypes.data_as(C.POINTER(C.c_int32)), len(cells),
+                     o.ctypes.data_as(C.POINTER(C.c_double)), f.ctypes.data_as(C.POINTER(C.c_double)))
+    return o, f
