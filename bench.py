@@ -1,0 +1,172 @@
+"""Benchmark of the K3DOM per-frame dynamic occupancy update on one MI355X.
+
+One step = one filter frame (DOM_c::updatePose + DOM::updateGrid, the
+reference's src/common/dom.cu:111-158 and src/k3dom/dom.cu:29-50) over the
+full-size plain configuration: 256x256x32 voxels at 0.1 m, 2,000,000
+particles + 200,000 newborn per frame, a 64x2048 ray-cast LiDAR on an ego
+moving at 10 m/s through 300 static + 50 moving boxes.  Point clouds of every
+frame are uploaded to device memory before timing (inputs resident in HBM).
+
+Prints one JSON line (rank 0).  `roofline` is for the dominant kernel, the
+kernel-inference gather, whose algorithmic work is its contributing
+(voxel, point) pairs; `cpu_baseline` times the serial restatement on a
+bounded sample of the same workload on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s at 2M particles, 256x256x32 voxels; fwd+bwd ms/frame vs plain"
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+# FP32 operations of the reference's per-pair terms (src/k3dom/kernel/mass_update.cu:43-72,24-26),
+# counting each sqrt, division and cospi/sinpi as one operation:
+#   occupied term: distance 9 + compare 1 + kernel 14 + accumulate 1 = 25
+#   free term:     projection 6 + compares 2 + distance 4 + compare 1 + kernel 14 + accumulate 1 = 28
+FLOPS_OCC_PAIR = 25
+FLOPS_FREE_PAIR = 28
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(workload, frames):
+    """Serial restatement (oracle) on the host: frame 0 (initialisation, not
+    timed) then frame 1, the first full predict-to-resample frame (timed)."""
+    from oracle.orc import OracleDOM
+    o = OracleDOM(workload.params)
+    t, pose, xs, ys, zs = frames[0]
+    o.updatePose(*pose)
+    o.updateGrid(t, xs, ys, zs)
+    t, pose, xs, ys, zs = frames[1]
+    o.updatePose(*pose)
+    t0 = time.perf_counter()
+    o.updateGrid(t, xs, ys, zs)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 5), "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"frame 1 (first full frame, {len(xs)} points) of the same workload after the untimed "
+                      f"initialisation frame; serial C++ restatement, 1 thread; {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true", help="skip the pair-count pass")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log("bench.py measures one GPU: this build has no multi-GPU path (the per-frame update runs on one "
+            "MI355X; see the design notes for how the map would shard). Run with --gpus 1.")
+        sys.exit(2)
+    if args.warmup < 1:
+        log("--warmup must be >= 1 (the first frame initialises the particles)")
+        sys.exit(2)
+
+    import torch
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import workload, Scene
+
+    w = workload(args.config)
+    total = args.warmup + args.steps
+    sc = Scene(w.scene)
+    frames = [sc.frame(f) for f in range(total)]
+    npts = [len(f[2]) for f in frames]
+    log(f"workload {w.name}: {total} frames, points/frame {min(npts)}..{max(npts)}")
+
+    # inputs resident in HBM before timing
+    dev = torch.device("cuda:0")
+    dpts = [tuple(torch.from_numpy(a).to(dev) for a in (xs, ys, zs)) for (_, _, xs, ys, zs) in frames]
+    torch.cuda.synchronize()
+
+    d = DOM(w.params)
+
+    def step(f):
+        t, pose, xs, ys, zs = frames[f]
+        dx, dy, dz = dpts[f]
+        d.updatePose(*pose)
+        d.updateGridDevice(t, dx.data_ptr(), dy.data_ptr(), dz.data_ptr(), len(xs))
+
+    for f in range(args.warmup):
+        step(f)
+    d.set_timing(True)
+    stage_sum = np.zeros(len(DOM.STAGES))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(args.warmup, total):
+        step(f)
+        stage_sum += d.stage_times()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    d.set_timing(False)
+    ms_per_step = elapsed / args.steps * 1e3
+    stage_avg = stage_sum / args.steps
+    log("stage ms/frame: " + " ".join(f"{n}={v:.3f}" for n, v in zip(DOM.STAGES, stage_avg)))
+
+    roof = None
+    if not args.no_roofline:
+        # algorithmic work of the timed frames: replay them on a second map
+        # (the update is deterministic, so the states and pairs are identical)
+        d2 = DOM(w.params)
+        occ = free = 0
+        for f in range(total):
+            t, pose, xs, ys, zs = frames[f]
+            dx, dy, dz = dpts[f]
+            d2.updatePose(*pose)
+            d2.updateGridDevice(t, dx.data_ptr(), dy.data_ptr(), dz.data_ptr(), len(xs))
+            if f >= args.warmup:
+                o, fr = d2.count_bgki_pairs()
+                occ += o
+                free += fr
+        d2.close()
+        flops = (occ * FLOPS_OCC_PAIR + free * FLOPS_FREE_PAIR) / args.steps
+        kernel_s = stage_avg[DOM.STAGES.index("bgki_kernel")] * 1e-3
+        achieved = flops / kernel_s / 1e12
+        roof = {"bound": "valu", "kernel": "k_bgki", "achieved": round(achieved, 3),
+                "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_VECTOR_PEAK_TFLOPS, 4), "traffic": None,
+                "launch_ms": round(kernel_s * 1e3, 4),
+                "pairs_per_launch": [occ // args.steps, free // args.steps]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(w, frames)
+
+    p = w.params
+    gs = int(np.float32(p.size) / np.float32(p.resolution))
+    gsz = int(np.float32(p.size_z) / np.float32(p.resolution))
+    line = {
+        "metric": METRIC,
+        "value": round(args.steps / elapsed, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "grid": [gs, gs, gsz], "resolution_m": float(np.float32(p.resolution)),
+                   "particles": p.particle_count, "newborn_per_frame": p.new_born_particle_count,
+                   "lidar": f"{w.scene.beams}x{w.scene.az_bins}", "points_per_frame_mean": int(np.mean(npts)),
+                   "ego_speed_mps": float(w.scene.ego_vx), "parallelism": "single GPU"},
+        "stage_ms": {n: round(float(v), 4) for n, v in zip(DOM.STAGES, stage_avg)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

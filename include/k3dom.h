@@ -183,10 +183,26 @@ k3dom_status k3dom_read_array(k3dom_handle* h, int32_t which, void* dst, int64_t
  * events on the handle's stream: [0] predict, [1] sort and ranges,
  * [2] point preparation and candidate lists, [3] kernel inference (BGKI),
  * [4] mass update, [5] persistent update, [6] newborn, [7] moments,
- * [8] resampling, [9] whole frame.  Timing is recorded only after
- * k3dom_set_timing(h, 1). */
+ * [8] resampling, [9] whole frame, [10] the kernel-inference gather kernel
+ * alone.  Timing is recorded only after k3dom_set_timing(h, 1). */
 k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable);
 k3dom_status k3dom_stage_times(k3dom_handle* h, float* ms, int32_t count);
+
+/* Diagnostic with no counterpart in the reference: number of (voxel, point)
+ * pairs of the last frame whose occupied (out[0]) and free (out[1]) kernel
+ * term is evaluated (distance < ls), i.e. the algorithmic work of the
+ * kernel-inference gather.  Re-evaluates the last frame's candidate lists. */
+k3dom_status k3dom_count_bgki_pairs(k3dom_handle* h, uint64_t* out2);
+
+/* Work statistics of the last frame (diagnostic, no reference counterpart). */
+typedef struct k3dom_stats {
+    uint64_t points;             /* points of the frame                          */
+    uint64_t level_entries[3];   /* candidate-list entries per level (coarse..tile) */
+    uint64_t gather_items;       /* work items of the kernel-inference gather    */
+    uint64_t gather_split_items; /* items of tiles split into point chunks       */
+    uint64_t capacity_regrowths; /* frames re-run after growing a list capacity (all frames) */
+} k3dom_stats;
+k3dom_status k3dom_get_stats(const k3dom_handle* h, k3dom_stats* out);
 
 /* Text of the last error on h (or of the last failed create when h is NULL). */
 const char* k3dom_last_error(const k3dom_handle* h);

@@ -49,6 +49,11 @@ class Info(C.Structure):
     ]
 
 
+class Stats(C.Structure):
+    _fields_ = [("points", C.c_uint64), ("level_entries", C.c_uint64 * 3), ("gather_items", C.c_uint64),
+                ("gather_split_items", C.c_uint64), ("capacity_regrowths", C.c_uint64)]
+
+
 # numbering of k3dom_array
 ARR = dict(PARTICLE_POS=0, PARTICLE_VEL=1, PARTICLE_WEIGHT=2, PARTICLE_KEY=3, SORT_PERM=4, SORTED_KEY=5,
            RESAMPLE_IDX=6, NEWBORN_POS=7, NEWBORN_VEL=8, NEWBORN_WEIGHT=9, NEWBORN_KEY=10, BORN_MASS=11,
@@ -71,6 +76,8 @@ SIGNATURES = {
     "k3dom_read_array": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     "k3dom_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "k3dom_stage_times": (C.c_int, [C.c_void_p, _fp, C.c_int32]),
+    "k3dom_count_bgki_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "k3dom_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "k3dom_last_error": (C.c_char_p, [C.c_void_p]),
 }
 

@@ -29,7 +29,7 @@ def _run(cmd):
 
 def build_lib(verbose_asm=False):
     out = os.path.join(HERE, "libk3dom.so")
-    srcs = [os.path.join(CSRC, "k3_kernels.hip"), os.path.join(CSRC, "k3_api.cpp")]
+    srcs = [os.path.join(CSRC, "k3_kernels.hip"), os.path.join(CSRC, "k3_bgki.hip"), os.path.join(CSRC, "k3_api.cpp")]
     cmd = ["hipcc", *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", *srcs]
     _run(cmd)
     os.replace(out + ".tmp", out)

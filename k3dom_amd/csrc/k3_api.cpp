@@ -23,15 +23,10 @@ namespace {
 
 thread_local std::string g_create_error;
 
-struct Level {
-    LevelGeom g;
-    int nboxes = 0;
-    uint32_t *len = nullptr, *off = nullptr, *list = nullptr;
-    size_t cap = 0;  // entries of list
-};
-
 enum { ST_PREDICT = 0, ST_SORT, ST_LISTS, ST_BGKI, ST_MASS, ST_PERSIST, ST_NEWBORN, ST_MOMENTS, ST_RESAMPLE,
-       ST_FRAME, ST_COUNT };
+       ST_FRAME, ST_BGKI_KERNEL, ST_COUNT };
+// extra events: frame start, end of the gather kernel
+enum { EV_FRAME_START = ST_COUNT, EV_BGKI_END, EV_COUNT };
 
 }  // namespace
 
@@ -62,13 +57,12 @@ struct k3dom_handle {
     float4* pxyz = nullptr;
     float* h_pts = nullptr;
 
-    Level lev[3];
-    uint32_t *heavy_cnt = nullptr, *light_cnt = nullptr, *heavy_off = nullptr, *light_off = nullptr;
-    uint32_t* tile_partial = nullptr;
-    uint4* items = nullptr;
-    size_t item_cap = 0;
-    float* partial = nullptr;
-    size_t partial_cap = 0;  // items
+    TileGeom tg{};
+    LevelDesc ldesc[3]{};
+    LevelBufs lbuf[3]{};
+    ItemBufs ibuf{};
+    bool tiny_caps = false;  // test hook: start every capacity small to exercise the regrowth path
+    int overflow_retries = 0;
     DevScalars* h_sc = nullptr;  // pinned mirror
 
     uint32_t frame = 0;
@@ -78,7 +72,10 @@ struct k3dom_handle {
     std::string err;
 
     bool timing = false;
-    hipEvent_t ev[ST_COUNT + 1]{};
+    hipEvent_t ev[EV_COUNT]{};
+    k3dom_stats stats{};
+    GridGeom last_geom{};
+    unsigned long long* pair_counts = nullptr;
     float stage_ms[ST_COUNT]{};
 };
 
@@ -129,26 +126,39 @@ k3dom_status validate(const k3dom_params* p, std::string* why) {
     return K3DOM_OK;
 }
 
+// Hierarchy of candidate-list boxes over the tile grid: level 0 boxes of
+// 8x8x4 tiles (32x32x16 voxels; doubled in x/y until there are at most
+// MAX_CHILDREN of them), level 1 of 2x2x2 tiles, level 2 single tiles.
 void setup_levels(k3dom_handle* h) {
-    // Boxes of 8x8x4 tiles (32x32x16 voxels), then 2x2x2 tiles, then single tiles.
-    const int sizes[3][3] = {{8, 8, 4}, {2, 2, 2}, {1, 1, 1}};
+    TileGeom& tg = h->tg;
+    int sizes[3][3] = {{8, 8, 4}, {2, 2, 2}, {1, 1, 1}};
+    auto nb = [&](int t, int b) { return (t + b - 1) / b; };
+    while (nb(tg.ntx, sizes[0][0]) * nb(tg.nty, sizes[0][1]) * nb(tg.ntz, sizes[0][2]) > MAX_CHILDREN) {
+        sizes[0][0] *= 2;
+        sizes[0][1] *= 2;
+    }
+    const int segs[3] = {256, 1024, 1024};
     for (int l = 0; l < 3; ++l) {
-        LevelGeom& g = h->lev[l].g;
-        g.bx = sizes[l][0]; g.by = sizes[l][1]; g.bz = sizes[l][2];
-        g.nbx = (h->ntx + g.bx - 1) / g.bx;
-        g.nby = (h->nty + g.by - 1) / g.by;
-        g.nbz = (h->ntz + g.bz - 1) / g.bz;
+        LevelDesc& L = h->ldesc[l];
+        L.bx = sizes[l][0]; L.by = sizes[l][1]; L.bz = sizes[l][2];
+        L.nbx = nb(tg.ntx, L.bx); L.nby = nb(tg.nty, L.by); L.nbz = nb(tg.ntz, L.bz);
+        L.seg = segs[l];
         if (l == 0) {
-            g.pbx = g.pby = g.pbz = 0;
-            g.pnbx = g.pnby = 0;
+            L.nparents = 1;
+            L.pnbx = L.pnby = 1;
+            L.ncx = L.nbx; L.ncy = L.nby; L.ncz = L.nbz;
+            L.nch = L.nbx * L.nby * L.nbz;
         } else {
-            const LevelGeom& pg = h->lev[l - 1].g;
-            g.pbx = pg.bx; g.pby = pg.by; g.pbz = pg.bz;
-            g.pnbx = pg.nbx; g.pnby = pg.nby;
+            const LevelDesc& P = h->ldesc[l - 1];
+            L.nparents = P.nbx * P.nby * P.nbz;
+            L.pnbx = P.nbx; L.pnby = P.nby;
+            L.ncx = P.bx / L.bx; L.ncy = P.by / L.by; L.ncz = P.bz / L.bz;
+            L.nch = L.ncx * L.ncy * L.ncz;
         }
-        h->lev[l].nboxes = g.nbx * g.nby * g.nbz;
     }
 }
+
+k3dom_status ensure_caps(k3dom_handle* h, const long long want[3], long long want_heavy);
 
 k3dom_status ensure_points(k3dom_handle* h, int m) {
     if (m <= h->m_cap) return K3DOM_OK;
@@ -162,17 +172,59 @@ k3dom_status ensure_points(k3dom_handle* h, int m) {
     HIP_TRY(h, dalloc(h, &h->pxyz, (size_t)cap));
     HIP_TRY(h, hipHostMalloc((void**)&h->h_pts, sizeof(float) * 3 * (size_t)cap, hipHostMallocDefault));
     h->m_cap = cap;
+    // first estimate of the candidate-list sizes; frames that need more grow them
+    const long long mc = cap;
+    long long want[3] = {8 * mc + 4096, 48 * mc + 16384, 160 * mc + 65536};
+    long long want_heavy = 4096;
+    if (h->tiny_caps) {
+        want[0] = want[1] = want[2] = 64;
+        want_heavy = 1;
+    }
+    return ensure_caps(h, want, want_heavy);
+}
+
+// (Re)allocates the buffers of one candidate level whose sizes follow from
+// its list capacity and its parent's.
+k3dom_status alloc_level(k3dom_handle* h, int l, long long list_cap) {
+    LevelBufs& B = h->lbuf[l];
+    const LevelDesc& L = h->ldesc[l];
+    for (void* p : {(void*)B.counts, (void*)B.slot_off, (void*)B.list, (void*)B.bsum}) dfree(h, p);
+    B.counts = B.slot_off = B.list = B.bsum = nullptr;
+    const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : h->lbuf[l - 1].list_cap;
+    B.items_cap = L.nparents + (parent_cap + L.seg - 1) / L.seg;
+    B.slot_cap = B.items_cap * L.nch;
+    B.list_cap = list_cap;
+    HIP_TRY(h, dalloc(h, &B.counts, (size_t)B.slot_cap));
+    HIP_TRY(h, dalloc(h, &B.slot_off, (size_t)B.slot_cap + 1));
+    HIP_TRY(h, dalloc(h, &B.list, (size_t)std::max(1ll, list_cap)));
+    HIP_TRY(h, dalloc(h, &B.bsum, (size_t)scan_blocks(std::max<long long>(B.slot_cap, L.nparents + 1))));
     return K3DOM_OK;
 }
 
-k3dom_status ensure_level(k3dom_handle* h, Level& L, size_t need) {
-    if (need <= L.cap) return K3DOM_OK;
-    size_t cap = std::max(need + need / 4 + 1024, L.cap * 3 / 2);
-    dfree(h, L.list);
-    L.list = nullptr;
-    L.cap = 0;
-    HIP_TRY(h, dalloc(h, &L.list, cap));
-    L.cap = cap;
+k3dom_status alloc_heavy(k3dom_handle* h, long long heavy_cap) {
+    ItemBufs& I = h->ibuf;
+    dfree(h, I.items);
+    dfree(h, I.partial);
+    I.items = nullptr;
+    I.partial = nullptr;
+    I.heavy_cap = heavy_cap;
+    HIP_TRY(h, dalloc(h, &I.items, (size_t)(h->ntiles + heavy_cap)));
+    HIP_TRY(h, dalloc(h, &I.partial, (size_t)std::max(1ll, heavy_cap) * 128));
+    return K3DOM_OK;
+}
+
+// Capacities for a frame of m points: grown for larger point sets, and by the
+// overflow path when a frame needs more than estimated.
+k3dom_status ensure_caps(k3dom_handle* h, const long long want[3], long long want_heavy) {
+    for (int l = 0; l < 3; ++l) {
+        const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : h->lbuf[l - 1].list_cap;
+        const long long items_need = h->ldesc[l].nparents + (parent_cap + h->ldesc[l].seg - 1) / h->ldesc[l].seg;
+        if (want[l] > h->lbuf[l].list_cap || items_need > h->lbuf[l].items_cap || !h->lbuf[l].list) {
+            k3dom_status s = alloc_level(h, l, std::max(want[l], h->lbuf[l].list_cap));
+            if (s != K3DOM_OK) return s;
+        }
+    }
+    if (want_heavy > h->ibuf.heavy_cap || !h->ibuf.items) return alloc_heavy(h, std::max(want_heavy, h->ibuf.heavy_cap));
     return K3DOM_OK;
 }
 
@@ -197,67 +249,85 @@ GridGeom grid_geom(const k3dom_handle* h) {
     return g;
 }
 
-// Kernel inference for the current point set (already in h->d_pts): writes
-// del_occ / del_free for every voxel.
+// Kernel inference for the current point set: writes del_occ / del_free for
+// every voxel, or sets the overflow flag when a capacity is exceeded.
 k3dom_status run_bgki(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int m) {
     const GridGeom g = grid_geom(h);
     mark(h, ST_LISTS);
     HIP_TRY(h, launch_point_prep(h->stream, xs, ys, zs, m, h->p.resolution, h->rec, h->pxyz));
-    for (int l = 0; l < 3; ++l) {
-        Level& L = h->lev[l];
-        const Level* P = l ? &h->lev[l - 1] : nullptr;
-        HIP_TRY(h, launch_list_level(h->stream, 1, g, L.g, L.nboxes, h->p.ls, h->pxyz, m, P ? P->off : nullptr,
-                                     P ? P->len : nullptr, P ? P->list : nullptr, L.len, nullptr, nullptr));
-        HIP_TRY(h, launch_scan_u32(h->stream, L.len, L.off, L.nboxes, &h->sc->level_total[l]));
-        k3dom_status s = sync_scalars(h);
-        if (s != K3DOM_OK) return s;
-        s = ensure_level(h, L, h->h_sc->level_total[l]);
-        if (s != K3DOM_OK) return s;
-        HIP_TRY(h, launch_list_level(h->stream, 0, g, L.g, L.nboxes, h->p.ls, h->pxyz, m, P ? P->off : nullptr,
-                                     P ? P->len : nullptr, P ? P->list : nullptr, L.len, L.off, L.list));
-    }
-    const Level& T = h->lev[2];
-    HIP_TRY(h, launch_count_items(h->stream, h->ntiles, T.off, T.len, T.list, h->heavy_cnt, h->light_cnt,
-                                  h->heavy_off, h->light_off, h->sc));
-    k3dom_status s = sync_scalars(h);
-    if (s != K3DOM_OK) return s;
-    size_t n_heavy = h->h_sc->n_heavy_items, n_items = n_heavy + h->h_sc->n_items;
-    if (n_items > h->item_cap) {
-        dfree(h, h->items);
-        h->items = nullptr;
-        h->item_cap = 0;
-        size_t cap = n_items + n_items / 4 + 1024;
-        HIP_TRY(h, dalloc(h, &h->items, cap));
-        h->item_cap = cap;
-    }
-    if (n_heavy > h->partial_cap) {
-        dfree(h, h->partial);
-        h->partial = nullptr;
-        h->partial_cap = 0;
-        size_t cap = n_heavy + n_heavy / 4 + 256;
-        HIP_TRY(h, dalloc(h, &h->partial, cap * 128));
-        h->partial_cap = cap;
-    }
-    HIP_TRY(h, launch_emit_items(h->stream, h->ntiles, T.off, T.len, T.list, h->heavy_cnt, h->heavy_off,
-                                 h->light_off, h->items, h->tile_partial, h->sc));
+    for (int l = 0; l < 3; ++l)
+        HIP_TRY(h, launch_build_level(h->stream, l, g, h->tg, h->ldesc[l], h->lbuf[l], l ? &h->lbuf[l - 1] : nullptr,
+                                      h->p.ls, h->pxyz, m, h->sc));
+    HIP_TRY(h, launch_build_items(h->stream, h->ntiles, h->lbuf[2], h->ibuf, h->sc));
     mark(h, ST_BGKI);
-    const int grid_blocks = std::max(1, std::min<int>((int)((n_items + 3) / 4), h->num_cus * 8));
-    HIP_TRY(h, launch_bgki(h->stream, grid_blocks, g, h->ntx, h->nty, h->p.ls, h->p.sigma, h->items, T.list, h->rec,
-                           h->del_occ, h->del_free, h->partial, h->sc));
-    HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, g, h->ntx, h->nty, h->tile_partial, h->partial,
-                                   h->del_occ, h->del_free));
+    const int grid_blocks = h->num_cus * 8;
+    mark(h, ST_BGKI_KERNEL);
+    HIP_TRY(h, launch_bgki(h->stream, grid_blocks, g, h->tg, h->p.ls, h->p.sigma, h->ibuf, h->lbuf[2], h->rec,
+                           h->del_occ, h->del_free, h->sc));
+    mark(h, EV_BGKI_END);
+    HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, g, h->tg, h->ibuf, h->del_occ, h->del_free, h->sc));
+    h->last_geom = g;
+    return K3DOM_OK;
+}
+
+// Kernel inference and everything after it.  Tail kernels return at once when
+// the overflow flag is set, so the frame can be re-run from here.
+k3dom_status enqueue_rest(k3dom_handle* h, bool normal, float decay, const float* dx, const float* dy,
+                          const float* dz, int m) {
+    VoxelArrays& v = h->vox[h->vcur];
+    const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
+    const uint32_t* ab = &h->sc->overflow;
+    k3dom_status s = run_bgki(h, dx, dy, dz, m);
+    if (s != K3DOM_OK) return s;
+    mark(h, ST_MASS);
+    if (normal) {
+        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[1].w, h->del_occ, h->del_free, decay,
+                                      h->p.mass_scale, h->p.birth_prob, h->born, h->sc));
+        mark(h, ST_PERSIST);
+        HIP_TRY(h, launch_voxel_particles(h->stream, ab, v, h->V, h->P[1], h->p.particle_min_vel, h->sc));
+        mark(h, ST_NEWBORN);
+        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, &h->sc->born_max_bits, &h->sc->E_born,
+                                     h->cdf_v, h->blocksums, &h->sc->total_born));
+        HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->B, h->born, h->bnd,
+                                 v.wua));
+        HIP_TRY(h, launch_newborn(h->stream, ab, h->bnd, h->V, h->B, h->gs, v.wua, h->NB, h->p.init_max_velocity,
+                                  h->p.particle_min_vel, h->frame, k0, k1, h->sc));
+        mark(h, ST_MOMENTS);
+        mark(h, ST_RESAMPLE);
+        HIP_TRY(h, launch_fixed_scan(h->stream, h->P[1].w, h->N, h->NB.w, h->B, &h->sc->joint_max_bits,
+                                     &h->sc->E_joint, h->cdf_j, h->blocksums, &h->sc->total_joint));
+        HIP_TRY(h, launch_resample(h->stream, ab, h->cdf_j, h->N + h->B, &h->sc->total_joint, &h->sc->E_joint,
+                                   h->P[1], h->NB, h->P[0], h->N, h->frame, k0, k1, h->ridx));
+    } else {
+        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[0].w, h->del_occ, h->del_free, 1.0f,
+                                      h->p.mass_scale, h->p.birth_prob, h->born, h->sc));
+        mark(h, ST_PERSIST);
+        mark(h, ST_NEWBORN);
+        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, &h->sc->born_max_bits, &h->sc->E_born,
+                                     h->cdf_v, h->blocksums, &h->sc->total_born));
+        HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->N, h->born, h->bnd,
+                                 nullptr));
+        HIP_TRY(h, launch_init_particles(h->stream, ab, h->bnd, h->V, h->N, h->gs, h->P[0], h->p.init_max_velocity,
+                                         h->p.particle_min_vel, h->frame, k0, k1));
+        mark(h, ST_MOMENTS);
+        mark(h, ST_RESAMPLE);
+    }
+    mark(h, ST_FRAME);
+    HIP_TRY(h, hipMemcpyAsync(h->h_sc, h->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->stream));
     return K3DOM_OK;
 }
 
 k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
     VoxelArrays& v = h->vox[h->vcur];
     const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
+    const bool normal = h->updated_time > 0;
     if (h->timing) for (float& x : h->stage_ms) x = 0.0f;
-    mark(h, ST_COUNT);  // frame start
+    mark(h, EV_FRAME_START);
     HIP_TRY(h, launch_reset_scalars(h->stream, h->sc));
-    if (h->updated_time > 0) {
+    float decay = 1.0f;
+    if (normal) {
         const float dt = (float)(t - h->updated_time);
-        const float decay = (float)std::pow((double)h->p.gamma, (double)dt);
+        decay = (float)std::pow((double)h->p.gamma, (double)dt);
         K3PredictConst c;
         c.dt = dt;
         c.p_S = h->p.persistence_prob;
@@ -276,51 +346,30 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
                                      &h->last_perm));
         HIP_TRY(h, launch_gather_sorted(h->stream, h->P[0], h->P[1], h->last_perm, h->last_skeys, h->N));
         HIP_TRY(h, launch_ranges(h->stream, h->P[1].key, h->N, v.start, v.end, h->V));
-        k3dom_status s = run_bgki(h, dx, dy, dz, m);
-        if (s != K3DOM_OK) return s;
-        mark(h, ST_MASS);
-        HIP_TRY(h, launch_mass_update(h->stream, v, h->V, h->P[1].w, h->del_occ, h->del_free, decay,
-                                      h->p.mass_scale, h->p.birth_prob, h->born, h->sc));
-        mark(h, ST_PERSIST);
-        HIP_TRY(h, launch_voxel_particles(h->stream, v, h->V, h->P[1], h->p.particle_min_vel, h->sc));
-        mark(h, ST_NEWBORN);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, &h->sc->born_max_bits, &h->sc->E_born,
-                                     h->cdf_v, h->blocksums, &h->sc->total_born));
-        HIP_TRY(h, launch_bounds(h->stream, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->B, h->born, h->bnd,
-                                 v.wua));
-        HIP_TRY(h, launch_newborn(h->stream, h->bnd, h->V, h->B, h->gs, v.wua, h->NB, h->p.init_max_velocity,
-                                  h->p.particle_min_vel, h->frame, k0, k1, h->sc));
-        mark(h, ST_MOMENTS);
-        mark(h, ST_RESAMPLE);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->P[1].w, h->N, h->NB.w, h->B, &h->sc->joint_max_bits,
-                                     &h->sc->E_joint, h->cdf_j, h->blocksums, &h->sc->total_joint));
-        HIP_TRY(h, launch_resample(h->stream, h->cdf_j, h->N + h->B, &h->sc->total_joint, &h->sc->E_joint, h->P[1],
-                                   h->NB, h->P[0], h->N, h->frame, k0, k1, h->ridx));
-    } else {
-        k3dom_status s = run_bgki(h, dx, dy, dz, m);
-        if (s != K3DOM_OK) return s;
-        mark(h, ST_MASS);
-        HIP_TRY(h, launch_mass_update(h->stream, v, h->V, h->P[0].w, h->del_occ, h->del_free, 1.0f,
-                                      h->p.mass_scale, h->p.birth_prob, h->born, h->sc));
-        mark(h, ST_PERSIST);
-        mark(h, ST_NEWBORN);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, &h->sc->born_max_bits, &h->sc->E_born,
-                                     h->cdf_v, h->blocksums, &h->sc->total_born));
-        HIP_TRY(h, launch_bounds(h->stream, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->N, h->born, h->bnd,
-                                 nullptr));
-        HIP_TRY(h, launch_init_particles(h->stream, h->bnd, h->V, h->N, h->gs, h->P[0], h->p.init_max_velocity,
-                                         h->p.particle_min_vel, h->frame, k0, k1));
-        mark(h, ST_MOMENTS);
-        mark(h, ST_RESAMPLE);
     }
-    mark(h, ST_FRAME);
-    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    for (int attempt = 0;; ++attempt) {
+        k3dom_status s = enqueue_rest(h, normal, decay, dx, dy, dz, m);
+        if (s != K3DOM_OK) return s;
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        if (!h->h_sc->overflow) break;
+        if (attempt >= 8) return fail(h, K3DOM_ERR_OUT_OF_MEMORY, "kernel-inference capacities keep overflowing");
+        // grow what overflowed (the frame's need is known up to that stage) and re-run
+        long long want[3], want_heavy = h->ibuf.heavy_cap;
+        for (int l = 0; l < 3; ++l) {
+            want[l] = h->lbuf[l].list_cap;
+            if (h->h_sc->overflow_mask & (1u << l))
+                want[l] = std::max(want[l] * 2, (long long)h->h_sc->level_total[l] * 3 / 2 + 1024);
+        }
+        if (h->h_sc->overflow_mask & (1u << 4))
+            want_heavy = std::max(want_heavy * 2, (long long)h->h_sc->n_heavy * 3 / 2 + 256);
+        s = ensure_caps(h, want, want_heavy);
+        if (s != K3DOM_OK) return s;
+        HIP_TRY(h, launch_clear_overflow(h->stream, h->sc));
+        h->overflow_retries++;
+    }
     if (h->timing) {
-        // stage boundaries: events ST_PREDICT..ST_FRAME in launch order; the
-        // frame start is ev[ST_COUNT]
         const int order[] = {ST_PREDICT, ST_SORT, ST_LISTS, ST_BGKI, ST_MASS, ST_PERSIST, ST_NEWBORN, ST_MOMENTS,
                              ST_RESAMPLE, ST_FRAME};
-        bool normal = h->updated_time > 0;
         for (int k = 0; k + 1 < 10; ++k) {
             int a = order[k], b = order[k + 1];
             if (!normal && (a == ST_PREDICT || a == ST_SORT)) continue;
@@ -328,8 +377,15 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
             if (hipEventElapsedTime(&ms, h->ev[a], h->ev[b]) == hipSuccess) h->stage_ms[a] = ms;
         }
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, h->ev[ST_COUNT], h->ev[ST_FRAME]) == hipSuccess) h->stage_ms[ST_FRAME] = ms;
+        if (hipEventElapsedTime(&ms, h->ev[EV_FRAME_START], h->ev[ST_FRAME]) == hipSuccess) h->stage_ms[ST_FRAME] = ms;
+        if (hipEventElapsedTime(&ms, h->ev[ST_BGKI_KERNEL], h->ev[EV_BGKI_END]) == hipSuccess)
+            h->stage_ms[ST_BGKI_KERNEL] = ms;
     }
+    h->stats.points = (uint64_t)m;
+    for (int l = 0; l < 3; ++l) h->stats.level_entries[l] = h->h_sc->level_total[l];
+    h->stats.gather_items = (uint64_t)h->h_sc->n_light + h->h_sc->n_heavy;
+    h->stats.gather_split_items = h->h_sc->n_heavy;
+    h->stats.capacity_regrowths = (uint64_t)h->overflow_retries;
     h->updated_time = t;
     h->frame++;
     return K3DOM_OK;
@@ -494,6 +550,9 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     h->nty = h->ntx;
     h->ntz = (h->gsz + 3) / 4;
     h->ntiles = h->ntx * h->nty * h->ntz;
+    h->tg.ntx = h->ntx;
+    h->tg.nty = h->nty;
+    h->tg.ntz = h->ntz;
     CREATE_TRY(hipGetDevice(&h->device));
     hipDeviceProp_t prop;
     CREATE_TRY(hipGetDeviceProperties(&prop, h->device));
@@ -522,7 +581,8 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(dalloc(h, &h->sort.k1, N));
     CREATE_TRY(dalloc(h, &h->sort.i0, N));
     CREATE_TRY(dalloc(h, &h->sort.i1, N));
-    CREATE_TRY(dalloc(h, &h->sort.hist, (size_t)256 * sort_blocks(h->N)));
+    CREATE_TRY(dalloc(h, &h->sort.hist, (size_t)256 * sort_blocks(h->N) + 1));
+    CREATE_TRY(dalloc(h, &h->sort.bsum, (size_t)scan_blocks(256ll * sort_blocks(h->N))));
     CREATE_TRY(dalloc(h, &h->cdf_v, V));
     CREATE_TRY(dalloc(h, &h->cdf_j, N + B));
     CREATE_TRY(dalloc(h, &h->blocksums, (size_t)fixed_scan_blocks((int)std::max(V, N + B))));
@@ -531,14 +591,29 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(dalloc(h, &h->sc, 1));
     setup_levels(h);
     for (int l = 0; l < 3; ++l) {
-        CREATE_TRY(dalloc(h, &h->lev[l].len, (size_t)h->lev[l].nboxes));
-        CREATE_TRY(dalloc(h, &h->lev[l].off, (size_t)h->lev[l].nboxes));
+        LevelBufs& LB = h->lbuf[l];
+        const LevelDesc& L = h->ldesc[l];
+        const size_t nboxes = (size_t)L.nbx * L.nby * L.nbz;
+        CREATE_TRY(dalloc(h, &LB.nseg, (size_t)L.nparents));
+        CREATE_TRY(dalloc(h, &LB.item_base, (size_t)L.nparents + 1));
+        CREATE_TRY(dalloc(h, &LB.box_off, nboxes));
+        CREATE_TRY(dalloc(h, &LB.box_len, nboxes));
     }
     const size_t T = (size_t)h->ntiles;
-    for (uint32_t** a : {&h->heavy_cnt, &h->light_cnt, &h->heavy_off, &h->light_off}) CREATE_TRY(dalloc(h, a, T));
-    CREATE_TRY(dalloc(h, &h->tile_partial, 2 * T));
+    ItemBufs& I = h->ibuf;
+    CREATE_TRY(dalloc(h, &I.heavy_cnt, T));
+    CREATE_TRY(dalloc(h, &I.light_cnt, T));
+    CREATE_TRY(dalloc(h, &I.heavy_off, T + 1));
+    CREATE_TRY(dalloc(h, &I.light_off, T + 1));
+    CREATE_TRY(dalloc(h, &I.bsum, (size_t)scan_blocks((long long)T)));
+    CREATE_TRY(dalloc(h, &I.tile_partial, 2 * T));
+    {
+        const char* tiny = std::getenv("K3DOM_TEST_TINY_CAPACITY");
+        h->tiny_caps = tiny && tiny[0] == '1';
+    }
     CREATE_TRY(hipHostMalloc((void**)&h->h_sc, sizeof(DevScalars), hipHostMallocDefault));
-    for (int e = 0; e <= ST_COUNT; ++e) CREATE_TRY(hipEventCreate(&h->ev[e]));
+    for (int e = 0; e < EV_COUNT; ++e) CREATE_TRY(hipEventCreate(&h->ev[e]));
+    CREATE_TRY(dalloc(h, &h->pair_counts, 2));
     // prior fill and zeroed particle state (src/common/kernel/init.cu:20-35)
     for (int b = 0; b < 2; ++b)
         CREATE_TRY(launch_fill_voxels(h->stream, h->vox[b], h->V, h->p.prior_free, h->p.prior_static,
@@ -566,7 +641,7 @@ k3dom_status k3dom_destroy(k3dom_handle* h) {
     h->allocs.clear();
     if (h->h_pts) (void)hipHostFree(h->h_pts);
     if (h->h_sc) (void)hipHostFree(h->h_sc);
-    for (int e = 0; e <= ST_COUNT; ++e)
+    for (int e = 0; e < EV_COUNT; ++e)
         if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -746,6 +821,27 @@ k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable) {
 k3dom_status k3dom_stage_times(k3dom_handle* h, float* ms, int32_t count) {
     if (!h || !ms || count < 0) return K3DOM_ERR_INVALID_ARG;
     for (int i = 0; i < count && i < ST_COUNT; ++i) ms[i] = h->stage_ms[i];
+    return K3DOM_OK;
+}
+
+k3dom_status k3dom_get_stats(const k3dom_handle* h, k3dom_stats* out) {
+    if (!h || !out) return K3DOM_ERR_INVALID_ARG;
+    *out = h->stats;
+    return K3DOM_OK;
+}
+
+k3dom_status k3dom_count_bgki_pairs(k3dom_handle* h, uint64_t* out2) {
+    if (!h || !out2) return K3DOM_ERR_INVALID_ARG;
+    if (h->frame == 0) return fail(h, K3DOM_ERR_STATE, "no frame processed yet");
+    HIP_TRY(h, hipMemsetAsync(h->pair_counts, 0, 2 * sizeof(unsigned long long), h->stream));
+    const uint32_t n_items = h->h_sc->n_light + h->h_sc->n_heavy;
+    HIP_TRY(h, launch_bgki_count(h->stream, h->num_cus * 8, h->last_geom, h->tg, h->p.ls, h->ibuf, n_items,
+                                 h->lbuf[2], h->rec, h->pair_counts));
+    unsigned long long c[2];
+    HIP_TRY(h, hipMemcpyAsync(c, h->pair_counts, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    out2[0] = c[0];
+    out2[1] = c[1];
     return K3DOM_OK;
 }
 

@@ -1,0 +1,652 @@
+// Kernel-inference (BGKI) evidence of one frame on gfx950, replacing the
+// reference's all-pairs loop (src/k3dom/kernel/mass_update.cu:32-77, called
+// once per voxel at :176).
+//
+// 1. k_point_prep: one 48-byte record per point (k3_point_prep).
+// 2. Candidate lists: for a hierarchy of boxes of 4x4x4-voxel tiles, the
+//    ascending list of points whose segment [sensor, point] passes within
+//    ls + box radius of the box (a superset of the points with a non-zero
+//    term for any voxel of the box).  A work item is a segment of a parent's
+//    list; its block tests each candidate against every child box once.
+//    count -> exclusive scan -> ordered fill keeps every list ascending.
+// 3. k_bgki: one wavefront per work item, lane = voxel of a tile; candidates
+//    are staged in LDS and visited in ascending order by all lanes.  Short
+//    lists are one item; long lists are split into one item per aligned
+//    2^K3_BGKI_CHUNK_LOG2-point chunk whose partial sums k_bgki_combine adds
+//    in chunk order.  Both realise the two-level association of k3_math.h,
+//    so the sums are bit-identical to the serial restatement.
+// Capacities are fixed per frame; when a list or the item array would
+// overflow, the kernels set DevScalars::overflow and the host grows the
+// buffers and re-runs this stage and the frame tail.
+#include "k3_kernels.h"
+
+#include <algorithm>
+
+namespace k3 {
+
+namespace {
+constexpr int TPB = 256;
+constexpr int BG_WAVES = TPB / 64;
+constexpr uint32_t BG_HEAVY_LEN = 1u << K3_BGKI_CHUNK_LOG2;
+constexpr uint32_t BG_DIRECT = 0xffffffffu;
+
+inline unsigned blocks_for(long long n, int per_block = TPB) {
+    long long b = (n + per_block - 1) / per_block;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+}  // namespace
+
+__device__ __forceinline__ uint32_t bg_lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__device__ __forceinline__ void bg_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Correctly rounded square root of x >= 0: the hardware estimate corrected by
+// one unit in the last place from the signs of the two neighbours' residuals.
+// Exact for x == 0 and x >= 2^-96; when some lane of the wave holds a
+// non-zero x below that, the wave takes sqrtf (same result, with rescaling).
+__device__ __forceinline__ float bg_sqrt(float x) {
+    if (__any(x > 0.0f && x < 0x1p-96f)) return sqrtf(x);
+    const float y = __builtin_amdgcn_sqrtf(x);
+    const uint32_t yb = __builtin_bit_cast(uint32_t, y);
+    const float ym = __builtin_bit_cast(float, yb - 1u);
+    const float yp = __builtin_bit_cast(float, yb + 1u);
+    float r = y;
+    r = fmaf(-ym, y, x) <= 0.0f ? ym : r;
+    r = fmaf(-yp, y, x) > 0.0f ? yp : r;
+    return r;
+}
+
+// ============================================================== points
+__global__ __launch_bounds__(TPB) void k_point_prep(const float* __restrict__ xs, const float* __restrict__ ys,
+                                                    const float* __restrict__ zs, int m, float res,
+                                                    K3PointRec* __restrict__ rec, float4* __restrict__ pxyz) {
+    int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= m) return;
+    K3PointRec r = k3_point_prep(xs[i], ys[i], zs[i], res);
+    rec[i] = r;
+    pxyz[i] = make_float4(r.x, r.y, r.z, r.valid != 0.0f ? r.dm : -1.0f);
+}
+hipError_t launch_point_prep(hipStream_t st, const float* xs, const float* ys, const float* zs, int m, float res,
+                             K3PointRec* rec, float4* pxyz) {
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(m)), dim3(TPB), 0, st, xs, ys, zs, m, res, rec, pxyz);
+    return hipGetLastError();
+}
+
+// ============================================================== geometry
+// Sensor-relative coordinate of the centre of voxel index i along an axis of
+// n voxels (the per-axis part of k3_voxel_center).
+__device__ __forceinline__ float axis_center(int i, int n, float res, float s) {
+    return (((float)i + 0.5f) - (float)n / 2.0f) * res - s;
+}
+
+// Bounding sphere of the voxel centres of tiles [t0, t1) per axis, as
+// (centre, ls + radius + slack); every voxel centre of the box lies within the
+// radius of the centre.
+__device__ float4 box_sphere(const GridGeom& g, float ls, int tx0, int ty0, int tz0, int tx1, int ty1, int tz1) {
+    int i0 = tx0 * 4, i1 = min(tx1 * 4, g.gs) - 1;
+    int j0 = ty0 * 4, j1 = min(ty1 * 4, g.gs) - 1;
+    int k0 = tz0 * 4, k1 = min(tz1 * 4, g.gsz) - 1;
+    float x0 = axis_center(i0, g.gs, g.res, g.sx), x1 = axis_center(i1, g.gs, g.res, g.sx);
+    float y0 = axis_center(j0, g.gs, g.res, g.sy), y1 = axis_center(j1, g.gs, g.res, g.sy);
+    float z0 = axis_center(k0, g.gsz, g.res, g.sz), z1 = axis_center(k1, g.gsz, g.res, g.sz);
+    float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1), cz = 0.5f * (z0 + z1);
+    float hx = 0.5f * (x1 - x0), hy = 0.5f * (y1 - y0), hz = 0.5f * (z1 - z0);
+    float R = sqrtf(hx * hx + hy * hy + hz * hz);
+    float cn = fabsf(cx) + fabsf(cy) + fabsf(cz);
+    return make_float4(cx, cy, cz, ls + R + 2e-3f + 1e-5f * cn);
+}
+
+struct Cand {
+    float x, y, z, inv_mm, slack;
+    bool valid;
+};
+__device__ __forceinline__ Cand make_cand(float4 p, bool in_range) {
+    Cand c;
+    c.x = p.x; c.y = p.y; c.z = p.z;
+    float mm = p.x * p.x + p.y * p.y + p.z * p.z;
+    c.inv_mm = mm > 0.0f ? 1.0f / mm : 0.0f;
+    c.slack = 1e-5f * p.w;
+    c.valid = in_range && p.w >= 0.0f;
+    return c;
+}
+// Distance from the box centre to the segment [0, p] below the threshold:
+// true for every box holding a voxel with a non-zero term for the point.
+__device__ __forceinline__ bool cand_hits(const float4& b, const Cand& c) {
+    float t = fminf(fmaxf((b.x * c.x + b.y * c.y + b.z * c.z) * c.inv_mm, 0.0f), 1.0f);
+    float dx = b.x - t * c.x, dy = b.y - t * c.y, dz = b.z - t * c.z;
+    float thr = b.w + c.slack;
+    return c.valid && b.w >= 0.0f && (dx * dx + dy * dy + dz * dz < thr * thr);
+}
+
+// Child c of parent P at this level (level 0: the box with id c).  Returns the
+// box id, or -1 when the child lies outside the tile grid.
+__device__ int child_box(int level, const LevelDesc& L, const TileGeom& tg, int P, int c, int* t0, int* t1) {
+    int bxi, byi, bzi;
+    if (level == 0) {
+        bxi = c % L.nbx;
+        byi = (c / L.nbx) % L.nby;
+        bzi = c / (L.nbx * L.nby);
+    } else {
+        int pxi = P % L.pnbx, pyi = (P / L.pnbx) % L.pnby, pzi = P / (L.pnbx * L.pnby);
+        int cx = c % L.ncx, cy = (c / L.ncx) % L.ncy, cz = c / (L.ncx * L.ncy);
+        bxi = pxi * L.ncx + cx;
+        byi = pyi * L.ncy + cy;
+        bzi = pzi * L.ncz + cz;
+        if (bxi >= L.nbx || byi >= L.nby || bzi >= L.nbz) return -1;
+    }
+    t0[0] = bxi * L.bx; t0[1] = byi * L.by; t0[2] = bzi * L.bz;
+    t1[0] = min(t0[0] + L.bx, tg.ntx); t1[1] = min(t0[1] + L.by, tg.nty); t1[2] = min(t0[2] + L.bz, tg.ntz);
+    return bxi + L.nbx * (byi + L.nby * bzi);
+}
+
+__device__ __forceinline__ int find_parent(const uint32_t* __restrict__ item_base, int nparents, uint32_t item) {
+    int lo = 0, hi = nparents - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (item_base[mid] <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ============================================================== candidate lists
+__global__ __launch_bounds__(TPB) void k_level_nseg(int level, LevelDesc L, int m,
+                                                    const uint32_t* __restrict__ parent_len,
+                                                    uint32_t* __restrict__ nseg, const DevScalars* sc) {
+    int P = blockIdx.x * TPB + threadIdx.x;
+    if (P >= L.nparents) return;
+    uint32_t len = level == 0 ? (uint32_t)m : (sc->overflow ? 0u : parent_len[P]);
+    nseg[P] = (len + (uint32_t)L.seg - 1) / (uint32_t)L.seg;
+}
+
+struct ItemCtx {
+    int P;
+    uint32_t seg, nsegP, ib, begin, end, poff;
+};
+__device__ __forceinline__ ItemCtx item_ctx(int level, const LevelDesc& L, uint32_t item,
+                                            const uint32_t* __restrict__ item_base,
+                                            const uint32_t* __restrict__ parent_off,
+                                            const uint32_t* __restrict__ parent_len, int m) {
+    ItemCtx x;
+    x.P = find_parent(item_base, L.nparents, item);
+    x.ib = item_base[x.P];
+    x.nsegP = item_base[x.P + 1] - x.ib;
+    x.seg = item - x.ib;
+    uint32_t plen = level ? parent_len[x.P] : (uint32_t)m;
+    x.poff = level ? parent_off[x.P] : 0u;
+    x.begin = x.seg * (uint32_t)L.seg;
+    x.end = min(plen, x.begin + (uint32_t)L.seg);
+    return x;
+}
+
+__global__ __launch_bounds__(TPB) void k_level_count(int level, GridGeom g, TileGeom tg, LevelDesc L, float ls,
+                                                     const float4* __restrict__ pxyz, int m,
+                                                     const uint32_t* __restrict__ parent_off,
+                                                     const uint32_t* __restrict__ parent_len,
+                                                     const uint32_t* __restrict__ parent_list,
+                                                     const uint32_t* __restrict__ item_base,
+                                                     uint32_t* __restrict__ counts, const DevScalars* sc) {
+    __shared__ float4 cb[MAX_CHILDREN];
+    __shared__ uint32_t cnt[MAX_CHILDREN];
+    if (sc->overflow) return;
+    const uint32_t item = blockIdx.x;
+    if (item >= item_base[L.nparents]) return;
+    const int t = threadIdx.x, lane = t & 63;
+    const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
+    for (int c = t; c < L.nch; c += TPB) {
+        int t0[3], t1[3];
+        int box = child_box(level, L, tg, x.P, c, t0, t1);
+        cb[c] = box < 0 ? make_float4(0.0f, 0.0f, 0.0f, -1.0f)
+                        : box_sphere(g, ls, t0[0], t0[1], t0[2], t1[0], t1[1], t1[2]);
+        cnt[c] = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = x.begin; base < x.end; base += TPB) {
+        const uint32_t e = base + t;
+        const bool in = e < x.end;
+        const uint32_t idx = in ? (level ? parent_list[x.poff + e] : e) : 0u;
+        const Cand cd = make_cand(pxyz[idx], in);
+        for (int c = 0; c < L.nch; ++c) {
+            uint64_t bal = __ballot(cand_hits(cb[c], cd));
+            if (lane == 0 && bal) atomicAdd(&cnt[c], (uint32_t)__popcll(bal));
+        }
+    }
+    __syncthreads();
+    for (int c = t; c < L.nch; c += TPB) counts[x.ib * L.nch + c * x.nsegP + x.seg] = cnt[c];
+}
+
+__global__ __launch_bounds__(TPB) void k_level_fill(int level, GridGeom g, TileGeom tg, LevelDesc L, float ls,
+                                                    const float4* __restrict__ pxyz, int m,
+                                                    const uint32_t* __restrict__ parent_off,
+                                                    const uint32_t* __restrict__ parent_len,
+                                                    const uint32_t* __restrict__ parent_list,
+                                                    const uint32_t* __restrict__ item_base,
+                                                    const uint32_t* __restrict__ slot_off,
+                                                    uint32_t* __restrict__ list, long long list_cap,
+                                                    DevScalars* sc) {
+    __shared__ float4 cb[MAX_CHILDREN];
+    __shared__ uint32_t ob[MAX_CHILDREN];
+    __shared__ uint32_t run[MAX_CHILDREN];
+    __shared__ uint32_t wc[4];
+    if (sc->overflow) return;
+    const uint32_t n_items = item_base[L.nparents];
+    const uint32_t total = slot_off[n_items * (uint32_t)L.nch];
+    if ((long long)total > list_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            sc->level_total[level] = total;
+            atomicOr(&sc->overflow_mask, 1u << level);
+            atomicExch(&sc->overflow, 1u);
+        }
+        return;
+    }
+    const uint32_t item = blockIdx.x;
+    if (item >= n_items) return;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
+    for (int c = t; c < L.nch; c += TPB) {
+        int t0[3], t1[3];
+        int box = child_box(level, L, tg, x.P, c, t0, t1);
+        cb[c] = box < 0 ? make_float4(0.0f, 0.0f, 0.0f, -1.0f)
+                        : box_sphere(g, ls, t0[0], t0[1], t0[2], t1[0], t1[1], t1[2]);
+        ob[c] = slot_off[x.ib * L.nch + c * x.nsegP + x.seg];
+        run[c] = 0;
+    }
+    __syncthreads();
+    for (uint32_t base = x.begin; base < x.end; base += TPB) {
+        const uint32_t e = base + t;
+        const bool in = e < x.end;
+        const uint32_t idx = in ? (level ? parent_list[x.poff + e] : e) : 0u;
+        const Cand cd = make_cand(pxyz[idx], in);
+        for (int c = 0; c < L.nch; ++c) {
+            const bool pass = cand_hits(cb[c], cd);
+            const uint64_t bal = __ballot(pass);
+            if (lane == 0) wc[w] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t pre = 0;
+            for (int ww = 0; ww < w; ++ww) pre += wc[ww];
+            const uint32_t tot = wc[0] + wc[1] + wc[2] + wc[3];
+            if (pass) list[ob[c] + run[c] + pre + bg_lanes_below(bal)] = idx;
+            __syncthreads();
+            if (t == 0) run[c] += tot;
+        }
+    }
+    if (item == 0 && t == 0) sc->level_total[level] = total;
+}
+
+__global__ __launch_bounds__(TPB) void k_level_boxes(int level, LevelDesc L, const uint32_t* __restrict__ item_base,
+                                                     const uint32_t* __restrict__ slot_off,
+                                                     uint32_t* __restrict__ box_off, uint32_t* __restrict__ box_len,
+                                                     const DevScalars* sc) {
+    const int box = blockIdx.x * TPB + threadIdx.x;
+    if (box >= L.nbx * L.nby * L.nbz) return;
+    if (sc->overflow) {
+        box_off[box] = 0;
+        box_len[box] = 0;
+        return;
+    }
+    int P, c;
+    if (level == 0) {
+        P = 0;
+        c = box;
+    } else {
+        int bxi = box % L.nbx, byi = (box / L.nbx) % L.nby, bzi = box / (L.nbx * L.nby);
+        P = (bxi / L.ncx) + L.pnbx * ((byi / L.ncy) + L.pnby * (bzi / L.ncz));
+        c = (bxi % L.ncx) + L.ncx * ((byi % L.ncy) + L.ncy * (bzi % L.ncz));
+    }
+    const uint32_t ib = item_base[P], ns = item_base[P + 1] - ib;
+    uint32_t off = 0, len = 0;
+    if (ns > 0) {
+        uint32_t s0 = ib * L.nch + c * ns;
+        off = slot_off[s0];
+        len = slot_off[s0 + ns] - off;
+    }
+    box_off[box] = off;
+    box_len[box] = len;
+}
+
+hipError_t launch_build_level(hipStream_t st, int level, const GridGeom& g, const TileGeom& tg, const LevelDesc& L,
+                              const LevelBufs& B, const LevelBufs* parent, float ls, const float4* pxyz, int m,
+                              DevScalars* sc) {
+    const uint32_t* poff = parent ? parent->box_off : nullptr;
+    const uint32_t* plen = parent ? parent->box_len : nullptr;
+    const uint32_t* plist = parent ? parent->list : nullptr;
+    hipLaunchKernelGGL(k_level_nseg, dim3(blocks_for(L.nparents)), dim3(TPB), 0, st, level, L, m, plen, B.nseg, sc);
+    (void)launch_scan_u32(st, B.nseg, B.item_base, B.bsum, L.nparents, nullptr, 1, &sc->level_items[level]);
+    const unsigned grid = (unsigned)std::max(1ll, B.items_cap);
+    hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(TPB), 0, st, level, g, tg, L, ls, pxyz, m, poff, plen, plist,
+                       B.item_base, B.counts, sc);
+    (void)launch_scan_u32(st, B.counts, B.slot_off, B.bsum, B.slot_cap, B.item_base + L.nparents, (uint32_t)L.nch,
+                    nullptr);
+    hipLaunchKernelGGL(k_level_fill, dim3(grid), dim3(TPB), 0, st, level, g, tg, L, ls, pxyz, m, poff, plen, plist,
+                       B.item_base, B.slot_off, B.list, B.list_cap, sc);
+    hipLaunchKernelGGL(k_level_boxes, dim3(blocks_for((long long)L.nbx * L.nby * L.nbz)), dim3(TPB), 0, st, level, L,
+                       B.item_base, B.slot_off, B.box_off, B.box_len, sc);
+    return hipGetLastError();
+}
+
+// ============================================================== work items
+__global__ __launch_bounds__(TPB) void k_count_items(int ntiles, const uint32_t* __restrict__ toff,
+                                                     const uint32_t* __restrict__ tlen,
+                                                     const uint32_t* __restrict__ tlist,
+                                                     uint32_t* __restrict__ heavy_cnt,
+                                                     uint32_t* __restrict__ light_cnt, const DevScalars* sc) {
+    int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= ntiles) return;
+    uint32_t len = sc->overflow ? 0u : tlen[t];
+    if (len > BG_HEAVY_LEN) {
+        uint32_t c0 = tlist[toff[t]] >> K3_BGKI_CHUNK_LOG2;
+        uint32_t c1 = tlist[toff[t] + len - 1] >> K3_BGKI_CHUNK_LOG2;
+        heavy_cnt[t] = c1 - c0 + 1;
+        light_cnt[t] = 0;
+    } else {
+        heavy_cnt[t] = 0;
+        light_cnt[t] = 1;
+    }
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t v) {
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(TPB) void k_emit_light(int ntiles, const uint32_t* __restrict__ toff,
+                                                    const uint32_t* __restrict__ tlen,
+                                                    const uint32_t* __restrict__ light_cnt,
+                                                    const uint32_t* __restrict__ light_off, uint4* __restrict__ items,
+                                                    uint32_t* __restrict__ tile_partial, long long heavy_cap,
+                                                    const DevScalars* sc) {
+    int t = blockIdx.x * TPB + threadIdx.x;
+    // items[] holds ntiles + heavy_cap entries: with more split items than
+    // that, k_emit_heavy flags the overflow and nothing may be written here
+    if (t >= ntiles || sc->overflow || (long long)sc->n_heavy > heavy_cap) return;
+    if (light_cnt[t]) {
+        uint32_t o = toff[t];
+        items[sc->n_heavy + light_off[t]] = make_uint4((uint32_t)t, o, o + tlen[t], BG_DIRECT);
+        tile_partial[2 * t] = 0;
+        tile_partial[2 * t + 1] = 0;
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_emit_heavy(int ntiles, const uint32_t* __restrict__ toff,
+                                                    const uint32_t* __restrict__ tlen,
+                                                    const uint32_t* __restrict__ tlist,
+                                                    const uint32_t* __restrict__ heavy_cnt,
+                                                    const uint32_t* __restrict__ heavy_off, uint4* __restrict__ items,
+                                                    uint32_t* __restrict__ tile_partial, long long heavy_cap,
+                                                    DevScalars* sc) {
+    if (sc->overflow) return;
+    const uint32_t H = sc->n_heavy;
+    if ((long long)H > heavy_cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            atomicOr(&sc->overflow_mask, 1u << 4);
+            atomicExch(&sc->overflow, 1u);
+        }
+        return;
+    }
+    const uint32_t h = blockIdx.x * TPB + threadIdx.x;
+    if (h >= H) return;
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (heavy_off[mid] <= h) lo = mid;
+        else hi = mid - 1;
+    }
+    const int t = lo;
+    const uint32_t k = h - heavy_off[t];
+    const uint32_t o = toff[t], len = tlen[t];
+    const uint32_t c = (tlist[o] >> K3_BGKI_CHUNK_LOG2) + k;
+    const uint32_t b = lower_bound_u32(tlist, o, o + len, c << K3_BGKI_CHUNK_LOG2);
+    const uint32_t e = lower_bound_u32(tlist, o, o + len, (c + 1) << K3_BGKI_CHUNK_LOG2);
+    items[h] = make_uint4((uint32_t)t, b, e, h);
+    if (k == 0) {
+        tile_partial[2 * t] = h;
+        tile_partial[2 * t + 1] = heavy_cnt[t];
+    }
+}
+
+hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& T, const ItemBufs& I, DevScalars* sc) {
+    hipLaunchKernelGGL(k_count_items, dim3(blocks_for(ntiles)), dim3(TPB), 0, st, ntiles, T.box_off, T.box_len,
+                       T.list, I.heavy_cnt, I.light_cnt, sc);
+    (void)launch_scan_u32(st, I.heavy_cnt, I.heavy_off, I.bsum, ntiles, nullptr, 1, &sc->n_heavy);
+    (void)launch_scan_u32(st, I.light_cnt, I.light_off, I.bsum, ntiles, nullptr, 1, &sc->n_light);
+    hipLaunchKernelGGL(k_emit_light, dim3(blocks_for(ntiles)), dim3(TPB), 0, st, ntiles, T.box_off, T.box_len,
+                       I.light_cnt, I.light_off, I.items, I.tile_partial, I.heavy_cap, sc);
+    hipLaunchKernelGGL(k_emit_heavy, dim3(blocks_for(std::max(1ll, I.heavy_cap))), dim3(TPB), 0, st, ntiles,
+                       T.box_off, T.box_len, T.list, I.heavy_cnt, I.heavy_off, I.items, I.tile_partial, I.heavy_cap,
+                       sc);
+    return hipGetLastError();
+}
+
+// ============================================================== gather
+// Per-candidate classes, decided once per (tile, point) by the staging lane
+// from the tile's bounding box with a safety margin, so that inside a class
+// the skipped branches are provably dead for every lane of the tile:
+//   C_GENERAL  the full three-case free distance,
+//   C_MAIN     every voxel projects strictly inside (0, mg): perpendicular case,
+//   C_BEHIND   every voxel projects to <= 0: free distance = |voxel|,
+//   C_NOFREE   every voxel projects beyond the point: no free term;
+// plus F_OCC when a voxel of the tile may lie within ls of the point.
+constexpr uint32_t C_GENERAL = 0, C_MAIN = 1, C_BEHIND = 2, C_NOFREE = 3, F_OCC = 4;
+
+__global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls, float ls2g, float inv_ls,
+                                              float sigma, const uint4* __restrict__ items,
+                                              const uint32_t* __restrict__ tlist,
+                                              const K3PointRec* __restrict__ rec, float* __restrict__ del_occ,
+                                              float* __restrict__ del_free, float* __restrict__ partial,
+                                              DevScalars* sc) {
+    __shared__ float4 srec[BG_WAVES][64 * 3];
+    __shared__ uint32_t sidx[BG_WAVES][64];
+    __shared__ uint32_t scls[BG_WAVES][64];
+    if (sc->overflow) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t n_items = sc->n_light + sc->n_heavy;
+    while (true) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(&sc->item_counter, 1u);
+        it = uniform(__shfl(it, 0));
+        if (it >= n_items) break;
+        const uint4 item = items[it];
+        const uint32_t tile = uniform(item.x), ib = uniform(item.y), ie = uniform(item.z), slot = uniform(item.w);
+        const int tx = (int)(tile % (uint32_t)tg.ntx), ty = (int)((tile / (uint32_t)tg.ntx) % (uint32_t)tg.nty);
+        const int tz = (int)(tile / (uint32_t)(tg.ntx * tg.nty));
+        const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
+        const float vx = axis_center(i, g.gs, g.res, g.sx);
+        const float vy = axis_center(j, g.gs, g.res, g.sy);
+        const float vz = axis_center(k, g.gsz, g.res, g.sz);
+        const float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
+        const float dc2 = dc * dc;
+        // free term of a voxel for every point in front of which it lies
+        const float kc = dc < ls ? k3_kernel_value(dc, inv_ls, sigma) : 0.0f;
+        // bounding box of the tile's 64 voxel centres (lanes outside the grid included)
+        const float tcx = 0.5f * (axis_center(tx * 4, g.gs, g.res, g.sx) + axis_center(tx * 4 + 3, g.gs, g.res, g.sx));
+        const float tcy = 0.5f * (axis_center(ty * 4, g.gs, g.res, g.sy) + axis_center(ty * 4 + 3, g.gs, g.res, g.sy));
+        const float tcz =
+            0.5f * (axis_center(tz * 4, g.gsz, g.res, g.sz) + axis_center(tz * 4 + 3, g.gsz, g.res, g.sz));
+        const float trad = 1.5f * g.res * 1.7320508f;
+        const float marg = 1e-3f + 1e-5f * (fabsf(tcx) + fabsf(tcy) + fabsf(tcz) + trad);
+        const float rr = trad + marg;
+        const float occ_r = ls + rr;
+        float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
+        uint32_t cur_chunk = 0xffffffffu;
+        for (uint32_t b = ib; b < ie; b += 64) {
+            const uint32_t e = b + lane;
+            if (e < ie) {
+                const uint32_t p = tlist[e];
+                const K3PointRec r = rec[p];
+                const float4* src = reinterpret_cast<const float4*>(rec + p);
+                sidx[w][lane] = p;
+                srec[w][lane * 3 + 0] = src[0];
+                srec[w][lane * 3 + 1] = src[1];
+                srec[w][lane * 3 + 2] = src[2];
+                const float proj = fmaf(tcz, r.z, fmaf(tcy, r.y, tcx * r.x)) * r.inv_dm;
+                uint32_t cls = C_GENERAL;
+                if (proj + rr <= 0.0f) cls = C_BEHIND;
+                else if (proj - rr > 0.0f && proj + rr < r.mg) cls = C_MAIN;
+                else if (proj - rr >= r.dm) cls = C_NOFREE;
+                const float ox = tcx - r.x, oy = tcy - r.y, oz = tcz - r.z;
+                if (!(ox * ox + oy * oy + oz * oz >= occ_r * occ_r)) cls |= F_OCC;
+                scls[w][lane] = cls;
+            }
+            bg_wave_sync();
+            const uint32_t cnt = uniform(min(64u, ie - b));
+            for (uint32_t q = 0; q < cnt; ++q) {
+                const uint32_t chunk = uniform(sidx[w][q]) >> K3_BGKI_CHUNK_LOG2;
+                const uint32_t cls = uniform(scls[w][q]);
+                if (chunk != cur_chunk) {
+                    tot_o += cur_o;
+                    tot_f += cur_f;
+                    cur_o = 0.0f;
+                    cur_f = 0.0f;
+                    cur_chunk = chunk;
+                }
+                // record: r0 = (x, y, z, inv_dm), r1 = (dm, mg, ex, ey), r2 = (ez, valid, -, -)
+                const float4 r0 = srec[w][q * 3 + 0];
+                if (cls & F_OCC) {  // occupied term (k3_occ_distance)
+                    const float ox = vx - r0.x, oy = vy - r0.y, oz = vz - r0.z;
+                    const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
+                    const bool co = so < ls2g;
+                    if (__any(co)) {
+                        const float d = bg_sqrt(so);
+                        const float kv = k3_kernel_value(d, inv_ls, sigma);
+                        cur_o += (co && d < ls) ? kv : 0.0f;
+                    }
+                }
+                const uint32_t fc = cls & 3u;
+                if (fc == C_MAIN) {  // 0 < dp < mg for every lane
+                    const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
+                    const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
+                    const bool cf = a2 < ls2g;
+                    if (__any(cf)) {
+                        const float d = bg_sqrt(a2);
+                        const float kv = k3_kernel_value(d, inv_ls, sigma);
+                        cur_f += (cf && d < ls) ? kv : 0.0f;
+                    }
+                } else if (fc == C_BEHIND) {  // dp <= 0 for every lane
+                    cur_f += kc;
+                } else if (fc == C_GENERAL) {  // k3_free_distance, the three cases as selects
+                    const float4 r1 = srec[w][q * 3 + 1];
+                    const float4 r2 = srec[w][q * 3 + 2];
+                    const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
+                    const bool c1 = dp <= 0.0f;
+                    const bool c2 = !c1 && dp < r1.y;
+                    const bool c3 = !c1 && !c2 && dp < r1.x;
+                    const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
+                    const float ex = vx - r1.z, ey = vy - r1.w, ez = vz - r2.x;
+                    const float a3 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+                    const float a = c2 ? a2 : a3;
+                    const bool cs = (c2 || c3) && a < ls2g;
+                    if (__any(cs)) {
+                        const float d = bg_sqrt(a);
+                        const float kv = k3_kernel_value(d, inv_ls, sigma);
+                        cur_f += cs ? (d < ls ? kv : 0.0f) : (c1 ? kc : 0.0f);
+                    } else {
+                        cur_f += c1 ? kc : 0.0f;
+                    }
+                }
+            }
+            bg_wave_sync();
+        }
+        tot_o += cur_o;
+        tot_f += cur_f;
+        if (slot == BG_DIRECT) {
+            if (i < g.gs && j < g.gs && k < g.gsz) {
+                const int cell = i + g.gs * j + g.gs * g.gs * k;
+                del_occ[cell] = tot_o;
+                del_free[cell] = tot_f;
+            }
+        } else {
+            partial[(size_t)slot * 128 + lane] = tot_o;
+            partial[(size_t)slot * 128 + 64 + lane] = tot_f;
+        }
+    }
+}
+hipError_t launch_bgki(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
+                       float sigma, const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec, float* del_occ,
+                       float* del_free, DevScalars* sc) {
+    hipLaunchKernelGGL(k_bgki, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls), 1.0f / ls, sigma,
+                       I.items, T.list, rec, del_occ, del_free, I.partial, sc);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(TPB) void k_bgki_combine(int ntiles, GridGeom g, TileGeom tg,
+                                                      const uint32_t* __restrict__ tile_partial,
+                                                      const float* __restrict__ partial, float* del_occ,
+                                                      float* del_free, const DevScalars* sc) {
+    if (sc->overflow) return;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * BG_WAVES + w;
+    if (tile >= ntiles) return;
+    const uint32_t first = tile_partial[2 * tile], cnt = tile_partial[2 * tile + 1];
+    if (cnt == 0) return;
+    const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+    const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
+    if (!(i < g.gs && j < g.gs && k < g.gsz)) return;
+    float ro = 0.0f, rf = 0.0f;
+    for (uint32_t c = 0; c < cnt; ++c) {
+        ro += partial[(size_t)(first + c) * 128 + lane];
+        rf += partial[(size_t)(first + c) * 128 + 64 + lane];
+    }
+    const int cell = i + g.gs * j + g.gs * g.gs * k;
+    del_occ[cell] = ro;
+    del_free[cell] = rf;
+}
+hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg, const ItemBufs& I,
+                               float* del_occ, float* del_free, DevScalars* sc) {
+    hipLaunchKernelGGL(k_bgki_combine, dim3(blocks_for(ntiles, BG_WAVES)), dim3(TPB), 0, st, ntiles, g, tg,
+                       I.tile_partial, I.partial, del_occ, del_free, sc);
+    return hipGetLastError();
+}
+
+// Pair counting over the same items (diagnostic): one wave per item.
+__global__ __launch_bounds__(TPB) void k_bgki_count(GridGeom g, TileGeom tg, float ls, float ls2g,
+                                                    const uint4* __restrict__ items, uint32_t n_items,
+                                                    const uint32_t* __restrict__ tlist,
+                                                    const K3PointRec* __restrict__ rec,
+                                                    unsigned long long* __restrict__ counts) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long co = 0, cf = 0;
+    for (uint32_t it = blockIdx.x * BG_WAVES + (threadIdx.x >> 6); it < n_items; it += gridDim.x * BG_WAVES) {
+        const uint4 item = items[it];
+        const int tile = (int)item.x;
+        const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+        const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
+        if (!(i < g.gs && j < g.gs && k < g.gsz)) continue;
+        const float vx = axis_center(i, g.gs, g.res, g.sx);
+        const float vy = axis_center(j, g.gs, g.res, g.sy);
+        const float vz = axis_center(k, g.gsz, g.res, g.sz);
+        const float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
+        const float dc2 = dc * dc;
+        for (uint32_t e = item.y; e < item.z; ++e) {
+            const K3PointRec r = rec[tlist[e]];
+            co += k3_occ_distance(vx, vy, vz, r, ls, ls2g) >= 0.0f;
+            cf += k3_free_distance(vx, vy, vz, dc, dc2, r, ls, ls2g) >= 0.0f;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        co += __shfl_xor(co, o);
+        cf += __shfl_xor(cf, o);
+    }
+    if (lane == 0) {
+        atomicAdd(&counts[0], co);
+        atomicAdd(&counts[1], cf);
+    }
+}
+hipError_t launch_bgki_count(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
+                             const ItemBufs& I, uint32_t n_items, const LevelBufs& T, const K3PointRec* rec,
+                             unsigned long long* counts) {
+    hipLaunchKernelGGL(k_bgki_count, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls), I.items,
+                       n_items, T.list, rec, counts);
+    return hipGetLastError();
+}
+
+}  // namespace k3

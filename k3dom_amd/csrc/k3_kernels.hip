@@ -25,6 +25,8 @@
 //   k_shift_*            src/common/kernel/ego_motion_compensation.cu:12-41
 #include "k3_kernels.h"
 
+#include <algorithm>
+
 namespace k3 {
 
 constexpr int TPB = 256;
@@ -180,16 +182,62 @@ __global__ __launch_bounds__(TPB) void k_radix_scatter(const uint32_t* __restric
     }
 }
 
-// Single-workgroup exclusive scan of u32 (n up to a few million).
-__global__ __launch_bounds__(1024) void k_scan_u32_single(const uint32_t* __restrict__ in,
-                                                          uint32_t* __restrict__ out, int n,
-                                                          uint32_t* __restrict__ total) {
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = t * per, e = min(n, b + per);
+// Exclusive scan of u32, three phases (tile sums, scan of tile sums, tile
+// scans).  The element count is n_host, or (*n_dev) * n_mul when n_dev is
+// given (a count produced on the device); the grid is sized for n_max.  The
+// total is also written to out[n] and, if given, to *total.  in may equal out.
+constexpr int SC_ITEMS = 8;
+constexpr int SC_TILE = TPB * SC_ITEMS;
+
+__device__ __forceinline__ uint32_t scan_count(const uint32_t* n_dev, uint32_t n_mul, uint32_t n_host) {
+    return n_dev ? (*n_dev) * n_mul : n_host;
+}
+
+// exclusive block scan of one value per thread (256 threads)
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* lds4, uint32_t* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(x, o);
+        if (lane >= o) x += u;
+    }
+    if (lane == 63) lds4[w] = x;
+    __syncthreads();
+    uint32_t add = 0;
+    for (int ww = 0; ww < w; ++ww) add += lds4[ww];
+    *total = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+    __syncthreads();
+    return x - v + add;
+}
+
+__global__ __launch_bounds__(TPB) void k_scan_reduce_u32(const uint32_t* __restrict__ in,
+                                                         const uint32_t* __restrict__ n_dev, uint32_t n_mul,
+                                                         uint32_t n_host, uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t n = scan_count(n_dev, n_mul, n_host);
+    const uint32_t base = blockIdx.x * SC_TILE;
+    if (base >= n) return;
+    const uint32_t b0 = base + threadIdx.x * SC_ITEMS;
     uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += in[i];
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) s += (b0 + k < n) ? in[b0 + k] : 0u;
+    uint32_t tot;
+    block_excl_scan_u32(s, lds4, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top_u32(uint32_t* __restrict__ bsum, const uint32_t* __restrict__ n_dev,
+                                                       uint32_t n_mul, uint32_t n_host, uint32_t* __restrict__ out,
+                                                       uint32_t* __restrict__ total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t n = scan_count(n_dev, n_mul, n_host);
+    const int nb = (int)((n + SC_TILE - 1) / SC_TILE);
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b = t * per, e = min(nb, b + per);
+    uint32_t s = 0;
+    for (int i = b; i < e; ++i) s += bsum[i];
     part[t] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -200,14 +248,95 @@ __global__ __launch_bounds__(1024) void k_scan_u32_single(const uint32_t* __rest
     }
     uint32_t run = part[t] - s;
     for (int i = b; i < e; ++i) {
-        uint32_t v = in[i];
-        out[i] = run;
+        uint32_t v = bsum[i];
+        bsum[i] = run;
         run += v;
     }
-    if (t == 1023 && total) *total = part[1023];
+    if (t == 1023) {
+        out[n] = part[1023];
+        if (total) *total = part[1023];
+    }
 }
-hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out_excl, int n, uint32_t* total) {
-    hipLaunchKernelGGL(k_scan_u32_single, dim3(1), dim3(1024), 0, st, in, out_excl, n, total);
+
+__global__ __launch_bounds__(TPB) void k_scan_down_u32(const uint32_t* in, const uint32_t* __restrict__ n_dev,
+                                                       uint32_t n_mul, uint32_t n_host,
+                                                       const uint32_t* __restrict__ bsum, uint32_t* out) {
+    __shared__ uint32_t lds4[4];
+    const uint32_t n = scan_count(n_dev, n_mul, n_host);
+    const uint32_t base = blockIdx.x * SC_TILE;
+    if (base >= n) return;
+    const uint32_t b0 = base + threadIdx.x * SC_ITEMS;
+    uint32_t v[SC_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) {
+        v[k] = (b0 + k < n) ? in[b0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan_u32(s, lds4, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) {
+        if (b0 + k < n) out[b0 + k] = run;
+        run += v[k];
+    }
+}
+
+int scan_blocks(long long n_max) { return (int)std::max(1ll, (n_max + SC_TILE - 1) / SC_TILE); }
+
+// One-workgroup scan for up to SC_SMALL elements (one launch instead of three).
+constexpr int SC_SMALL_ITEMS = 16;
+constexpr int SC_SMALL = 1024 * SC_SMALL_ITEMS;
+__global__ __launch_bounds__(1024) void k_scan_small_u32(const uint32_t* in, const uint32_t* __restrict__ n_dev,
+                                                         uint32_t n_mul, uint32_t n_host, uint32_t* out,
+                                                         uint32_t* __restrict__ total) {
+    __shared__ uint32_t wsum[16];
+    const uint32_t n = scan_count(n_dev, n_mul, n_host);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t b0 = (uint32_t)t * SC_SMALL_ITEMS;
+    uint32_t v[SC_SMALL_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SC_SMALL_ITEMS; ++k) {
+        v[k] = (b0 + k < n) ? in[b0 + k] : 0u;
+        s += v[k];
+    }
+    uint32_t x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(x, o);
+        if (lane >= o) x += u;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t add = 0, all = 0;
+    for (int ww = 0; ww < 16; ++ww) {
+        add += ww < w ? wsum[ww] : 0u;
+        all += wsum[ww];
+    }
+    uint32_t run = x - s + add;
+#pragma unroll
+    for (int k = 0; k < SC_SMALL_ITEMS; ++k) {
+        if (b0 + k < n) out[b0 + k] = run;
+        run += v[k];
+    }
+    if (t == 0) {
+        out[n] = all;
+        if (total) *total = all;
+    }
+}
+
+hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* bsum, long long n_max,
+                           const uint32_t* n_dev, uint32_t n_mul, uint32_t* total) {
+    if (n_max <= SC_SMALL) {
+        hipLaunchKernelGGL(k_scan_small_u32, dim3(1), dim3(1024), 0, st, in, n_dev, n_mul, (uint32_t)n_max, out, total);
+        return hipGetLastError();
+    }
+    const unsigned nb = (unsigned)scan_blocks(n_max);
+    const uint32_t nh = (uint32_t)n_max;
+    hipLaunchKernelGGL(k_scan_reduce_u32, dim3(nb), dim3(TPB), 0, st, in, n_dev, n_mul, nh, bsum);
+    hipLaunchKernelGGL(k_scan_top_u32, dim3(1), dim3(1024), 0, st, bsum, n_dev, n_mul, nh, out, total);
+    hipLaunchKernelGGL(k_scan_down_u32, dim3(nb), dim3(TPB), 0, st, in, n_dev, n_mul, nh, bsum, out);
     return hipGetLastError();
 }
 
@@ -222,8 +351,7 @@ hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int 
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(TPB), 0, st, kin, n, shift, s.hist, nblk);
-        hipLaunchKernelGGL(k_scan_u32_single, dim3(1), dim3(1024), 0, st, s.hist, s.hist, 256 * nblk,
-                           (uint32_t*)nullptr);
+        (void)launch_scan_u32(st, s.hist, s.hist, s.bsum, 256ll * nblk, nullptr, 1, nullptr);
         uint32_t* ko = kb[p & 1];
         uint32_t* io = ib[p & 1];
         hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(TPB), 0, st, kin, iin, n, shift, s.hist, nblk, ko,
@@ -273,325 +401,17 @@ hipError_t launch_ranges(hipStream_t st, const int32_t* sorted_keys, int n, int3
     return hipGetLastError();
 }
 
-// ============================================================== BGKI
-__global__ __launch_bounds__(TPB) void k_point_prep(const float* __restrict__ xs, const float* __restrict__ ys,
-                                                    const float* __restrict__ zs, int m, float res,
-                                                    K3PointRec* __restrict__ rec, float4* __restrict__ pxyz) {
-    int i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= m) return;
-    K3PointRec r = k3_point_prep(xs[i], ys[i], zs[i], res);
-    rec[i] = r;
-    pxyz[i] = make_float4(r.x, r.y, r.z, r.valid != 0.0f ? r.dm : -1.0f);
-}
-hipError_t launch_point_prep(hipStream_t st, const float* xs, const float* ys, const float* zs, int m, float res,
-                             K3PointRec* rec, float4* pxyz) {
-    if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(m)), dim3(TPB), 0, st, xs, ys, zs, m, res, rec, pxyz);
-    return hipGetLastError();
-}
-
-// Sensor-relative coordinate of the centre of voxel index i along an axis of
-// n voxels (the per-axis part of k3_voxel_center).
-__device__ __forceinline__ float axis_center(int i, int n, float res, float s) {
-    return (((float)i + 0.5f) - (float)n / 2.0f) * res - s;
-}
-
-struct BoxSphere {
-    float cx, cy, cz, R;
-};
-// Bounding sphere of the voxel centres of tiles [t0, t1) per axis.
-__device__ BoxSphere box_sphere(const GridGeom& g, int tx0, int ty0, int tz0, int tx1, int ty1, int tz1) {
-    int i0 = tx0 * 4, i1 = min(tx1 * 4, g.gs) - 1;
-    int j0 = ty0 * 4, j1 = min(ty1 * 4, g.gs) - 1;
-    int k0 = tz0 * 4, k1 = min(tz1 * 4, g.gsz) - 1;
-    float x0 = axis_center(i0, g.gs, g.res, g.sx), x1 = axis_center(i1, g.gs, g.res, g.sx);
-    float y0 = axis_center(j0, g.gs, g.res, g.sy), y1 = axis_center(j1, g.gs, g.res, g.sy);
-    float z0 = axis_center(k0, g.gsz, g.res, g.sz), z1 = axis_center(k1, g.gsz, g.res, g.sz);
-    BoxSphere b;
-    b.cx = 0.5f * (x0 + x1);
-    b.cy = 0.5f * (y0 + y1);
-    b.cz = 0.5f * (z0 + z1);
-    float hx = 0.5f * (x1 - x0), hy = 0.5f * (y1 - y0), hz = 0.5f * (z1 - z0);
-    b.R = sqrtf(hx * hx + hy * hy + hz * hz);
-    return b;
-}
-
-// True when some voxel centre of the box may lie within ls of the segment
-// [0, p]: distance(box centre, segment) < ls + R + slack.  Every (voxel, point)
-// pair with a non-zero term passes (the slack covers FP32 rounding of both
-// this test and the exact per-pair distances).
-__device__ __forceinline__ bool capsule_hits(const BoxSphere& b, float4 p, float ls) {
-    if (!(p.w >= 0.0f)) return false;
-    float mm = p.x * p.x + p.y * p.y + p.z * p.z;
-    float t = 0.0f;
-    if (mm > 0.0f) t = fminf(fmaxf((b.cx * p.x + b.cy * p.y + b.cz * p.z) / mm, 0.0f), 1.0f);
-    float dx = b.cx - t * p.x, dy = b.cy - t * p.y, dz = b.cz - t * p.z;
-    float d2 = dx * dx + dy * dy + dz * dz;
-    float cn = fabsf(b.cx) + fabsf(b.cy) + fabsf(b.cz);
-    float thr = ls + b.R + 2e-3f + 1e-5f * (cn + p.w);
-    return d2 < thr * thr;
-}
-
-// One workgroup per box of this level: scans the parent's candidate list (all
-// points at the root) in order and counts (count_mode) or writes, in the same
-// ascending order, the candidates whose capsule reaches the box.
-__global__ __launch_bounds__(TPB) void k_list_level(int count_mode, GridGeom g, LevelGeom L, int ntx, int nty,
-                                                   int ntz, float ls, const float4* __restrict__ pxyz, int m,
-                                                   const uint32_t* __restrict__ parent_off,
-                                                   const uint32_t* __restrict__ parent_len,
-                                                   const uint32_t* __restrict__ parent_list,
-                                                   uint32_t* __restrict__ len, const uint32_t* __restrict__ off,
-                                                   uint32_t* __restrict__ list) {
-    __shared__ uint32_t wcnt[4];
-    const int box = blockIdx.x;
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    const int bxi = box % L.nbx, byi = (box / L.nbx) % L.nby, bzi = box / (L.nbx * L.nby);
-    const int tx0 = bxi * L.bx, ty0 = byi * L.by, tz0 = bzi * L.bz;
-    const int tx1 = min(tx0 + L.bx, ntx), ty1 = min(ty0 + L.by, nty), tz1 = min(tz0 + L.bz, ntz);
-    uint32_t poff = 0, plen = (uint32_t)m;
-    if (L.pbx > 0) {
-        int pb = (tx0 / L.pbx) + L.pnbx * ((ty0 / L.pby) + L.pnby * (tz0 / L.pbz));
-        poff = parent_off[pb];
-        plen = parent_len[pb];
-    }
-    const BoxSphere bs = box_sphere(g, tx0, ty0, tz0, tx1, ty1, tz1);
-    uint32_t running = 0;
-    const uint32_t obase = count_mode ? 0u : off[box];
-    for (uint32_t base = 0; base < plen; base += TPB) {
-        uint32_t e = base + t;
-        bool pass = false;
-        uint32_t idx = 0;
-        if (e < plen) {
-            idx = parent_list ? parent_list[poff + e] : e;
-            pass = capsule_hits(bs, pxyz[idx], ls);
-        }
-        uint64_t mask = __ballot(pass);
-        if (lane == 0) wcnt[w] = (uint32_t)__popcll(mask);
-        __syncthreads();
-        uint32_t pre = 0;
-        for (int ww = 0; ww < w; ++ww) pre += wcnt[ww];
-        uint32_t tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-        if (!count_mode && pass) list[obase + running + pre + lanes_below(mask)] = idx;
-        running += tot;
-        __syncthreads();
-    }
-    if (count_mode && t == 0) len[box] = running;
-}
-hipError_t launch_list_level(hipStream_t st, int count_mode, const GridGeom& g, const LevelGeom& L, int nboxes,
-                             float ls, const float4* pxyz, int m, const uint32_t* parent_off,
-                             const uint32_t* parent_len, const uint32_t* parent_list, uint32_t* len,
-                             const uint32_t* off, uint32_t* list) {
-    int ntx = (g.gs + 3) / 4, nty = ntx, ntz = (g.gsz + 3) / 4;
-    hipLaunchKernelGGL(k_list_level, dim3(nboxes), dim3(TPB), 0, st, count_mode, g, L, ntx, nty, ntz, ls, pxyz, m,
-                       parent_off, parent_len, parent_list, len, off, list);
-    return hipGetLastError();
-}
-
-// Work items of the BGKI gather.  A tile whose candidate list is short is one
-// item; a long list is split into one item per 2^K3_BGKI_CHUNK_LOG2-point chunk
-// (each producing that chunk's partial sums), combined afterwards in chunk
-// order.  Both give the same two-level association.
-constexpr uint32_t BG_HEAVY_LEN = 1u << K3_BGKI_CHUNK_LOG2;
-constexpr uint32_t BG_DIRECT = 0xffffffffu;
-
-__global__ __launch_bounds__(TPB) void k_count_items(int ntiles, const uint32_t* __restrict__ toff,
-                                                     const uint32_t* __restrict__ tlen,
-                                                     const uint32_t* __restrict__ tlist,
-                                                     uint32_t* __restrict__ heavy_cnt,
-                                                     uint32_t* __restrict__ light_cnt) {
-    int t = blockIdx.x * TPB + threadIdx.x;
-    if (t >= ntiles) return;
-    uint32_t len = tlen[t];
-    if (len > BG_HEAVY_LEN) {
-        uint32_t c0 = tlist[toff[t]] >> K3_BGKI_CHUNK_LOG2;
-        uint32_t c1 = tlist[toff[t] + len - 1] >> K3_BGKI_CHUNK_LOG2;
-        heavy_cnt[t] = c1 - c0 + 1;
-        light_cnt[t] = 0;
-    } else {
-        heavy_cnt[t] = 0;
-        light_cnt[t] = 1;
-    }
-}
-
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t v) {
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-__global__ __launch_bounds__(TPB) void k_emit_items(int ntiles, const uint32_t* __restrict__ toff,
-                                                    const uint32_t* __restrict__ tlen,
-                                                    const uint32_t* __restrict__ tlist,
-                                                    const uint32_t* __restrict__ heavy_cnt,
-                                                    const uint32_t* __restrict__ heavy_off,
-                                                    const uint32_t* __restrict__ light_off,
-                                                    const uint32_t* __restrict__ n_heavy, uint4* __restrict__ items,
-                                                    uint32_t* __restrict__ tile_partial) {
-    int t = blockIdx.x * TPB + threadIdx.x;
-    if (t >= ntiles) return;
-    uint32_t len = tlen[t], o = toff[t];
-    uint32_t hc = heavy_cnt[t];
-    if (hc > 0) {
-        uint32_t c0 = tlist[o] >> K3_BGKI_CHUNK_LOG2;
-        uint32_t first = heavy_off[t];
-        for (uint32_t k = 0; k < hc; ++k) {
-            uint32_t c = c0 + k;
-            uint32_t b = lower_bound_u32(tlist, o, o + len, c << K3_BGKI_CHUNK_LOG2);
-            uint32_t e = lower_bound_u32(tlist, o, o + len, (c + 1) << K3_BGKI_CHUNK_LOG2);
-            items[first + k] = make_uint4((uint32_t)t, b, e, first + k);
-        }
-        tile_partial[2 * t] = first;
-        tile_partial[2 * t + 1] = hc;
-    } else {
-        items[*n_heavy + light_off[t]] = make_uint4((uint32_t)t, o, o + len, BG_DIRECT);
-        tile_partial[2 * t] = 0;
-        tile_partial[2 * t + 1] = 0;
-    }
-}
-
-hipError_t launch_count_items(hipStream_t st, int ntiles, const uint32_t* tile_off, const uint32_t* tile_len,
-                              const uint32_t* tile_list, uint32_t* heavy_cnt, uint32_t* light_cnt,
-                              uint32_t* heavy_off, uint32_t* light_off, DevScalars* sc) {
-    hipLaunchKernelGGL(k_count_items, dim3(blocks_for(ntiles)), dim3(TPB), 0, st, ntiles, tile_off, tile_len,
-                       tile_list, heavy_cnt, light_cnt);
-    hipLaunchKernelGGL(k_scan_u32_single, dim3(1), dim3(1024), 0, st, heavy_cnt, heavy_off, ntiles,
-                       &sc->n_heavy_items);
-    hipLaunchKernelGGL(k_scan_u32_single, dim3(1), dim3(1024), 0, st, light_cnt, light_off, ntiles, &sc->n_items);
-    return hipGetLastError();
-}
-hipError_t launch_emit_items(hipStream_t st, int ntiles, const uint32_t* tile_off, const uint32_t* tile_len,
-                             const uint32_t* tile_list, const uint32_t* heavy_cnt, const uint32_t* heavy_off,
-                             const uint32_t* light_off, uint4* items, uint32_t* tile_partial, DevScalars* sc) {
-    hipLaunchKernelGGL(k_emit_items, dim3(blocks_for(ntiles)), dim3(TPB), 0, st, ntiles, tile_off, tile_len,
-                       tile_list, heavy_cnt, heavy_off, light_off, &sc->n_heavy_items, items, tile_partial);
-    return hipGetLastError();
-}
-
-// Kernel-inference gather: one wavefront per work item, lane = voxel of the
-// 4x4x4 tile.  Candidates are staged 64 at a time in LDS and visited in
-// ascending point order by all lanes (uniform control flow); each lane adds
-// its occupied / free terms with the chunked association.
-constexpr int BG_WAVES = TPB / 64;
-
-__global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, int ntx, int nty, float ls, float ls2g, float inv_ls,
-                                              float sigma, const uint4* __restrict__ items,
-                                              const uint32_t* __restrict__ tlist,
-                                              const K3PointRec* __restrict__ rec, float* __restrict__ del_occ,
-                                              float* __restrict__ del_free, float* __restrict__ partial,
-                                              DevScalars* sc) {
-    __shared__ K3PointRec srec[BG_WAVES][64];
-    __shared__ uint32_t sidx[BG_WAVES][64];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t n_items = sc->n_items + sc->n_heavy_items;
-    while (true) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(&sc->item_counter, 1u);
-        it = __shfl(it, 0);
-        if (it >= n_items) break;
-        const uint4 item = items[it];
-        const int tile = (int)item.x;
-        const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-        const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
-        const bool vvalid = i < g.gs && j < g.gs && k < g.gsz;
-        const float vx = axis_center(i, g.gs, g.res, g.sx);
-        const float vy = axis_center(j, g.gs, g.res, g.sy);
-        const float vz = axis_center(k, g.gsz, g.res, g.sz);
-        const float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
-        const float dc2 = dc * dc;
-        float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
-        uint32_t cur_chunk = 0xffffffffu;
-        for (uint32_t b = item.y; b < item.z; b += 64) {
-            uint32_t e = b + lane;
-            if (e < item.z) {
-                uint32_t p = tlist[e];
-                sidx[w][lane] = p;
-                srec[w][lane] = rec[p];
-            }
-            wave_sync();
-            const int cnt = (int)min(64u, item.z - b);
-            for (int q = 0; q < cnt; ++q) {
-                const uint32_t p = sidx[w][q];
-                const uint32_t chunk = p >> K3_BGKI_CHUNK_LOG2;
-                if (chunk != cur_chunk) {
-                    tot_o += cur_o;
-                    tot_f += cur_f;
-                    cur_o = 0.0f;
-                    cur_f = 0.0f;
-                    cur_chunk = chunk;
-                }
-                const K3PointRec r = srec[w][q];
-                float d_o = k3_occ_distance(vx, vy, vz, r, ls, ls2g);
-                if (d_o >= 0.0f) cur_o += k3_kernel_value(d_o, inv_ls, sigma);
-                float d_f = k3_free_distance(vx, vy, vz, dc, dc2, r, ls, ls2g);
-                if (d_f >= 0.0f) cur_f += k3_kernel_value(d_f, inv_ls, sigma);
-            }
-            wave_sync();
-        }
-        tot_o += cur_o;
-        tot_f += cur_f;
-        if (item.w == BG_DIRECT) {
-            if (vvalid) {
-                int cell = i + g.gs * j + g.gs * g.gs * k;
-                del_occ[cell] = tot_o;
-                del_free[cell] = tot_f;
-            }
-        } else {
-            partial[(size_t)item.w * 128 + lane] = tot_o;
-            partial[(size_t)item.w * 128 + 64 + lane] = tot_f;
-        }
-    }
-}
-hipError_t launch_bgki(hipStream_t st, int grid_blocks, const GridGeom& g, int ntx, int nty, float ls, float sigma,
-                       const uint4* items, const uint32_t* tile_list, const K3PointRec* rec, float* del_occ,
-                       float* del_free, float* partial, DevScalars* sc) {
-    float ls2g = k3_ls2_guard(ls);
-    float inv_ls = 1.0f / ls;
-    hipLaunchKernelGGL(k_bgki, dim3(grid_blocks), dim3(TPB), 0, st, g, ntx, nty, ls, ls2g, inv_ls, sigma, items,
-                       tile_list, rec, del_occ, del_free, partial, sc);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(TPB) void k_bgki_combine(int ntiles, GridGeom g, int ntx, int nty,
-                                                      const uint32_t* __restrict__ tile_partial,
-                                                      const float* __restrict__ partial, float* del_occ,
-                                                      float* del_free) {
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * BG_WAVES + w;
-    if (tile >= ntiles) return;
-    const uint32_t first = tile_partial[2 * tile], cnt = tile_partial[2 * tile + 1];
-    if (cnt == 0) return;
-    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-    const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
-    if (!(i < g.gs && j < g.gs && k < g.gsz)) return;
-    float ro = 0.0f, rf = 0.0f;
-    for (uint32_t c = 0; c < cnt; ++c) {
-        ro += partial[(size_t)(first + c) * 128 + lane];
-        rf += partial[(size_t)(first + c) * 128 + 64 + lane];
-    }
-    int cell = i + g.gs * j + g.gs * g.gs * k;
-    del_occ[cell] = ro;
-    del_free[cell] = rf;
-}
-hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, int ntx, int nty,
-                               const uint32_t* tile_partial, const float* partial, float* del_occ, float* del_free) {
-    hipLaunchKernelGGL(k_bgki_combine, dim3(blocks_for(ntiles, BG_WAVES)), dim3(TPB), 0, st, ntiles, g, ntx, nty,
-                       tile_partial, partial, del_occ, del_free);
-    return hipGetLastError();
-}
-
 // ============================================================== mass update
 __device__ __forceinline__ void atomic_max_nonneg(uint32_t* dst, float v) {
     if (v > 0.0f && v - v == 0.0f) atomicMax(dst, k3_f2u(v));
 }
 
-__global__ __launch_bounds__(TPB) void k_mass_update(VoxelArrays v, int V, const float* __restrict__ weights,
+__global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict__ abort_flag, VoxelArrays v, int V, const float* __restrict__ weights,
                                                      const float* __restrict__ del_occ,
                                                      const float* __restrict__ del_free, float decay,
                                                      float mass_scale, float p_B, float* __restrict__ born,
                                                      DevScalars* sc) {
+    if (*abort_flag) return;
     int c = blockIdx.x * TPB + threadIdx.x;
     float bmax = 0.0f;
     if (c < V) {
@@ -616,18 +436,19 @@ __global__ __launch_bounds__(TPB) void k_mass_update(VoxelArrays v, int V, const
     for (int o = 32; o > 0; o >>= 1) bmax = fmaxf(bmax, __shfl_xor(bmax, o));
     if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->born_max_bits, bmax);
 }
-hipError_t launch_mass_update(hipStream_t st, VoxelArrays v, int V, const float* weights, const float* del_occ,
+hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V, const float* weights, const float* del_occ,
                               const float* del_free, float decay, float mass_scale, float p_B, float* born,
                               DevScalars* sc) {
-    hipLaunchKernelGGL(k_mass_update, dim3(blocks_for(V)), dim3(TPB), 0, st, v, V, weights, del_occ, del_free,
+    hipLaunchKernelGGL(k_mass_update, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, weights, del_occ, del_free,
                        decay, mass_scale, p_B, born, sc);
     return hipGetLastError();
 }
 
 // Per voxel with particles: persistent weight update (mu = pers/pred), mean
 // velocity from the updated weights, then the static-particle penalty.
-__global__ __launch_bounds__(TPB) void k_voxel_particles(VoxelArrays v, int V, ParticleArrays P, float min_vel,
+__global__ __launch_bounds__(TPB) void k_voxel_particles(const uint32_t* __restrict__ abort_flag, VoxelArrays v, int V, ParticleArrays P, float min_vel,
                                                          DevScalars* sc) {
+    if (*abort_flag) return;
     int c = blockIdx.x * TPB + threadIdx.x;
     float wmax = 0.0f;
     if (c < V) {
@@ -664,9 +485,9 @@ __global__ __launch_bounds__(TPB) void k_voxel_particles(VoxelArrays v, int V, P
     for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
     if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
 }
-hipError_t launch_voxel_particles(hipStream_t st, VoxelArrays v, int V, ParticleArrays P, float min_vel,
+hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V, ParticleArrays P, float min_vel,
                                   DevScalars* sc) {
-    hipLaunchKernelGGL(k_voxel_particles, dim3(blocks_for(V)), dim3(TPB), 0, st, v, V, P, min_vel, sc);
+    hipLaunchKernelGGL(k_voxel_particles, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, P, min_vel, sc);
     return hipGetLastError();
 }
 
@@ -785,10 +606,11 @@ __device__ __forceinline__ uint32_t slot_bound(uint64_t c, uint64_t tot, uint32_
 
 // Slot boundaries of a fixed-point CDF split into cnt particles, and the
 // unassociated newborn weight of each cell (src/common/kernel/init_new_particles.cu:119-134).
-__global__ __launch_bounds__(TPB) void k_bounds(const uint64_t* __restrict__ cdf,
+__global__ __launch_bounds__(TPB) void k_bounds(const uint32_t* __restrict__ abort_flag, const uint64_t* __restrict__ cdf,
                                                 const uint64_t* __restrict__ total, int V, uint32_t cnt,
                                                 const float* __restrict__ born, uint32_t* __restrict__ bnd,
                                                 float* __restrict__ wua) {
+    if (*abort_flag) return;
     int j = blockIdx.x * TPB + threadIdx.x;
     if (j >= V) return;
     const uint64_t tot = *total;
@@ -808,9 +630,9 @@ __global__ __launch_bounds__(TPB) void k_bounds(const uint64_t* __restrict__ cdf
         wua[j] = nn > 0 ? (float)(((1.0 - (double)pA) * (double)born[j]) / (double)nn) : 0.0f;
     }
 }
-hipError_t launch_bounds(hipStream_t st, const uint64_t* cdf, const uint64_t* total, int V, uint32_t cnt,
+hipError_t launch_bounds(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, const uint64_t* total, int V, uint32_t cnt,
                          const float* born, uint32_t* bnd, float* wua) {
-    hipLaunchKernelGGL(k_bounds, dim3(blocks_for(V)), dim3(TPB), 0, st, cdf, total, V, cnt, born, bnd, wua);
+    hipLaunchKernelGGL(k_bounds, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, cdf, total, V, cnt, born, bnd, wua);
     return hipGetLastError();
 }
 
@@ -824,10 +646,11 @@ __device__ __forceinline__ int slot_owner(const uint32_t* __restrict__ bnd, int 
     return lo;
 }
 
-__global__ __launch_bounds__(TPB) void k_newborn(const uint32_t* __restrict__ bnd, int V, int B, int gs,
+__global__ __launch_bounds__(TPB) void k_newborn(const uint32_t* __restrict__ abort_flag, const uint32_t* __restrict__ bnd, int V, int B, int gs,
                                                  const float* __restrict__ wua, ParticleArrays NB, float vmax,
                                                  float min_vel, uint32_t frame, uint32_t k0, uint32_t k1,
                                                  DevScalars* sc) {
+    if (*abort_flag) return;
     int k = blockIdx.x * TPB + threadIdx.x;
     float wmax = 0.0f;
     if (k < B) {
@@ -849,18 +672,19 @@ __global__ __launch_bounds__(TPB) void k_newborn(const uint32_t* __restrict__ bn
     for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
     if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
 }
-hipError_t launch_newborn(hipStream_t st, const uint32_t* bnd, int V, int B, int gs, const float* wua,
+hipError_t launch_newborn(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int B, int gs, const float* wua,
                           ParticleArrays NB, float vmax, float min_vel, uint32_t frame, uint32_t k0, uint32_t k1,
                           DevScalars* sc) {
     if (B <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_newborn, dim3(blocks_for(B)), dim3(TPB), 0, st, bnd, V, B, gs, wua, NB, vmax, min_vel,
+    hipLaunchKernelGGL(k_newborn, dim3(blocks_for(B)), dim3(TPB), 0, st, abort_flag, bnd, V, B, gs, wua, NB, vmax, min_vel,
                        frame, k0, k1, sc);
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(TPB) void k_init_particles(const uint32_t* __restrict__ bnd, int V, int N, int gs,
+__global__ __launch_bounds__(TPB) void k_init_particles(const uint32_t* __restrict__ abort_flag, const uint32_t* __restrict__ bnd, int V, int N, int gs,
                                                         ParticleArrays P, float vmax, float min_vel,
                                                         uint32_t frame, uint32_t k0, uint32_t k1) {
+    if (*abort_flag) return;
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= N) return;
     int c = slot_owner(bnd, V, (uint32_t)i);
@@ -875,9 +699,9 @@ __global__ __launch_bounds__(TPB) void k_init_particles(const uint32_t* __restri
     P.w[i] = 1.0f / (float)N;
     P.key[i] = c;
 }
-hipError_t launch_init_particles(hipStream_t st, const uint32_t* bnd, int V, int N, int gs, ParticleArrays P,
+hipError_t launch_init_particles(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int N, int gs, ParticleArrays P,
                                  float vmax, float min_vel, uint32_t frame, uint32_t k0, uint32_t k1) {
-    hipLaunchKernelGGL(k_init_particles, dim3(blocks_for(N)), dim3(TPB), 0, st, bnd, V, N, gs, P, vmax, min_vel,
+    hipLaunchKernelGGL(k_init_particles, dim3(blocks_for(N)), dim3(TPB), 0, st, abort_flag, bnd, V, N, gs, P, vmax, min_vel,
                        frame, k0, k1);
     return hipGetLastError();
 }
@@ -885,11 +709,12 @@ hipError_t launch_init_particles(hipStream_t st, const uint32_t* bnd, int V, int
 // Systematic resampling on the u64 CDF of [persistent ++ newborn] weights:
 // positions T_i = trunc((i + u) * Q / N) (IEEE double), index = first CDF
 // entry > T_i; then the gather into the next particle set with weight W/N.
-__global__ __launch_bounds__(TPB) void k_resample(const uint64_t* __restrict__ cdf, int L,
+__global__ __launch_bounds__(TPB) void k_resample(const uint32_t* __restrict__ abort_flag, const uint64_t* __restrict__ cdf, int L,
                                                   const uint64_t* __restrict__ total,
                                                   const int32_t* __restrict__ Ep, ParticleArrays S,
                                                   ParticleArrays NB, ParticleArrays O, int N, uint32_t frame,
                                                   uint32_t k0, uint32_t k1, int32_t* __restrict__ idx_out) {
+    if (*abort_flag) return;
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= N) return;
     const uint64_t Q = *total;
@@ -927,10 +752,10 @@ __global__ __launch_bounds__(TPB) void k_resample(const uint64_t* __restrict__ c
     }
     O.w[i] = nw;
 }
-hipError_t launch_resample(hipStream_t st, const uint64_t* cdf, int total_len, const uint64_t* total,
+hipError_t launch_resample(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, int total_len, const uint64_t* total,
                            const int32_t* E, ParticleArrays sorted, ParticleArrays nb, ParticleArrays out, int N,
                            uint32_t frame, uint32_t k0, uint32_t k1, int32_t* idx_out) {
-    hipLaunchKernelGGL(k_resample, dim3(blocks_for(N)), dim3(TPB), 0, st, cdf, total_len, total, E, sorted, nb, out,
+    hipLaunchKernelGGL(k_resample, dim3(blocks_for(N)), dim3(TPB), 0, st, abort_flag, cdf, total_len, total, E, sorted, nb, out,
                        N, frame, k0, k1, idx_out);
     return hipGetLastError();
 }
@@ -943,6 +768,20 @@ __global__ void k_reset_scalars(DevScalars* sc) {
 }
 hipError_t launch_reset_scalars(hipStream_t st, DevScalars* sc) {
     hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, st, sc);
+    return hipGetLastError();
+}
+
+// Clears the overflow state and the work-queue head before a re-run of the
+// kernel-inference stage (the other scalars are untouched by the aborted tail).
+__global__ void k_clear_overflow(DevScalars* sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        sc->overflow = 0;
+        sc->overflow_mask = 0;
+        sc->item_counter = 0;
+    }
+}
+hipError_t launch_clear_overflow(hipStream_t st, DevScalars* sc) {
+    hipLaunchKernelGGL(k_clear_overflow, dim3(1), dim3(64), 0, st, sc);
     return hipGetLastError();
 }
 

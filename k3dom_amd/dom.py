@@ -160,7 +160,21 @@ class DOM:
     def set_timing(self, on=True):
         check(self._lib.k3dom_set_timing(self._h, 1 if on else 0), self._h, "set_timing")
 
+    STAGES = ["predict", "sort", "lists", "bgki", "mass", "persist", "newborn", "moments", "resample", "frame",
+              "bgki_kernel"]
+
     def stage_times(self):
-        out = np.zeros(10, np.float32)
-        check(self._lib.k3dom_stage_times(self._h, _fptr(out), 10), self._h, "stage_times")
+        out = np.zeros(len(self.STAGES), np.float32)
+        check(self._lib.k3dom_stage_times(self._h, _fptr(out), len(out)), self._h, "stage_times")
         return out
+
+    def stats(self):
+        out = _lib.Stats()
+        check(self._lib.k3dom_get_stats(self._h, C.byref(out)), self._h, "get_stats")
+        return {"points": out.points, "level_entries": list(out.level_entries), "gather_items": out.gather_items,
+                "gather_split_items": out.gather_split_items, "capacity_regrowths": out.capacity_regrowths}
+
+    def count_bgki_pairs(self):
+        out = (C.c_uint64 * 2)()
+        check(self._lib.k3dom_count_bgki_pairs(self._h, out), self._h, "count_bgki_pairs")
+        return int(out[0]), int(out[1])

@@ -56,10 +56,11 @@ def run_pair(workload, frames, check_every=1, mutate=None):
         if mutate is not None:
             t, pose, xs, ys, zs = mutate(f, t, pose, xs, ys, zs)
         print(f"  frame {f}: {len(xs)} points", flush=True)
+        normal = gd.info().updated_time > 0
         gd.updatePose(*pose)
         od.updatePose(*pose)
         gd.updateGrid(t, xs, ys, zs)
         od.updateGrid(t, xs, ys, zs)
         if f % check_every == 0 or f == frames - 1:
-            compare_state(gd, od, normal_frame=f > 0, what=f"{workload.name} frame {f}:")
+            compare_state(gd, od, normal_frame=normal, what=f"{workload.name} frame {f}:")
     return gd, od

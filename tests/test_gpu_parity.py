@@ -30,3 +30,41 @@ def test_config1_bit_exact(gpu):
     from k3dom_amd.scene import workload
     w = workload(1)
     run_pair(w, 5)
+
+
+def test_capacity_regrowth_is_bit_exact(gpu, monkeypatch):
+    """Every candidate-list capacity starts tiny, so frames overflow, grow the
+    buffers and re-run the kernel-inference stage and the frame tail; the
+    results must not change."""
+    from k3dom_amd.scene import small_workload
+    monkeypatch.setenv("K3DOM_TEST_TINY_CAPACITY", "1")
+    w = small_workload(frames=4, seed=5)
+    gd, od = run_pair(w, w.frames)
+    assert gd.stats()["capacity_regrowths"] >= 3
+
+
+def test_empty_and_invalid_point_clouds(gpu):
+    """Frames with no points, and points with NaN / inf / zero coordinates."""
+    from k3dom_amd.scene import small_workload
+
+    def mutate(f, t, pose, xs, ys, zs):
+        if f == 2:
+            return t, pose, xs[:0], ys[:0], zs[:0]
+        if f == 3:
+            bad = np.array([np.nan, np.inf, 0.0, -np.inf, 1e30], np.float32)
+            return (t, pose, np.concatenate([xs, bad]), np.concatenate([ys, bad[::-1]]),
+                    np.concatenate([zs, np.zeros(5, np.float32)]))
+        return t, pose, xs, ys, zs
+    w = small_workload(frames=5, seed=9)
+    run_pair(w, w.frames, mutate=mutate)
+
+
+def test_zero_time_first_frame_reinitialises(gpu):
+    """A first frame at t = 0 leaves the stored time at 0, so the next frame
+    initialises again (src/k3dom/dom.cu:31); both paths must agree."""
+    from k3dom_amd.scene import small_workload
+
+    def mutate(f, t, pose, xs, ys, zs):
+        return t - 1.0, pose, xs, ys, zs
+    w = small_workload(frames=4, seed=13)
+    run_pair(w, w.frames, mutate=mutate)

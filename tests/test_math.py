@@ -151,3 +151,28 @@ def test_key_matches_truncate_then_clamp(oracle_lib):
         exp = ref_idx(x, gs) + gs * ref_idx(y, gs) + gs * gs * ref_idx(z, gsz)
         assert oracle_lib.orc_key(float(x), float(y), float(z), gs, gsz) == exp
     assert oracle_lib.orc_key(float("nan"), 1.0, 1.0, gs, gsz) == 0 + gs * 1 + gs * gs * 1
+
+
+def test_kernel_polynomial_matches_generator_and_formula(oracle_lib):
+    """The header's kernel coefficients are those gen_kernel_poly.py prints,
+    and the float32 evaluation stays within 2e-7 (relative to sigma) of the
+    reference formula in double over the whole support."""
+    import importlib.util
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("gkp", os.path.join(root, "k3dom_amd", "csrc", "gen_kernel_poly.py"))
+    gkp = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gkp)
+    header = open(os.path.join(root, "k3dom_amd", "csrc", "k3_math.h")).read()
+    got = {int(i): np.float32(float(v)) for i, v in re.findall(r"#define K3_KP(\d+) \(?([-0-9.e]+)f\)?", header)}
+    want = gkp.coefficients()
+    assert len(got) == len(want)
+    for i, c in enumerate(want):
+        assert got[i] == c, (i, got[i], c)
+    ls, sigma = 0.5, 1.0
+    inv = float(np.float32(1) / np.float32(ls))
+    ds = np.linspace(0, ls, 20001, endpoint=False).astype(np.float32)
+    vals = np.array([oracle_lib.orc_kernel(float(d), inv, sigma) for d in ds])
+    ref = np.maximum(0.0, gkp.k(ds.astype(np.float64) * np.float64(np.float32(inv))))
+    assert np.max(np.abs(vals - ref)) < 2e-7
