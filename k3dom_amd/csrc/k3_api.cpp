@@ -49,6 +49,7 @@ struct k3dom_handle {
     uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;
     uint32_t* bnd = nullptr;
     int32_t* ridx = nullptr;
+    int32_t* heavy_voxels = nullptr;  // V entries: voxels queued for the wavefront paths
     DevScalars* sc = nullptr;
 
     int m_cap = 0;
@@ -282,9 +283,10 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, float decay, const float
     mark(h, ST_MASS);
     if (normal) {
         HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[1].w, h->del_occ, h->del_free, decay,
-                                      h->p.mass_scale, h->p.birth_prob, h->born, h->sc));
+                                      h->p.mass_scale, h->p.birth_prob, h->born, h->heavy_voxels, h->sc));
         mark(h, ST_PERSIST);
-        HIP_TRY(h, launch_voxel_particles(h->stream, ab, v, h->V, h->P[1], h->p.particle_min_vel, h->sc));
+        HIP_TRY(h, launch_voxel_particles(h->stream, ab, v, h->V, h->P[1], h->p.particle_min_vel, h->heavy_voxels,
+                                          h->sc));
         mark(h, ST_NEWBORN);
         HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, &h->sc->born_max_bits, &h->sc->E_born,
                                      h->cdf_v, h->blocksums, &h->sc->total_born));
@@ -300,7 +302,7 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, float decay, const float
                                    h->P[1], h->NB, h->P[0], h->N, h->frame, k0, k1, h->ridx));
     } else {
         HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[0].w, h->del_occ, h->del_free, 1.0f,
-                                      h->p.mass_scale, h->p.birth_prob, h->born, h->sc));
+                                      h->p.mass_scale, h->p.birth_prob, h->born, h->heavy_voxels, h->sc));
         mark(h, ST_PERSIST);
         mark(h, ST_NEWBORN);
         HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, &h->sc->born_max_bits, &h->sc->E_born,
@@ -588,6 +590,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(dalloc(h, &h->blocksums, (size_t)fixed_scan_blocks((int)std::max(V, N + B))));
     CREATE_TRY(dalloc(h, &h->bnd, V));
     CREATE_TRY(dalloc(h, &h->ridx, N));
+    CREATE_TRY(dalloc(h, &h->heavy_voxels, V));
     CREATE_TRY(dalloc(h, &h->sc, 1));
     setup_levels(h);
     for (int l = 0; l < 3; ++l) {

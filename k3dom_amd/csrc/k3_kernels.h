@@ -23,7 +23,9 @@ struct DevScalars {
     uint32_t overflow_mask;   // bit l: candidate level l, bit 4: split items
     uint32_t level_total[3];  // candidate-list entries per level
     uint32_t level_items[3];  // work items per level
-    uint32_t pad[3];
+    uint32_t n_heavy_mass;    // voxels queued for the wavefront mass-update path
+    uint32_t n_heavy_vp;      // voxels queued for the wavefront particle path
+    uint32_t pad;
 };
 
 // Voxel state arrays (one buffer set).
@@ -126,11 +128,12 @@ hipError_t launch_bgki_count(hipStream_t st, int grid_blocks, const GridGeom& g,
 // ------------------------------------------------------------------ frame tail
 // Every tail kernel returns at once when *abort_flag is set (a capacity of
 // the kernel-inference stage overflowed; the host grows it and re-runs).
+// heavy: scratch list of V voxel ids (voxels with many particles)
 hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
                               const float* weights, const float* del_occ, const float* del_free, float decay,
-                              float mass_scale, float p_B, float* born, DevScalars* sc);
+                              float mass_scale, float p_B, float* born, int32_t* heavy, DevScalars* sc);
 hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
-                                  ParticleArrays P, float min_vel, DevScalars* sc);
+                                  ParticleArrays P, float min_vel, int32_t* heavy, DevScalars* sc);
 hipError_t launch_fixed_scan(hipStream_t st, const float* a, int na, const float* b, int nb,
                              const uint32_t* max_bits, int32_t* E_out, uint64_t* cdf, uint64_t* blocksums,
                              uint64_t* total);

@@ -406,88 +406,242 @@ __device__ __forceinline__ void atomic_max_nonneg(uint32_t* dst, float v) {
     if (v > 0.0f && v - v == 0.0f) atomicMax(dst, k3_f2u(v));
 }
 
-__global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict__ abort_flag, VoxelArrays v, int V, const float* __restrict__ weights,
+// Per-voxel particle loops.  A voxel's segment sums run over its particles in
+// ascending order (one sequential FP32 sum).  Voxels with at most
+// HEAVY_PARTICLES particles are summed by one lane with 8 loads in flight;
+// larger ones are queued and summed by a whole wavefront (coalesced loads,
+// the same left-to-right accumulation broadcast with readlane), so the
+// results do not depend on which path a voxel takes.
+constexpr int HEAVY_PARTICLES = 64;
+constexpr int HEAVY_GRID = 1024;
+
+__device__ __forceinline__ float seq_sum(const float* __restrict__ x, int st, int en) {
+    float s = 0.0f;
+    int j = st;
+    for (; j + 7 <= en; j += 8) {
+        const float a0 = x[j], a1 = x[j + 1], a2 = x[j + 2], a3 = x[j + 3];
+        const float a4 = x[j + 4], a5 = x[j + 5], a6 = x[j + 6], a7 = x[j + 7];
+        s += a0; s += a1; s += a2; s += a3;
+        s += a4; s += a5; s += a6; s += a7;
+    }
+    for (; j <= en; ++j) s += x[j];
+    return s;
+}
+
+__device__ __forceinline__ float lane_value(float v, int q) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), q));
+}
+
+__device__ __forceinline__ float wave_seq_sum(const float* __restrict__ x, int st, int en) {
+    const int lane = threadIdx.x & 63;
+    float s = 0.0f;
+    for (int b = st; b <= en; b += 64) {
+        const int j = b + lane;
+        const float v = j <= en ? x[j] : 0.0f;
+        const int cnt = min(64, en - b + 1);
+        for (int q = 0; q < cnt; ++q) s += lane_value(v, q);
+    }
+    return s;
+}
+
+// prediction, evidence update and newborn split of one voxel
+// (src/k3dom/kernel/mass_update.cu:163-181); returns the born mass
+__device__ __forceinline__ float mass_voxel(const VoxelArrays& v, int c, float dyn_pred,
+                                            const float* __restrict__ del_occ, const float* __restrict__ del_free,
+                                            float decay, float mass_scale, float p_B, float* __restrict__ born) {
+    float f = v.f[c], s = v.s[c], d = v.d[c];
+    const float pred_total = (f + d) * decay;
+    k3_predict_mass(&f, &s, &d, dyn_pred, pred_total, decay);
+    k3_update_mass(&f, &s, &d, del_occ[c], del_free[c], mass_scale);
+    const float b = k3_newborn_part(dyn_pred, pred_total, d, p_B);
+    v.f[c] = f;
+    v.s[c] = s;
+    v.d[c] = d;
+    born[c] = b;
+    v.pers[c] = d - b;
+    v.pred[c] = dyn_pred;
+    return b;
+}
+
+__global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict__ abort_flag, VoxelArrays v, int V,
+                                                     const float* __restrict__ weights,
                                                      const float* __restrict__ del_occ,
                                                      const float* __restrict__ del_free, float decay,
                                                      float mass_scale, float p_B, float* __restrict__ born,
-                                                     DevScalars* sc) {
+                                                     int32_t* __restrict__ heavy, DevScalars* sc) {
     if (*abort_flag) return;
     int c = blockIdx.x * TPB + threadIdx.x;
     float bmax = 0.0f;
     if (c < V) {
-        float f = v.f[c], s = v.s[c], d = v.d[c];
-        int st = v.start[c], en = v.end[c];
-        float pred_total = (f + d) * decay;
-        float dyn_pred = 0.0f;
-        if (st != -1)
-            for (int j = st; j < en + 1; ++j) dyn_pred += weights[j];
-        k3_predict_mass(&f, &s, &d, dyn_pred, pred_total, decay);
-        k3_update_mass(&f, &s, &d, del_occ[c], del_free[c], mass_scale);
-        float b = k3_newborn_part(dyn_pred, pred_total, d, p_B);
-        v.f[c] = f;
-        v.s[c] = s;
-        v.d[c] = d;
-        born[c] = b;
-        v.pers[c] = d - b;
-        v.pred[c] = dyn_pred;
-        if (b > 0.0f && b - b == 0.0f) bmax = b;
+        const int st = v.start[c], en = v.end[c];
+        if (st != -1 && en - st + 1 > HEAVY_PARTICLES) {
+            heavy[atomicAdd(&sc->n_heavy_mass, 1u)] = c;
+        } else {
+            const float dyn_pred = st != -1 ? seq_sum(weights, st, en) : 0.0f;
+            const float b = mass_voxel(v, c, dyn_pred, del_occ, del_free, decay, mass_scale, p_B, born);
+            if (b > 0.0f && b - b == 0.0f) bmax = b;
+        }
     }
     // wave-level max before one atomic per wave
     for (int o = 32; o > 0; o >>= 1) bmax = fmaxf(bmax, __shfl_xor(bmax, o));
     if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->born_max_bits, bmax);
 }
-hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V, const float* weights, const float* del_occ,
-                              const float* del_free, float decay, float mass_scale, float p_B, float* born,
-                              DevScalars* sc) {
-    hipLaunchKernelGGL(k_mass_update, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, weights, del_occ, del_free,
-                       decay, mass_scale, p_B, born, sc);
+
+__global__ __launch_bounds__(TPB) void k_mass_update_heavy(const uint32_t* __restrict__ abort_flag, VoxelArrays v,
+                                                           const float* __restrict__ weights,
+                                                           const float* __restrict__ del_occ,
+                                                           const float* __restrict__ del_free, float decay,
+                                                           float mass_scale, float p_B, float* __restrict__ born,
+                                                           const int32_t* __restrict__ heavy, DevScalars* sc) {
+    if (*abort_flag) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = sc->n_heavy_mass;
+    for (uint32_t h = (blockIdx.x * TPB + threadIdx.x) >> 6; h < n; h += gridDim.x * (TPB / 64)) {
+        const int c = __builtin_amdgcn_readfirstlane(heavy[h]);
+        const int st = __builtin_amdgcn_readfirstlane(v.start[c]), en = __builtin_amdgcn_readfirstlane(v.end[c]);
+        const float dyn_pred = wave_seq_sum(weights, st, en);
+        if (lane == 0)
+            atomic_max_nonneg(&sc->born_max_bits,
+                              mass_voxel(v, c, dyn_pred, del_occ, del_free, decay, mass_scale, p_B, born));
+    }
+}
+
+hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
+                              const float* weights, const float* del_occ, const float* del_free, float decay,
+                              float mass_scale, float p_B, float* born, int32_t* heavy, DevScalars* sc) {
+    hipLaunchKernelGGL(k_mass_update, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, weights, del_occ,
+                       del_free, decay, mass_scale, p_B, born, heavy, sc);
+    hipLaunchKernelGGL(k_mass_update_heavy, dim3(HEAVY_GRID), dim3(TPB), 0, st, abort_flag, v, weights, del_occ,
+                       del_free, decay, mass_scale, p_B, born, heavy, sc);
     return hipGetLastError();
 }
 
 // Per voxel with particles: persistent weight update (mu = pers/pred), mean
 // velocity from the updated weights, then the static-particle penalty.
-__global__ __launch_bounds__(TPB) void k_voxel_particles(const uint32_t* __restrict__ abort_flag, VoxelArrays v, int V, ParticleArrays P, float min_vel,
-                                                         DevScalars* sc) {
+// One particle's persistent update (src/common/kernel/update_persistent_particles.cu:23-24,52-53
+// with likelihood 1 and p_A 0, which K3DOM never changes): returns the new
+// weight and, through *pen, the weight after the static penalty
+// (src/common/kernel/resampling.cu:24-28).
+__device__ __forceinline__ float persist_weight(float w, float m, float vx, float vy, float vz, float min2,
+                                                float* pen) {
+    const float pA = 0.0f, lik = 1.0f;
+    const float post = lik * w;
+    const float wn = pA * m * post + (1.0f - pA) * m * w;
+    *pen = (vx * vx + vy * vy + vz * vz < min2) ? 0.0f : wn;
+    return wn;
+}
+
+__global__ __launch_bounds__(TPB) void k_voxel_particles(const uint32_t* __restrict__ abort_flag, VoxelArrays v,
+                                                         int V, ParticleArrays P, float min_vel,
+                                                         int32_t* __restrict__ heavy, DevScalars* sc) {
     if (*abort_flag) return;
     int c = blockIdx.x * TPB + threadIdx.x;
     float wmax = 0.0f;
     if (c < V) {
-        int st = v.start[c];
+        const int st = v.start[c];
         if (st == -1) {
             v.mvx[c] = 0.0f;
             v.mvy[c] = 0.0f;
             v.mvz[c] = 0.0f;
         } else {
-            int en = v.end[c];
-            float rho = v.pers[c], pr = v.pred[c];
-            float m = pr > 0.0f ? rho / pr : 0.0f;
-            v.mu[c] = m;
-            const float pA = 0.0f, lik = 1.0f;
-            const float min2 = min_vel * min_vel;
-            float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-            for (int j = st; j <= en; ++j) {
-                float w = P.w[j];
-                float post = lik * w;
-                float wn = pA * m * post + (1.0f - pA) * m * w;
-                float vx = P.vx[j], vy = P.vy[j], vz = P.vz[j];
-                sx += wn * vx;
-                sy += wn * vy;
-                sz += wn * vz;
-                if (vx * vx + vy * vy + vz * vz < min2) wn = 0.0f;
-                P.w[j] = wn;
-                if (wn > wmax && wn - wn == 0.0f) wmax = wn;
+            const int en = v.end[c];
+            if (en - st + 1 > HEAVY_PARTICLES) {
+                heavy[atomicAdd(&sc->n_heavy_vp, 1u)] = c;
+            } else {
+                const float rho = v.pers[c], pr = v.pred[c];
+                const float m = pr > 0.0f ? rho / pr : 0.0f;
+                v.mu[c] = m;
+                const float min2 = min_vel * min_vel;
+                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+                int j = st;
+                for (; j + 3 <= en; j += 4) {
+                    float w[4], x[4], y[4], z[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        w[k] = P.w[j + k]; x[k] = P.vx[j + k]; y[k] = P.vy[j + k]; z[k] = P.vz[j + k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        float pen;
+                        const float wn = persist_weight(w[k], m, x[k], y[k], z[k], min2, &pen);
+                        sx += wn * x[k];
+                        sy += wn * y[k];
+                        sz += wn * z[k];
+                        P.w[j + k] = pen;
+                        if (pen > wmax && pen - pen == 0.0f) wmax = pen;
+                    }
+                }
+                for (; j <= en; ++j) {
+                    const float x = P.vx[j], y = P.vy[j], z = P.vz[j];
+                    float pen;
+                    const float wn = persist_weight(P.w[j], m, x, y, z, min2, &pen);
+                    sx += wn * x;
+                    sy += wn * y;
+                    sz += wn * z;
+                    P.w[j] = pen;
+                    if (pen > wmax && pen - pen == 0.0f) wmax = pen;
+                }
+                v.mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
+                v.mvy[c] = rho > 0.0f ? sy / rho : 0.0f;
+                v.mvz[c] = rho > 0.0f ? sz / rho : 0.0f;
             }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
+}
+
+__global__ __launch_bounds__(TPB) void k_voxel_particles_heavy(const uint32_t* __restrict__ abort_flag,
+                                                               VoxelArrays v, ParticleArrays P, float min_vel,
+                                                               const int32_t* __restrict__ heavy, DevScalars* sc) {
+    if (*abort_flag) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t n = sc->n_heavy_vp;
+    const float min2 = min_vel * min_vel;
+    float wmax = 0.0f;
+    for (uint32_t h = (blockIdx.x * TPB + threadIdx.x) >> 6; h < n; h += gridDim.x * (TPB / 64)) {
+        const int c = __builtin_amdgcn_readfirstlane(heavy[h]);
+        const int st = __builtin_amdgcn_readfirstlane(v.start[c]), en = __builtin_amdgcn_readfirstlane(v.end[c]);
+        const float rho = v.pers[c], pr = v.pred[c];
+        const float m = pr > 0.0f ? rho / pr : 0.0f;
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        for (int b = st; b <= en; b += 64) {
+            const int j = b + lane;
+            float px = 0.0f, py = 0.0f, pz = 0.0f;
+            if (j <= en) {
+                const float x = P.vx[j], y = P.vy[j], z = P.vz[j];
+                float pen;
+                const float wn = persist_weight(P.w[j], m, x, y, z, min2, &pen);
+                px = wn * x;
+                py = wn * y;
+                pz = wn * z;
+                P.w[j] = pen;
+                if (pen > wmax && pen - pen == 0.0f) wmax = pen;
+            }
+            const int cnt = min(64, en - b + 1);
+            for (int q = 0; q < cnt; ++q) {
+                sx += lane_value(px, q);
+                sy += lane_value(py, q);
+                sz += lane_value(pz, q);
+            }
+        }
+        if (lane == 0) {
+            v.mu[c] = m;
             v.mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
             v.mvy[c] = rho > 0.0f ? sy / rho : 0.0f;
             v.mvz[c] = rho > 0.0f ? sz / rho : 0.0f;
         }
     }
     for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
+    if (lane == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
 }
-hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V, ParticleArrays P, float min_vel,
-                                  DevScalars* sc) {
-    hipLaunchKernelGGL(k_voxel_particles, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, P, min_vel, sc);
+
+hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
+                                  ParticleArrays P, float min_vel, int32_t* heavy, DevScalars* sc) {
+    hipLaunchKernelGGL(k_voxel_particles, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, P, min_vel,
+                       heavy, sc);
+    hipLaunchKernelGGL(k_voxel_particles_heavy, dim3(HEAVY_GRID), dim3(TPB), 0, st, abort_flag, v, P, min_vel,
+                       heavy, sc);
     return hipGetLastError();
 }
 

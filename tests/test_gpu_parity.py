@@ -68,3 +68,15 @@ def test_zero_time_first_frame_reinitialises(gpu):
         return t - 1.0, pose, xs, ys, zs
     w = small_workload(frames=4, seed=13)
     run_pair(w, w.frames, mutate=mutate)
+
+
+def test_crowded_voxels_bit_exact(gpu):
+    """60,000 particles on 2,048 voxels: many voxels hold more than 64
+    particles (the wavefront-per-voxel paths of the segment sums)."""
+    from k3dom_amd.scene import small_workload
+    w = small_workload(particles=60000, newborn=6000, frames=4, seed=21)
+    gd, od = run_pair(w, w.frames)
+    cells = gd.copyCellsToHost()
+    occ = cells["start_idx"] >= 0
+    counts = cells["end_idx"][occ] - cells["start_idx"][occ] + 1
+    assert counts.max() > 64
