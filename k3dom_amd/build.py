@@ -3,6 +3,7 @@
 libk3dom.so    the product: gfx950 kernels + C ABI (hipcc --offload-arch=gfx950)
 libk3scene.so  the synthetic box-scene / LiDAR generator used by the benchmark
                and the tests (host C++)
+k3dom_replay   frame-replay driver over the C++ drop-in header include/k3dom/dom.h
 """
 import os
 import subprocess
@@ -45,12 +46,22 @@ def build_scene():
     return out
 
 
+def build_replay():
+    out = os.path.join(HERE, "k3dom_replay")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "k3_replay.cpp"),
+           "-L", HERE, "-lk3dom", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_scene()
     build_lib()
+    build_replay()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "lib": build_lib, "scene": build_scene}[w]()
+        {"all": build_all, "lib": build_lib, "scene": build_scene, "replay": build_replay}[w]()

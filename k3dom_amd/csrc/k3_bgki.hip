@@ -45,12 +45,15 @@ __device__ __forceinline__ void bg_wave_sync() {
 }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Correctly rounded square root of x >= 0: the hardware estimate corrected by
-// one unit in the last place from the signs of the two neighbours' residuals.
-// Exact for x == 0 and x >= 2^-96; when some lane of the wave holds a
-// non-zero x below that, the wave takes sqrtf (same result, with rescaling).
+// Correctly rounded square root of max(x, 2^-96) for x >= 0: the hardware
+// estimate corrected by one unit in the last place from the signs of the two
+// neighbours' residuals (exact for arguments >= 2^-96).  Only kernel values
+// are taken of it: for x < 2^-96 both this result and sqrtf(x) are <= 2^-48,
+// and fmaf(d, k3_kernel_scale(ls), -1) rounds to -1 for every d <= 2^-48
+// when ls >= 2^-22 (the handle rejects smaller length scales), so the
+// kernel value equals that of sqrtf(x).
 __device__ __forceinline__ float bg_sqrt(float x) {
-    if (__any(x > 0.0f && x < 0x1p-96f)) return sqrtf(x);
+    x = fmaxf(x, 0x1p-96f);
     const float y = __builtin_amdgcn_sqrtf(x);
     const uint32_t yb = __builtin_bit_cast(uint32_t, y);
     const float ym = __builtin_bit_cast(float, yb - 1u);
@@ -438,7 +441,7 @@ hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& T, co
 // plus F_OCC when a voxel of the tile may lie within ls of the point.
 constexpr uint32_t C_GENERAL = 0, C_MAIN = 1, C_BEHIND = 2, C_NOFREE = 3, F_OCC = 4;
 
-__global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls, float ls2g, float inv_ls,
+__global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls, float ls2g, float kscale,
                                               float sigma, const uint4* __restrict__ items,
                                               const uint32_t* __restrict__ tlist,
                                               const K3PointRec* __restrict__ rec, float* __restrict__ del_occ,
@@ -466,7 +469,8 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
         const float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
         const float dc2 = dc * dc;
         // free term of a voxel for every point in front of which it lies
-        const float kc = dc < ls ? k3_kernel_value(dc, inv_ls, sigma) : 0.0f;
+        // (k3_kernel_value is an exact zero for distances >= ls)
+        const float kc = k3_kernel_value(dc, kscale, sigma);
         // bounding box of the tile's 64 voxel centres (lanes outside the grid included)
         const float tcx = 0.5f * (axis_center(tx * 4, g.gs, g.res, g.sx) + axis_center(tx * 4 + 3, g.gs, g.res, g.sx));
         const float tcy = 0.5f * (axis_center(ty * 4, g.gs, g.res, g.sy) + axis_center(ty * 4 + 3, g.gs, g.res, g.sy));
@@ -515,22 +519,14 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
                     const float ox = vx - r0.x, oy = vy - r0.y, oz = vz - r0.z;
                     const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
                     const bool co = so < ls2g;
-                    if (__any(co)) {
-                        const float d = bg_sqrt(so);
-                        const float kv = k3_kernel_value(d, inv_ls, sigma);
-                        cur_o += (co && d < ls) ? kv : 0.0f;
-                    }
+                    if (__any(co)) cur_o += k3_kernel_value(bg_sqrt(so), kscale, sigma);
                 }
                 const uint32_t fc = cls & 3u;
                 if (fc == C_MAIN) {  // 0 < dp < mg for every lane
                     const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
                     const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
                     const bool cf = a2 < ls2g;
-                    if (__any(cf)) {
-                        const float d = bg_sqrt(a2);
-                        const float kv = k3_kernel_value(d, inv_ls, sigma);
-                        cur_f += (cf && d < ls) ? kv : 0.0f;
-                    }
+                    if (__any(cf)) cur_f += k3_kernel_value(bg_sqrt(a2), kscale, sigma);
                 } else if (fc == C_BEHIND) {  // dp <= 0 for every lane
                     cur_f += kc;
                 } else if (fc == C_GENERAL) {  // k3_free_distance, the three cases as selects
@@ -546,9 +542,8 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
                     const float a = c2 ? a2 : a3;
                     const bool cs = (c2 || c3) && a < ls2g;
                     if (__any(cs)) {
-                        const float d = bg_sqrt(a);
-                        const float kv = k3_kernel_value(d, inv_ls, sigma);
-                        cur_f += cs ? (d < ls ? kv : 0.0f) : (c1 ? kc : 0.0f);
+                        const float kv = k3_kernel_value(bg_sqrt(a), kscale, sigma);
+                        cur_f += cs ? kv : (c1 ? kc : 0.0f);
                     } else {
                         cur_f += c1 ? kc : 0.0f;
                     }
@@ -573,7 +568,8 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
                        float sigma, const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec, float* del_occ,
                        float* del_free, DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls), 1.0f / ls, sigma,
+    hipLaunchKernelGGL(k_bgki, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls), k3_kernel_scale(ls),
+                       sigma,
                        I.items, T.list, rec, del_occ, del_free, I.partial, sc);
     return hipGetLastError();
 }

@@ -406,13 +406,13 @@ __device__ __forceinline__ void atomic_max_nonneg(uint32_t* dst, float v) {
     if (v > 0.0f && v - v == 0.0f) atomicMax(dst, k3_f2u(v));
 }
 
-// Per-voxel particle loops.  A voxel's segment sums run over its particles in
-// ascending order (one sequential FP32 sum).  Voxels with at most
-// HEAVY_PARTICLES particles are summed by one lane with 8 loads in flight;
-// larger ones are queued and summed by a whole wavefront (coalesced loads,
-// the same left-to-right accumulation broadcast with readlane), so the
-// results do not depend on which path a voxel takes.
-constexpr int HEAVY_PARTICLES = 64;
+// Per-voxel particle loops.  A voxel's segment sums use the two-level
+// association of K3_SEG_BLOCK: voxels with at most K3_SEG_BLOCK particles (one
+// block, a plain left-to-right sum) are summed by one lane with 8 loads in
+// flight; larger ones are queued and handled by a whole wavefront whose lane l
+// sums blocks l, l + 64, ... and then adds the block sums in block order
+// (broadcast with readlane), so the results do not depend on the path taken.
+constexpr int HEAVY_PARTICLES = K3_SEG_BLOCK;
 constexpr int HEAVY_GRID = 1024;
 
 __device__ __forceinline__ float seq_sum(const float* __restrict__ x, int st, int en) {
@@ -432,14 +432,20 @@ __device__ __forceinline__ float lane_value(float v, int q) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), q));
 }
 
-__device__ __forceinline__ float wave_seq_sum(const float* __restrict__ x, int st, int en) {
+// Two-level segment sum of x[st..en] by one wavefront (st, en uniform).
+__device__ __forceinline__ float wave_seg_sum(const float* __restrict__ x, int st, int en) {
     const int lane = threadIdx.x & 63;
+    const int nb = (en - st + K3_SEG_BLOCK) / K3_SEG_BLOCK;
     float s = 0.0f;
-    for (int b = st; b <= en; b += 64) {
-        const int j = b + lane;
-        const float v = j <= en ? x[j] : 0.0f;
-        const int cnt = min(64, en - b + 1);
-        for (int q = 0; q < cnt; ++q) s += lane_value(v, q);
+    for (int b0 = 0; b0 < nb; b0 += 64) {
+        const int b = b0 + lane;
+        float bs = 0.0f;
+        if (b < nb) {
+            const int j0 = st + b * K3_SEG_BLOCK;
+            bs = seq_sum(x, j0, min(en, j0 + K3_SEG_BLOCK - 1));
+        }
+        const int cnt = min(64, nb - b0);
+        for (int q = 0; q < cnt; ++q) s += lane_value(bs, q);
     }
     return s;
 }
@@ -499,7 +505,7 @@ __global__ __launch_bounds__(TPB) void k_mass_update_heavy(const uint32_t* __res
     for (uint32_t h = (blockIdx.x * TPB + threadIdx.x) >> 6; h < n; h += gridDim.x * (TPB / 64)) {
         const int c = __builtin_amdgcn_readfirstlane(heavy[h]);
         const int st = __builtin_amdgcn_readfirstlane(v.start[c]), en = __builtin_amdgcn_readfirstlane(v.end[c]);
-        const float dyn_pred = wave_seq_sum(weights, st, en);
+        const float dyn_pred = wave_seg_sum(weights, st, en);
         if (lane == 0)
             atomic_max_nonneg(&sc->born_max_bits,
                               mass_voxel(v, c, dyn_pred, del_occ, del_free, decay, mass_scale, p_B, born));
@@ -604,25 +610,29 @@ __global__ __launch_bounds__(TPB) void k_voxel_particles_heavy(const uint32_t* _
         const int st = __builtin_amdgcn_readfirstlane(v.start[c]), en = __builtin_amdgcn_readfirstlane(v.end[c]);
         const float rho = v.pers[c], pr = v.pred[c];
         const float m = pr > 0.0f ? rho / pr : 0.0f;
+        const int nb = (en - st + K3_SEG_BLOCK) / K3_SEG_BLOCK;
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        for (int b = st; b <= en; b += 64) {
-            const int j = b + lane;
-            float px = 0.0f, py = 0.0f, pz = 0.0f;
-            if (j <= en) {
-                const float x = P.vx[j], y = P.vy[j], z = P.vz[j];
-                float pen;
-                const float wn = persist_weight(P.w[j], m, x, y, z, min2, &pen);
-                px = wn * x;
-                py = wn * y;
-                pz = wn * z;
-                P.w[j] = pen;
-                if (pen > wmax && pen - pen == 0.0f) wmax = pen;
+        for (int b0 = 0; b0 < nb; b0 += 64) {
+            const int b = b0 + lane;
+            float bx = 0.0f, by = 0.0f, bz = 0.0f;
+            if (b < nb) {
+                const int j0 = st + b * K3_SEG_BLOCK, j1 = min(en, j0 + K3_SEG_BLOCK - 1);
+                for (int j = j0; j <= j1; ++j) {
+                    const float x = P.vx[j], y = P.vy[j], z = P.vz[j];
+                    float pen;
+                    const float wn = persist_weight(P.w[j], m, x, y, z, min2, &pen);
+                    bx += wn * x;
+                    by += wn * y;
+                    bz += wn * z;
+                    P.w[j] = pen;
+                    if (pen > wmax && pen - pen == 0.0f) wmax = pen;
+                }
             }
-            const int cnt = min(64, en - b + 1);
+            const int cnt = min(64, nb - b0);
             for (int q = 0; q < cnt; ++q) {
-                sx += lane_value(px, q);
-                sy += lane_value(py, q);
-                sz += lane_value(pz, q);
+                sx += lane_value(bx, q);
+                sy += lane_value(by, q);
+                sz += lane_value(bz, q);
             }
         }
         if (lane == 0) {

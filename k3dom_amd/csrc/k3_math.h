@@ -61,6 +61,13 @@
 // this association, so the sum does not depend on how the work is split.
 #define K3_BGKI_CHUNK_LOG2 10
 
+// A voxel's particle sums (predicted dynamic mass, velocity moments) are also
+// two-level: the voxel's particles, in sorted order, form blocks of
+// K3_SEG_BLOCK starting at its first particle; each block is summed left to
+// right and the block sums are added left to right.  A voxel with at most
+// K3_SEG_BLOCK particles is one plain left-to-right sum.
+#define K3_SEG_BLOCK 64
+
 // Philox stream ids (third counter word) of the random draws of one frame.
 #define K3_STREAM_PREDICT 1u
 #define K3_STREAM_NEWBORN 2u
@@ -186,12 +193,13 @@ K3_HD void k3_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) {
 // ---------------------------------------------------------------- kernel inference
 // Sparse kernel of src/k3dom/kernel/mass_update.cu:24-26:
 //   k(d) = max(0, sigma * ((2 + cospi(2d/ls)) (1 - d/ls) / 3 + sinpi(2d/ls) / (2*3.141592)))
-// (with the reference's truncated pi).  With q = d * (1/ls), the bracket is a
+// (with the reference's truncated pi).  With q = d / ls, the bracket is a
 // smooth function of q on [0, 1]; it is evaluated as a degree-14 polynomial in
 // x = 2q - 1 interpolated at Chebyshev points (coefficients printed by
-// gen_kernel_poly.py beside this file): float32 Horner error <= 1.6e-7 of the
-// bracket over [0, 1], against 4e-7 for a sin/cos evaluation.
-#define K3_KP0 0.1666666716337204f
+// gen_kernel_poly.py beside this file): float32 Horner error <= 1.7e-7 of the
+// bracket over [0, 1], against 4e-7 for a sin/cos evaluation.  K3_KP0 makes
+// the float32 Horner value at x = 1 exactly zero.
+#define K3_KP0 0.1666666865348816f
 #define K3_KP1 (-0.6666668057441711f)
 #define K3_KP2 0.822467029094696f
 #define K3_KP3 5.713734481105348e-07f
@@ -206,9 +214,20 @@ K3_HD void k3_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) {
 #define K3_KP12 (-0.0003179309132974595f)
 #define K3_KP13 0.00019976771727669984f
 #define K3_KP14 1.4936805200704839e-05f
-K3_HD float k3_kernel_value(float d, float inv_ls, float sigma) {
-    float q = d * inv_ls;
-    float x = fmaf(2.0f, q, -1.0f);
+
+// Scale of the kernel argument: 2/ls rounded up so that ls * scale >= 2
+// exactly.  Then x = fmaf(d, scale, -1) >= 1 for every d >= ls, the clamp to 1
+// and the exact zero at x = 1 give a zero term, and the evaluator needs no
+// separate d < ls test.
+K3_HD float k3_kernel_scale(float ls) {
+    float s = 2.0f / ls;
+    if ((double)s * (double)ls < 2.0) s = nextafterf(s, 3.0e38f);
+    return s;
+}
+
+// k(d) for d >= 0 with scale = k3_kernel_scale(ls); exactly +0 or -0 for d >= ls.
+K3_HD float k3_kernel_value(float d, float scale, float sigma) {
+    float x = fminf(fmaf(d, scale, -1.0f), 1.0f);
     float p = fmaf(K3_KP14, x, K3_KP13);
     p = fmaf(p, x, K3_KP12);
     p = fmaf(p, x, K3_KP11);

@@ -108,20 +108,32 @@ struct Geom {
     }
 };
 
+// Two-level sum of x[st..en] (K3_SEG_BLOCK): blocks of K3_SEG_BLOCK elements
+// from st, each summed left to right, block sums added left to right.
+static float seg_sum(const float* x, int st, int en) {
+    float s = 0.0f;
+    for (int b = st; b <= en; b += K3_SEG_BLOCK) {
+        float bs = 0.0f;
+        for (int j = b; j <= std::min(en, b + K3_SEG_BLOCK - 1); ++j) bs += x[j];
+        s += bs;
+    }
+    return s;
+}
+
 static void bgki_pair(const Geom& g, int i, int j, int k, const K3PointRec& pr, float ls, float ls2g,
-                      float inv_ls, float sigma, int32_t chunk, ChunkAcc& acc, BgkiStats& st) {
+                      float kscale, float sigma, int32_t chunk, ChunkAcc& acc, BgkiStats& st) {
     float vx = g.cx[i], vy = g.cy[j], vz = g.cz[k];
     size_t v = (size_t)i + (size_t)g.gs * j + (size_t)g.gs * g.gs * k;
     float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
     float dc2 = dc * dc;
     float d_o = k3_occ_distance(vx, vy, vz, pr, ls, ls2g);
     if (d_o >= 0.0f) {
-        acc.add(acc.tot_o, acc.cur_o, acc.chunk_o, v, chunk, k3_kernel_value(d_o, inv_ls, sigma));
+        acc.add(acc.tot_o, acc.cur_o, acc.chunk_o, v, chunk, k3_kernel_value(d_o, kscale, sigma));
         st.occ_pairs++;
     }
     float d_f = k3_free_distance(vx, vy, vz, dc, dc2, pr, ls, ls2g);
     if (d_f >= 0.0f) {
-        acc.add(acc.tot_f, acc.cur_f, acc.chunk_f, v, chunk, k3_kernel_value(d_f, inv_ls, sigma));
+        acc.add(acc.tot_f, acc.cur_f, acc.chunk_f, v, chunk, k3_kernel_value(d_f, kscale, sigma));
         st.free_pairs++;
     }
 }
@@ -155,7 +167,7 @@ static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const
     acc.init(V);
     BgkiStats st;
     const float ls2g = k3_ls2_guard(ls);
-    const float inv_ls = 1.0f / ls;
+    const float kscale = k3_kernel_scale(ls);
     const double R = (double)ls * (1.0 + 1e-5) + 1e-4;
     for (int p = 0; p < n; ++p) {
         K3PointRec pr = k3_point_prep(xs[p], ys[p], zs[p], g.res);
@@ -165,7 +177,7 @@ static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const
             for (int k = 0; k < g.gsz; ++k)
                 for (int j = 0; j < g.gs; ++j)
                     for (int i = 0; i < g.gs; ++i)
-                        bgki_pair(g, i, j, k, pr, ls, ls2g, inv_ls, sigma, chunk, acc, st);
+                        bgki_pair(g, i, j, k, pr, ls, ls2g, kscale, sigma, chunk, acc, st);
             continue;
         }
         double mx = pr.x, my = pr.y, mz = pr.z;
@@ -184,7 +196,7 @@ static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const
                 double zlo = std::min(tc * mz, td * mz) - R, zhi = std::max(tc * mz, td * mz) + R;
                 if (!axis_range(zlo, zhi, g.res, g.sz, g.gsz, &k0, &k1)) continue;
                 for (int k = k0; k <= k1; ++k)
-                    bgki_pair(g, i, j, k, pr, ls, ls2g, inv_ls, sigma, chunk, acc, st);
+                    bgki_pair(g, i, j, k, pr, ls, ls2g, kscale, sigma, chunk, acc, st);
             }
         }
     }
@@ -350,9 +362,7 @@ struct Oracle {
         t0 = clk::now();
         for (int i = 0; i < V; ++i) {
             float pred_total = (f[i] + d[i]) * decay;
-            float dyn_pred = 0.0f;
-            if (start[i] != -1)
-                for (int j = start[i]; j < end[i] + 1; ++j) dyn_pred += wsorted[j];
+            const float dyn_pred = start[i] != -1 ? seg_sum(wsorted.data(), start[i], end[i]) : 0.0f;
             k3_predict_mass(&f[i], &s[i], &d[i], dyn_pred, pred_total, decay);
             k3_update_mass(&f[i], &s[i], &d[i], del_occ[i], del_free[i], p.mass_scale);
             born[i] = k3_newborn_part(dyn_pred, pred_total, d[i], p.birth_prob);
@@ -477,11 +487,17 @@ struct Oracle {
     void moments() {
         for (int c = 0; c < V; ++c) {
             if (start[c] != -1) {
-                float sx = 0, sy = 0, sz = 0;
-                for (int j = start[c]; j <= end[c]; ++j) {
-                    sx += w[j] * vx[j];
-                    sy += w[j] * vy[j];
-                    sz += w[j] * vz[j];
+                float sx = 0, sy = 0, sz = 0;  // two-level segment sums (K3_SEG_BLOCK)
+                for (int b = start[c]; b <= end[c]; b += K3_SEG_BLOCK) {
+                    float bx = 0, by = 0, bz = 0;
+                    for (int j = b; j <= std::min(end[c], b + K3_SEG_BLOCK - 1); ++j) {
+                        bx += w[j] * vx[j];
+                        by += w[j] * vy[j];
+                        bz += w[j] * vz[j];
+                    }
+                    sx += bx;
+                    sy += by;
+                    sz += bz;
                 }
                 float rho = pers[c];
                 mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
@@ -717,7 +733,8 @@ void orc_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
 float orc_ln(float x) { return k3_ln(x); }
 float orc_exp(float x) { return k3_exp(x); }
 void orc_sincospi(float t, float* s, float* c) { k3_sincospi(t, s, c); }
-float orc_kernel(float d, float inv_ls, float sigma) { return k3_kernel_value(d, inv_ls, sigma); }
+float orc_seg_sum(const float* x, int32_t st, int32_t en) { return seg_sum(x, st, en); }
+float orc_kernel(float d, float ls, float sigma) { return k3_kernel_value(d, k3_kernel_scale(ls), sigma); }
 float orc_credit(float m, float scale) { return k3_credit(m, scale); }
 void orc_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) { k3_box_muller(a, b, n0, n1); }
 void orc_update_mass(float* fsd, float del_occ, float del_free, float mass_scale) {

@@ -41,6 +41,7 @@ def lib():
             "orc_exp": (C.c_float, [C.c_float]),
             "orc_sincospi": (None, [C.c_float, fp, fp]),
             "orc_kernel": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+            "orc_seg_sum": (C.c_float, [C.POINTER(C.c_float), C.c_int32, C.c_int32]),
             "orc_credit": (C.c_float, [C.c_float, C.c_float]),
             "orc_box_muller": (None, [C.c_uint32, C.c_uint32, fp, fp]),
             "orc_update_mass": (None, [fp, C.c_float, C.c_float, C.c_float]),

@@ -80,3 +80,15 @@ def test_crowded_voxels_bit_exact(gpu):
     occ = cells["start_idx"] >= 0
     counts = cells["end_idx"][occ] - cells["start_idx"][occ] + 1
     assert counts.max() > 64
+
+
+def test_very_crowded_voxels_bit_exact(gpu):
+    """200,000 particles on an 8x8x4 grid: voxels with more than 64 blocks of
+    64 particles (several rounds of the wavefront's block loop)."""
+    from k3dom_amd.scene import small_workload
+    w = small_workload(edge=8, edge_z=4, particles=200000, newborn=20000, frames=2, seed=23, beams=8, az=64)
+    gd, od = run_pair(w, w.frames)
+    cells = gd.copyCellsToHost()
+    occ = cells["start_idx"] >= 0
+    counts = cells["end_idx"][occ] - cells["start_idx"][occ] + 1
+    assert counts.max() > 64 * 64

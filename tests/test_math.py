@@ -91,7 +91,7 @@ def _kern64(d, ls, sigma):
 
 def test_kernel_closed_form(oracle_lib):
     ls, sigma = np.float32(0.5), np.float32(0.1)
-    inv = np.float32(1) / ls
+    inv = ls  # the evaluator takes the length scale itself
     # k(0) = sigma exactly; k(d) -> 0 as d -> ls; monotone non-increasing
     assert oracle_lib.orc_kernel(0.0, float(inv), float(sigma)) == float(sigma)
     ds = np.linspace(0, float(ls), 5001, endpoint=False).astype(np.float32)
@@ -171,8 +171,22 @@ def test_kernel_polynomial_matches_generator_and_formula(oracle_lib):
     for i, c in enumerate(want):
         assert got[i] == c, (i, got[i], c)
     ls, sigma = 0.5, 1.0
-    inv = float(np.float32(1) / np.float32(ls))
     ds = np.linspace(0, ls, 20001, endpoint=False).astype(np.float32)
-    vals = np.array([oracle_lib.orc_kernel(float(d), inv, sigma) for d in ds])
-    ref = np.maximum(0.0, gkp.k(ds.astype(np.float64) * np.float64(np.float32(inv))))
+    vals = np.array([oracle_lib.orc_kernel(float(d), ls, sigma) for d in ds])
+    ref = np.maximum(0.0, gkp.k(ds.astype(np.float64) / ls))
     assert np.max(np.abs(vals - ref)) < 2e-7
+
+
+@pytest.mark.parametrize("ls", [0.5, 0.3, 0.1, 0.7, 1.3, 0.05, 2.0, 0.123456])
+def test_kernel_is_exactly_zero_from_the_length_scale_on(oracle_lib, ls):
+    """Every d >= ls gives an exact zero (so a term outside the support adds
+    nothing, and culling such pairs cannot change a sum), for either sign of
+    sigma; just below ls the value is tiny and non-negative."""
+    ls32 = np.float32(ls)
+    above = [ls32, np.nextafter(ls32, np.float32(np.inf)), ls32 * np.float32(1.0001), ls32 * 2, np.float32(1e30)]
+    for d in above:
+        for sigma in (0.1, 1.0, -0.1):
+            assert oracle_lib.orc_kernel(float(d), float(ls32), sigma) == 0.0, (ls, d, sigma)
+    below = np.nextafter(ls32, np.float32(0))
+    v = oracle_lib.orc_kernel(float(below), float(ls32), 1.0)
+    assert 0.0 <= v < 1e-6

@@ -295,3 +295,30 @@ def test_first_frame_places_particles_by_born_mass(small_run):
     exp = born.astype(np.float64) / born.astype(np.float64).sum() * w.params.particle_count
     assert np.all(np.abs(counts - exp) <= 1.0 + 1e-9)
     assert np.all(o.read_array("PARTICLE_WEIGHT") == np.float32(1.0) / np.float32(w.params.particle_count))
+
+
+def test_segment_sum_two_level_association(oracle_lib):
+    """Voxel particle sums: blocks of 64 from the voxel's first particle, each
+    summed left to right, block sums added left to right."""
+    def seg(x, st, en):
+        a = np.ascontiguousarray(x, np.float32)
+        return oracle_lib.orc_seg_sum(a.ctypes.data_as(C.POINTER(C.c_float)), st, en)
+    # exactly representable terms: every element is added once
+    x = (2.0 ** -np.arange(0, 20)).astype(np.float32)
+    x = np.tile(x, 20)
+    for st, en in [(0, 0), (3, 66), (0, 63), (0, 64), (5, 399), (64, 127)]:
+        assert seg(x, st, en) == float(np.sum(x[st:en + 1].astype(np.float64)))
+    # 1e8 absorbs the ones of its own block (ulp 8) but not the next block's 64,
+    # which are summed exactly before being added: 1e8 + 64
+    y = np.ones(1 + 63 + 64, np.float32)
+    y[0] = 1e8
+    assert seg(y, 0, len(y) - 1) == 100000064.0
+    # a plain left-to-right sum would give 1e8; blocks start at st, not at 0
+    z = np.concatenate([np.ones(10, np.float32), y])
+    assert seg(z, 10, len(z) - 1) == 100000064.0
+    # at most one block: the plain left-to-right float32 sum
+    r = np.random.default_rng(1).uniform(0, 1, 64).astype(np.float32)
+    acc = np.float32(0)
+    for v in r:
+        acc = np.float32(acc + v)
+    assert seg(r, 0, 63) == float(acc)
