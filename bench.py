@@ -31,6 +31,14 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 #   free term:     projection 6 + compares 2 + distance 4 + compare 1 + kernel 14 + accumulate 1 = 28
 FLOPS_OCC_PAIR = 25
 FLOPS_FREE_PAIR = 28
+HBM_PEAK_GBPS = 8000.0
+# Compulsory HBM traffic of one frame in this structure-of-arrays layout (each
+# stage reads its inputs once and writes its outputs once; the sort counted as
+# a key histogram read plus one key+payload read and write; u64 CDFs):
+# per persistent particle 252 B, per newborn 64 B, per voxel 108 B, per voxel
+# of a map shift 24 B more, per point 28 B.
+BYTES_PARTICLE, BYTES_NEWBORN, BYTES_VOXEL, BYTES_SHIFT, BYTES_POINT = 252, 64, 108, 24, 28
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bgki_cfg2_traffic.json")
 
 
 def log(*a):
@@ -63,6 +71,8 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="skip the pair-count pass")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after timing, run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build has no multi-GPU path (the per-frame update runs on one "
@@ -89,11 +99,15 @@ def main():
     torch.cuda.synchronize()
 
     d = DOM(w.params)
+    shifts = [0]
 
     def step(f):
         t, pose, xs, ys, zs = frames[f]
         dx, dy, dz = dpts[f]
+        c0 = d.getCenterPosition()
         d.updatePose(*pose)
+        if f >= args.warmup and d.getCenterPosition() != c0:
+            shifts[0] += 1
         d.updateGridDevice(t, dx.data_ptr(), dy.data_ptr(), dz.data_ptr(), len(xs))
 
     for f in range(args.warmup):
@@ -137,13 +151,35 @@ def main():
                 "launch_ms": round(kernel_s * 1e3, 4),
                 "pairs_per_launch": [occ // args.steps, free // args.steps]}
 
+    p = w.params
+    gs = int(np.float32(p.size) / np.float32(p.resolution))
+    gsz = int(np.float32(p.size_z) / np.float32(p.resolution))
+    V = gs * gs * gsz
+    pts = float(np.mean(npts[args.warmup:]))
+    model = (p.particle_count * BYTES_PARTICLE + p.new_born_particle_count * BYTES_NEWBORN + V * BYTES_VOXEL +
+             shifts[0] / args.steps * V * BYTES_SHIFT + pts * BYTES_POINT)
+    frame_gbps = model / (ms_per_step * 1e-3) / 1e9
+    frame_hbm = {"bytes_per_frame": int(model), "achieved": round(frame_gbps, 2), "peak": HBM_PEAK_GBPS,
+                 "unit": "GB/s", "frac": round(frame_gbps / HBM_PEAK_GBPS, 5)}
+
+    if roof is not None and os.path.exists(TRAFFIC_FILE) and w.name.startswith("cfg2"):
+        with open(TRAFFIC_FILE) as fh:
+            tr = json.load(fh)
+        roof["traffic"] = tr.get("bytes_per_launch")
+        roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+
+    if args.copy_probe:
+        a = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        log("copy probe: one neg kernel, 256 MiB read + 256 MiB written")
+        del a, b
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(w, frames)
 
-    p = w.params
-    gs = int(np.float32(p.size) / np.float32(p.resolution))
-    gsz = int(np.float32(p.size_z) / np.float32(p.resolution))
     line = {
         "metric": METRIC,
         "value": round(args.steps / elapsed, 3),
@@ -163,6 +199,7 @@ def main():
                    "ego_speed_mps": float(w.scene.ego_vx), "parallelism": "single GPU"},
         "stage_ms": {n: round(float(v), 4) for n, v in zip(DOM.STAGES, stage_avg)},
         "roofline": roof,
+        "frame_hbm": frame_hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -18,14 +18,18 @@
 //     instead of per-thread cuRAND XORWOW states;
 //   * systematic resampling on a u64 fixed-point CDF instead of multinomial
 //     resampling on an FP32 scan (src/common/dom.cu:265-289);
-//   * per-voxel segment sums in ascending particle order instead of
+//   * per-voxel segment sums (predicted dynamic mass, velocity moments) with
+//     the two-level association of K3_SEG_BLOCK in k3_math.h (left-to-right
+//     sums of 64-particle blocks, added left to right) instead of
 //     differences of a global FP32 inclusive scan (include/common/common.h:24-28);
 //   * newborn / initial particle counts from a u64 fixed-point CDF of the born
 //     masses (every slot is assigned; a zero total spreads slots uniformly)
 //     instead of truncating a normalised FP32 CDF
 //     (src/common/kernel/init_new_particles.cu:14-52);
 //   * BGKI sums use the two-level chunked association of k3_math.h and skip
-//     pairs whose distance is >= ls (their term is exactly zero);
+//     pairs whose distance is >= ls (their term is exactly zero); the sparse
+//     kernel is the degree-14 polynomial of k3_math.h (<= 1.7e-7 of sigma from
+//     the reference formula), exactly zero from d = ls on;
 //   * gamma^dt is computed once per frame on the host.
 //
 // Parity pins for this file live in tests/test_oracle.py: brute-force BGKI ==
