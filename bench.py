@@ -9,8 +9,11 @@ frame are uploaded to device memory before timing (inputs resident in HBM).
 
 Prints one JSON line (rank 0).  `roofline` is for the dominant kernel, the
 kernel-inference gather, whose algorithmic work is its contributing
-(voxel, point) pairs; `cpu_baseline` times the serial restatement on a
-bounded sample of the same workload on the host.
+(voxel, point) pairs; `frame_hbm` states the frame's compulsory HBM traffic
+against its time; `adjoint` times the differentiated window (forward +
+backward ms/frame against the plain forward, config-1 scene, ls 0.3, 8-frame
+windows); `cpu_baseline` times the serial restatement on a bounded sample of
+the same workload on the host.
 """
 import argparse
 import json
@@ -63,6 +66,55 @@ def cpu_baseline(workload, frames):
                       f"initialisation frame; serial C++ restatement, 1 thread; {dt:.2f} s"}
 
 
+def adjoint_leg(torch, reps=5):
+    """fwd+bwd ms/frame of the differentiated window against the plain forward
+    ms/frame, on the differentiated configuration (config-1 scene, ls 0.3,
+    8-frame windows, loss over every voxel).  Plain forward: the same 8 frames
+    from a fresh map through DOM (points resident in device memory)."""
+    from k3dom_amd.adjoint import Window, theta_of
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, workload
+    w = workload(3)
+    T = 8
+    win = Window(w.params, T)
+    recs = win.set_scene(Scene(w.scene))
+    dev = torch.device("cuda:0")
+    dpts = [tuple(torch.from_numpy(a).to(dev) for a in r[2:]) for r in recs]
+    th = theta_of(w.params)
+
+    def plain():
+        d = DOM(w.params)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for (t, pose, xs, _, _), (dx, dy, dz) in zip(recs, dpts):
+            d.updatePose(*pose)
+            d.updateGridDevice(t, dx.data_ptr(), dy.data_ptr(), dz.data_ptr(), len(xs))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        d.close()
+        return el
+
+    plain()
+    win.forward(th)
+    win.backward()
+    tp, tf, tb = [], [], []
+    for _ in range(reps):
+        tp.append(plain())
+        t0 = time.perf_counter()
+        win.forward(th)
+        t1 = time.perf_counter()
+        g = win.backward()
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+    win.close()
+    p_ms, f_ms, b_ms = (np.median(x) / T * 1e3 for x in (tp, tf, tb))
+    return {"workload": w.name, "frames": T, "reps": reps, "plain_fwd_ms_per_frame": round(p_ms, 4),
+            "fwd_ms_per_frame": round(f_ms, 4), "bwd_ms_per_frame": round(b_ms, 4),
+            "fwd_bwd_vs_plain": round((f_ms + b_ms) / p_ms, 3),
+            "grad": [float(x) for x in g], "theta": [float(x) for x in th]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +123,7 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="skip the pair-count pass")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiated-window leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -176,6 +229,11 @@ def main():
         log("copy probe: one neg kernel, 256 MiB read + 256 MiB written")
         del a, b
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg(torch)
+        log(f"differentiated window: fwd+bwd / plain = {adj['fwd_bwd_vs_plain']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(w, frames)
@@ -200,6 +258,7 @@ def main():
         "stage_ms": {n: round(float(v), 4) for n, v in zip(DOM.STAGES, stage_avg)},
         "roofline": roof,
         "frame_hbm": frame_hbm,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

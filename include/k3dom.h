@@ -204,6 +204,57 @@ typedef struct k3dom_stats {
 } k3dom_stats;
 k3dom_status k3dom_get_stats(const k3dom_handle* h, k3dom_stats* out);
 
+/* ---------------------------------------------------------------------------
+ * Differentiated window.  No counterpart in the reference (whose mass update is
+ * forward-only, src/k3dom/kernel/mass_update.cu:152-183): a fixed sequence of
+ * frames replayed on a fresh map, with the gradient of an occupancy loss with
+ * respect to six parameters
+ *     theta = {ls, sigma_occ, sigma_free, prior_free, prior_static, prior_dynamic}
+ * (sigma_occ / sigma_free weight the occupied / free kernel terms; the
+ * reference uses one sigma for both, src/k3dom/kernel/mass_update.cu:51,70).
+ * loss = mean over frames of the mean over masked voxels of
+ *     BCE(occ_mean, gt),  occ_mean = (s + d) / (f + s + d) clamped to [1e-6, 1 - 1e-6].
+ * Sort order, counts, penalties and resampled indices of the forward frames
+ * are frozen; the backward pass recomputes each frame's kernel-inference
+ * evidence (and its length-scale derivative) instead of storing it.
+ * ------------------------------------------------------------------------- */
+typedef struct k3dom_window k3dom_window;
+
+/* A window of `frames` frames over a map with these parameters (theta
+ * overrides ls, sigma and the priors at each forward). */
+k3dom_status k3dom_window_create(const k3dom_params* params, int32_t frames, k3dom_window** out);
+k3dom_status k3dom_window_destroy(k3dom_window* w);
+
+/* Input of frame f: time (s), sensor position (global frame, m), n points
+ * relative to the sensor (host arrays), and the 0/1 ground-truth occupancy of
+ * every voxel (grid_cell_count bytes, index x + gs*y + gs*gs*z). */
+k3dom_status k3dom_window_set_frame(k3dom_window* w, int32_t f, double t, const float pose[3], const float* xs,
+                                    const float* ys, const float* zs, int32_t n, const uint8_t* gt_occ);
+
+/* Voxels entering the loss (grid_cell_count bytes, non-zero = in); NULL
+ * selects every voxel. */
+k3dom_status k3dom_window_set_mask(k3dom_window* w, const uint8_t* mask);
+
+/* Runs the frames from a fresh map with parameters theta[6]; writes the loss. */
+k3dom_status k3dom_window_forward(k3dom_window* w, const float theta[6], double* loss);
+
+/* Gradient of dloss * loss with respect to theta after a forward.  When
+ * grad_prior_voxels is not NULL it receives 3 * grid_cell_count floats, the
+ * gradient with respect to each voxel's initial (free, static, dynamic) mass. */
+k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6], float* grad_prior_voxels);
+
+/* Record of forward frame f: which = 0 masses after the frame (3 * V float:
+ * free, static, dynamic), 1 predicted dynamic mass (V float), 2 frozen counts
+ * (4 * V int32: particles inside the grid, of them not penalised, newborns,
+ * of them not penalised), 3 the frame's map shift (3 int32). */
+k3dom_status k3dom_window_read_record(k3dom_window* w, int32_t f, int32_t which, void* dst, int64_t capacity);
+
+/* Milliseconds of the last forward and the last backward (host wall clock
+ * around the whole pass, device synchronised). */
+k3dom_status k3dom_window_times(const k3dom_window* w, float ms[2]);
+
+const char* k3dom_window_last_error(const k3dom_window* w);
+
 /* Text of the last error on h (or of the last failed create when h is NULL). */
 const char* k3dom_last_error(const k3dom_handle* h);
 

@@ -79,6 +79,16 @@ SIGNATURES = {
     "k3dom_count_bgki_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "k3dom_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "k3dom_last_error": (C.c_char_p, [C.c_void_p]),
+    "k3dom_window_create": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
+    "k3dom_window_destroy": (C.c_int, [C.c_void_p]),
+    "k3dom_window_set_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _fp, _fp, _fp, _fp, C.c_int32,
+                                         C.POINTER(C.c_uint8)]),
+    "k3dom_window_set_mask": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8)]),
+    "k3dom_window_forward": (C.c_int, [C.c_void_p, _fp, C.POINTER(C.c_double)]),
+    "k3dom_window_backward": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), _fp]),
+    "k3dom_window_read_record": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]),
+    "k3dom_window_times": (C.c_int, [C.c_void_p, _fp]),
+    "k3dom_window_last_error": (C.c_char_p, [C.c_void_p]),
 }
 
 _lib = None

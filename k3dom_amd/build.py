@@ -4,6 +4,7 @@ libk3dom.so    the product: gfx950 kernels + C ABI (hipcc --offload-arch=gfx950)
 libk3scene.so  the synthetic box-scene / LiDAR generator used by the benchmark
                and the tests (host C++)
 k3dom_replay   frame-replay driver over the C++ drop-in header include/k3dom/dom.h
+_k3window      libtorch autograd function over the differentiated window (links libk3dom.so)
 """
 import os
 import subprocess
@@ -30,7 +31,8 @@ def _run(cmd):
 
 def build_lib(verbose_asm=False):
     out = os.path.join(HERE, "libk3dom.so")
-    srcs = [os.path.join(CSRC, "k3_kernels.hip"), os.path.join(CSRC, "k3_bgki.hip"), os.path.join(CSRC, "k3_api.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("k3_kernels.hip", "k3_bgki.hip", "k3_adjoint.hip", "k3_api.cpp",
+                                             "k3_window.cpp")]
     cmd = ["hipcc", *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", *srcs]
     _run(cmd)
     os.replace(out + ".tmp", out)
@@ -55,13 +57,33 @@ def build_replay():
     return out
 
 
+def build_torch_ext():
+    import sysconfig
+    import torch
+    from torch.utils import cpp_extension as ce
+    out = os.path.join(HERE, "_k3window" + sysconfig.get_config_var("EXT_SUFFIX"))
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+           "-DTORCH_EXTENSION_NAME=_k3window", "-DTORCH_API_INCLUDE_EXTENSION_H",
+           *[f"-I{p}" for p in ce.include_paths()], f"-I{sysconfig.get_paths()['include']}",
+           os.path.join(CSRC, "k3_torch_window.cpp"),
+           *[f"-L{p}" for p in ce.library_paths()], "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python",
+           "-L", HERE, "-lk3dom", "-Wl,-rpath,$ORIGIN",
+           *[f"-Wl,-rpath,{p}" for p in ce.library_paths()], "-o", out + ".tmp"]
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
     build_scene()
     build_lib()
     build_replay()
+    build_torch_ext()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "lib": build_lib, "scene": build_scene, "replay": build_replay}[w]()
+        {"all": build_all, "lib": build_lib, "scene": build_scene, "replay": build_replay,
+         "torch": build_torch_ext}[w]()

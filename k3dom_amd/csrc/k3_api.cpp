@@ -1,8 +1,7 @@
 // C ABI of the K3DOM update on MI355X: handle lifetime, device buffers, the
 // per-frame launch sequence and readback.  Declarations and the reference
 // interface each entry point replaces are in include/k3dom.h.
-#include "../../include/k3dom.h"
-#include "k3_kernels.h"
+#include "k3_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -20,91 +19,16 @@
 using namespace k3;
 
 namespace {
-
 thread_local std::string g_create_error;
-
-enum { ST_PREDICT = 0, ST_SORT, ST_LISTS, ST_BGKI, ST_MASS, ST_PERSIST, ST_NEWBORN, ST_MOMENTS, ST_RESAMPLE,
-       ST_FRAME, ST_BGKI_KERNEL, ST_COUNT };
-// extra events: frame start, end of the gather kernel
-enum { EV_FRAME_START = ST_COUNT, EV_BGKI_END, EV_COUNT };
-
 }  // namespace
 
-struct k3dom_handle {
-    k3dom_params p{};
-    int gs = 0, gsz = 0, V = 0, N = 0, B = 0, key_bits = 1;
-    int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
-    int device = 0, num_cus = 256;
-    hipStream_t stream = nullptr;
 
-    std::vector<void*> allocs;
-    VoxelArrays vox[2]{};
-    int vcur = 0;
-    float *born = nullptr, *del_occ = nullptr, *del_free = nullptr;
-    ParticleArrays P[2]{};  // 0: current set, 1: sorted set
-    ParticleArrays NB{};
-    SortScratch sort{};
-    const uint32_t* last_perm = nullptr;
-    const uint32_t* last_skeys = nullptr;
-    uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;
-    uint32_t* bnd = nullptr;
-    int32_t* ridx = nullptr;
-    int32_t* heavy_voxels = nullptr;  // V entries: voxels queued for the wavefront paths
-    DevScalars* sc = nullptr;
-
-    int m_cap = 0;
-    float* d_pts = nullptr;
-    K3PointRec* rec = nullptr;
-    float4* pxyz = nullptr;
-    float* h_pts = nullptr;
-
-    TileGeom tg{};
-    LevelDesc ldesc[3]{};
-    LevelBufs lbuf[3]{};
-    ItemBufs ibuf{};
-    bool tiny_caps = false;  // test hook: start every capacity small to exercise the regrowth path
-    int overflow_retries = 0;
-    DevScalars* h_sc = nullptr;  // pinned mirror
-
-    uint32_t frame = 0;
-    bool first_pose = false;
-    float sensor[3] = {0, 0, 0}, center[3] = {0, 0, 0};
-    double updated_time = -1.0;
-    std::string err;
-
-    bool timing = false;
-    hipEvent_t ev[EV_COUNT]{};
-    k3dom_stats stats{};
-    GridGeom last_geom{};
-    unsigned long long* pair_counts = nullptr;
-    float stage_ms[ST_COUNT]{};
-};
-
-namespace {
+namespace k3api {
 
 k3dom_status fail(k3dom_handle* h, k3dom_status s, const std::string& msg) {
     if (h) h->err = msg;
     else g_create_error = msg;
     return s;
-}
-
-#define HIP_TRY(h, expr)                                                                          \
-    do {                                                                                          \
-        hipError_t _e = (expr);                                                                   \
-        if (_e != hipSuccess)                                                                     \
-            return fail((h), _e == hipErrorOutOfMemory ? K3DOM_ERR_OUT_OF_MEMORY : K3DOM_ERR_HIP, \
-                        std::string(#expr) + ": " + hipGetErrorString(_e));                       \
-    } while (0)
-
-template <class T>
-hipError_t dalloc(k3dom_handle* h, T** p, size_t count) {
-    void* q = nullptr;
-    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess) return e;
-    h->allocs.push_back(q);
-    *p = (T*)q;
-    return hipSuccess;
 }
 
 void dfree(k3dom_handle* h, void* p) {
@@ -212,7 +136,7 @@ k3dom_status alloc_heavy(k3dom_handle* h, long long heavy_cap) {
     I.partial = nullptr;
     I.heavy_cap = heavy_cap;
     HIP_TRY(h, dalloc(h, &I.items, (size_t)(h->ntiles + heavy_cap)));
-    HIP_TRY(h, dalloc(h, &I.partial, (size_t)std::max(1ll, heavy_cap) * 128));
+    HIP_TRY(h, dalloc(h, &I.partial, (size_t)std::max(1ll, heavy_cap) * I.partial_stride));
     return K3DOM_OK;
 }
 
@@ -252,24 +176,31 @@ GridGeom grid_geom(const k3dom_handle* h) {
     return g;
 }
 
-// Kernel inference for the current point set: writes del_occ / del_free for
-// every voxel, or sets the overflow flag when a capacity is exceeded.
-k3dom_status run_bgki(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int m) {
+k3dom_status build_lists(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int m) {
     const GridGeom g = grid_geom(h);
-    mark(h, ST_LISTS);
     HIP_TRY(h, launch_point_prep(h->stream, xs, ys, zs, m, h->p.resolution, h->rec, h->pxyz));
     for (int l = 0; l < 3; ++l)
         HIP_TRY(h, launch_build_level(h->stream, l, g, h->tg, h->ldesc[l], h->lbuf[l], l ? &h->lbuf[l - 1] : nullptr,
                                       h->p.ls, h->pxyz, m, h->sc));
     HIP_TRY(h, launch_build_items(h->stream, h->ntiles, h->lbuf[2], h->ibuf, h->sc));
+    h->last_geom = g;
+    return K3DOM_OK;
+}
+
+// Kernel inference for the current point set: writes del_occ / del_free for
+// every voxel, or sets the overflow flag when a capacity is exceeded.
+k3dom_status run_bgki(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int m) {
+    mark(h, ST_LISTS);
+    k3dom_status s = build_lists(h, xs, ys, zs, m);
+    if (s != K3DOM_OK) return s;
     mark(h, ST_BGKI);
     const int grid_blocks = h->num_cus * 8;
     mark(h, ST_BGKI_KERNEL);
-    HIP_TRY(h, launch_bgki(h->stream, grid_blocks, g, h->tg, h->p.ls, h->p.sigma, h->ibuf, h->lbuf[2], h->rec,
-                           h->del_occ, h->del_free, h->sc));
+    HIP_TRY(h, launch_bgki(h->stream, grid_blocks, h->last_geom, h->tg, h->p.ls, h->sigma_occ, h->sigma_free, h->ibuf,
+                           h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc));
     mark(h, EV_BGKI_END);
-    HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, g, h->tg, h->ibuf, h->del_occ, h->del_free, h->sc));
-    h->last_geom = g;
+    HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, h->last_geom, h->tg, h->ibuf, h->del_occ, h->del_free,
+                                   h->sc));
     return K3DOM_OK;
 }
 
@@ -396,6 +327,58 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
 }
 
 // ---------------------------------------------------------------- YAML params
+// Back to the state after construction: priors in every voxel, no pose, no
+// frame (the random counters restart with the frame count).
+k3dom_status reset_map(k3dom_handle* h) {
+    for (int b = 0; b < 2; ++b)
+        HIP_TRY(h, launch_fill_voxels(h->stream, h->vox[b], h->V, h->p.prior_free, h->p.prior_static,
+                                      h->p.prior_dynamic));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    h->vcur = 0;
+    h->frame = 0;
+    h->first_pose = false;
+    h->updated_time = -1.0;
+    for (int i = 0; i < 3; ++i) h->sensor[i] = h->center[i] = 0.0f;
+    return K3DOM_OK;
+}
+
+k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted[3]) {
+    shifted[0] = shifted[1] = shifted[2] = 0;
+    h->sensor[0] = x;
+    h->sensor[1] = y;
+    h->sensor[2] = z;
+    if (!h->first_pose) {
+        h->first_pose = true;
+        h->center[0] = h->sensor[0] - h->p.sensor_off_x;
+        h->center[1] = h->sensor[1] - h->p.sensor_off_y;
+        h->center[2] = h->sensor[2] - h->p.sensor_off_z;
+        return K3DOM_OK;
+    }
+    const int mx = static_cast<int>((h->sensor[0] - h->center[0] - h->p.sensor_off_x) / h->p.resolution);
+    const int my = static_cast<int>((h->sensor[1] - h->center[1] - h->p.sensor_off_y) / h->p.resolution);
+    const int mz = static_cast<int>((h->sensor[2] - h->center[2] - h->p.sensor_off_z) / h->p.resolution);
+    if (std::abs(mx) >= h->p.map_shift_thresh || std::abs(my) >= h->p.map_shift_thresh ||
+        std::abs(mz) >= h->p.map_shift_thresh) {
+        const int nxt = 1 - h->vcur;
+        HIP_TRY(h, launch_shift_voxels(h->stream, h->vox[nxt], h->vox[h->vcur], h->gs, h->gsz, mx, my, mz,
+                                       h->p.prior_free, h->p.prior_static, h->p.prior_dynamic));
+        HIP_TRY(h, launch_shift_particles(h->stream, h->P[0], h->N, mx, my, mz));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        h->vcur = nxt;
+        h->center[0] += mx * h->p.resolution;
+        h->center[1] += my * h->p.resolution;
+        h->center[2] += mz * h->p.resolution;
+        shifted[0] = mx;
+        shifted[1] = my;
+        shifted[2] = mz;
+    }
+    return K3DOM_OK;
+}
+
+}  // namespace k3api
+
+namespace {
+
 std::string trim(const std::string& s) {
     size_t a = 0, b = s.size();
     while (a < b && std::isspace((unsigned char)s[a])) ++a;
@@ -438,6 +421,8 @@ void get_float(const std::map<std::string, std::string>& kv, const char* key, fl
 }  // namespace
 
 // ============================================================== C ABI
+using namespace k3api;
+
 extern "C" {
 
 k3dom_status k3dom_params_default(k3dom_params* out) {
@@ -543,6 +528,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
         }                                                                                          \
     } while (0)
     h->p = *params;
+    h->sigma_occ = h->sigma_free = h->p.sigma;
     h->gs = (int)(h->p.size / h->p.resolution);
     h->gsz = (int)(h->p.size_z / h->p.resolution);
     h->V = h->gs * h->gs * h->gsz;
@@ -656,32 +642,8 @@ k3dom_status k3dom_destroy(k3dom_handle* h) {
 // src/common/dom.cu:111-158
 k3dom_status k3dom_update_pose(k3dom_handle* h, float x, float y, float z) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
-    h->sensor[0] = x;
-    h->sensor[1] = y;
-    h->sensor[2] = z;
-    if (!h->first_pose) {
-        h->first_pose = true;
-        h->center[0] = h->sensor[0] - h->p.sensor_off_x;
-        h->center[1] = h->sensor[1] - h->p.sensor_off_y;
-        h->center[2] = h->sensor[2] - h->p.sensor_off_z;
-        return K3DOM_OK;
-    }
-    const int mx = static_cast<int>((h->sensor[0] - h->center[0] - h->p.sensor_off_x) / h->p.resolution);
-    const int my = static_cast<int>((h->sensor[1] - h->center[1] - h->p.sensor_off_y) / h->p.resolution);
-    const int mz = static_cast<int>((h->sensor[2] - h->center[2] - h->p.sensor_off_z) / h->p.resolution);
-    if (std::abs(mx) >= h->p.map_shift_thresh || std::abs(my) >= h->p.map_shift_thresh ||
-        std::abs(mz) >= h->p.map_shift_thresh) {
-        const int nxt = 1 - h->vcur;
-        HIP_TRY(h, launch_shift_voxels(h->stream, h->vox[nxt], h->vox[h->vcur], h->gs, h->gsz, mx, my, mz,
-                                       h->p.prior_free, h->p.prior_static, h->p.prior_dynamic));
-        HIP_TRY(h, launch_shift_particles(h->stream, h->P[0], h->N, mx, my, mz));
-        HIP_TRY(h, hipStreamSynchronize(h->stream));
-        h->vcur = nxt;
-        h->center[0] += mx * h->p.resolution;
-        h->center[1] += my * h->p.resolution;
-        h->center[2] += mz * h->p.resolution;
-    }
-    return K3DOM_OK;
+    int shifted[3];
+    return update_pose(h, x, y, z, shifted);
 }
 
 k3dom_status k3dom_update_grid(k3dom_handle* h, double t, const float* xs, const float* ys, const float* zs,

@@ -441,11 +441,32 @@ hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& T, co
 // plus F_OCC when a voxel of the tile may lie within ls of the point.
 constexpr uint32_t C_GENERAL = 0, C_MAIN = 1, C_BEHIND = 2, C_NOFREE = 3, F_OCC = 4;
 
+// Kernel term and, when GRAD, its derivative in the length scale.
+template <bool GRAD>
+struct BgTerm {
+    float v, g;
+};
+template <bool GRAD>
+__device__ __forceinline__ BgTerm<GRAD> bg_term(float d, float kscale, float sigma, float ls, float inv_ls) {
+    BgTerm<GRAD> t;
+    t.v = k3_kernel_value(d, kscale, sigma);
+    t.g = 0.0f;
+    if (GRAD) t.g = t.v > 0.0f ? k3_kernel_dls(d, ls, inv_ls, sigma) : 0.0f;
+    return t;
+}
+
+// Partial sums of a split item: occupied, free (and, when GRAD, their
+// length-scale derivatives), 64 lanes each.
+template <bool GRAD>
+constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
+
+template <bool GRAD>
 __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls, float ls2g, float kscale,
-                                              float sigma, const uint4* __restrict__ items,
-                                              const uint32_t* __restrict__ tlist,
+                                              float inv_ls, float sigma_o, float sigma_f,
+                                              const uint4* __restrict__ items, const uint32_t* __restrict__ tlist,
                                               const K3PointRec* __restrict__ rec, float* __restrict__ del_occ,
-                                              float* __restrict__ del_free, float* __restrict__ partial,
+                                              float* __restrict__ del_free, float* __restrict__ dls_occ,
+                                              float* __restrict__ dls_free, float* __restrict__ partial,
                                               DevScalars* sc) {
     __shared__ float4 srec[BG_WAVES][64 * 3];
     __shared__ uint32_t sidx[BG_WAVES][64];
@@ -470,7 +491,7 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
         const float dc2 = dc * dc;
         // free term of a voxel for every point in front of which it lies
         // (k3_kernel_value is an exact zero for distances >= ls)
-        const float kc = k3_kernel_value(dc, kscale, sigma);
+        const BgTerm<GRAD> kc = bg_term<GRAD>(dc, kscale, sigma_f, ls, inv_ls);
         // bounding box of the tile's 64 voxel centres (lanes outside the grid included)
         const float tcx = 0.5f * (axis_center(tx * 4, g.gs, g.res, g.sx) + axis_center(tx * 4 + 3, g.gs, g.res, g.sx));
         const float tcy = 0.5f * (axis_center(ty * 4, g.gs, g.res, g.sy) + axis_center(ty * 4 + 3, g.gs, g.res, g.sy));
@@ -481,6 +502,7 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
         const float rr = trad + marg;
         const float occ_r = ls + rr;
         float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
+        float gsum_o = 0.0f, gsum_f = 0.0f;  // length-scale derivatives (GRAD)
         uint32_t cur_chunk = 0xffffffffu;
         for (uint32_t b = ib; b < ie; b += 64) {
             const uint32_t e = b + lane;
@@ -519,16 +541,25 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
                     const float ox = vx - r0.x, oy = vy - r0.y, oz = vz - r0.z;
                     const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
                     const bool co = so < ls2g;
-                    if (__any(co)) cur_o += k3_kernel_value(bg_sqrt(so), kscale, sigma);
+                    if (__any(co)) {
+                        const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(so), kscale, sigma_o, ls, inv_ls);
+                        cur_o += t.v;
+                        if (GRAD) gsum_o += t.g;
+                    }
                 }
                 const uint32_t fc = cls & 3u;
                 if (fc == C_MAIN) {  // 0 < dp < mg for every lane
                     const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
                     const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
                     const bool cf = a2 < ls2g;
-                    if (__any(cf)) cur_f += k3_kernel_value(bg_sqrt(a2), kscale, sigma);
+                    if (__any(cf)) {
+                        const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(a2), kscale, sigma_f, ls, inv_ls);
+                        cur_f += t.v;
+                        if (GRAD) gsum_f += t.g;
+                    }
                 } else if (fc == C_BEHIND) {  // dp <= 0 for every lane
-                    cur_f += kc;
+                    cur_f += kc.v;
+                    if (GRAD) gsum_f += kc.g;
                 } else if (fc == C_GENERAL) {  // k3_free_distance, the three cases as selects
                     const float4 r1 = srec[w][q * 3 + 1];
                     const float4 r2 = srec[w][q * 3 + 2];
@@ -542,10 +573,12 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
                     const float a = c2 ? a2 : a3;
                     const bool cs = (c2 || c3) && a < ls2g;
                     if (__any(cs)) {
-                        const float kv = k3_kernel_value(bg_sqrt(a), kscale, sigma);
-                        cur_f += cs ? kv : (c1 ? kc : 0.0f);
+                        const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(a), kscale, sigma_f, ls, inv_ls);
+                        cur_f += cs ? t.v : (c1 ? kc.v : 0.0f);
+                        if (GRAD) gsum_f += cs ? t.g : (c1 ? kc.g : 0.0f);
                     } else {
-                        cur_f += c1 ? kc : 0.0f;
+                        cur_f += c1 ? kc.v : 0.0f;
+                        if (GRAD) gsum_f += c1 ? kc.g : 0.0f;
                     }
                 }
             }
@@ -558,26 +591,46 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
                 const int cell = i + g.gs * j + g.gs * g.gs * k;
                 del_occ[cell] = tot_o;
                 del_free[cell] = tot_f;
+                if (GRAD) {
+                    dls_occ[cell] = gsum_o;
+                    dls_free[cell] = gsum_f;
+                }
             }
         } else {
-            partial[(size_t)slot * 128 + lane] = tot_o;
-            partial[(size_t)slot * 128 + 64 + lane] = tot_f;
+            float* pp = partial + (size_t)slot * bg_partial_stride<GRAD>();
+            pp[lane] = tot_o;
+            pp[64 + lane] = tot_f;
+            if (GRAD) {
+                pp[128 + lane] = gsum_o;
+                pp[192 + lane] = gsum_f;
+            }
         }
     }
 }
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
-                       float sigma, const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec, float* del_occ,
-                       float* del_free, DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls), k3_kernel_scale(ls),
-                       sigma,
-                       I.items, T.list, rec, del_occ, del_free, I.partial, sc);
+                       float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec,
+                       float* del_occ, float* del_free, DevScalars* sc) {
+    hipLaunchKernelGGL(k_bgki<false>, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls),
+                       k3_kernel_scale(ls), 1.0f / ls, sigma_o, sigma_f, I.items, T.list, rec, del_occ, del_free,
+                       (float*)nullptr, (float*)nullptr, I.partial, sc);
+    return hipGetLastError();
+}
+hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
+                            float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T,
+                            const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
+                            DevScalars* sc) {
+    hipLaunchKernelGGL(k_bgki<true>, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls),
+                       k3_kernel_scale(ls), 1.0f / ls, sigma_o, sigma_f, I.items, T.list, rec, del_occ, del_free,
+                       dls_occ, dls_free, I.partial, sc);
     return hipGetLastError();
 }
 
+template <bool GRAD>
 __global__ __launch_bounds__(TPB) void k_bgki_combine(int ntiles, GridGeom g, TileGeom tg,
                                                       const uint32_t* __restrict__ tile_partial,
                                                       const float* __restrict__ partial, float* del_occ,
-                                                      float* del_free, const DevScalars* sc) {
+                                                      float* del_free, float* dls_occ, float* dls_free,
+                                                      const DevScalars* sc) {
     if (sc->overflow) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tile = blockIdx.x * BG_WAVES + w;
@@ -587,19 +640,35 @@ __global__ __launch_bounds__(TPB) void k_bgki_combine(int ntiles, GridGeom g, Ti
     const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
     const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
     if (!(i < g.gs && j < g.gs && k < g.gsz)) return;
-    float ro = 0.0f, rf = 0.0f;
+    float ro = 0.0f, rf = 0.0f, go = 0.0f, gf = 0.0f;
     for (uint32_t c = 0; c < cnt; ++c) {
-        ro += partial[(size_t)(first + c) * 128 + lane];
-        rf += partial[(size_t)(first + c) * 128 + 64 + lane];
+        const float* pp = partial + (size_t)(first + c) * bg_partial_stride<GRAD>();
+        ro += pp[lane];
+        rf += pp[64 + lane];
+        if (GRAD) {
+            go += pp[128 + lane];
+            gf += pp[192 + lane];
+        }
     }
     const int cell = i + g.gs * j + g.gs * g.gs * k;
     del_occ[cell] = ro;
     del_free[cell] = rf;
+    if (GRAD) {
+        dls_occ[cell] = go;
+        dls_free[cell] = gf;
+    }
 }
 hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg, const ItemBufs& I,
                                float* del_occ, float* del_free, DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki_combine, dim3(blocks_for(ntiles, BG_WAVES)), dim3(TPB), 0, st, ntiles, g, tg,
-                       I.tile_partial, I.partial, del_occ, del_free, sc);
+    hipLaunchKernelGGL(k_bgki_combine<false>, dim3(blocks_for(ntiles, BG_WAVES)), dim3(TPB), 0, st, ntiles, g, tg,
+                       I.tile_partial, I.partial, del_occ, del_free, (float*)nullptr, (float*)nullptr, sc);
+    return hipGetLastError();
+}
+hipError_t launch_bgki_combine_grad(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg,
+                                    const ItemBufs& I, float* del_occ, float* del_free, float* dls_occ,
+                                    float* dls_free, DevScalars* sc) {
+    hipLaunchKernelGGL(k_bgki_combine<true>, dim3(blocks_for(ntiles, BG_WAVES)), dim3(TPB), 0, st, ntiles, g, tg,
+                       I.tile_partial, I.partial, del_occ, del_free, dls_occ, dls_free, sc);
     return hipGetLastError();
 }
 

@@ -1,0 +1,109 @@
+// Internal state of a k3dom_handle and the frame-level helpers shared by the
+// C ABI (k3_api.cpp) and the differentiated-window driver (k3_window.cpp).
+#pragma once
+
+#include "../../include/k3dom.h"
+#include "k3_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+using namespace k3;
+
+enum { ST_PREDICT = 0, ST_SORT, ST_LISTS, ST_BGKI, ST_MASS, ST_PERSIST, ST_NEWBORN, ST_MOMENTS, ST_RESAMPLE,
+       ST_FRAME, ST_BGKI_KERNEL, ST_COUNT };
+// extra events: frame start, end of the gather kernel
+enum { EV_FRAME_START = ST_COUNT, EV_BGKI_END, EV_COUNT };
+
+struct k3dom_handle {
+    k3dom_params p{};
+    // evidence weights of the occupied and free terms (both p.sigma unless a
+    // differentiated window sets them apart)
+    float sigma_occ = 0.0f, sigma_free = 0.0f;
+    int gs = 0, gsz = 0, V = 0, N = 0, B = 0, key_bits = 1;
+    int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
+    int device = 0, num_cus = 256;
+    hipStream_t stream = nullptr;
+
+    std::vector<void*> allocs;
+    VoxelArrays vox[2]{};
+    int vcur = 0;
+    float *born = nullptr, *del_occ = nullptr, *del_free = nullptr;
+    ParticleArrays P[2]{};  // 0: current set, 1: sorted set
+    ParticleArrays NB{};
+    SortScratch sort{};
+    const uint32_t* last_perm = nullptr;
+    const uint32_t* last_skeys = nullptr;
+    uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;
+    uint32_t* bnd = nullptr;
+    int32_t* ridx = nullptr;
+    int32_t* heavy_voxels = nullptr;  // V entries: voxels queued for the wavefront paths
+    DevScalars* sc = nullptr;
+
+    int m_cap = 0;
+    float* d_pts = nullptr;
+    K3PointRec* rec = nullptr;
+    float4* pxyz = nullptr;
+    float* h_pts = nullptr;
+
+    TileGeom tg{};
+    LevelDesc ldesc[3]{};
+    LevelBufs lbuf[3]{};
+    ItemBufs ibuf{};
+    bool tiny_caps = false;  // test hook: start every capacity small to exercise the regrowth path
+    int overflow_retries = 0;
+    DevScalars* h_sc = nullptr;  // pinned mirror
+
+    uint32_t frame = 0;
+    bool first_pose = false;
+    float sensor[3] = {0, 0, 0}, center[3] = {0, 0, 0};
+    double updated_time = -1.0;
+    std::string err;
+
+    bool timing = false;
+    hipEvent_t ev[EV_COUNT]{};
+    k3dom_stats stats{};
+    GridGeom last_geom{};
+    unsigned long long* pair_counts = nullptr;
+    float stage_ms[ST_COUNT]{};
+};
+
+namespace k3api {
+
+k3dom_status fail(k3dom_handle* h, k3dom_status s, const std::string& msg);
+
+#define HIP_TRY(h, expr)                                                                          \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            return k3api::fail((h), _e == hipErrorOutOfMemory ? K3DOM_ERR_OUT_OF_MEMORY : K3DOM_ERR_HIP, \
+                               std::string(#expr) + ": " + hipGetErrorString(_e));                \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(k3dom_handle* h, T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = sizeof(T) * (count ? count : 1);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e == hipSuccess) {
+        h->allocs.push_back(q);
+        *p = static_cast<T*>(q);
+    }
+    return e;
+}
+void dfree(k3dom_handle* h, void* p);
+k3dom_status ensure_points(k3dom_handle* h, int m);
+GridGeom grid_geom(const k3dom_handle* h);
+// One filter frame on device-resident points (DOM::updateGrid).
+k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m);
+// Candidate lists of the current points and geometry (point preparation and
+// the three list levels); the gather itself is launched by the caller.
+k3dom_status build_lists(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int m);
+// State after construction (priors, no pose, frame count 0).
+k3dom_status reset_map(k3dom_handle* h);
+// DOM_c::updatePose; *shifted receives the map shift in voxels (0 when none).
+k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted[3]);
+
+}  // namespace k3api

@@ -111,16 +111,28 @@ struct ItemBufs {
     uint32_t* bsum;
     uint4* items;             // ntiles + heavy_cap
     uint32_t* tile_partial;   // 2 * ntiles
-    float* partial;           // heavy_cap * 128
+    float* partial;           // heavy_cap * partial_stride
+    int partial_stride = 128; // floats per split item (256 with length-scale derivatives)
     long long heavy_cap;
 };
 hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& tiles, const ItemBufs& I,
                               DevScalars* sc);
+// sigma_o / sigma_f weight the occupied / free terms (both the reference's
+// single sigma in a plain frame).  Split items need I.partial with 128 floats
+// per item (256 for the _grad variants).
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
-                       float sigma, const ItemBufs& I, const LevelBufs& tiles, const K3PointRec* rec,
+                       float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles, const K3PointRec* rec,
                        float* del_occ, float* del_free, DevScalars* sc);
 hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg, const ItemBufs& I,
                                float* del_occ, float* del_free, DevScalars* sc);
+// The same evidence plus its derivative in the length scale (dls_*).
+hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
+                            float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles,
+                            const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
+                            DevScalars* sc);
+hipError_t launch_bgki_combine_grad(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg,
+                                    const ItemBufs& I, float* del_occ, float* del_free, float* dls_occ,
+                                    float* dls_free, DevScalars* sc);
 hipError_t launch_bgki_count(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
                              const ItemBufs& I, uint32_t n_items, const LevelBufs& tiles, const K3PointRec* rec,
                              unsigned long long* counts);
@@ -149,6 +161,40 @@ hipError_t launch_init_particles(hipStream_t st, const uint32_t* abort_flag, con
 hipError_t launch_resample(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, int total_len,
                            const uint64_t* total, const int32_t* E, ParticleArrays sorted, ParticleArrays nb,
                            ParticleArrays out, int N, uint32_t frame, uint32_t k0, uint32_t k1, int32_t* idx_out);
+// ------------------------------------------------------------------ differentiated window (k3_adjoint.hip)
+// Record of one forward frame: masses after the frame, predicted dynamic mass,
+// and the frozen counts per voxel (particles inside the grid, of them not
+// penalised; newborns, of them not penalised).
+struct WindowFrame {
+    float *f, *s, *d, *dyn;
+    int32_t *n, *m, *nb, *mb;
+};
+struct AdjointFrameArgs {
+    int V, gs, gsz;
+    WindowFrame post;          // this frame
+    WindowFrame prev;          // previous frame (prev.f == nullptr: the pre-state is the priors)
+    int move[3];               // map shift applied before this frame
+    int move_next[3];          // map shift applied before the next frame
+    const double *lam_next_f, *lam_next_s, *lam_next_d;  // adjoint of the next frame's pre-state (or null)
+    double *lam_f, *lam_s, *lam_d;                        // out: adjoint of this frame's pre-state
+    const float *del_occ, *del_free, *dls_occ, *dls_free;
+    const uint8_t* gt;
+    const uint8_t* mask;       // null: every voxel
+    double loss_scale;         // d loss / d (per-voxel BCE term)
+    float prior_f, prior_s, prior_d;
+    float decay, mass_scale, p_B, sigma_occ, sigma_free;
+    float w_coef;              // p_S / N
+    int has_w, has_w_prev;     // W of this / of the previous frame depends on the parameters
+};
+int window_blocks(int V);
+hipError_t launch_window_record(hipStream_t st, int V, int gs, int gsz, bool normal, float min_vel,
+                                const VoxelArrays& v, const ParticleArrays& Ps, const ParticleArrays& NB,
+                                const uint32_t* bnd, const WindowFrame& rec, const uint8_t* gt, const uint8_t* mask,
+                                double* loss_partial);
+// acc[0]: lambda_W (in: this frame's, out: the previous frame's); acc[1..6] += gradients
+hipError_t launch_window_adjoint(hipStream_t st, const AdjointFrameArgs& a, double* acc, double* partial);
+hipError_t launch_window_clear(hipStream_t st, double* acc, int n);
+
 hipError_t launch_reset_scalars(hipStream_t st, DevScalars* sc);
 hipError_t launch_clear_overflow(hipStream_t st, DevScalars* sc);
 

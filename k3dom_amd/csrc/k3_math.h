@@ -245,6 +245,22 @@ K3_HD float k3_kernel_value(float d, float scale, float sigma) {
     return fmaxf(0.0f, sigma * p);
 }
 
+// Derivative in ls of sigma * kbar(d / ls), kbar the bracket of the
+// reference's kernel (src/k3dom/kernel/mass_update.cu:24-26) differentiated in
+// closed form, q = d / ls < 1:
+//   kbar'(q) = -(2 pi / 3) sin(2 pi q) (1 - q) - (2 + cos(2 pi q)) / 3 + cos(2 pi q) pi / 3.141592
+//   d/dls [sigma kbar(d / ls)] = sigma kbar'(q) (-q / ls)
+// Used only by the differentiated window (gradients, not parity values).
+K3_HD float k3_kernel_dls(float d, float ls, float inv_ls, float sigma) {
+    (void)ls;
+    const float q = d * inv_ls;
+    float sn, cs;
+    k3_sincospi(2.0f * q, &sn, &cs);
+    const float two_pi = 6.28318530717958647692f;
+    const float dk = -(two_pi / 3.0f) * sn * (1.0f - q) - (2.0f + cs) / 3.0f + cs * (3.14159265358979323846f / 3.141592f);
+    return sigma * dk * (-q * inv_ls);
+}
+
 // distance_with_margin, src/k3dom/kernel/mass_update.cu:29-31
 K3_HD float k3_margin(float distance, float resolution) {
     return fmaxf(0.9f * distance, distance - 2.0f * resolution);

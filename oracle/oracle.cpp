@@ -39,6 +39,7 @@
 
 #include "../include/k3dom.h"
 #include "../k3dom_amd/csrc/k3_math.h"
+#include "../k3dom_amd/csrc/k3_dual.h"
 
 #include <algorithm>
 #include <chrono>
@@ -257,6 +258,9 @@ struct Oracle {
     // diagnostics of the last frame
     std::vector<int32_t> perm, sorted_key, resample_idx;
     std::vector<float> sorted_w, del_occ, del_free;
+    // frozen counts of the last frame per voxel (differentiated window): sorted
+    // particles inside the grid, of them not penalised; newborns, of them not penalised
+    std::vector<int32_t> cnt_n, cnt_m, cnt_nb, cnt_mb;
     BgkiStats bgki_stats;
     double stage_ms[10];
     uint32_t frame = 0;
@@ -565,6 +569,24 @@ struct Oracle {
         w.swap(nw); key.swap(nk);
     }
 
+    // inside test of k3_predict_particle, penalty test of resample()
+    void record_counts() {
+        const float min2 = p.particle_min_vel * p.particle_min_vel;
+        cnt_n.assign(V, 0); cnt_m.assign(V, 0); cnt_nb.assign(V, 0); cnt_mb.assign(V, 0);
+        for (int i = 0; i < N; ++i) {
+            const float x = px[i], y = py[i], z = pz[i];
+            const bool inb = !((x >= (float)gs || x < 0.0f) || (y >= (float)gs || y < 0.0f) ||
+                               (z >= (float)gsz || z < 0.0f));
+            const bool pen = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i] < min2;
+            cnt_n[key[i]] += inb;
+            cnt_m[key[i]] += inb && !pen;
+        }
+        for (int k = 0; k < B; ++k) {
+            cnt_nb[bkey[k]] += 1;
+            cnt_mb[bkey[k]] += !(bvx[k] * bvx[k] + bvy[k] * bvy[k] + bvz[k] * bvz[k] < min2);
+        }
+    }
+
     // DOM::updateGrid, src/k3dom/dom.cu:29-50
     void update_grid(double t, const float* xs, const float* ys, const float* zs, int n) {
         for (double& x : stage_ms) x = 0.0;
@@ -588,11 +610,13 @@ struct Oracle {
             t0 = clk::now();
             moments();
             stage_ms[7] = ms_since(t0);
+            record_counts();
             t0 = clk::now();
             resample();
             stage_ms[8] = ms_since(t0);
         } else {
             initialize_particles(xs, ys, zs, n);
+            cnt_n.assign(V, 0); cnt_m.assign(V, 0); cnt_nb.assign(V, 0); cnt_mb.assign(V, 0);
         }
         stage_ms[9] = ms_since(tf);
         updated_time = t;
@@ -634,6 +658,144 @@ static void bgki64_voxel(double x, double y, double z, const float* xs, const fl
     }
     *o = occ;
     *fr = fre;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiated window in double precision: the continuous chain of a window
+// (masses per voxel and the joint weight W) with the frozen integer decisions
+// of a forward run (map shifts, counts), evaluated with X = double for the
+// loss and central differences, and with X = K3Dual<double, 6> for the
+// gradient by forward-mode differentiation.  theta = (ls, sigma_occ,
+// sigma_free, prior_free, prior_static, prior_dynamic).
+struct WindowInput {
+    const k3dom_params* p;
+    int T, gs, gsz;
+    const int32_t* npts;      // T
+    const float* pts;         // per frame x[n], y[n], z[n], frames back to back
+    const float* sensor_rel;  // 3 T: sensor minus map centre (float, as the frame uses)
+    const int32_t* move;      // 3 T: map shift before each frame
+    const float* decay;       // T
+    const int32_t* counts;    // T * 4 * V: n, m, nb, mb
+    const uint8_t* gt;        // T * V
+    const uint8_t* mask;      // V or null
+};
+
+template <class X>
+static X kern_x(double dd, const X& ls, const X& sigma) {
+    const double PI = 3.14159265358979323846;
+    const X q = X(k3d_cast<X>(dd)) / ls;
+    const X k = (2.0 + k3g_cos(2.0 * PI * q)) * (1.0 - q) / 3.0 + k3g_sin(2.0 * PI * q) / (2.0 * (double)3.141592f);
+    const X v = sigma * k;
+    return k3g_val(v) > 0.0 ? v : v * 0.0;
+}
+
+template <class X>
+static void bgki_x(const WindowInput& in, int t, const X& ls, const X& so, const X& sf, std::vector<X>& dO,
+                   std::vector<X>& dF) {
+    const int gs = in.gs, gsz = in.gsz;
+    Geom g{gs, gsz, in.p->resolution, in.sensor_rel[3 * t], in.sensor_rel[3 * t + 1], in.sensor_rel[3 * t + 2]};
+    g.build();
+    const double lsv = k3g_val(ls);
+    const double R = lsv * (1.0 + 1e-5) + 1e-4;
+    size_t off = 0;
+    for (int f = 0; f < t; ++f) off += 3 * (size_t)in.npts[f];
+    const int n = in.npts[t];
+    const float *xs = in.pts + off, *ys = xs + n, *zs = ys + n;
+    const double res = in.p->resolution;
+    for (int pi = 0; pi < n; ++pi) {
+        const K3PointRec pr = k3_point_prep(xs[pi], ys[pi], zs[pi], in.p->resolution);
+        if (pr.valid == 0.0f) continue;
+        const double xm = xs[pi], ym = ys[pi], zm = zs[pi];
+        const double dm = std::sqrt(xm * xm + ym * ym + zm * zm);
+        const double mg = std::max(0.9 * dm, dm - 2.0 * res);
+        int i0, i1;
+        if (!axis_range(std::min(0.0, xm) - R, std::max(0.0, xm) + R, g.res, g.sx, gs, &i0, &i1)) continue;
+        for (int i = i0; i <= i1; ++i) {
+            double ta = 0.0, tb = 1.0;
+            if (!narrow(xm, g.cx[i], R, &ta, &tb)) continue;
+            int j0, j1;
+            if (!axis_range(std::min(ta * ym, tb * ym) - R, std::max(ta * ym, tb * ym) + R, g.res, g.sy, gs, &j0, &j1))
+                continue;
+            for (int j = j0; j <= j1; ++j) {
+                double tc = ta, td = tb;
+                if (!narrow(ym, g.cy[j], R, &tc, &td)) continue;
+                int k0, k1;
+                if (!axis_range(std::min(tc * zm, td * zm) - R, std::max(tc * zm, td * zm) + R, g.res, g.sz, gsz,
+                                &k0, &k1))
+                    continue;
+                for (int k = k0; k <= k1; ++k) {
+                    const double x = g.cx[i], y = g.cy[j], z = g.cz[k];
+                    const size_t v = (size_t)i + (size_t)gs * j + (size_t)gs * gs * k;
+                    const double dd = std::sqrt((x - xm) * (x - xm) + (y - ym) * (y - ym) + (z - zm) * (z - zm));
+                    if (dd < lsv) dO[v] = dO[v] + kern_x(dd, ls, so);
+                    if (!(dm > 0)) continue;
+                    const double dc = std::sqrt(x * x + y * y + z * z);
+                    const double dp = (x * xm + y * ym + z * zm) / dm;
+                    double df;
+                    if (dp <= 0) df = dc;
+                    else if (dp < mg) df = std::sqrt(std::max(0.0, dc * dc - dp * dp));
+                    else if (dp < dm) {
+                        const double r = mg / dm;
+                        df = std::sqrt((x - r * xm) * (x - r * xm) + (y - r * ym) * (y - r * ym) +
+                                       (z - r * zm) * (z - r * zm));
+                    } else continue;
+                    if (df < lsv) dF[v] = dF[v] + kern_x(df, ls, sf);
+                }
+            }
+        }
+    }
+}
+
+template <class X>
+static X window_chain(const WindowInput& in, const X th[6]) {
+    const int gs = in.gs, gsz = in.gsz, V = gs * gs * gsz;
+    const k3dom_params& p = *in.p;
+    std::vector<X> f(V, th[3]), s(V, th[4]), d(V, th[5]);
+    std::vector<X> dO(V), dF(V);
+    X W = k3d_cast<X>(1.0);
+    X loss = k3d_cast<X>(0.0);
+    long long cnt_mask = 0;
+    for (int v = 0; v < V; ++v) cnt_mask += (!in.mask || in.mask[v]) ? 1 : 0;
+    for (int t = 0; t < in.T; ++t) {
+        const int* mv = in.move + 3 * t;
+        if (t > 0 && (mv[0] || mv[1] || mv[2])) {
+            std::vector<X> nf(V), ns(V), nd(V);
+            for (int c = 0; c < V; ++c) {
+                const int x = c % gs, y = (c / gs) % gs, z = c / (gs * gs);
+                const int sx = x + mv[0], sy = y + mv[1], sz = z + mv[2];
+                if (sx >= 0 && sx < gs && sy >= 0 && sy < gs && sz >= 0 && sz < gsz) {
+                    const int src = sx + gs * sy + gs * gs * sz;
+                    nf[c] = f[src]; ns[c] = s[src]; nd[c] = d[src];
+                } else {
+                    nf[c] = th[3]; ns[c] = th[4]; nd[c] = th[5];
+                }
+            }
+            f.swap(nf); s.swap(ns); d.swap(nd);
+        }
+        for (int v = 0; v < V; ++v) {
+            dO[v] = k3d_cast<X>(0.0);
+            dF[v] = k3d_cast<X>(0.0);
+        }
+        bgki_x(in, t, th[0], th[1], th[2], dO, dF);
+        const int32_t* cn = in.counts + (size_t)t * 4 * V;
+        X Wn = k3d_cast<X>(0.0);
+        const double decay = t == 0 ? 1.0 : (double)in.decay[t];
+        for (int v = 0; v < V; ++v) {
+            const int n = cn[v], m = cn[V + v], nb = cn[2 * V + v], mb = cn[3 * V + v];
+            const X dyn = t == 0 ? k3d_cast<X>(0.0) : W * ((double)n * (double)p.persistence_prob / (double)p.particle_count);
+            X fo, so, dout, born;
+            k3g_mass_step<X, double>(f[v], s[v], d[v], dyn, dO[v], dF[v], decay, (double)p.mass_scale,
+                                     (double)p.birth_prob, &fo, &so, &dout, &born);
+            f[v] = fo; s[v] = so; d[v] = dout;
+            if (t >= 1) {
+                if (n > 0) Wn = Wn + (dout - born) * ((double)m / (double)n);
+                if (nb > 0) Wn = Wn + born * ((double)mb / (double)nb);
+            }
+            if (!in.mask || in.mask[v]) loss = loss + k3g_bce_occ<X, double>(fo, so, dout, in.gt[(size_t)t * V + v]);
+        }
+        if (t >= 1) W = Wn;
+    }
+    return loss / ((double)in.T * (double)cnt_mask);
 }
 
 }  // namespace
@@ -738,6 +900,42 @@ float orc_ln(float x) { return k3_ln(x); }
 float orc_exp(float x) { return k3_exp(x); }
 void orc_sincospi(float t, float* s, float* c) { k3_sincospi(t, s, c); }
 float orc_seg_sum(const float* x, int32_t st, int32_t en) { return seg_sum(x, st, en); }
+
+// Frozen counts of the last frame (4 * V int32: n, m, nb, mb).
+int64_t orc_read_counts(void* hh, int32_t* dst, int64_t capacity) {
+    Oracle* h = (Oracle*)hh;
+    const int64_t bytes = 16 * (int64_t)h->V;
+    if (capacity < bytes || (int64_t)h->cnt_n.size() != h->V) return -1;
+    std::memcpy(dst, h->cnt_n.data(), 4 * (size_t)h->V);
+    std::memcpy(dst + h->V, h->cnt_m.data(), 4 * (size_t)h->V);
+    std::memcpy(dst + 2 * h->V, h->cnt_nb.data(), 4 * (size_t)h->V);
+    std::memcpy(dst + 3 * h->V, h->cnt_mb.data(), 4 * (size_t)h->V);
+    return bytes;
+}
+
+// Loss of a window in double (grad == NULL), or loss and its gradient by
+// forward-mode differentiation (grad[6]).
+int orc_window_chain64(const k3dom_params* p, int32_t T, const int32_t* npts, const float* pts,
+                       const float* sensor_rel, const int32_t* move, const float* decay, const int32_t* counts,
+                       const uint8_t* gt, const uint8_t* mask, const double* theta, double* loss, double* grad) {
+    if (!p || T <= 0 || !npts || !pts || !sensor_rel || !move || !decay || !counts || !gt || !theta || !loss)
+        return K3DOM_ERR_INVALID_ARG;
+    WindowInput in{p, T, (int)(p->size / p->resolution), (int)(p->size_z / p->resolution), npts, pts, sensor_rel,
+                   move, decay, counts, gt, mask};
+    if (!grad) {
+        double th[6];
+        for (int k = 0; k < 6; ++k) th[k] = theta[k];
+        *loss = window_chain<double>(in, th);
+        return K3DOM_OK;
+    }
+    typedef K3Dual<double, 6> D;
+    D th[6];
+    for (int k = 0; k < 6; ++k) th[k] = k3d_var<double, 6>(theta[k], k);
+    const D L = window_chain<D>(in, th);
+    *loss = L.v;
+    for (int k = 0; k < 6; ++k) grad[k] = L.d[k];
+    return K3DOM_OK;
+}
 float orc_kernel(float d, float ls, float sigma) { return k3_kernel_value(d, k3_kernel_scale(ls), sigma); }
 float orc_credit(float m, float scale) { return k3_credit(m, scale); }
 void orc_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) { k3_box_muller(a, b, n0, n1); }

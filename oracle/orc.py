@@ -42,6 +42,11 @@ def lib():
             "orc_sincospi": (None, [C.c_float, fp, fp]),
             "orc_kernel": (C.c_float, [C.c_float, C.c_float, C.c_float]),
             "orc_seg_sum": (C.c_float, [C.POINTER(C.c_float), C.c_int32, C.c_int32]),
+            "orc_read_counts": (C.c_int64, [C.c_void_p, C.POINTER(C.c_int32), C.c_int64]),
+            "orc_window_chain64": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_int32), fp, fp,
+                                             C.POINTER(C.c_int32), fp, C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "orc_credit": (C.c_float, [C.c_float, C.c_float]),
             "orc_box_muller": (None, [C.c_uint32, C.c_uint32, fp, fp]),
             "orc_update_mass": (None, [fp, C.c_float, C.c_float, C.c_float]),
@@ -142,6 +147,15 @@ class OracleDOM:
         self._L.orc_stage_times(self._h, out)
         return np.array(out[:])
 
+    def counts(self):
+        """Frozen counts of the last frame: (4, V) int32 (inside the grid, of them
+        not penalised; newborns, of them not penalised)."""
+        out = np.empty((4, self.grid_cell_count), np.int32)
+        n = self._L.orc_read_counts(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), out.nbytes)
+        if n < 0:
+            raise ValueError("counts not available")
+        return out
+
     def bgki_pairs(self):
         out = (C.c_uint64 * 2)()
         self._L.orc_bgki_pairs(self._h, out)
@@ -173,3 +187,55 @@ def bgki64(gs, gsz, res, sensor_rel, ls, sigma, xs, ys, zs, cells):
                      _fp(zs), len(xs), cells.ctypes.data_as(C.POINTER(C.c_int32)), len(cells),
                      o.ctypes.data_as(C.POINTER(C.c_double)), f.ctypes.data_as(C.POINTER(C.c_double)))
     return o, f
+
+
+class WindowChain64:
+    """The differentiated window's continuous chain in double precision, with
+    the frozen decisions of a forward run: loss(theta) and, by forward-mode
+    differentiation, grad(theta)."""
+
+    def __init__(self, params, frames, sensor_rel, moves, decays, counts, gts, mask=None):
+        from k3dom_amd._lib import Params
+        self.params = params if isinstance(params, Params) else Params.from_buffer_copy(bytes(params))
+        self.T = len(frames)
+        self.npts = np.array([len(f[0]) for f in frames], np.int32)
+        self.pts = np.concatenate([np.concatenate([np.asarray(a, np.float32) for a in f]) for f in frames]).astype(
+            np.float32) if self.T else np.zeros(1, np.float32)
+        self.sensor_rel = np.ascontiguousarray(np.asarray(sensor_rel, np.float32).reshape(-1))
+        self.moves = np.ascontiguousarray(np.asarray(moves, np.int32).reshape(-1))
+        self.decays = np.ascontiguousarray(np.asarray(decays, np.float32))
+        self.counts = np.ascontiguousarray(np.asarray(counts, np.int32).reshape(-1))
+        self.gts = np.ascontiguousarray(np.asarray(gts, np.uint8).reshape(-1))
+        self.mask = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+
+    def _call(self, theta, grad):
+        th = np.ascontiguousarray(theta, np.float64)
+        loss = C.c_double()
+        g = (C.c_double * 6)() if grad else None
+        u8 = C.POINTER(C.c_uint8)
+        st = lib().orc_window_chain64(
+            C.byref(self.params), self.T, self.npts.ctypes.data_as(C.POINTER(C.c_int32)), _fp(self.pts),
+            _fp(self.sensor_rel), self.moves.ctypes.data_as(C.POINTER(C.c_int32)), _fp(self.decays),
+            self.counts.ctypes.data_as(C.POINTER(C.c_int32)), self.gts.ctypes.data_as(u8),
+            None if self.mask is None else self.mask.ctypes.data_as(u8),
+            th.ctypes.data_as(C.POINTER(C.c_double)), C.byref(loss), g)
+        if st != 0:
+            raise ValueError(f"orc_window_chain64 status {st}")
+        return (loss.value, np.array(list(g))) if grad else loss.value
+
+    def loss(self, theta):
+        return self._call(theta, False)
+
+    def grad(self, theta):
+        return self._call(theta, True)
+
+    def central_differences(self, theta, rel_step=1e-4):
+        theta = np.asarray(theta, np.float64)
+        g = np.empty(6)
+        for k in range(6):
+            h = rel_step * abs(theta[k])
+            tp, tm = theta.copy(), theta.copy()
+            tp[k] += h
+            tm[k] -= h
+            g[k] = (self.loss(tp) - self.loss(tm)) / (tp[k] - tm[k])
+        return g

@@ -1,0 +1,183 @@
+"""Differentiated window: the GPU adjoint against the double-precision chain
+of the serial restatement (forward-mode derivatives and central differences).
+
+The chain is pinned on the CPU first: its loss equals the loss of the float
+filter's own masses (so the continuous model with frozen counts is the
+filter), and its forward-mode gradient equals central differences."""
+import numpy as np
+import pytest
+
+from tests._window import bce_loss, oracle_window
+
+
+def _small(**kw):
+    from k3dom_amd.scene import small_workload
+    return small_workload(**kw)
+
+
+@pytest.mark.parametrize("ego_vx", [0.0, 4.0])
+def test_chain_loss_is_the_float_filter_loss(oracle_lib, ego_vx):
+    from k3dom_amd.adjoint import theta_of
+    w = _small(frames=6, seed=7, ego_vx=ego_vx)
+    chain, recs = oracle_window(w, 6)
+    if ego_vx:
+        assert any(r["move"].any() for r in recs), "the moving case must shift the map"
+    L = chain.loss(theta_of(w.params).astype(np.float64))
+    ref = bce_loss(recs)
+    assert abs(L - ref) <= 1e-5 * abs(ref), (L, ref)
+
+
+@pytest.mark.parametrize("ego_vx", [0.0, 4.0])
+def test_chain_forward_mode_equals_central_differences(oracle_lib, ego_vx):
+    from k3dom_amd.adjoint import theta_of
+    w = _small(frames=5, seed=8, ego_vx=ego_vx)
+    chain, _ = oracle_window(w, 5)
+    th = theta_of(w.params).astype(np.float64)
+    th[1], th[2] = 0.12, 0.08  # the chain takes the two evidence weights apart
+    _, g = chain.grad(th)
+    assert np.all(np.isfinite(g)) and np.any(g != 0)
+    # step 1e-4 of each parameter: within 1e-3 (a step that wide straddles a
+    # few voxels' min/max branch switches of the mass update)
+    fd = chain.central_differences(th)
+    assert np.all(np.abs(g - fd) <= 1e-3 * np.abs(fd)), (g, fd)
+    # narrower steps converge to the forward-mode derivative
+    fd6 = chain.central_differences(th, rel_step=1e-6)
+    assert np.all(np.abs(g - fd6) <= 2e-5 * np.abs(fd6)), (g, fd6)
+
+
+def test_chain_with_mask(oracle_lib):
+    """A voxel subset in the loss."""
+    from k3dom_amd.adjoint import theta_of
+    w = _small(frames=4, seed=9)
+    rng = np.random.default_rng(0)
+    V = int(np.prod([16, 16, 8]))
+    mask = np.zeros(V, np.uint8)
+    mask[rng.choice(V, 200, replace=False)] = 1
+    chain, recs = oracle_window(w, 4, mask=mask)
+    th = theta_of(w.params).astype(np.float64)
+    L = chain.loss(th)
+    assert abs(L - bce_loss(recs, mask)) <= 1e-5 * L
+    _, g = chain.grad(th)
+    fd = chain.central_differences(th, rel_step=1e-6)
+    assert np.all(np.abs(g - fd) <= 2e-5 * np.abs(fd))
+
+
+# ---------------------------------------------------------------------------- GPU
+def _gpu_window(w, frames, first_frame=0, mask=None):
+    from k3dom_amd.adjoint import Window
+    from k3dom_amd.scene import Scene
+    win = Window(w.params, frames)
+    win.set_scene(Scene(w.scene), first_frame)
+    win.set_mask(mask)
+    return win
+
+
+def _chain_from_gpu(win, w, frames, first_frame=0, mask=None):
+    """The double chain fed with the GPU forward's own record (moves, counts)."""
+    from k3dom_amd.adjoint import map_centers
+    from k3dom_amd.scene import Scene
+    from oracle.orc import WindowChain64
+    p = w.params
+    sc = Scene(w.scene)
+    recs = [sc.frame(first_frame + f) for f in range(frames)]
+    centers = map_centers(p, [r[1] for r in recs])
+    srel = [np.float32(r[1]) - np.float32(c) for r, c in zip(recs, centers)]
+    decays, prev = [], None
+    for r in recs:
+        decays.append(1.0 if prev is None else float(np.float32(float(np.float32(p.gamma)) **
+                                                               float(np.float32(r[0] - prev)))))
+        prev = r[0]
+    gs = win.grid[0]
+    gts = [sc.occupancy(first_frame + f, gs, win.grid[2], p.resolution, centers[f], p.size, p.size_z)
+           for f in range(frames)]
+    moves = [win.record(f, "move") for f in range(frames)]
+    counts = [win.record(f, "counts") for f in range(frames)]
+    return WindowChain64(p, [(r[2], r[3], r[4]) for r in recs], srel, moves, decays, counts, gts, mask)
+
+
+@pytest.mark.gpu
+def test_window_forward_record_is_bit_exact(gpu):
+    """The window's forward frames are the filter's frames: masses and frozen
+    counts equal the serial restatement's bit for bit."""
+    from k3dom_amd.adjoint import theta_of
+    w = _small(frames=6, seed=7, ego_vx=4.0)
+    print("window: small scene, 6 frames, moving ego", flush=True)
+    win = _gpu_window(w, 6)
+    L = win.forward(theta_of(w.params))
+    _, recs = oracle_window(w, 6)
+    for f, r in enumerate(recs):
+        m = win.record(f, "masses")
+        for k, name in enumerate("fsd"):
+            assert np.array_equal(m[k].view(np.uint32), r[name].view(np.uint32)), (f, name)
+        assert np.array_equal(win.record(f, "counts"), r["counts"]), f
+        assert np.array_equal(win.record(f, "move"), r["move"]), f
+    assert abs(L - bce_loss(recs)) <= 1e-6 * L
+
+
+def _rel_err(a, b):
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ego_vx,sig", [(0.0, (0.1, 0.1)), (4.0, (0.12, 0.08))])
+def test_window_gradient_matches_fp64(gpu, ego_vx, sig):
+    from k3dom_amd.adjoint import theta_of
+    w = _small(frames=5, seed=11, ego_vx=ego_vx)
+    th = theta_of(w.params)
+    th[1], th[2] = sig
+    print(f"window gradient: small scene, ego {ego_vx} m/s, sigmas {sig}", flush=True)
+    win = _gpu_window(w, 5)
+    L = win.forward(th)
+    g = win.backward()
+    chain = _chain_from_gpu(win, w, 5)
+    L64, g64 = chain.grad(th.astype(np.float64))
+    # a narrow step: the small scene's loss has branch switches of the mass
+    # update within 1e-4 of ls (central differences converge to g64 below it)
+    fd = chain.central_differences(th.astype(np.float64), rel_step=1e-6)
+    print("grad", g, "fp64", g64, "fd", fd, flush=True)
+    assert abs(L - L64) <= 1e-5 * abs(L64)
+    assert np.all(_rel_err(g, g64) <= 1e-3), (g, g64)
+    assert np.all(_rel_err(g, fd) <= 1e-3), (g, fd)
+
+
+@pytest.mark.gpu
+def test_window_gradient_config3(gpu):
+    """The differentiated configuration: config-1 scene, ls 0.3, 8 frames,
+    loss over 1,000 random voxels; GPU FP32 gradient within 1e-3 relative of
+    FP64 central differences (step 1e-4 of each parameter)."""
+    from k3dom_amd.adjoint import theta_of
+    from k3dom_amd.scene import workload
+    w = workload(3)
+    V = 64 * 64 * 16
+    mask = np.zeros(V, np.uint8)
+    mask[np.random.default_rng(3).choice(V, 1000, replace=False)] = 1
+    print("window gradient: config 3, 8 frames, 1000 voxels", flush=True)
+    win = _gpu_window(w, 8, mask=mask)
+    th = theta_of(w.params)
+    L = win.forward(th)
+    g = win.backward()
+    chain = _chain_from_gpu(win, w, 8, mask=mask)
+    fd = chain.central_differences(th.astype(np.float64))
+    L64 = chain.loss(th.astype(np.float64))
+    print("loss", L, L64, "grad", g, "fd", fd, flush=True)
+    assert abs(L - L64) <= 1e-5 * abs(L64)
+    assert np.all(_rel_err(g, fd) <= 1e-3), (g, fd)
+
+
+@pytest.mark.gpu
+def test_window_torch_autograd_and_per_voxel_priors(gpu):
+    """torch.autograd over the libtorch function gives the window's gradient;
+    per-voxel initial-mass gradients sum to the prior gradients (no shift)."""
+    import torch
+    from k3dom_amd.adjoint import theta_of
+    w = _small(frames=4, seed=12)
+    win = _gpu_window(w, 4)
+    th = torch.tensor(theta_of(w.params), dtype=torch.float32, requires_grad=True)
+    loss = win.loss(th)
+    loss.backward()
+    L = win.forward(theta_of(w.params))
+    g, pv = win.backward(per_voxel=True)
+    assert abs(loss.item() - L) <= 1e-12 * abs(L)
+    assert np.allclose(th.grad.numpy().astype(np.float64), g.astype(np.float32), rtol=1e-6, atol=0)
+    for k in range(3):
+        assert abs(pv[k].astype(np.float64).sum() - g[3 + k]) <= 1e-4 * abs(g[3 + k]) + 1e-6
