@@ -215,7 +215,7 @@ def main():
     frame_hbm = {"bytes_per_frame": int(model), "achieved": round(frame_gbps, 2), "peak": HBM_PEAK_GBPS,
                  "unit": "GB/s", "frac": round(frame_gbps / HBM_PEAK_GBPS, 5)}
 
-    if roof is not None and os.path.exists(TRAFFIC_FILE) and w.name.startswith("cfg2"):
+    if roof is not None and os.path.exists(TRAFFIC_FILE) and args.config == 2:
         with open(TRAFFIC_FILE) as fh:
             tr = json.load(fh)
         roof["traffic"] = tr.get("bytes_per_launch")
