@@ -188,6 +188,13 @@ k3dom_status k3dom_read_array(k3dom_handle* h, int32_t which, void* dst, int64_t
 k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable);
 k3dom_status k3dom_stage_times(k3dom_handle* h, float* ms, int32_t count);
 
+/* Launch mode, no counterpart in the reference: with enable != 0 every normal
+ * frame (after the first) replays one captured HIP graph from the frame
+ * constants upload to the readback of the frame's scalars; the graph is
+ * captured again when a buffer is reallocated.  Results are identical to the
+ * direct launches.  Stage timing (k3dom_set_timing) forces direct launches. */
+k3dom_status k3dom_set_graph(k3dom_handle* h, int32_t enable);
+
 /* Diagnostic with no counterpart in the reference: number of (voxel, point)
  * pairs of the last frame whose occupied (out[0]) and free (out[1]) kernel
  * term is evaluated (distance < ls), i.e. the algorithmic work of the
@@ -203,6 +210,46 @@ typedef struct k3dom_stats {
     uint64_t capacity_regrowths; /* frames re-run after growing a list capacity (all frames) */
 } k3dom_stats;
 k3dom_status k3dom_get_stats(const k3dom_handle* h, k3dom_stats* out);
+
+/* Launch-shape tuning, no counterpart in the reference: after a normal frame,
+ * times the kernel-inference gather over that frame's work items for several
+ * grid sizes and keeps the fastest (blocks per compute unit written to
+ * *blocks_per_cu when not NULL).  Results do not depend on the choice. */
+k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu);
+
+/* ---------------------------------------------------------------------------
+ * Batched maps.  No counterpart in the reference (one map per process): many
+ * independent maps ("tiles", each with its own parameters, grid size, particle
+ * count and point cloud) updated together.  Every map's prediction, sort and
+ * candidate lists run concurrently on the maps' streams, the kernel-inference
+ * gather of all maps runs in one persistent launch over a joint work queue,
+ * then every map's tail runs concurrently.  Each map's results are identical
+ * to those of the same map updated alone.
+ * ------------------------------------------------------------------------- */
+typedef struct k3dom_batch k3dom_batch;
+
+/* params: array of n_maps parameter sets. */
+k3dom_status k3dom_batch_create(const k3dom_params* params, int32_t n_maps, k3dom_batch** out);
+k3dom_status k3dom_batch_destroy(k3dom_batch* b);
+
+/* One frame of every map at time t: poses = 3 * n_maps sensor positions;
+ * xs[i], ys[i], zs[i] the n[i] points of map i (host arrays). */
+k3dom_status k3dom_batch_update(k3dom_batch* b, double t, const float* poses, const float* const* xs,
+                                const float* const* ys, const float* const* zs, const int32_t* n);
+
+/* Handle of map i, for readback (k3dom_read_cells, ...); owned by the batch. */
+k3dom_status k3dom_batch_map(k3dom_batch* b, int32_t i, k3dom_handle** out);
+
+/* Replay each normal batch frame as one captured HIP graph (fork and join
+ * over the map streams); captured again when a buffer is reallocated. */
+k3dom_status k3dom_batch_set_graph(k3dom_batch* b, int32_t enable);
+
+/* As k3dom_autotune for the joint gather of the batch. */
+k3dom_status k3dom_batch_autotune(k3dom_batch* b, int32_t* blocks_per_cu);
+
+/* Host wall time of the last k3dom_batch_update, milliseconds. */
+k3dom_status k3dom_batch_last_ms(const k3dom_batch* b, float* ms);
+const char* k3dom_batch_last_error(const k3dom_batch* b);
 
 /* ---------------------------------------------------------------------------
  * Differentiated window.  No counterpart in the reference (whose mass update is

@@ -75,6 +75,17 @@ SIGNATURES = {
     "k3dom_read_masses": (C.c_int, [C.c_void_p, _fp, _fp, _fp, _fp, _fp, _fp]),
     "k3dom_read_array": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     "k3dom_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "k3dom_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "k3dom_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "k3dom_batch_create": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
+    "k3dom_batch_destroy": (C.c_int, [C.c_void_p]),
+    "k3dom_batch_update": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_int32)]),
+    "k3dom_batch_map": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "k3dom_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
+    "k3dom_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "k3dom_batch_last_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "k3dom_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "k3dom_stage_times": (C.c_int, [C.c_void_p, _fp, C.c_int32]),
     "k3dom_count_bgki_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "k3dom_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),

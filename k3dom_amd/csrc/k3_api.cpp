@@ -99,6 +99,7 @@ k3dom_status ensure_points(k3dom_handle* h, int m) {
     HIP_TRY(h, dalloc(h, &h->pxyz, (size_t)cap));
     HIP_TRY(h, hipHostMalloc((void**)&h->h_pts, sizeof(float) * 3 * (size_t)cap, hipHostMallocDefault));
     h->m_cap = cap;
+    h->caps_version++;
     // first estimate of the candidate-list sizes; frames that need more grow them
     const long long mc = cap;
     long long want[3] = {8 * mc + 4096, 48 * mc + 16384, 160 * mc + 65536};
@@ -121,6 +122,7 @@ k3dom_status alloc_level(k3dom_handle* h, int l, long long list_cap) {
     B.items_cap = L.nparents + (parent_cap + L.seg - 1) / L.seg;
     B.slot_cap = B.items_cap * L.nch;
     B.list_cap = list_cap;
+    h->caps_version++;
     HIP_TRY(h, dalloc(h, &B.counts, (size_t)B.slot_cap));
     HIP_TRY(h, dalloc(h, &B.slot_off, (size_t)B.slot_cap + 1));
     HIP_TRY(h, dalloc(h, &B.list, (size_t)std::max(1ll, list_cap)));
@@ -135,6 +137,7 @@ k3dom_status alloc_heavy(k3dom_handle* h, long long heavy_cap) {
     I.items = nullptr;
     I.partial = nullptr;
     I.heavy_cap = heavy_cap;
+    h->caps_version++;
     HIP_TRY(h, dalloc(h, &I.items, (size_t)(h->ntiles + heavy_cap)));
     HIP_TRY(h, dalloc(h, &I.partial, (size_t)std::max(1ll, heavy_cap) * I.partial_stride));
     return K3DOM_OK;
@@ -176,46 +179,48 @@ GridGeom grid_geom(const k3dom_handle* h) {
     return g;
 }
 
-k3dom_status build_lists(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int m) {
-    const GridGeom g = grid_geom(h);
-    HIP_TRY(h, launch_point_prep(h->stream, xs, ys, zs, m, h->p.resolution, h->rec, h->pxyz));
+k3dom_status build_lists(k3dom_handle* h, const FrameConsts* fc, const float* xs, const float* ys, const float* zs) {
+    HIP_TRY(h, launch_point_prep(h->stream, fc, xs, ys, zs, h->m_cap, h->p.resolution, h->rec, h->pxyz));
     for (int l = 0; l < 3; ++l)
-        HIP_TRY(h, launch_build_level(h->stream, l, g, h->tg, h->ldesc[l], h->lbuf[l], l ? &h->lbuf[l - 1] : nullptr,
-                                      h->p.ls, h->pxyz, m, h->sc));
+        HIP_TRY(h, launch_build_level(h->stream, l, fc, h->tg, h->ldesc[l], h->lbuf[l], l ? &h->lbuf[l - 1] : nullptr,
+                                      h->p.ls, h->pxyz, h->sc));
     HIP_TRY(h, launch_build_items(h->stream, h->ntiles, h->lbuf[2], h->ibuf, h->sc));
-    h->last_geom = g;
     return K3DOM_OK;
 }
 
-// Kernel inference for the current point set: writes del_occ / del_free for
-// every voxel, or sets the overflow flag when a capacity is exceeded.
-k3dom_status run_bgki(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int m) {
-    mark(h, ST_LISTS);
-    k3dom_status s = build_lists(h, xs, ys, zs, m);
-    if (s != K3DOM_OK) return s;
-    mark(h, ST_BGKI);
-    const int grid_blocks = h->num_cus * 8;
-    mark(h, ST_BGKI_KERNEL);
-    HIP_TRY(h, launch_bgki(h->stream, grid_blocks, h->last_geom, h->tg, h->p.ls, h->sigma_occ, h->sigma_free, h->ibuf,
-                           h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc));
-    mark(h, EV_BGKI_END);
-    HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, h->last_geom, h->tg, h->ibuf, h->del_occ, h->del_free,
-                                   h->sc));
-    return K3DOM_OK;
+void fill_frame_consts(k3dom_handle* h, FrameConsts* c, double t, int m) {
+    const bool normal = h->updated_time > 0;
+    const float dt = normal ? (float)(t - h->updated_time) : 0.0f;
+    c->g = grid_geom(h);
+    c->decay = normal ? (float)std::pow((double)h->p.gamma, (double)dt) : 1.0f;
+    c->frame = h->frame;
+    c->m = m;
+    c->pad = 0;
+    K3PredictConst& pc = c->pc;
+    pc.dt = dt;
+    pc.p_S = h->p.persistence_prob;
+    pc.sigma_pos = h->p.stddev_process_noise_position;
+    pc.sigma_vel = h->p.stddev_process_noise_velocity;
+    pc.vclamp = 5 * h->p.init_max_velocity;
+    pc.gs = h->gs;
+    pc.gsz = h->gsz;
+    pc.frame = h->frame;
+    pc.key0 = (uint32_t)h->p.seed;
+    pc.key1 = (uint32_t)(h->p.seed >> 32);
 }
 
-// Kernel inference and everything after it.  Tail kernels return at once when
-// the overflow flag is set, so the frame can be re-run from here.
-k3dom_status enqueue_rest(k3dom_handle* h, bool normal, float decay, const float* dx, const float* dy,
-                          const float* dz, int m) {
+// Everything after the gather: combination of split items, mass update,
+// persistent update, newborns, resampling, scalar readback.  Tail kernels
+// return at once when the overflow flag is set, so the frame can be re-run.
+k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
     VoxelArrays& v = h->vox[h->vcur];
     const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
     const uint32_t* ab = &h->sc->overflow;
-    k3dom_status s = run_bgki(h, dx, dy, dz, m);
-    if (s != K3DOM_OK) return s;
+    HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, grid_geom(h), h->tg, h->ibuf, h->del_occ, h->del_free,
+                                   h->sc));
     mark(h, ST_MASS);
     if (normal) {
-        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[1].w, h->del_occ, h->del_free, decay,
+        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[1].w, h->del_occ, h->del_free, h->fc,
                                       h->p.mass_scale, h->p.birth_prob, h->born, h->heavy_voxels, h->sc));
         mark(h, ST_PERSIST);
         HIP_TRY(h, launch_voxel_particles(h->stream, ab, v, h->V, h->P[1], h->p.particle_min_vel, h->heavy_voxels,
@@ -226,15 +231,15 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, float decay, const float
         HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->B, h->born, h->bnd,
                                  v.wua));
         HIP_TRY(h, launch_newborn(h->stream, ab, h->bnd, h->V, h->B, h->gs, v.wua, h->NB, h->p.init_max_velocity,
-                                  h->p.particle_min_vel, h->frame, k0, k1, h->sc));
+                                  h->p.particle_min_vel, h->fc, k0, k1, h->sc));
         mark(h, ST_MOMENTS);
         mark(h, ST_RESAMPLE);
         HIP_TRY(h, launch_fixed_scan(h->stream, h->P[1].w, h->N, h->NB.w, h->B, &h->sc->joint_max_bits,
                                      &h->sc->E_joint, h->cdf_j, h->blocksums, &h->sc->total_joint));
         HIP_TRY(h, launch_resample(h->stream, ab, h->cdf_j, h->N + h->B, &h->sc->total_joint, &h->sc->E_joint,
-                                   h->P[1], h->NB, h->P[0], h->N, h->frame, k0, k1, h->ridx));
+                                   h->P[1], h->NB, h->P[0], h->N, h->fc, k0, k1, h->ridx));
     } else {
-        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[0].w, h->del_occ, h->del_free, 1.0f,
+        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[0].w, h->del_occ, h->del_free, nullptr,
                                       h->p.mass_scale, h->p.birth_prob, h->born, h->heavy_voxels, h->sc));
         mark(h, ST_PERSIST);
         mark(h, ST_NEWBORN);
@@ -243,7 +248,7 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, float decay, const float
         HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->N, h->born, h->bnd,
                                  nullptr));
         HIP_TRY(h, launch_init_particles(h->stream, ab, h->bnd, h->V, h->N, h->gs, h->P[0], h->p.init_max_velocity,
-                                         h->p.particle_min_vel, h->frame, k0, k1));
+                                         h->p.particle_min_vel, h->fc, k0, k1));
         mark(h, ST_MOMENTS);
         mark(h, ST_RESAMPLE);
     }
@@ -252,56 +257,147 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, float decay, const float
     return K3DOM_OK;
 }
 
-k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
+// Kernel inference (lists, gather) and everything after it.
+k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const float* dy, const float* dz) {
+    mark(h, ST_LISTS);
+    k3dom_status s = build_lists(h, h->fc, dx, dy, dz);
+    if (s != K3DOM_OK) return s;
+    mark(h, ST_BGKI);
+    mark(h, ST_BGKI_KERNEL);
+    HIP_TRY(h, launch_bgki(h->stream, h->gather_blocks, h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free, h->ibuf,
+                           h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc));
+    mark(h, EV_BGKI_END);
+    return enqueue_after_gather(h, normal);
+}
+
+// The part of a frame before kernel inference (prediction, sort, ranges),
+// after the frame constants reach the device.
+k3dom_status enqueue_head(k3dom_handle* h, bool normal) {
     VoxelArrays& v = h->vox[h->vcur];
-    const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
-    const bool normal = h->updated_time > 0;
-    if (h->timing) for (float& x : h->stage_ms) x = 0.0f;
     mark(h, EV_FRAME_START);
+    HIP_TRY(h, hipMemcpyAsync(h->fc, h->h_fc, sizeof(FrameConsts), hipMemcpyHostToDevice, h->stream));
     HIP_TRY(h, launch_reset_scalars(h->stream, h->sc));
-    float decay = 1.0f;
     if (normal) {
-        const float dt = (float)(t - h->updated_time);
-        decay = (float)std::pow((double)h->p.gamma, (double)dt);
-        K3PredictConst c;
-        c.dt = dt;
-        c.p_S = h->p.persistence_prob;
-        c.sigma_pos = h->p.stddev_process_noise_position;
-        c.sigma_vel = h->p.stddev_process_noise_velocity;
-        c.vclamp = 5 * h->p.init_max_velocity;
-        c.gs = h->gs;
-        c.gsz = h->gsz;
-        c.frame = h->frame;
-        c.key0 = k0;
-        c.key1 = k1;
         mark(h, ST_PREDICT);
-        HIP_TRY(h, launch_predict(h->stream, c, h->P[0], h->N));
+        HIP_TRY(h, launch_predict(h->stream, h->fc, h->P[0], h->N));
         mark(h, ST_SORT);
         HIP_TRY(h, launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys,
                                      &h->last_perm));
         HIP_TRY(h, launch_gather_sorted(h->stream, h->P[0], h->P[1], h->last_perm, h->last_skeys, h->N));
         HIP_TRY(h, launch_ranges(h->stream, h->P[1].key, h->N, v.start, v.end, h->V));
     }
-    for (int attempt = 0;; ++attempt) {
-        k3dom_status s = enqueue_rest(h, normal, decay, dx, dy, dz, m);
+    return K3DOM_OK;
+}
+
+void drop_graph(k3dom_handle* h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+}
+
+// Captures the whole normal frame (constants upload to scalar readback) as a
+// graph over the handle's own point buffer.
+k3dom_status capture_graph(k3dom_handle* h) {
+    drop_graph(h);
+    const size_t cap = (size_t)h->m_cap;
+    const bool timing = h->timing;
+    h->timing = false;  // no event records inside the graph
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    k3dom_status s = K3DOM_OK;
+    if (e == hipSuccess) {
+        s = enqueue_head(h, true);
+        if (s == K3DOM_OK) s = enqueue_rest(h, true, h->d_pts, h->d_pts + cap, h->d_pts + 2 * cap);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+        h->graph = g;
+        if (e2 != hipSuccess && e == hipSuccess) e = e2;
+    }
+    h->timing = timing;
+    if (s != K3DOM_OK) {
+        drop_graph(h);
+        return s;
+    }
+    HIP_TRY(h, e);
+    HIP_TRY(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    h->graph_caps = h->caps_version;
+    h->graph_vcur = h->vcur;
+    return K3DOM_OK;
+}
+
+// Runs kernel inference and the frame tail until no capacity overflows.
+// checked: the frame already ran once and `done` is its outcome (graph path);
+// otherwise the first iteration enqueues the kernel-inference stage itself.
+k3dom_status resolve_overflow(k3dom_handle* h, bool normal, const float* dx, const float* dy, const float* dz,
+                              bool done, bool checked) {
+    for (int attempt = 0; !done; ++attempt) {
+        if (checked || attempt > 0) {
+            if (attempt >= 8) return fail(h, K3DOM_ERR_OUT_OF_MEMORY, "kernel-inference capacities keep overflowing");
+            // grow what overflowed (the frame's need is known up to that stage) and re-run
+            long long want[3], want_heavy = h->ibuf.heavy_cap;
+            for (int l = 0; l < 3; ++l) {
+                want[l] = h->lbuf[l].list_cap;
+                if (h->h_sc->overflow_mask & (1u << l))
+                    want[l] = std::max(want[l] * 2, (long long)h->h_sc->level_total[l] * 3 / 2 + 1024);
+            }
+            if (h->h_sc->overflow_mask & (1u << 4))
+                want_heavy = std::max(want_heavy * 2, (long long)h->h_sc->n_heavy * 3 / 2 + 256);
+            k3dom_status s = ensure_caps(h, want, want_heavy);
+            if (s != K3DOM_OK) return s;
+            HIP_TRY(h, launch_clear_overflow(h->stream, h->sc));
+            h->overflow_retries++;
+        }
+        k3dom_status s = enqueue_rest(h, normal, dx, dy, dz);
         if (s != K3DOM_OK) return s;
         HIP_TRY(h, hipStreamSynchronize(h->stream));
-        if (!h->h_sc->overflow) break;
-        if (attempt >= 8) return fail(h, K3DOM_ERR_OUT_OF_MEMORY, "kernel-inference capacities keep overflowing");
-        // grow what overflowed (the frame's need is known up to that stage) and re-run
-        long long want[3], want_heavy = h->ibuf.heavy_cap;
-        for (int l = 0; l < 3; ++l) {
-            want[l] = h->lbuf[l].list_cap;
-            if (h->h_sc->overflow_mask & (1u << l))
-                want[l] = std::max(want[l] * 2, (long long)h->h_sc->level_total[l] * 3 / 2 + 1024);
-        }
-        if (h->h_sc->overflow_mask & (1u << 4))
-            want_heavy = std::max(want_heavy * 2, (long long)h->h_sc->n_heavy * 3 / 2 + 256);
-        s = ensure_caps(h, want, want_heavy);
-        if (s != K3DOM_OK) return s;
-        HIP_TRY(h, launch_clear_overflow(h->stream, h->sc));
-        h->overflow_retries++;
+        done = !h->h_sc->overflow;
     }
+    return K3DOM_OK;
+}
+
+void finish_frame(k3dom_handle* h, double t, int m) {
+    h->last_geom = h->h_fc->g;
+    h->stats.points = (uint64_t)m;
+    for (int l = 0; l < 3; ++l) h->stats.level_entries[l] = h->h_sc->level_total[l];
+    h->stats.gather_items = (uint64_t)h->h_sc->n_light + h->h_sc->n_heavy;
+    h->stats.gather_split_items = h->h_sc->n_heavy;
+    h->stats.capacity_regrowths = (uint64_t)h->overflow_retries;
+    h->updated_time = t;
+    h->frame++;
+}
+
+k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
+    const bool normal = h->updated_time > 0;
+    if (h->timing) for (float& x : h->stage_ms) x = 0.0f;
+    fill_frame_consts(h, h->h_fc, t, m);
+    const bool use_graph = normal && h->use_graph && !h->timing;
+    bool done = false;
+    if (use_graph) {
+        const size_t cap = (size_t)h->m_cap;
+        if (dx != h->d_pts) {  // the graph reads the handle's own point buffer
+            for (int k = 0; k < 3; ++k) {
+                const float* src = k == 0 ? dx : (k == 1 ? dy : dz);
+                if (m > 0)
+                    HIP_TRY(h, hipMemcpyAsync(h->d_pts + k * cap, src, sizeof(float) * m, hipMemcpyDeviceToDevice,
+                                              h->stream));
+            }
+        }
+        if (!h->gexec || h->graph_caps != h->caps_version || h->graph_vcur != h->vcur) {
+            k3dom_status s = capture_graph(h);
+            if (s != K3DOM_OK) return s;
+        }
+        HIP_TRY(h, hipGraphLaunch(h->gexec, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        done = !h->h_sc->overflow;
+        dx = h->d_pts;
+        dy = h->d_pts + cap;
+        dz = h->d_pts + 2 * cap;
+    } else {
+        k3dom_status s = enqueue_head(h, normal);
+        if (s != K3DOM_OK) return s;
+    }
+    k3dom_status s = resolve_overflow(h, normal, dx, dy, dz, done, use_graph);
+    if (s != K3DOM_OK) return s;
     if (h->timing) {
         const int order[] = {ST_PREDICT, ST_SORT, ST_LISTS, ST_BGKI, ST_MASS, ST_PERSIST, ST_NEWBORN, ST_MOMENTS,
                              ST_RESAMPLE, ST_FRAME};
@@ -316,13 +412,7 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
         if (hipEventElapsedTime(&ms, h->ev[ST_BGKI_KERNEL], h->ev[EV_BGKI_END]) == hipSuccess)
             h->stage_ms[ST_BGKI_KERNEL] = ms;
     }
-    h->stats.points = (uint64_t)m;
-    for (int l = 0; l < 3; ++l) h->stats.level_entries[l] = h->h_sc->level_total[l];
-    h->stats.gather_items = (uint64_t)h->h_sc->n_light + h->h_sc->n_heavy;
-    h->stats.gather_split_items = h->h_sc->n_heavy;
-    h->stats.capacity_regrowths = (uint64_t)h->overflow_retries;
-    h->updated_time = t;
-    h->frame++;
+    finish_frame(h, t, m);
     return K3DOM_OK;
 }
 
@@ -603,6 +693,9 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
         h->tiny_caps = tiny && tiny[0] == '1';
     }
     CREATE_TRY(hipHostMalloc((void**)&h->h_sc, sizeof(DevScalars), hipHostMallocDefault));
+    CREATE_TRY(hipHostMalloc((void**)&h->h_fc, sizeof(FrameConsts), hipHostMallocDefault));
+    CREATE_TRY(dalloc(h, &h->fc, 1));
+    h->gather_blocks = h->num_cus * 8;
     for (int e = 0; e < EV_COUNT; ++e) CREATE_TRY(hipEventCreate(&h->ev[e]));
     CREATE_TRY(dalloc(h, &h->pair_counts, 2));
     // prior fill and zeroed particle state (src/common/kernel/init.cu:20-35)
@@ -632,6 +725,8 @@ k3dom_status k3dom_destroy(k3dom_handle* h) {
     h->allocs.clear();
     if (h->h_pts) (void)hipHostFree(h->h_pts);
     if (h->h_sc) (void)hipHostFree(h->h_sc);
+    if (h->h_fc) (void)hipHostFree(h->h_fc);
+    drop_graph(h);
     for (int e = 0; e < EV_COUNT; ++e)
         if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -782,6 +877,49 @@ k3dom_status k3dom_read_array(k3dom_handle* h, int32_t which, void* dst, int64_t
 k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
     h->timing = enable != 0;
+    return K3DOM_OK;
+}
+
+k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
+    if (!h) return K3DOM_ERR_INVALID_ARG;
+    if (h->frame < 2) return fail(h, K3DOM_ERR_STATE, "autotune needs a normal frame first");
+    hipEvent_t e0, e1;
+    HIP_TRY(h, hipEventCreate(&e0));
+    HIP_TRY(h, hipEventCreate(&e1));
+    const int cands[] = {2, 4, 6, 8, 10, 12, 16};
+    float best = 1e30f;
+    int best_c = h->gather_blocks / h->num_cus;
+    for (int c : cands) {
+        float tot = 0.0f;
+        for (int rep = 0; rep < 4; ++rep) {
+            // the last frame's items are still in place: re-run the gather over them
+            HIP_TRY(h, launch_reset_item_counter(h->stream, h->sc));
+            HIP_TRY(h, hipEventRecord(e0, h->stream));
+            HIP_TRY(h, launch_bgki(h->stream, c * h->num_cus, h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
+                                   h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc));
+            HIP_TRY(h, hipEventRecord(e1, h->stream));
+            HIP_TRY(h, hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_TRY(h, hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) tot += ms;
+        }
+        if (tot < best) {
+            best = tot;
+            best_c = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h->gather_blocks = best_c * h->num_cus;
+    drop_graph(h);
+    if (blocks_per_cu) *blocks_per_cu = best_c;
+    return K3DOM_OK;
+}
+
+k3dom_status k3dom_set_graph(k3dom_handle* h, int32_t enable) {
+    if (!h) return K3DOM_ERR_INVALID_ARG;
+    h->use_graph = enable != 0;
+    if (!h->use_graph) drop_graph(h);
     return K3DOM_OK;
 }
 

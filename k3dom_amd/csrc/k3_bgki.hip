@@ -65,19 +65,20 @@ __device__ __forceinline__ float bg_sqrt(float x) {
 }
 
 // ============================================================== points
-__global__ __launch_bounds__(TPB) void k_point_prep(const float* __restrict__ xs, const float* __restrict__ ys,
-                                                    const float* __restrict__ zs, int m, float res,
+__global__ __launch_bounds__(TPB) void k_point_prep(const FrameConsts* __restrict__ fc,
+                                                    const float* __restrict__ xs, const float* __restrict__ ys,
+                                                    const float* __restrict__ zs, float res,
                                                     K3PointRec* __restrict__ rec, float4* __restrict__ pxyz) {
     int i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= m) return;
+    if (i >= fc->m) return;
     K3PointRec r = k3_point_prep(xs[i], ys[i], zs[i], res);
     rec[i] = r;
     pxyz[i] = make_float4(r.x, r.y, r.z, r.valid != 0.0f ? r.dm : -1.0f);
 }
-hipError_t launch_point_prep(hipStream_t st, const float* xs, const float* ys, const float* zs, int m, float res,
-                             K3PointRec* rec, float4* pxyz) {
-    if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(m)), dim3(TPB), 0, st, xs, ys, zs, m, res, rec, pxyz);
+hipError_t launch_point_prep(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
+                             const float* zs, int m_cap, float res, K3PointRec* rec, float4* pxyz) {
+    if (m_cap <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(m_cap)), dim3(TPB), 0, st, fc, xs, ys, zs, res, rec, pxyz);
     return hipGetLastError();
 }
 
@@ -159,12 +160,12 @@ __device__ __forceinline__ int find_parent(const uint32_t* __restrict__ item_bas
 }
 
 // ============================================================== candidate lists
-__global__ __launch_bounds__(TPB) void k_level_nseg(int level, LevelDesc L, int m,
+__global__ __launch_bounds__(TPB) void k_level_nseg(int level, LevelDesc L, const FrameConsts* __restrict__ fc,
                                                     const uint32_t* __restrict__ parent_len,
                                                     uint32_t* __restrict__ nseg, const DevScalars* sc) {
     int P = blockIdx.x * TPB + threadIdx.x;
     if (P >= L.nparents) return;
-    uint32_t len = level == 0 ? (uint32_t)m : (sc->overflow ? 0u : parent_len[P]);
+    uint32_t len = level == 0 ? (uint32_t)fc->m : (sc->overflow ? 0u : parent_len[P]);
     nseg[P] = (len + (uint32_t)L.seg - 1) / (uint32_t)L.seg;
 }
 
@@ -188,8 +189,8 @@ __device__ __forceinline__ ItemCtx item_ctx(int level, const LevelDesc& L, uint3
     return x;
 }
 
-__global__ __launch_bounds__(TPB) void k_level_count(int level, GridGeom g, TileGeom tg, LevelDesc L, float ls,
-                                                     const float4* __restrict__ pxyz, int m,
+__global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConsts* __restrict__ fc, TileGeom tg,
+                                                     LevelDesc L, float ls, const float4* __restrict__ pxyz,
                                                      const uint32_t* __restrict__ parent_off,
                                                      const uint32_t* __restrict__ parent_len,
                                                      const uint32_t* __restrict__ parent_list,
@@ -198,6 +199,8 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, GridGeom g, Tile
     __shared__ float4 cb[MAX_CHILDREN];
     __shared__ uint32_t cnt[MAX_CHILDREN];
     if (sc->overflow) return;
+    const GridGeom g = fc->g;
+    const int m = fc->m;
     const uint32_t item = blockIdx.x;
     if (item >= item_base[L.nparents]) return;
     const int t = threadIdx.x, lane = t & 63;
@@ -224,8 +227,8 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, GridGeom g, Tile
     for (int c = t; c < L.nch; c += TPB) counts[x.ib * L.nch + c * x.nsegP + x.seg] = cnt[c];
 }
 
-__global__ __launch_bounds__(TPB) void k_level_fill(int level, GridGeom g, TileGeom tg, LevelDesc L, float ls,
-                                                    const float4* __restrict__ pxyz, int m,
+__global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts* __restrict__ fc, TileGeom tg,
+                                                     LevelDesc L, float ls, const float4* __restrict__ pxyz,
                                                     const uint32_t* __restrict__ parent_off,
                                                     const uint32_t* __restrict__ parent_len,
                                                     const uint32_t* __restrict__ parent_list,
@@ -238,6 +241,8 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, GridGeom g, TileG
     __shared__ uint32_t run[MAX_CHILDREN];
     __shared__ uint32_t wc[4];
     if (sc->overflow) return;
+    const GridGeom g = fc->g;
+    const int m = fc->m;
     const uint32_t n_items = item_base[L.nparents];
     const uint32_t total = slot_off[n_items * (uint32_t)L.nch];
     if ((long long)total > list_cap) {
@@ -313,20 +318,20 @@ __global__ __launch_bounds__(TPB) void k_level_boxes(int level, LevelDesc L, con
     box_len[box] = len;
 }
 
-hipError_t launch_build_level(hipStream_t st, int level, const GridGeom& g, const TileGeom& tg, const LevelDesc& L,
-                              const LevelBufs& B, const LevelBufs* parent, float ls, const float4* pxyz, int m,
+hipError_t launch_build_level(hipStream_t st, int level, const FrameConsts* fc, const TileGeom& tg, const LevelDesc& L,
+                              const LevelBufs& B, const LevelBufs* parent, float ls, const float4* pxyz,
                               DevScalars* sc) {
     const uint32_t* poff = parent ? parent->box_off : nullptr;
     const uint32_t* plen = parent ? parent->box_len : nullptr;
     const uint32_t* plist = parent ? parent->list : nullptr;
-    hipLaunchKernelGGL(k_level_nseg, dim3(blocks_for(L.nparents)), dim3(TPB), 0, st, level, L, m, plen, B.nseg, sc);
+    hipLaunchKernelGGL(k_level_nseg, dim3(blocks_for(L.nparents)), dim3(TPB), 0, st, level, L, fc, plen, B.nseg, sc);
     (void)launch_scan_u32(st, B.nseg, B.item_base, B.bsum, L.nparents, nullptr, 1, &sc->level_items[level]);
     const unsigned grid = (unsigned)std::max(1ll, B.items_cap);
-    hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(TPB), 0, st, level, g, tg, L, ls, pxyz, m, poff, plen, plist,
+    hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(TPB), 0, st, level, fc, tg, L, ls, pxyz, poff, plen, plist,
                        B.item_base, B.counts, sc);
     (void)launch_scan_u32(st, B.counts, B.slot_off, B.bsum, B.slot_cap, B.item_base + L.nparents, (uint32_t)L.nch,
                     nullptr);
-    hipLaunchKernelGGL(k_level_fill, dim3(grid), dim3(TPB), 0, st, level, g, tg, L, ls, pxyz, m, poff, plen, plist,
+    hipLaunchKernelGGL(k_level_fill, dim3(grid), dim3(TPB), 0, st, level, fc, tg, L, ls, pxyz, poff, plen, plist,
                        B.item_base, B.slot_off, B.list, B.list_cap, sc);
     hipLaunchKernelGGL(k_level_boxes, dim3(blocks_for((long long)L.nbx * L.nby * L.nbz)), dim3(TPB), 0, st, level, L,
                        B.item_base, B.slot_off, B.box_off, B.box_len, sc);
@@ -460,17 +465,147 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float d, float kscale, float sig
 template <bool GRAD>
 constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 
+// Gather of one work item (a whole tile, or one 1024-point chunk of a long
+// tile list) by one wavefront: lane = voxel of the 4x4x4 tile.
 template <bool GRAD>
-__global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls, float ls2g, float kscale,
-                                              float inv_ls, float sigma_o, float sigma_f,
-                                              const uint4* __restrict__ items, const uint32_t* __restrict__ tlist,
-                                              const K3PointRec* __restrict__ rec, float* __restrict__ del_occ,
-                                              float* __restrict__ del_free, float* __restrict__ dls_occ,
-                                              float* __restrict__ dls_free, float* __restrict__ partial,
-                                              DevScalars* sc) {
+__device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, int lane,
+                                          float4 (&srec)[BG_WAVES][64 * 3], uint32_t (&sidx)[BG_WAVES][64],
+                                          uint32_t (&scls)[BG_WAVES][64]) {
+    const GridGeom g = a.fc->g;
+    const uint4 item = a.items[it];
+    const uint32_t tile = uniform(item.x), ib = uniform(item.y), ie = uniform(item.z), slot = uniform(item.w);
+    const int tx = (int)(tile % (uint32_t)a.tg.ntx), ty = (int)((tile / (uint32_t)a.tg.ntx) % (uint32_t)a.tg.nty);
+    const int tz = (int)(tile / (uint32_t)(a.tg.ntx * a.tg.nty));
+    const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
+    const float vx = axis_center(i, g.gs, g.res, g.sx);
+    const float vy = axis_center(j, g.gs, g.res, g.sy);
+    const float vz = axis_center(k, g.gsz, g.res, g.sz);
+    const float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
+    const float dc2 = dc * dc;
+    // free term of a voxel for every point in front of which it lies
+    // (k3_kernel_value is an exact zero for distances >= ls)
+    const BgTerm<GRAD> kc = bg_term<GRAD>(dc, a.kscale, a.sigma_f, a.ls, a.inv_ls);
+    // bounding box of the tile's 64 voxel centres (lanes outside the grid included)
+    const float tcx = 0.5f * (axis_center(tx * 4, g.gs, g.res, g.sx) + axis_center(tx * 4 + 3, g.gs, g.res, g.sx));
+    const float tcy = 0.5f * (axis_center(ty * 4, g.gs, g.res, g.sy) + axis_center(ty * 4 + 3, g.gs, g.res, g.sy));
+    const float tcz =
+        0.5f * (axis_center(tz * 4, g.gsz, g.res, g.sz) + axis_center(tz * 4 + 3, g.gsz, g.res, g.sz));
+    const float trad = 1.5f * g.res * 1.7320508f;
+    const float marg = 1e-3f + 1e-5f * (fabsf(tcx) + fabsf(tcy) + fabsf(tcz) + trad);
+    const float rr = trad + marg;
+    const float occ_r = a.ls + rr;
+    float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
+    float gsum_o = 0.0f, gsum_f = 0.0f;  // length-scale derivatives (GRAD)
+    uint32_t cur_chunk = 0xffffffffu;
+    for (uint32_t b = ib; b < ie; b += 64) {
+        const uint32_t e = b + lane;
+        if (e < ie) {
+            const uint32_t p = a.tlist[e];
+            const K3PointRec r = a.rec[p];
+            const float4* src = reinterpret_cast<const float4*>(a.rec + p);
+            sidx[w][lane] = p;
+            srec[w][lane * 3 + 0] = src[0];
+            srec[w][lane * 3 + 1] = src[1];
+            srec[w][lane * 3 + 2] = src[2];
+            const float proj = fmaf(tcz, r.z, fmaf(tcy, r.y, tcx * r.x)) * r.inv_dm;
+            uint32_t cls = C_GENERAL;
+            if (proj + rr <= 0.0f) cls = C_BEHIND;
+            else if (proj - rr > 0.0f && proj + rr < r.mg) cls = C_MAIN;
+            else if (proj - rr >= r.dm) cls = C_NOFREE;
+            const float ox = tcx - r.x, oy = tcy - r.y, oz = tcz - r.z;
+            if (!(ox * ox + oy * oy + oz * oz >= occ_r * occ_r)) cls |= F_OCC;
+            scls[w][lane] = cls;
+        }
+        bg_wave_sync();
+        const uint32_t cnt = uniform(min(64u, ie - b));
+        for (uint32_t q = 0; q < cnt; ++q) {
+            const uint32_t chunk = uniform(sidx[w][q]) >> K3_BGKI_CHUNK_LOG2;
+            const uint32_t cls = uniform(scls[w][q]);
+            if (chunk != cur_chunk) {
+                tot_o += cur_o;
+                tot_f += cur_f;
+                cur_o = 0.0f;
+                cur_f = 0.0f;
+                cur_chunk = chunk;
+            }
+            // record: r0 = (x, y, z, inv_dm), r1 = (dm, mg, ex, ey), r2 = (ez, valid, -, -)
+            const float4 r0 = srec[w][q * 3 + 0];
+            if (cls & F_OCC) {  // occupied term (k3_occ_distance)
+                const float ox = vx - r0.x, oy = vy - r0.y, oz = vz - r0.z;
+                const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
+                const bool co = so < a.ls2g;
+                if (__any(co)) {
+                    const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(so), a.kscale, a.sigma_o, a.ls, a.inv_ls);
+                    cur_o += t.v;
+                    if (GRAD) gsum_o += t.g;
+                }
+            }
+            const uint32_t fc = cls & 3u;
+            if (fc == C_MAIN) {  // 0 < dp < mg for every lane
+                const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
+                const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
+                const bool cf = a2 < a.ls2g;
+                if (__any(cf)) {
+                    const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(a2), a.kscale, a.sigma_f, a.ls, a.inv_ls);
+                    cur_f += t.v;
+                    if (GRAD) gsum_f += t.g;
+                }
+            } else if (fc == C_BEHIND) {  // dp <= 0 for every lane
+                cur_f += kc.v;
+                if (GRAD) gsum_f += kc.g;
+            } else if (fc == C_GENERAL) {  // k3_free_distance, the three cases as selects
+                const float4 r1 = srec[w][q * 3 + 1];
+                const float4 r2 = srec[w][q * 3 + 2];
+                const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
+                const bool c1 = dp <= 0.0f;
+                const bool c2 = !c1 && dp < r1.y;
+                const bool c3 = !c1 && !c2 && dp < r1.x;
+                const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
+                const float ex = vx - r1.z, ey = vy - r1.w, ez = vz - r2.x;
+                const float a3 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+                const float aa = c2 ? a2 : a3;
+                const bool cs = (c2 || c3) && aa < a.ls2g;
+                if (__any(cs)) {
+                    const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(aa), a.kscale, a.sigma_f, a.ls, a.inv_ls);
+                    cur_f += cs ? t.v : (c1 ? kc.v : 0.0f);
+                    if (GRAD) gsum_f += cs ? t.g : (c1 ? kc.g : 0.0f);
+                } else {
+                    cur_f += c1 ? kc.v : 0.0f;
+                    if (GRAD) gsum_f += c1 ? kc.g : 0.0f;
+                }
+            }
+        }
+        bg_wave_sync();
+    }
+    tot_o += cur_o;
+    tot_f += cur_f;
+    if (slot == BG_DIRECT) {
+        if (i < g.gs && j < g.gs && k < g.gsz) {
+            const int cell = i + g.gs * j + g.gs * g.gs * k;
+            a.del_occ[cell] = tot_o;
+            a.del_free[cell] = tot_f;
+            if (GRAD) {
+                a.dls_occ[cell] = gsum_o;
+                a.dls_free[cell] = gsum_f;
+            }
+        }
+    } else {
+        float* pp = a.partial + (size_t)slot * bg_partial_stride<GRAD>();
+        pp[lane] = tot_o;
+        pp[64 + lane] = tot_f;
+        if (GRAD) {
+            pp[128 + lane] = gsum_o;
+            pp[192 + lane] = gsum_f;
+        }
+    }
+}
+
+template <bool GRAD>
+__global__ __launch_bounds__(TPB) void k_bgki(BgArgs a) {
     __shared__ float4 srec[BG_WAVES][64 * 3];
     __shared__ uint32_t sidx[BG_WAVES][64];
     __shared__ uint32_t scls[BG_WAVES][64];
+    DevScalars* sc = a.sc;
     if (sc->overflow) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n_items = sc->n_light + sc->n_heavy;
@@ -479,149 +614,104 @@ __global__ __launch_bounds__(TPB) void k_bgki(GridGeom g, TileGeom tg, float ls,
         if (lane == 0) it = atomicAdd(&sc->item_counter, 1u);
         it = uniform(__shfl(it, 0));
         if (it >= n_items) break;
-        const uint4 item = items[it];
-        const uint32_t tile = uniform(item.x), ib = uniform(item.y), ie = uniform(item.z), slot = uniform(item.w);
-        const int tx = (int)(tile % (uint32_t)tg.ntx), ty = (int)((tile / (uint32_t)tg.ntx) % (uint32_t)tg.nty);
-        const int tz = (int)(tile / (uint32_t)(tg.ntx * tg.nty));
-        const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
-        const float vx = axis_center(i, g.gs, g.res, g.sx);
-        const float vy = axis_center(j, g.gs, g.res, g.sy);
-        const float vz = axis_center(k, g.gsz, g.res, g.sz);
-        const float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
-        const float dc2 = dc * dc;
-        // free term of a voxel for every point in front of which it lies
-        // (k3_kernel_value is an exact zero for distances >= ls)
-        const BgTerm<GRAD> kc = bg_term<GRAD>(dc, kscale, sigma_f, ls, inv_ls);
-        // bounding box of the tile's 64 voxel centres (lanes outside the grid included)
-        const float tcx = 0.5f * (axis_center(tx * 4, g.gs, g.res, g.sx) + axis_center(tx * 4 + 3, g.gs, g.res, g.sx));
-        const float tcy = 0.5f * (axis_center(ty * 4, g.gs, g.res, g.sy) + axis_center(ty * 4 + 3, g.gs, g.res, g.sy));
-        const float tcz =
-            0.5f * (axis_center(tz * 4, g.gsz, g.res, g.sz) + axis_center(tz * 4 + 3, g.gsz, g.res, g.sz));
-        const float trad = 1.5f * g.res * 1.7320508f;
-        const float marg = 1e-3f + 1e-5f * (fabsf(tcx) + fabsf(tcy) + fabsf(tcz) + trad);
-        const float rr = trad + marg;
-        const float occ_r = ls + rr;
-        float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
-        float gsum_o = 0.0f, gsum_f = 0.0f;  // length-scale derivatives (GRAD)
-        uint32_t cur_chunk = 0xffffffffu;
-        for (uint32_t b = ib; b < ie; b += 64) {
-            const uint32_t e = b + lane;
-            if (e < ie) {
-                const uint32_t p = tlist[e];
-                const K3PointRec r = rec[p];
-                const float4* src = reinterpret_cast<const float4*>(rec + p);
-                sidx[w][lane] = p;
-                srec[w][lane * 3 + 0] = src[0];
-                srec[w][lane * 3 + 1] = src[1];
-                srec[w][lane * 3 + 2] = src[2];
-                const float proj = fmaf(tcz, r.z, fmaf(tcy, r.y, tcx * r.x)) * r.inv_dm;
-                uint32_t cls = C_GENERAL;
-                if (proj + rr <= 0.0f) cls = C_BEHIND;
-                else if (proj - rr > 0.0f && proj + rr < r.mg) cls = C_MAIN;
-                else if (proj - rr >= r.dm) cls = C_NOFREE;
-                const float ox = tcx - r.x, oy = tcy - r.y, oz = tcz - r.z;
-                if (!(ox * ox + oy * oy + oz * oz >= occ_r * occ_r)) cls |= F_OCC;
-                scls[w][lane] = cls;
-            }
-            bg_wave_sync();
-            const uint32_t cnt = uniform(min(64u, ie - b));
-            for (uint32_t q = 0; q < cnt; ++q) {
-                const uint32_t chunk = uniform(sidx[w][q]) >> K3_BGKI_CHUNK_LOG2;
-                const uint32_t cls = uniform(scls[w][q]);
-                if (chunk != cur_chunk) {
-                    tot_o += cur_o;
-                    tot_f += cur_f;
-                    cur_o = 0.0f;
-                    cur_f = 0.0f;
-                    cur_chunk = chunk;
-                }
-                // record: r0 = (x, y, z, inv_dm), r1 = (dm, mg, ex, ey), r2 = (ez, valid, -, -)
-                const float4 r0 = srec[w][q * 3 + 0];
-                if (cls & F_OCC) {  // occupied term (k3_occ_distance)
-                    const float ox = vx - r0.x, oy = vy - r0.y, oz = vz - r0.z;
-                    const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
-                    const bool co = so < ls2g;
-                    if (__any(co)) {
-                        const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(so), kscale, sigma_o, ls, inv_ls);
-                        cur_o += t.v;
-                        if (GRAD) gsum_o += t.g;
-                    }
-                }
-                const uint32_t fc = cls & 3u;
-                if (fc == C_MAIN) {  // 0 < dp < mg for every lane
-                    const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
-                    const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
-                    const bool cf = a2 < ls2g;
-                    if (__any(cf)) {
-                        const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(a2), kscale, sigma_f, ls, inv_ls);
-                        cur_f += t.v;
-                        if (GRAD) gsum_f += t.g;
-                    }
-                } else if (fc == C_BEHIND) {  // dp <= 0 for every lane
-                    cur_f += kc.v;
-                    if (GRAD) gsum_f += kc.g;
-                } else if (fc == C_GENERAL) {  // k3_free_distance, the three cases as selects
-                    const float4 r1 = srec[w][q * 3 + 1];
-                    const float4 r2 = srec[w][q * 3 + 2];
-                    const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
-                    const bool c1 = dp <= 0.0f;
-                    const bool c2 = !c1 && dp < r1.y;
-                    const bool c3 = !c1 && !c2 && dp < r1.x;
-                    const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
-                    const float ex = vx - r1.z, ey = vy - r1.w, ez = vz - r2.x;
-                    const float a3 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-                    const float a = c2 ? a2 : a3;
-                    const bool cs = (c2 || c3) && a < ls2g;
-                    if (__any(cs)) {
-                        const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(a), kscale, sigma_f, ls, inv_ls);
-                        cur_f += cs ? t.v : (c1 ? kc.v : 0.0f);
-                        if (GRAD) gsum_f += cs ? t.g : (c1 ? kc.g : 0.0f);
-                    } else {
-                        cur_f += c1 ? kc.v : 0.0f;
-                        if (GRAD) gsum_f += c1 ? kc.g : 0.0f;
-                    }
-                }
-            }
-            bg_wave_sync();
-        }
-        tot_o += cur_o;
-        tot_f += cur_f;
-        if (slot == BG_DIRECT) {
-            if (i < g.gs && j < g.gs && k < g.gsz) {
-                const int cell = i + g.gs * j + g.gs * g.gs * k;
-                del_occ[cell] = tot_o;
-                del_free[cell] = tot_f;
-                if (GRAD) {
-                    dls_occ[cell] = gsum_o;
-                    dls_free[cell] = gsum_f;
-                }
-            }
-        } else {
-            float* pp = partial + (size_t)slot * bg_partial_stride<GRAD>();
-            pp[lane] = tot_o;
-            pp[64 + lane] = tot_f;
-            if (GRAD) {
-                pp[128 + lane] = gsum_o;
-                pp[192 + lane] = gsum_f;
-            }
-        }
+        bgki_item<GRAD>(a, it, w, lane, srec, sidx, scls);
     }
 }
-hipError_t launch_bgki(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
+
+BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float sigma_o, float sigma_f,
+                    const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec, float* del_occ, float* del_free,
+                    float* dls_occ, float* dls_free, DevScalars* sc) {
+    BgArgs a;
+    a.fc = fc;
+    a.tg = tg;
+    a.ls = ls;
+    a.ls2g = k3_ls2_guard(ls);
+    a.kscale = k3_kernel_scale(ls);
+    a.inv_ls = 1.0f / ls;
+    a.sigma_o = sigma_o;
+    a.sigma_f = sigma_f;
+    a.items = I.items;
+    a.tlist = T.list;
+    a.rec = rec;
+    a.del_occ = del_occ;
+    a.del_free = del_free;
+    a.dls_occ = dls_occ;
+    a.dls_free = dls_free;
+    a.partial = I.partial;
+    a.sc = sc;
+    return a;
+}
+
+hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                        float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec,
                        float* del_occ, float* del_free, DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki<false>, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls),
-                       k3_kernel_scale(ls), 1.0f / ls, sigma_o, sigma_f, I.items, T.list, rec, del_occ, del_free,
-                       (float*)nullptr, (float*)nullptr, I.partial, sc);
+    hipLaunchKernelGGL(k_bgki<false>, dim3(grid_blocks), dim3(TPB), 0, st,
+                       make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, nullptr, nullptr, sc));
     return hipGetLastError();
 }
-hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
+hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T,
                             const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
                             DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki<true>, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls),
-                       k3_kernel_scale(ls), 1.0f / ls, sigma_o, sigma_f, I.items, T.list, rec, del_occ, del_free,
-                       dls_occ, dls_free, I.partial, sc);
+    hipLaunchKernelGGL(k_bgki<true>, dim3(grid_blocks), dim3(TPB), 0, st,
+                       make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, dls_occ, dls_free, sc));
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- many maps
+// Work items of every map of a batch in one queue: prefix[m] is the first
+// item of map m (a map whose lists overflowed contributes none).
+__global__ void k_bgki_batch_prefix(const BgArgs* __restrict__ maps, int nmaps, uint32_t* __restrict__ prefix,
+                                    uint32_t* __restrict__ counter) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t acc = 0;
+    for (int m = 0; m < nmaps; ++m) {
+        prefix[m] = acc;
+        const DevScalars* sc = maps[m].sc;
+        if (!sc->overflow) acc += sc->n_light + sc->n_heavy;
+    }
+    prefix[nmaps] = acc;
+    *counter = 0;
+}
+
+// One persistent launch over the items of all maps.
+__global__ __launch_bounds__(TPB) void k_bgki_batch(const BgArgs* __restrict__ maps, int nmaps,
+                                                    const uint32_t* __restrict__ prefix, uint32_t* counter) {
+    __shared__ float4 srec[BG_WAVES][64 * 3];
+    __shared__ uint32_t sidx[BG_WAVES][64];
+    __shared__ uint32_t scls[BG_WAVES][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t total = prefix[nmaps];
+    while (true) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd(counter, 1u);
+        it = uniform(__shfl(it, 0));
+        if (it >= total) break;
+        int lo = 0, hi = nmaps - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= it) lo = mid;
+            else hi = mid - 1;
+        }
+        const BgArgs a = maps[lo];
+        bgki_item<false>(a, it - prefix[lo], w, lane, srec, sidx, scls);
+    }
+}
+
+hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps, int nmaps, uint32_t* prefix,
+                             uint32_t* counter) {
+    hipLaunchKernelGGL(k_bgki_batch_prefix, dim3(1), dim3(64), 0, st, maps, nmaps, prefix, counter);
+    hipLaunchKernelGGL(k_bgki_batch, dim3(grid_blocks), dim3(TPB), 0, st, maps, nmaps, prefix, counter);
+    return hipGetLastError();
+}
+
+// Resets the work-queue head so that the gather can be launched again over
+// the same items (launch-shape tuning).
+__global__ void k_reset_item_counter(DevScalars* sc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) sc->item_counter = 0;
+}
+hipError_t launch_reset_item_counter(hipStream_t st, DevScalars* sc) {
+    hipLaunchKernelGGL(k_reset_item_counter, dim3(1), dim3(64), 0, st, sc);
     return hipGetLastError();
 }
 

@@ -62,6 +62,16 @@ struct k3dom_handle {
     double updated_time = -1.0;
     std::string err;
 
+    // frame constants (device copy read by the kernels, pinned host staging)
+    FrameConsts* fc = nullptr;
+    FrameConsts* h_fc = nullptr;
+    int gather_blocks = 0;     // blocks of the kernel-inference gather (autotuned)
+    // captured normal-frame graph; re-captured when buffers move or the voxel buffer flips
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int caps_version = 0, graph_caps = -1, graph_vcur = -1;
+
     bool timing = false;
     hipEvent_t ev[EV_COUNT]{};
     k3dom_stats stats{};
@@ -98,9 +108,20 @@ k3dom_status ensure_points(k3dom_handle* h, int m);
 GridGeom grid_geom(const k3dom_handle* h);
 // One filter frame on device-resident points (DOM::updateGrid).
 k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m);
-// Candidate lists of the current points and geometry (point preparation and
-// the three list levels); the gather itself is launched by the caller.
-k3dom_status build_lists(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int m);
+// Candidate lists of the points of frame constants fc (point preparation, the
+// three list levels, work items); the gather itself is launched by the caller.
+k3dom_status build_lists(k3dom_handle* h, const FrameConsts* fc, const float* xs, const float* ys, const float* zs);
+// Pieces of a frame for drivers that interleave several maps: the head
+// (constants upload, prediction, sort, ranges), the part after the gather,
+// overflow handling with re-runs, and the bookkeeping at the end.
+k3dom_status enqueue_head(k3dom_handle* h, bool normal);
+k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal);
+k3dom_status resolve_overflow(k3dom_handle* h, bool normal, const float* dx, const float* dy, const float* dz,
+                              bool done, bool checked);
+void finish_frame(k3dom_handle* h, double t, int m);
+// Host copy of the frame constants of a frame at time t with m points.
+void fill_frame_consts(k3dom_handle* h, FrameConsts* c, double t, int m);
+void drop_graph(k3dom_handle* h);
 // State after construction (priors, no pose, frame count 0).
 k3dom_status reset_map(k3dom_handle* h);
 // DOM_c::updatePose; *shifted receives the map shift in voxels (0 when none).

@@ -46,6 +46,18 @@ struct GridGeom {
     float sx, sy, sz;  // sensor position relative to the map centre
 };
 
+// Values of one frame that kernels read from device memory (so that a
+// captured frame graph replays with each frame's values): written by one
+// host-to-device copy at the start of the frame.
+struct FrameConsts {
+    GridGeom g;          // sensor position relative to the map centre this frame
+    K3PredictConst pc;   // prediction constants (dt, frame, noise, keys)
+    float decay;         // gamma^dt
+    uint32_t frame;      // frame counter: counter word of every random draw
+    int32_t m;           // points of the frame
+    int32_t pad;
+};
+
 // ------------------------------------------------------------------ scans
 int scan_blocks(long long n_max);
 // Exclusive scan of n = n_dev ? (*n_dev) * n_mul : n_max elements; writes the
@@ -54,7 +66,7 @@ hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, ui
                            const uint32_t* n_dev, uint32_t n_mul, uint32_t* total);
 
 // ------------------------------------------------------------------ particles / voxels
-hipError_t launch_predict(hipStream_t st, const K3PredictConst& c, ParticleArrays P, int n);
+hipError_t launch_predict(hipStream_t st, const FrameConsts* fc, ParticleArrays P, int n);
 hipError_t launch_shift_particles(hipStream_t st, ParticleArrays P, int n, int mx, int my, int mz);
 hipError_t launch_shift_voxels(hipStream_t st, VoxelArrays dst, VoxelArrays src, int gs, int gsz, int mx, int my,
                                int mz, float prior_f, float prior_s, float prior_d);
@@ -100,10 +112,11 @@ struct TileGeom {
 };
 constexpr int MAX_CHILDREN = 1024;
 
-hipError_t launch_point_prep(hipStream_t st, const float* xs, const float* ys, const float* zs, int m, float res,
-                             K3PointRec* rec, float4* pxyz);
-hipError_t launch_build_level(hipStream_t st, int level, const GridGeom& g, const TileGeom& tg, const LevelDesc& L,
-                              const LevelBufs& B, const LevelBufs* parent, float ls, const float4* pxyz, int m,
+// m_cap: capacity of the point arrays (the launch covers it; fc->m points are live)
+hipError_t launch_point_prep(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
+                             const float* zs, int m_cap, float res, K3PointRec* rec, float4* pxyz);
+hipError_t launch_build_level(hipStream_t st, int level, const FrameConsts* fc, const TileGeom& tg, const LevelDesc& L,
+                              const LevelBufs& B, const LevelBufs* parent, float ls, const float4* pxyz,
                               DevScalars* sc);
 
 struct ItemBufs {
@@ -117,16 +130,36 @@ struct ItemBufs {
 };
 hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& tiles, const ItemBufs& I,
                               DevScalars* sc);
+// Arguments of the kernel-inference gather over one map.
+struct BgArgs {
+    const FrameConsts* fc;
+    TileGeom tg;
+    float ls, ls2g, kscale, inv_ls, sigma_o, sigma_f;
+    const uint4* items;
+    const uint32_t* tlist;
+    const K3PointRec* rec;
+    float *del_occ, *del_free, *dls_occ, *dls_free, *partial;
+    DevScalars* sc;
+};
+BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float sigma_o, float sigma_f,
+                    const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec, float* del_occ, float* del_free,
+                    float* dls_occ, float* dls_free, DevScalars* sc);
+// The gather over the items of several maps in one persistent launch
+// (maps: device array of nmaps argument sets; prefix: nmaps + 1 entries).
+hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps, int nmaps, uint32_t* prefix,
+                             uint32_t* counter);
+hipError_t launch_reset_item_counter(hipStream_t st, DevScalars* sc);
+
 // sigma_o / sigma_f weight the occupied / free terms (both the reference's
 // single sigma in a plain frame).  Split items need I.partial with 128 floats
 // per item (256 for the _grad variants).
-hipError_t launch_bgki(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
+hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                        float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles, const K3PointRec* rec,
                        float* del_occ, float* del_free, DevScalars* sc);
 hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg, const ItemBufs& I,
                                float* del_occ, float* del_free, DevScalars* sc);
 // The same evidence plus its derivative in the length scale (dls_*).
-hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
+hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles,
                             const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
                             DevScalars* sc);
@@ -141,9 +174,11 @@ hipError_t launch_bgki_count(hipStream_t st, int grid_blocks, const GridGeom& g,
 // Every tail kernel returns at once when *abort_flag is set (a capacity of
 // the kernel-inference stage overflowed; the host grows it and re-runs).
 // heavy: scratch list of V voxel ids (voxels with many particles)
+// decay: fc->decay (1 in the initialisation frame when fc is null)
 hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
-                              const float* weights, const float* del_occ, const float* del_free, float decay,
-                              float mass_scale, float p_B, float* born, int32_t* heavy, DevScalars* sc);
+                              const float* weights, const float* del_occ, const float* del_free,
+                              const FrameConsts* fc, float mass_scale, float p_B, float* born, int32_t* heavy,
+                              DevScalars* sc);
 hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
                                   ParticleArrays P, float min_vel, int32_t* heavy, DevScalars* sc);
 hipError_t launch_fixed_scan(hipStream_t st, const float* a, int na, const float* b, int nb,
@@ -153,14 +188,15 @@ int fixed_scan_blocks(int n);
 hipError_t launch_bounds(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, const uint64_t* total,
                          int V, uint32_t cnt, const float* born, uint32_t* bnd, float* wua);
 hipError_t launch_newborn(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int B, int gs,
-                          const float* wua, ParticleArrays NB, float vmax, float min_vel, uint32_t frame,
+                          const float* wua, ParticleArrays NB, float vmax, float min_vel, const FrameConsts* fc,
                           uint32_t k0, uint32_t k1, DevScalars* sc);
 hipError_t launch_init_particles(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int N,
-                                 int gs, ParticleArrays P, float vmax, float min_vel, uint32_t frame, uint32_t k0,
-                                 uint32_t k1);
+                                 int gs, ParticleArrays P, float vmax, float min_vel, const FrameConsts* fc,
+                                 uint32_t k0, uint32_t k1);
 hipError_t launch_resample(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, int total_len,
                            const uint64_t* total, const int32_t* E, ParticleArrays sorted, ParticleArrays nb,
-                           ParticleArrays out, int N, uint32_t frame, uint32_t k0, uint32_t k1, int32_t* idx_out);
+                           ParticleArrays out, int N, const FrameConsts* fc, uint32_t k0, uint32_t k1,
+                           int32_t* idx_out);
 // ------------------------------------------------------------------ differentiated window (k3_adjoint.hip)
 // Record of one forward frame: masses after the frame, predicted dynamic mass,
 // and the frozen counts per voxel (particles inside the grid, of them not

@@ -44,9 +44,10 @@ static inline unsigned blocks_for(long long n, int per_block = TPB) {
 }
 
 // ============================================================== particles
-__global__ __launch_bounds__(TPB) void k_predict(K3PredictConst c, ParticleArrays P, int n) {
+__global__ __launch_bounds__(TPB) void k_predict(const FrameConsts* __restrict__ fc, ParticleArrays P, int n) {
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
+    const K3PredictConst c = fc->pc;
     float px = P.px[i], py = P.py[i], pz = P.pz[i];
     float vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
     float w = P.w[i];
@@ -57,8 +58,8 @@ __global__ __launch_bounds__(TPB) void k_predict(K3PredictConst c, ParticleArray
     P.w[i] = w;
     P.key[i] = key;
 }
-hipError_t launch_predict(hipStream_t st, const K3PredictConst& c, ParticleArrays P, int n) {
-    hipLaunchKernelGGL(k_predict, dim3(blocks_for(n)), dim3(TPB), 0, st, c, P, n);
+hipError_t launch_predict(hipStream_t st, const FrameConsts* fc, ParticleArrays P, int n) {
+    hipLaunchKernelGGL(k_predict, dim3(blocks_for(n)), dim3(TPB), 0, st, fc, P, n);
     return hipGetLastError();
 }
 
@@ -472,10 +473,12 @@ __device__ __forceinline__ float mass_voxel(const VoxelArrays& v, int c, float d
 __global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict__ abort_flag, VoxelArrays v, int V,
                                                      const float* __restrict__ weights,
                                                      const float* __restrict__ del_occ,
-                                                     const float* __restrict__ del_free, float decay,
-                                                     float mass_scale, float p_B, float* __restrict__ born,
-                                                     int32_t* __restrict__ heavy, DevScalars* sc) {
+                                                     const float* __restrict__ del_free,
+                                                     const FrameConsts* __restrict__ fc, float mass_scale, float p_B,
+                                                     float* __restrict__ born, int32_t* __restrict__ heavy,
+                                                     DevScalars* sc) {
     if (*abort_flag) return;
+    const float decay = fc ? fc->decay : 1.0f;
     int c = blockIdx.x * TPB + threadIdx.x;
     float bmax = 0.0f;
     if (c < V) {
@@ -496,10 +499,12 @@ __global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict_
 __global__ __launch_bounds__(TPB) void k_mass_update_heavy(const uint32_t* __restrict__ abort_flag, VoxelArrays v,
                                                            const float* __restrict__ weights,
                                                            const float* __restrict__ del_occ,
-                                                           const float* __restrict__ del_free, float decay,
-                                                           float mass_scale, float p_B, float* __restrict__ born,
+                                                           const float* __restrict__ del_free,
+                                                           const FrameConsts* __restrict__ fc, float mass_scale,
+                                                           float p_B, float* __restrict__ born,
                                                            const int32_t* __restrict__ heavy, DevScalars* sc) {
     if (*abort_flag) return;
+    const float decay = fc ? fc->decay : 1.0f;
     const int lane = threadIdx.x & 63;
     const uint32_t n = sc->n_heavy_mass;
     for (uint32_t h = (blockIdx.x * TPB + threadIdx.x) >> 6; h < n; h += gridDim.x * (TPB / 64)) {
@@ -513,12 +518,13 @@ __global__ __launch_bounds__(TPB) void k_mass_update_heavy(const uint32_t* __res
 }
 
 hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
-                              const float* weights, const float* del_occ, const float* del_free, float decay,
-                              float mass_scale, float p_B, float* born, int32_t* heavy, DevScalars* sc) {
+                              const float* weights, const float* del_occ, const float* del_free,
+                              const FrameConsts* fc, float mass_scale, float p_B, float* born, int32_t* heavy,
+                              DevScalars* sc) {
     hipLaunchKernelGGL(k_mass_update, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, weights, del_occ,
-                       del_free, decay, mass_scale, p_B, born, heavy, sc);
+                       del_free, fc, mass_scale, p_B, born, heavy, sc);
     hipLaunchKernelGGL(k_mass_update_heavy, dim3(HEAVY_GRID), dim3(TPB), 0, st, abort_flag, v, weights, del_occ,
-                       del_free, decay, mass_scale, p_B, born, heavy, sc);
+                       del_free, fc, mass_scale, p_B, born, heavy, sc);
     return hipGetLastError();
 }
 
@@ -812,9 +818,10 @@ __device__ __forceinline__ int slot_owner(const uint32_t* __restrict__ bnd, int 
 
 __global__ __launch_bounds__(TPB) void k_newborn(const uint32_t* __restrict__ abort_flag, const uint32_t* __restrict__ bnd, int V, int B, int gs,
                                                  const float* __restrict__ wua, ParticleArrays NB, float vmax,
-                                                 float min_vel, uint32_t frame, uint32_t k0, uint32_t k1,
-                                                 DevScalars* sc) {
+                                                 float min_vel, const FrameConsts* __restrict__ fc, uint32_t k0,
+                                                 uint32_t k1, DevScalars* sc) {
     if (*abort_flag) return;
+    const uint32_t frame = fc->frame;
     int k = blockIdx.x * TPB + threadIdx.x;
     float wmax = 0.0f;
     if (k < B) {
@@ -837,20 +844,21 @@ __global__ __launch_bounds__(TPB) void k_newborn(const uint32_t* __restrict__ ab
     if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
 }
 hipError_t launch_newborn(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int B, int gs, const float* wua,
-                          ParticleArrays NB, float vmax, float min_vel, uint32_t frame, uint32_t k0, uint32_t k1,
-                          DevScalars* sc) {
+                          ParticleArrays NB, float vmax, float min_vel, const FrameConsts* fc, uint32_t k0,
+                          uint32_t k1, DevScalars* sc) {
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_newborn, dim3(blocks_for(B)), dim3(TPB), 0, st, abort_flag, bnd, V, B, gs, wua, NB, vmax, min_vel,
-                       frame, k0, k1, sc);
+                       fc, k0, k1, sc);
     return hipGetLastError();
 }
 
 __global__ __launch_bounds__(TPB) void k_init_particles(const uint32_t* __restrict__ abort_flag, const uint32_t* __restrict__ bnd, int V, int N, int gs,
                                                         ParticleArrays P, float vmax, float min_vel,
-                                                        uint32_t frame, uint32_t k0, uint32_t k1) {
+                                                        const FrameConsts* __restrict__ fc, uint32_t k0, uint32_t k1) {
     if (*abort_flag) return;
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= N) return;
+    const uint32_t frame = fc->frame;
     int c = slot_owner(bnd, V, (uint32_t)i);
     uint32_t r[4];
     k3_philox4x32_10((uint32_t)i, frame, K3_STREAM_INIT, 0u, k0, k1, r);
@@ -864,9 +872,9 @@ __global__ __launch_bounds__(TPB) void k_init_particles(const uint32_t* __restri
     P.key[i] = c;
 }
 hipError_t launch_init_particles(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int N, int gs, ParticleArrays P,
-                                 float vmax, float min_vel, uint32_t frame, uint32_t k0, uint32_t k1) {
+                                 float vmax, float min_vel, const FrameConsts* fc, uint32_t k0, uint32_t k1) {
     hipLaunchKernelGGL(k_init_particles, dim3(blocks_for(N)), dim3(TPB), 0, st, abort_flag, bnd, V, N, gs, P, vmax, min_vel,
-                       frame, k0, k1);
+                       fc, k0, k1);
     return hipGetLastError();
 }
 
@@ -876,9 +884,11 @@ hipError_t launch_init_particles(hipStream_t st, const uint32_t* abort_flag, con
 __global__ __launch_bounds__(TPB) void k_resample(const uint32_t* __restrict__ abort_flag, const uint64_t* __restrict__ cdf, int L,
                                                   const uint64_t* __restrict__ total,
                                                   const int32_t* __restrict__ Ep, ParticleArrays S,
-                                                  ParticleArrays NB, ParticleArrays O, int N, uint32_t frame,
-                                                  uint32_t k0, uint32_t k1, int32_t* __restrict__ idx_out) {
+                                                  ParticleArrays NB, ParticleArrays O, int N,
+                                                  const FrameConsts* __restrict__ fc, uint32_t k0, uint32_t k1,
+                                                  int32_t* __restrict__ idx_out) {
     if (*abort_flag) return;
+    const uint32_t frame = fc->frame;
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= N) return;
     const uint64_t Q = *total;
@@ -918,9 +928,9 @@ __global__ __launch_bounds__(TPB) void k_resample(const uint32_t* __restrict__ a
 }
 hipError_t launch_resample(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, int total_len, const uint64_t* total,
                            const int32_t* E, ParticleArrays sorted, ParticleArrays nb, ParticleArrays out, int N,
-                           uint32_t frame, uint32_t k0, uint32_t k1, int32_t* idx_out) {
+                           const FrameConsts* fc, uint32_t k0, uint32_t k1, int32_t* idx_out) {
     hipLaunchKernelGGL(k_resample, dim3(blocks_for(N)), dim3(TPB), 0, st, abort_flag, cdf, total_len, total, E, sorted, nb, out,
-                       N, frame, k0, k1, idx_out);
+                       N, fc, k0, k1, idx_out);
     return hipGetLastError();
 }
 

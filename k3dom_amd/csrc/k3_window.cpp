@@ -51,6 +51,8 @@ struct k3dom_window {
     double* lam[2][3] = {};
     double* acc = nullptr;
     double* adj_partial = nullptr;
+    FrameConsts* d_fc = nullptr;   // T: the forward frames' constants (device)
+    FrameConsts* h_fc = nullptr;   // T: pinned host copies
     float theta[6] = {0, 0, 0, 0, 0, 0};
     bool forward_done = false;
     float ms[2] = {0, 0};
@@ -142,6 +144,8 @@ k3dom_status k3dom_window_create(const k3dom_params* params, int32_t frames, k3d
         for (int k = 0; k < 3; ++k) WC(walloc(w, &w->lam[b][k], V));
     WC(walloc(w, &w->acc, 8));
     WC(walloc(w, &w->adj_partial, (size_t)window_blocks(w->V) * 8));
+    WC(walloc(w, &w->d_fc, (size_t)frames));
+    WC(hipHostMalloc((void**)&w->h_fc, sizeof(FrameConsts) * frames, hipHostMallocDefault));
 #undef WC
     *out = w;
     return K3DOM_OK;
@@ -149,6 +153,8 @@ k3dom_status k3dom_window_create(const k3dom_params* params, int32_t frames, k3d
 
 k3dom_status k3dom_window_destroy(k3dom_window* w) {
     if (!w) return K3DOM_OK;
+    if (w->h) (void)hipStreamSynchronize(w->h->stream);
+    if (w->h_fc) (void)hipHostFree(w->h_fc);
     k3dom_destroy(w->h);  // frees every device buffer of the window too
     delete w;
     return K3DOM_OK;
@@ -236,6 +242,7 @@ k3dom_status k3dom_window_forward(k3dom_window* w, const float theta[6], double*
         if ((s = from_handle(w, ensure_points(h, std::max(fi.n, 1)))) != K3DOM_OK) return s;
         const float* px = w->d_pts + fi.off;
         if ((s = from_handle(w, frame(h, fi.t, px, px + fi.n, px + 2 * fi.n, fi.n))) != K3DOM_OK) return s;
+        w->h_fc[f] = *h->h_fc;
         W_HIP(w, launch_window_record(h->stream, w->V, h->gs, h->gsz, fo.normal, h->p.particle_min_vel,
                                       h->vox[h->vcur], h->P[1], h->NB, h->bnd, w->rec[f], w->d_gt + (size_t)f * w->V,
                                       w->use_mask ? w->d_mask : nullptr, w->loss_partial + (size_t)f * nb));
@@ -263,18 +270,15 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
     for (int f = w->T - 1; f >= 0; --f) {
         const FrameIn& fi = w->in[f];
         const FrameOut& fo = w->out[f];
-        // evidence of frame f and its length-scale derivative, from the frame's geometry
-        for (int k = 0; k < 3; ++k) {
-            h->sensor[k] = fo.sensor[k];
-            h->center[k] = fo.center[k];
-        }
+        // evidence of frame f and its length-scale derivative, from the frame's constants
+        const FrameConsts* fc = w->d_fc + f;
+        W_HIP(w, hipMemcpyAsync(w->d_fc + f, w->h_fc + f, sizeof(FrameConsts), hipMemcpyHostToDevice, h->stream));
         W_HIP(w, launch_reset_scalars(h->stream, h->sc));
         const float* px = w->d_pts + fi.off;
-        if ((s = from_handle(w, build_lists(h, px, px + fi.n, px + 2 * fi.n, fi.n))) != K3DOM_OK) return s;
-        W_HIP(w, launch_bgki_grad(h->stream, h->num_cus * 8, h->last_geom, h->tg, h->p.ls, h->sigma_occ,
-                                  h->sigma_free, h->ibuf, h->lbuf[2], h->rec, w->del_o, w->del_f, w->dls_o, w->dls_f,
-                                  h->sc));
-        W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, h->last_geom, h->tg, h->ibuf, w->del_o, w->del_f,
+        if ((s = from_handle(w, build_lists(h, fc, px, px + fi.n, px + 2 * fi.n))) != K3DOM_OK) return s;
+        W_HIP(w, launch_bgki_grad(h->stream, h->gather_blocks, fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
+                                  h->ibuf, h->lbuf[2], h->rec, w->del_o, w->del_f, w->dls_o, w->dls_f, h->sc));
+        W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_fc[f].g, h->tg, h->ibuf, w->del_o, w->del_f,
                                           w->dls_o, w->dls_f, h->sc));
         AdjointFrameArgs a{};
         a.V = w->V;

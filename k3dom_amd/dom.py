@@ -47,11 +47,17 @@ def _fptr(a):
 class DOM:
     """One map on the current HIP device."""
 
-    def __init__(self, params):
+    def __init__(self, params, _handle=None, _owner=None):
         self._lib = _lib.load()
         self.params = params
-        h = C.c_void_p()
-        check(self._lib.k3dom_create(C.byref(params), C.byref(h)), None, "k3dom_create")
+        if _handle is None:
+            h = C.c_void_p()
+            check(self._lib.k3dom_create(C.byref(params), C.byref(h)), None, "k3dom_create")
+            self._owned = True
+        else:  # a map owned by a batch (kept alive through _owner)
+            h = _handle
+            self._owned = False
+            self._owner = _owner
         self._h = h
         info = self.info()
         self.grid_size = info.grid_size
@@ -62,7 +68,8 @@ class DOM:
 
     def close(self):
         if getattr(self, "_h", None):
-            self._lib.k3dom_destroy(self._h)
+            if getattr(self, "_owned", True):
+                self._lib.k3dom_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -159,6 +166,17 @@ class DOM:
 
     def set_timing(self, on=True):
         check(self._lib.k3dom_set_timing(self._h, 1 if on else 0), self._h, "set_timing")
+
+    def autotune(self):
+        """Times the gather over the last frame's items for several grid sizes and
+        keeps the fastest; returns the chosen blocks per compute unit."""
+        c = C.c_int32()
+        check(self._lib.k3dom_autotune(self._h, C.byref(c)), self._h, "autotune")
+        return c.value
+
+    def set_graph(self, on=True):
+        """Replay each normal frame as one captured HIP graph (same results)."""
+        check(self._lib.k3dom_set_graph(self._h, 1 if on else 0), self._h, "set_graph")
 
     STAGES = ["predict", "sort", "lists", "bgki", "mass", "persist", "newborn", "moments", "resample", "frame",
               "bgki_kernel"]

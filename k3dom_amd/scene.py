@@ -177,6 +177,27 @@ def workload(cfg):
     raise ValueError(f"unknown configuration {cfg}")
 
 
+def batch_workloads(n_tiles=32, seed=4, edge_min=32, edge_max=128, particles_min=16_384, particles_max=1_048_576,
+                    res=0.1, beams=32, az=1024, frames=50):
+    """The batched-tiles configuration: n_tiles independent cubic maps with
+    edges log-uniform in [edge_min, edge_max] voxels at res, particle counts
+    log-uniform in [particles_min, particles_max] (+10% newborn), each with its
+    own box scene around a static ego and a beams x az LiDAR."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_tiles):
+        edge = int(np.exp(rng.uniform(np.log(edge_min), np.log(edge_max + 1))))
+        edge = max(edge_min, min(edge_max, edge))
+        n = int(np.exp(rng.uniform(np.log(particles_min), np.log(particles_max))))
+        p = k3dom_params(res, edge, edge, n, max(1, n // 10), seed=123456 + i)
+        half = edge * res / 2
+        s = _desc(seed=1000 * seed + i, n_static=max(4, int(edge * edge * res * res / 4)), n_moving=max(1, edge // 32),
+                  speed_max=2.0, xmin=-half, xmax=half, ymin=-half, ymax=half, ground_z=-half * 0.3,
+                  clearance=0.5, beams=beams, az_bins=az, max_range=1.8 * half, range_noise=0.03)
+        out.append(Workload(f"tile {i} {edge}^3@{res}m N={n}", p, s, frames))
+    return out
+
+
 def small_workload(edge=16, edge_z=8, res=0.2, particles=4096, newborn=409, beams=8, az=128, frames=6, seed=11,
                    moving=2, static=6, ego_vx=0.0, **over):
     """A tiny scene for fast parity tests."""

@@ -1,0 +1,98 @@
+"""Batched maps: every map of a batch (joint gather in one persistent launch,
+maps' stages on concurrent streams, optionally one captured graph per frame)
+gives exactly the results of the same map updated alone."""
+import numpy as np
+import pytest
+
+from tests._parity import FRAME_ARRAYS, PARTICLE_ARRAYS, bits_equal, compare_cells, first_diff
+
+
+def _tiles(n=4, seed=7, frames=4):
+    from k3dom_amd.scene import batch_workloads
+    return batch_workloads(n_tiles=n, seed=seed, edge_min=12, edge_max=40, particles_min=2048,
+                           particles_max=40_000, res=0.2, beams=8, az=256, frames=frames)
+
+
+def test_batch_workloads_are_uneven_and_deterministic():
+    from k3dom_amd.scene import batch_workloads
+    a = batch_workloads()
+    b = batch_workloads()
+    assert len(a) == 32
+    sizes = {w.params.size for w in a}
+    counts = {w.params.particle_count for w in a}
+    assert len(sizes) > 8 and len(counts) > 16
+    assert [w.params.particle_count for w in a] == [w.params.particle_count for w in b]
+    assert all(16_384 <= w.params.particle_count <= 1_048_576 for w in a)
+
+
+def _compare_maps(batch, solos, normal, what):
+    for i, (bm, sm) in enumerate(zip(batch.maps, solos)):
+        compare_cells(bm.copyCellsToHost(), sm.copyCellsToHost(), f"{what} map {i}")
+        names = PARTICLE_ARRAYS + (FRAME_ARRAYS if normal else [])
+        for name in names:
+            a, b = bm.read_array(name), sm.read_array(name)
+            assert bits_equal(a, b), f"{what} map {i} {name}: {first_diff(a, b)}"
+
+
+def _run(graph, tiny, monkeypatch, autotune=False):
+    from k3dom_amd.batch import Batch
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene
+    if tiny:
+        monkeypatch.setenv("K3DOM_TEST_TINY_CAPACITY", "1")
+    ws = _tiles()
+    scenes = [Scene(w.scene) for w in ws]
+    batch = Batch([w.params for w in ws])
+    if graph:
+        batch.set_graph(True)
+    solos = [DOM(w.params) for w in ws]
+    for f in range(ws[0].frames):
+        recs = [s.frame(f) for s in scenes]
+        t = recs[0][0]
+        print(f"  batch frame {f}: points {[len(r[2]) for r in recs]}", flush=True)
+        normal = batch.maps[0].info().updated_time > 0
+        batch.update(t, [r[1] for r in recs], [r[2:] for r in recs])
+        for sm, r in zip(solos, recs):
+            sm.updatePose(*r[1])
+            sm.updateGrid(t, *r[2:])
+        _compare_maps(batch, solos, normal, f"frame {f}")
+        if autotune and f == 1:
+            bpc = batch.autotune()
+            assert bpc in (2, 4, 6, 8, 10, 12, 16)
+    return batch, solos
+
+
+@pytest.mark.gpu
+def test_batch_equals_solo_maps(gpu, monkeypatch):
+    _run(graph=False, tiny=False, monkeypatch=monkeypatch, autotune=True)
+
+
+@pytest.mark.gpu
+def test_batch_graph_equals_solo_maps(gpu, monkeypatch):
+    _run(graph=True, tiny=False, monkeypatch=monkeypatch)
+
+
+@pytest.mark.gpu
+def test_batch_capacity_regrowth_equals_solo_maps(gpu, monkeypatch):
+    batch, _ = _run(graph=True, tiny=True, monkeypatch=monkeypatch)
+    assert any(m.stats()["capacity_regrowths"] > 0 for m in batch.maps)
+
+
+@pytest.mark.gpu
+def test_single_map_autotune_keeps_results(gpu):
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, small_workload
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_state
+    w = small_workload(frames=5, seed=13)
+    sc = Scene(w.scene)
+    gd, od = DOM(w.params), OracleDOM(w.params)
+    for f in range(w.frames):
+        t, pose, xs, ys, zs = sc.frame(f)
+        normal = gd.info().updated_time > 0
+        for m in (gd, od):
+            m.updatePose(*pose)
+            m.updateGrid(t, xs, ys, zs)
+        if f == 1:
+            assert gd.autotune() in (2, 4, 6, 8, 10, 12, 16)
+        compare_state(gd, od, normal_frame=normal, what=f"autotuned frame {f}:")

@@ -92,3 +92,25 @@ def test_very_crowded_voxels_bit_exact(gpu):
     occ = cells["start_idx"] >= 0
     counts = cells["end_idx"][occ] - cells["start_idx"][occ] + 1
     assert counts.max() > 64 * 64
+
+
+def test_graph_replay_is_bit_exact(gpu, monkeypatch):
+    """Frames replayed as a captured graph (re-captured after map shifts and
+    capacity regrowth) equal the serial restatement bit for bit."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, small_workload
+    from oracle.orc import OracleDOM
+    monkeypatch.setenv("K3DOM_TEST_TINY_CAPACITY", "1")
+    w = small_workload(ego_vx=4.0, frames=8, seed=31)
+    sc = Scene(w.scene)
+    gd, od = DOM(w.params), OracleDOM(w.params)
+    gd.set_graph(True)
+    for f in range(w.frames):
+        t, pose, xs, ys, zs = sc.frame(f)
+        print(f"  graph frame {f}: {len(xs)} points", flush=True)
+        normal = gd.info().updated_time > 0
+        for m in (gd, od):
+            m.updatePose(*pose)
+            m.updateGrid(t, xs, ys, zs)
+        compare_state(gd, od, normal_frame=normal, what=f"graph frame {f}:")
+    assert gd.stats()["capacity_regrowths"] >= 1
