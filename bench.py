@@ -12,8 +12,9 @@ kernel-inference gather, whose algorithmic work is its contributing
 (voxel, point) pairs; `frame_hbm` states the frame's compulsory HBM traffic
 against its time; `adjoint` times the differentiated window (forward +
 backward ms/frame against the plain forward, config-1 scene, ls 0.3, 8-frame
-windows); `cpu_baseline` times the serial restatement on a bounded sample of
-the same workload on the host.
+windows); `batch` times 32 independent uneven map tiles advanced together;
+`cpu_baseline` times the serial restatement on a bounded sample of the same
+workload on the host.
 """
 import argparse
 import json
@@ -115,6 +116,61 @@ def adjoint_leg(torch, reps=5):
             "grad": [float(x) for x in g], "theta": [float(x) for x in th]}
 
 
+def batch_leg(torch, warmup=3, steps=8):
+    """Batched tiles: 32 independent cubic maps (edges log-uniform 32-128
+    voxels at 0.1 m, 16k-1M particles, 32x1024 LiDAR each).  Frames/s of the
+    whole batch (every map advanced one frame) for: the maps updated one after
+    another, the batch (joint persistent gather, concurrent map streams), and
+    the batch replayed as one captured graph per frame; the joint gather's
+    launch shape autotuned after the warm-up."""
+    from k3dom_amd.batch import Batch
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, batch_workloads
+    ws = batch_workloads()
+    scenes = [Scene(w.scene) for w in ws]
+    F = warmup + steps
+    frames = [[s.frame(f) for s in scenes] for f in range(F)]
+    res = {"workload": "32 tiles, edges 32-128 voxels @0.1m, 16k-1M particles, 32x1024 LiDAR",
+           "voxels": int(sum(int(np.float32(w.params.size) / np.float32(w.params.resolution)) ** 3 for w in ws)),
+           "particles": int(sum(w.params.particle_count for w in ws)),
+           "points_per_frame": int(np.mean([sum(len(r[2]) for r in fr) for fr in frames])),
+           "warmup": warmup, "steps": steps}
+
+    def timed(step_fn):
+        for f in range(warmup):
+            step_fn(f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(warmup, F):
+            step_fn(f)
+        torch.cuda.synchronize()
+        return steps / (time.perf_counter() - t0)
+
+    solos = [DOM(w.params) for w in ws]
+
+    def solo_step(f):
+        for d, r in zip(solos, frames[f]):
+            d.updatePose(*r[1])
+            d.updateGrid(r[0], *r[2:])
+    res["sequential_frames_per_s"] = round(timed(solo_step), 3)
+    for d in solos:
+        d.close()
+    del solos
+    for mode in ("batch", "batch_graph"):
+        b = Batch([w.params for w in ws])
+        if mode == "batch_graph":
+            b.set_graph(True)
+
+        def bstep(f):
+            fr = frames[f]
+            b.update(fr[0][0], [r[1] for r in fr], [r[2:] for r in fr])
+            if f == warmup - 1:
+                res["autotuned_blocks_per_cu"] = b.autotune()
+        res[mode + "_frames_per_s"] = round(timed(bstep), 3)
+        b.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +180,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="skip the pair-count pass")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiated-window leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-tiles leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -234,6 +291,11 @@ def main():
         adj = adjoint_leg(torch)
         log(f"differentiated window: fwd+bwd / plain = {adj['fwd_bwd_vs_plain']}")
 
+    bat = None
+    if not args.no_batch:
+        bat = batch_leg(torch)
+        log(f"batched tiles: {bat}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(w, frames)
@@ -259,6 +321,7 @@ def main():
         "roofline": roof,
         "frame_hbm": frame_hbm,
         "adjoint": adj,
+        "batch": bat,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
