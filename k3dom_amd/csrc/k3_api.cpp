@@ -46,7 +46,6 @@ k3dom_status validate(const k3dom_params* p, std::string* why) {
     if ((double)gs * gs * gsz > 2147483647.0 / 2) { *why = "grid too large"; return K3DOM_ERR_INVALID_ARG; }
     if (p->particle_count <= 0) { *why = "particle_count must be > 0"; return K3DOM_ERR_INVALID_ARG; }
     if (p->new_born_particle_count < 0) { *why = "new_born_particle_count must be >= 0"; return K3DOM_ERR_INVALID_ARG; }
-    // the gather's square root relies on ls >= 2^-22 (see bg_sqrt)
     if (!(p->ls >= 1e-6f) || !std::isfinite(p->ls)) { *why = "ls must be >= 1e-6 and finite"; return K3DOM_ERR_INVALID_ARG; }
     if (!std::isfinite(p->sigma)) { *why = "sigma must be finite"; return K3DOM_ERR_INVALID_ARG; }
     if (!(p->mass_scale != 0.0f)) { *why = "mass_scale must be non-zero"; return K3DOM_ERR_INVALID_ARG; }

@@ -45,25 +45,6 @@ __device__ __forceinline__ void bg_wave_sync() {
 }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Correctly rounded square root of max(x, 2^-96) for x >= 0: the hardware
-// estimate corrected by one unit in the last place from the signs of the two
-// neighbours' residuals (exact for arguments >= 2^-96).  Only kernel values
-// are taken of it: for x < 2^-96 both this result and sqrtf(x) are <= 2^-48,
-// and fmaf(d, k3_kernel_scale(ls), -1) rounds to -1 for every d <= 2^-48
-// when ls >= 2^-22 (the handle rejects smaller length scales), so the
-// kernel value equals that of sqrtf(x).
-__device__ __forceinline__ float bg_sqrt(float x) {
-    x = fmaxf(x, 0x1p-96f);
-    const float y = __builtin_amdgcn_sqrtf(x);
-    const uint32_t yb = __builtin_bit_cast(uint32_t, y);
-    const float ym = __builtin_bit_cast(float, yb - 1u);
-    const float yp = __builtin_bit_cast(float, yb + 1u);
-    float r = y;
-    r = fmaf(-ym, y, x) <= 0.0f ? ym : r;
-    r = fmaf(-yp, y, x) > 0.0f ? yp : r;
-    return r;
-}
-
 // ============================================================== points
 __global__ __launch_bounds__(TPB) void k_point_prep(const FrameConsts* __restrict__ fc,
                                                     const float* __restrict__ xs, const float* __restrict__ ys,
@@ -446,18 +427,38 @@ hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& T, co
 // plus F_OCC when a voxel of the tile may lie within ls of the point.
 constexpr uint32_t C_GENERAL = 0, C_MAIN = 1, C_BEHIND = 2, C_NOFREE = 3, F_OCC = 4;
 
-// Kernel term and, when GRAD, its derivative in the length scale.
+// Kernel term of squared distance a2 (the table of k3_kernel_table.h, staged
+// in LDS as float4 entries) and, when GRAD, its derivative in the length scale.
 template <bool GRAD>
 struct BgTerm {
     float v, g;
 };
+// The table in LDS as three arrays (a, b, c coefficients): three 4-byte
+// reads per lookup, lanes in different banks unless their indices differ by a
+// multiple of the bank count.
+struct BgTable {
+    float a[K3_KTAB_N + 1], b[K3_KTAB_N + 1], c[K3_KTAB_N + 1];
+};
 template <bool GRAD>
-__device__ __forceinline__ BgTerm<GRAD> bg_term(float d, float kscale, float sigma, float ls, float inv_ls) {
+__device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float sigma, float ls, float inv_ls,
+                                                const BgTable* stab) {
     BgTerm<GRAD> t;
-    t.v = k3_kernel_value(d, kscale, sigma);
+    float f;
+    const int i = k3_kernel_index(a2, kscale, &f);
+    t.v = k3_kernel_poly(stab->a[i], stab->b[i], stab->c[i], f, sigma);
     t.g = 0.0f;
-    if (GRAD) t.g = t.v > 0.0f ? k3_kernel_dls(d, ls, inv_ls, sigma) : 0.0f;
+    if (GRAD) t.g = t.v > 0.0f ? k3_kernel_dls(sqrtf(a2), ls, inv_ls, sigma) : 0.0f;
     return t;
+}
+
+// Copies the kernel table into LDS (every thread of the block takes part).
+__device__ __forceinline__ void bg_load_table(BgTable* stab) {
+    for (int i = threadIdx.x; i <= K3_KTAB_N; i += TPB) {
+        stab->a[i] = k3_ktab[4 * i];
+        stab->b[i] = k3_ktab[4 * i + 1];
+        stab->c[i] = k3_ktab[4 * i + 2];
+    }
+    __syncthreads();
 }
 
 // Partial sums of a split item: occupied, free (and, when GRAD, their
@@ -470,7 +471,7 @@ constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 template <bool GRAD>
 __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, int lane,
                                           float4 (&srec)[BG_WAVES][64 * 3], uint32_t (&sidx)[BG_WAVES][64],
-                                          uint32_t (&scls)[BG_WAVES][64]) {
+                                          uint32_t (&scls)[BG_WAVES][64], const BgTable* stab) {
     const GridGeom g = a.fc->g;
     const uint4 item = a.items[it];
     const uint32_t tile = uniform(item.x), ib = uniform(item.y), ie = uniform(item.z), slot = uniform(item.w);
@@ -480,11 +481,10 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
     const float vx = axis_center(i, g.gs, g.res, g.sx);
     const float vy = axis_center(j, g.gs, g.res, g.sy);
     const float vz = axis_center(k, g.gsz, g.res, g.sz);
-    const float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
-    const float dc2 = dc * dc;
+    const float dc2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
     // free term of a voxel for every point in front of which it lies
-    // (k3_kernel_value is an exact zero for distances >= ls)
-    const BgTerm<GRAD> kc = bg_term<GRAD>(dc, a.kscale, a.sigma_f, a.ls, a.inv_ls);
+    // (the kernel is an exact zero from distance ls on)
+    const BgTerm<GRAD> kc = bg_term<GRAD>(dc2, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
     // bounding box of the tile's 64 voxel centres (lanes outside the grid included)
     const float tcx = 0.5f * (axis_center(tx * 4, g.gs, g.res, g.sx) + axis_center(tx * 4 + 3, g.gs, g.res, g.sx));
     const float tcy = 0.5f * (axis_center(ty * 4, g.gs, g.res, g.sy) + axis_center(ty * 4 + 3, g.gs, g.res, g.sy));
@@ -535,7 +535,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
                 const bool co = so < a.ls2g;
                 if (__any(co)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(so), a.kscale, a.sigma_o, a.ls, a.inv_ls);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(so, a.kscale, a.sigma_o, a.ls, a.inv_ls, stab);
                     cur_o += t.v;
                     if (GRAD) gsum_o += t.g;
                 }
@@ -546,7 +546,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
                 const bool cf = a2 < a.ls2g;
                 if (__any(cf)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(a2), a.kscale, a.sigma_f, a.ls, a.inv_ls);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(a2, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
                     cur_f += t.v;
                     if (GRAD) gsum_f += t.g;
                 }
@@ -566,7 +566,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 const float aa = c2 ? a2 : a3;
                 const bool cs = (c2 || c3) && aa < a.ls2g;
                 if (__any(cs)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(bg_sqrt(aa), a.kscale, a.sigma_f, a.ls, a.inv_ls);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(aa, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
                     cur_f += cs ? t.v : (c1 ? kc.v : 0.0f);
                     if (GRAD) gsum_f += cs ? t.g : (c1 ? kc.g : 0.0f);
                 } else {
@@ -605,8 +605,11 @@ __global__ __launch_bounds__(TPB) void k_bgki(BgArgs a) {
     __shared__ float4 srec[BG_WAVES][64 * 3];
     __shared__ uint32_t sidx[BG_WAVES][64];
     __shared__ uint32_t scls[BG_WAVES][64];
+    __shared__ BgTable stab_s;
+    BgTable* stab = &stab_s;
     DevScalars* sc = a.sc;
     if (sc->overflow) return;
+    bg_load_table(stab);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n_items = sc->n_light + sc->n_heavy;
     while (true) {
@@ -614,7 +617,7 @@ __global__ __launch_bounds__(TPB) void k_bgki(BgArgs a) {
         if (lane == 0) it = atomicAdd(&sc->item_counter, 1u);
         it = uniform(__shfl(it, 0));
         if (it >= n_items) break;
-        bgki_item<GRAD>(a, it, w, lane, srec, sidx, scls);
+        bgki_item<GRAD>(a, it, w, lane, srec, sidx, scls, stab);
     }
 }
 
@@ -680,6 +683,9 @@ __global__ __launch_bounds__(TPB) void k_bgki_batch(const BgArgs* __restrict__ m
     __shared__ float4 srec[BG_WAVES][64 * 3];
     __shared__ uint32_t sidx[BG_WAVES][64];
     __shared__ uint32_t scls[BG_WAVES][64];
+    __shared__ BgTable stab_s;
+    BgTable* stab = &stab_s;
+    bg_load_table(stab);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t total = prefix[nmaps];
     while (true) {
@@ -694,7 +700,7 @@ __global__ __launch_bounds__(TPB) void k_bgki_batch(const BgArgs* __restrict__ m
             else hi = mid - 1;
         }
         const BgArgs a = maps[lo];
-        bgki_item<false>(a, it - prefix[lo], w, lane, srec, sidx, scls);
+        bgki_item<false>(a, it - prefix[lo], w, lane, srec, sidx, scls, stab);
     }
 }
 
@@ -763,7 +769,7 @@ hipError_t launch_bgki_combine_grad(hipStream_t st, int ntiles, const GridGeom& 
 }
 
 // Pair counting over the same items (diagnostic): one wave per item.
-__global__ __launch_bounds__(TPB) void k_bgki_count(GridGeom g, TileGeom tg, float ls, float ls2g,
+__global__ __launch_bounds__(TPB) void k_bgki_count(GridGeom g, TileGeom tg, float kscale,
                                                     const uint4* __restrict__ items, uint32_t n_items,
                                                     const uint32_t* __restrict__ tlist,
                                                     const K3PointRec* __restrict__ rec,
@@ -779,12 +785,11 @@ __global__ __launch_bounds__(TPB) void k_bgki_count(GridGeom g, TileGeom tg, flo
         const float vx = axis_center(i, g.gs, g.res, g.sx);
         const float vy = axis_center(j, g.gs, g.res, g.sy);
         const float vz = axis_center(k, g.gsz, g.res, g.sz);
-        const float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
-        const float dc2 = dc * dc;
+        const float dc2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
         for (uint32_t e = item.y; e < item.z; ++e) {
             const K3PointRec r = rec[tlist[e]];
-            co += k3_occ_distance(vx, vy, vz, r, ls, ls2g) >= 0.0f;
-            cf += k3_free_distance(vx, vy, vz, dc, dc2, r, ls, ls2g) >= 0.0f;
+            co += k3_occ_dist2(vx, vy, vz, r, kscale) >= 0.0f;
+            cf += k3_free_dist2(vx, vy, vz, dc2, r, kscale) >= 0.0f;
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -799,7 +804,7 @@ __global__ __launch_bounds__(TPB) void k_bgki_count(GridGeom g, TileGeom tg, flo
 hipError_t launch_bgki_count(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
                              const ItemBufs& I, uint32_t n_items, const LevelBufs& T, const K3PointRec* rec,
                              unsigned long long* counts) {
-    hipLaunchKernelGGL(k_bgki_count, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, ls, k3_ls2_guard(ls), I.items,
+    hipLaunchKernelGGL(k_bgki_count, dim3(grid_blocks), dim3(TPB), 0, st, g, tg, k3_kernel_scale(ls), I.items,
                        n_items, T.list, rec, counts);
     return hipGetLastError();
 }

@@ -193,56 +193,35 @@ K3_HD void k3_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) {
 // ---------------------------------------------------------------- kernel inference
 // Sparse kernel of src/k3dom/kernel/mass_update.cu:24-26:
 //   k(d) = max(0, sigma * ((2 + cospi(2d/ls)) (1 - d/ls) / 3 + sinpi(2d/ls) / (2*3.141592)))
-// (with the reference's truncated pi).  With q = d / ls, the bracket is a
-// smooth function of q on [0, 1]; it is evaluated as a degree-14 polynomial in
-// x = 2q - 1 interpolated at Chebyshev points (coefficients printed by
-// gen_kernel_poly.py beside this file): float32 Horner error <= 1.7e-7 of the
-// bracket over [0, 1], against 4e-7 for a sin/cos evaluation.  K3_KP0 makes
-// the float32 Horner value at x = 1 exactly zero.
-#define K3_KP0 0.1666666865348816f
-#define K3_KP1 (-0.6666668057441711f)
-#define K3_KP2 0.822467029094696f
-#define K3_KP3 5.713734481105348e-07f
-#define K3_KP4 (-0.6764519810676575f)
-#define K3_KP5 0.2705763280391693f
-#define K3_KP6 0.22254355251789093f
-#define K3_KP7 (-0.12714676558971405f)
-#define K3_KP8 (-0.039220646023750305f)
-#define K3_KP9 0.026095815002918243f
-#define K3_KP10 0.004298381507396698f
-#define K3_KP11 (-0.0030589315574616194f)
-#define K3_KP12 (-0.0003179309132974595f)
-#define K3_KP13 0.00019976771727669984f
-#define K3_KP14 1.4936805200704839e-05f
+// (with the reference's truncated pi), evaluated as a function of the squared
+// distance a2 = d^2 so that no square root is taken: with u = a2 / ls^2 the
+// bracket is smooth in u on [0, 1] and is read from the piecewise-quadratic
+// table of k3_kernel_table.h (K3_KTAB_N intervals; float32 error <= 8e-8 of the
+// bracket, against 4e-7 for a sin/cos evaluation).
+#include "k3_kernel_table.h"
 
-// Scale of the kernel argument: 2/ls rounded up so that ls * scale >= 2
-// exactly.  Then x = fmaf(d, scale, -1) >= 1 for every d >= ls, the clamp to 1
-// and the exact zero at x = 1 give a zero term, and the evaluator needs no
-// separate d < ls test.
+// Scale of the table argument: s = a2 * scale = N u.
 K3_HD float k3_kernel_scale(float ls) {
-    float s = 2.0f / ls;
-    if ((double)s * (double)ls < 2.0) s = nextafterf(s, 3.0e38f);
-    return s;
+    return (float)((double)K3_KTAB_N / ((double)ls * (double)ls));
 }
 
-// k(d) for d >= 0 with scale = k3_kernel_scale(ls); exactly +0 or -0 for d >= ls.
-K3_HD float k3_kernel_value(float d, float scale, float sigma) {
-    float x = fminf(fmaf(d, scale, -1.0f), 1.0f);
-    float p = fmaf(K3_KP14, x, K3_KP13);
-    p = fmaf(p, x, K3_KP12);
-    p = fmaf(p, x, K3_KP11);
-    p = fmaf(p, x, K3_KP10);
-    p = fmaf(p, x, K3_KP9);
-    p = fmaf(p, x, K3_KP8);
-    p = fmaf(p, x, K3_KP7);
-    p = fmaf(p, x, K3_KP6);
-    p = fmaf(p, x, K3_KP5);
-    p = fmaf(p, x, K3_KP4);
-    p = fmaf(p, x, K3_KP3);
-    p = fmaf(p, x, K3_KP2);
-    p = fmaf(p, x, K3_KP1);
-    p = fmaf(p, x, K3_KP0);
-    return fmaxf(0.0f, sigma * p);
+// One table entry (a, b, c): a + f (b + f c), f in [0, 1).  Entries at
+// i >= K3_KTAB_N are zero, so a2 * scale >= N (d >= ls) gives exactly 0.
+K3_HD float k3_kernel_poly(float a, float b, float c, float f, float sigma) {
+    const float v = fmaf(f, fmaf(f, c, b), a);
+    return fmaxf(0.0f, sigma * v);
+}
+K3_HD int k3_kernel_index(float a2, float scale, float* f) {
+    const float s = fminf(a2 * scale, (float)K3_KTAB_N);
+    const int i = (int)s;
+    *f = s - (float)i;
+    return i;
+}
+// k(sqrt(a2)) for a2 >= 0 with scale = k3_kernel_scale(ls).
+K3_HD float k3_kernel_value_sq(float a2, float scale, float sigma) {
+    float f;
+    const int i = k3_kernel_index(a2, scale, &f);
+    return k3_kernel_poly(k3_ktab[4 * i], k3_ktab[4 * i + 1], k3_ktab[4 * i + 2], f, sigma);
 }
 
 // Derivative in ls of sigma * kbar(d / ls), kbar the bracket of the
@@ -295,46 +274,40 @@ K3_HD K3PointRec k3_point_prep(float x, float y, float z, float resolution) {
     return r;
 }
 
-// Conservative bound for sqrtf(s) < ls: s >= ls2g implies sqrtf(s) >= ls.
+// Conservative bound: s >= ls2g implies s * k3_kernel_scale(ls) >= K3_KTAB_N,
+// i.e. a zero kernel term.
 K3_HD float k3_ls2_guard(float ls) {
     return (float)((double)ls * (double)ls * (1.0 + 0x1p-20));
 }
 
-// Occupied-evidence distance of BGKI (src/k3dom/kernel/mass_update.cu:49):
-// returns d if d < ls, else a negative value.
-K3_HD float k3_occ_distance(float vx, float vy, float vz, const K3PointRec& p, float ls, float ls2g) {
+// Squared occupied-evidence distance of BGKI (src/k3dom/kernel/mass_update.cu:49):
+// |v - m|^2 when its kernel term can be non-zero (a2 * scale < N), else -1.
+K3_HD float k3_occ_dist2(float vx, float vy, float vz, const K3PointRec& p, float scale) {
     float dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
     float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    if (!(s < ls2g)) return -1.0f;
-    float d = sqrtf(s);
-    return (d < ls) ? d : -1.0f;
+    return (s * scale < (float)K3_KTAB_N) ? s : -1.0f;
 }
 
-// Free-evidence distance of BGKI (src/k3dom/kernel/mass_update.cu:54-68):
-// distance from the voxel centre to the free segment [0, e] of the ray, with
-// the region beyond the measured point excluded.  Returns d if d < ls, else a
-// negative value.  dc = |v|, dc2 = dc*dc.  The perpendicular distance squared
-// is clamped at 0 (the reference can take sqrtf of a rounding-negative value
-// and drop the term).
-K3_HD float k3_free_distance(float vx, float vy, float vz, float dc, float dc2,
-                             const K3PointRec& p, float ls, float ls2g) {
+// Squared free-evidence distance of BGKI (src/k3dom/kernel/mass_update.cu:54-68):
+// squared distance from the voxel centre to the free segment [0, e] of the
+// ray, with the region beyond the measured point excluded; -1 when there is
+// no free term or its kernel term is zero.  dc2 = |v|^2.  The perpendicular
+// distance squared is clamped at 0 (the reference can take sqrtf of a
+// rounding-negative value and drop the term).
+K3_HD float k3_free_dist2(float vx, float vy, float vz, float dc2, const K3PointRec& p, float scale) {
     float dp = fmaf(vz, p.z, fmaf(vy, p.y, vx * p.x)) * p.inv_dm;
-    float d;
+    float a;
     if (dp <= 0.0f) {
-        d = dc;
+        a = dc2;
     } else if (dp < p.mg) {
-        float a = fmaxf(0.0f, fmaf(-dp, dp, dc2));
-        if (!(a < ls2g)) return -1.0f;
-        d = sqrtf(a);
+        a = fmaxf(0.0f, fmaf(-dp, dp, dc2));
     } else if (dp < p.dm) {
         float dx = vx - p.ex, dy = vy - p.ey, dz = vz - p.ez;
-        float a = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        if (!(a < ls2g)) return -1.0f;
-        d = sqrtf(a);
+        a = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
     } else {
         return -1.0f;
     }
-    return (d < ls) ? d : -1.0f;
+    return (a * scale < (float)K3_KTAB_N) ? a : -1.0f;
 }
 
 // Voxel centre relative to the sensor, src/k3dom/kernel/mass_update.cu:92-99.

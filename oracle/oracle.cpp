@@ -28,8 +28,9 @@
 //     (src/common/kernel/init_new_particles.cu:14-52);
 //   * BGKI sums use the two-level chunked association of k3_math.h and skip
 //     pairs whose distance is >= ls (their term is exactly zero); the sparse
-//     kernel is the degree-14 polynomial of k3_math.h (<= 1.7e-7 of sigma from
-//     the reference formula), exactly zero from d = ls on;
+//     kernel is read as a function of the squared distance from the
+//     piecewise-quadratic table of k3_kernel_table.h (<= 8e-8 of sigma from the
+//     reference formula, no square root), exactly zero from d^2 = ls^2 on;
 //   * gamma^dt is computed once per frame on the host.
 //
 // Parity pins for this file live in tests/test_oracle.py: brute-force BGKI ==
@@ -127,18 +128,19 @@ static float seg_sum(const float* x, int st, int en) {
 
 static void bgki_pair(const Geom& g, int i, int j, int k, const K3PointRec& pr, float ls, float ls2g,
                       float kscale, float sigma, int32_t chunk, ChunkAcc& acc, BgkiStats& st) {
+    (void)ls;
+    (void)ls2g;
     float vx = g.cx[i], vy = g.cy[j], vz = g.cz[k];
     size_t v = (size_t)i + (size_t)g.gs * j + (size_t)g.gs * g.gs * k;
-    float dc = sqrtf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)));
-    float dc2 = dc * dc;
-    float d_o = k3_occ_distance(vx, vy, vz, pr, ls, ls2g);
-    if (d_o >= 0.0f) {
-        acc.add(acc.tot_o, acc.cur_o, acc.chunk_o, v, chunk, k3_kernel_value(d_o, kscale, sigma));
+    float dc2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
+    float a_o = k3_occ_dist2(vx, vy, vz, pr, kscale);
+    if (a_o >= 0.0f) {
+        acc.add(acc.tot_o, acc.cur_o, acc.chunk_o, v, chunk, k3_kernel_value_sq(a_o, kscale, sigma));
         st.occ_pairs++;
     }
-    float d_f = k3_free_distance(vx, vy, vz, dc, dc2, pr, ls, ls2g);
-    if (d_f >= 0.0f) {
-        acc.add(acc.tot_f, acc.cur_f, acc.chunk_f, v, chunk, k3_kernel_value(d_f, kscale, sigma));
+    float a_f = k3_free_dist2(vx, vy, vz, dc2, pr, kscale);
+    if (a_f >= 0.0f) {
+        acc.add(acc.tot_f, acc.cur_f, acc.chunk_f, v, chunk, k3_kernel_value_sq(a_f, kscale, sigma));
         st.free_pairs++;
     }
 }
@@ -936,7 +938,8 @@ int orc_window_chain64(const k3dom_params* p, int32_t T, const int32_t* npts, co
     for (int k = 0; k < 6; ++k) grad[k] = L.d[k];
     return K3DOM_OK;
 }
-float orc_kernel(float d, float ls, float sigma) { return k3_kernel_value(d, k3_kernel_scale(ls), sigma); }
+// kernel at distance d (evaluated from the float32 square d * d)
+float orc_kernel(float d, float ls, float sigma) { return k3_kernel_value_sq(d * d, k3_kernel_scale(ls), sigma); }
 float orc_credit(float m, float scale) { return k3_credit(m, scale); }
 void orc_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) { k3_box_muller(a, b, n0, n1); }
 void orc_update_mass(float* fsd, float del_occ, float del_free, float mass_scale) {

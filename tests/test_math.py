@@ -153,37 +153,40 @@ def test_key_matches_truncate_then_clamp(oracle_lib):
     assert oracle_lib.orc_key(float("nan"), 1.0, 1.0, gs, gsz) == 0 + gs * 1 + gs * gs * 1
 
 
-def test_kernel_polynomial_matches_generator_and_formula(oracle_lib):
-    """The header's kernel coefficients are those gen_kernel_poly.py prints,
-    and the float32 evaluation stays within 2e-7 (relative to sigma) of the
-    reference formula in double over the whole support."""
+def test_kernel_table_matches_generator_and_formula(oracle_lib):
+    """The header's kernel table is what gen_kernel_table.py prints, and the
+    float32 evaluation through it stays within 1.5e-7 (relative to sigma) of
+    the reference formula in double over the whole support."""
     import importlib.util
     import os
     import re
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    spec = importlib.util.spec_from_file_location("gkp", os.path.join(root, "k3dom_amd", "csrc", "gen_kernel_poly.py"))
-    gkp = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(gkp)
-    header = open(os.path.join(root, "k3dom_amd", "csrc", "k3_math.h")).read()
-    got = {int(i): np.float32(float(v)) for i, v in re.findall(r"#define K3_KP(\d+) \(?([-0-9.e]+)f\)?", header)}
-    want = gkp.coefficients()
-    assert len(got) == len(want)
-    for i, c in enumerate(want):
-        assert got[i] == c, (i, got[i], c)
+    spec = importlib.util.spec_from_file_location("gkt", os.path.join(root, "k3dom_amd", "csrc", "gen_kernel_table.py"))
+    gkt = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gkt)
+    header = open(os.path.join(root, "k3dom_amd", "csrc", "k3_kernel_table.h")).read()
+    body = header[header.index("{") + 1:header.index("}")]
+    got = np.array([np.float32(float(v)) for v in re.findall(r"([-0-9.e]+)f", body)], np.float32)
+    want = gkt.table().reshape(-1)
+    assert got.shape == want.shape and np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert np.all(want[-4:] == 0)  # beyond the support
     ls, sigma = 0.5, 1.0
     ds = np.linspace(0, ls, 20001, endpoint=False).astype(np.float32)
     vals = np.array([oracle_lib.orc_kernel(float(d), ls, sigma) for d in ds])
-    ref = np.maximum(0.0, gkp.k(ds.astype(np.float64) / ls))
-    assert np.max(np.abs(vals - ref)) < 2e-7
+    ref = np.maximum(0.0, gkt.kbar(ds.astype(np.float64) / ls))
+    assert np.max(np.abs(vals - ref)) < 1.5e-7
+    # a dropped coefficient or a shifted interval is far outside that bound
+    assert gkt.evaluate(gkt.table(), np.float32(0.3)) == gkt.evaluate(want.reshape(-1, 4), np.float32(0.3))
 
 
 @pytest.mark.parametrize("ls", [0.5, 0.3, 0.1, 0.7, 1.3, 0.05, 2.0, 0.123456])
 def test_kernel_is_exactly_zero_from_the_length_scale_on(oracle_lib, ls):
-    """Every d >= ls gives an exact zero (so a term outside the support adds
-    nothing, and culling such pairs cannot change a sum), for either sign of
-    sigma; just below ls the value is tiny and non-negative."""
+    """Every d beyond ls (past the rounding of d^2 / ls^2) gives an exact zero,
+    so a term outside the support adds nothing and culling such pairs cannot
+    change a sum, for either sign of sigma; just below ls the value is tiny and
+    non-negative."""
     ls32 = np.float32(ls)
-    above = [ls32, np.nextafter(ls32, np.float32(np.inf)), ls32 * np.float32(1.0001), ls32 * 2, np.float32(1e30)]
+    above = [ls32 * np.float32(1 + 3e-7), ls32 * np.float32(1.0001), ls32 * 2, np.float32(1e30)]
     for d in above:
         for sigma in (0.1, 1.0, -0.1):
             assert oracle_lib.orc_kernel(float(d), float(ls32), sigma) == 0.0, (ls, d, sigma)
