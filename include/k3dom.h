@@ -213,8 +213,9 @@ k3dom_status k3dom_get_stats(const k3dom_handle* h, k3dom_stats* out);
 
 /* Launch-shape tuning, no counterpart in the reference: after a normal frame,
  * times the kernel-inference gather over that frame's work items for several
- * grid sizes and keeps the fastest (blocks per compute unit written to
- * *blocks_per_cu when not NULL).  Results do not depend on the choice. */
+ * grid sizes (1 to 8 blocks of 512 threads per compute unit) and keeps the
+ * fastest (written to *blocks_per_cu when not NULL).  Results do not depend
+ * on the choice. */
 k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu);
 
 /* ---------------------------------------------------------------------------

@@ -694,7 +694,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(hipHostMalloc((void**)&h->h_sc, sizeof(DevScalars), hipHostMallocDefault));
     CREATE_TRY(hipHostMalloc((void**)&h->h_fc, sizeof(FrameConsts), hipHostMallocDefault));
     CREATE_TRY(dalloc(h, &h->fc, 1));
-    h->gather_blocks = h->num_cus * 8;
+    h->gather_blocks = h->num_cus * 4;
     for (int e = 0; e < EV_COUNT; ++e) CREATE_TRY(hipEventCreate(&h->ev[e]));
     CREATE_TRY(dalloc(h, &h->pair_counts, 2));
     // prior fill and zeroed particle state (src/common/kernel/init.cu:20-35)
@@ -885,7 +885,7 @@ k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
     hipEvent_t e0, e1;
     HIP_TRY(h, hipEventCreate(&e0));
     HIP_TRY(h, hipEventCreate(&e1));
-    const int cands[] = {2, 4, 6, 8, 10, 12, 16};
+    const int cands[] = {1, 2, 3, 4, 5, 6, 8};
     float best = 1e30f;
     int best_c = h->gather_blocks / h->num_cus;
     for (int c : cands) {

@@ -148,7 +148,7 @@ k3dom_status k3dom_batch_create(const k3dom_params* params, int32_t n_maps, k3do
         b->maps.push_back(h);
     }
     b->num_cus = b->maps[0]->num_cus;
-    b->gather_blocks = b->num_cus * 8;
+    b->gather_blocks = b->num_cus * 4;
     b->head_done.assign(n_maps, nullptr);
     b->tail_done.assign(n_maps, nullptr);
     b->m_now.assign(n_maps, 0);
@@ -283,7 +283,7 @@ k3dom_status k3dom_batch_autotune(k3dom_batch* b, int32_t* blocks_per_cu) {
     hipEvent_t e0, e1;
     B_HIP(b, hipEventCreate(&e0));
     B_HIP(b, hipEventCreate(&e1));
-    const int cands[] = {2, 4, 6, 8, 10, 12, 16};
+    const int cands[] = {1, 2, 3, 4, 5, 6, 8};
     float best = 1e30f;
     int best_c = b->gather_blocks / b->num_cus;
     for (int c : cands) {

@@ -27,6 +27,9 @@ namespace k3 {
 namespace {
 constexpr int TPB = 256;
 constexpr int BG_WAVES = TPB / 64;
+// the gather runs blocks of GA_TPB threads: one kernel table in LDS per 8 wavefronts
+constexpr int GA_TPB = 512;
+constexpr int GA_WAVES = GA_TPB / 64;
 constexpr uint32_t BG_HEAVY_LEN = 1u << K3_BGKI_CHUNK_LOG2;
 constexpr uint32_t BG_DIRECT = 0xffffffffu;
 
@@ -453,7 +456,7 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float si
 
 // Copies the kernel table into LDS (every thread of the block takes part).
 __device__ __forceinline__ void bg_load_table(BgTable* stab) {
-    for (int i = threadIdx.x; i <= K3_KTAB_N; i += TPB) {
+    for (int i = threadIdx.x; i <= K3_KTAB_N; i += GA_TPB) {
         stab->a[i] = k3_ktab[4 * i];
         stab->b[i] = k3_ktab[4 * i + 1];
         stab->c[i] = k3_ktab[4 * i + 2];
@@ -470,8 +473,8 @@ constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 // tile list) by one wavefront: lane = voxel of the 4x4x4 tile.
 template <bool GRAD>
 __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, int lane,
-                                          float4 (&srec)[BG_WAVES][64 * 3], uint32_t (&sidx)[BG_WAVES][64],
-                                          uint32_t (&scls)[BG_WAVES][64], const BgTable* stab) {
+                                          float4 (&srec)[GA_WAVES][64 * 3], uint32_t (&smeta)[GA_WAVES][64],
+                                          const BgTable* stab) {
     const GridGeom g = a.fc->g;
     const uint4 item = a.items[it];
     const uint32_t tile = uniform(item.x), ib = uniform(item.y), ie = uniform(item.z), slot = uniform(item.w);
@@ -503,7 +506,6 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
             const uint32_t p = a.tlist[e];
             const K3PointRec r = a.rec[p];
             const float4* src = reinterpret_cast<const float4*>(a.rec + p);
-            sidx[w][lane] = p;
             srec[w][lane * 3 + 0] = src[0];
             srec[w][lane * 3 + 1] = src[1];
             srec[w][lane * 3 + 2] = src[2];
@@ -514,13 +516,15 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
             else if (proj - rr >= r.dm) cls = C_NOFREE;
             const float ox = tcx - r.x, oy = tcy - r.y, oz = tcz - r.z;
             if (!(ox * ox + oy * oy + oz * oz >= occ_r * occ_r)) cls |= F_OCC;
-            scls[w][lane] = cls;
+            smeta[w][lane] = (p << 3) | cls;  // point index and class
         }
         bg_wave_sync();
         const uint32_t cnt = uniform(min(64u, ie - b));
         for (uint32_t q = 0; q < cnt; ++q) {
-            const uint32_t chunk = uniform(sidx[w][q]) >> K3_BGKI_CHUNK_LOG2;
-            const uint32_t cls = uniform(scls[w][q]);
+            const uint32_t meta = uniform(smeta[w][q]);
+            const float4 r0 = srec[w][q * 3];
+            const uint32_t chunk = meta >> (3 + K3_BGKI_CHUNK_LOG2);
+            const uint32_t cls = meta & 7u;
             if (chunk != cur_chunk) {
                 tot_o += cur_o;
                 tot_f += cur_f;
@@ -528,8 +532,34 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 cur_f = 0.0f;
                 cur_chunk = chunk;
             }
+            if (cls == C_MAIN) {
+                // the common case: free term only, perpendicular distance for
+                // every lane; two such candidates of one chunk are evaluated
+                // together (independent table reads), then added in order
+                const float dpa = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
+                const float a2a = fmaxf(0.0f, fmaf(-dpa, dpa, dc2));
+                const uint32_t meta2 = q + 1 < cnt ? uniform(smeta[w][q + 1]) : 0u;
+                if (q + 1 < cnt && (meta2 & 7u) == C_MAIN && (meta2 >> (3 + K3_BGKI_CHUNK_LOG2)) == chunk) {
+                    const float4 r0b = srec[w][(q + 1) * 3];
+                    const float dpb = fmaf(vz, r0b.z, fmaf(vy, r0b.y, vx * r0b.x)) * r0b.w;
+                    const float a2b = fmaxf(0.0f, fmaf(-dpb, dpb, dc2));
+                    const BgTerm<GRAD> ta = bg_term<GRAD>(a2a, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
+                    const BgTerm<GRAD> tb = bg_term<GRAD>(a2b, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
+                    cur_f += ta.v;
+                    cur_f += tb.v;
+                    if (GRAD) {
+                        gsum_f += ta.g;
+                        gsum_f += tb.g;
+                    }
+                    ++q;
+                } else {
+                    const BgTerm<GRAD> ta = bg_term<GRAD>(a2a, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
+                    cur_f += ta.v;
+                    if (GRAD) gsum_f += ta.g;
+                }
+                continue;
+            }
             // record: r0 = (x, y, z, inv_dm), r1 = (dm, mg, ex, ey), r2 = (ez, valid, -, -)
-            const float4 r0 = srec[w][q * 3 + 0];
             if (cls & F_OCC) {  // occupied term (k3_occ_distance)
                 const float ox = vx - r0.x, oy = vy - r0.y, oz = vz - r0.z;
                 const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
@@ -601,10 +631,9 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
 }
 
 template <bool GRAD>
-__global__ __launch_bounds__(TPB) void k_bgki(BgArgs a) {
-    __shared__ float4 srec[BG_WAVES][64 * 3];
-    __shared__ uint32_t sidx[BG_WAVES][64];
-    __shared__ uint32_t scls[BG_WAVES][64];
+__global__ __launch_bounds__(GA_TPB) void k_bgki(BgArgs a) {
+    __shared__ float4 srec[GA_WAVES][64 * 3];
+    __shared__ uint32_t smeta[GA_WAVES][64];
     __shared__ BgTable stab_s;
     BgTable* stab = &stab_s;
     DevScalars* sc = a.sc;
@@ -617,7 +646,7 @@ __global__ __launch_bounds__(TPB) void k_bgki(BgArgs a) {
         if (lane == 0) it = atomicAdd(&sc->item_counter, 1u);
         it = uniform(__shfl(it, 0));
         if (it >= n_items) break;
-        bgki_item<GRAD>(a, it, w, lane, srec, sidx, scls, stab);
+        bgki_item<GRAD>(a, it, w, lane, srec, smeta, stab);
     }
 }
 
@@ -648,7 +677,7 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                        float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec,
                        float* del_occ, float* del_free, DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki<false>, dim3(grid_blocks), dim3(TPB), 0, st,
+    hipLaunchKernelGGL(k_bgki<false>, dim3(grid_blocks), dim3(GA_TPB), 0, st,
                        make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, nullptr, nullptr, sc));
     return hipGetLastError();
 }
@@ -656,7 +685,7 @@ hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* 
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T,
                             const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
                             DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki<true>, dim3(grid_blocks), dim3(TPB), 0, st,
+    hipLaunchKernelGGL(k_bgki<true>, dim3(grid_blocks), dim3(GA_TPB), 0, st,
                        make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, dls_occ, dls_free, sc));
     return hipGetLastError();
 }
@@ -678,11 +707,10 @@ __global__ void k_bgki_batch_prefix(const BgArgs* __restrict__ maps, int nmaps, 
 }
 
 // One persistent launch over the items of all maps.
-__global__ __launch_bounds__(TPB) void k_bgki_batch(const BgArgs* __restrict__ maps, int nmaps,
+__global__ __launch_bounds__(GA_TPB) void k_bgki_batch(const BgArgs* __restrict__ maps, int nmaps,
                                                     const uint32_t* __restrict__ prefix, uint32_t* counter) {
-    __shared__ float4 srec[BG_WAVES][64 * 3];
-    __shared__ uint32_t sidx[BG_WAVES][64];
-    __shared__ uint32_t scls[BG_WAVES][64];
+    __shared__ float4 srec[GA_WAVES][64 * 3];
+    __shared__ uint32_t smeta[GA_WAVES][64];
     __shared__ BgTable stab_s;
     BgTable* stab = &stab_s;
     bg_load_table(stab);
@@ -700,14 +728,14 @@ __global__ __launch_bounds__(TPB) void k_bgki_batch(const BgArgs* __restrict__ m
             else hi = mid - 1;
         }
         const BgArgs a = maps[lo];
-        bgki_item<false>(a, it - prefix[lo], w, lane, srec, sidx, scls, stab);
+        bgki_item<false>(a, it - prefix[lo], w, lane, srec, smeta, stab);
     }
 }
 
 hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps, int nmaps, uint32_t* prefix,
                              uint32_t* counter) {
     hipLaunchKernelGGL(k_bgki_batch_prefix, dim3(1), dim3(64), 0, st, maps, nmaps, prefix, counter);
-    hipLaunchKernelGGL(k_bgki_batch, dim3(grid_blocks), dim3(TPB), 0, st, maps, nmaps, prefix, counter);
+    hipLaunchKernelGGL(k_bgki_batch, dim3(grid_blocks), dim3(GA_TPB), 0, st, maps, nmaps, prefix, counter);
     return hipGetLastError();
 }
 

@@ -58,7 +58,7 @@ def _run(graph, tiny, monkeypatch, autotune=False):
         _compare_maps(batch, solos, normal, f"frame {f}")
         if autotune and f == 1:
             bpc = batch.autotune()
-            assert bpc in (2, 4, 6, 8, 10, 12, 16)
+            assert bpc in (1, 2, 3, 4, 5, 6, 8)
     return batch, solos
 
 
@@ -94,5 +94,5 @@ def test_single_map_autotune_keeps_results(gpu):
             m.updatePose(*pose)
             m.updateGrid(t, xs, ys, zs)
         if f == 1:
-            assert gd.autotune() in (2, 4, 6, 8, 10, 12, 16)
+            assert gd.autotune() in (1, 2, 3, 4, 5, 6, 8)
         compare_state(gd, od, normal_frame=normal, what=f"autotuned frame {f}:")
