@@ -218,31 +218,37 @@ k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
     HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, grid_geom(h), h->tg, h->ibuf, h->del_occ, h->del_free,
                                    h->sc));
     mark(h, ST_MASS);
+    // per-block maxima: born masses first, then the joint particle weights
+    uint32_t* born_parts = h->maxparts;
+    uint32_t* joint_parts = h->maxparts + mass_update_parts(h->V);
+    const int n_born_parts = mass_update_parts(h->V);
     if (normal) {
-        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[1].w, h->del_occ, h->del_free, h->fc,
-                                      h->p.mass_scale, h->p.birth_prob, h->born, h->heavy_voxels, h->sc));
+        const int n_persist_parts = voxel_particles_parts(h->N);
+        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->wsort, h->P[1].key, h->N, h->usum, h->del_occ,
+                                      h->del_free, h->fc, h->p.mass_scale, h->p.birth_prob, h->born, born_parts));
         mark(h, ST_PERSIST);
-        HIP_TRY(h, launch_voxel_particles(h->stream, ab, v, h->V, h->P[1], h->p.particle_min_vel, h->heavy_voxels,
-                                          h->sc));
+        HIP_TRY(h, launch_voxel_particles(h->stream, ab, v, h->V, h->P[1], h->wsort, h->N, h->p.particle_min_vel,
+                                          h->usum, joint_parts));
         mark(h, ST_NEWBORN);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, &h->sc->born_max_bits, &h->sc->E_born,
+        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, born_parts, n_born_parts, &h->sc->E_born,
                                      h->cdf_v, h->blocksums, &h->sc->total_born));
         HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->B, h->born, h->bnd,
                                  v.wua));
         HIP_TRY(h, launch_newborn(h->stream, ab, h->bnd, h->V, h->B, h->gs, v.wua, h->NB, h->p.init_max_velocity,
-                                  h->p.particle_min_vel, h->fc, k0, k1, h->sc));
+                                  h->p.particle_min_vel, h->fc, k0, k1, joint_parts + n_persist_parts));
         mark(h, ST_MOMENTS);
         mark(h, ST_RESAMPLE);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->P[1].w, h->N, h->NB.w, h->B, &h->sc->joint_max_bits,
-                                     &h->sc->E_joint, h->cdf_j, h->blocksums, &h->sc->total_joint));
+        HIP_TRY(h, launch_fixed_scan(h->stream, h->P[1].w, h->N, h->NB.w, h->B, joint_parts,
+                                     n_persist_parts + newborn_parts(h->B), &h->sc->E_joint, h->cdf_j, h->blocksums,
+                                     &h->sc->total_joint));
         HIP_TRY(h, launch_resample(h->stream, ab, h->cdf_j, h->N + h->B, &h->sc->total_joint, &h->sc->E_joint,
                                    h->P[1], h->NB, h->P[0], h->N, h->fc, k0, k1, h->ridx));
     } else {
-        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->P[0].w, h->del_occ, h->del_free, nullptr,
-                                      h->p.mass_scale, h->p.birth_prob, h->born, h->heavy_voxels, h->sc));
+        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, nullptr, nullptr, 0, h->usum, h->del_occ,
+                                      h->del_free, nullptr, h->p.mass_scale, h->p.birth_prob, h->born, born_parts));
         mark(h, ST_PERSIST);
         mark(h, ST_NEWBORN);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, &h->sc->born_max_bits, &h->sc->E_born,
+        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, born_parts, n_born_parts, &h->sc->E_born,
                                      h->cdf_v, h->blocksums, &h->sc->total_born));
         HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->N, h->born, h->bnd,
                                  nullptr));
@@ -282,7 +288,7 @@ k3dom_status enqueue_head(k3dom_handle* h, bool normal) {
         mark(h, ST_SORT);
         HIP_TRY(h, launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys,
                                      &h->last_perm));
-        HIP_TRY(h, launch_gather_sorted(h->stream, h->P[0], h->P[1], h->last_perm, h->last_skeys, h->N));
+        HIP_TRY(h, launch_gather_sorted(h->stream, h->P[0], h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N));
         HIP_TRY(h, launch_ranges(h->stream, h->P[1].key, h->N, v.start, v.end, h->V));
     }
     return K3DOM_OK;
@@ -656,6 +662,10 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     for (float** f : {&h->NB.px, &h->NB.py, &h->NB.pz, &h->NB.vx, &h->NB.vy, &h->NB.vz, &h->NB.w})
         CREATE_TRY(dalloc(h, f, B));
     CREATE_TRY(dalloc(h, &h->NB.key, B));
+    CREATE_TRY(dalloc(h, &h->wsort, N));
+    CREATE_TRY(dalloc(h, &h->usum, 3 * N));
+    CREATE_TRY(dalloc(h, &h->maxparts, (size_t)mass_update_parts(h->V) + voxel_particles_parts(h->N) +
+                                           newborn_parts(h->B)));
     CREATE_TRY(dalloc(h, &h->sort.k0, N));
     CREATE_TRY(dalloc(h, &h->sort.k1, N));
     CREATE_TRY(dalloc(h, &h->sort.i0, N));
@@ -667,7 +677,6 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(dalloc(h, &h->blocksums, (size_t)fixed_scan_blocks((int)std::max(V, N + B))));
     CREATE_TRY(dalloc(h, &h->bnd, V));
     CREATE_TRY(dalloc(h, &h->ridx, N));
-    CREATE_TRY(dalloc(h, &h->heavy_voxels, V));
     CREATE_TRY(dalloc(h, &h->sc, 1));
     setup_levels(h);
     for (int l = 0; l < 3; ++l) {

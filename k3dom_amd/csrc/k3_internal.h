@@ -34,12 +34,14 @@ struct k3dom_handle {
     ParticleArrays P[2]{};  // 0: current set, 1: sorted set
     ParticleArrays NB{};
     SortScratch sort{};
+    float* wsort = nullptr;  // N: sorted particle weights before the persistent update
+    float* usum = nullptr;   // 3N: per-unit sums of the particle-parallel voxel sums
+    uint32_t* maxparts = nullptr;  // per-block maxima feeding the fixed-point CDF exponents
     const uint32_t* last_perm = nullptr;
     const uint32_t* last_skeys = nullptr;
     uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;
     uint32_t* bnd = nullptr;
     int32_t* ridx = nullptr;
-    int32_t* heavy_voxels = nullptr;  // V entries: voxels queued for the wavefront paths
     DevScalars* sc = nullptr;
 
     int m_cap = 0;

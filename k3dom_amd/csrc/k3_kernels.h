@@ -11,8 +11,6 @@ namespace k3 {
 
 // Device-side scalars of one map, written and read by kernels within a frame.
 struct DevScalars {
-    uint32_t born_max_bits;   // max born mass (float bits; masses are >= 0)
-    uint32_t joint_max_bits;  // max particle weight after the static penalty
     int32_t E_born, E_joint;  // fixed-point exponents
     uint64_t total_born;      // u64 fixed-point totals
     uint64_t total_joint;
@@ -23,9 +21,7 @@ struct DevScalars {
     uint32_t overflow_mask;   // bit l: candidate level l, bit 4: split items
     uint32_t level_total[3];  // candidate-list entries per level
     uint32_t level_items[3];  // work items per level
-    uint32_t n_heavy_mass;    // voxels queued for the wavefront mass-update path
-    uint32_t n_heavy_vp;      // voxels queued for the wavefront particle path
-    uint32_t pad;
+    uint32_t pad[3];
 };
 
 // Voxel state arrays (one buffer set).
@@ -82,8 +78,9 @@ struct SortScratch {
 int sort_blocks(int n);
 hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int key_bits, SortScratch s,
                              const uint32_t** out_keys, const uint32_t** out_idx);
-hipError_t launch_gather_sorted(hipStream_t st, ParticleArrays src, ParticleArrays dst, const uint32_t* perm,
-                                const uint32_t* sorted_keys, int n);
+// Sorted particles to dst, except the weights, which go to wsorted.
+hipError_t launch_gather_sorted(hipStream_t st, ParticleArrays src, ParticleArrays dst, float* wsorted,
+                                const uint32_t* perm, const uint32_t* sorted_keys, int n);
 hipError_t launch_ranges(hipStream_t st, const int32_t* sorted_keys, int n, int32_t* start, int32_t* end, int V);
 
 // ------------------------------------------------------------------ kernel inference (k3_bgki.hip)
@@ -173,23 +170,37 @@ hipError_t launch_bgki_count(hipStream_t st, int grid_blocks, const GridGeom& g,
 // ------------------------------------------------------------------ frame tail
 // Every tail kernel returns at once when *abort_flag is set (a capacity of
 // the kernel-inference stage overflowed; the host grows it and re-runs).
-// heavy: scratch list of V voxel ids (voxels with many particles)
+// The largest born mass and the largest joint particle weight, which fix the
+// exponents of the fixed-point CDFs, are kept as one maximum per block of
+// their producers (float bits) and reduced by launch_fixed_scan: the born
+// maxima come from launch_mass_update (mass_update_parts(V) slots), the joint
+// ones from launch_voxel_particles (voxel_particles_parts(N) slots) followed by
+// launch_newborn (newborn_parts(B) slots).
+int mass_update_parts(int V);
+int voxel_particles_parts(int n);
+int newborn_parts(int B);
 // decay: fc->decay (1 in the initialisation frame when fc is null)
+// weights, keys: the n sorted particles (n = 0: no particles, every voxel
+// start is -1); usum: n floats of scratch for per-unit sums
 hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
-                              const float* weights, const float* del_occ, const float* del_free,
-                              const FrameConsts* fc, float mass_scale, float p_B, float* born, int32_t* heavy,
-                              DevScalars* sc);
+                              const float* weights, const int32_t* keys, int n, float* usum, const float* del_occ,
+                              const float* del_free, const FrameConsts* fc, float mass_scale, float p_B, float* born,
+                              uint32_t* born_parts);
+// P: the n sorted particles, whose weights P.w are written (penalised) from
+// wsrc; usum3: 3n floats of scratch
 hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
-                                  ParticleArrays P, float min_vel, int32_t* heavy, DevScalars* sc);
+                                  ParticleArrays P, const float* wsrc, int n, float min_vel, float* usum3,
+                                  uint32_t* joint_parts);
+// max_parts: nparts per-block maxima of the values scanned
 hipError_t launch_fixed_scan(hipStream_t st, const float* a, int na, const float* b, int nb,
-                             const uint32_t* max_bits, int32_t* E_out, uint64_t* cdf, uint64_t* blocksums,
-                             uint64_t* total);
+                             const uint32_t* max_parts, int nparts, int32_t* E_out, uint64_t* cdf,
+                             uint64_t* blocksums, uint64_t* total);
 int fixed_scan_blocks(int n);
 hipError_t launch_bounds(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, const uint64_t* total,
                          int V, uint32_t cnt, const float* born, uint32_t* bnd, float* wua);
 hipError_t launch_newborn(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int B, int gs,
                           const float* wua, ParticleArrays NB, float vmax, float min_vel, const FrameConsts* fc,
-                          uint32_t k0, uint32_t k1, DevScalars* sc);
+                          uint32_t k0, uint32_t k1, uint32_t* joint_parts);
 hipError_t launch_init_particles(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int N,
                                  int gs, ParticleArrays P, float vmax, float min_vel, const FrameConsts* fc,
                                  uint32_t k0, uint32_t k1);

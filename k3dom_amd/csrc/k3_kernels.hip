@@ -14,8 +14,10 @@
 //   k_mark_ranges        src/common/kernel/particle_to_grid.cu:13-39
 //   k_point_prep, k_list_level, k_bgki, k_bgki_combine
 //                        src/k3dom/kernel/mass_update.cu:32-77 (BGKI)
-//   k_mass_update        src/k3dom/kernel/mass_update.cu:152-183
-//   k_voxel_particles    src/common/kernel/update_persistent_particles.cu:17-55,
+//   k_unit_weight_sums, k_mass_update
+//                        src/k3dom/kernel/mass_update.cu:152-183
+//   k_unit_persist, k_voxel_particles
+//                        src/common/kernel/update_persistent_particles.cu:17-55,
 //                        src/common/kernel/statistical_moments.cu:22-63,
 //                        src/common/kernel/resampling.cu:13-36 (persistent part)
 //   k_fixed_*, k_bounds  src/common/dom.cu:213-219 + init_new_particles.cu:44-52 (CDF)
@@ -366,6 +368,7 @@ hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int 
 }
 
 __global__ __launch_bounds__(TPB) void k_gather(ParticleArrays src, ParticleArrays dst,
+                                                float* __restrict__ wsorted,
                                                 const uint32_t* __restrict__ perm,
                                                 const uint32_t* __restrict__ skeys, int n) {
     int i = blockIdx.x * TPB + threadIdx.x;
@@ -373,12 +376,12 @@ __global__ __launch_bounds__(TPB) void k_gather(ParticleArrays src, ParticleArra
     uint32_t j = perm[i];
     dst.px[i] = src.px[j]; dst.py[i] = src.py[j]; dst.pz[i] = src.pz[j];
     dst.vx[i] = src.vx[j]; dst.vy[i] = src.vy[j]; dst.vz[i] = src.vz[j];
-    dst.w[i] = src.w[j];
+    wsorted[i] = src.w[j];
     dst.key[i] = (int32_t)skeys[i];
 }
-hipError_t launch_gather_sorted(hipStream_t st, ParticleArrays src, ParticleArrays dst, const uint32_t* perm,
-                                const uint32_t* sorted_keys, int n) {
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(TPB), 0, st, src, dst, perm, sorted_keys, n);
+hipError_t launch_gather_sorted(hipStream_t st, ParticleArrays src, ParticleArrays dst, float* wsorted,
+                                const uint32_t* perm, const uint32_t* sorted_keys, int n) {
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(TPB), 0, st, src, dst, wsorted, perm, sorted_keys, n);
     return hipGetLastError();
 }
 
@@ -403,52 +406,147 @@ hipError_t launch_ranges(hipStream_t st, const int32_t* sorted_keys, int n, int3
 }
 
 // ============================================================== mass update
-__device__ __forceinline__ void atomic_max_nonneg(uint32_t* dst, float v) {
-    if (v > 0.0f && v - v == 0.0f) atomicMax(dst, k3_f2u(v));
+// Largest positive finite v over the block (TPB threads, all of which make
+// this call) to *slot as float bits, 0 when there is none.  The per-block
+// slots are reduced by the consumer (k_fixed_exponent), so no two blocks
+// write one address: atomics on a single word from every wave serialise.
+__device__ void block_max_store(float v, uint32_t* slot) {
+    __shared__ float wm[TPB / 64];
+    v = (v > 0.0f && v - v == 0.0f) ? v : 0.0f;
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = wm[0];
+        for (int i = 1; i < TPB / 64; ++i) m = fmaxf(m, wm[i]);
+        *slot = k3_f2u(m);
+    }
 }
 
-// Per-voxel particle loops.  A voxel's segment sums use the two-level
-// association of K3_SEG_BLOCK: voxels with at most K3_SEG_BLOCK particles (one
-// block, a plain left-to-right sum) are summed by one lane with 8 loads in
-// flight; larger ones are queued and handled by a whole wavefront whose lane l
-// sums blocks l, l + 64, ... and then adds the block sums in block order
-// (broadcast with readlane), so the results do not depend on the path taken.
-constexpr int HEAVY_PARTICLES = K3_SEG_BLOCK;
-constexpr int HEAVY_GRID = 1024;
+// Per-voxel particle sums.  A voxel's particles are one contiguous segment of
+// the sorted arrays, summed with the two-level association of K3_SEG_BLOCK:
+// the units (blocks of K3_SEG_BLOCK particles from the segment start) are each
+// summed left to right from 0; a voxel of one unit takes that sum, and a voxel
+// of several adds its unit sums, in order, to 0.
+//
+// Unit sums are computed particle-parallel, so the work per wavefront does not
+// depend on how the particles crowd into voxels: a wavefront owns UNIT_WAVE
+// consecutive sorted positions and lane l the UNIT_LANE of them from
+// UNIT_LANE * l.  The wave copies its positions, and the K3_SEG_BLOCK after
+// them (the farthest a unit runs past its owner), into LDS with coalesced
+// loads; each lane then sums, from LDS, every unit that starts in its
+// positions and stores the sum at the unit's start position.  A voxel pass
+// adds each voxel's unit sums with coalesced access to the voxel arrays.
+constexpr int UNIT_LANE = 16;
+constexpr int UNIT_WAVE = 64 * UNIT_LANE;
+constexpr int UNIT_SPAN = UNIT_WAVE + K3_SEG_BLOCK;  // staged positions per wave, a multiple of 64
+constexpr int UNIT_TPB = 256;
+constexpr int UNIT_WAVES = UNIT_TPB / 64;
 
-__device__ __forceinline__ float seq_sum(const float* __restrict__ x, int st, int en) {
-    float s = 0.0f;
-    int j = st;
-    for (; j + 7 <= en; j += 8) {
-        const float a0 = x[j], a1 = x[j + 1], a2 = x[j + 2], a3 = x[j + 3];
-        const float a4 = x[j + 4], a5 = x[j + 5], a6 = x[j + 6], a7 = x[j + 7];
-        s += a0; s += a1; s += a2; s += a3;
-        s += a4; s += a5; s += a6; s += a7;
-    }
-    for (; j <= en; ++j) s += x[j];
-    return s;
+static inline unsigned unit_blocks(int n) { return (unsigned)((n + UNIT_TPB / 64 * UNIT_WAVE - 1) / (UNIT_TPB / 64 * UNIT_WAVE)); }
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ float lane_value(float v, int q) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), q));
 }
 
-// Two-level segment sum of x[st..en] by one wavefront (st, en uniform).
-__device__ __forceinline__ float wave_seg_sum(const float* __restrict__ x, int st, int en) {
+// First unit start at or after position q of voxel k (k holds q); one past
+// the voxel's segment when no unit of it starts there.
+__device__ __forceinline__ int first_unit_start(int q, int k, const int32_t* __restrict__ start,
+                                                const int32_t* __restrict__ end) {
+    const int s0 = start[k], e0 = end[k];
+    const int u = q + (K3_SEG_BLOCK - (q - s0) % K3_SEG_BLOCK) % K3_SEG_BLOCK;
+    return u > e0 ? e0 + 1 : u;
+}
+
+// End (exclusive, local) of the unit starting at local position u: the first
+// position after u with another key, at most K3_SEG_BLOCK after u, or m.
+// Keys are sorted, so one key compare covers four positions.
+__device__ __forceinline__ int unit_end(const int32_t* lk, int u, int m) {
+    const int kk = lk[u];
+    const int lim = min(u + K3_SEG_BLOCK, m);
+    int i = u + 1;
+    while (i + 4 <= lim && lk[i + 3] == kk) i += 4;
+    while (i < lim && lk[i] == kk) ++i;
+    return i;
+}
+
+// Two-level sum of a voxel's unit sums u[st + K3_SEG_BLOCK * j] (one lane).
+__device__ __forceinline__ float chain_sum(const float* __restrict__ u, int st, int nb) {
+    if (nb == 1) return u[st];
+    float s = 0.0f;
+    int j = 0;
+    for (; j + 4 <= nb; j += 4) {
+        const float a0 = u[st + j * K3_SEG_BLOCK], a1 = u[st + (j + 1) * K3_SEG_BLOCK];
+        const float a2 = u[st + (j + 2) * K3_SEG_BLOCK], a3 = u[st + (j + 3) * K3_SEG_BLOCK];
+        s += a0; s += a1; s += a2; s += a3;
+    }
+    for (; j < nb; ++j) s += u[st + j * K3_SEG_BLOCK];
+    return s;
+}
+
+// The same sum by the whole wave, for a voxel of many units (st, nb uniform):
+// 64 unit sums are loaded at once and added in order via readlane.
+__device__ __forceinline__ float wave_chain_sum(const float* __restrict__ u, int st, int nb) {
     const int lane = threadIdx.x & 63;
-    const int nb = (en - st + K3_SEG_BLOCK) / K3_SEG_BLOCK;
     float s = 0.0f;
     for (int b0 = 0; b0 < nb; b0 += 64) {
-        const int b = b0 + lane;
-        float bs = 0.0f;
-        if (b < nb) {
-            const int j0 = st + b * K3_SEG_BLOCK;
-            bs = seq_sum(x, j0, min(en, j0 + K3_SEG_BLOCK - 1));
-        }
+        const float a = b0 + lane < nb ? u[st + (b0 + lane) * K3_SEG_BLOCK] : 0.0f;
         const int cnt = min(64, nb - b0);
-        for (int q = 0; q < cnt; ++q) s += lane_value(bs, q);
+        for (int q = 0; q < cnt; ++q) s += lane_value(a, q);
     }
     return s;
+}
+
+// Units whose voxel has more than this many are summed by the whole wave.
+constexpr int WAVE_CHAIN_UNITS = 64;
+
+__device__ __forceinline__ int unit_count(int st, int en) { return (en - st + K3_SEG_BLOCK) / K3_SEG_BLOCK; }
+
+// Unit sums of the sorted weights.
+__global__ __launch_bounds__(UNIT_TPB) void k_unit_weight_sums(const uint32_t* __restrict__ abort_flag,
+                                                               const float* __restrict__ w,
+                                                               const int32_t* __restrict__ key,
+                                                               const int32_t* __restrict__ start,
+                                                               const int32_t* __restrict__ end, int n,
+                                                               float* __restrict__ usum) {
+    __shared__ float lw_all[UNIT_WAVES][UNIT_SPAN];
+    __shared__ int32_t lk_all[UNIT_WAVES][UNIT_SPAN];
+    if (*abort_flag) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r0 = (blockIdx.x * UNIT_WAVES + wv) * UNIT_WAVE;
+    if (r0 >= n) return;
+    float* lw = lw_all[wv];
+    int32_t* lk = lk_all[wv];
+    const int m = min(n - r0, UNIT_SPAN);
+#pragma unroll
+    for (int k = 0; k < UNIT_SPAN / 64; ++k) {
+        const int j = k * 64 + lane;
+        if (j < m) {
+            lw[j] = w[r0 + j];
+            lk[j] = key[r0 + j];
+        }
+    }
+    wave_lds_sync();
+    const int q0 = UNIT_LANE * lane, q1 = min(q0 + UNIT_LANE, m);
+    if (q0 >= q1) return;
+    for (int u = first_unit_start(r0 + q0, lk[q0], start, end) - r0; u < q1;) {
+        const int e = unit_end(lk, u, m);
+        float s = 0.0f;
+        int i = u;
+        for (; i + 4 <= e; i += 4) {
+            const float a0 = lw[i], a1 = lw[i + 1], a2 = lw[i + 2], a3 = lw[i + 3];
+            s += a0; s += a1; s += a2; s += a3;
+        }
+        for (; i < e; ++i) s += lw[i];
+        usum[r0 + u] = s;
+        u = e;
+    }
 }
 
 // prediction, evidence update and newborn split of one voxel
@@ -470,66 +568,49 @@ __device__ __forceinline__ float mass_voxel(const VoxelArrays& v, int c, float d
     return b;
 }
 
+// Mass update of every voxel (src/k3dom/kernel/mass_update.cu:152-183), the
+// predicted dynamic mass being the sum of the voxel's particle weights.
 __global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict__ abort_flag, VoxelArrays v, int V,
-                                                     const float* __restrict__ weights,
+                                                     const float* __restrict__ usum,
                                                      const float* __restrict__ del_occ,
                                                      const float* __restrict__ del_free,
                                                      const FrameConsts* __restrict__ fc, float mass_scale, float p_B,
-                                                     float* __restrict__ born, int32_t* __restrict__ heavy,
-                                                     DevScalars* sc) {
-    if (*abort_flag) return;
-    const float decay = fc ? fc->decay : 1.0f;
-    int c = blockIdx.x * TPB + threadIdx.x;
-    float bmax = 0.0f;
-    if (c < V) {
-        const int st = v.start[c], en = v.end[c];
-        if (st != -1 && en - st + 1 > HEAVY_PARTICLES) {
-            heavy[atomicAdd(&sc->n_heavy_mass, 1u)] = c;
-        } else {
-            const float dyn_pred = st != -1 ? seq_sum(weights, st, en) : 0.0f;
-            const float b = mass_voxel(v, c, dyn_pred, del_occ, del_free, decay, mass_scale, p_B, born);
-            if (b > 0.0f && b - b == 0.0f) bmax = b;
-        }
-    }
-    // wave-level max before one atomic per wave
-    for (int o = 32; o > 0; o >>= 1) bmax = fmaxf(bmax, __shfl_xor(bmax, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->born_max_bits, bmax);
-}
-
-__global__ __launch_bounds__(TPB) void k_mass_update_heavy(const uint32_t* __restrict__ abort_flag, VoxelArrays v,
-                                                           const float* __restrict__ weights,
-                                                           const float* __restrict__ del_occ,
-                                                           const float* __restrict__ del_free,
-                                                           const FrameConsts* __restrict__ fc, float mass_scale,
-                                                           float p_B, float* __restrict__ born,
-                                                           const int32_t* __restrict__ heavy, DevScalars* sc) {
+                                                     float* __restrict__ born, uint32_t* __restrict__ born_parts) {
     if (*abort_flag) return;
     const float decay = fc ? fc->decay : 1.0f;
     const int lane = threadIdx.x & 63;
-    const uint32_t n = sc->n_heavy_mass;
-    for (uint32_t h = (blockIdx.x * TPB + threadIdx.x) >> 6; h < n; h += gridDim.x * (TPB / 64)) {
-        const int c = __builtin_amdgcn_readfirstlane(heavy[h]);
-        const int st = __builtin_amdgcn_readfirstlane(v.start[c]), en = __builtin_amdgcn_readfirstlane(v.end[c]);
-        const float dyn_pred = wave_seg_sum(weights, st, en);
-        if (lane == 0)
-            atomic_max_nonneg(&sc->born_max_bits,
-                              mass_voxel(v, c, dyn_pred, del_occ, del_free, decay, mass_scale, p_B, born));
+    const int c = blockIdx.x * TPB + threadIdx.x;
+    int st = -1, nb = 0;
+    if (c < V) {
+        st = v.start[c];
+        if (st != -1) nb = unit_count(st, v.end[c]);
     }
+    const bool wide = nb > WAVE_CHAIN_UNITS;
+    float dyn = (st != -1 && !wide) ? chain_sum(usum, st, nb) : 0.0f;
+    for (uint64_t hm = __ballot(wide); hm; hm &= hm - 1) {
+        const int h = __builtin_ctzll(hm);
+        const float s = wave_chain_sum(usum, __builtin_amdgcn_readlane(st, h), __builtin_amdgcn_readlane(nb, h));
+        if (lane == h) dyn = s;
+    }
+    float b = 0.0f;
+    if (c < V) b = mass_voxel(v, c, dyn, del_occ, del_free, decay, mass_scale, p_B, born);
+    block_max_store(b, born_parts + blockIdx.x);
 }
 
+int mass_update_parts(int V) { return (int)blocks_for(V); }
+
 hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
-                              const float* weights, const float* del_occ, const float* del_free,
-                              const FrameConsts* fc, float mass_scale, float p_B, float* born, int32_t* heavy,
-                              DevScalars* sc) {
-    hipLaunchKernelGGL(k_mass_update, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, weights, del_occ,
-                       del_free, fc, mass_scale, p_B, born, heavy, sc);
-    hipLaunchKernelGGL(k_mass_update_heavy, dim3(HEAVY_GRID), dim3(TPB), 0, st, abort_flag, v, weights, del_occ,
-                       del_free, fc, mass_scale, p_B, born, heavy, sc);
+                              const float* weights, const int32_t* keys, int n, float* usum, const float* del_occ,
+                              const float* del_free, const FrameConsts* fc, float mass_scale, float p_B, float* born,
+                              uint32_t* born_parts) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_unit_weight_sums, dim3(unit_blocks(n)), dim3(UNIT_TPB), 0, st, abort_flag, weights,
+                           keys, v.start, v.end, n, usum);
+    hipLaunchKernelGGL(k_mass_update, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, usum, del_occ,
+                       del_free, fc, mass_scale, p_B, born, born_parts);
     return hipGetLastError();
 }
 
-// Per voxel with particles: persistent weight update (mu = pers/pred), mean
-// velocity from the updated weights, then the static-particle penalty.
 // One particle's persistent update (src/common/kernel/update_persistent_particles.cu:23-24,52-53
 // with likelihood 1 and p_A 0, which K3DOM never changes): returns the new
 // weight and, through *pen, the weight after the static penalty
@@ -543,121 +624,133 @@ __device__ __forceinline__ float persist_weight(float w, float m, float vx, floa
     return wn;
 }
 
-__global__ __launch_bounds__(TPB) void k_voxel_particles(const uint32_t* __restrict__ abort_flag, VoxelArrays v,
-                                                         int V, ParticleArrays P, float min_vel,
-                                                         int32_t* __restrict__ heavy, DevScalars* sc) {
-    if (*abort_flag) return;
-    int c = blockIdx.x * TPB + threadIdx.x;
-    float wmax = 0.0f;
-    if (c < V) {
-        const int st = v.start[c];
-        if (st == -1) {
-            v.mvx[c] = 0.0f;
-            v.mvy[c] = 0.0f;
-            v.mvz[c] = 0.0f;
-        } else {
-            const int en = v.end[c];
-            if (en - st + 1 > HEAVY_PARTICLES) {
-                heavy[atomicAdd(&sc->n_heavy_vp, 1u)] = c;
-            } else {
-                const float rho = v.pers[c], pr = v.pred[c];
-                const float m = pr > 0.0f ? rho / pr : 0.0f;
-                v.mu[c] = m;
-                const float min2 = min_vel * min_vel;
-                float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-                int j = st;
-                for (; j + 3 <= en; j += 4) {
-                    float w[4], x[4], y[4], z[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        w[k] = P.w[j + k]; x[k] = P.vx[j + k]; y[k] = P.vy[j + k]; z[k] = P.vz[j + k];
-                    }
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        float pen;
-                        const float wn = persist_weight(w[k], m, x[k], y[k], z[k], min2, &pen);
-                        sx += wn * x[k];
-                        sy += wn * y[k];
-                        sz += wn * z[k];
-                        P.w[j + k] = pen;
-                        if (pen > wmax && pen - pen == 0.0f) wmax = pen;
-                    }
-                }
-                for (; j <= en; ++j) {
-                    const float x = P.vx[j], y = P.vy[j], z = P.vz[j];
-                    float pen;
-                    const float wn = persist_weight(P.w[j], m, x, y, z, min2, &pen);
-                    sx += wn * x;
-                    sy += wn * y;
-                    sz += wn * z;
-                    P.w[j] = pen;
-                    if (pen > wmax && pen - pen == 0.0f) wmax = pen;
-                }
-                v.mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
-                v.mvy[c] = rho > 0.0f ? sy / rho : 0.0f;
-                v.mvz[c] = rho > 0.0f ? sz / rho : 0.0f;
-            }
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
-}
+// mu of a voxel with particles: persistent over predicted dynamic mass
+__device__ __forceinline__ float voxel_mu(float rho, float pr) { return pr > 0.0f ? rho / pr : 0.0f; }
 
-__global__ __launch_bounds__(TPB) void k_voxel_particles_heavy(const uint32_t* __restrict__ abort_flag,
-                                                               VoxelArrays v, ParticleArrays P, float min_vel,
-                                                               const int32_t* __restrict__ heavy, DevScalars* sc) {
+// Persistent update of every sorted particle, with mu of its voxel: the
+// penalised weight goes to P.w (each position by its owning wave), and the
+// unit sums of new weight x velocity to ux, uy, uz.
+__global__ __launch_bounds__(UNIT_TPB) void k_unit_persist(const uint32_t* __restrict__ abort_flag,
+                                                           ParticleArrays P, const float* __restrict__ wsrc,
+                                                           VoxelArrays v, int n, float min_vel,
+                                                           float* __restrict__ ux, float* __restrict__ uy,
+                                                           float* __restrict__ uz, uint32_t* __restrict__ joint_parts) {
+    __shared__ float lx_all[UNIT_WAVES][UNIT_SPAN];
+    __shared__ float ly_all[UNIT_WAVES][UNIT_SPAN];
+    __shared__ float lz_all[UNIT_WAVES][UNIT_SPAN];
+    __shared__ int32_t lk_all[UNIT_WAVES][UNIT_SPAN];
     if (*abort_flag) return;
-    const int lane = threadIdx.x & 63;
-    const uint32_t n = sc->n_heavy_vp;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r0 = (blockIdx.x * UNIT_WAVES + wv) * UNIT_WAVE;
+    float* lx = lx_all[wv];
+    float* ly = ly_all[wv];
+    float* lz = lz_all[wv];
+    int32_t* lk = lk_all[wv];
+    const int m = max(0, min(n - r0, UNIT_SPAN));
     const float min2 = min_vel * min_vel;
     float wmax = 0.0f;
-    for (uint32_t h = (blockIdx.x * TPB + threadIdx.x) >> 6; h < n; h += gridDim.x * (TPB / 64)) {
-        const int c = __builtin_amdgcn_readfirstlane(heavy[h]);
-        const int st = __builtin_amdgcn_readfirstlane(v.start[c]), en = __builtin_amdgcn_readfirstlane(v.end[c]);
-        const float rho = v.pers[c], pr = v.pred[c];
-        const float m = pr > 0.0f ? rho / pr : 0.0f;
-        const int nb = (en - st + K3_SEG_BLOCK) / K3_SEG_BLOCK;
-        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-        for (int b0 = 0; b0 < nb; b0 += 64) {
-            const int b = b0 + lane;
-            float bx = 0.0f, by = 0.0f, bz = 0.0f;
-            if (b < nb) {
-                const int j0 = st + b * K3_SEG_BLOCK, j1 = min(en, j0 + K3_SEG_BLOCK - 1);
-                for (int j = j0; j <= j1; ++j) {
-                    const float x = P.vx[j], y = P.vy[j], z = P.vz[j];
-                    float pen;
-                    const float wn = persist_weight(P.w[j], m, x, y, z, min2, &pen);
-                    bx += wn * x;
-                    by += wn * y;
-                    bz += wn * z;
-                    P.w[j] = pen;
-                    if (pen > wmax && pen - pen == 0.0f) wmax = pen;
-                }
+#pragma unroll 4
+    for (int k = 0; k < UNIT_SPAN / 64; ++k) {
+        const int j = k * 64 + lane;
+        if (j < m) {
+            const int i = r0 + j;
+            const int kj = P.key[i];
+            const float x = P.vx[i], y = P.vy[i], z = P.vz[i];
+            const float mu = voxel_mu(v.pers[kj], v.pred[kj]);
+            float pen;
+            const float wn = persist_weight(wsrc[i], mu, x, y, z, min2, &pen);
+            lx[j] = wn * x;
+            ly[j] = wn * y;
+            lz[j] = wn * z;
+            lk[j] = kj;
+            if (j < UNIT_WAVE) {
+                P.w[i] = pen;
+                if (pen > wmax && pen - pen == 0.0f) wmax = pen;
             }
-            const int cnt = min(64, nb - b0);
-            for (int q = 0; q < cnt; ++q) {
-                sx += lane_value(bx, q);
-                sy += lane_value(by, q);
-                sz += lane_value(bz, q);
-            }
-        }
-        if (lane == 0) {
-            v.mu[c] = m;
-            v.mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
-            v.mvy[c] = rho > 0.0f ? sy / rho : 0.0f;
-            v.mvz[c] = rho > 0.0f ? sz / rho : 0.0f;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
-    if (lane == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
+    block_max_store(wmax, joint_parts + blockIdx.x);
+    const int q0 = UNIT_LANE * lane, q1 = min(q0 + UNIT_LANE, m);
+    if (q0 >= q1) return;
+    for (int u = first_unit_start(r0 + q0, lk[q0], v.start, v.end) - r0; u < q1;) {
+        const int e = unit_end(lk, u, m);
+        float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+        int i = u;
+        for (; i + 2 <= e; i += 2) {
+            const float x0 = lx[i], y0 = ly[i], z0 = lz[i];
+            const float x1 = lx[i + 1], y1 = ly[i + 1], z1 = lz[i + 1];
+            sx += x0; sy += y0; sz += z0;
+            sx += x1; sy += y1; sz += z1;
+        }
+        if (i < e) {
+            sx += lx[i]; sy += ly[i]; sz += lz[i];
+        }
+        ux[r0 + u] = sx;
+        uy[r0 + u] = sy;
+        uz[r0 + u] = sz;
+        u = e;
+    }
 }
 
+// Per voxel: mu and the mean velocity from the updated weights
+// (src/common/kernel/update_persistent_particles.cu:17-55).
+__global__ __launch_bounds__(TPB) void k_voxel_particles(const uint32_t* __restrict__ abort_flag, VoxelArrays v,
+                                                         int V, const float* __restrict__ ux,
+                                                         const float* __restrict__ uy,
+                                                         const float* __restrict__ uz) {
+    if (*abort_flag) return;
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * TPB + threadIdx.x;
+    int st = -1, nb = 0;
+    if (c < V) {
+        st = v.start[c];
+        if (st != -1) nb = unit_count(st, v.end[c]);
+    }
+    const bool wide = nb > WAVE_CHAIN_UNITS;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    if (st != -1 && !wide) {
+        sx = chain_sum(ux, st, nb);
+        sy = chain_sum(uy, st, nb);
+        sz = chain_sum(uz, st, nb);
+    }
+    for (uint64_t hm = __ballot(wide); hm; hm &= hm - 1) {
+        const int h = __builtin_ctzll(hm);
+        const int hs = __builtin_amdgcn_readlane(st, h), hn = __builtin_amdgcn_readlane(nb, h);
+        const float tx = wave_chain_sum(ux, hs, hn);
+        const float ty = wave_chain_sum(uy, hs, hn);
+        const float tz = wave_chain_sum(uz, hs, hn);
+        if (lane == h) {
+            sx = tx;
+            sy = ty;
+            sz = tz;
+        }
+    }
+    if (c >= V) return;
+    if (st == -1) {
+        v.mvx[c] = 0.0f;
+        v.mvy[c] = 0.0f;
+        v.mvz[c] = 0.0f;
+    } else {
+        const float rho = v.pers[c];
+        v.mu[c] = voxel_mu(rho, v.pred[c]);
+        v.mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
+        v.mvy[c] = rho > 0.0f ? sy / rho : 0.0f;
+        v.mvz[c] = rho > 0.0f ? sz / rho : 0.0f;
+    }
+}
+
+int voxel_particles_parts(int n) { return n > 0 ? (int)unit_blocks(n) : 0; }
+
 hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
-                                  ParticleArrays P, float min_vel, int32_t* heavy, DevScalars* sc) {
-    hipLaunchKernelGGL(k_voxel_particles, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, P, min_vel,
-                       heavy, sc);
-    hipLaunchKernelGGL(k_voxel_particles_heavy, dim3(HEAVY_GRID), dim3(TPB), 0, st, abort_flag, v, P, min_vel,
-                       heavy, sc);
+                                  ParticleArrays P, const float* wsrc, int n, float min_vel, float* usum3,
+                                  uint32_t* joint_parts) {
+    float* ux = usum3;
+    float* uy = usum3 + n;
+    float* uz = usum3 + 2 * (size_t)n;
+    if (n > 0)
+        hipLaunchKernelGGL(k_unit_persist, dim3(unit_blocks(n)), dim3(UNIT_TPB), 0, st, abort_flag, P, wsrc, v, n,
+                           min_vel, ux, uy, uz, joint_parts);
+    hipLaunchKernelGGL(k_voxel_particles, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, ux, uy, uz);
     return hipGetLastError();
 }
 
@@ -670,8 +763,20 @@ __device__ __forceinline__ float cat_load(const float* a, int na, const float* b
     return i < na ? a[i] : (i < na + nb ? b[i - na] : 0.0f);
 }
 
-__global__ void k_fixed_exponent(const uint32_t* max_bits, int count, int32_t* E) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *E = k3_fixed_exponent(k3_u2f(*max_bits), (uint32_t)count);
+// Exponent of the fixed-point CDF from the largest of nparts per-block maxima
+// (float bits of non-negative values, so their unsigned order is the float order).
+__global__ __launch_bounds__(1024) void k_fixed_exponent(const uint32_t* __restrict__ parts, int nparts, int count,
+                                                         int32_t* E) {
+    __shared__ uint32_t wm[16];
+    uint32_t m = 0;
+    for (int i = threadIdx.x; i < nparts; i += 1024) m = max(m, parts[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < 16; ++i) m = max(m, wm[i]);
+        *E = k3_fixed_exponent(k3_u2f(m), (uint32_t)count);
+    }
 }
 
 // inclusive block scan of one u64 per thread (256 threads); returns the block total
@@ -757,11 +862,11 @@ __global__ __launch_bounds__(TPB) void k_fixed_down(const float* __restrict__ a,
 }
 
 hipError_t launch_fixed_scan(hipStream_t st, const float* a, int na, const float* b, int nb,
-                             const uint32_t* max_bits, int32_t* E_out, uint64_t* cdf, uint64_t* blocksums,
-                             uint64_t* total) {
+                             const uint32_t* max_parts, int nparts, int32_t* E_out, uint64_t* cdf,
+                             uint64_t* blocksums, uint64_t* total) {
     const int n = na + nb;
     const int nblk = fixed_scan_blocks(n);
-    hipLaunchKernelGGL(k_fixed_exponent, dim3(1), dim3(64), 0, st, max_bits, n, E_out);
+    hipLaunchKernelGGL(k_fixed_exponent, dim3(1), dim3(1024), 0, st, max_parts, nparts, n, E_out);
     hipLaunchKernelGGL(k_fixed_reduce, dim3(nblk), dim3(TPB), 0, st, a, na, b, nb, E_out, blocksums);
     hipLaunchKernelGGL(k_scan_top_u64, dim3(1), dim3(1024), 0, st, blocksums, nblk, total);
     hipLaunchKernelGGL(k_fixed_down, dim3(nblk), dim3(TPB), 0, st, a, na, b, nb, E_out, blocksums, cdf);
@@ -819,7 +924,7 @@ __device__ __forceinline__ int slot_owner(const uint32_t* __restrict__ bnd, int 
 __global__ __launch_bounds__(TPB) void k_newborn(const uint32_t* __restrict__ abort_flag, const uint32_t* __restrict__ bnd, int V, int B, int gs,
                                                  const float* __restrict__ wua, ParticleArrays NB, float vmax,
                                                  float min_vel, const FrameConsts* __restrict__ fc, uint32_t k0,
-                                                 uint32_t k1, DevScalars* sc) {
+                                                 uint32_t k1, uint32_t* __restrict__ joint_parts) {
     if (*abort_flag) return;
     const uint32_t frame = fc->frame;
     int k = blockIdx.x * TPB + threadIdx.x;
@@ -838,17 +943,17 @@ __global__ __launch_bounds__(TPB) void k_newborn(const uint32_t* __restrict__ ab
         NB.vx[k] = vx; NB.vy[k] = vy; NB.vz[k] = vz;
         NB.w[k] = w;
         NB.key[k] = c;
-        if (w > 0.0f && w - w == 0.0f) wmax = w;
+        wmax = w;
     }
-    for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&sc->joint_max_bits, wmax);
+    block_max_store(wmax, joint_parts + blockIdx.x);
 }
+int newborn_parts(int B) { return B > 0 ? (int)blocks_for(B) : 0; }
 hipError_t launch_newborn(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int B, int gs, const float* wua,
                           ParticleArrays NB, float vmax, float min_vel, const FrameConsts* fc, uint32_t k0,
-                          uint32_t k1, DevScalars* sc) {
+                          uint32_t k1, uint32_t* joint_parts) {
     if (B <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_newborn, dim3(blocks_for(B)), dim3(TPB), 0, st, abort_flag, bnd, V, B, gs, wua, NB, vmax, min_vel,
-                       fc, k0, k1, sc);
+                       fc, k0, k1, joint_parts);
     return hipGetLastError();
 }
 
