@@ -125,7 +125,7 @@ k3dom_status alloc_level(k3dom_handle* h, int l, long long list_cap) {
     HIP_TRY(h, dalloc(h, &B.counts, (size_t)B.slot_cap));
     HIP_TRY(h, dalloc(h, &B.slot_off, (size_t)B.slot_cap + 1));
     HIP_TRY(h, dalloc(h, &B.list, (size_t)std::max(1ll, list_cap)));
-    HIP_TRY(h, dalloc(h, &B.bsum, (size_t)scan_blocks(std::max<long long>(B.slot_cap, L.nparents + 1))));
+    HIP_TRY(h, dalloc_zero(h, &B.bsum, (size_t)scan_scratch_words(std::max<long long>(B.slot_cap, L.nparents + 1))));
     return K3DOM_OK;
 }
 
@@ -671,7 +671,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(dalloc(h, &h->sort.i0, N));
     CREATE_TRY(dalloc(h, &h->sort.i1, N));
     CREATE_TRY(dalloc(h, &h->sort.hist, (size_t)256 * sort_blocks(h->N) + 1));
-    CREATE_TRY(dalloc(h, &h->sort.bsum, (size_t)scan_blocks(256ll * sort_blocks(h->N))));
+    CREATE_TRY(dalloc_zero(h, &h->sort.bsum, (size_t)scan_scratch_words(256ll * sort_blocks(h->N))));
     CREATE_TRY(dalloc(h, &h->cdf_v, V));
     CREATE_TRY(dalloc(h, &h->cdf_j, N + B));
     CREATE_TRY(dalloc(h, &h->blocksums, (size_t)fixed_scan_blocks((int)std::max(V, N + B))));
@@ -694,7 +694,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(dalloc(h, &I.light_cnt, T));
     CREATE_TRY(dalloc(h, &I.heavy_off, T + 1));
     CREATE_TRY(dalloc(h, &I.light_off, T + 1));
-    CREATE_TRY(dalloc(h, &I.bsum, (size_t)scan_blocks((long long)T)));
+    CREATE_TRY(dalloc_zero(h, &I.bsum, (size_t)scan_scratch_words((long long)T)));
     CREATE_TRY(dalloc(h, &I.tile_partial, 2 * T));
     {
         const char* tiny = std::getenv("K3DOM_TEST_TINY_CAPACITY");

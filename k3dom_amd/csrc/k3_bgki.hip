@@ -173,6 +173,15 @@ __device__ __forceinline__ ItemCtx item_ctx(int level, const LevelDesc& L, uint3
     return x;
 }
 
+// Children are handled in groups of LV_CG per block, so a box with many
+// children (level 0) is spread over several blocks: block b takes item
+// b / ncg and children [LV_CG * (b % ncg), ...) of it.
+constexpr int LV_CG = 32;
+__host__ __device__ inline int child_groups(const LevelDesc& L) { return (L.nch + LV_CG - 1) / LV_CG; }
+
+// Per-item, per-child counts of the candidates of the item's segment that
+// pass the child's test.  Lane c of each wavefront keeps child c's count of
+// that wavefront's candidates; the four are added at the end.
 __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConsts* __restrict__ fc, TileGeom tg,
                                                      LevelDesc L, float ls, const float4* __restrict__ pxyz,
                                                      const uint32_t* __restrict__ parent_off,
@@ -180,39 +189,51 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
                                                      const uint32_t* __restrict__ parent_list,
                                                      const uint32_t* __restrict__ item_base,
                                                      uint32_t* __restrict__ counts, const DevScalars* sc) {
-    __shared__ float4 cb[MAX_CHILDREN];
-    __shared__ uint32_t cnt[MAX_CHILDREN];
+    __shared__ float4 cb[LV_CG];
+    __shared__ uint32_t wcnt[TPB / 64][LV_CG];
     if (sc->overflow) return;
     const GridGeom g = fc->g;
     const int m = fc->m;
-    const uint32_t item = blockIdx.x;
+    const int ncg = child_groups(L);
+    const uint32_t item = blockIdx.x / ncg;
+    const int c0 = (blockIdx.x % ncg) * LV_CG, nc = min(LV_CG, L.nch - c0);
     if (item >= item_base[L.nparents]) return;
-    const int t = threadIdx.x, lane = t & 63;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
-    for (int c = t; c < L.nch; c += TPB) {
+    if (t < nc) {
         int t0[3], t1[3];
-        int box = child_box(level, L, tg, x.P, c, t0, t1);
-        cb[c] = box < 0 ? make_float4(0.0f, 0.0f, 0.0f, -1.0f)
+        const int box = child_box(level, L, tg, x.P, c0 + t, t0, t1);
+        cb[t] = box < 0 ? make_float4(0.0f, 0.0f, 0.0f, -1.0f)
                         : box_sphere(g, ls, t0[0], t0[1], t0[2], t1[0], t1[1], t1[2]);
-        cnt[c] = 0;
     }
     __syncthreads();
+    uint32_t mine = 0;  // lane c: candidates of this wavefront passing child c
     for (uint32_t base = x.begin; base < x.end; base += TPB) {
         const uint32_t e = base + t;
         const bool in = e < x.end;
         const uint32_t idx = in ? (level ? parent_list[x.poff + e] : e) : 0u;
         const Cand cd = make_cand(pxyz[idx], in);
-        for (int c = 0; c < L.nch; ++c) {
-            uint64_t bal = __ballot(cand_hits(cb[c], cd));
-            if (lane == 0 && bal) atomicAdd(&cnt[c], (uint32_t)__popcll(bal));
+        for (int c = 0; c < nc; ++c) {
+            const uint32_t n = (uint32_t)__popcll(__ballot(cand_hits(cb[c], cd)));
+            mine += lane == c ? n : 0u;
         }
     }
+    if (lane < nc) wcnt[w][lane] = mine;
     __syncthreads();
-    for (int c = t; c < L.nch; c += TPB) counts[x.ib * L.nch + c * x.nsegP + x.seg] = cnt[c];
+    if (t < nc) counts[x.ib * L.nch + (c0 + t) * x.nsegP + x.seg] = wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
 }
 
+// Ordered fill: each candidate passing child c goes to that child's list at
+// the scanned offset of (item, c) plus the number of earlier passing
+// candidates of the item.  Per chunk of TPB candidates, every wavefront
+// records its ballot mask per child in LDS; after one barrier each wavefront
+// writes its candidates behind those of the lower wavefronts and of earlier
+// chunks.  Masks and running counts are double-buffered by chunk parity, so
+// one barrier per chunk suffices (a wavefront reaching chunk k + 2 has passed
+// the barrier of chunk k + 1, which every wavefront reaches only after
+// finishing chunk k).
 __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts* __restrict__ fc, TileGeom tg,
-                                                     LevelDesc L, float ls, const float4* __restrict__ pxyz,
+                                                    LevelDesc L, float ls, const float4* __restrict__ pxyz,
                                                     const uint32_t* __restrict__ parent_off,
                                                     const uint32_t* __restrict__ parent_len,
                                                     const uint32_t* __restrict__ parent_list,
@@ -220,10 +241,10 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
                                                     const uint32_t* __restrict__ slot_off,
                                                     uint32_t* __restrict__ list, long long list_cap,
                                                     DevScalars* sc) {
-    __shared__ float4 cb[MAX_CHILDREN];
-    __shared__ uint32_t ob[MAX_CHILDREN];
-    __shared__ uint32_t run[MAX_CHILDREN];
-    __shared__ uint32_t wc[4];
+    __shared__ float4 cb[LV_CG];
+    __shared__ uint32_t ob[LV_CG];
+    __shared__ uint32_t run[2][LV_CG];
+    __shared__ uint64_t wm[2][TPB / 64][LV_CG];
     if (sc->overflow) return;
     const GridGeom g = fc->g;
     const int m = fc->m;
@@ -237,35 +258,46 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
         }
         return;
     }
-    const uint32_t item = blockIdx.x;
+    const int ncg = child_groups(L);
+    const uint32_t item = blockIdx.x / ncg;
+    const int c0 = (blockIdx.x % ncg) * LV_CG, nc = min(LV_CG, L.nch - c0);
     if (item >= n_items) return;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
-    for (int c = t; c < L.nch; c += TPB) {
+    if (t < nc) {
         int t0[3], t1[3];
-        int box = child_box(level, L, tg, x.P, c, t0, t1);
-        cb[c] = box < 0 ? make_float4(0.0f, 0.0f, 0.0f, -1.0f)
+        const int box = child_box(level, L, tg, x.P, c0 + t, t0, t1);
+        cb[t] = box < 0 ? make_float4(0.0f, 0.0f, 0.0f, -1.0f)
                         : box_sphere(g, ls, t0[0], t0[1], t0[2], t1[0], t1[1], t1[2]);
-        ob[c] = slot_off[x.ib * L.nch + c * x.nsegP + x.seg];
-        run[c] = 0;
+        ob[t] = slot_off[x.ib * L.nch + (c0 + t) * x.nsegP + x.seg];
+        run[0][t] = 0;
     }
     __syncthreads();
-    for (uint32_t base = x.begin; base < x.end; base += TPB) {
+    int buf = 0;
+    for (uint32_t base = x.begin; base < x.end; base += TPB, buf ^= 1) {
         const uint32_t e = base + t;
         const bool in = e < x.end;
         const uint32_t idx = in ? (level ? parent_list[x.poff + e] : e) : 0u;
         const Cand cd = make_cand(pxyz[idx], in);
-        for (int c = 0; c < L.nch; ++c) {
-            const bool pass = cand_hits(cb[c], cd);
-            const uint64_t bal = __ballot(pass);
-            if (lane == 0) wc[w] = (uint32_t)__popcll(bal);
-            __syncthreads();
-            uint32_t pre = 0;
-            for (int ww = 0; ww < w; ++ww) pre += wc[ww];
-            const uint32_t tot = wc[0] + wc[1] + wc[2] + wc[3];
-            if (pass) list[ob[c] + run[c] + pre + bg_lanes_below(bal)] = idx;
-            __syncthreads();
-            if (t == 0) run[c] += tot;
+        uint64_t mymask = 0;  // lane c: this wavefront's mask for child c
+        for (int c = 0; c < nc; ++c) {
+            const uint64_t bal = __ballot(cand_hits(cb[c], cd));
+            mymask = lane == c ? bal : mymask;
+        }
+        if (lane < nc) wm[buf][w][lane] = mymask;
+        __syncthreads();
+        for (int c = 0; c < nc; ++c) {
+            const uint64_t bal = wm[buf][w][c];
+            if ((bal >> lane) & 1ull) {
+                uint32_t pre = run[buf][c];
+                for (int ww = 0; ww < w; ++ww) pre += (uint32_t)__popcll(wm[buf][ww][c]);
+                list[ob[c] + pre + bg_lanes_below(bal)] = idx;
+            }
+        }
+        if (t < nc) {
+            uint32_t r = run[buf][t];
+            for (int ww = 0; ww < TPB / 64; ++ww) r += (uint32_t)__popcll(wm[buf][ww][t]);
+            run[buf ^ 1][t] = r;
         }
     }
     if (item == 0 && t == 0) sc->level_total[level] = total;
@@ -310,7 +342,7 @@ hipError_t launch_build_level(hipStream_t st, int level, const FrameConsts* fc, 
     const uint32_t* plist = parent ? parent->list : nullptr;
     hipLaunchKernelGGL(k_level_nseg, dim3(blocks_for(L.nparents)), dim3(TPB), 0, st, level, L, fc, plen, B.nseg, sc);
     (void)launch_scan_u32(st, B.nseg, B.item_base, B.bsum, L.nparents, nullptr, 1, &sc->level_items[level]);
-    const unsigned grid = (unsigned)std::max(1ll, B.items_cap);
+    const unsigned grid = (unsigned)std::max(1ll, B.items_cap * child_groups(L));
     hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(TPB), 0, st, level, fc, tg, L, ls, pxyz, poff, plen, plist,
                        B.item_base, B.counts, sc);
     (void)launch_scan_u32(st, B.counts, B.slot_off, B.bsum, B.slot_cap, B.item_base + L.nparents, (uint32_t)L.nch,

@@ -105,6 +105,13 @@ hipError_t dalloc(k3dom_handle* h, T** p, size_t count) {
     }
     return e;
 }
+// dalloc, then zero the memory on the map's stream (ahead of its later work)
+template <class T>
+hipError_t dalloc_zero(k3dom_handle* h, T** p, size_t count) {
+    hipError_t e = dalloc(h, p, count);
+    if (e == hipSuccess) e = hipMemsetAsync(*p, 0, sizeof(T) * (count ? count : 1), h->stream);
+    return e;
+}
 void dfree(k3dom_handle* h, void* p);
 k3dom_status ensure_points(k3dom_handle* h, int m);
 GridGeom grid_geom(const k3dom_handle* h);

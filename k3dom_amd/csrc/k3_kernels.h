@@ -55,10 +55,12 @@ struct FrameConsts {
 };
 
 // ------------------------------------------------------------------ scans
-int scan_blocks(long long n_max);
-// Exclusive scan of n = n_dev ? (*n_dev) * n_mul : n_max elements; writes the
-// total to out[n] and *total.  bsum needs scan_blocks(n_max) entries.
-hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* bsum, long long n_max,
+// u32 words of scan scratch for up to n_max elements; the scratch must be
+// zeroed once after allocation (every scan leaves it zeroed again).
+long long scan_scratch_words(long long n_max);
+// Exclusive scan of n = n_dev ? (*n_dev) * n_mul : n_max elements in one
+// launch; writes the total to out[n] and *total.  scratch: see above.
+hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* scratch, long long n_max,
                            const uint32_t* n_dev, uint32_t n_mul, uint32_t* total);
 
 // ------------------------------------------------------------------ particles / voxels
@@ -73,7 +75,7 @@ hipError_t launch_fill_voxels(hipStream_t st, VoxelArrays v, int V, float prior_
 struct SortScratch {
     uint32_t *k0, *k1, *i0, *i1;
     uint32_t* hist;  // 256 * nblocks + 1
-    uint32_t* bsum;  // scan_blocks(256 * nblocks)
+    uint32_t* bsum;  // scan scratch for 256 * nblocks elements
 };
 int sort_blocks(int n);
 hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int key_bits, SortScratch s,

@@ -185,12 +185,10 @@ __global__ __launch_bounds__(TPB) void k_radix_scatter(const uint32_t* __restric
     }
 }
 
-// Exclusive scan of u32, three phases (tile sums, scan of tile sums, tile
-// scans).  The element count is n_host, or (*n_dev) * n_mul when n_dev is
-// given (a count produced on the device); the grid is sized for n_max.  The
-// total is also written to out[n] and, if given, to *total.  in may equal out.
-constexpr int SC_ITEMS = 8;
-constexpr int SC_TILE = TPB * SC_ITEMS;
+// Exclusive scans of u32.  The element count is n_host, or (*n_dev) * n_mul
+// when n_dev is given (a count produced on the device); the grid is sized for
+// n_max.  The total is also written to out[n] and, if given, to *total.  in
+// may equal out.
 
 __device__ __forceinline__ uint32_t scan_count(const uint32_t* n_dev, uint32_t n_mul, uint32_t n_host) {
     return n_dev ? (*n_dev) * n_mul : n_host;
@@ -213,79 +211,6 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* ld
     __syncthreads();
     return x - v + add;
 }
-
-__global__ __launch_bounds__(TPB) void k_scan_reduce_u32(const uint32_t* __restrict__ in,
-                                                         const uint32_t* __restrict__ n_dev, uint32_t n_mul,
-                                                         uint32_t n_host, uint32_t* __restrict__ bsum) {
-    __shared__ uint32_t lds4[4];
-    const uint32_t n = scan_count(n_dev, n_mul, n_host);
-    const uint32_t base = blockIdx.x * SC_TILE;
-    if (base >= n) return;
-    const uint32_t b0 = base + threadIdx.x * SC_ITEMS;
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < SC_ITEMS; ++k) s += (b0 + k < n) ? in[b0 + k] : 0u;
-    uint32_t tot;
-    block_excl_scan_u32(s, lds4, &tot);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_top_u32(uint32_t* __restrict__ bsum, const uint32_t* __restrict__ n_dev,
-                                                       uint32_t n_mul, uint32_t n_host, uint32_t* __restrict__ out,
-                                                       uint32_t* __restrict__ total) {
-    __shared__ uint32_t part[1024];
-    const uint32_t n = scan_count(n_dev, n_mul, n_host);
-    const int nb = (int)((n + SC_TILE - 1) / SC_TILE);
-    const int t = threadIdx.x;
-    const int per = (nb + 1023) / 1024;
-    const int b = t * per, e = min(nb, b + per);
-    uint32_t s = 0;
-    for (int i = b; i < e; ++i) s += bsum[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = (t >= off) ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[t] - s;
-    for (int i = b; i < e; ++i) {
-        uint32_t v = bsum[i];
-        bsum[i] = run;
-        run += v;
-    }
-    if (t == 1023) {
-        out[n] = part[1023];
-        if (total) *total = part[1023];
-    }
-}
-
-__global__ __launch_bounds__(TPB) void k_scan_down_u32(const uint32_t* in, const uint32_t* __restrict__ n_dev,
-                                                       uint32_t n_mul, uint32_t n_host,
-                                                       const uint32_t* __restrict__ bsum, uint32_t* out) {
-    __shared__ uint32_t lds4[4];
-    const uint32_t n = scan_count(n_dev, n_mul, n_host);
-    const uint32_t base = blockIdx.x * SC_TILE;
-    if (base >= n) return;
-    const uint32_t b0 = base + threadIdx.x * SC_ITEMS;
-    uint32_t v[SC_ITEMS];
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < SC_ITEMS; ++k) {
-        v[k] = (b0 + k < n) ? in[b0 + k] : 0u;
-        s += v[k];
-    }
-    uint32_t tot;
-    uint32_t run = block_excl_scan_u32(s, lds4, &tot) + bsum[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < SC_ITEMS; ++k) {
-        if (b0 + k < n) out[b0 + k] = run;
-        run += v[k];
-    }
-}
-
-int scan_blocks(long long n_max) { return (int)std::max(1ll, (n_max + SC_TILE - 1) / SC_TILE); }
 
 // One-workgroup scan for up to SC_SMALL elements (one launch instead of three).
 constexpr int SC_SMALL_ITEMS = 16;
@@ -329,17 +254,121 @@ __global__ __launch_bounds__(1024) void k_scan_small_u32(const uint32_t* in, con
     }
 }
 
-hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* bsum, long long n_max,
+// Single-pass exclusive scan (decoupled look-back).  Tiles of LB_TILE
+// elements are taken in the order blocks start (a ticket counter), so a block
+// only waits for tiles whose blocks are already running.  Each block
+// publishes its tile sum at once, then wavefront 0 reads the status words of
+// the preceding tiles, 64 at a time, adding tile sums back to the nearest
+// tile that has published its inclusive prefix, and publishes its own.  A
+// status word is (flag << 32) | value, flag 1 = tile sum, 2 = inclusive
+// prefix, 0 = not yet written.  The last block to finish clears the status
+// words and both counters, so the scratch is ready for the next launch (it
+// must be zero before the first one).
+// Scratch (u32 words): [0] ticket, [1] finished blocks, [2..] u64 statuses.
+constexpr int LB_ITEMS = 16;
+constexpr int LB_TILE = TPB * LB_ITEMS;
+constexpr uint64_t LB_SUM = 1ull << 32, LB_PREFIX = 2ull << 32;
+
+static inline long long lb_tiles(long long n_max) { return std::max(1ll, (n_max + LB_TILE - 1) / LB_TILE); }
+
+__global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, const uint32_t* __restrict__ n_dev,
+                                                           uint32_t n_mul, uint32_t n_host, uint32_t* out,
+                                                           uint32_t* __restrict__ total, uint32_t* __restrict__ scr,
+                                                           uint32_t nblocks) {
+    __shared__ uint32_t s_tile, s_excl, s_last;
+    __shared__ uint32_t lds4[4];
+    uint32_t* ticket = scr;
+    uint32_t* finished = scr + 1;
+    uint64_t* status = reinterpret_cast<uint64_t*>(scr + 2);
+    const int t = threadIdx.x, lane = t & 63;
+    if (t == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t n = scan_count(n_dev, n_mul, n_host);
+    const uint32_t ntiles = (n + LB_TILE - 1) / LB_TILE;
+    if (tile < ntiles) {
+        const uint32_t b0 = tile * LB_TILE + t * LB_ITEMS;
+        uint32_t v[LB_ITEMS];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < LB_ITEMS; ++k) {
+            v[k] = (b0 + k < n) ? in[b0 + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t tot;
+        const uint32_t x = block_excl_scan_u32(sum, lds4, &tot);
+        if (t == 0)
+            __hip_atomic_store(&status[tile], (tile == 0 ? LB_PREFIX : LB_SUM) | tot, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        if (t < 64) {
+            uint32_t excl = 0;
+            for (int j = (int)tile - 1; j >= 0; j -= 64) {
+                const int jj = j - lane;
+                uint64_t st = LB_PREFIX;  // before tile 0: an empty prefix
+                if (jj >= 0) {
+                    st = __hip_atomic_load(&status[jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    // the tile of jj was taken before this one, so its sum is on its way
+                    while ((st >> 32) == 0) {
+                        __builtin_amdgcn_s_sleep(1);
+                        st = __hip_atomic_load(&status[jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+                const uint64_t pm = __ballot((st >> 32) == 2);
+                const int stop = pm ? __builtin_ctzll(pm) : 64;
+                uint32_t add = lane <= stop ? (uint32_t)st : 0u;
+                for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o);
+                excl += add;
+                if (pm) break;
+            }
+            if (t == 0) {
+                if (tile > 0)
+                    __hip_atomic_store(&status[tile], LB_PREFIX | (excl + tot), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                s_excl = excl;
+            }
+        }
+        __syncthreads();
+        uint32_t run = s_excl + x;
+#pragma unroll
+        for (int k = 0; k < LB_ITEMS; ++k) {
+            if (b0 + k < n) out[b0 + k] = run;
+            run += v[k];
+        }
+        if (tile == ntiles - 1 && t == 0) {
+            out[n] = s_excl + tot;
+            if (total) *total = s_excl + tot;
+        }
+    } else if (ntiles == 0 && tile == 0 && t == 0) {
+        out[0] = 0;
+        if (total) *total = 0;
+    }
+    // the last block to finish resets the scratch for the next launch
+    if (t == 0) {
+        __threadfence();
+        s_last = atomicAdd(finished, 1u) == nblocks - 1;
+    }
+    __syncthreads();
+    if (s_last) {
+        for (uint32_t i = t; i < nblocks; i += TPB)
+            __hip_atomic_store(&status[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == 0) {
+            __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(finished, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+long long scan_scratch_words(long long n_max) { return 2 + 2 * lb_tiles(n_max); }
+
+hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* scratch, long long n_max,
                            const uint32_t* n_dev, uint32_t n_mul, uint32_t* total) {
     if (n_max <= SC_SMALL) {
         hipLaunchKernelGGL(k_scan_small_u32, dim3(1), dim3(1024), 0, st, in, n_dev, n_mul, (uint32_t)n_max, out, total);
         return hipGetLastError();
     }
-    const unsigned nb = (unsigned)scan_blocks(n_max);
-    const uint32_t nh = (uint32_t)n_max;
-    hipLaunchKernelGGL(k_scan_reduce_u32, dim3(nb), dim3(TPB), 0, st, in, n_dev, n_mul, nh, bsum);
-    hipLaunchKernelGGL(k_scan_top_u32, dim3(1), dim3(1024), 0, st, bsum, n_dev, n_mul, nh, out, total);
-    hipLaunchKernelGGL(k_scan_down_u32, dim3(nb), dim3(TPB), 0, st, in, n_dev, n_mul, nh, bsum, out);
+    const uint32_t nb = (uint32_t)lb_tiles(n_max);
+    hipLaunchKernelGGL(k_scan_lookback_u32, dim3(nb), dim3(TPB), 0, st, in, n_dev, n_mul, (uint32_t)n_max, out,
+                       total, scratch, nb);
     return hipGetLastError();
 }
 
