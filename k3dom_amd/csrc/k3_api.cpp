@@ -373,6 +373,10 @@ void finish_frame(k3dom_handle* h, double t, int m) {
 
 k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
     const bool normal = h->updated_time > 0;
+    {
+        k3dom_status st = begin_frame_voxels(h, normal);
+        if (st != K3DOM_OK) return st;
+    }
     if (h->timing) for (float& x : h->stage_ms) x = 0.0f;
     fill_frame_consts(h, h->h_fc, t, m);
     const bool use_graph = normal && h->use_graph && !h->timing;
@@ -430,11 +434,29 @@ k3dom_status reset_map(k3dom_handle* h) {
                                       h->p.prior_dynamic));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     h->vcur = 0;
+    h->shift_pending = false;
     h->frame = 0;
     h->first_pose = false;
     h->updated_time = -1.0;
     for (int i = 0; i < 3; ++i) h->sensor[i] = h->center[i] = 0.0f;
     return K3DOM_OK;
+}
+
+k3dom_status settle_shift(k3dom_handle* h) {
+    if (!h->shift_pending) return K3DOM_OK;
+    HIP_TRY(h, launch_shift_voxels(h->stream, h->vox[h->vcur], h->vox[1 - h->vcur], h->gs, h->gsz, h->pend_move[0],
+                                   h->pend_move[1], h->pend_move[2], h->p.prior_free, h->p.prior_static,
+                                   h->p.prior_dynamic, SHIFT_REWRITTEN));
+    h->shift_pending = false;
+    return K3DOM_OK;
+}
+
+k3dom_status begin_frame_voxels(k3dom_handle* h, bool normal) {
+    if (normal) {
+        h->shift_pending = false;
+        return K3DOM_OK;
+    }
+    return settle_shift(h);
 }
 
 k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted[3]) {
@@ -454,11 +476,16 @@ k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted
     const int mz = static_cast<int>((h->sensor[2] - h->center[2] - h->p.sensor_off_z) / h->p.resolution);
     if (std::abs(mx) >= h->p.map_shift_thresh || std::abs(my) >= h->p.map_shift_thresh ||
         std::abs(mz) >= h->p.map_shift_thresh) {
+        k3dom_status st = settle_shift(h);
+        if (st != K3DOM_OK) return st;
         const int nxt = 1 - h->vcur;
         HIP_TRY(h, launch_shift_voxels(h->stream, h->vox[nxt], h->vox[h->vcur], h->gs, h->gsz, mx, my, mz,
-                                       h->p.prior_free, h->p.prior_static, h->p.prior_dynamic));
+                                       h->p.prior_free, h->p.prior_static, h->p.prior_dynamic, SHIFT_KEPT));
         HIP_TRY(h, launch_shift_particles(h->stream, h->P[0], h->N, mx, my, mz));
-        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        h->shift_pending = true;
+        h->pend_move[0] = mx;
+        h->pend_move[1] = my;
+        h->pend_move[2] = mz;
         h->vcur = nxt;
         h->center[0] += mx * h->p.resolution;
         h->center[1] += my * h->p.resolution;
@@ -793,6 +820,10 @@ k3dom_status k3dom_get_info(const k3dom_handle* h, k3dom_info* out) {
 
 k3dom_status k3dom_read_masses(k3dom_handle* h, float* f, float* s, float* d, float* mvx, float* mvy, float* mvz) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
+    {
+        k3dom_status st = settle_shift(h);
+        if (st != K3DOM_OK) return st;
+    }
     const VoxelArrays& v = h->vox[h->vcur];
     const size_t bytes = sizeof(float) * (size_t)h->V;
     const float* src[6] = {v.f, v.s, v.d, v.mvx, v.mvy, v.mvz};
@@ -805,6 +836,10 @@ k3dom_status k3dom_read_masses(k3dom_handle* h, float* f, float* s, float* d, fl
 
 k3dom_status k3dom_read_cells(k3dom_handle* h, k3dom_cell* out) {
     if (!h || !out) return K3DOM_ERR_INVALID_ARG;
+    {
+        k3dom_status st = settle_shift(h);
+        if (st != K3DOM_OK) return st;
+    }
     const VoxelArrays& v = h->vox[h->vcur];
     const size_t V = (size_t)h->V;
     std::vector<float> buf(V * 10);

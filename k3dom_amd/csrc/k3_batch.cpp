@@ -225,6 +225,8 @@ k3dom_status k3dom_batch_update(k3dom_batch* b, double t, const float* poses, co
         k3dom_status s = update_pose(h, poses[3 * i], poses[3 * i + 1], poses[3 * i + 2], moved);
         if (s == K3DOM_OK) s = ensure_points(h, std::max(n[i], 1));
         if (s != K3DOM_OK) return from_map(b, i, s);
+        // the batch frame runs on the batch's streams: let a map's shift finish first
+        if (moved[0] || moved[1] || moved[2]) B_HIP(b, hipStreamSynchronize(h->stream));
         const size_t cap = (size_t)h->m_cap;
         if (n[i] > 0) {
             std::memcpy(h->h_pts, xs[i], sizeof(float) * n[i]);
@@ -247,6 +249,8 @@ k3dom_status k3dom_batch_update(k3dom_batch* b, double t, const float* poses, co
     }
     for (int i = 0; i < nm; ++i) {
         k3dom_handle* h = b->maps[i];
+        k3dom_status s = begin_frame_voxels(h, true);
+        if (s != K3DOM_OK) return from_map(b, i, s);
         fill_frame_consts(h, h->h_fc, t, n[i]);
         b->h_args[i] = make_bg_args(h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free, h->ibuf, h->lbuf[2], h->rec,
                                     h->del_occ, h->del_free, nullptr, nullptr, h->sc);

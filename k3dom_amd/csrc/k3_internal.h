@@ -30,6 +30,11 @@ struct k3dom_handle {
     std::vector<void*> allocs;
     VoxelArrays vox[2]{};
     int vcur = 0;
+    // A map shift moves the SHIFT_KEPT fields at once; the SHIFT_REWRITTEN ones
+    // of vox[vcur] stay due from vox[1 - vcur] moved by pend_move until a full
+    // frame rewrites them or something reads them (settle_shift).
+    bool shift_pending = false;
+    int pend_move[3] = {0, 0, 0};
     float *born = nullptr, *del_occ = nullptr, *del_free = nullptr;
     ParticleArrays P[2]{};  // 0: current set, 1: sorted set
     ParticleArrays NB{};
@@ -135,5 +140,10 @@ void drop_graph(k3dom_handle* h);
 k3dom_status reset_map(k3dom_handle* h);
 // DOM_c::updatePose; *shifted receives the map shift in voxels (0 when none).
 k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted[3]);
+// Completes a deferred map shift (before reads of the voxel state and before
+// frames that do not rewrite every voxel).
+k3dom_status settle_shift(k3dom_handle* h);
+// Voxel state at the start of a frame: a full frame makes a deferred shift moot.
+k3dom_status begin_frame_voxels(k3dom_handle* h, bool normal);
 
 }  // namespace k3api

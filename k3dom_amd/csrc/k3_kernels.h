@@ -66,8 +66,14 @@ hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 // ------------------------------------------------------------------ particles / voxels
 hipError_t launch_predict(hipStream_t st, const FrameConsts* fc, ParticleArrays P, int n);
 hipError_t launch_shift_particles(hipStream_t st, ParticleArrays P, int n, int mx, int my, int mz);
+// Field sets of a map shift: SHIFT_KEPT are carried from frame to frame (the
+// masses and mu, which a frame rewrites only where particles are);
+// SHIFT_REWRITTEN are rewritten for every voxel by the next full frame before
+// anything reads them (ranges, persistent and predicted mass, unassociated
+// newborn weight, mean velocity), so their shift can be deferred.
+constexpr int SHIFT_KEPT = 1, SHIFT_REWRITTEN = 2, SHIFT_ALL = 3;
 hipError_t launch_shift_voxels(hipStream_t st, VoxelArrays dst, VoxelArrays src, int gs, int gsz, int mx, int my,
-                               int mz, float prior_f, float prior_s, float prior_d);
+                               int mz, float prior_f, float prior_s, float prior_d, int fields);
 hipError_t launch_fill_voxels(hipStream_t st, VoxelArrays v, int V, float prior_f, float prior_s, float prior_d);
 
 // Stable LSD radix sort of keys (values in [0, 2^key_bits)).  in_keys is not

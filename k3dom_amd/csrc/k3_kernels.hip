@@ -81,7 +81,7 @@ hipError_t launch_shift_particles(hipStream_t st, ParticleArrays P, int n, int m
 // the fields it sets (masses = priors, ranges = -1) and the cell's own old value
 // for the fields it leaves alone (src/common/dom.cu:132-151).
 __global__ __launch_bounds__(TPB) void k_shift_voxels(VoxelArrays dst, VoxelArrays src, int gs, int gsz, int mx,
-                                                      int my, int mz, float pf, float ps, float pd) {
+                                                      int my, int mz, float pf, float ps, float pd, int fields) {
     int V = gs * gs * gsz;
     int c = blockIdx.x * TPB + threadIdx.x;
     if (c >= V) return;
@@ -89,23 +89,27 @@ __global__ __launch_bounds__(TPB) void k_shift_voxels(VoxelArrays dst, VoxelArra
     int nx = x + mx, ny = y + my, nz = z + mz;
     bool in = nx >= 0 && nx < gs && ny >= 0 && ny < gs && nz >= 0 && nz < gsz;
     int sc = in ? nx + gs * ny + gs * gs * nz : c;
-    dst.f[c] = in ? src.f[sc] : pf;
-    dst.s[c] = in ? src.s[sc] : ps;
-    dst.d[c] = in ? src.d[sc] : pd;
-    dst.start[c] = in ? src.start[sc] : -1;
-    dst.end[c] = in ? src.end[sc] : -1;
-    dst.pers[c] = src.pers[sc];
-    dst.pred[c] = src.pred[sc];
-    dst.mu[c] = src.mu[sc];
-    dst.wua[c] = src.wua[sc];
-    dst.mvx[c] = src.mvx[sc];
-    dst.mvy[c] = src.mvy[sc];
-    dst.mvz[c] = src.mvz[sc];
+    if (fields & SHIFT_KEPT) {
+        dst.f[c] = in ? src.f[sc] : pf;
+        dst.s[c] = in ? src.s[sc] : ps;
+        dst.d[c] = in ? src.d[sc] : pd;
+        dst.mu[c] = src.mu[sc];
+    }
+    if (fields & SHIFT_REWRITTEN) {
+        dst.start[c] = in ? src.start[sc] : -1;
+        dst.end[c] = in ? src.end[sc] : -1;
+        dst.pers[c] = src.pers[sc];
+        dst.pred[c] = src.pred[sc];
+        dst.wua[c] = src.wua[sc];
+        dst.mvx[c] = src.mvx[sc];
+        dst.mvy[c] = src.mvy[sc];
+        dst.mvz[c] = src.mvz[sc];
+    }
 }
 hipError_t launch_shift_voxels(hipStream_t st, VoxelArrays dst, VoxelArrays src, int gs, int gsz, int mx, int my,
-                               int mz, float pf, float ps, float pd) {
+                               int mz, float pf, float ps, float pd, int fields) {
     hipLaunchKernelGGL(k_shift_voxels, dim3(blocks_for((long long)gs * gs * gsz)), dim3(TPB), 0, st, dst, src, gs,
-                       gsz, mx, my, mz, pf, ps, pd);
+                       gsz, mx, my, mz, pf, ps, pd, fields);
     return hipGetLastError();
 }
 
@@ -466,7 +470,7 @@ __device__ void block_max_store(float v, uint32_t* slot) {
 // loads; each lane then sums, from LDS, every unit that starts in its
 // positions and stores the sum at the unit's start position.  A voxel pass
 // adds each voxel's unit sums with coalesced access to the voxel arrays.
-constexpr int UNIT_LANE = 16;
+constexpr int UNIT_LANE = 4;
 constexpr int UNIT_WAVE = 64 * UNIT_LANE;
 constexpr int UNIT_SPAN = UNIT_WAVE + K3_SEG_BLOCK;  // staged positions per wave, a multiple of 64
 constexpr int UNIT_TPB = 256;

@@ -26,6 +26,41 @@ def test_small_scene_moving_ego_map_shift(gpu):
     assert gd.info().center_pos[0] > 0.0
 
 
+def test_cells_read_between_pose_and_grid_update(gpu):
+    """A map shift moves the carried fields at once and the rest only when
+    needed: cells read right after a shifting updatePose, and a second shift
+    before the next updateGrid, must still equal the serial restatement."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, small_workload
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_cells, compare_state
+    w = small_workload(ego_vx=4.0, frames=10, seed=3)
+    sc = Scene(w.scene)
+    gd, od = DOM(w.params), OracleDOM(w.params)
+    shifts = 0
+    for f in range(w.frames):
+        t, pose, xs, ys, zs = sc.frame(f)
+        c0 = gd.info().center_pos[0]
+        for m in (gd, od):
+            m.updatePose(*pose)
+        if gd.info().center_pos[0] != c0:
+            shifts += 1
+            compare_cells(gd.copyCellsToHost(), od.copyCellsToHost(), f"after the shift of frame {f}:")
+            if shifts == 2:
+                # a second shift with the first still pending
+                far = (pose[0] + 2.0, pose[1], pose[2])
+                for m in (gd, od):
+                    m.updatePose(*far)
+                compare_cells(gd.copyCellsToHost(), od.copyCellsToHost(), f"after a double shift in frame {f}:")
+                for m in (gd, od):
+                    m.updatePose(*pose)
+        normal = gd.info().updated_time > 0
+        for m in (gd, od):
+            m.updateGrid(t, xs, ys, zs)
+        compare_state(gd, od, normal_frame=normal, what=f"frame {f}:")
+    assert shifts >= 2
+
+
 def test_config1_bit_exact(gpu):
     from k3dom_amd.scene import workload
     w = workload(1)
