@@ -284,11 +284,11 @@ k3dom_status enqueue_head(k3dom_handle* h, bool normal) {
     HIP_TRY(h, launch_reset_scalars(h->stream, h->sc));
     if (normal) {
         mark(h, ST_PREDICT);
-        HIP_TRY(h, launch_predict(h->stream, h->fc, h->P[0], h->N));
+        HIP_TRY(h, launch_predict(h->stream, h->fc, h->P[0], h->prec, h->N));
         mark(h, ST_SORT);
         HIP_TRY(h, launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys,
                                      &h->last_perm));
-        HIP_TRY(h, launch_gather_sorted(h->stream, h->P[0], h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N));
+        HIP_TRY(h, launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N));
         HIP_TRY(h, launch_ranges(h->stream, h->P[1].key, h->N, v.start, v.end, h->V));
     }
     return K3DOM_OK;
@@ -690,6 +690,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
         CREATE_TRY(dalloc(h, f, B));
     CREATE_TRY(dalloc(h, &h->NB.key, B));
     CREATE_TRY(dalloc(h, &h->wsort, N));
+    CREATE_TRY(dalloc(h, &h->prec, 2 * N));
     CREATE_TRY(dalloc(h, &h->usum, 3 * N));
     CREATE_TRY(dalloc(h, &h->maxparts, (size_t)mass_update_parts(h->V) + voxel_particles_parts(h->N) +
                                            newborn_parts(h->B)));
@@ -701,7 +702,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(dalloc_zero(h, &h->sort.bsum, (size_t)scan_scratch_words(256ll * sort_blocks(h->N))));
     CREATE_TRY(dalloc(h, &h->cdf_v, V));
     CREATE_TRY(dalloc(h, &h->cdf_j, N + B));
-    CREATE_TRY(dalloc(h, &h->blocksums, (size_t)fixed_scan_blocks((int)std::max(V, N + B))));
+    CREATE_TRY(dalloc(h, &h->blocksums, (size_t)fixed_scan_scratch_words((long long)std::max(V, N + B))));
     CREATE_TRY(dalloc(h, &h->bnd, V));
     CREATE_TRY(dalloc(h, &h->ridx, N));
     CREATE_TRY(dalloc(h, &h->sc, 1));

@@ -474,15 +474,21 @@ struct BgTerm {
 struct BgTable {
     float a[K3_KTAB_N + 1], b[K3_KTAB_N + 1], c[K3_KTAB_N + 1];
 };
+// a2 may be slightly negative (a rounded difference of squares standing for
+// 0): the table argument is clamp(a2 * kscale, 0, N) in one med3, which for
+// every non-NaN a2 equals min(max(a2, 0) * kscale, N) of k3_kernel_index, and
+// the fraction is taken with fract, which equals s - (float)(int)s exactly
+// for 0 <= s < 2^24, so the result is that of k3_kernel_value_sq bit for bit.
 template <bool GRAD>
 __device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float sigma, float ls, float inv_ls,
                                                 const BgTable* stab) {
     BgTerm<GRAD> t;
-    float f;
-    const int i = k3_kernel_index(a2, kscale, &f);
+    const float s = __builtin_amdgcn_fmed3f(a2 * kscale, 0.0f, (float)K3_KTAB_N);
+    const int i = (int)s;
+    const float f = __builtin_amdgcn_fractf(s);
     t.v = k3_kernel_poly(stab->a[i], stab->b[i], stab->c[i], f, sigma);
     t.g = 0.0f;
-    if (GRAD) t.g = t.v > 0.0f ? k3_kernel_dls(sqrtf(a2), ls, inv_ls, sigma) : 0.0f;
+    if (GRAD) t.g = t.v > 0.0f ? k3_kernel_dls(sqrtf(fmaxf(0.0f, a2)), ls, inv_ls, sigma) : 0.0f;
     return t;
 }
 
@@ -569,12 +575,12 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 // every lane; two such candidates of one chunk are evaluated
                 // together (independent table reads), then added in order
                 const float dpa = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
-                const float a2a = fmaxf(0.0f, fmaf(-dpa, dpa, dc2));
+                const float a2a = fmaf(-dpa, dpa, dc2);  // clamped at 0 by bg_term
                 const uint32_t meta2 = q + 1 < cnt ? uniform(smeta[w][q + 1]) : 0u;
                 if (q + 1 < cnt && (meta2 & 7u) == C_MAIN && (meta2 >> (3 + K3_BGKI_CHUNK_LOG2)) == chunk) {
                     const float4 r0b = srec[w][(q + 1) * 3];
                     const float dpb = fmaf(vz, r0b.z, fmaf(vy, r0b.y, vx * r0b.x)) * r0b.w;
-                    const float a2b = fmaxf(0.0f, fmaf(-dpb, dpb, dc2));
+                    const float a2b = fmaf(-dpb, dpb, dc2);
                     const BgTerm<GRAD> ta = bg_term<GRAD>(a2a, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
                     const BgTerm<GRAD> tb = bg_term<GRAD>(a2b, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
                     cur_f += ta.v;
@@ -605,7 +611,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
             const uint32_t fc = cls & 3u;
             if (fc == C_MAIN) {  // 0 < dp < mg for every lane
                 const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
-                const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
+                const float a2 = fmaf(-dp, dp, dc2);  // clamped at 0 by bg_term
                 const bool cf = a2 < a.ls2g;
                 if (__any(cf)) {
                     const BgTerm<GRAD> t = bg_term<GRAD>(a2, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
@@ -622,7 +628,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 const bool c1 = dp <= 0.0f;
                 const bool c2 = !c1 && dp < r1.y;
                 const bool c3 = !c1 && !c2 && dp < r1.x;
-                const float a2 = fmaxf(0.0f, fmaf(-dp, dp, dc2));
+                const float a2 = fmaf(-dp, dp, dc2);  // clamped at 0 by bg_term
                 const float ex = vx - r1.z, ey = vy - r1.w, ez = vz - r2.x;
                 const float a3 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
                 const float aa = c2 ? a2 : a3;
@@ -797,13 +803,33 @@ __global__ __launch_bounds__(TPB) void k_bgki_combine(int ntiles, GridGeom g, Ti
     const int i = tx * 4 + (lane & 3), j = ty * 4 + ((lane >> 2) & 3), k = tz * 4 + (lane >> 4);
     if (!(i < g.gs && j < g.gs && k < g.gsz)) return;
     float ro = 0.0f, rf = 0.0f, go = 0.0f, gf = 0.0f;
-    for (uint32_t c = 0; c < cnt; ++c) {
-        const float* pp = partial + (size_t)(first + c) * bg_partial_stride<GRAD>();
-        ro += pp[lane];
-        rf += pp[64 + lane];
-        if (GRAD) {
-            go += pp[128 + lane];
-            gf += pp[192 + lane];
+    // chunk sums added in chunk order; loads of up to CB chunks issued together
+    constexpr uint32_t CB = 8;
+    for (uint32_t c0 = 0; c0 < cnt; c0 += CB) {
+        const uint32_t nc = min(CB, cnt - c0);
+        float vo[CB], vf[CB], wo[CB], wf[CB];
+#pragma unroll
+        for (uint32_t c = 0; c < CB; ++c) {
+            if (c < nc) {
+                const float* pp = partial + (size_t)(first + c0 + c) * bg_partial_stride<GRAD>();
+                vo[c] = pp[lane];
+                vf[c] = pp[64 + lane];
+                if (GRAD) {
+                    wo[c] = pp[128 + lane];
+                    wf[c] = pp[192 + lane];
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t c = 0; c < CB; ++c) {
+            if (c < nc) {
+                ro += vo[c];
+                rf += vf[c];
+                if (GRAD) {
+                    go += wo[c];
+                    gf += wf[c];
+                }
+            }
         }
     }
     const int cell = i + g.gs * j + g.gs * g.gs * k;

@@ -40,11 +40,12 @@ struct k3dom_handle {
     ParticleArrays NB{};
     SortScratch sort{};
     float* wsort = nullptr;  // N: sorted particle weights before the persistent update
+    float4* prec = nullptr;  // 2N: predicted particle state, one 32-byte record per particle
     float* usum = nullptr;   // 3N: per-unit sums of the particle-parallel voxel sums
     uint32_t* maxparts = nullptr;  // per-block maxima feeding the fixed-point CDF exponents
     const uint32_t* last_perm = nullptr;
     const uint32_t* last_skeys = nullptr;
-    uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;
+    uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;  // blocksums: fixed-point scan scratch
     uint32_t* bnd = nullptr;
     int32_t* ridx = nullptr;
     DevScalars* sc = nullptr;

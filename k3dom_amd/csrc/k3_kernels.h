@@ -64,7 +64,9 @@ hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, ui
                            const uint32_t* n_dev, uint32_t n_mul, uint32_t* total);
 
 // ------------------------------------------------------------------ particles / voxels
-hipError_t launch_predict(hipStream_t st, const FrameConsts* fc, ParticleArrays P, int n);
+// Predicted state of P to prec (2 float4 per particle: px, py, pz, vx | vy, vz,
+// w, 0) and the voxel keys to P.key.
+hipError_t launch_predict(hipStream_t st, const FrameConsts* fc, ParticleArrays P, float4* prec, int n);
 hipError_t launch_shift_particles(hipStream_t st, ParticleArrays P, int n, int mx, int my, int mz);
 // Field sets of a map shift: SHIFT_KEPT are carried from frame to frame (the
 // masses and mu, which a frame rewrites only where particles are);
@@ -86,8 +88,9 @@ struct SortScratch {
 int sort_blocks(int n);
 hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int key_bits, SortScratch s,
                              const uint32_t** out_keys, const uint32_t** out_idx);
-// Sorted particles to dst, except the weights, which go to wsorted.
-hipError_t launch_gather_sorted(hipStream_t st, ParticleArrays src, ParticleArrays dst, float* wsorted,
+// Sorted particles from the predicted records to dst, except the weights,
+// which go to wsorted.
+hipError_t launch_gather_sorted(hipStream_t st, const float4* prec, ParticleArrays dst, float* wsorted,
                                 const uint32_t* perm, const uint32_t* sorted_keys, int n);
 hipError_t launch_ranges(hipStream_t st, const int32_t* sorted_keys, int n, int32_t* start, int32_t* end, int V);
 
@@ -199,11 +202,12 @@ hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelA
 hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,
                                   ParticleArrays P, const float* wsrc, int n, float min_vel, float* usum3,
                                   uint32_t* joint_parts);
-// max_parts: nparts per-block maxima of the values scanned
+// Inclusive fixed-point CDF of [a ++ b].  max_parts: nparts per-block maxima
+// of the values; scratch: fixed_scan_scratch_words(na + nb) words.
+long long fixed_scan_scratch_words(long long n);
 hipError_t launch_fixed_scan(hipStream_t st, const float* a, int na, const float* b, int nb,
                              const uint32_t* max_parts, int nparts, int32_t* E_out, uint64_t* cdf,
-                             uint64_t* blocksums, uint64_t* total);
-int fixed_scan_blocks(int n);
+                             uint64_t* scratch, uint64_t* total);
 hipError_t launch_bounds(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, const uint64_t* total,
                          int V, uint32_t cnt, const float* born, uint32_t* bnd, float* wua);
 hipError_t launch_newborn(hipStream_t st, const uint32_t* abort_flag, const uint32_t* bnd, int V, int B, int gs,

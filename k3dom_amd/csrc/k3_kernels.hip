@@ -46,7 +46,11 @@ static inline unsigned blocks_for(long long n, int per_block = TPB) {
 }
 
 // ============================================================== particles
-__global__ __launch_bounds__(TPB) void k_predict(const FrameConsts* __restrict__ fc, ParticleArrays P, int n) {
+// The predicted state goes to one 32-byte record per particle
+// (px, py, pz, vx | vy, vz, w, 0), read only by the sort's gather, which then
+// fetches each particle's state in two 16-byte loads; the keys go to P.key.
+__global__ __launch_bounds__(TPB) void k_predict(const FrameConsts* __restrict__ fc, ParticleArrays P,
+                                                 float4* __restrict__ prec, int n) {
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
     const K3PredictConst c = fc->pc;
@@ -55,13 +59,12 @@ __global__ __launch_bounds__(TPB) void k_predict(const FrameConsts* __restrict__
     float w = P.w[i];
     int key;
     k3_predict_particle((uint32_t)i, c, &px, &py, &pz, &vx, &vy, &vz, &w, &key);
-    P.px[i] = px; P.py[i] = py; P.pz[i] = pz;
-    P.vx[i] = vx; P.vy[i] = vy; P.vz[i] = vz;
-    P.w[i] = w;
+    prec[2 * i] = make_float4(px, py, pz, vx);
+    prec[2 * i + 1] = make_float4(vy, vz, w, 0.0f);
     P.key[i] = key;
 }
-hipError_t launch_predict(hipStream_t st, const FrameConsts* fc, ParticleArrays P, int n) {
-    hipLaunchKernelGGL(k_predict, dim3(blocks_for(n)), dim3(TPB), 0, st, fc, P, n);
+hipError_t launch_predict(hipStream_t st, const FrameConsts* fc, ParticleArrays P, float4* prec, int n) {
+    hipLaunchKernelGGL(k_predict, dim3(blocks_for(n)), dim3(TPB), 0, st, fc, P, prec, n);
     return hipGetLastError();
 }
 
@@ -400,21 +403,22 @@ hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int 
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(TPB) void k_gather(ParticleArrays src, ParticleArrays dst,
+__global__ __launch_bounds__(TPB) void k_gather(const float4* __restrict__ prec, ParticleArrays dst,
                                                 float* __restrict__ wsorted,
                                                 const uint32_t* __restrict__ perm,
                                                 const uint32_t* __restrict__ skeys, int n) {
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
-    uint32_t j = perm[i];
-    dst.px[i] = src.px[j]; dst.py[i] = src.py[j]; dst.pz[i] = src.pz[j];
-    dst.vx[i] = src.vx[j]; dst.vy[i] = src.vy[j]; dst.vz[i] = src.vz[j];
-    wsorted[i] = src.w[j];
+    const uint32_t j = perm[i];
+    const float4 a = prec[2 * j], b = prec[2 * j + 1];
+    dst.px[i] = a.x; dst.py[i] = a.y; dst.pz[i] = a.z;
+    dst.vx[i] = a.w; dst.vy[i] = b.x; dst.vz[i] = b.y;
+    wsorted[i] = b.z;
     dst.key[i] = (int32_t)skeys[i];
 }
-hipError_t launch_gather_sorted(hipStream_t st, ParticleArrays src, ParticleArrays dst, float* wsorted,
+hipError_t launch_gather_sorted(hipStream_t st, const float4* prec, ParticleArrays dst, float* wsorted,
                                 const uint32_t* perm, const uint32_t* sorted_keys, int n) {
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(TPB), 0, st, src, dst, wsorted, perm, sorted_keys, n);
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(TPB), 0, st, prec, dst, wsorted, perm, sorted_keys, n);
     return hipGetLastError();
 }
 
@@ -788,10 +792,6 @@ hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, Vo
 }
 
 // ============================================================== fixed-point CDF
-constexpr int FS_ITEMS = 8;
-constexpr int FS_TILE = TPB * FS_ITEMS;
-int fixed_scan_blocks(int n) { return n <= 0 ? 1 : (n + FS_TILE - 1) / FS_TILE; }
-
 __device__ __forceinline__ float cat_load(const float* a, int na, const float* b, int nb, int i) {
     return i < na ? a[i] : (i < na + nb ? b[i - na] : 0.0f);
 }
@@ -828,19 +828,36 @@ __device__ uint64_t block_scan_incl_u64(uint64_t v, uint64_t* lds4, uint64_t* to
     return v + add;
 }
 
+// Inclusive u64 scan of the fixed-point values of [a ++ b] in three phases:
+// tile sums (u64 sums are exact, so consecutive lanes can take consecutive
+// elements), a one-block scan of the tile sums, then per tile a scan with the
+// tile staged in LDS (padded by one word in 16 against bank conflicts), each
+// thread scanning 16 consecutive elements, and stored with consecutive lanes
+// on consecutive elements.
+constexpr int FL_ITEMS = 16;
+constexpr int FL_TILE = TPB * FL_ITEMS;
+__device__ __forceinline__ int fl_pad(int i) { return i + (i >> 4); }
+
+static inline int fl_tiles(long long n) { return (int)std::max(1ll, (n + FL_TILE - 1) / FL_TILE); }
+long long fixed_scan_scratch_words(long long n) { return fl_tiles(n); }
+
 __global__ __launch_bounds__(TPB) void k_fixed_reduce(const float* __restrict__ a, int na,
                                                       const float* __restrict__ b, int nb,
                                                       const int32_t* __restrict__ Ep,
-                                                      uint64_t* __restrict__ blocksums) {
+                                                      uint64_t* __restrict__ tile_sums) {
     __shared__ uint64_t lds4[4];
     const int E = *Ep;
-    const int base = blockIdx.x * FS_TILE + threadIdx.x * FS_ITEMS;
+    const int n = na + nb;
+    const int base = blockIdx.x * FL_TILE + threadIdx.x;
     uint64_t s = 0;
 #pragma unroll
-    for (int k = 0; k < FS_ITEMS; ++k) s += k3_to_fixed(cat_load(a, na, b, nb, base + k), E);
+    for (int k = 0; k < FL_ITEMS; ++k) {
+        const int e = base + k * TPB;
+        if (e < n) s += k3_to_fixed(cat_load(a, na, b, nb, e), E);
+    }
     uint64_t tot;
     block_scan_incl_u64(s, lds4, &tot);
-    if (threadIdx.x == 0) blocksums[blockIdx.x] = tot;
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
 }
 
 __global__ __launch_bounds__(1024) void k_scan_top_u64(uint64_t* __restrict__ sums, int n,
@@ -871,38 +888,51 @@ __global__ __launch_bounds__(1024) void k_scan_top_u64(uint64_t* __restrict__ su
 __global__ __launch_bounds__(TPB) void k_fixed_down(const float* __restrict__ a, int na,
                                                     const float* __restrict__ b, int nb,
                                                     const int32_t* __restrict__ Ep,
-                                                    const uint64_t* __restrict__ blocksums,
+                                                    const uint64_t* __restrict__ tile_excl,
                                                     uint64_t* __restrict__ cdf) {
+    __shared__ uint64_t tile_v[FL_TILE + FL_TILE / 16];
     __shared__ uint64_t lds4[4];
     const int E = *Ep;
     const int n = na + nb;
-    const int base = blockIdx.x * FS_TILE + threadIdx.x * FS_ITEMS;
-    uint64_t q[FS_ITEMS];
+    const int t = threadIdx.x;
+    const int base = blockIdx.x * FL_TILE;
+#pragma unroll
+    for (int k = 0; k < FL_ITEMS; ++k) {
+        const int e = base + k * TPB + t;
+        tile_v[fl_pad(k * TPB + t)] = e < n ? k3_to_fixed(cat_load(a, na, b, nb, e), E) : 0ull;
+    }
+    __syncthreads();
+    uint64_t q[FL_ITEMS];
     uint64_t s = 0;
 #pragma unroll
-    for (int k = 0; k < FS_ITEMS; ++k) {
-        q[k] = k3_to_fixed(cat_load(a, na, b, nb, base + k), E);
+    for (int k = 0; k < FL_ITEMS; ++k) {
+        q[k] = tile_v[fl_pad(t * FL_ITEMS + k)];
         s += q[k];
     }
     uint64_t tot;
-    uint64_t incl = block_scan_incl_u64(s, lds4, &tot);
-    uint64_t run = blocksums[blockIdx.x] + incl - s;
+    uint64_t run = tile_excl[blockIdx.x] + block_scan_incl_u64(s, lds4, &tot) - s;
 #pragma unroll
-    for (int k = 0; k < FS_ITEMS; ++k) {
+    for (int k = 0; k < FL_ITEMS; ++k) {
         run += q[k];
-        if (base + k < n) cdf[base + k] = run;
+        tile_v[fl_pad(t * FL_ITEMS + k)] = run;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < FL_ITEMS; ++k) {
+        const int e = base + k * TPB + t;
+        if (e < n) cdf[e] = tile_v[fl_pad(k * TPB + t)];
     }
 }
 
 hipError_t launch_fixed_scan(hipStream_t st, const float* a, int na, const float* b, int nb,
                              const uint32_t* max_parts, int nparts, int32_t* E_out, uint64_t* cdf,
-                             uint64_t* blocksums, uint64_t* total) {
+                             uint64_t* scratch, uint64_t* total) {
     const int n = na + nb;
-    const int nblk = fixed_scan_blocks(n);
+    const int nblk = fl_tiles(n);
     hipLaunchKernelGGL(k_fixed_exponent, dim3(1), dim3(1024), 0, st, max_parts, nparts, n, E_out);
-    hipLaunchKernelGGL(k_fixed_reduce, dim3(nblk), dim3(TPB), 0, st, a, na, b, nb, E_out, blocksums);
-    hipLaunchKernelGGL(k_scan_top_u64, dim3(1), dim3(1024), 0, st, blocksums, nblk, total);
-    hipLaunchKernelGGL(k_fixed_down, dim3(nblk), dim3(TPB), 0, st, a, na, b, nb, E_out, blocksums, cdf);
+    hipLaunchKernelGGL(k_fixed_reduce, dim3(nblk), dim3(TPB), 0, st, a, na, b, nb, E_out, scratch);
+    hipLaunchKernelGGL(k_scan_top_u64, dim3(1), dim3(1024), 0, st, scratch, nblk, total);
+    hipLaunchKernelGGL(k_fixed_down, dim3(nblk), dim3(TPB), 0, st, a, na, b, nb, E_out, scratch, cdf);
     return hipGetLastError();
 }
 
