@@ -179,6 +179,10 @@ GridGeom grid_geom(const k3dom_handle* h) {
 }
 
 k3dom_status build_lists(k3dom_handle* h, const FrameConsts* fc, const float* xs, const float* ys, const float* zs) {
+    if (h->pts_pending) {
+        HIP_TRY(h, hipStreamWaitEvent(h->stream, h->pts_ready, 0));
+        h->pts_pending = false;
+    }
     HIP_TRY(h, launch_point_prep(h->stream, fc, xs, ys, zs, h->m_cap, h->p.resolution, h->rec, h->pxyz));
     for (int l = 0; l < 3; ++l)
         HIP_TRY(h, launch_build_level(h->stream, l, fc, h->tg, h->ldesc[l], h->lbuf[l], l ? &h->lbuf[l - 1] : nullptr,
@@ -195,6 +199,10 @@ void fill_frame_consts(k3dom_handle* h, FrameConsts* c, double t, int m) {
     c->frame = h->frame;
     c->m = m;
     c->pad = 0;
+    // the pending particle shift goes to this frame's predict (an
+    // initialisation frame draws new particles, so it is dropped there)
+    for (int k = 0; k < 3; ++k) c->pshift[k] = h->pshift_pending ? (float)h->pshift[k] : 0.0f;
+    h->pshift_pending = false;
     K3PredictConst& pc = c->pc;
     pc.dt = dt;
     pc.p_S = h->p.persistence_prob;
@@ -371,6 +379,8 @@ void finish_frame(k3dom_handle* h, double t, int m) {
     h->frame++;
 }
 
+bool frame_uses_graph(const k3dom_handle* h) { return h->updated_time > 0 && h->use_graph && !h->timing; }
+
 k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
     const bool normal = h->updated_time > 0;
     {
@@ -379,7 +389,7 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
     }
     if (h->timing) for (float& x : h->stage_ms) x = 0.0f;
     fill_frame_consts(h, h->h_fc, t, m);
-    const bool use_graph = normal && h->use_graph && !h->timing;
+    const bool use_graph = frame_uses_graph(h);
     bool done = false;
     if (use_graph) {
         const size_t cap = (size_t)h->m_cap;
@@ -435,6 +445,7 @@ k3dom_status reset_map(k3dom_handle* h) {
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     h->vcur = 0;
     h->shift_pending = false;
+    h->pshift_pending = false;
     h->frame = 0;
     h->first_pose = false;
     h->updated_time = -1.0;
@@ -448,6 +459,13 @@ k3dom_status settle_shift(k3dom_handle* h) {
                                    h->pend_move[1], h->pend_move[2], h->p.prior_free, h->p.prior_static,
                                    h->p.prior_dynamic, SHIFT_REWRITTEN));
     h->shift_pending = false;
+    return K3DOM_OK;
+}
+
+k3dom_status settle_particles(k3dom_handle* h) {
+    if (!h->pshift_pending) return K3DOM_OK;
+    HIP_TRY(h, launch_shift_particles(h->stream, h->P[0], h->N, h->pshift[0], h->pshift[1], h->pshift[2]));
+    h->pshift_pending = false;
     return K3DOM_OK;
 }
 
@@ -481,7 +499,12 @@ k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted
         const int nxt = 1 - h->vcur;
         HIP_TRY(h, launch_shift_voxels(h->stream, h->vox[nxt], h->vox[h->vcur], h->gs, h->gsz, mx, my, mz,
                                        h->p.prior_free, h->p.prior_static, h->p.prior_dynamic, SHIFT_KEPT));
-        HIP_TRY(h, launch_shift_particles(h->stream, h->P[0], h->N, mx, my, mz));
+        st = settle_particles(h);
+        if (st != K3DOM_OK) return st;
+        h->pshift_pending = true;
+        h->pshift[0] = mx;
+        h->pshift[1] = my;
+        h->pshift[2] = mz;
         h->shift_pending = true;
         h->pend_move[0] = mx;
         h->pend_move[1] = my;
@@ -670,6 +693,8 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(hipGetDeviceProperties(&prop, h->device));
     h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     CREATE_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    CREATE_TRY(hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
+    CREATE_TRY(hipEventCreateWithFlags(&h->pts_ready, hipEventDisableTiming));
     const size_t V = (size_t)h->V, N = (size_t)h->N, B = (size_t)std::max(h->B, 1);
     for (int b = 0; b < 2; ++b) {
         VoxelArrays& v = h->vox[b];
@@ -766,6 +791,8 @@ k3dom_status k3dom_destroy(k3dom_handle* h) {
     for (int e = 0; e < EV_COUNT; ++e)
         if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->cstream) (void)hipStreamDestroy(h->cstream);
+    if (h->pts_ready) (void)hipEventDestroy(h->pts_ready);
     delete h;
     return K3DOM_OK;
 }
@@ -783,14 +810,28 @@ k3dom_status k3dom_update_grid(k3dom_handle* h, double t, const float* xs, const
     if (n < 0 || (n > 0 && (!xs || !ys || !zs))) return fail(h, K3DOM_ERR_INVALID_ARG, "bad point arrays");
     k3dom_status s = ensure_points(h, std::max(n, 1));
     if (s != K3DOM_OK) return s;
-    const size_t cap = (size_t)h->m_cap;
-    if (n > 0) {
-        std::memcpy(h->h_pts, xs, sizeof(float) * n);
-        std::memcpy(h->h_pts + cap, ys, sizeof(float) * n);
-        std::memcpy(h->h_pts + 2 * cap, zs, sizeof(float) * n);
-        HIP_TRY(h, hipMemcpyAsync(h->d_pts, h->h_pts, sizeof(float) * 3 * cap, hipMemcpyHostToDevice, h->stream));
+    if (frame_uses_graph(h)) {
+        // the graph reads the points at fixed offsets of the handle's buffer
+        const size_t cap = (size_t)h->m_cap;
+        if (n > 0) {
+            std::memcpy(h->h_pts, xs, sizeof(float) * n);
+            std::memcpy(h->h_pts + cap, ys, sizeof(float) * n);
+            std::memcpy(h->h_pts + 2 * cap, zs, sizeof(float) * n);
+            HIP_TRY(h, hipMemcpyAsync(h->d_pts, h->h_pts, sizeof(float) * 3 * cap, hipMemcpyHostToDevice, h->stream));
+        }
+        return frame(h, t, h->d_pts, h->d_pts + cap, h->d_pts + 2 * cap, n);
     }
-    return frame(h, t, h->d_pts, h->d_pts + cap, h->d_pts + 2 * cap, n);
+    // eager frame: the n points packed, uploaded beside predict and sort
+    const size_t m = (size_t)n;
+    if (n > 0) {
+        std::memcpy(h->h_pts, xs, sizeof(float) * m);
+        std::memcpy(h->h_pts + m, ys, sizeof(float) * m);
+        std::memcpy(h->h_pts + 2 * m, zs, sizeof(float) * m);
+        HIP_TRY(h, hipMemcpyAsync(h->d_pts, h->h_pts, sizeof(float) * 3 * m, hipMemcpyHostToDevice, h->cstream));
+        HIP_TRY(h, hipEventRecord(h->pts_ready, h->cstream));
+        h->pts_pending = true;
+    }
+    return frame(h, t, h->d_pts, h->d_pts + m, h->d_pts + 2 * m, n);
 }
 
 k3dom_status k3dom_update_grid_device(k3dom_handle* h, double t, const float* d_xs, const float* d_ys,
@@ -876,6 +917,10 @@ k3dom_status k3dom_read_cells(k3dom_handle* h, k3dom_cell* out) {
 
 k3dom_status k3dom_read_array(k3dom_handle* h, int32_t which, void* dst, int64_t capacity) {
     if (!h || !dst) return K3DOM_ERR_INVALID_ARG;
+    {
+        k3dom_status st = settle_particles(h);
+        if (st != K3DOM_OK) return st;
+    }
     const size_t N = (size_t)h->N, B = (size_t)h->B, V = (size_t)h->V;
     auto cp = [&](const void* src, size_t bytes, size_t at) -> k3dom_status {
         if ((int64_t)(at + bytes) > capacity) return fail(h, K3DOM_ERR_INVALID_ARG, "capacity too small");

@@ -26,6 +26,11 @@ struct k3dom_handle {
     int ntx = 0, nty = 0, ntz = 0, ntiles = 0;
     int device = 0, num_cus = 256;
     hipStream_t stream = nullptr;
+    // point uploads of eager frames run on cstream and are waited for (event
+    // pts_ready) just before the points are first read, after predict and sort
+    hipStream_t cstream = nullptr;
+    hipEvent_t pts_ready = nullptr;
+    bool pts_pending = false;
 
     std::vector<void*> allocs;
     VoxelArrays vox[2]{};
@@ -35,6 +40,10 @@ struct k3dom_handle {
     // frame rewrites them or something reads them (settle_shift).
     bool shift_pending = false;
     int pend_move[3] = {0, 0, 0};
+    // The particle part of a map shift waits for the next frame's predict
+    // (FrameConsts::pshift), or for a read of the particles (settle_particles).
+    bool pshift_pending = false;
+    int pshift[3] = {0, 0, 0};
     float *born = nullptr, *del_occ = nullptr, *del_free = nullptr;
     ParticleArrays P[2]{};  // 0: current set, 1: sorted set
     ParticleArrays NB{};
@@ -144,7 +153,11 @@ k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted
 // Completes a deferred map shift (before reads of the voxel state and before
 // frames that do not rewrite every voxel).
 k3dom_status settle_shift(k3dom_handle* h);
+// Applies a pending particle shift (before the particle arrays are read).
+k3dom_status settle_particles(k3dom_handle* h);
 // Voxel state at the start of a frame: a full frame makes a deferred shift moot.
 k3dom_status begin_frame_voxels(k3dom_handle* h, bool normal);
+// Whether the next frame runs as a captured graph.
+bool frame_uses_graph(const k3dom_handle* h);
 
 }  // namespace k3api

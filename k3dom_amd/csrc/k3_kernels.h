@@ -52,6 +52,7 @@ struct FrameConsts {
     uint32_t frame;      // frame counter: counter word of every random draw
     int32_t m;           // points of the frame
     int32_t pad;
+    float pshift[3];     // map shift (voxels) still due on the particle positions, applied by predict
 };
 
 // ------------------------------------------------------------------ scans

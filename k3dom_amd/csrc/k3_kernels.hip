@@ -54,7 +54,8 @@ __global__ __launch_bounds__(TPB) void k_predict(const FrameConsts* __restrict__
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
     const K3PredictConst c = fc->pc;
-    float px = P.px[i], py = P.py[i], pz = P.pz[i];
+    // a pending map shift first, as k_shift_particles would have applied it
+    float px = P.px[i] - fc->pshift[0], py = P.py[i] - fc->pshift[1], pz = P.pz[i] - fc->pshift[2];
     float vx = P.vx[i], vy = P.vy[i], vz = P.vz[i];
     float w = P.w[i];
     int key;

@@ -33,7 +33,7 @@ def test_cells_read_between_pose_and_grid_update(gpu):
     from k3dom_amd.dom import DOM
     from k3dom_amd.scene import Scene, small_workload
     from oracle.orc import OracleDOM
-    from tests._parity import compare_cells, compare_state
+    from tests._parity import bits_equal, compare_cells, compare_state, first_diff
     w = small_workload(ego_vx=4.0, frames=10, seed=3)
     sc = Scene(w.scene)
     gd, od = DOM(w.params), OracleDOM(w.params)
@@ -46,6 +46,9 @@ def test_cells_read_between_pose_and_grid_update(gpu):
         if gd.info().center_pos[0] != c0:
             shifts += 1
             compare_cells(gd.copyCellsToHost(), od.copyCellsToHost(), f"after the shift of frame {f}:")
+            if shifts % 2:
+                a, b = gd.read_array("PARTICLE_POS"), od.read_array("PARTICLE_POS")
+                assert bits_equal(a, b), f"particles after the shift of frame {f}: {first_diff(a, b)}"
             if shifts == 2:
                 # a second shift with the first still pending
                 far = (pose[0] + 2.0, pose[1], pose[2])
