@@ -36,6 +36,7 @@ FP32_VECTOR_PEAK_TFLOPS = 157.3
 FLOPS_OCC_PAIR = 25
 FLOPS_FREE_PAIR = 28
 HBM_PEAK_GBPS = 8000.0
+BREAKDOWN = 10  # frames of the per-stage breakdown, after the timed ones
 # Compulsory HBM traffic of one frame in this structure-of-arrays layout (each
 # stage reads its inputs once and writes its outputs once; the sort counted as
 # a key histogram read plus one key+payload read and write; u64 CDFs):
@@ -198,8 +199,10 @@ def main():
 
     w = workload(args.config)
     total = args.warmup + args.steps
+    # after the timed frames, BREAKDOWN more frames with every stage timed
+    # (the timed frames carry only the two events around the gather kernel)
     sc = Scene(w.scene)
-    frames = [sc.frame(f) for f in range(total)]
+    frames = [sc.frame(f) for f in range(total + BREAKDOWN)]
     npts = [len(f[2]) for f in frames]
     log(f"workload {w.name}: {total} frames, points/frame {min(npts)}..{max(npts)}")
 
@@ -222,18 +225,25 @@ def main():
 
     for f in range(args.warmup):
         step(f)
-    d.set_timing(True)
-    stage_sum = np.zeros(len(DOM.STAGES))
+    d.set_timing(2)  # HIP events around the gather kernel only
+    kernel_idx = DOM.STAGES.index("bgki_kernel")
+    gather_ms = 0.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for f in range(args.warmup, total):
         step(f)
-        stage_sum += d.stage_times()
+        gather_ms += float(d.stage_times()[kernel_idx])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    d.set_timing(False)
     ms_per_step = elapsed / args.steps * 1e3
-    stage_avg = stage_sum / args.steps
+    # stage breakdown on the following frames, every stage timed
+    d.set_timing(True)
+    stage_sum = np.zeros(len(DOM.STAGES))
+    for f in range(total, total + BREAKDOWN):
+        step(f)
+        stage_sum += d.stage_times()
+    d.set_timing(False)
+    stage_avg = stage_sum / BREAKDOWN
     log("stage ms/frame: " + " ".join(f"{n}={v:.3f}" for n, v in zip(DOM.STAGES, stage_avg)))
 
     roof = None
@@ -253,7 +263,7 @@ def main():
                 free += fr
         d2.close()
         flops = (occ * FLOPS_OCC_PAIR + free * FLOPS_FREE_PAIR) / args.steps
-        kernel_s = stage_avg[DOM.STAGES.index("bgki_kernel")] * 1e-3
+        kernel_s = gather_ms / args.steps * 1e-3
         achieved = flops / kernel_s / 1e12
         roof = {"bound": "valu", "kernel": "k_bgki", "achieved": round(achieved, 3),
                 "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -318,6 +328,7 @@ def main():
                    "lidar": f"{w.scene.beams}x{w.scene.az_bins}", "points_per_frame_mean": int(np.mean(npts)),
                    "ego_speed_mps": float(w.scene.ego_vx), "parallelism": "single GPU"},
         "stage_ms": {n: round(float(v), 4) for n, v in zip(DOM.STAGES, stage_avg)},
+        "stage_ms_frames": f"{BREAKDOWN} frames after the timed ones, every stage timed with HIP events",
         "roofline": roof,
         "frame_hbm": frame_hbm,
         "adjoint": adj,

@@ -184,7 +184,10 @@ k3dom_status k3dom_read_array(k3dom_handle* h, int32_t which, void* dst, int64_t
  * [2] point preparation and candidate lists, [3] kernel inference (BGKI),
  * [4] mass update, [5] persistent update, [6] newborn, [7] moments,
  * [8] resampling, [9] whole frame, [10] the kernel-inference gather kernel
- * alone.  Timing is recorded only after k3dom_set_timing(h, 1). */
+ * alone.  Timing is recorded only after k3dom_set_timing(h, 1); with
+ * k3dom_set_timing(h, 2) only the two events around the gather kernel are
+ * recorded (entry [10]; the others read 0), which leaves the frame's launches
+ * otherwise undisturbed.  0 turns timing off. */
 k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable);
 k3dom_status k3dom_stage_times(k3dom_handle* h, float* ms, int32_t count);
 

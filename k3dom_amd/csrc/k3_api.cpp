@@ -163,8 +163,11 @@ k3dom_status sync_scalars(k3dom_handle* h) {
     return K3DOM_OK;
 }
 
+// Timing level 1 records every stage event, level 2 only the two around the
+// kernel-inference gather.
 void mark(k3dom_handle* h, int stage) {
-    if (h->timing) (void)hipEventRecord(h->ev[stage], h->stream);
+    if (h->timing == 1 || (h->timing == 2 && (stage == ST_BGKI_KERNEL || stage == EV_BGKI_END)))
+        (void)hipEventRecord(h->ev[stage], h->stream);
 }
 
 GridGeom grid_geom(const k3dom_handle* h) {
@@ -302,20 +305,27 @@ k3dom_status enqueue_head(k3dom_handle* h, bool normal) {
     return K3DOM_OK;
 }
 
+static void drop_graph_of(k3dom_handle* h, int p) {
+    if (h->gexec[p]) (void)hipGraphExecDestroy(h->gexec[p]);
+    if (h->graph[p]) (void)hipGraphDestroy(h->graph[p]);
+    h->gexec[p] = nullptr;
+    h->graph[p] = nullptr;
+    h->graph_caps[p] = -1;
+}
+
 void drop_graph(k3dom_handle* h) {
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-    if (h->graph) (void)hipGraphDestroy(h->graph);
-    h->gexec = nullptr;
-    h->graph = nullptr;
+    drop_graph_of(h, 0);
+    drop_graph_of(h, 1);
 }
 
 // Captures the whole normal frame (constants upload to scalar readback) as a
-// graph over the handle's own point buffer.
+// graph over the handle's own point buffer and the current voxel buffer.
 k3dom_status capture_graph(k3dom_handle* h) {
-    drop_graph(h);
+    const int p = h->vcur;
+    drop_graph_of(h, p);
     const size_t cap = (size_t)h->m_cap;
-    const bool timing = h->timing;
-    h->timing = false;  // no event records inside the graph
+    const int timing = h->timing;
+    h->timing = 0;  // no event records inside the graph
     hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
     k3dom_status s = K3DOM_OK;
     if (e == hipSuccess) {
@@ -323,18 +333,17 @@ k3dom_status capture_graph(k3dom_handle* h) {
         if (s == K3DOM_OK) s = enqueue_rest(h, true, h->d_pts, h->d_pts + cap, h->d_pts + 2 * cap);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(h->stream, &g);
-        h->graph = g;
+        h->graph[p] = g;
         if (e2 != hipSuccess && e == hipSuccess) e = e2;
     }
     h->timing = timing;
     if (s != K3DOM_OK) {
-        drop_graph(h);
+        drop_graph_of(h, p);
         return s;
     }
     HIP_TRY(h, e);
-    HIP_TRY(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
-    h->graph_caps = h->caps_version;
-    h->graph_vcur = h->vcur;
+    HIP_TRY(h, hipGraphInstantiate(&h->gexec[p], h->graph[p], nullptr, nullptr, 0));
+    h->graph_caps[p] = h->caps_version;
     return K3DOM_OK;
 }
 
@@ -401,11 +410,12 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
                                               h->stream));
             }
         }
-        if (!h->gexec || h->graph_caps != h->caps_version || h->graph_vcur != h->vcur) {
+        const int p = h->vcur;
+        if (!h->gexec[p] || h->graph_caps[p] != h->caps_version) {
             k3dom_status s = capture_graph(h);
             if (s != K3DOM_OK) return s;
         }
-        HIP_TRY(h, hipGraphLaunch(h->gexec, h->stream));
+        HIP_TRY(h, hipGraphLaunch(h->gexec[p], h->stream));
         HIP_TRY(h, hipStreamSynchronize(h->stream));
         done = !h->h_sc->overflow;
         dx = h->d_pts;
@@ -417,7 +427,7 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
     }
     k3dom_status s = resolve_overflow(h, normal, dx, dy, dz, done, use_graph);
     if (s != K3DOM_OK) return s;
-    if (h->timing) {
+    if (h->timing == 1) {
         const int order[] = {ST_PREDICT, ST_SORT, ST_LISTS, ST_BGKI, ST_MASS, ST_PERSIST, ST_NEWBORN, ST_MOMENTS,
                              ST_RESAMPLE, ST_FRAME};
         for (int k = 0; k + 1 < 10; ++k) {
@@ -428,6 +438,9 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
         }
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, h->ev[EV_FRAME_START], h->ev[ST_FRAME]) == hipSuccess) h->stage_ms[ST_FRAME] = ms;
+    }
+    if (h->timing) {
+        float ms = 0.0f;
         if (hipEventElapsedTime(&ms, h->ev[ST_BGKI_KERNEL], h->ev[EV_BGKI_END]) == hipSuccess)
             h->stage_ms[ST_BGKI_KERNEL] = ms;
     }
@@ -964,8 +977,8 @@ k3dom_status k3dom_read_array(k3dom_handle* h, int32_t which, void* dst, int64_t
 }
 
 k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable) {
-    if (!h) return K3DOM_ERR_INVALID_ARG;
-    h->timing = enable != 0;
+    if (!h || enable < 0 || enable > 2) return K3DOM_ERR_INVALID_ARG;
+    h->timing = enable;
     return K3DOM_OK;
 }
 

@@ -472,7 +472,8 @@ struct BgTerm {
 // reads per lookup, lanes in different banks unless their indices differ by a
 // multiple of the bank count.
 struct BgTable {
-    float a[K3_KTAB_N + 1], b[K3_KTAB_N + 1], c[K3_KTAB_N + 1];
+    float2 ab[K3_KTAB_N + 1];
+    float c[K3_KTAB_N + 1];
 };
 // a2 may be slightly negative (a rounded difference of squares standing for
 // 0): the table argument is clamp(a2 * kscale, 0, N) in one med3, which for
@@ -486,7 +487,8 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float si
     const float s = __builtin_amdgcn_fmed3f(a2 * kscale, 0.0f, (float)K3_KTAB_N);
     const int i = (int)s;
     const float f = __builtin_amdgcn_fractf(s);
-    t.v = k3_kernel_poly(stab->a[i], stab->b[i], stab->c[i], f, sigma);
+    const float2 ab = stab->ab[i];
+    t.v = k3_kernel_poly(ab.x, ab.y, stab->c[i], f, sigma);
     t.g = 0.0f;
     if (GRAD) t.g = t.v > 0.0f ? k3_kernel_dls(sqrtf(fmaxf(0.0f, a2)), ls, inv_ls, sigma) : 0.0f;
     return t;
@@ -495,8 +497,7 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float si
 // Copies the kernel table into LDS (every thread of the block takes part).
 __device__ __forceinline__ void bg_load_table(BgTable* stab) {
     for (int i = threadIdx.x; i <= K3_KTAB_N; i += GA_TPB) {
-        stab->a[i] = k3_ktab[4 * i];
-        stab->b[i] = k3_ktab[4 * i + 1];
+        stab->ab[i] = make_float2(k3_ktab[4 * i], k3_ktab[4 * i + 1]);
         stab->c[i] = k3_ktab[4 * i + 2];
     }
     __syncthreads();

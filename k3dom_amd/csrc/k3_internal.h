@@ -83,13 +83,15 @@ struct k3dom_handle {
     FrameConsts* fc = nullptr;
     FrameConsts* h_fc = nullptr;
     int gather_blocks = 0;     // blocks of the kernel-inference gather (autotuned)
-    // captured normal-frame graph; re-captured when buffers move or the voxel buffer flips
+    // captured normal-frame graphs, one per voxel buffer (vcur), each
+    // re-captured when buffers move
     bool use_graph = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int caps_version = 0, graph_caps = -1, graph_vcur = -1;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int caps_version = 0;
+    int graph_caps[2] = {-1, -1};
 
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 every stage, 2 the kernel-inference gather only
     hipEvent_t ev[EV_COUNT]{};
     k3dom_stats stats{};
     GridGeom last_geom{};

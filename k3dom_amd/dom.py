@@ -165,7 +165,9 @@ class DOM:
         return out
 
     def set_timing(self, on=True):
-        check(self._lib.k3dom_set_timing(self._h, 1 if on else 0), self._h, "set_timing")
+        """True (or 1): time every stage; 2: time only the gather kernel; False: off."""
+        level = 0 if not on else (2 if on == 2 else 1)
+        check(self._lib.k3dom_set_timing(self._h, level), self._h, "set_timing")
 
     def autotune(self):
         """Times the gather over the last frame's items for several grid sizes and
