@@ -115,13 +115,15 @@ k3dom_status ensure_points(k3dom_handle* h, int m) {
 k3dom_status alloc_level(k3dom_handle* h, int l, long long list_cap) {
     LevelBufs& B = h->lbuf[l];
     const LevelDesc& L = h->ldesc[l];
-    for (void* p : {(void*)B.counts, (void*)B.slot_off, (void*)B.list, (void*)B.bsum}) dfree(h, p);
-    B.counts = B.slot_off = B.list = B.bsum = nullptr;
+    for (void* p : {(void*)B.counts, (void*)B.slot_off, (void*)B.list, (void*)B.bsum, (void*)B.mask}) dfree(h, p);
+    B.counts = B.slot_off = B.list = B.bsum = B.mask = nullptr;
     const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : h->lbuf[l - 1].list_cap;
     B.items_cap = L.nparents + (parent_cap + L.seg - 1) / L.seg;
     B.slot_cap = B.items_cap * L.nch;
     B.list_cap = list_cap;
+    B.mask_stride = parent_cap;
     h->caps_version++;
+    HIP_TRY(h, dalloc(h, &B.mask, (size_t)parent_cap * (size_t)level_child_groups(L)));
     HIP_TRY(h, dalloc(h, &B.counts, (size_t)B.slot_cap));
     HIP_TRY(h, dalloc(h, &B.slot_off, (size_t)B.slot_cap + 1));
     HIP_TRY(h, dalloc(h, &B.list, (size_t)std::max(1ll, list_cap)));
@@ -148,7 +150,8 @@ k3dom_status ensure_caps(k3dom_handle* h, const long long want[3], long long wan
     for (int l = 0; l < 3; ++l) {
         const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : h->lbuf[l - 1].list_cap;
         const long long items_need = h->ldesc[l].nparents + (parent_cap + h->ldesc[l].seg - 1) / h->ldesc[l].seg;
-        if (want[l] > h->lbuf[l].list_cap || items_need > h->lbuf[l].items_cap || !h->lbuf[l].list) {
+        if (want[l] > h->lbuf[l].list_cap || items_need > h->lbuf[l].items_cap || parent_cap > h->lbuf[l].mask_stride ||
+            !h->lbuf[l].list) {
             k3dom_status s = alloc_level(h, l, std::max(want[l], h->lbuf[l].list_cap));
             if (s != K3DOM_OK) return s;
         }

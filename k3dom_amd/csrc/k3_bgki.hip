@@ -178,6 +178,22 @@ __device__ __forceinline__ ItemCtx item_ctx(int level, const LevelDesc& L, uint3
 // b / ncg and children [LV_CG * (b % ncg), ...) of it.
 constexpr int LV_CG = 32;
 __host__ __device__ inline int child_groups(const LevelDesc& L) { return (L.nch + LV_CG - 1) / LV_CG; }
+int level_child_groups(const LevelDesc& L) { return child_groups(L); }
+
+// Candidate e of an item's segment: its point index and packed position.
+struct CandLoad {
+    uint32_t idx;
+    float4 p;
+    bool in;
+};
+__device__ __forceinline__ CandLoad load_cand(int level, const uint32_t* __restrict__ parent_list, const ItemCtx& x,
+                                              const float4* __restrict__ pxyz, uint32_t e) {
+    CandLoad r;
+    r.in = e < x.end;
+    r.idx = r.in ? (level ? parent_list[x.poff + e] : e) : 0u;
+    r.p = pxyz[r.idx];
+    return r;
+}
 
 // Per-item, per-child counts of the candidates of the item's segment that
 // pass the child's test.  Lane c of each wavefront keeps child c's count of
@@ -188,7 +204,8 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
                                                      const uint32_t* __restrict__ parent_len,
                                                      const uint32_t* __restrict__ parent_list,
                                                      const uint32_t* __restrict__ item_base,
-                                                     uint32_t* __restrict__ counts, const DevScalars* sc) {
+                                                     uint32_t* __restrict__ counts, uint32_t* __restrict__ masks,
+                                                     long long mask_stride, const DevScalars* sc) {
     __shared__ float4 cb[LV_CG];
     __shared__ uint32_t wcnt[TPB / 64][LV_CG];
     if (sc->overflow) return;
@@ -208,15 +225,21 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
     }
     __syncthreads();
     uint32_t mine = 0;  // lane c: candidates of this wavefront passing child c
+    uint32_t* gmask = masks + (size_t)(blockIdx.x % ncg) * (size_t)mask_stride + (level ? x.poff : 0u);
+    // the next chunk's candidate is loaded while this one is tested
+    CandLoad nx = load_cand(level, parent_list, x, pxyz, x.begin + t);
     for (uint32_t base = x.begin; base < x.end; base += TPB) {
-        const uint32_t e = base + t;
-        const bool in = e < x.end;
-        const uint32_t idx = in ? (level ? parent_list[x.poff + e] : e) : 0u;
-        const Cand cd = make_cand(pxyz[idx], in);
+        const CandLoad cur = nx;
+        if (base + TPB < x.end) nx = load_cand(level, parent_list, x, pxyz, base + TPB + t);
+        const Cand cd = make_cand(cur.p, cur.in);
+        uint32_t hits = 0;  // bit c: passes child c (kept for the fill)
         for (int c = 0; c < nc; ++c) {
-            const uint32_t n = (uint32_t)__popcll(__ballot(cand_hits(cb[c], cd)));
+            const bool hit = cand_hits(cb[c], cd);
+            hits |= (uint32_t)hit << c;
+            const uint32_t n = (uint32_t)__popcll(__ballot(hit));
             mine += lane == c ? n : 0u;
         }
+        if (cur.in) gmask[base + t] = hits;
     }
     if (lane < nc) wcnt[w][lane] = mine;
     __syncthreads();
@@ -239,9 +262,9 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
                                                     const uint32_t* __restrict__ parent_list,
                                                     const uint32_t* __restrict__ item_base,
                                                     const uint32_t* __restrict__ slot_off,
+                                                    const uint32_t* __restrict__ masks, long long mask_stride,
                                                     uint32_t* __restrict__ list, long long list_cap,
                                                     DevScalars* sc) {
-    __shared__ float4 cb[LV_CG];
     __shared__ uint32_t ob[LV_CG];
     __shared__ uint32_t run[2][LV_CG];
     __shared__ uint64_t wm[2][TPB / 64][LV_CG];
@@ -265,23 +288,21 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
     if (t < nc) {
-        int t0[3], t1[3];
-        const int box = child_box(level, L, tg, x.P, c0 + t, t0, t1);
-        cb[t] = box < 0 ? make_float4(0.0f, 0.0f, 0.0f, -1.0f)
-                        : box_sphere(g, ls, t0[0], t0[1], t0[2], t1[0], t1[1], t1[2]);
         ob[t] = slot_off[x.ib * L.nch + (c0 + t) * x.nsegP + x.seg];
         run[0][t] = 0;
     }
     __syncthreads();
+    // the children each candidate passes, as the count kernel found them
+    const uint32_t* gmask = masks + (size_t)(blockIdx.x % ncg) * (size_t)mask_stride + (level ? x.poff : 0u);
     int buf = 0;
     for (uint32_t base = x.begin; base < x.end; base += TPB, buf ^= 1) {
         const uint32_t e = base + t;
         const bool in = e < x.end;
         const uint32_t idx = in ? (level ? parent_list[x.poff + e] : e) : 0u;
-        const Cand cd = make_cand(pxyz[idx], in);
+        const uint32_t hits = in ? gmask[e] : 0u;
         uint64_t mymask = 0;  // lane c: this wavefront's mask for child c
         for (int c = 0; c < nc; ++c) {
-            const uint64_t bal = __ballot(cand_hits(cb[c], cd));
+            const uint64_t bal = __ballot((hits >> c) & 1u);
             mymask = lane == c ? bal : mymask;
         }
         if (lane < nc) wm[buf][w][lane] = mymask;
@@ -344,11 +365,11 @@ hipError_t launch_build_level(hipStream_t st, int level, const FrameConsts* fc, 
     (void)launch_scan_u32(st, B.nseg, B.item_base, B.bsum, L.nparents, nullptr, 1, &sc->level_items[level]);
     const unsigned grid = (unsigned)std::max(1ll, B.items_cap * child_groups(L));
     hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(TPB), 0, st, level, fc, tg, L, ls, pxyz, poff, plen, plist,
-                       B.item_base, B.counts, sc);
+                       B.item_base, B.counts, B.mask, B.mask_stride, sc);
     (void)launch_scan_u32(st, B.counts, B.slot_off, B.bsum, B.slot_cap, B.item_base + L.nparents, (uint32_t)L.nch,
                     nullptr);
     hipLaunchKernelGGL(k_level_fill, dim3(grid), dim3(TPB), 0, st, level, fc, tg, L, ls, pxyz, poff, plen, plist,
-                       B.item_base, B.slot_off, B.list, B.list_cap, sc);
+                       B.item_base, B.slot_off, B.mask, B.mask_stride, B.list, B.list_cap, sc);
     hipLaunchKernelGGL(k_level_boxes, dim3(blocks_for((long long)L.nbx * L.nby * L.nbz)), dim3(TPB), 0, st, level, L,
                        B.item_base, B.slot_off, B.box_off, B.box_len, sc);
     return hipGetLastError();
