@@ -119,8 +119,22 @@ k3dom_status k3dom_params_from_yaml(const char* method_yaml, const char* dataset
 
 /* Replaces dom::DOM::DOM / DOM_c::DOM_c + initialize (src/k3dom/dom.cu:20-23,
  * src/common/dom.cu:29-66,90-109): allocates the map on the current HIP device
- * and sets every voxel to the priors. */
+ * and sets every voxel to the priors.  The map runs the K3DOM method. */
 k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out);
+
+/* The reference's two filters share DOM_c and its particle stages and differ
+ * in the mass update: K3DOM's kernel inference (src/k3dom/dom.cu) or the
+ * DS-PHD/MIB measurement grid (src/dsphdmib/dom.cu:20-118). */
+#define K3DOM_METHOD_K3DOM 0
+#define K3DOM_METHOD_DSPHDMIB 1
+
+/* Replaces the constructor of either filter (dom::DOM of src/k3dom/dom.cu:20-23
+ * or of src/dsphdmib/dom.cu:20-30): as k3dom_create, running `method`.  A
+ * DS-PHD/MIB map uses resolution, max_range and meas_occ_stddev for its
+ * measurement grid and ignores sigma, ls, gamma and mass_scale; it keeps
+ * its occupied mass in GridCell::occ_mass, and static/dynamic masses stay at
+ * their priors.  Batches and differentiated windows take K3DOM maps only. */
+k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3dom_handle** out);
 
 /* Replaces DOM_c::~DOM_c (src/common/dom.cu:68-88).  NULL is accepted. */
 k3dom_status k3dom_destroy(k3dom_handle* h);

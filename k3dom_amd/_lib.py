@@ -65,6 +65,7 @@ SIGNATURES = {
     "k3dom_params_default": (C.c_int, [C.POINTER(Params)]),
     "k3dom_params_from_yaml": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(Params), C.c_char_p, C.c_int32]),
     "k3dom_create": (C.c_int, [C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "k3dom_create_method": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
     "k3dom_destroy": (C.c_int, [C.c_void_p]),
     "k3dom_update_pose": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
     "k3dom_update_grid": (C.c_int, [C.c_void_p, C.c_double, _fp, _fp, _fp, C.c_int32]),

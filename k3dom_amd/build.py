@@ -4,6 +4,7 @@ libk3dom.so    the product: gfx950 kernels + C ABI (hipcc --offload-arch=gfx950)
 libk3scene.so  the synthetic box-scene / LiDAR generator used by the benchmark
                and the tests (host C++)
 k3dom_replay   frame-replay driver over the C++ drop-in header include/k3dom/dom.h
+k3dom_replay_dsphdmib  the same over include/dsphdmib/dom.h
 _k3window      libtorch autograd function over the differentiated window (links libk3dom.so)
 """
 import os
@@ -49,12 +50,15 @@ def build_scene():
 
 
 def build_replay():
-    out = os.path.join(HERE, "k3dom_replay")
-    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "k3_replay.cpp"),
-           "-L", HERE, "-lk3dom", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
-    _run(cmd)
-    os.replace(out + ".tmp", out)
-    return out
+    outs = []
+    for name, defs in (("k3dom_replay", []), ("k3dom_replay_dsphdmib", ["-DK3_REPLAY_DSPHDMIB"])):
+        out = os.path.join(HERE, name)
+        cmd = ["g++", "-O2", "-std=c++17", *defs, "-I", os.path.join(ROOT, "include"),
+               os.path.join(CSRC, "k3_replay.cpp"), "-L", HERE, "-lk3dom", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+        _run(cmd)
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_torch_ext():

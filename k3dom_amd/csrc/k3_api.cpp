@@ -229,8 +229,10 @@ k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
     VoxelArrays& v = h->vox[h->vcur];
     const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
     const uint32_t* ab = &h->sc->overflow;
-    HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, grid_geom(h), h->tg, h->ibuf, h->del_occ, h->del_free,
-                                   h->sc));
+    const bool ds = h->method == K3DOM_METHOD_DSPHDMIB;
+    if (!ds)
+        HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, grid_geom(h), h->tg, h->ibuf, h->del_occ, h->del_free,
+                                       h->sc));
     mark(h, ST_MASS);
     // per-block maxima: born masses first, then the joint particle weights
     uint32_t* born_parts = h->maxparts;
@@ -238,8 +240,14 @@ k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
     const int n_born_parts = mass_update_parts(h->V);
     if (normal) {
         const int n_persist_parts = voxel_particles_parts(h->N);
-        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->wsort, h->P[1].key, h->N, h->usum, h->del_occ,
-                                      h->del_free, h->fc, h->p.mass_scale, h->p.birth_prob, h->born, born_parts));
+        if (ds)
+            HIP_TRY(h, launch_ds_mass_update(h->stream, ab, v, h->V, h->wsort, h->P[1].key, h->N, h->usum,
+                                             h->del_occ, h->del_free, h->p.birth_prob, h->born, h->ds_norm,
+                                             born_parts));
+        else
+            HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->wsort, h->P[1].key, h->N, h->usum, h->del_occ,
+                                          h->del_free, h->fc, h->p.mass_scale, h->p.birth_prob, h->born,
+                                          born_parts));
         mark(h, ST_PERSIST);
         HIP_TRY(h, launch_voxel_particles(h->stream, ab, v, h->V, h->P[1], h->wsort, h->N, h->p.particle_min_vel,
                                           h->usum, joint_parts));
@@ -258,8 +266,12 @@ k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
         HIP_TRY(h, launch_resample(h->stream, ab, h->cdf_j, h->N + h->B, &h->sc->total_joint, &h->sc->E_joint,
                                    h->P[1], h->NB, h->P[0], h->N, h->fc, k0, k1, h->ridx));
     } else {
-        HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, nullptr, nullptr, 0, h->usum, h->del_occ,
-                                      h->del_free, nullptr, h->p.mass_scale, h->p.birth_prob, h->born, born_parts));
+        if (ds)
+            HIP_TRY(h, launch_ds_born_from_meas(h->stream, h->del_occ, h->born, h->V, born_parts));
+        else
+            HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, nullptr, nullptr, 0, h->usum, h->del_occ,
+                                          h->del_free, nullptr, h->p.mass_scale, h->p.birth_prob, h->born,
+                                          born_parts));
         mark(h, ST_PERSIST);
         mark(h, ST_NEWBORN);
         HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, born_parts, n_born_parts, &h->sc->E_born,
@@ -276,8 +288,22 @@ k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
     return K3DOM_OK;
 }
 
-// Kernel inference (lists, gather) and everything after it.
+// Kernel inference (lists, gather), or the DS-PHD/MIB measurement grid, and
+// everything after it.
 k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const float* dy, const float* dz) {
+    if (h->method == K3DOM_METHOD_DSPHDMIB) {
+        mark(h, ST_LISTS);
+        if (h->pts_pending) {
+            HIP_TRY(h, hipStreamWaitEvent(h->stream, h->pts_ready, 0));
+            h->pts_pending = false;
+        }
+        mark(h, ST_BGKI);
+        mark(h, ST_BGKI_KERNEL);
+        HIP_TRY(h, launch_ds_meas_grid(h->stream, h->fc, dx, dy, dz, h->p.resolution, h->p.meas_occ_stddev,
+                                       h->p.max_range, h->V, h->del_occ, h->del_free));
+        mark(h, EV_BGKI_END);
+        return enqueue_after_gather(h, normal);
+    }
     mark(h, ST_LISTS);
     k3dom_status s = build_lists(h, h->fc, dx, dy, dz);
     if (s != K3DOM_OK) return s;
@@ -456,7 +482,7 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
 // frame (the random counters restart with the frame count).
 k3dom_status reset_map(k3dom_handle* h) {
     for (int b = 0; b < 2; ++b)
-        HIP_TRY(h, launch_fill_voxels(h->stream, h->vox[b], h->V, h->p.prior_free, h->p.prior_static,
+        HIP_TRY(h, launch_fill_voxels(h->stream, h->vox[b], h->V, h->p.prior_free, prior_s_slot(h),
                                       h->p.prior_dynamic));
     HIP_TRY(h, hipStreamSynchronize(h->stream));
     h->vcur = 0;
@@ -472,7 +498,7 @@ k3dom_status reset_map(k3dom_handle* h) {
 k3dom_status settle_shift(k3dom_handle* h) {
     if (!h->shift_pending) return K3DOM_OK;
     HIP_TRY(h, launch_shift_voxels(h->stream, h->vox[h->vcur], h->vox[1 - h->vcur], h->gs, h->gsz, h->pend_move[0],
-                                   h->pend_move[1], h->pend_move[2], h->p.prior_free, h->p.prior_static,
+                                   h->pend_move[1], h->pend_move[2], h->p.prior_free, prior_s_slot(h),
                                    h->p.prior_dynamic, SHIFT_REWRITTEN));
     h->shift_pending = false;
     return K3DOM_OK;
@@ -514,7 +540,7 @@ k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted
         if (st != K3DOM_OK) return st;
         const int nxt = 1 - h->vcur;
         HIP_TRY(h, launch_shift_voxels(h->stream, h->vox[nxt], h->vox[h->vcur], h->gs, h->gsz, mx, my, mz,
-                                       h->p.prior_free, h->p.prior_static, h->p.prior_dynamic, SHIFT_KEPT));
+                                       h->p.prior_free, prior_s_slot(h), h->p.prior_dynamic, SHIFT_KEPT));
         st = settle_particles(h);
         if (st != K3DOM_OK) return st;
         h->pshift_pending = true;
@@ -668,13 +694,20 @@ k3dom_status k3dom_params_from_yaml(const char* method_yaml, const char* dataset
 }
 
 k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
+    return k3dom_create_method(params, K3DOM_METHOD_K3DOM, out);
+}
+
+k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3dom_handle** out) {
     if (!out) return fail(nullptr, K3DOM_ERR_INVALID_ARG, "out is NULL");
     *out = nullptr;
+    if (method != K3DOM_METHOD_K3DOM && method != K3DOM_METHOD_DSPHDMIB)
+        return fail(nullptr, K3DOM_ERR_INVALID_ARG, "unknown method");
     std::string why;
     k3dom_status vs = validate(params, &why);
     if (vs != K3DOM_OK) return fail(nullptr, vs, why);
     k3dom_handle* h = new (std::nothrow) k3dom_handle();
     if (!h) return fail(nullptr, K3DOM_ERR_OUT_OF_MEMORY, "host allocation failed");
+    h->method = method;
     auto bail = [&](k3dom_status s) {
         g_create_error = h->err;
         k3dom_destroy(h);
@@ -720,6 +753,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
         CREATE_TRY(dalloc(h, &v.end, V));
     }
     CREATE_TRY(dalloc(h, &h->born, V));
+    if (h->method == K3DOM_METHOD_DSPHDMIB) CREATE_TRY(dalloc(h, &h->ds_norm, V));
     CREATE_TRY(dalloc(h, &h->del_occ, V));
     CREATE_TRY(dalloc(h, &h->del_free, V));
     for (int s = 0; s < 2; ++s) {
@@ -777,7 +811,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out) {
     CREATE_TRY(dalloc(h, &h->pair_counts, 2));
     // prior fill and zeroed particle state (src/common/kernel/init.cu:20-35)
     for (int b = 0; b < 2; ++b)
-        CREATE_TRY(launch_fill_voxels(h->stream, h->vox[b], h->V, h->p.prior_free, h->p.prior_static,
+        CREATE_TRY(launch_fill_voxels(h->stream, h->vox[b], h->V, h->p.prior_free, prior_s_slot(h),
                                       h->p.prior_dynamic));
     for (int s = 0; s < 2; ++s) {
         ParticleArrays& P = h->P[s];
@@ -922,9 +956,16 @@ k3dom_status k3dom_read_cells(k3dom_handle* h, k3dom_cell* out) {
         o.likelihood = 1.0f;
         o.p_A = 0.0f;
         o.free_mass = buf[0 * V + c];
-        o.static_mass = buf[1 * V + c];
-        o.dynamic_mass = buf[2 * V + c];
-        o.occ_mass = h->p.prior_occ;
+        if (h->method == K3DOM_METHOD_DSPHDMIB) {
+            // the second mass slot holds the occupied mass; static and dynamic keep their priors
+            o.static_mass = h->p.prior_static;
+            o.dynamic_mass = h->p.prior_dynamic;
+            o.occ_mass = buf[1 * V + c];
+        } else {
+            o.static_mass = buf[1 * V + c];
+            o.dynamic_mass = buf[2 * V + c];
+            o.occ_mass = h->p.prior_occ;
+        }
         o.pers_mass = buf[3 * V + c];
         o.pred_mass = buf[4 * V + c];
     }
@@ -987,6 +1028,7 @@ k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable) {
 
 k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
+    if (h->method != K3DOM_METHOD_K3DOM) return fail(h, K3DOM_ERR_STATE, "autotune tunes the K3DOM gather");
     if (h->frame < 2) return fail(h, K3DOM_ERR_STATE, "autotune needs a normal frame first");
     hipEvent_t e0, e1;
     HIP_TRY(h, hipEventCreate(&e0));
@@ -1042,6 +1084,7 @@ k3dom_status k3dom_get_stats(const k3dom_handle* h, k3dom_stats* out) {
 
 k3dom_status k3dom_count_bgki_pairs(k3dom_handle* h, uint64_t* out2) {
     if (!h || !out2) return K3DOM_ERR_INVALID_ARG;
+    if (h->method != K3DOM_METHOD_K3DOM) return fail(h, K3DOM_ERR_STATE, "pairs are counted for K3DOM maps");
     if (h->frame == 0) return fail(h, K3DOM_ERR_STATE, "no frame processed yet");
     HIP_TRY(h, hipMemsetAsync(h->pair_counts, 0, 2 * sizeof(unsigned long long), h->stream));
     const uint32_t n_items = h->h_sc->n_light + h->h_sc->n_heavy;

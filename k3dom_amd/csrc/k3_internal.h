@@ -35,6 +35,7 @@ struct k3dom_handle {
     std::vector<void*> allocs;
     VoxelArrays vox[2]{};
     int vcur = 0;
+    int method = K3DOM_METHOD_K3DOM;  // DS-PHD/MIB maps keep the occupied mass in the s slot
     // A map shift moves the SHIFT_KEPT fields at once; the SHIFT_REWRITTEN ones
     // of vox[vcur] stay due from vox[1 - vcur] moved by pend_move until a full
     // frame rewrites them or something reads them (settle_shift).
@@ -52,6 +53,7 @@ struct k3dom_handle {
     float4* prec = nullptr;  // 2N: predicted particle state, one 32-byte record per particle
     float* usum = nullptr;   // 3N: per-unit sums of the particle-parallel voxel sums
     uint32_t* maxparts = nullptr;  // per-block maxima feeding the fixed-point CDF exponents
+    float* ds_norm = nullptr;      // V (DS-PHD/MIB): sums above 1 of the voxels' weights, else 0
     const uint32_t* last_perm = nullptr;
     const uint32_t* last_skeys = nullptr;
     uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;  // blocksums: fixed-point scan scratch
@@ -155,6 +157,10 @@ k3dom_status update_pose(k3dom_handle* h, float x, float y, float z, int shifted
 // Completes a deferred map shift (before reads of the voxel state and before
 // frames that do not rewrite every voxel).
 k3dom_status settle_shift(k3dom_handle* h);
+// Prior of the second mass slot: static mass (K3DOM) or occupied mass (DS-PHD/MIB).
+inline float prior_s_slot(const k3dom_handle* h) {
+    return h->method == K3DOM_METHOD_DSPHDMIB ? h->p.prior_occ : h->p.prior_static;
+}
 // Applies a pending particle shift (before the particle arrays are read).
 k3dom_status settle_particles(k3dom_handle* h);
 // Voxel state at the start of a frame: a full frame makes a deferred shift moot.

@@ -203,6 +203,19 @@ hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelA
                               const float* weights, const int32_t* keys, int n, float* usum, const float* del_occ,
                               const float* del_free, const FrameConsts* fc, float mass_scale, float p_B, float* born,
                               uint32_t* born_parts);
+// DS-PHD/MIB method: measurement grid of the frame's points into
+// meas_occ / meas_free; cell update (born masses into born, block maxima into
+// born_parts as launch_mass_update) that also divides the sorted weights of
+// voxels whose sum exceeds 1 (norm: V floats of scratch); born masses of the
+// first frame from the measurement grid.
+hipError_t launch_ds_meas_grid(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
+                               const float* zs, float resolution, float meas_occ_stddev, float max_range, int V,
+                               float* meas_occ, float* meas_free);
+hipError_t launch_ds_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V, float* weights,
+                                 const int32_t* keys, int n, float* usum, const float* meas_occ,
+                                 const float* meas_free, float p_B, float* born, float* norm,
+                                 uint32_t* born_parts);
+hipError_t launch_ds_born_from_meas(hipStream_t st, const float* meas_occ, float* born, int V, uint32_t* born_parts);
 // P: the n sorted particles, whose weights P.w are written (penalised) from
 // wsrc; usum3: 3n floats of scratch
 hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V,

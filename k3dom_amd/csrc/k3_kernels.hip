@@ -649,6 +649,160 @@ hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelA
     return hipGetLastError();
 }
 
+// ============================================================== DS-PHD/MIB method
+// Measurement grid (src/dsphdmib/kernel/measurement_grid.cu:64-160): every
+// voxel scans the points in index order and keeps the first
+// K3DS_MAX_NEIGHBORS whose half-line from the sensor passes within the window
+// of its centre.  Points are staged in LDS 256 at a time (with |m|^2) and read
+// as broadcasts; a block stops when all its voxels have their rays.  The
+// masses go to meas_occ / meas_free (the evidence arrays of the K3DOM path).
+constexpr int DS_TPB = 256;
+__global__ __launch_bounds__(DS_TPB) void k_ds_meas_grid(const FrameConsts* __restrict__ fc,
+                                                         const float* __restrict__ xs, const float* __restrict__ ys,
+                                                         const float* __restrict__ zs, float w2, float delta,
+                                                         float max_range, int V, float* __restrict__ meas_occ,
+                                                         float* __restrict__ meas_free) {
+    __shared__ float px[DS_TPB], py[DS_TPB], pz[DS_TPB], pm2[DS_TPB];
+    const GridGeom g = fc->g;
+    const int n = fc->m;
+    const int t = threadIdx.x;
+    const int c = blockIdx.x * DS_TPB + t;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (c < V) k3_voxel_center(c, g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, &x, &y, &z);
+    const float c2 = x * x + y * y + z * z;
+    int nb = 0;
+    bool all_farther = true;
+    float max_occ = 0.0f;
+    for (int base = 0; base < n; base += DS_TPB) {
+        if (__syncthreads_and(c >= V || nb >= K3DS_MAX_NEIGHBORS)) break;
+        const int q = base + t;
+        if (q < n) {
+            const float a = xs[q], b = ys[q], d = zs[q];
+            px[t] = a;
+            py[t] = b;
+            pz[t] = d;
+            pm2[t] = a * a + b * b + d * d;
+        }
+        __syncthreads();
+        const int cnt = min(DS_TPB, n - base);
+        if (c < V && nb < K3DS_MAX_NEIGHBORS) {
+            for (int k = 0; k < cnt; ++k) {
+                const float mx = px[k], my = py[k], mz = pz[k], m2 = pm2[k];
+                if (!k3ds_ray_near(x, y, z, c2, mx, my, mz, m2, w2)) continue;
+                if (all_farther && c2 > m2) all_farther = false;
+                const float o = k3ds_pocc(x, y, z, mx, my, mz, delta);
+                if (o > max_occ) max_occ = o;
+                if (++nb == K3DS_MAX_NEIGHBORS) break;
+            }
+        }
+    }
+    if (c < V) k3ds_meas_masses(nb, all_farther, max_occ, x, y, z, max_range, &meas_occ[c], &meas_free[c]);
+}
+
+hipError_t launch_ds_meas_grid(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
+                               const float* zs, float resolution, float meas_occ_stddev, float max_range, int V,
+                               float* meas_occ, float* meas_free) {
+    const float wr = resolution * 0.866f;  // window_radius, src/dsphdmib/dom.cu:21-22
+    hipLaunchKernelGGL(k_ds_meas_grid, dim3(blocks_for(V, DS_TPB)), dim3(DS_TPB), 0, st, fc, xs, ys, zs, wr * wr,
+                       meas_occ_stddev, max_range, V, meas_occ, meas_free);
+    return hipGetLastError();
+}
+
+// Cell update (src/dsphdmib/kernel/mass_update.cu:44-85) of every voxel, the
+// predicted occupied mass being the voxel's particle weights (unit sums as in
+// k_mass_update).  A voxel above 1 keeps its sum in norm (0 otherwise) for
+// k_ds_normalize and continues with 1.  The second mass slot (v.s) holds the
+// occupied mass.
+__global__ __launch_bounds__(TPB) void k_ds_mass_update(const uint32_t* __restrict__ abort_flag, VoxelArrays v,
+                                                        int V, const float* __restrict__ usum,
+                                                        const float* __restrict__ meas_occ,
+                                                        const float* __restrict__ meas_free, float p_B,
+                                                        float* __restrict__ born, float* __restrict__ norm,
+                                                        uint32_t* __restrict__ born_parts) {
+    if (*abort_flag) return;
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * TPB + threadIdx.x;
+    int st = -1, nb = 0;
+    if (c < V) {
+        st = v.start[c];
+        if (st != -1) nb = unit_count(st, v.end[c]);
+    }
+    const bool wide = nb > WAVE_CHAIN_UNITS;
+    float sum = (st != -1 && !wide) ? chain_sum(usum, st, nb) : 0.0f;
+    for (uint64_t hm = __ballot(wide); hm; hm &= hm - 1) {
+        const int h = __builtin_ctzll(hm);
+        const float s = wave_chain_sum(usum, __builtin_amdgcn_readlane(st, h), __builtin_amdgcn_readlane(nb, h));
+        if (lane == h) sum = s;
+    }
+    float rho = 0.0f;
+    if (c < V) {
+        float occ_up, free_up, m_occ_pred = 0.0f;
+        if (st != -1) {
+            m_occ_pred = sum;
+            norm[c] = sum > 1.0f ? sum : 0.0f;
+            if (sum > 1.0f) m_occ_pred = 1.0f;
+            const float m_free_pred = k3ds_predict_free(v.f[c], m_occ_pred);
+            k3ds_update_masses(m_occ_pred, m_free_pred, meas_occ[c], meas_free[c], &occ_up, &free_up);
+            rho = k3ds_newborn_part(m_occ_pred, occ_up, p_B);
+        } else {
+            const float m_occ = v.s[c];
+            const float m_free = k3ds_predict_free(v.f[c], m_occ);
+            k3ds_update_masses(m_occ, m_free, meas_occ[c], meas_free[c], &occ_up, &free_up);
+        }
+        born[c] = rho;
+        v.pers[c] = occ_up - rho;
+        v.f[c] = free_up;
+        v.s[c] = occ_up;
+        v.pred[c] = m_occ_pred;
+    }
+    block_max_store(rho, born_parts + blockIdx.x);
+}
+
+// Division of the weights of voxels whose sum exceeded 1 (normalize_weights,
+// src/dsphdmib/kernel/mass_update.cu:37-42), on the sorted weights that the
+// persistent update reads.
+__global__ __launch_bounds__(TPB) void k_ds_normalize(const uint32_t* __restrict__ abort_flag,
+                                                      const int32_t* __restrict__ keys,
+                                                      const float* __restrict__ norm, float* __restrict__ w, int n) {
+    if (*abort_flag) return;
+    const int j = blockIdx.x * TPB + threadIdx.x;
+    if (j >= n) return;
+    const float s = norm[keys[j]];
+    if (s > 0.0f) w[j] = w[j] / s;
+}
+
+hipError_t launch_ds_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V, float* weights,
+                                 const int32_t* keys, int n, float* usum, const float* meas_occ,
+                                 const float* meas_free, float p_B, float* born, float* norm,
+                                 uint32_t* born_parts) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_unit_weight_sums, dim3(unit_blocks(n)), dim3(UNIT_TPB), 0, st, abort_flag, weights,
+                           keys, v.start, v.end, n, usum);
+    hipLaunchKernelGGL(k_ds_mass_update, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, v, V, usum, meas_occ,
+                       meas_free, p_B, born, norm, born_parts);
+    if (n > 0)
+        hipLaunchKernelGGL(k_ds_normalize, dim3(blocks_for(n)), dim3(TPB), 0, st, abort_flag, keys, norm, weights, n);
+    return hipGetLastError();
+}
+
+// First frame (src/dsphdmib/dom.cu:55-82, copyMassesKernel at
+// measurement_grid.cu:55-62): the born masses are the measurement grid's
+// occupied masses.
+__global__ __launch_bounds__(TPB) void k_ds_born_from_meas(const float* __restrict__ meas_occ, float* __restrict__ born,
+                                                           int V, uint32_t* __restrict__ born_parts) {
+    const int c = blockIdx.x * TPB + threadIdx.x;
+    float b = 0.0f;
+    if (c < V) {
+        b = meas_occ[c];
+        born[c] = b;
+    }
+    block_max_store(b, born_parts + blockIdx.x);
+}
+hipError_t launch_ds_born_from_meas(hipStream_t st, const float* meas_occ, float* born, int V, uint32_t* born_parts) {
+    hipLaunchKernelGGL(k_ds_born_from_meas, dim3(blocks_for(V)), dim3(TPB), 0, st, meas_occ, born, V, born_parts);
+    return hipGetLastError();
+}
+
 // One particle's persistent update (src/common/kernel/update_persistent_particles.cu:23-24,52-53
 // with likelihood 1 and p_A 0, which K3DOM never changes): returns the new
 // weight and, through *pen, the weight after the static penalty

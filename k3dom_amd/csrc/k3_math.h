@@ -371,6 +371,89 @@ K3_HD float k3_newborn_part(float m_dyn_pred, float m_total_pred, float m_dyn_up
     return (m_dyn_up * p_B * (m_total_pred - m_dyn_pred)) / (m_dyn_pred + p_B * (m_total_pred - m_dyn_pred));
 }
 
+// ---------------------------------------------------------------- DS-PHD/MIB method
+// The reference's second filter (src/dsphdmib/): a measurement grid of
+// occupied/free masses from the rays near each voxel, then a Dempster-Shafer
+// update of the cell's occupied/free masses (src/dsphdmib/kernel/
+// measurement_grid.cu:12-163, src/dsphdmib/kernel/mass_update.cu:12-87).
+// Where the reference mixes double literals into float expressions, the
+// restatement computes in double at the same points.
+#define K3DS_MAX_NEIGHBORS 100   // rays per voxel (first ones in point order), measurement_grid.cu:82
+#define K3DS_OCC_MAX 0.95f       // measurement_grid.cu:37
+#define K3DS_P_MIN 0.15f         // measurement_grid.cu:50-51
+#define K3DS_P_MAX 1.0f
+#define K3DS_EPS 0.00001f        // mass clipping, measurement_grid.cu:146
+#define K3DS_ALPHA 0.9f          // free-mass decay of predict_free_mass, mass_update.cu:12
+
+// Half-line of point m passes near the voxel at c (|c|^2 = c2): positive
+// projection and squared distance to the line below w2 (measurement_grid.cu:86-93).
+K3_HD bool k3ds_ray_near(float x, float y, float z, float c2, float mx, float my, float mz, float m2, float w2) {
+    const float inner = x * mx + y * my + z * mz;
+    return inner > 0.0f && c2 - inner * inner / m2 < w2;
+}
+
+// pOcc, measurement_grid.cu:31-41 (own exp)
+K3_HD float k3ds_pocc(float x, float y, float z, float mx, float my, float mz, float delta) {
+    const float delta_sqr = delta * delta;
+    const float r_sqr = (x - mx) * (x - mx) + (y - my) * (y - my) + (z - mz) * (z - mz);
+    return K3DS_OCC_MAX * k3_exp(-0.5f * r_sqr / delta_sqr);
+}
+
+// pFree, measurement_grid.cu:44-52
+K3_HD float k3ds_pfree(float x, float y, float z, float max_range) {
+    const float r = sqrtf(x * x + y * y + z * z);
+    return K3DS_P_MIN + r / max_range * (K3DS_P_MAX - K3DS_P_MIN);
+}
+
+// Measurement masses of a voxel from its near rays: n of them (at most
+// K3DS_MAX_NEIGHBORS), all_farther = none of their points is nearer the
+// sensor than the voxel, max_occ = their largest pOcc (measurement_grid.cu:100-151).
+K3_HD void k3ds_meas_masses(int n, bool all_farther, float max_occ, float x, float y, float z, float max_range,
+                            float* m_occ, float* m_free) {
+    if (n == 0) {
+        *m_occ = 0.0f;
+        *m_free = 0.0f;
+        return;
+    }
+    float mo = 0.0f, mf = 0.0f;
+    if (all_farther) {
+        const float uncertain = k3ds_pfree(x, y, z, max_range);
+        if (max_occ > uncertain) {
+            mo = max_occ;
+        } else {
+            mf = 1.0f - uncertain;
+        }
+    } else {
+        mo = max_occ > 0.5f ? max_occ : 0.0f;
+    }
+    *m_occ = fmaxf(K3DS_EPS, fminf(1.0f - K3DS_EPS, mo));
+    *m_free = fmaxf(K3DS_EPS, fminf(1.0f - K3DS_EPS, mf));
+}
+
+// predict_free_mass, mass_update.cu:12-15
+K3_HD float k3ds_predict_free(float free_mass, float m_occ_pred) {
+    return fminf(K3DS_ALPHA * free_mass, 1.0f - m_occ_pred);
+}
+
+// update_masses, mass_update.cu:17-30 (Dempster's rule with the measurement;
+// the 1.0 literals make those terms double)
+K3_HD void k3ds_update_masses(float m_occ_pred, float m_free_pred, float meas_occ, float meas_free, float* occ_up,
+                              float* free_up) {
+    const float unknown_pred = (float)(1.0 - (double)m_occ_pred - (double)m_free_pred);
+    const float meas_unknown = (float)(1.0 - (double)meas_free - (double)meas_occ);
+    const float K = m_free_pred * meas_occ + m_occ_pred * meas_free;
+    const float no = m_occ_pred * meas_unknown + unknown_pred * meas_occ + m_occ_pred * meas_occ;
+    const float nf = m_free_pred * meas_unknown + unknown_pred * meas_free + m_free_pred * meas_free;
+    *occ_up = (float)((double)no / (1.0 - (double)K));
+    *free_up = (float)((double)nf / (1.0 - (double)K));
+}
+
+// separate_newborn_part, mass_update.cu:32-35 (double where the literal makes it)
+K3_HD float k3ds_newborn_part(float m_occ_pred, float m_occ_up, float p_B) {
+    const double rest = 1.0 - (double)m_occ_pred;
+    return (float)(((double)(m_occ_up * p_B) * rest) / ((double)m_occ_pred + (double)p_B * rest));
+}
+
 // ---------------------------------------------------------------- particles
 // Voxel key of a position in voxel-index units, src/common/kernel/predict.cu:51-55.
 // Clamping in float before truncation equals the reference's truncate-then-clamp

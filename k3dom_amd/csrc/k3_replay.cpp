@@ -1,21 +1,28 @@
-// Frame-replay driver over the C++ drop-in header (include/k3dom/dom.h): the
-// per-frame sequence of the reference's ROS node callback
-// (src/k3dom/k3dom_node.cpp:64-97): updatePose with the sensor position, then
-// updateGrid with the sensor-relative points, then copyCellsToHost.
+// Frame-replay driver over the C++ drop-in headers: the per-frame sequence
+// of the reference's ROS node callbacks (src/k3dom/k3dom_node.cpp:64-97,
+// src/dsphdmib/dsphdmib_node.cpp:61-94): updatePose with the sensor position,
+// then updateGrid with the sensor-relative points, then copyCellsToHost.
+// Built twice: k3dom_replay over include/k3dom/dom.h and, with
+// K3_REPLAY_DSPHDMIB defined, k3dom_replay_dsphdmib over include/dsphdmib/dom.h.
 //
 //   k3dom_replay PARAMS FRAMES OUT
 //     PARAMS  raw k3dom_params record (sizeof(k3dom_params) bytes)
 //     FRAMES  "K3FR", int32 frame count, then per frame: double t,
 //             float pose[3], int32 n, float x[n], float y[n], float z[n]
 //     OUT     int32 grid_cell_count, then that many 68-byte GridCell records
-//             after the last frame
+//             after the last frame (DS-PHD/MIB: then as many MeasurementCell
+//             records of the last frame)
 // Exit status 0 on success; any failed call is reported and gives status 1.
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <vector>
 
+#ifdef K3_REPLAY_DSPHDMIB
+#include "dsphdmib/dom.h"
+#else
 #include "k3dom/dom.h"
+#endif
 
 namespace {
 bool read_exact(FILE* f, void* dst, size_t n) { return std::fread(dst, 1, n, f) == n; }
@@ -103,6 +110,15 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "cannot write %s\n", argv[3]);
         return 2;
     }
+#ifdef K3_REPLAY_DSPHDMIB
+    std::vector<dom::MeasurementCell> meas;
+    grid_map.copyMeasGridToHost(meas);
+    if (grid_map.lastStatus() != K3DOM_OK) return 1;
+    if (std::fwrite(meas.data(), sizeof(dom::MeasurementCell), meas.size(), of) != meas.size()) {
+        std::fprintf(stderr, "cannot write %s\n", argv[3]);
+        return 2;
+    }
+#endif
     std::fclose(of);
     std::printf("replayed %d frames, %d cells\n", nframes, V);
     return 0;

@@ -44,15 +44,23 @@ def _fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+# Mass-update methods (include/k3dom.h: K3DOM_METHOD_*).
+METHODS = {"k3dom": 0, "dsphdmib": 1}
+
+
 class DOM:
     """One map on the current HIP device."""
 
-    def __init__(self, params, _handle=None, _owner=None):
+    def __init__(self, params, _handle=None, _owner=None, method="k3dom"):
+        """method: "k3dom" (kernel inference) or "dsphdmib" (the reference's
+        second filter, a measurement grid of occupied / free masses)."""
         self._lib = _lib.load()
         self.params = params
+        self.method = method
         if _handle is None:
             h = C.c_void_p()
-            check(self._lib.k3dom_create(C.byref(params), C.byref(h)), None, "k3dom_create")
+            check(self._lib.k3dom_create_method(C.byref(params), METHODS[method], C.byref(h)), None,
+                  "k3dom_create_method")
             self._owned = True
         else:  # a map owned by a batch (kept alive through _owner)
             h = _handle

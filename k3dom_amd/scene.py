@@ -198,10 +198,20 @@ def batch_workloads(n_tiles=32, seed=4, edge_min=32, edge_max=128, particles_min
     return out
 
 
+def dsphdmib_params(res, edge_xy, edge_z, particles, newborn, seed=123456, **over):
+    """Filter parameters of the DS-PHD/MIB method configuration at resolution
+    res (config/dsphdmib.yaml: meas_occ_stddev 0.1 m, particle_min_vel 0, zero
+    priors; max_range 50 m as src/dsphdmib/dsphdmib_node.cpp:346)."""
+    over = dict(dict(particle_min_vel=0.0, prior_free=0.0, prior_static=0.0, prior_dynamic=0.0, prior_occ=0.0,
+                     meas_occ_stddev=0.1, max_range=50.0), **over)
+    return k3dom_params(res, edge_xy, edge_z, particles, newborn, seed=seed, **over)
+
+
 def small_workload(edge=16, edge_z=8, res=0.2, particles=4096, newborn=409, beams=8, az=128, frames=6, seed=11,
-                   moving=2, static=6, ego_vx=0.0, **over):
-    """A tiny scene for fast parity tests."""
-    p = k3dom_params(res, edge, edge_z, particles, newborn, **over)
+                   moving=2, static=6, ego_vx=0.0, method="k3dom", **over):
+    """A tiny scene for fast parity tests (method "dsphdmib": that method's parameters)."""
+    make = dsphdmib_params if method == "dsphdmib" else k3dom_params
+    p = make(res, edge, edge_z, particles, newborn, **over)
     half = edge * res / 2
     s = _desc(seed=seed, n_static=static, n_moving=moving, speed_max=2.0, xmin=-2 * half, xmax=2 * half,
               ymin=-2 * half, ymax=2 * half, ground_z=-edge_z * res / 2, clearance=0.6, ego_vx=ego_vx,

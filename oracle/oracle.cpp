@@ -11,6 +11,8 @@
 //                                    init_new_particles, statistical_moments, resampling,
 //                                    ego_motion_compensation, init
 //   include/common/dom_types.h       GridCell fields and accessors
+//   src/dsphdmib/dom.cu, src/dsphdmib/kernel/{measurement_grid,mass_update}.cu
+//                                    the DS-PHD/MIB method (measurement grid and cell update)
 //
 // Deliberate, documented deviations from the reference (each also in the
 // repository design notes):
@@ -270,6 +272,7 @@ struct Oracle {
     float sensor[3] = {0, 0, 0}, center[3] = {0, 0, 0};
     double updated_time = -1.0;
     bool brute = false;
+    int method = K3DOM_METHOD_K3DOM;
 
     explicit Oracle(const k3dom_params& prm) : p(prm) {
         gs = (int)(p.size / p.resolution);
@@ -382,6 +385,71 @@ struct Oracle {
         stage_ms[4] = ms_since(t0);
     }
 
+    // DS-PHD/MIB measurement grid, src/dsphdmib/kernel/measurement_grid.cu:64-160:
+    // every voxel scans the points in index order and keeps the first
+    // K3DS_MAX_NEIGHBORS whose half-line from the sensor passes within
+    // 0.866 resolution of its centre (src/dsphdmib/dom.cu:21-22).  The masses go
+    // to del_occ / del_free, the arrays that hold the evidence in the K3DOM path.
+    void ds_meas_grid(const float* xs, const float* ys, const float* zs, int n) {
+        auto t0 = clk::now();
+        Geom g{gs, gsz, p.resolution, sensor[0] - center[0], sensor[1] - center[1], sensor[2] - center[2]};
+        g.build();
+        const float wr = p.resolution * 0.866f;
+        const float w2 = wr * wr;
+        std::vector<float> m2(n);
+        for (int q = 0; q < n; ++q) m2[q] = xs[q] * xs[q] + ys[q] * ys[q] + zs[q] * zs[q];
+        del_occ.assign(V, 0.0f);
+        del_free.assign(V, 0.0f);
+        for (int c = 0; c < V; ++c) {
+            const float x = g.cx[c % gs], y = g.cy[(c % (gs * gs)) / gs], z = g.cz[c / (gs * gs)];
+            const float c2 = x * x + y * y + z * z;
+            int nb = 0;
+            bool all_farther = true;
+            float max_occ = 0.0f;
+            for (int q = 0; q < n && nb < K3DS_MAX_NEIGHBORS; ++q) {
+                if (!k3ds_ray_near(x, y, z, c2, xs[q], ys[q], zs[q], m2[q], w2)) continue;
+                if (all_farther && c2 > m2[q]) all_farther = false;
+                const float o = k3ds_pocc(x, y, z, xs[q], ys[q], zs[q], p.meas_occ_stddev);
+                if (o > max_occ) max_occ = o;
+                ++nb;
+            }
+            k3ds_meas_masses(nb, all_farther, max_occ, x, y, z, p.max_range, &del_occ[c], &del_free[c]);
+        }
+        stage_ms[3] = ms_since(t0);
+    }
+
+    // DS-PHD/MIB cell update, gridCellPredictionUpdateKernel
+    // (src/dsphdmib/kernel/mass_update.cu:44-85): predicted occupied mass =
+    // the voxel's particle weights (two-level sum); above 1 the weights are
+    // divided by it and it becomes 1.  The sorted weights are updated in place.
+    void ds_mass_update(const float* xs, const float* ys, const float* zs, int n) {
+        ds_meas_grid(xs, ys, zs, n);
+        auto t0 = clk::now();
+        for (int c = 0; c < V; ++c) {
+            float occ_up, free_up, rho = 0.0f, m_occ_pred = 0.0f;
+            if (start[c] != -1) {
+                m_occ_pred = seg_sum(w.data(), start[c], end[c]);
+                if (m_occ_pred > 1.0f) {
+                    for (int j = start[c]; j <= end[c]; ++j) w[j] /= m_occ_pred;
+                    m_occ_pred = 1.0f;
+                }
+                const float m_free_pred = k3ds_predict_free(f[c], m_occ_pred);
+                k3ds_update_masses(m_occ_pred, m_free_pred, del_occ[c], del_free[c], &occ_up, &free_up);
+                rho = k3ds_newborn_part(m_occ_pred, occ_up, p.birth_prob);
+            } else {
+                const float m_occ = occ[c];
+                const float m_free = k3ds_predict_free(f[c], m_occ);
+                k3ds_update_masses(m_occ, m_free, del_occ[c], del_free[c], &occ_up, &free_up);
+            }
+            born[c] = rho;
+            pers[c] = occ_up - rho;
+            f[c] = free_up;
+            occ[c] = occ_up;
+            pred[c] = m_occ_pred;
+        }
+        stage_ms[4] = ms_since(t0);
+    }
+
     // particle slots of a fixed-point CDF over the born masses
     void slots_from_born(uint32_t cnt, std::vector<uint32_t>& bnd) {
         std::vector<uint64_t> q;
@@ -394,10 +462,17 @@ struct Oracle {
         return (int)(std::upper_bound(bnd.begin(), bnd.end(), k) - bnd.begin());
     }
 
-    // DOM::initializeParticles, src/k3dom/dom.cu:52-77 and
-    // initParticlesKernel1/2, src/common/kernel/init_new_particles.cu:54-104
+    // DOM::initializeParticles, src/k3dom/dom.cu:52-77 (DS-PHD/MIB:
+    // src/dsphdmib/dom.cu:55-82, born masses = the measurement grid's occupied
+    // masses, cells untouched) and initParticlesKernel1/2,
+    // src/common/kernel/init_new_particles.cu:54-104
     void initialize_particles(const float* xs, const float* ys, const float* zs, int n) {
-        mass_update(1.0f, xs, ys, zs, n, w);
+        if (method == K3DOM_METHOD_DSPHDMIB) {
+            ds_meas_grid(xs, ys, zs, n);
+            born = del_occ;
+        } else {
+            mass_update(1.0f, xs, ys, zs, n, w);
+        }
         auto t0 = clk::now();
         std::vector<uint32_t> bnd;
         slots_from_born((uint32_t)N, bnd);
@@ -602,7 +677,8 @@ struct Oracle {
             t0 = clk::now();
             assign();
             stage_ms[1] = ms_since(t0);
-            mass_update(decay, xs, ys, zs, n, w);
+            if (method == K3DOM_METHOD_DSPHDMIB) ds_mass_update(xs, ys, zs, n);
+            else mass_update(decay, xs, ys, zs, n, w);
             t0 = clk::now();
             update_persistent();
             stage_ms[5] = ms_since(t0);
@@ -812,6 +888,12 @@ void* orc_create(const k3dom_params* p) {
         return nullptr;
     return new Oracle(*p);
 }
+void* orc_create_method(const k3dom_params* p, int method) {
+    if (method != K3DOM_METHOD_K3DOM && method != K3DOM_METHOD_DSPHDMIB) return nullptr;
+    Oracle* o = (Oracle*)orc_create(p);
+    if (o) o->method = method;
+    return o;
+}
 void orc_destroy(void* h) { delete (Oracle*)h; }
 void orc_set_brute(void* h, int brute) { ((Oracle*)h)->brute = brute != 0; }
 void orc_update_pose(void* h, float x, float y, float z) { ((Oracle*)h)->update_pose(x, y, z); }
@@ -941,6 +1023,14 @@ int orc_window_chain64(const k3dom_params* p, int32_t T, const int32_t* npts, co
 // kernel at distance d (evaluated from the float32 square d * d)
 float orc_kernel(float d, float ls, float sigma) { return k3_kernel_value_sq(d * d, k3_kernel_scale(ls), sigma); }
 float orc_credit(float m, float scale) { return k3_credit(m, scale); }
+float orc_ds_pocc(float x, float y, float z, float mx, float my, float mz, float delta) {
+    return k3ds_pocc(x, y, z, mx, my, mz, delta);
+}
+float orc_ds_pfree(float x, float y, float z, float max_range) { return k3ds_pfree(x, y, z, max_range); }
+void orc_ds_update_masses(float occ_pred, float free_pred, float meas_occ, float meas_free, float* out2) {
+    k3ds_update_masses(occ_pred, free_pred, meas_occ, meas_free, &out2[0], &out2[1]);
+}
+float orc_ds_newborn_part(float occ_pred, float occ_up, float p_B) { return k3ds_newborn_part(occ_pred, occ_up, p_B); }
 void orc_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) { k3_box_muller(a, b, n0, n1); }
 void orc_update_mass(float* fsd, float del_occ, float del_free, float mass_scale) {
     k3_update_mass(&fsd[0], &fsd[1], &fsd[2], del_occ, del_free, mass_scale);

@@ -27,6 +27,11 @@ def lib():
         fp, u32p = C.POINTER(C.c_float), C.POINTER(C.c_uint32)
         sig = {
             "orc_create": (C.c_void_p, [C.POINTER(Params)]),
+            "orc_create_method": (C.c_void_p, [C.POINTER(Params), C.c_int]),
+            "orc_ds_pocc": (C.c_float, [C.c_float] * 7),
+            "orc_ds_pfree": (C.c_float, [C.c_float] * 4),
+            "orc_ds_update_masses": (None, [C.c_float] * 4 + [C.POINTER(C.c_float)]),
+            "orc_ds_newborn_part": (C.c_float, [C.c_float] * 3),
             "orc_destroy": (None, [C.c_void_p]),
             "orc_set_brute": (None, [C.c_void_p, C.c_int]),
             "orc_update_pose": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
@@ -76,11 +81,12 @@ def _fp(a):
 class OracleDOM:
     """Serial restatement with the same surface as k3dom_amd.dom.DOM."""
 
-    def __init__(self, params, brute=False):
+    def __init__(self, params, brute=False, method="k3dom"):
         from k3dom_amd._lib import Info
+        from k3dom_amd.dom import METHODS
         self._L = lib()
         self.params = params
-        self._h = self._L.orc_create(C.byref(params))
+        self._h = self._L.orc_create_method(C.byref(params), METHODS[method])
         if not self._h:
             raise ValueError("invalid parameters")
         if brute:

@@ -42,15 +42,16 @@ def compare_state(gd, od, normal_frame, what=""):
         assert bits_equal(a, b), f"{what} {name}: {first_diff(a, b)}"
 
 
-def run_pair(workload, frames, check_every=1, mutate=None):
-    """Drives both implementations through `frames` frames of the workload's
-    scene and compares after each checked frame."""
+def run_pair(workload, frames, check_every=1, mutate=None, method="k3dom"):
+    """Drives both implementations (mass-update method `method`) through
+    `frames` frames of the workload's scene and compares after each checked
+    frame."""
     from k3dom_amd.dom import DOM
     from k3dom_amd.scene import Scene
     from oracle.orc import OracleDOM
     sc = Scene(workload.scene)
-    gd = DOM(workload.params)
-    od = OracleDOM(workload.params)
+    gd = DOM(workload.params, method=method)
+    od = OracleDOM(workload.params, method=method)
     for f in range(frames):
         t, pose, xs, ys, zs = sc.frame(f)
         if mutate is not None:

@@ -12,6 +12,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REPLAY = os.path.join(ROOT, "k3dom_amd", "k3dom_replay")
+REPLAY_DS = os.path.join(ROOT, "k3dom_amd", "k3dom_replay_dsphdmib")
 
 
 def test_header_compiles_with_reference_names(tmp_path):
@@ -84,3 +85,62 @@ def test_replay_through_cpp_header_matches_oracle(gpu, tmp_path):
         od.updatePose(*pose)
         od.updateGrid(t, xs, ys, zs)
     compare_cells(cells, od.copyCellsToHost(), "replay")
+
+
+def test_dsphdmib_header_compiles_with_reference_names(tmp_path):
+    """A caller of the reference's DS-PHD/MIB class (include/dsphdmib/dom.h:17-41)
+    compiles unchanged: same class and member names, MeasurementCell records,
+    window_radius = 0.866 resolution."""
+    src = tmp_path / "caller_ds.cpp"
+    src.write_text(r'''
+#include "dsphdmib/dom.h"
+int main() {
+    static_assert(sizeof(dom::MeasurementCell) == 8, "MeasurementCell size");
+    dom::MeasurementCell m{0.25f, 0.5f};
+    if (m.free_mass != 0.25f || m.occ_mass != 0.5f) return 1;
+    dom::GridCell c{};
+    c.occ_mass = 0.5f; c.free_mass = 0.25f;
+    if (c.occ_prob() != 0.625f) return 2;
+    void (dom::DOM::*upd)(float, std::vector<float>&, std::vector<float>&, std::vector<float>&) =
+        &dom::DOM::updateGrid;
+    void (dom::DOM::*meas)(std::vector<dom::MeasurementCell>&) = &dom::DOM::copyMeasGridToHost;
+    float dom::DOM::*wr = &dom::DOM::window_radius;
+    (void)upd; (void)meas; (void)wr;
+    return 0;
+}
+''')
+    exe = tmp_path / "caller_ds"
+    lib = os.path.join(ROOT, "k3dom_amd")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-L", lib, "-lk3dom",
+                    "-Wl,-rpath," + lib, "-o", str(exe)], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
+
+
+@pytest.mark.gpu
+def test_replay_through_dsphdmib_header_matches_oracle(gpu, tmp_path):
+    """The DS-PHD/MIB dom::DOM, in its own process, gives the oracle's cells
+    and measurement grid bit for bit after a map-shifting sequence of frames."""
+    from k3dom_amd.dom import CELL_DTYPE
+    from k3dom_amd.scene import small_workload
+    from oracle.orc import OracleDOM
+    from tests._parity import bits_equal, compare_cells, first_diff
+    assert os.path.exists(REPLAY_DS), "k3dom_replay_dsphdmib is not built (run __graft_entry__.build())"
+    w = small_workload(ego_vx=4.0, frames=6, seed=4, method="dsphdmib")
+    recs = _write_inputs(tmp_path, w, w.frames)
+    out = tmp_path / "cells.bin"
+    print("DS replay: small workload, 6 frames, moving ego", flush=True)
+    r = subprocess.run([REPLAY_DS, str(tmp_path / "params.bin"), str(tmp_path / "frames.bin"), str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = out.read_bytes()
+    V = struct.unpack("<i", raw[:4])[0]
+    cells = np.frombuffer(raw[4:4 + 68 * V], CELL_DTYPE)
+    meas = np.frombuffer(raw[4 + 68 * V:], np.float32).reshape(V, 2)
+    od = OracleDOM(w.params, method="dsphdmib")
+    for t, pose, xs, ys, zs in recs:
+        od.updatePose(*pose)
+        od.updateGrid(t, xs, ys, zs)
+    compare_cells(cells, od.copyCellsToHost(), "DS replay")
+    for k, name in ((0, "DEL_FREE"), (1, "DEL_OCC")):
+        a, b = np.ascontiguousarray(meas[:, k]), od.read_array(name)
+        assert bits_equal(a, b), f"DS replay measurement grid {name}: {first_diff(a, b)}"

@@ -1,0 +1,252 @@
+"""The DS-PHD/MIB mass update (src/dsphdmib/): measurement grid, Dempster-
+Shafer cell update, newborn split and weight renormalisation.
+
+CPU tests pin the serial restatement against closed forms and hand-computed
+geometry; GPU tests hold the HIP path to it bit for bit."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+EPS = 1e-5          # mass clipping, src/dsphdmib/kernel/measurement_grid.cu:146
+P_MIN, OCC_MAX = 0.15, 0.95
+
+
+def _ds(L, name, *args):
+    return getattr(L, f"orc_ds_{name}")(*args)
+
+
+def _update(L, op, fp, om, fm):
+    out = (C.c_float * 2)()
+    L.orc_ds_update_masses(op, fp, om, fm, out)
+    return out[0], out[1]
+
+
+# ---------------------------------------------------------------- closed forms
+
+def test_pocc_closed_form(oracle_lib):
+    L = oracle_lib
+    assert _ds(L, "pocc", 1.0, 2.0, 3.0, 1.0, 2.0, 3.0, 0.1) == np.float32(OCC_MAX)
+    # r = delta: 0.95 exp(-1/2); r = 2 delta: 0.95 exp(-2)
+    assert abs(_ds(L, "pocc", 0.1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.1) - OCC_MAX * math.exp(-0.5)) < 1e-6
+    assert abs(_ds(L, "pocc", 0.0, 0.0, 0.5, 0.0, 0.0, 0.3, 0.1) - OCC_MAX * math.exp(-2.0)) < 1e-6
+    # symmetric in voxel and point
+    assert _ds(L, "pocc", 0.3, -0.2, 0.1, 0.25, -0.1, 0.0, 0.1) == _ds(L, "pocc", 0.25, -0.1, 0.0, 0.3, -0.2, 0.1, 0.1)
+
+
+def test_pfree_is_linear_in_range(oracle_lib):
+    L = oracle_lib
+    assert _ds(L, "pfree", 0.0, 0.0, 0.0, 50.0) == np.float32(P_MIN)
+    assert _ds(L, "pfree", 30.0, 40.0, 0.0, 50.0) == np.float32(1.0)   # |c| = max_range
+    assert abs(_ds(L, "pfree", 0.0, 12.0, 5.0, 26.0) - (P_MIN + 0.5 * (1 - P_MIN))) < 1e-6  # |c| = 13
+
+
+def test_dempster_vacuous_operands_are_identities(oracle_lib):
+    L = oracle_lib
+    for op, fp in [(0.3, 0.5), (0.0, 0.9), (0.999, 0.0), (0.25, 0.25)]:
+        # a vacuous measurement leaves the prediction, a vacuous prediction takes the measurement
+        assert _update(L, op, fp, 0.0, 0.0) == (np.float32(op), np.float32(fp))
+        assert _update(L, 0.0, 0.0, op, fp) == (np.float32(op), np.float32(fp))
+
+
+def test_dempster_known_combination(oracle_lib):
+    # prediction (occ 0.6, free 0.3, unknown 0.1), measurement (0.5, 0.2, 0.3):
+    # conflict K = 0.3*0.5 + 0.6*0.2 = 0.27, occ = 0.53/0.73, free = 0.17/0.73,
+    # unknown = 0.1*0.3/0.73
+    o, f = _update(oracle_lib, 0.6, 0.3, 0.5, 0.2)
+    assert abs(o - 0.53 / 0.73) < 1e-6 and abs(f - 0.17 / 0.73) < 1e-6
+    assert abs((1 - o - f) - 0.03 / 0.73) < 1e-6
+
+
+def test_dempster_commutes_and_conserves_mass(oracle_lib):
+    L = oracle_lib
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        a = rng.dirichlet([1, 1, 1])
+        b = rng.dirichlet([1, 1, 1])
+        op, fp, om, fm = (float(np.float32(v)) for v in (a[0], a[1], b[0], b[1]))
+        x = _update(L, op, fp, om, fm)
+        assert x == _update(L, om, fm, op, fp)            # exact: the same float terms in the same order
+        K = fp * om + op * fm
+        if K > 0.999:
+            continue
+        unknown = (1 - op - fp) * (1 - om - fm) / (1 - K)
+        assert abs(x[0] + x[1] + unknown - 1.0) < 1e-5
+        assert x[0] >= 0 and x[1] >= 0
+
+
+def test_newborn_split_limits(oracle_lib):
+    L = oracle_lib
+    # nothing predicted: all updated mass is newborn; fully predicted: none is
+    assert abs(_ds(L, "newborn_part", 0.0, 0.7, 0.02) - 0.7) < 1e-7
+    assert _ds(L, "newborn_part", 1.0, 0.7, 0.02) == 0.0
+    # pred 0.5, p_B 0.02: 0.7 * 0.02*0.5 / (0.5 + 0.01)
+    assert abs(_ds(L, "newborn_part", 0.5, 0.7, 0.02) - 0.7 * 0.01 / 0.51) < 1e-7
+
+
+# ------------------------------------------------------- measurement geometry
+
+def _tiny_map(particles=64):
+    from k3dom_amd.scene import dsphdmib_params
+    return dsphdmib_params(0.2, 8, 4, particles, 8)
+
+
+def _meas(dom_cls, pts, **kw):
+    p = _tiny_map()
+    d = dom_cls(p, method="dsphdmib", **kw)
+    d.updatePose(0.0, 0.0, 0.0)
+    xs, ys, zs = (np.ascontiguousarray(np.asarray(pts, np.float32)[:, k]) for k in range(3))
+    d.updateGrid(1.0, xs, ys, zs)
+    return d.read_array("DEL_OCC"), d.read_array("DEL_FREE"), d.read_array("BORN_MASS")
+
+
+def _cell(i, j, k):
+    # 8x8x4 grid, x fastest; voxel centre ((i + 0.5 - 4) 0.2, (j + 0.5 - 4) 0.2, (k + 0.5 - 2) 0.2)
+    return i + 8 * j + 64 * k
+
+
+def _free(r):
+    return 1.0 - (P_MIN + r / 50.0 * (1.0 - P_MIN))
+
+
+def test_meas_grid_single_ray_by_hand(oracle_lib):
+    """One point at the centre (0.5, 0.1, 0.1) of voxel (6, 4, 2), sensor at
+    the origin: the voxel itself is occupied, the voxels the ray crosses before
+    it are free with 1 - pFree(|c|), the voxel behind it gets the clipped
+    minimum, voxels off the ray or behind the sensor get nothing."""
+    from oracle.orc import OracleDOM
+    occ, free, born = _meas(OracleDOM, [(0.5, 0.1, 0.1)])
+    assert abs(occ[_cell(6, 4, 2)] - OCC_MAX) < 1e-7 and free[_cell(6, 4, 2)] == np.float32(EPS)
+    assert occ[_cell(5, 4, 2)] == np.float32(EPS) and abs(free[_cell(5, 4, 2)] - _free(math.sqrt(0.11))) < 1e-6
+    assert occ[_cell(4, 4, 2)] == np.float32(EPS) and abs(free[_cell(4, 4, 2)] - _free(math.sqrt(0.03))) < 1e-6
+    # behind the point: not all farther, pOcc(r = 0.2) = 0.95 e^-2 < 0.5
+    assert occ[_cell(7, 4, 2)] == np.float32(EPS) and free[_cell(7, 4, 2)] == np.float32(EPS)
+    # behind the sensor (negative projection) and far from the ray
+    for c in (_cell(3, 4, 2), _cell(4, 7, 2), _cell(0, 0, 0)):
+        assert occ[c] == 0.0 and free[c] == 0.0
+    # the first frame's born masses are the occupied masses (src/dsphdmib/dom.cu:55-82)
+    assert np.array_equal(born, occ)
+
+
+def _ray_points(first_centre, n_far):
+    """n_far points on the half-line through voxel (6, 4, 2) well beyond it
+    (all farther, negligible pOcc) and one point at the voxel centre, first or
+    last."""
+    d = np.array([0.5, 0.1, 0.1])
+    far = [tuple(d * (2.0 + 0.01 * q)) for q in range(n_far)]
+    return [tuple(d)] + far if first_centre else far + [tuple(d)]
+
+
+def test_meas_grid_keeps_the_first_100_rays(oracle_lib):
+    """Each voxel considers only the first 100 rays passing near it in point
+    order (measurement_grid.cu:82-96): a hit after 100 far rays is not seen, a
+    hit as the 100th ray is."""
+    from oracle.orc import OracleDOM
+    c = _cell(6, 4, 2)
+    free_c = _free(math.sqrt(0.27))
+    occ, free, _ = _meas(OracleDOM, _ray_points(False, 100))
+    assert occ[c] == np.float32(EPS) and abs(free[c] - free_c) < 1e-6
+    occ, free, _ = _meas(OracleDOM, _ray_points(False, 99))
+    assert abs(occ[c] - OCC_MAX) < 1e-7 and free[c] == np.float32(EPS)
+    occ, free, _ = _meas(OracleDOM, _ray_points(True, 150))
+    assert abs(occ[c] - OCC_MAX) < 1e-7 and free[c] == np.float32(EPS)
+
+
+def test_renormalisation_occurs_in_small_scene(oracle_lib):
+    """From the fourth frame of the small scene some voxels' predicted occupied
+    mass exceeds 1, so their weights are divided by it (mass_update.cu:61-67):
+    afterwards such a voxel's weights sum to 1 within rounding."""
+    from oracle.orc import OracleDOM
+    from k3dom_amd.scene import Scene, small_workload
+    w = small_workload(method="dsphdmib")
+    sc = Scene(w.scene)
+    od = OracleDOM(w.params, method="dsphdmib")
+    seen = 0
+    for f in range(w.frames):
+        t, pose, xs, ys, zs = sc.frame(f)
+        od.updatePose(*pose)
+        od.updateGrid(t, xs, ys, zs)
+        cells = od.copyCellsToHost()
+        hit = np.nonzero((cells["pred_mass"] == 1.0) & (cells["start_idx"] >= 0))[0]
+        if f == 0 or len(hit) == 0:
+            continue
+        ws = od.read_array("SORTED_WEIGHT")
+        for c in hit[:4]:
+            s = ws[cells["start_idx"][c]:cells["end_idx"][c] + 1].astype(np.float64).sum()
+            assert abs(s - 1.0) < 1e-4
+        seen += len(hit)
+    assert seen > 0
+
+
+# ------------------------------------------------------------------ GPU parity
+
+@pytest.mark.gpu
+def test_ds_meas_grid_single_ray_gpu(gpu):
+    from k3dom_amd.dom import DOM
+    from oracle.orc import OracleDOM
+    from tests._parity import bits_equal, first_diff
+    for pts in ([(0.5, 0.1, 0.1)], _ray_points(False, 100), _ray_points(False, 99), _ray_points(True, 150)):
+        print(f"  single-ray case with {len(pts)} points", flush=True)
+        g = _meas(DOM, pts)
+        o = _meas(OracleDOM, pts)
+        for a, b, name in zip(g, o, ("occ", "free", "born")):
+            assert bits_equal(a, b), f"{len(pts)} points, {name}: {first_diff(a, b)}"
+
+
+@pytest.mark.gpu
+def test_ds_small_scene_bit_exact(gpu):
+    from k3dom_amd.scene import small_workload
+    from tests._parity import run_pair
+    w = small_workload(method="dsphdmib")
+    run_pair(w, w.frames, method="dsphdmib")
+
+
+@pytest.mark.gpu
+def test_ds_moving_ego_map_shift(gpu):
+    from k3dom_amd.scene import small_workload
+    from tests._parity import run_pair
+    w = small_workload(ego_vx=4.0, frames=8, seed=3, method="dsphdmib")
+    gd, od = run_pair(w, w.frames, method="dsphdmib")
+    assert gd.info().center_pos[0] > 0.0
+
+
+@pytest.mark.gpu
+def test_ds_crowded_renormalisation_bit_exact(gpu):
+    from k3dom_amd.scene import small_workload
+    from tests._parity import run_pair
+    # renormalisation from frame 7 on; more than 64 particles per voxel
+    w = small_workload(particles=60000, newborn=6000, frames=8, seed=21, method="dsphdmib")
+    gd, od = run_pair(w, w.frames, method="dsphdmib")
+    cells = gd.copyCellsToHost()
+    occ = cells["start_idx"] >= 0
+    assert (cells["end_idx"][occ] - cells["start_idx"][occ] + 1).max() > 64
+
+
+@pytest.mark.gpu
+def test_ds_empty_frames_and_graph_replay(gpu, monkeypatch):
+    """Empty and invalid point clouds, tiny capacities and graph replay with
+    map shifts, in DS-PHD/MIB mode."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, small_workload
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_state
+    monkeypatch.setenv("K3DOM_TEST_TINY_CAPACITY", "1")
+    w = small_workload(ego_vx=4.0, frames=8, seed=31, method="dsphdmib")
+    sc = Scene(w.scene)
+    gd, od = DOM(w.params, method="dsphdmib"), OracleDOM(w.params, method="dsphdmib")
+    gd.set_graph(True)
+    bad = np.array([np.nan, np.inf, 0.0, -np.inf, 1e30], np.float32)
+    for f in range(w.frames):
+        t, pose, xs, ys, zs = sc.frame(f)
+        if f == 2:
+            xs, ys, zs = xs[:0], ys[:0], zs[:0]
+        elif f == 3:
+            xs, ys, zs = (np.concatenate([xs, bad]), np.concatenate([ys, bad[::-1]]),
+                          np.concatenate([zs, np.zeros(5, np.float32)]))
+        print(f"  DS graph frame {f}: {len(xs)} points", flush=True)
+        normal = gd.info().updated_time > 0
+        for m in (gd, od):
+            m.updatePose(*pose)
+            m.updateGrid(t, xs, ys, zs)
+        compare_state(gd, od, normal_frame=normal, what=f"DS graph frame {f}:")
