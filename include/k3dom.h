@@ -221,7 +221,8 @@ k3dom_status k3dom_count_bgki_pairs(k3dom_handle* h, uint64_t* out2);
 /* Work statistics of the last frame (diagnostic, no reference counterpart). */
 typedef struct k3dom_stats {
     uint64_t points;             /* points of the frame                          */
-    uint64_t level_entries[3];   /* candidate-list entries per level (coarse..tile) */
+    uint64_t level_entries[3];   /* candidate-list entries per level (coarse..tile);
+                                    DS-PHD/MIB: [0] measurement-grid candidates */
     uint64_t gather_items;       /* work items of the kernel-inference gather    */
     uint64_t gather_split_items; /* items of tiles split into point chunks       */
     uint64_t capacity_regrowths; /* frames re-run after growing a list capacity (all frames) */

@@ -86,6 +86,18 @@ void setup_levels(k3dom_handle* h) {
 
 k3dom_status ensure_caps(k3dom_handle* h, const long long want[3], long long want_heavy);
 
+// DS-PHD/MIB measurement-grid candidate list of at least want entries.
+k3dom_status ensure_ds_list(k3dom_handle* h, long long want) {
+    if (want <= h->ds.cap && h->ds.list) return K3DOM_OK;
+    dfree(h, h->ds.list);
+    h->ds.list = nullptr;
+    h->ds.cap = 0;
+    h->caps_version++;
+    HIP_TRY(h, dalloc(h, &h->ds.list, (size_t)want));
+    h->ds.cap = want;
+    return K3DOM_OK;
+}
+
 k3dom_status ensure_points(k3dom_handle* h, int m) {
     if (m <= h->m_cap) return K3DOM_OK;
     int cap = std::max(m, h->m_cap * 3 / 2);
@@ -106,6 +118,10 @@ k3dom_status ensure_points(k3dom_handle* h, int m) {
     if (h->tiny_caps) {
         want[0] = want[1] = want[2] = 64;
         want_heavy = 1;
+    }
+    if (h->method == K3DOM_METHOD_DSPHDMIB) {
+        k3dom_status s = ensure_ds_list(h, h->tiny_caps ? 64 : 32 * mc + 65536);
+        if (s != K3DOM_OK) return s;
     }
     return ensure_caps(h, want, want_heavy);
 }
@@ -299,8 +315,9 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const f
         }
         mark(h, ST_BGKI);
         mark(h, ST_BGKI_KERNEL);
-        HIP_TRY(h, launch_ds_meas_grid(h->stream, h->fc, dx, dy, dz, h->p.resolution, h->p.meas_occ_stddev,
-                                       h->p.max_range, h->V, h->del_occ, h->del_free));
+        HIP_TRY(h, launch_ds_meas_grid(h->stream, &h->sc->overflow, h->fc, h->gs, h->gsz, dx, dy, dz,
+                                       h->p.resolution, h->p.meas_occ_stddev, h->p.max_range, h->ds, h->sc,
+                                       h->del_occ, h->del_free));
         mark(h, EV_BGKI_END);
         return enqueue_after_gather(h, normal);
     }
@@ -393,6 +410,11 @@ k3dom_status resolve_overflow(k3dom_handle* h, bool normal, const float* dx, con
             }
             if (h->h_sc->overflow_mask & (1u << 4))
                 want_heavy = std::max(want_heavy * 2, (long long)h->h_sc->n_heavy * 3 / 2 + 256);
+            if (h->h_sc->overflow_mask & (1u << 5)) {
+                const long long need = (long long)h->h_sc->ds_total * 5 / 4 + 1024;
+                k3dom_status s = ensure_ds_list(h, std::max(h->ds.cap * 2, need));
+                if (s != K3DOM_OK) return s;
+            }
             k3dom_status s = ensure_caps(h, want, want_heavy);
             if (s != K3DOM_OK) return s;
             HIP_TRY(h, launch_clear_overflow(h->stream, h->sc));
@@ -410,6 +432,7 @@ void finish_frame(k3dom_handle* h, double t, int m) {
     h->last_geom = h->h_fc->g;
     h->stats.points = (uint64_t)m;
     for (int l = 0; l < 3; ++l) h->stats.level_entries[l] = h->h_sc->level_total[l];
+    if (h->method == K3DOM_METHOD_DSPHDMIB) h->stats.level_entries[0] = h->h_sc->ds_total;
     h->stats.gather_items = (uint64_t)h->h_sc->n_light + h->h_sc->n_heavy;
     h->stats.gather_split_items = h->h_sc->n_heavy;
     h->stats.capacity_regrowths = (uint64_t)h->overflow_retries;
@@ -753,7 +776,13 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
         CREATE_TRY(dalloc(h, &v.end, V));
     }
     CREATE_TRY(dalloc(h, &h->born, V));
-    if (h->method == K3DOM_METHOD_DSPHDMIB) CREATE_TRY(dalloc(h, &h->ds_norm, V));
+    if (h->method == K3DOM_METHOD_DSPHDMIB) {
+        const size_t nst = (size_t)ds_supertiles(h->gs, h->gsz);
+        CREATE_TRY(dalloc(h, &h->ds_norm, V));
+        CREATE_TRY(dalloc(h, &h->ds.counts, nst));
+        CREATE_TRY(dalloc(h, &h->ds.off, nst + 1));
+        CREATE_TRY(dalloc_zero(h, &h->ds.scratch, (size_t)scan_scratch_words((long long)nst)));
+    }
     CREATE_TRY(dalloc(h, &h->del_occ, V));
     CREATE_TRY(dalloc(h, &h->del_free, V));
     for (int s = 0; s < 2; ++s) {

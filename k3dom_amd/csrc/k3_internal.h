@@ -54,6 +54,7 @@ struct k3dom_handle {
     float* usum = nullptr;   // 3N: per-unit sums of the particle-parallel voxel sums
     uint32_t* maxparts = nullptr;  // per-block maxima feeding the fixed-point CDF exponents
     float* ds_norm = nullptr;      // V (DS-PHD/MIB): sums above 1 of the voxels' weights, else 0
+    DsLists ds{};                  // DS-PHD/MIB: measurement-grid candidate lists
     const uint32_t* last_perm = nullptr;
     const uint32_t* last_skeys = nullptr;
     uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;  // blocksums: fixed-point scan scratch

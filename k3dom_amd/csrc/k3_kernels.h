@@ -18,10 +18,11 @@ struct DevScalars {
     uint32_t n_light;         // items of tiles handled whole
     uint32_t n_heavy;         // items of tiles split into point chunks
     uint32_t overflow;        // non-zero: a capacity was exceeded, the frame tail is skipped
-    uint32_t overflow_mask;   // bit l: candidate level l, bit 4: split items
+    uint32_t overflow_mask;   // bit l: candidate level l, bit 4: split items, bit 5: DS-PHD/MIB lists
     uint32_t level_total[3];  // candidate-list entries per level
     uint32_t level_items[3];  // work items per level
-    uint32_t pad[3];
+    uint32_t ds_total;        // DS-PHD/MIB measurement-grid candidate entries
+    uint32_t pad[2];
 };
 
 // Voxel state arrays (one buffer set).
@@ -208,8 +209,19 @@ hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelA
 // born_parts as launch_mass_update) that also divides the sorted weights of
 // voxels whose sum exceeds 1 (norm: V floats of scratch); born masses of the
 // first frame from the measurement grid.
-hipError_t launch_ds_meas_grid(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
-                               const float* zs, float resolution, float meas_occ_stddev, float max_range, int V,
+// Candidate lists of the culled measurement grid: counts and off hold
+// ds_supertiles(gs, gsz) (+1) words, scratch scan_scratch_words of that many
+// (zeroed once), list cap point indices.  A frame needing more than cap raises
+// the overflow flag (mask bit 5, total in ds_total) and writes no masses.
+struct DsLists {
+    uint32_t *counts, *off, *scratch;
+    int32_t* list;
+    long long cap;
+};
+int ds_supertiles(int gs, int gsz);
+hipError_t launch_ds_meas_grid(hipStream_t st, const uint32_t* abort_flag, const FrameConsts* fc, int gs, int gsz,
+                               const float* xs, const float* ys, const float* zs, float resolution,
+                               float meas_occ_stddev, float max_range, const DsLists& L, DevScalars* sc,
                                float* meas_occ, float* meas_free);
 hipError_t launch_ds_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelArrays v, int V, float* weights,
                                  const int32_t* keys, int n, float* usum, const float* meas_occ,

@@ -653,58 +653,182 @@ hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelA
 // Measurement grid (src/dsphdmib/kernel/measurement_grid.cu:64-160): every
 // voxel scans the points in index order and keeps the first
 // K3DS_MAX_NEIGHBORS whose half-line from the sensor passes within the window
-// of its centre.  Points are staged in LDS 256 at a time (with |m|^2) and read
-// as broadcasts; a block stops when all its voxels have their rays.  The
-// masses go to meas_occ / meas_free (the evidence arrays of the K3DOM path).
+// of its centre (k3ds_ray_near); the masses go to meas_occ / meas_free (the
+// evidence arrays of the K3DOM path).
+//
+// Culled in two levels without changing any result.  A supertile of
+// DS_ST_XY x DS_ST_XY x DS_ST_Z voxels keeps the ordered list of the points
+// whose line may pass near any of its voxel centres; a tile of 4x4x4 voxels
+// (one wavefront, a voxel per lane) filters its supertile's list with the
+// same bound for its own box and applies the exact test to what remains, in
+// list order.  The bound accepts every point the exact float
+// test can accept, so each voxel sees the same accepted points in the same
+// order, hence the same first K3DS_MAX_NEIGHBORS (bound: k3ds_box,
+// k3ds_may_pass).
 constexpr int DS_TPB = 256;
-__global__ __launch_bounds__(DS_TPB) void k_ds_meas_grid(const FrameConsts* __restrict__ fc,
-                                                         const float* __restrict__ xs, const float* __restrict__ ys,
-                                                         const float* __restrict__ zs, float w2, float delta,
-                                                         float max_range, int V, float* __restrict__ meas_occ,
-                                                         float* __restrict__ meas_free) {
-    __shared__ float px[DS_TPB], py[DS_TPB], pz[DS_TPB], pm2[DS_TPB];
+constexpr int DS_ST_XY = 16, DS_ST_Z = 8;
+constexpr int DS_TILE = 4;
+constexpr int DS_TILES_XY = DS_ST_XY / DS_TILE, DS_TILES_Z = DS_ST_Z / DS_TILE;
+constexpr int DS_WAVES_PER_ST = DS_TILES_XY * DS_TILES_XY * DS_TILES_Z;
+constexpr int DS_BLOCKS_PER_ST = DS_WAVES_PER_ST / (DS_TPB / 64);
+static_assert(DS_WAVES_PER_ST % (DS_TPB / 64) == 0, "whole blocks per supertile");
+static_assert(DS_TILE * DS_TILE * DS_TILE == 64, "one voxel per lane");
+
+int ds_supertiles(int gs, int gsz) {
+    const int nxy = (gs + DS_ST_XY - 1) / DS_ST_XY;
+    return nxy * nxy * ((gsz + DS_ST_Z - 1) / DS_ST_Z);
+}
+
+struct DsSuper {
+    int i0, j0, k0;  // first voxel
+};
+__device__ __forceinline__ DsSuper ds_super(int st, int gs) {
+    const int nxy = (gs + DS_ST_XY - 1) / DS_ST_XY;
+    DsSuper s;
+    s.i0 = (st % nxy) * DS_ST_XY;
+    s.j0 = ((st / nxy) % nxy) * DS_ST_XY;
+    s.k0 = (st / (nxy * nxy)) * DS_ST_Z;
+    return s;
+}
+
+// Candidate points of each supertile, one block per supertile, in point
+// order: with off == nullptr the count to counts[st], else the indices to
+// list[off[st]...] (off: exclusive scan of the counts, off[nst] the total;
+// above cap the overflow flag is raised with the total instead).
+__global__ __launch_bounds__(DS_TPB) void k_ds_super_lists(const FrameConsts* __restrict__ fc,
+                                                           const float* __restrict__ xs,
+                                                           const float* __restrict__ ys,
+                                                           const float* __restrict__ zs, float w2, int nst,
+                                                           const uint32_t* __restrict__ off,
+                                                           uint32_t* __restrict__ counts, int32_t* __restrict__ list,
+                                                           long long cap, DevScalars* __restrict__ sc) {
+    __shared__ uint32_t wsum[DS_TPB / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const GridGeom g = fc->g;
     const int n = fc->m;
-    const int t = threadIdx.x;
-    const int c = blockIdx.x * DS_TPB + t;
+    const int st = blockIdx.x;
+    uint32_t pos = 0;
+    if (off) {
+        const uint32_t total = off[nst];
+        if ((long long)total > cap) {
+            if (st == 0 && t == 0) {
+                sc->ds_total = total;
+                atomicOr(&sc->overflow_mask, 1u << 5);
+                atomicExch(&sc->overflow, 1u);
+            }
+            return;
+        }
+        pos = off[st];
+    }
+    const DsSuper s = ds_super(st, g.gs);
+    const K3DsBox b = k3ds_box(g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, s.i0, min(s.i0 + DS_ST_XY, g.gs) - 1, s.j0,
+                               min(s.j0 + DS_ST_XY, g.gs) - 1, s.k0, min(s.k0 + DS_ST_Z, g.gsz) - 1, w2);
+    uint32_t running = 0;
+    for (int base = 0; base < n; base += DS_TPB) {
+        const int q = base + t;
+        bool pass = false;
+        if (q < n) {
+            const float mx = xs[q], my = ys[q], mz = zs[q];
+            pass = k3ds_may_pass(b, mx, my, mz, k3ds_m2(mx, my, mz));
+        }
+        const uint64_t bits = __ballot(pass);
+        if (lane == 0) wsum[wave] = (uint32_t)__popcll(bits);
+        __syncthreads();
+        uint32_t before = 0, chunk = 0;
+        for (int k = 0; k < DS_TPB / 64; ++k) {
+            before += k < wave ? wsum[k] : 0u;
+            chunk += wsum[k];
+        }
+        if (off && pass) {
+            const uint32_t rank = (uint32_t)__popcll(bits & ((1ull << lane) - 1ull));
+            list[pos + running + before + rank] = q;
+        }
+        running += chunk;
+        __syncthreads();
+    }
+    if (!off && t == 0) counts[st] = running;
+}
+
+// Exact measurement masses of every voxel from its supertile's list, one
+// wavefront per 4x4x4 tile (lane = voxel): chunks of 64 listed points are
+// loaded one per lane, filtered by the tile's bound, and the survivors are
+// broadcast in order to every lane for the exact test.
+__global__ __launch_bounds__(DS_TPB) void k_ds_meas_tiles(const uint32_t* __restrict__ abort_flag,
+                                                          const FrameConsts* __restrict__ fc,
+                                                          const float* __restrict__ xs,
+                                                          const float* __restrict__ ys,
+                                                          const float* __restrict__ zs,
+                                                          const uint32_t* __restrict__ off,
+                                                          const int32_t* __restrict__ list, float w2, float delta,
+                                                          float max_range, float* __restrict__ meas_occ,
+                                                          float* __restrict__ meas_free) {
+    if (*abort_flag) return;
+    const int lane = threadIdx.x & 63;
+    const GridGeom g = fc->g;
+    const int st = blockIdx.x / DS_BLOCKS_PER_ST;
+    const int wl = (blockIdx.x % DS_BLOCKS_PER_ST) * (DS_TPB / 64) + (threadIdx.x >> 6);
+    const DsSuper s = ds_super(st, g.gs);
+    const int i0 = s.i0 + (wl % DS_TILES_XY) * DS_TILE;
+    const int j0 = s.j0 + ((wl / DS_TILES_XY) % DS_TILES_XY) * DS_TILE;
+    const int k0 = s.k0 + (wl / (DS_TILES_XY * DS_TILES_XY)) * DS_TILE;
+    if (i0 >= g.gs || j0 >= g.gs || k0 >= g.gsz) return;  // whole wavefront: tile outside the grid
+    const K3DsBox b = k3ds_box(g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, i0, min(i0 + DS_TILE, g.gs) - 1, j0,
+                               min(j0 + DS_TILE, g.gs) - 1, k0, min(k0 + DS_TILE, g.gsz) - 1, w2);
+    const int i = i0 + (lane & 3), j = j0 + ((lane >> 2) & 3), k = k0 + (lane >> 4);
+    const bool valid = i < g.gs && j < g.gs && k < g.gsz;
+    const int c = (k * g.gs + j) * g.gs + i;
     float x = 0.0f, y = 0.0f, z = 0.0f;
-    if (c < V) k3_voxel_center(c, g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, &x, &y, &z);
+    if (valid) k3_voxel_center(c, g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, &x, &y, &z);
     const float c2 = x * x + y * y + z * z;
     int nb = 0;
-    bool all_farther = true;
+    bool all_farther = true, done = !valid;
     float max_occ = 0.0f;
-    for (int base = 0; base < n; base += DS_TPB) {
-        if (__syncthreads_and(c >= V || nb >= K3DS_MAX_NEIGHBORS)) break;
-        const int q = base + t;
-        if (q < n) {
-            const float a = xs[q], b = ys[q], d = zs[q];
-            px[t] = a;
-            py[t] = b;
-            pz[t] = d;
-            pm2[t] = a * a + b * b + d * d;
+    const uint32_t end = off[st + 1];
+    for (uint32_t base = off[st]; base < end; base += 64) {
+        if (__ballot(!done) == 0) break;
+        const uint32_t e = base + (uint32_t)lane;
+        float mx = 0.0f, my = 0.0f, mz = 0.0f, m2 = 0.0f;
+        bool pass = false;
+        if (e < end) {
+            const int q = list[e];
+            mx = xs[q];
+            my = ys[q];
+            mz = zs[q];
+            m2 = k3ds_m2(mx, my, mz);
+            pass = k3ds_may_pass(b, mx, my, mz, m2);
         }
-        __syncthreads();
-        const int cnt = min(DS_TPB, n - base);
-        if (c < V && nb < K3DS_MAX_NEIGHBORS) {
-            for (int k = 0; k < cnt; ++k) {
-                const float mx = px[k], my = py[k], mz = pz[k], m2 = pm2[k];
-                if (!k3ds_ray_near(x, y, z, c2, mx, my, mz, m2, w2)) continue;
-                if (all_farther && c2 > m2) all_farther = false;
-                const float o = k3ds_pocc(x, y, z, mx, my, mz, delta);
+        for (uint64_t bits = __ballot(pass); bits; bits &= bits - 1) {
+            const int src = __builtin_ctzll(bits);
+            const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mx), src));
+            const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my), src));
+            const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mz), src));
+            const float pm2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m2), src));
+            if (!done && k3ds_ray_near(x, y, z, c2, px, py, pz, pm2, w2)) {
+                if (all_farther && c2 > pm2) all_farther = false;
+                const float o = k3ds_pocc(x, y, z, px, py, pz, delta);
                 if (o > max_occ) max_occ = o;
-                if (++nb == K3DS_MAX_NEIGHBORS) break;
+                if (++nb == K3DS_MAX_NEIGHBORS) done = true;
             }
         }
     }
-    if (c < V) k3ds_meas_masses(nb, all_farther, max_occ, x, y, z, max_range, &meas_occ[c], &meas_free[c]);
+    if (valid) k3ds_meas_masses(nb, all_farther, max_occ, x, y, z, max_range, &meas_occ[c], &meas_free[c]);
 }
 
-hipError_t launch_ds_meas_grid(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
-                               const float* zs, float resolution, float meas_occ_stddev, float max_range, int V,
+hipError_t launch_ds_meas_grid(hipStream_t st, const uint32_t* abort_flag, const FrameConsts* fc, int gs, int gsz,
+                               const float* xs, const float* ys, const float* zs, float resolution,
+                               float meas_occ_stddev, float max_range, const DsLists& L, DevScalars* sc,
                                float* meas_occ, float* meas_free) {
     const float wr = resolution * 0.866f;  // window_radius, src/dsphdmib/dom.cu:21-22
-    hipLaunchKernelGGL(k_ds_meas_grid, dim3(blocks_for(V, DS_TPB)), dim3(DS_TPB), 0, st, fc, xs, ys, zs, wr * wr,
-                       meas_occ_stddev, max_range, V, meas_occ, meas_free);
+    const float w2 = wr * wr;
+    const int nst = ds_supertiles(gs, gsz);
+    hipLaunchKernelGGL(k_ds_super_lists, dim3(nst), dim3(DS_TPB), 0, st, fc, xs, ys, zs, w2, nst, nullptr, L.counts,
+                       nullptr, 0ll, sc);
+    hipError_t e = launch_scan_u32(st, L.counts, L.off, L.scratch, nst, nullptr, 0, &sc->ds_total);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ds_super_lists, dim3(nst), dim3(DS_TPB), 0, st, fc, xs, ys, zs, w2, nst, L.off, L.counts,
+                       L.list, L.cap, sc);
+    hipLaunchKernelGGL(k_ds_meas_tiles, dim3(nst * DS_BLOCKS_PER_ST), dim3(DS_TPB), 0, st, abort_flag, fc, xs, ys,
+                       zs, L.off, L.list, w2, meas_occ_stddev, max_range, meas_occ, meas_free);
     return hipGetLastError();
 }
 

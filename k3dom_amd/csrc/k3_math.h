@@ -430,6 +430,54 @@ K3_HD void k3ds_meas_masses(int n, bool all_farther, float max_occ, float x, flo
     *m_free = fmaxf(K3DS_EPS, fminf(1.0f - K3DS_EPS, mf));
 }
 
+// |m|^2 of a point, the reference's expression (measurement_grid.cu:87)
+K3_HD float k3ds_m2(float x, float y, float z) { return x * x + y * y + z * z; }
+
+// ---- culling bound of the GPU measurement grid (no counterpart in the
+// reference, whose kernel tests every point against every voxel)
+//
+// Voxel-centre coordinate along one axis, the expression of k3_voxel_center
+// (non-decreasing in i, so the centres of voxels i0..i1 lie in [c(i0), c(i1)]).
+K3_HD float k3ds_axis(int i, int n, float res, float s) { return (((float)i + 0.5f) - (float)n / 2.0f) * res - s; }
+
+// Box of the voxel centres [i0, i1] x [j0, j1] x [k0, k1] (sensor at the
+// origin) and the squared distance from its centre below which a point's line
+// may pass within the window (w2 = window^2) of one of them.  A voxel centre
+// c lies within R of the box centre C.  When 1e-20 <= m2 <= 1e30 and
+// |c| <= 1e4, the float test c2 - inner^2/m2 < w2 is off the true squared
+// distance d(c)^2 by at most about 15 ulps of c2, so it accepts only if
+// d(c)^2 < w2 + 4e-6 c2; then d(C) <= d(c) + R.  The extra terms cover the
+// rounding of C, R and of the cross product in k3ds_may_pass.  Beyond
+// |c| = 1e4 every point is kept.
+struct K3DsBox {
+    float cx, cy, cz, thr;
+};
+K3_HD K3DsBox k3ds_box(int gs, int gsz, float res, float sx, float sy, float sz, int i0, int i1, int j0, int j1,
+                       int k0, int k1, float w2) {
+    const float lx = k3ds_axis(i0, gs, res, sx), hx = k3ds_axis(i1, gs, res, sx);
+    const float ly = k3ds_axis(j0, gs, res, sy), hy = k3ds_axis(j1, gs, res, sy);
+    const float lz = k3ds_axis(k0, gsz, res, sz), hz = k3ds_axis(k1, gsz, res, sz);
+    K3DsBox b;
+    b.cx = 0.5f * (lx + hx);
+    b.cy = 0.5f * (ly + hy);
+    b.cz = 0.5f * (lz + hz);
+    const float ex = 0.5f * (hx - lx), ey = 0.5f * (hy - ly), ez = 0.5f * (hz - lz);
+    const float R = sqrtf(ex * ex + ey * ey + ez * ez);
+    const float D = sqrtf(b.cx * b.cx + b.cy * b.cy + b.cz * b.cz) + R;
+    const float T = sqrtf(w2 + 4e-6f * D * D) + R + 2e-6f * D + 1e-6f;
+    b.thr = D <= 1e4f ? T * T * 1.00001f : INFINITY;
+    return b;
+}
+
+// Whether the line of point m (|m|^2 = m2) may pass near a voxel of box b;
+// true for every point the bound does not cover (m2 outside [1e-20, 1e30], NaN).
+K3_HD bool k3ds_may_pass(const K3DsBox& b, float mx, float my, float mz, float m2) {
+    if (!(m2 >= 1e-20f && m2 <= 1e30f)) return true;
+    const float qx = b.cy * mz - b.cz * my, qy = b.cz * mx - b.cx * mz, qz = b.cx * my - b.cy * mx;
+    const float d2 = (qx * qx + qy * qy + qz * qz) / m2;
+    return !(d2 > b.thr);
+}
+
 // predict_free_mass, mass_update.cu:12-15
 K3_HD float k3ds_predict_free(float free_mass, float m_occ_pred) {
     return fminf(K3DS_ALPHA * free_mass, 1.0f - m_occ_pred);

@@ -179,6 +179,84 @@ def test_renormalisation_occurs_in_small_scene(oracle_lib):
     assert seen > 0
 
 
+# ------------------------------------------------- culling bound of the GPU path
+
+_BOUND_CHECK = r"""
+#include <cstdio>
+#include <random>
+#include "k3_math.h"
+// For random grids, boxes of voxels and points whose line passes near a
+// voxel of the box (perpendicular distance 0.9..1.1 window or within 1e-3 of
+// it, at ranges from 1e-9 to 1e17 times the voxel's distance): whenever the reference's float
+// test accepts the point for that voxel, the box bound must keep it.
+int main() {
+    std::mt19937_64 rng(12345);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    long long accepted = 0, kept_by_bound_only = 0;
+    for (int it = 0; it < 400000; ++it) {
+        const int gs = 8 + (int)(U(rng) * 600), gsz = 4 + (int)(U(rng) * 120);
+        const float res = (float)(0.02 + U(rng) * 0.5);
+        const float sx = (float)((U(rng) - 0.5) * gs * res), sy = (float)((U(rng) - 0.5) * gs * res),
+                    sz = (float)((U(rng) - 0.5) * gsz * res);
+        // supertiles, tiles, and single voxels (R = 0: no geometric slack)
+        const double kind = U(rng);
+        const int bx = kind < 0.3 ? 16 : (kind < 0.6 ? 4 : 1), bz = kind < 0.3 ? 8 : (kind < 0.6 ? 4 : 1);
+        const int i0 = (int)(U(rng) * gs), j0 = (int)(U(rng) * gs), k0 = (int)(U(rng) * gsz);
+        const int i1 = std::min(i0 + bx, gs) - 1, j1 = std::min(j0 + bx, gs) - 1, k1 = std::min(k0 + bz, gsz) - 1;
+        const float wr = res * 0.866f, w2 = wr * wr;
+        const K3DsBox b = k3ds_box(gs, gsz, res, sx, sy, sz, i0, i1, j0, j1, k0, k1, w2);
+        const int i = i0 + (int)(U(rng) * (i1 - i0 + 1)), j = j0 + (int)(U(rng) * (j1 - j0 + 1)),
+                  k = k0 + (int)(U(rng) * (k1 - k0 + 1));
+        float x, y, z;
+        k3_voxel_center((k * gs + j) * gs + i, gs, gsz, res, sx, sy, sz, &x, &y, &z);
+        // a direction at perpendicular distance ~window from c, scaled
+        double ux = U(rng) - 0.5, uy = U(rng) - 0.5, uz = U(rng) - 0.5;
+        const double cn = std::sqrt((double)x * x + (double)y * y + (double)z * z);
+        if (cn < 1e-6) continue;
+        const double dot = (ux * x + uy * y + uz * z) / (cn * cn);
+        ux -= dot * x; uy -= dot * y; uz -= dot * z;
+        const double un = std::sqrt(ux * ux + uy * uy + uz * uz);
+        if (un < 1e-12) continue;
+        const double off = wr * (U(rng) < 0.5 ? 0.9 + 0.2 * U(rng) : 1.0 + 2e-3 * (U(rng) - 0.5)) / un;
+        const double scale = std::pow(10.0, -9.0 + 26.0 * U(rng));
+        const float mx = (float)((x + ux * off) * scale), my = (float)((y + uy * off) * scale),
+                    mz = (float)((z + uz * off) * scale);
+        const float c2 = x * x + y * y + z * z, m2 = k3ds_m2(mx, my, mz);
+        if (!k3ds_ray_near(x, y, z, c2, mx, my, mz, m2, w2)) continue;
+        ++accepted;
+        if (!k3ds_may_pass(b, mx, my, mz, m2)) {
+            std::printf("MISS gs=%d gsz=%d res=%g s=(%g,%g,%g) box=[%d,%d]x[%d,%d]x[%d,%d] c=(%g,%g,%g) m=(%g,%g,%g)\n",
+                        gs, gsz, res, sx, sy, sz, i0, i1, j0, j1, k0, k1, x, y, z, mx, my, mz);
+            return 1;
+        }
+        if (!(m2 >= 1e-20f && m2 <= 1e30f)) ++kept_by_bound_only;
+    }
+    std::printf("%lld %lld\n", accepted, kept_by_bound_only);
+    return 0;
+}
+"""
+
+
+def test_culling_bound_keeps_every_accepted_point(tmp_path):
+    """The GPU measurement grid skips points by a per-box bound
+    (k3ds_box / k3ds_may_pass); a point that the exact float test accepts for
+    a voxel of the box must never be skipped, also at the test's rounding
+    edge (points within 1e-3 of the window, single-voxel boxes) and at extreme
+    ranges."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "bound.cpp"
+    src.write_text(_BOUND_CHECK)
+    exe = tmp_path / "bound"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-march=x86-64-v3",
+                    "-I", os.path.join(root, "k3dom_amd", "csrc"), str(src), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    accepted, _ = (int(v) for v in r.stdout.split())
+    assert accepted > 50000
+
+
 # ------------------------------------------------------------------ GPU parity
 
 @pytest.mark.gpu
@@ -250,3 +328,54 @@ def test_ds_empty_frames_and_graph_replay(gpu, monkeypatch):
             m.updatePose(*pose)
             m.updateGrid(t, xs, ys, zs)
         compare_state(gd, od, normal_frame=normal, what=f"DS graph frame {f}:")
+
+
+def _window_edge_points(info, gs, gsz, res, rng, n):
+    """Points whose line passes at 0.99..1.01 window from random voxel
+    centres (sensor-relative, float32 like the voxel-centre expression), at
+    several ranges."""
+    sx, sy, sz = (np.float32(info.sensor_pos[k] - info.center_pos[k]) for k in range(3))
+    i, j, k = rng.integers(0, gs, n), rng.integers(0, gs, n), rng.integers(0, gsz, n)
+    r = np.float32(res)
+    c = np.stack([(((i + np.float32(0.5)).astype(np.float32) - np.float32(gs / 2)) * r - sx),
+                  (((j + np.float32(0.5)).astype(np.float32) - np.float32(gs / 2)) * r - sy),
+                  (((k + np.float32(0.5)).astype(np.float32) - np.float32(gsz / 2)) * r - sz)], 1).astype(np.float64)
+    u = rng.normal(size=(n, 3))
+    u -= (np.sum(u * c, 1) / np.maximum(np.sum(c * c, 1), 1e-12))[:, None] * c
+    u /= np.maximum(np.linalg.norm(u, axis=1), 1e-12)[:, None]
+    wr = res * 0.866 * rng.uniform(0.99, 1.01, n)
+    scale = rng.choice([0.5, 1.0, 1.5, 3.0], n)
+    return ((c + u * wr[:, None]) * scale[:, None]).astype(np.float32)
+
+
+@pytest.mark.gpu
+def test_ds_culled_grid_edge_cases(gpu, monkeypatch):
+    """The culled GPU measurement grid against the oracle's test of every
+    point for every voxel: a 20x20x6 grid (partial supertiles and tiles), a
+    sensor offset, points at the window's edge, points at extreme ranges,
+    NaN / inf / zero points, and candidate-list regrowth from a tiny capacity."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, small_workload
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_state
+    monkeypatch.setenv("K3DOM_TEST_TINY_CAPACITY", "1")
+    w = small_workload(edge=20, edge_z=6, particles=2048, newborn=200, frames=4, seed=17, method="dsphdmib",
+                       sensor_off_x=0.13, sensor_off_z=-0.07)
+    sc = Scene(w.scene)
+    gd, od = DOM(w.params, method="dsphdmib"), OracleDOM(w.params, method="dsphdmib")
+    rng = np.random.default_rng(5)
+    odd = np.array([[1e-12, 2e-12, 0.0], [3e16, -1e16, 2e15], [1e20, 1e20, 1e20], [np.nan, 1.0, 1.0],
+                    [np.inf, 0.0, 0.0], [0.0, 0.0, 0.0], [-1e-25, 1e-25, 0.0], [1e30, 0.0, -1e30]], np.float32)
+    for f in range(w.frames):
+        t, pose, xs, ys, zs = sc.frame(f)
+        for m in (gd, od):
+            m.updatePose(*pose)
+        edge = _window_edge_points(gd.info(), 20, 6, 0.2, rng, 3000)
+        pts = np.concatenate([np.stack([xs, ys, zs], 1), edge, odd])[rng.permutation(len(xs) + 3000 + len(odd))]
+        xs, ys, zs = (np.ascontiguousarray(pts[:, k]) for k in range(3))
+        print(f"  DS edge-case frame {f}: {len(xs)} points", flush=True)
+        normal = gd.info().updated_time > 0
+        for m in (gd, od):
+            m.updateGrid(t, xs, ys, zs)
+        compare_state(gd, od, normal_frame=normal, what=f"DS edge-case frame {f}:")
+    assert gd.stats()["capacity_regrowths"] >= 1
