@@ -13,8 +13,9 @@ kernel-inference gather, whose algorithmic work is its contributing
 against its time; `adjoint` times the differentiated window (forward +
 backward ms/frame against the plain forward, config-1 scene, ls 0.3, 8-frame
 windows); `batch` times 32 independent uneven map tiles advanced together;
-`cpu_baseline` times the serial restatement on a bounded sample of the same
-workload on the host.
+`dsphdmib` times the DS-PHD/MIB method on the same frames; `cpu_baseline`
+times the serial restatement on a bounded sample of the same workload on the
+host.
 """
 import argparse
 import json
@@ -172,6 +173,49 @@ def batch_leg(torch, warmup=3, steps=8):
     return res
 
 
+def dsphdmib_leg(torch, w, frames, dpts, warmup=3, steps=20, breakdown=5):
+    """The DS-PHD/MIB method (src/dsphdmib/) on the same scene, grid,
+    particle counts and device-resident point clouds: frames/s, the per-stage
+    breakdown of a few further frames (the measurement grid is the
+    kernel-inference stage's slot) and its candidate-list entries."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import dsphdmib_params
+    p0 = w.params
+    gs = int(np.float32(p0.size) / np.float32(p0.resolution))
+    gsz = int(np.float32(p0.size_z) / np.float32(p0.resolution))
+    p = dsphdmib_params(p0.resolution, gs, gsz, p0.particle_count, p0.new_born_particle_count)
+    d = DOM(p, method="dsphdmib")
+
+    def step(f):
+        t, pose, xs, _, _ = frames[f]
+        dx, dy, dz = dpts[f]
+        d.updatePose(*pose)
+        d.updateGridDevice(t, dx.data_ptr(), dy.data_ptr(), dz.data_ptr(), len(xs))
+
+    for f in range(warmup):
+        step(f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, warmup + steps):
+        step(f)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    d.set_timing(True)
+    st = np.zeros(len(DOM.STAGES))
+    for f in range(warmup + steps, warmup + steps + breakdown):
+        step(f)
+        st += d.stage_times()
+    st /= breakdown
+    res = {"workload": f"DS-PHD/MIB method, {w.name}", "warmup": warmup, "steps": steps,
+           "frames_per_s": round(steps / el, 3), "ms_per_frame": round(el / steps * 1e3, 4),
+           "meas_grid_ms": round(float(st[DOM.STAGES.index("bgki_kernel")]), 4),
+           "frame_gpu_ms": round(float(st[DOM.STAGES.index("frame")]), 4),
+           "meas_grid_candidates": int(d.stats()["level_entries"][0]),
+           "stage_ms": {n: round(float(v), 4) for n, v in zip(DOM.STAGES, st)}}
+    d.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +226,7 @@ def main():
     ap.add_argument("--no-roofline", action="store_true", help="skip the pair-count pass")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiated-window leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-tiles leg")
+    ap.add_argument("--no-dsphdmib", action="store_true", help="skip the DS-PHD/MIB method leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -306,6 +351,11 @@ def main():
         bat = batch_leg(torch)
         log(f"batched tiles: {bat}")
 
+    ds = None
+    if not args.no_dsphdmib and len(frames) >= 28:
+        ds = dsphdmib_leg(torch, w, frames, dpts)
+        log(f"DS-PHD/MIB: {ds}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(w, frames)
@@ -333,6 +383,7 @@ def main():
         "frame_hbm": frame_hbm,
         "adjoint": adj,
         "batch": bat,
+        "dsphdmib": ds,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

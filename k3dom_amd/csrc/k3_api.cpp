@@ -86,14 +86,20 @@ void setup_levels(k3dom_handle* h) {
 
 k3dom_status ensure_caps(k3dom_handle* h, const long long want[3], long long want_heavy);
 
-// DS-PHD/MIB measurement-grid candidate list of at least want entries.
+// DS-PHD/MIB measurement-grid candidate list of at least want entries, with
+// the per-segment buffers its segment count implies.
 k3dom_status ensure_ds_list(k3dom_handle* h, long long want) {
     if (want <= h->ds.cap && h->ds.list) return K3DOM_OK;
-    dfree(h, h->ds.list);
+    for (void* p : {(void*)h->ds.list, (void*)h->ds.segcnt, (void*)h->ds.part}) dfree(h, p);
     h->ds.list = nullptr;
+    h->ds.segcnt = nullptr;
+    h->ds.part = nullptr;
     h->ds.cap = 0;
     h->caps_version++;
+    const size_t slots = (size_t)ds_segments_max(h->gs, h->gsz, want, h->ds.seg_len) * 32 * 64;
     HIP_TRY(h, dalloc(h, &h->ds.list, (size_t)want));
+    HIP_TRY(h, dalloc(h, &h->ds.segcnt, slots));
+    HIP_TRY(h, dalloc(h, &h->ds.part, slots));
     h->ds.cap = want;
     return K3DOM_OK;
 }
@@ -782,6 +788,9 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
         CREATE_TRY(dalloc(h, &h->ds.counts, nst));
         CREATE_TRY(dalloc(h, &h->ds.off, nst + 1));
         CREATE_TRY(dalloc_zero(h, &h->ds.scratch, (size_t)scan_scratch_words((long long)nst)));
+        CREATE_TRY(dalloc(h, &h->ds.nseg, nst));
+        CREATE_TRY(dalloc(h, &h->ds.seg_off, nst + 1));
+        CREATE_TRY(dalloc_zero(h, &h->ds.seg_scratch, (size_t)scan_scratch_words((long long)nst)));
     }
     CREATE_TRY(dalloc(h, &h->del_occ, V));
     CREATE_TRY(dalloc(h, &h->del_free, V));
@@ -831,6 +840,13 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
     {
         const char* tiny = std::getenv("K3DOM_TEST_TINY_CAPACITY");
         h->tiny_caps = tiny && tiny[0] == '1';
+        // measurement-grid list segments: the power of two at or below sqrt(V)
+        // within [256, 1024] (short segments shorten the longest wavefront,
+        // which bounds small grids; long ones cost less per entry, which bounds
+        // large grids); 64 under the test hook (many segments per list)
+        int seg = 256;
+        while (seg < 1024 && (long long)(2 * seg) * (2 * seg) <= (long long)h->V) seg *= 2;
+        h->ds.seg_len = h->tiny_caps ? 64 : seg;
     }
     CREATE_TRY(hipHostMalloc((void**)&h->h_sc, sizeof(DevScalars), hipHostMallocDefault));
     CREATE_TRY(hipHostMalloc((void**)&h->h_fc, sizeof(FrameConsts), hipHostMallocDefault));

@@ -22,7 +22,8 @@ struct DevScalars {
     uint32_t level_total[3];  // candidate-list entries per level
     uint32_t level_items[3];  // work items per level
     uint32_t ds_total;        // DS-PHD/MIB measurement-grid candidate entries
-    uint32_t pad[2];
+    uint32_t ds_segments;     // and their segments
+    uint32_t pad[1];
 };
 
 // Voxel state arrays (one buffer set).
@@ -209,16 +210,23 @@ hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelA
 // born_parts as launch_mass_update) that also divides the sorted weights of
 // voxels whose sum exceeds 1 (norm: V floats of scratch); born masses of the
 // first frame from the measurement grid.
-// Candidate lists of the culled measurement grid: counts and off hold
-// ds_supertiles(gs, gsz) (+1) words, scratch scan_scratch_words of that many
-// (zeroed once), list cap point indices.  A frame needing more than cap raises
-// the overflow flag (mask bit 5, total in ds_total) and writes no masses.
+// Candidate lists of the culled measurement grid.  counts, nseg: one word per
+// supertile (ds_supertiles(gs, gsz)); off, seg_off: that many + 1; scratch,
+// seg_scratch: scan_scratch_words of that many (zeroed once); list: cap point
+// indices, processed in segments of seg_len; segcnt:
+// 64 * 32 * ds_segments_max(gs, gsz, cap, seg_len) bytes; part: as many uint2.
+// A frame needing more than cap raises the overflow flag (mask bit 5, total
+// in ds_total) and writes no masses.
 struct DsLists {
-    uint32_t *counts, *off, *scratch;
+    uint32_t *counts, *off, *scratch, *nseg, *seg_off, *seg_scratch;
     int32_t* list;
+    uint8_t* segcnt;
+    uint2* part;
     long long cap;
+    int seg_len;
 };
 int ds_supertiles(int gs, int gsz);
+long long ds_segments_max(int gs, int gsz, long long cap, int seg_len);
 hipError_t launch_ds_meas_grid(hipStream_t st, const uint32_t* abort_flag, const FrameConsts* fc, int gs, int gsz,
                                const float* xs, const float* ys, const float* zs, float resolution,
                                float meas_occ_stddev, float max_range, const DsLists& L, DevScalars* sc,

@@ -691,16 +691,22 @@ __device__ __forceinline__ DsSuper ds_super(int st, int gs) {
     return s;
 }
 
+long long ds_segments_max(int gs, int gsz, long long cap, int seg_len) {
+    return ds_supertiles(gs, gsz) + cap / seg_len + 1;
+}
+
 // Candidate points of each supertile, one block per supertile, in point
 // order: with off == nullptr the count to counts[st], else the indices to
 // list[off[st]...] (off: exclusive scan of the counts, off[nst] the total;
-// above cap the overflow flag is raised with the total instead).
+// above cap the overflow flag is raised with the total instead) and the
+// number of seg_len-entry segments of the list (at least 1) to nseg[st].
 __global__ __launch_bounds__(DS_TPB) void k_ds_super_lists(const FrameConsts* __restrict__ fc,
                                                            const float* __restrict__ xs,
                                                            const float* __restrict__ ys,
                                                            const float* __restrict__ zs, float w2, int nst,
                                                            const uint32_t* __restrict__ off,
                                                            uint32_t* __restrict__ counts, int32_t* __restrict__ list,
+                                                           uint32_t* __restrict__ nseg, int seg_len,
                                                            long long cap, DevScalars* __restrict__ sc) {
     __shared__ uint32_t wsum[DS_TPB / 64];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -746,72 +752,261 @@ __global__ __launch_bounds__(DS_TPB) void k_ds_super_lists(const FrameConsts* __
         running += chunk;
         __syncthreads();
     }
-    if (!off && t == 0) counts[st] = running;
+    if (t == 0) {
+        if (off)
+            nseg[st] = running > 0 ? (running + (uint32_t)seg_len - 1) / (uint32_t)seg_len : 1u;
+        else
+            counts[st] = running;
+    }
 }
 
-// Exact measurement masses of every voxel from its supertile's list, one
-// wavefront per 4x4x4 tile (lane = voxel): chunks of 64 listed points are
-// loaded one per lane, filtered by the tile's bound, and the survivors are
-// broadcast in order to every lane for the exact test.
+// One voxel's state while it collects its first K3DS_MAX_NEIGHBORS rays.
+struct DsVoxel {
+    float x, y, z, c2, max_occ;
+    int nb;
+    bool all_farther, done;
+};
+
+// Applies point p (x, y, z, |m|^2) to the voxel when near is its exact test
+// (measurement_grid.cu:86-120, in point order); COUNT: only counts.
+template <bool COUNT>
+__device__ __forceinline__ void ds_take(DsVoxel& v, bool near, const float4& p, float delta) {
+    if (v.done || !near) return;
+    if (!COUNT) {
+        if (v.all_farther && v.c2 > p.w) v.all_farther = false;
+        const float o = k3ds_pocc(v.x, v.y, v.z, p.x, p.y, p.z, delta);
+        if (o > v.max_occ) v.max_occ = o;
+    }
+    if (++v.nb == K3DS_MAX_NEIGHBORS) v.done = true;
+}
+
+// A wavefront's 4x4x4 tile (lane = voxel): its bound, the bounds of its eight
+// 2x2x2 sub-boxes (q = x half | y half << 1 | z half << 2, lane bits 1, 3, 5)
+// and the lane's voxel.  False when the tile lies outside the grid.
+struct DsTile {
+    K3DsBox tb, sb[8];
+    uint64_t sub_lanes[8];
+    int c;
+    bool valid;
+};
+__device__ __forceinline__ bool ds_tile(const GridGeom& g, int st, int wl, int lane, float w2, DsTile& T,
+                                        DsVoxel& v) {
+    const DsSuper s = ds_super(st, g.gs);
+    const int i0 = s.i0 + (wl % DS_TILES_XY) * DS_TILE;
+    const int j0 = s.j0 + ((wl / DS_TILES_XY) % DS_TILES_XY) * DS_TILE;
+    const int k0 = s.k0 + (wl / (DS_TILES_XY * DS_TILES_XY)) * DS_TILE;
+    if (i0 >= g.gs || j0 >= g.gs || k0 >= g.gsz) return false;
+    T.tb = k3ds_box(g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, i0, min(i0 + DS_TILE, g.gs) - 1, j0,
+                    min(j0 + DS_TILE, g.gs) - 1, k0, min(k0 + DS_TILE, g.gsz) - 1, w2);
+    const int my_sub = ((lane >> 1) & 1) | ((lane >> 2) & 2) | ((lane >> 3) & 4);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int a = i0 + 2 * (q & 1), b = j0 + ((q >> 1) & 1) * 2, c = k0 + ((q >> 2) & 1) * 2;
+        T.sb[q] = k3ds_box(g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, a, min(a + 2, g.gs) - 1, b, min(b + 2, g.gs) - 1,
+                           c, min(c + 2, g.gsz) - 1, w2);
+        T.sub_lanes[q] = __ballot(my_sub == q);
+    }
+    const int i = i0 + (lane & 3), j = j0 + ((lane >> 2) & 3), k = k0 + (lane >> 4);
+    T.valid = i < g.gs && j < g.gs && k < g.gsz;
+    T.c = (k * g.gs + j) * g.gs + i;
+    v.x = v.y = v.z = 0.0f;
+    if (T.valid) k3_voxel_center(T.c, g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, &v.x, &v.y, &v.z);
+    v.c2 = v.x * v.x + v.y * v.y + v.z * v.z;
+    v.max_occ = 0.0f;
+    v.nb = 0;
+    v.all_farther = true;
+    v.done = !T.valid;
+    return true;
+}
+
+// The tile's voxels take the listed points list[b..e) in order.  Chunks of 64
+// are loaded one per lane and filtered by the tile's bound, then by the bounds
+// of the sub-boxes that still hold a voxel short of its rays; the survivors
+// are staged in LDS (sw: 64 float4, sr: 64 reciprocals of |m|^2, of this
+// wavefront) in list order and every lane runs the exact test on them, four
+// at a time, applying hits in order.
+template <bool COUNT>
+__device__ __forceinline__ void ds_run(DsVoxel& v, const DsTile& T, float4* sw, float* sr,
+                                       const int32_t* __restrict__ list,
+                                       uint32_t b, uint32_t e, const float* __restrict__ xs,
+                                       const float* __restrict__ ys, const float* __restrict__ zs, float w2,
+                                       float delta, int lane) {
+    for (uint32_t base = b; base < e; base += 64) {
+        const uint64_t open = __ballot(!v.done);
+        if (open == 0) break;
+        uint32_t active = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) active |= (open & T.sub_lanes[q]) ? (1u << q) : 0u;
+        const uint32_t at = base + (uint32_t)lane;
+        float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool pass = false;
+        if (at < e) {
+            const int qi = list[at];
+            p.x = xs[qi];
+            p.y = ys[qi];
+            p.z = zs[qi];
+            p.w = k3ds_m2(p.x, p.y, p.z);
+            if (k3ds_may_pass(T.tb, p.x, p.y, p.z, p.w)) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if ((active >> q) & 1u) pass = pass || k3ds_may_pass(T.sb[q], p.x, p.y, p.z, p.w);
+            }
+        }
+        const uint64_t bits = __ballot(pass);
+        if (bits == 0) continue;
+        const int nc = __popcll(bits);
+        if (pass) {
+            const int at_s = __popcll(bits & ((1ull << lane) - 1ull));
+            sw[at_s] = p;
+            sr[at_s] = __builtin_amdgcn_rcpf(p.w);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int t = 0; t < nc; t += 4) {
+            const int t1 = min(t + 1, nc - 1), t2 = min(t + 2, nc - 1), t3 = min(t + 3, nc - 1);
+            const float4 p0 = sw[t], p1 = sw[t1], p2 = sw[t2], p3 = sw[t3];
+            const float r0 = sr[t], r1 = sr[t1], r2 = sr[t2], r3 = sr[t3];
+            const bool n0 = k3ds_ray_near_rcp(v.x, v.y, v.z, v.c2, p0.x, p0.y, p0.z, p0.w, r0, w2);
+            const bool n1 = t + 1 < nc && k3ds_ray_near_rcp(v.x, v.y, v.z, v.c2, p1.x, p1.y, p1.z, p1.w, r1, w2);
+            const bool n2 = t + 2 < nc && k3ds_ray_near_rcp(v.x, v.y, v.z, v.c2, p2.x, p2.y, p2.z, p2.w, r2, w2);
+            const bool n3 = t + 3 < nc && k3ds_ray_near_rcp(v.x, v.y, v.z, v.c2, p3.x, p3.y, p3.z, p3.w, r3, w2);
+            ds_take<COUNT>(v, n0, p0, delta);
+            ds_take<COUNT>(v, n1, p1, delta);
+            ds_take<COUNT>(v, n2, p2, delta);
+            ds_take<COUNT>(v, n3, p3, delta);
+        }
+        // every lane has read the staged points before the next chunk overwrites them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// Work item of a segment pass: block -> (supertile, segment), wavefront ->
+// tile.  Segments are numbered supertile by supertile (seg_off: exclusive
+// scan of nseg, seg_off[nst] the total); false past the last segment.
+struct DsItem {
+    int st, seg, nseg, wl;
+    uint32_t gseg, seg0;
+};
+__device__ __forceinline__ bool ds_item(const uint32_t* __restrict__ seg_off, int nst, DsItem& it) {
+    it.gseg = blockIdx.x / DS_BLOCKS_PER_ST;
+    if (it.gseg >= seg_off[nst]) return false;
+    int lo = 0, hi = nst - 1;  // last st with seg_off[st] <= gseg (every supertile has a segment)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg_off[mid] <= it.gseg)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    it.st = lo;
+    it.seg0 = seg_off[lo];
+    it.seg = (int)(it.gseg - it.seg0);
+    it.nseg = (int)(seg_off[lo + 1] - it.seg0);
+    it.wl = (blockIdx.x % DS_BLOCKS_PER_ST) * (DS_TPB / 64) + (threadIdx.x >> 6);
+    return true;
+}
+
+// Pass A (lists of more than one segment): hits of each voxel in every
+// segment but the last, capped at K3DS_MAX_NEIGHBORS, to segcnt.
+__global__ __launch_bounds__(DS_TPB) void k_ds_seg_count(const uint32_t* __restrict__ abort_flag,
+                                                         const FrameConsts* __restrict__ fc,
+                                                         const float* __restrict__ xs, const float* __restrict__ ys,
+                                                         const float* __restrict__ zs,
+                                                         const uint32_t* __restrict__ off,
+                                                         const int32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ seg_off, int nst, int seg_len,
+                                                         float w2, uint8_t* __restrict__ segcnt) {
+    __shared__ float4 stage[DS_TPB / 64][64];
+    __shared__ float stage_r[DS_TPB / 64][64];
+    if (*abort_flag) return;
+    DsItem it;
+    if (!ds_item(seg_off, nst, it) || it.seg == it.nseg - 1) return;
+    const int lane = threadIdx.x & 63;
+    const GridGeom g = fc->g;
+    DsTile T;
+    DsVoxel v;
+    if (!ds_tile(g, it.st, it.wl, lane, w2, T, v)) return;
+    const uint32_t b = off[it.st] + (uint32_t)it.seg * (uint32_t)seg_len;
+    const uint32_t e = min(b + (uint32_t)seg_len, off[it.st + 1]);
+    ds_run<true>(v, T, stage[threadIdx.x >> 6], stage_r[threadIdx.x >> 6], list, b, e, xs, ys, zs, w2, 0.0f, lane);
+    segcnt[((size_t)it.gseg * DS_WAVES_PER_ST + it.wl) * 64 + lane] = (uint8_t)v.nb;
+}
+
+// Pass B: every segment, each voxel starting from its hits in the earlier
+// segments (so it takes exactly its first K3DS_MAX_NEIGHBORS in point order).
+// A one-segment list writes the masses; otherwise (nb | all_farther << 8,
+// max pOcc) go to part for pass C.
 __global__ __launch_bounds__(DS_TPB) void k_ds_meas_tiles(const uint32_t* __restrict__ abort_flag,
                                                           const FrameConsts* __restrict__ fc,
                                                           const float* __restrict__ xs,
                                                           const float* __restrict__ ys,
                                                           const float* __restrict__ zs,
                                                           const uint32_t* __restrict__ off,
-                                                          const int32_t* __restrict__ list, float w2, float delta,
-                                                          float max_range, float* __restrict__ meas_occ,
+                                                          const int32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ seg_off, int nst, int seg_len,
+                                                          const uint8_t* __restrict__ segcnt, float w2, float delta,
+                                                          float max_range, uint2* __restrict__ part,
+                                                          float* __restrict__ meas_occ,
                                                           float* __restrict__ meas_free) {
+    __shared__ float4 stage[DS_TPB / 64][64];
+    __shared__ float stage_r[DS_TPB / 64][64];
     if (*abort_flag) return;
+    DsItem it;
+    if (!ds_item(seg_off, nst, it)) return;
     const int lane = threadIdx.x & 63;
     const GridGeom g = fc->g;
-    const int st = blockIdx.x / DS_BLOCKS_PER_ST;
-    const int wl = (blockIdx.x % DS_BLOCKS_PER_ST) * (DS_TPB / 64) + (threadIdx.x >> 6);
-    const DsSuper s = ds_super(st, g.gs);
-    const int i0 = s.i0 + (wl % DS_TILES_XY) * DS_TILE;
-    const int j0 = s.j0 + ((wl / DS_TILES_XY) % DS_TILES_XY) * DS_TILE;
-    const int k0 = s.k0 + (wl / (DS_TILES_XY * DS_TILES_XY)) * DS_TILE;
-    if (i0 >= g.gs || j0 >= g.gs || k0 >= g.gsz) return;  // whole wavefront: tile outside the grid
-    const K3DsBox b = k3ds_box(g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, i0, min(i0 + DS_TILE, g.gs) - 1, j0,
-                               min(j0 + DS_TILE, g.gs) - 1, k0, min(k0 + DS_TILE, g.gsz) - 1, w2);
-    const int i = i0 + (lane & 3), j = j0 + ((lane >> 2) & 3), k = k0 + (lane >> 4);
-    const bool valid = i < g.gs && j < g.gs && k < g.gsz;
-    const int c = (k * g.gs + j) * g.gs + i;
-    float x = 0.0f, y = 0.0f, z = 0.0f;
-    if (valid) k3_voxel_center(c, g.gs, g.gsz, g.res, g.sx, g.sy, g.sz, &x, &y, &z);
-    const float c2 = x * x + y * y + z * z;
-    int nb = 0;
-    bool all_farther = true, done = !valid;
-    float max_occ = 0.0f;
-    const uint32_t end = off[st + 1];
-    for (uint32_t base = off[st]; base < end; base += 64) {
-        if (__ballot(!done) == 0) break;
-        const uint32_t e = base + (uint32_t)lane;
-        float mx = 0.0f, my = 0.0f, mz = 0.0f, m2 = 0.0f;
-        bool pass = false;
-        if (e < end) {
-            const int q = list[e];
-            mx = xs[q];
-            my = ys[q];
-            mz = zs[q];
-            m2 = k3ds_m2(mx, my, mz);
-            pass = k3ds_may_pass(b, mx, my, mz, m2);
-        }
-        for (uint64_t bits = __ballot(pass); bits; bits &= bits - 1) {
-            const int src = __builtin_ctzll(bits);
-            const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mx), src));
-            const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my), src));
-            const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mz), src));
-            const float pm2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m2), src));
-            if (!done && k3ds_ray_near(x, y, z, c2, px, py, pz, pm2, w2)) {
-                if (all_farther && c2 > pm2) all_farther = false;
-                const float o = k3ds_pocc(x, y, z, px, py, pz, delta);
-                if (o > max_occ) max_occ = o;
-                if (++nb == K3DS_MAX_NEIGHBORS) done = true;
-            }
-        }
+    DsTile T;
+    DsVoxel v;
+    if (!ds_tile(g, it.st, it.wl, lane, w2, T, v)) return;
+    int prior = 0;
+    for (int s = 0; s < it.seg; ++s) prior += segcnt[((size_t)(it.seg0 + s) * DS_WAVES_PER_ST + it.wl) * 64 + lane];
+    v.nb = min(prior, K3DS_MAX_NEIGHBORS);
+    v.done = v.done || v.nb == K3DS_MAX_NEIGHBORS;
+    const uint32_t b = off[it.st] + (uint32_t)it.seg * (uint32_t)seg_len;
+    const uint32_t e = min(b + (uint32_t)seg_len, off[it.st + 1]);
+    ds_run<false>(v, T, stage[threadIdx.x >> 6], stage_r[threadIdx.x >> 6], list, b, e, xs, ys, zs, w2, delta, lane);
+    if (it.nseg == 1) {
+        if (T.valid)
+            k3ds_meas_masses(v.nb, v.all_farther, v.max_occ, v.x, v.y, v.z, max_range, &meas_occ[T.c],
+                             &meas_free[T.c]);
+    } else {
+        part[((size_t)it.gseg * DS_WAVES_PER_ST + it.wl) * 64 + lane] =
+            make_uint2((uint32_t)v.nb | (v.all_farther ? 256u : 0u), __float_as_uint(v.max_occ));
     }
-    if (valid) k3ds_meas_masses(nb, all_farther, max_occ, x, y, z, max_range, &meas_occ[c], &meas_free[c]);
+}
+
+// Pass C: voxels of lists of more than one segment combine their segments:
+// the count after the last, all_farther over all, the largest pOcc.
+__global__ __launch_bounds__(DS_TPB) void k_ds_seg_combine(const uint32_t* __restrict__ abort_flag,
+                                                           const FrameConsts* __restrict__ fc,
+                                                           const uint32_t* __restrict__ seg_off, float w2,
+                                                           float max_range, const uint2* __restrict__ part,
+                                                           float* __restrict__ meas_occ,
+                                                           float* __restrict__ meas_free) {
+    if (*abort_flag) return;
+    const int st = blockIdx.x / DS_BLOCKS_PER_ST;
+    const uint32_t seg0 = seg_off[st];
+    const int nseg = (int)(seg_off[st + 1] - seg0);
+    if (nseg == 1) return;
+    const int lane = threadIdx.x & 63;
+    const int wl = (blockIdx.x % DS_BLOCKS_PER_ST) * (DS_TPB / 64) + (threadIdx.x >> 6);
+    const GridGeom g = fc->g;
+    DsTile T;
+    DsVoxel v;
+    if (!ds_tile(g, st, wl, lane, w2, T, v) || !T.valid) return;
+    int nb = 0;
+    bool all_farther = true;
+    float max_occ = 0.0f;
+    for (int s = 0; s < nseg; ++s) {
+        const uint2 r = part[((size_t)(seg0 + s) * DS_WAVES_PER_ST + wl) * 64 + lane];
+        nb = (int)(r.x & 255u);
+        all_farther = all_farther && (r.x & 256u);
+        max_occ = fmaxf(max_occ, __uint_as_float(r.y));
+    }
+    k3ds_meas_masses(nb, all_farther, max_occ, v.x, v.y, v.z, max_range, &meas_occ[T.c], &meas_free[T.c]);
 }
 
 hipError_t launch_ds_meas_grid(hipStream_t st, const uint32_t* abort_flag, const FrameConsts* fc, int gs, int gsz,
@@ -821,14 +1016,22 @@ hipError_t launch_ds_meas_grid(hipStream_t st, const uint32_t* abort_flag, const
     const float wr = resolution * 0.866f;  // window_radius, src/dsphdmib/dom.cu:21-22
     const float w2 = wr * wr;
     const int nst = ds_supertiles(gs, gsz);
+    const unsigned seg_blocks = (unsigned)(ds_segments_max(gs, gsz, L.cap, L.seg_len) * DS_BLOCKS_PER_ST);
     hipLaunchKernelGGL(k_ds_super_lists, dim3(nst), dim3(DS_TPB), 0, st, fc, xs, ys, zs, w2, nst, nullptr, L.counts,
-                       nullptr, 0ll, sc);
+                       nullptr, nullptr, L.seg_len, 0ll, sc);
     hipError_t e = launch_scan_u32(st, L.counts, L.off, L.scratch, nst, nullptr, 0, &sc->ds_total);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_ds_super_lists, dim3(nst), dim3(DS_TPB), 0, st, fc, xs, ys, zs, w2, nst, L.off, L.counts,
-                       L.list, L.cap, sc);
-    hipLaunchKernelGGL(k_ds_meas_tiles, dim3(nst * DS_BLOCKS_PER_ST), dim3(DS_TPB), 0, st, abort_flag, fc, xs, ys,
-                       zs, L.off, L.list, w2, meas_occ_stddev, max_range, meas_occ, meas_free);
+                       L.list, L.nseg, L.seg_len, L.cap, sc);
+    e = launch_scan_u32(st, L.nseg, L.seg_off, L.seg_scratch, nst, nullptr, 0, &sc->ds_segments);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_ds_seg_count, dim3(seg_blocks), dim3(DS_TPB), 0, st, abort_flag, fc, xs, ys, zs, L.off,
+                       L.list, L.seg_off, nst, L.seg_len, w2, L.segcnt);
+    hipLaunchKernelGGL(k_ds_meas_tiles, dim3(seg_blocks), dim3(DS_TPB), 0, st, abort_flag, fc, xs, ys, zs, L.off,
+                       L.list, L.seg_off, nst, L.seg_len, L.segcnt, w2, meas_occ_stddev, max_range, L.part, meas_occ,
+                       meas_free);
+    hipLaunchKernelGGL(k_ds_seg_combine, dim3(nst * DS_BLOCKS_PER_ST), dim3(DS_TPB), 0, st, abort_flag, fc,
+                       L.seg_off, w2, max_range, L.part, meas_occ, meas_free);
     return hipGetLastError();
 }
 

@@ -469,13 +469,32 @@ K3_HD K3DsBox k3ds_box(int gs, int gsz, float res, float sx, float sy, float sz,
     return b;
 }
 
-// Whether the line of point m (|m|^2 = m2) may pass near a voxel of box b;
+// Whether the line of point m (|m|^2 = m2) may pass near a voxel of box b:
+// |C x m|^2 <= thr |m|^2 (the rounding of both sides is within thr's margin);
 // true for every point the bound does not cover (m2 outside [1e-20, 1e30], NaN).
 K3_HD bool k3ds_may_pass(const K3DsBox& b, float mx, float my, float mz, float m2) {
     if (!(m2 >= 1e-20f && m2 <= 1e30f)) return true;
     const float qx = b.cy * mz - b.cz * my, qy = b.cz * mx - b.cx * mz, qz = b.cx * my - b.cy * mx;
-    const float d2 = (qx * qx + qy * qy + qz * qz) / m2;
-    return !(d2 > b.thr);
+    return !(qx * qx + qy * qy + qz * qz > b.thr * m2);
+}
+
+// k3ds_ray_near decided from r, an approximation of 1/m2 within 1 ulp,
+// where that is safe: the value c2 - inner^2 r is within 4.8e-7 c2 of the
+// test's c2 - inner^2/m2 (1e-20 <= m2 <= 1e30, c2 <= 1e8), so when it is
+// farther than 1e-6 c2 + 1e-12 from w2 it falls on the same side; otherwise,
+// and outside those ranges, the division decides.
+K3_HD bool k3ds_ray_near_rcp(float x, float y, float z, float c2, float mx, float my, float mz, float m2, float r,
+                             float w2) {
+    const float inner = x * mx + y * my + z * mz;
+    if (!(inner > 0.0f)) return false;
+    const float q = inner * inner;
+    if (m2 >= 1e-20f && m2 <= 1e30f && c2 <= 1e8f) {
+        const float a = c2 - q * r;
+        const float tol = 1e-6f * c2 + 1e-12f;
+        if (a > w2 + tol) return false;
+        if (a < w2 - tol) return true;
+    }
+    return c2 - q / m2 < w2;
 }
 
 // predict_free_mass, mass_update.cu:12-15

@@ -182,6 +182,7 @@ def test_renormalisation_occurs_in_small_scene(oracle_lib):
 # ------------------------------------------------- culling bound of the GPU path
 
 _BOUND_CHECK = r"""
+#include <cmath>
 #include <cstdio>
 #include <random>
 #include "k3_math.h"
@@ -222,7 +223,15 @@ int main() {
         const float mx = (float)((x + ux * off) * scale), my = (float)((y + uy * off) * scale),
                     mz = (float)((z + uz * off) * scale);
         const float c2 = x * x + y * y + z * z, m2 = k3ds_m2(mx, my, mz);
-        if (!k3ds_ray_near(x, y, z, c2, mx, my, mz, m2, w2)) continue;
+        // the reciprocal form agrees with the division for 1/m2 off by up to 1 ulp
+        const bool exact = k3ds_ray_near(x, y, z, c2, mx, my, mz, m2, w2);
+        const float r = 1.0f / m2;
+        for (float rr : {r, std::nextafter(r, 0.0f), std::nextafter(r, INFINITY)})
+            if (k3ds_ray_near_rcp(x, y, z, c2, mx, my, mz, m2, rr, w2) != exact) {
+                std::printf("RCP gs=%d res=%g c=(%g,%g,%g) m=(%g,%g,%g) r=%a\n", gs, res, x, y, z, mx, my, mz, rr);
+                return 1;
+            }
+        if (!exact) continue;
         ++accepted;
         if (!k3ds_may_pass(b, mx, my, mz, m2)) {
             std::printf("MISS gs=%d gsz=%d res=%g s=(%g,%g,%g) box=[%d,%d]x[%d,%d]x[%d,%d] c=(%g,%g,%g) m=(%g,%g,%g)\n",
@@ -242,7 +251,9 @@ def test_culling_bound_keeps_every_accepted_point(tmp_path):
     (k3ds_box / k3ds_may_pass); a point that the exact float test accepts for
     a voxel of the box must never be skipped, also at the test's rounding
     edge (points within 1e-3 of the window, single-voxel boxes) and at extreme
-    ranges."""
+    ranges.  Its exact test takes 1/|m|^2 from the hardware reciprocal
+    (k3ds_ray_near_rcp); with the reciprocal off by up to 1 ulp it must give
+    the division's answer."""
     import os
     import subprocess
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -260,10 +271,16 @@ def test_culling_bound_keeps_every_accepted_point(tmp_path):
 # ------------------------------------------------------------------ GPU parity
 
 @pytest.mark.gpu
-def test_ds_meas_grid_single_ray_gpu(gpu):
+@pytest.mark.parametrize("short_segments", [False, True])
+def test_ds_meas_grid_single_ray_gpu(gpu, monkeypatch, short_segments):
+    """The hand-built single-ray cases on the GPU; with short_segments the
+    candidate lists are split into segments of 64 points, so the first-100
+    rule is carried across segments."""
     from k3dom_amd.dom import DOM
     from oracle.orc import OracleDOM
     from tests._parity import bits_equal, first_diff
+    if short_segments:
+        monkeypatch.setenv("K3DOM_TEST_TINY_CAPACITY", "1")
     for pts in ([(0.5, 0.1, 0.1)], _ray_points(False, 100), _ray_points(False, 99), _ray_points(True, 150)):
         print(f"  single-ray case with {len(pts)} points", flush=True)
         g = _meas(DOM, pts)
