@@ -133,7 +133,7 @@ k3dom_status k3dom_create(const k3dom_params* params, k3dom_handle** out);
  * DS-PHD/MIB map uses resolution, max_range and meas_occ_stddev for its
  * measurement grid and ignores sigma, ls, gamma and mass_scale; it keeps
  * its occupied mass in GridCell::occ_mass, and static/dynamic masses stay at
- * their priors.  Batches and differentiated windows take K3DOM maps only. */
+ * their priors.  Batches and differentiated windows build K3DOM maps only. */
 k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3dom_handle** out);
 
 /* Replaces DOM_c::~DOM_c (src/common/dom.cu:68-88).  NULL is accepted. */
