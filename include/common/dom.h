@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../k3dom.h"
+#include "mappoint.h"
 
 namespace dom {
 
@@ -164,6 +165,21 @@ public:
     // Fills grid_cell_array_host with every voxel (index x + gs*y + gs*gs*z).
     void copyCellsToHost() {
         check(k3dom_read_cells(handle_, reinterpret_cast<k3dom_cell*>(grid_cell_array_host)), "copyCellsToHost");
+    }
+
+    // The node's /map_data records (src/k3dom/k3dom_node.cpp:161-184,
+    // src/dsphdmib/dsphdmib_node.cpp:153-177) built on the GPU: every voxel
+    // in index order, or with visible = true only the voxels passing the
+    // visualisation filter (min_occ, min_mass) of publish_map, in index order.
+    void copyMapPointsToHost(std::vector<MapPoint>& out, bool visible = false, float min_occ = 0.0f,
+                             float min_mass = 0.0f) {
+        out.resize((size_t)grid_cell_count);
+        int64_t n = 0;
+        check(k3dom_export_mappoints(handle_, visible ? K3DOM_MAPPOINTS_VISIBLE : K3DOM_MAPPOINTS_ALL, min_occ,
+                                     min_mass, reinterpret_cast<k3dom_mappoint*>(out.data()), (int64_t)out.size(),
+                                     &n),
+              "copyMapPointsToHost");
+        out.resize(status_ == K3DOM_OK ? (size_t)n : 0);
     }
 
     k3dom_status lastStatus() const { return status_; }

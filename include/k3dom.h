@@ -170,6 +170,39 @@ k3dom_status k3dom_read_cells(k3dom_handle* h, k3dom_cell* out);
 k3dom_status k3dom_read_masses(k3dom_handle* h, float* free_mass, float* static_mass,
                                float* dynamic_mass, float* mean_vx, float* mean_vy, float* mean_vz);
 
+/* One voxel of the evaluation export, MapPoint of include/common/mappoint.h:5-28
+ * (PCL_ADD_POINT4D x, y, z + padding, then seven floats, 16-byte aligned:
+ * 48 bytes).  Values as the node's publish_mapdata:
+ *   K3DOM (src/k3dom/k3dom_node.cpp:161-184): v = mean velocity * resolution
+ *     (m/s), occ_val = occ_mean(), dyn_val = dyn_mean(), eval_aug =
+ *     total_mass(), dyn_aug = 1.1;
+ *   DS-PHD/MIB (src/dsphdmib/dsphdmib_node.cpp:153-177): occ_val =
+ *     occ_prob(), dyn_val = mean_vel_sq() * resolution^2, eval_aug =
+ *     -(1 - occ_mass - free_mass), dyn_aug = occ_mass.
+ * x, y, z = indexToCoordX/Y/Z(i) (voxel centre, global frame); w = 1, pad = 0. */
+typedef struct k3dom_mappoint {
+    float x, y, z, w;
+    float v_x, v_y, v_z, occ_val;
+    float dyn_val, eval_aug, dyn_aug, pad;
+} k3dom_mappoint;
+
+enum {
+    K3DOM_MAPPOINTS_ALL = 0,     /* every voxel, index order (publish_mapdata)            */
+    K3DOM_MAPPOINTS_VISIBLE = 1  /* voxels passing the visualisation filter of publish_map,
+                                    compacted on the GPU in index order: K3DOM
+                                    occ_mean() > min_occ && total_mass() > min_mass
+                                    (src/k3dom/k3dom_node.cpp:137-138); DS-PHD/MIB
+                                    occ_prob() > min_occ && occ_mass + free_mass > min_mass
+                                    (src/dsphdmib/dsphdmib_node.cpp:134-135)              */
+};
+
+/* Builds the MapPoint records of the current map on the GPU and copies them to
+ * out.  *count receives the number of records; with out == NULL and
+ * capacity == 0 only the count is returned.  A capacity below the count is
+ * K3DOM_ERR_INVALID_ARG (count still written). */
+k3dom_status k3dom_export_mappoints(k3dom_handle* h, int32_t mode, float min_occ, float min_mass,
+                                   k3dom_mappoint* out, int64_t capacity, int64_t* count);
+
 /* Internal arrays of the last frame, for parity checks and diagnostics. */
 typedef enum k3dom_array {
     K3DOM_ARR_PARTICLE_POS = 0,  /* 3*N float: x[N], y[N], z[N] (current set)     */

@@ -1017,6 +1017,59 @@ k3dom_status k3dom_read_cells(k3dom_handle* h, k3dom_cell* out) {
     return K3DOM_OK;
 }
 
+// src/k3dom/k3dom_node.cpp:161-184 / src/dsphdmib/dsphdmib_node.cpp:153-177
+// (all voxels), :128-150 / :127-141 (the visualisation filter)
+k3dom_status k3dom_export_mappoints(k3dom_handle* h, int32_t mode, float min_occ, float min_mass,
+                                   k3dom_mappoint* out, int64_t capacity, int64_t* count) {
+    if (!h || !count || (mode != K3DOM_MAPPOINTS_ALL && mode != K3DOM_MAPPOINTS_VISIBLE) || capacity < 0 ||
+        (!out && capacity > 0))
+        return K3DOM_ERR_INVALID_ARG;
+    {
+        k3dom_status st = settle_shift(h);
+        if (st != K3DOM_OK) return st;
+    }
+    const size_t V = (size_t)h->V;
+    if (!h->mp_out) {
+        HIP_TRY(h, dalloc(h, &h->mp_out, 3 * V));
+        HIP_TRY(h, dalloc(h, &h->mp_flags, V));
+        HIP_TRY(h, dalloc(h, &h->mp_off, V + 1));
+        HIP_TRY(h, dalloc_zero(h, &h->mp_scratch, (size_t)scan_scratch_words((long long)V)));
+    }
+    MapPointArgs a;
+    a.gs = h->gs;
+    a.gsz = h->gsz;
+    a.res = h->p.resolution;
+    a.size = h->p.size;
+    a.size_z = h->p.size_z;
+    a.cx = h->center[0];
+    a.cy = h->center[1];
+    a.cz = h->center[2];
+    a.ds = h->method == K3DOM_METHOD_DSPHDMIB ? 1 : 0;
+    a.prior_s = h->p.prior_static;
+    a.prior_d = h->p.prior_dynamic;
+    a.prior_occ = h->p.prior_occ;
+    a.min_occ = min_occ;
+    a.min_mass = min_mass;
+    HIP_TRY(h, launch_export_mappoints(h->stream, h->vox[h->vcur], h->V, a, mode, h->mp_flags, h->mp_off,
+                                       h->mp_scratch, &h->sc->mp_total, h->mp_out));
+    int64_t n = (int64_t)V;
+    if (mode == K3DOM_MAPPOINTS_VISIBLE) {
+        uint32_t total = 0;
+        HIP_TRY(h, hipMemcpyAsync(&total, &h->sc->mp_total, 4, hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        n = total;
+    }
+    *count = n;
+    if (!out && capacity == 0) return K3DOM_OK;
+    if (capacity < n) return fail(h, K3DOM_ERR_INVALID_ARG, "export_mappoints: capacity below the record count");
+    static_assert(sizeof(k3dom_mappoint) == 3 * sizeof(float4), "MapPoint record layout");
+    if (n > 0)
+        HIP_TRY(h, hipMemcpyAsync(out, h->mp_out, (size_t)n * sizeof(k3dom_mappoint), hipMemcpyDeviceToHost,
+                                  h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return K3DOM_OK;
+}
+
 k3dom_status k3dom_read_array(k3dom_handle* h, int32_t which, void* dst, int64_t capacity) {
     if (!h || !dst) return K3DOM_ERR_INVALID_ARG;
     {

@@ -55,6 +55,10 @@ struct k3dom_handle {
     uint32_t* maxparts = nullptr;  // per-block maxima feeding the fixed-point CDF exponents
     float* ds_norm = nullptr;      // V (DS-PHD/MIB): sums above 1 of the voxels' weights, else 0
     DsLists ds{};                  // DS-PHD/MIB: measurement-grid candidate lists
+    // MapPoint export (allocated at the first export): 3 float4 per voxel,
+    // visibility flags, their scan and its scratch
+    float4* mp_out = nullptr;
+    uint32_t *mp_flags = nullptr, *mp_off = nullptr, *mp_scratch = nullptr;
     const uint32_t* last_perm = nullptr;
     const uint32_t* last_skeys = nullptr;
     uint64_t *cdf_v = nullptr, *cdf_j = nullptr, *blocksums = nullptr;  // blocksums: fixed-point scan scratch

@@ -23,7 +23,7 @@ struct DevScalars {
     uint32_t level_items[3];  // work items per level
     uint32_t ds_total;        // DS-PHD/MIB measurement-grid candidate entries
     uint32_t ds_segments;     // and their segments
-    uint32_t pad[1];
+    uint32_t mp_total;        // records of the last MapPoint export
 };
 
 // Voxel state arrays (one buffer set).
@@ -56,6 +56,23 @@ struct FrameConsts {
     int32_t pad;
     float pshift[3];     // map shift (voxels) still due on the particle positions, applied by predict
 };
+
+// ------------------------------------------------------------------ MapPoint export
+// Geometry and field rules of one export (k3dom_export_mappoints).
+struct MapPointArgs {
+    int gs, gsz;
+    float res, size, size_z;
+    float cx, cy, cz;   // map centre (global frame)
+    int ds;             // 1: DS-PHD/MIB fields (occupied mass in the s slot)
+    float prior_s, prior_d, prior_occ;
+    float min_occ, min_mass;
+};
+// mode 0: every voxel to out[c]; mode 1: visible voxels compacted in index
+// order (flags, off: V and V + 1 words, scratch: scan_scratch_words(V),
+// zeroed once; the count to *total).  out: 3 float4 per record.
+hipError_t launch_export_mappoints(hipStream_t st, const VoxelArrays& v, int V, const MapPointArgs& a, int mode,
+                                   uint32_t* flags, uint32_t* off, uint32_t* scratch, uint32_t* total,
+                                   float4* out);
 
 // ------------------------------------------------------------------ scans
 // u32 words of scan scratch for up to n_max elements; the scratch must be

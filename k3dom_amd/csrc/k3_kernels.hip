@@ -649,6 +649,86 @@ hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelA
     return hipGetLastError();
 }
 
+// ============================================================== MapPoint export
+// The node's evaluation records (publish_mapdata, src/k3dom/k3dom_node.cpp:161-184,
+// src/dsphdmib/dsphdmib_node.cpp:153-177) with the GridCell accessors of
+// include/common/dom_types.h:47-64 and DOM_c::indexToCoord of
+// include/common/dom.h:70-81, same float expressions.
+struct MpCell {
+    float f, s, d, occ, vx, vy, vz;
+};
+__device__ __forceinline__ MpCell mp_cell(const VoxelArrays& v, int c, const MapPointArgs& a) {
+    MpCell m;
+    m.f = v.f[c];
+    if (a.ds) {
+        m.s = a.prior_s;
+        m.d = a.prior_d;
+        m.occ = v.s[c];
+    } else {
+        m.s = v.s[c];
+        m.d = v.d[c];
+        m.occ = a.prior_occ;
+    }
+    m.vx = v.mvx[c];
+    m.vy = v.mvy[c];
+    m.vz = v.mvz[c];
+    return m;
+}
+__device__ __forceinline__ bool mp_visible(const MpCell& m, const MapPointArgs& a) {
+    if (a.ds) return m.occ + (1.0f - m.occ - m.f) / 2.0f > a.min_occ && m.occ + m.f > a.min_mass;
+    const float total = m.f + m.s + m.d;
+    return (m.s + m.d) / total > a.min_occ && total > a.min_mass;
+}
+
+__global__ __launch_bounds__(TPB) void k_mappoint_flags(VoxelArrays v, int V, MapPointArgs a,
+                                                        uint32_t* __restrict__ flags) {
+    const int c = blockIdx.x * TPB + threadIdx.x;
+    if (c < V) flags[c] = mp_visible(mp_cell(v, c, a), a) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(TPB) void k_mappoint_fill(VoxelArrays v, int V, MapPointArgs a,
+                                                       const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ off, float4* __restrict__ out) {
+    const int c = blockIdx.x * TPB + threadIdx.x;
+    if (c >= V || (flags && !flags[c])) return;
+    const MpCell m = mp_cell(v, c, a);
+    const int gs2 = a.gs * a.gs;
+    const float x = ((float)(c % a.gs) + 0.5f) * a.res - a.size / 2.0f + a.cx;
+    const float y = ((float)((c % gs2) / a.gs) + 0.5f) * a.res - a.size / 2.0f + a.cy;
+    const float z = ((float)(c / gs2) + 0.5f) * a.res - a.size_z / 2.0f + a.cz;
+    float occ_val, dyn_val, eval_aug, dyn_aug;
+    if (a.ds) {
+        occ_val = m.occ + (1.0f - m.occ - m.f) / 2.0f;
+        dyn_val = (m.vx * m.vx + m.vy * m.vy + m.vz * m.vz) * a.res * a.res;
+        eval_aug = -(1.0f - m.occ - m.f);
+        dyn_aug = m.occ;
+    } else {
+        const float total = m.f + m.s + m.d;
+        occ_val = (m.s + m.d) / total;
+        dyn_val = m.d / total;
+        eval_aug = total;
+        dyn_aug = 1.1f;
+    }
+    const size_t o = 3 * (size_t)(off ? off[c] : (uint32_t)c);
+    out[o] = make_float4(x, y, z, 1.0f);
+    out[o + 1] = make_float4(m.vx * a.res, m.vy * a.res, m.vz * a.res, occ_val);
+    out[o + 2] = make_float4(dyn_val, eval_aug, dyn_aug, 0.0f);
+}
+
+hipError_t launch_export_mappoints(hipStream_t st, const VoxelArrays& v, int V, const MapPointArgs& a, int mode,
+                                   uint32_t* flags, uint32_t* off, uint32_t* scratch, uint32_t* total,
+                                   float4* out) {
+    if (mode == 0) {
+        hipLaunchKernelGGL(k_mappoint_fill, dim3(blocks_for(V)), dim3(TPB), 0, st, v, V, a, nullptr, nullptr, out);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_mappoint_flags, dim3(blocks_for(V)), dim3(TPB), 0, st, v, V, a, flags);
+    hipError_t e = launch_scan_u32(st, flags, off, scratch, V, nullptr, 0, total);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mappoint_fill, dim3(blocks_for(V)), dim3(TPB), 0, st, v, V, a, flags, off, out);
+    return hipGetLastError();
+}
+
 // ============================================================== DS-PHD/MIB method
 // Measurement grid (src/dsphdmib/kernel/measurement_grid.cu:64-160): every
 // voxel scans the points in index order and keeps the first

@@ -20,6 +20,11 @@ CELL_DTYPE = np.dtype([
 ])
 assert CELL_DTYPE.itemsize == C.sizeof(Cell) == 68
 
+# MapPoint (include/common/mappoint.h:5-28): x, y, z + padding, seven floats, 16-byte aligned
+MAPPOINT_DTYPE = np.dtype([(n, np.float32) for n in (
+    "x", "y", "z", "w", "v_x", "v_y", "v_z", "occ_val", "dyn_val", "eval_aug", "dyn_aug", "pad")])
+assert MAPPOINT_DTYPE.itemsize == 48
+
 
 def params_default():
     p = Params()
@@ -147,6 +152,18 @@ class DOM:
         out = np.empty(self.grid_cell_count, dtype=CELL_DTYPE)
         check(self._lib.k3dom_read_cells(self._h, out.ctypes.data_as(C.POINTER(Cell))), self._h, "read_cells")
         return out
+
+    def export_mappoints(self, visible=False, min_occ=0.0, min_mass=0.0):
+        """The node's evaluation records (MapPoint) built on the GPU: every
+        voxel in index order, or (visible=True) only the voxels passing the
+        visualisation filter with thresholds min_occ / min_mass, compacted in
+        index order (k3dom_export_mappoints)."""
+        mode = 1 if visible else 0
+        n = C.c_int64(0)
+        out = np.empty(self.grid_cell_count, MAPPOINT_DTYPE)  # at most one record per voxel
+        check(self._lib.k3dom_export_mappoints(self._h, mode, min_occ, min_mass, out.ctypes.data_as(C.c_void_p),
+                                               len(out), C.byref(n)), self._h, "export_mappoints")
+        return out[:n.value]
 
     # ---------------------------------------------------------------- extras
     def read_masses(self):
