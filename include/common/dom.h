@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <cassert>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "../k3dom.h"
@@ -182,6 +183,43 @@ public:
         out.resize(status_ == K3DOM_OK ? (size_t)n : 0);
     }
 
+    // Checkpoint of the filter state (k3dom_checkpoint_save): parameters,
+    // frame counter, time, pose, voxels and particles, versioned and checksummed.
+    bool saveCheckpoint(std::vector<unsigned char>& out) {
+        int64_t n = 0, written = 0;
+        check(k3dom_checkpoint_size(handle_, &n), "saveCheckpoint");
+        if (status_ != K3DOM_OK) return false;
+        out.resize((size_t)n);
+        check(k3dom_checkpoint_save(handle_, out.data(), n, &written), "saveCheckpoint");
+        return status_ == K3DOM_OK;
+    }
+    // Replaces this map with the one a checkpoint holds (its parameters and
+    // method included); later frames continue it bit for bit.
+    bool loadCheckpoint(const void* data, size_t size) {
+        k3dom_handle* nh = nullptr;
+        status_ = k3dom_checkpoint_load(data, (int64_t)size, &nh);
+        if (status_ != K3DOM_OK) {
+            std::fprintf(stderr, "k3dom: loadCheckpoint failed: %s\n", k3dom_last_error(nullptr));
+            return false;
+        }
+        k3dom_destroy(handle_);
+        handle_ = nh;
+        k3dom_params c{};
+        std::memcpy(&c, static_cast<const unsigned char*>(data) + 128, sizeof(c));
+        params = fromC(c);
+        k3dom_info i = info();
+        grid_size = i.grid_size;
+        grid_size_z = i.grid_size_z;
+        particle_count = i.particle_count;
+        new_born_particle_count = i.new_born_particle_count;
+        if (i.grid_cell_count != grid_cell_count) {
+            delete[] grid_cell_array_host;
+            grid_cell_count = i.grid_cell_count;
+            grid_cell_array_host = new GridCell[(size_t)grid_cell_count];
+        }
+        return true;
+    }
+
     k3dom_status lastStatus() const { return status_; }
     k3dom_handle* handle() const { return handle_; }
 
@@ -202,6 +240,37 @@ protected:
         k3dom_info i{};
         if (handle_) k3dom_get_info(handle_, &i);
         return i;
+    }
+    static Params fromC(const k3dom_params& c) {
+        Params p{};
+        p.size = c.size;
+        p.size_z = c.size_z;
+        p.mass_scale = c.mass_scale;
+        p.resolution = c.resolution;
+        p.particle_count = c.particle_count;
+        p.new_born_particle_count = c.new_born_particle_count;
+        p.persistence_prob = c.persistence_prob;
+        p.stddev_process_noise_position = c.stddev_process_noise_position;
+        p.stddev_process_noise_velocity = c.stddev_process_noise_velocity;
+        p.birth_prob = c.birth_prob;
+        p.stddev_velocity = c.stddev_velocity;
+        p.init_max_velocity = c.init_max_velocity;
+        p.particle_min_vel = c.particle_min_vel;
+        p.sensor_off_x = c.sensor_off_x;
+        p.sensor_off_y = c.sensor_off_y;
+        p.sensor_off_z = c.sensor_off_z;
+        p.map_shift_thresh = c.map_shift_thresh;
+        p.prior_free = c.prior_free;
+        p.prior_static = c.prior_static;
+        p.prior_dynamic = c.prior_dynamic;
+        p.prior_occ = c.prior_occ;
+        p.sigma = c.sigma;
+        p.ls = c.ls;
+        p.gamma = c.gamma;
+        p.max_range = c.max_range;
+        p.meas_occ_stddev = c.meas_occ_stddev;
+        p.seed = c.seed;
+        return p;
     }
     static k3dom_params toC(const Params& p) {
         k3dom_params c{};

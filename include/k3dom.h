@@ -203,6 +203,28 @@ enum {
 k3dom_status k3dom_export_mappoints(k3dom_handle* h, int32_t mode, float min_occ, float min_mass,
                                    k3dom_mappoint* out, int64_t capacity, int64_t* count);
 
+/* Checkpoint and resume, no counterpart in the reference (whose filter state
+ * exists only in device memory for the life of the process).  A checkpoint
+ * holds the parameters and method, the frame counter (which keys every random
+ * draw, so the random streams continue), the time of the last frame, the
+ * pose, every voxel array and the current particle set: a map restored from
+ * it and given the same later frames produces the same results bit for bit
+ * as the map it was taken from.  A pending map shift is applied before the
+ * state is written.  The format is versioned and checksummed (FNV-1a over the
+ * payload's 64-bit words); a 128-byte header is followed by the k3dom_params
+ * record, then the arrays (layout in k3dom_amd/csrc/k3_checkpoint.cpp); the
+ * per-frame arrays of k3dom_read_array describe frames the handle processed
+ * itself, so a restored handle has them only after its first frame.
+ *
+ * k3dom_checkpoint_size: bytes of a checkpoint of h.
+ * k3dom_checkpoint_save: writes it to host memory dst (capacity bytes).
+ * k3dom_checkpoint_load: creates a handle from a checkpoint in host memory;
+ *   K3DOM_ERR_INVALID_ARG for a buffer that is not a whole, intact
+ *   checkpoint of this version (checked before any device work). */
+k3dom_status k3dom_checkpoint_size(k3dom_handle* h, int64_t* bytes);
+k3dom_status k3dom_checkpoint_save(k3dom_handle* h, void* dst, int64_t capacity, int64_t* written);
+k3dom_status k3dom_checkpoint_load(const void* src, int64_t size, k3dom_handle** out);
+
 /* Internal arrays of the last frame, for parity checks and diagnostics. */
 typedef enum k3dom_array {
     K3DOM_ARR_PARTICLE_POS = 0,  /* 3*N float: x[N], y[N], z[N] (current set)     */

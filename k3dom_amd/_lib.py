@@ -75,6 +75,9 @@ SIGNATURES = {
     "k3dom_read_cells": (C.c_int, [C.c_void_p, C.POINTER(Cell)]),
     "k3dom_read_masses": (C.c_int, [C.c_void_p, _fp, _fp, _fp, _fp, _fp, _fp]),
     "k3dom_read_array": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
+    "k3dom_checkpoint_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "k3dom_checkpoint_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "k3dom_checkpoint_load": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
     "k3dom_export_mappoints": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_int64,
                                          C.POINTER(C.c_int64)]),
     "k3dom_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),

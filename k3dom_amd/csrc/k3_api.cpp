@@ -442,6 +442,8 @@ void finish_frame(k3dom_handle* h, double t, int m) {
     h->stats.gather_items = (uint64_t)h->h_sc->n_light + h->h_sc->n_heavy;
     h->stats.gather_split_items = h->h_sc->n_heavy;
     h->stats.capacity_regrowths = (uint64_t)h->overflow_retries;
+    if (h->updated_time > 0) h->normal_here++;
+    h->frames_here++;
     h->updated_time = t;
     h->frame++;
 }
@@ -518,6 +520,7 @@ k3dom_status reset_map(k3dom_handle* h) {
     h->shift_pending = false;
     h->pshift_pending = false;
     h->frame = 0;
+    h->frames_here = h->normal_here = 0;
     h->first_pose = false;
     h->updated_time = -1.0;
     for (int i = 0; i < 3; ++i) h->sensor[i] = h->center[i] = 0.0f;
@@ -1127,7 +1130,7 @@ k3dom_status k3dom_set_timing(k3dom_handle* h, int32_t enable) {
 k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
     if (h->method != K3DOM_METHOD_K3DOM) return fail(h, K3DOM_ERR_STATE, "autotune tunes the K3DOM gather");
-    if (h->frame < 2) return fail(h, K3DOM_ERR_STATE, "autotune needs a normal frame first");
+    if (h->normal_here < 1) return fail(h, K3DOM_ERR_STATE, "autotune needs a normal frame first");
     hipEvent_t e0, e1;
     HIP_TRY(h, hipEventCreate(&e0));
     HIP_TRY(h, hipEventCreate(&e1));
@@ -1183,7 +1186,7 @@ k3dom_status k3dom_get_stats(const k3dom_handle* h, k3dom_stats* out) {
 k3dom_status k3dom_count_bgki_pairs(k3dom_handle* h, uint64_t* out2) {
     if (!h || !out2) return K3DOM_ERR_INVALID_ARG;
     if (h->method != K3DOM_METHOD_K3DOM) return fail(h, K3DOM_ERR_STATE, "pairs are counted for K3DOM maps");
-    if (h->frame == 0) return fail(h, K3DOM_ERR_STATE, "no frame processed yet");
+    if (h->frames_here == 0) return fail(h, K3DOM_ERR_STATE, "no frame processed yet");
     HIP_TRY(h, hipMemsetAsync(h->pair_counts, 0, 2 * sizeof(unsigned long long), h->stream));
     const uint32_t n_items = h->h_sc->n_light + h->h_sc->n_heavy;
     HIP_TRY(h, launch_bgki_count(h->stream, h->num_cus * 8, h->last_geom, h->tg, h->p.ls, h->ibuf, n_items,

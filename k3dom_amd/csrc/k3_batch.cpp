@@ -279,7 +279,7 @@ k3dom_status k3dom_batch_update(k3dom_batch* b, double t, const float* poses, co
 k3dom_status k3dom_batch_autotune(k3dom_batch* b, int32_t* blocks_per_cu) {
     if (!b) return K3DOM_ERR_INVALID_ARG;
     for (k3dom_handle* h : b->maps)
-        if (h->frame < 2) return bfail(b, K3DOM_ERR_STATE, "autotune needs a normal frame on every map first");
+        if (h->normal_here < 1) return bfail(b, K3DOM_ERR_STATE, "autotune needs a normal frame on every map first");
     const int n = (int)b->maps.size();
     // the last frame's lists are still in place: re-run the joint gather over
     // them for each launch shape (the gather only rewrites the same evidence)

@@ -81,6 +81,10 @@ struct k3dom_handle {
     DevScalars* h_sc = nullptr;  // pinned mirror
 
     uint32_t frame = 0;
+    // frames (and normal frames) this handle has processed since it was
+    // created, reset or restored from a checkpoint: the lists and per-frame
+    // arrays are its own only after one
+    uint32_t frames_here = 0, normal_here = 0;
     bool first_pose = false;
     float sensor[3] = {0, 0, 0}, center[3] = {0, 0, 0};
     double updated_time = -1.0;

@@ -6,6 +6,7 @@ readback replaces the caller-side cudaMemcpy of grid_cell_array
 (src/k3dom/k3dom_node.cpp:131-132) with copyCellsToHost().
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -56,13 +57,16 @@ METHODS = {"k3dom": 0, "dsphdmib": 1}
 class DOM:
     """One map on the current HIP device."""
 
-    def __init__(self, params, _handle=None, _owner=None, method="k3dom"):
+    def __init__(self, params, _handle=None, _owner=None, method="k3dom", _restored=None):
         """method: "k3dom" (kernel inference) or "dsphdmib" (the reference's
         second filter, a measurement grid of occupied / free masses)."""
         self._lib = _lib.load()
         self.params = params
         self.method = method
-        if _handle is None:
+        if _restored is not None:  # a handle created from a checkpoint
+            h = _restored
+            self._owned = True
+        elif _handle is None:
             h = C.c_void_p()
             check(self._lib.k3dom_create_method(C.byref(params), METHODS[method], C.byref(h)), None,
                   "k3dom_create_method")
@@ -152,6 +156,44 @@ class DOM:
         out = np.empty(self.grid_cell_count, dtype=CELL_DTYPE)
         check(self._lib.k3dom_read_cells(self._h, out.ctypes.data_as(C.POINTER(Cell))), self._h, "read_cells")
         return out
+
+    # ---------------------------------------------------------------- checkpoints
+    def checkpoint(self):
+        """The filter state as bytes (k3dom_checkpoint_save)."""
+        n = C.c_int64(0)
+        check(self._lib.k3dom_checkpoint_size(self._h, C.byref(n)), self._h, "checkpoint_size")
+        buf = np.empty(n.value, np.uint8)
+        w = C.c_int64(0)
+        check(self._lib.k3dom_checkpoint_save(self._h, buf.ctypes.data_as(C.c_void_p), n.value, C.byref(w)),
+              self._h, "checkpoint_save")
+        return buf[:w.value].tobytes()
+
+    @classmethod
+    def from_checkpoint(cls, data):
+        """A map restored from checkpoint bytes (k3dom_checkpoint_load): same
+        parameters and method, continues bit for bit."""
+        lib = _lib.load()
+        buf = np.frombuffer(data, np.uint8)
+        h = C.c_void_p()
+        check(lib.k3dom_checkpoint_load(buf.ctypes.data_as(C.c_void_p), len(buf), C.byref(h)), None,
+              "checkpoint_load")
+        params = _lib.Params.from_buffer_copy(bytes(buf[128:128 + C.sizeof(_lib.Params)]))
+        method = {v: k for k, v in METHODS.items()}[int(np.frombuffer(bytes(buf[20:24]), np.int32)[0])]
+        return cls(params, method=method, _restored=h)
+
+    def save(self, path):
+        """Writes a checkpoint file."""
+        data = self.checkpoint()
+        tmp = path + ".tmp"
+        with open(tmp, "wb") as fh:
+            fh.write(data)
+        os.replace(tmp, path)
+
+    @classmethod
+    def load(cls, path):
+        """A map restored from a checkpoint file."""
+        with open(path, "rb") as fh:
+            return cls.from_checkpoint(fh.read())
 
     def export_mappoints(self, visible=False, min_occ=0.0, min_mass=0.0):
         """The node's evaluation records (MapPoint) built on the GPU: every

@@ -34,7 +34,10 @@ int main() {
     void (dom::DOM::*upd)(float, std::vector<float>&, std::vector<float>&, std::vector<float>&) =
         &dom::DOM::updateGrid;
     void (dom::DOM_c::*pose)(float, float, float) = &dom::DOM_c::updatePose;
-    (void)upd; (void)pose; (void)p;
+    bool (dom::DOM_c::*save)(std::vector<unsigned char>&) = &dom::DOM_c::saveCheckpoint;
+    bool (dom::DOM_c::*load)(const void*, size_t) = &dom::DOM_c::loadCheckpoint;
+    void (dom::DOM_c::*mp)(std::vector<MapPoint>&, bool, float, float) = &dom::DOM_c::copyMapPointsToHost;
+    (void)upd; (void)pose; (void)p; (void)save; (void)load; (void)mp;
     return 0;
 }
 ''')
