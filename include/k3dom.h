@@ -170,6 +170,32 @@ k3dom_status k3dom_read_cells(k3dom_handle* h, k3dom_cell* out);
 k3dom_status k3dom_read_masses(k3dom_handle* h, float* free_mass, float* static_mass,
                                float* dynamic_mass, float* mean_vx, float* mean_vy, float* mean_vz);
 
+/* Several LiDARs (the node's PCL transform, src/k3dom/k3dom_node.cpp:73-83,
+ * done on the GPU and generalised; no counterpart in the reference, whose
+ * node serves one LiDAR).  Each LiDAR gives its points in its own frame, the
+ * rotation from its axes to the map's (row-major, rot[0..2] the first row)
+ * and its origin relative to the position given to k3dom_update_pose
+ * (metres, map axes).  A point becomes rot * p, relative to its LiDAR: each
+ * LiDAR's evidence is computed as the reference computes its one sensor's
+ * with the sensor at that LiDAR's origin, and the LiDARs' evidence is added
+ * in LiDAR order.  One LiDAR with rot = identity and origin 0 is exactly
+ * k3dom_update_grid.  DS-PHD/MIB maps take one LiDAR (any rot and origin). */
+#define K3DOM_MAX_LIDARS 16
+typedef struct k3dom_lidar {
+    float rot[9];
+    float origin[3];
+    const float* xs;  /* points in the LiDAR's frame: host arrays, or device */
+    const float* ys;  /* arrays when points_on_device is non-zero            */
+    const float* zs;
+    int32_t n;
+    int32_t pad;
+} k3dom_lidar;
+
+/* One filter frame at time t (seconds) from n_lidars (1..K3DOM_MAX_LIDARS)
+ * LiDARs, whose points are transformed and kept by LiDAR on the GPU. */
+k3dom_status k3dom_update_grid_lidars(k3dom_handle* h, double t, const k3dom_lidar* lidars, int32_t n_lidars,
+                                      int32_t points_on_device);
+
 /* One voxel of the evaluation export, MapPoint of include/common/mappoint.h:5-28
  * (PCL_ADD_POINT4D x, y, z + padding, then seven floats, 16-byte aligned:
  * 48 bytes).  Values as the node's publish_mapdata:

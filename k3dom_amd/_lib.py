@@ -40,6 +40,12 @@ class Cell(C.Structure):
     ]
 
 
+class Lidar(C.Structure):
+    """k3dom_lidar (include/k3dom.h)."""
+    _fields_ = [("rot", C.c_float * 9), ("origin", C.c_float * 3), ("xs", C.c_void_p), ("ys", C.c_void_p),
+                ("zs", C.c_void_p), ("n", C.c_int32), ("pad", C.c_int32)]
+
+
 class Info(C.Structure):
     _fields_ = [
         ("grid_size", C.c_int32), ("grid_size_z", C.c_int32), ("grid_cell_count", C.c_int32),
@@ -75,6 +81,7 @@ SIGNATURES = {
     "k3dom_read_cells": (C.c_int, [C.c_void_p, C.POINTER(Cell)]),
     "k3dom_read_masses": (C.c_int, [C.c_void_p, _fp, _fp, _fp, _fp, _fp, _fp]),
     "k3dom_read_array": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
+    "k3dom_update_grid_lidars": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(Lidar), C.c_int32, C.c_int32]),
     "k3dom_checkpoint_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "k3dom_checkpoint_save": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "k3dom_checkpoint_load": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),

@@ -252,7 +252,7 @@ k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
     const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
     const uint32_t* ab = &h->sc->overflow;
     const bool ds = h->method == K3DOM_METHOD_DSPHDMIB;
-    if (!ds)
+    if (!ds && h->ngroups == 0)
         HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, grid_geom(h), h->tg, h->ibuf, h->del_occ, h->del_free,
                                        h->sc));
     mark(h, ST_MASS);
@@ -321,9 +321,44 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const f
         }
         mark(h, ST_BGKI);
         mark(h, ST_BGKI_KERNEL);
-        HIP_TRY(h, launch_ds_meas_grid(h->stream, &h->sc->overflow, h->fc, h->gs, h->gsz, dx, dy, dz,
+        // one LiDAR at most: with the sensor at its origin
+        const FrameConsts* dfc = h->ngroups > 0 ? h->gfc : h->fc;
+        HIP_TRY(h, launch_ds_meas_grid(h->stream, &h->sc->overflow, dfc, h->gs, h->gsz, dx, dy, dz,
                                        h->p.resolution, h->p.meas_occ_stddev, h->p.max_range, h->ds, h->sc,
                                        h->del_occ, h->del_free));
+        mark(h, EV_BGKI_END);
+        return enqueue_after_gather(h, normal);
+    }
+    if (h->ngroups > 0) {
+        // one kernel-inference pass per LiDAR (sensor at its origin, its own
+        // points), each combined into its evidence; later LiDARs' evidence is
+        // added to the first's in LiDAR order
+        mark(h, ST_LISTS);
+        for (int g = 0; g < h->ngroups; ++g) {
+            const FrameConsts* gfc = h->gfc + g;
+            const size_t o = (size_t)h->gstart[g];
+            k3dom_status s = build_lists(h, gfc, dx + o, dy + o, dz + o);
+            if (s != K3DOM_OK) return s;
+            if (g == 0) {
+                mark(h, ST_BGKI);
+                mark(h, ST_BGKI_KERNEL);
+            }
+            float* to = g == 0 ? h->del_occ : h->gdel_occ;
+            float* tf = g == 0 ? h->del_free : h->gdel_free;
+            // the gather's work queue restarts for every pass (the frame start
+            // and the overflow re-run clear it before the first)
+            if (g > 0) HIP_TRY(h, launch_reset_item_counter(h->stream, h->sc));
+            GridGeom gg = grid_geom(h);
+            gg.sx = h->h_gfc[g].g.sx;
+            gg.sy = h->h_gfc[g].g.sy;
+            gg.sz = h->h_gfc[g].g.sz;
+            HIP_TRY(h, launch_bgki(h->stream, h->gather_blocks, gfc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
+                                   h->ibuf, h->lbuf[2], h->rec, to, tf, h->sc));
+            HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, gg, h->tg, h->ibuf, to, tf, h->sc));
+            if (g > 0)
+                HIP_TRY(h, launch_add_evidence(h->stream, &h->sc->overflow, h->del_occ, h->del_free, h->gdel_occ,
+                                               h->gdel_free, h->V));
+        }
         mark(h, EV_BGKI_END);
         return enqueue_after_gather(h, normal);
     }
@@ -344,6 +379,9 @@ k3dom_status enqueue_head(k3dom_handle* h, bool normal) {
     VoxelArrays& v = h->vox[h->vcur];
     mark(h, EV_FRAME_START);
     HIP_TRY(h, hipMemcpyAsync(h->fc, h->h_fc, sizeof(FrameConsts), hipMemcpyHostToDevice, h->stream));
+    if (h->ngroups > 0)
+        HIP_TRY(h, hipMemcpyAsync(h->gfc, h->h_gfc, sizeof(FrameConsts) * h->ngroups, hipMemcpyHostToDevice,
+                                  h->stream));
     HIP_TRY(h, launch_reset_scalars(h->stream, h->sc));
     if (normal) {
         mark(h, ST_PREDICT);
@@ -444,11 +482,14 @@ void finish_frame(k3dom_handle* h, double t, int m) {
     h->stats.capacity_regrowths = (uint64_t)h->overflow_retries;
     if (h->updated_time > 0) h->normal_here++;
     h->frames_here++;
+    h->last_grouped = h->ngroups > 0;
     h->updated_time = t;
     h->frame++;
 }
 
-bool frame_uses_graph(const k3dom_handle* h) { return h->updated_time > 0 && h->use_graph && !h->timing; }
+bool frame_uses_graph(const k3dom_handle* h) {
+    return h->updated_time > 0 && h->use_graph && !h->timing && h->ngroups == 0;
+}
 
 k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
     const bool normal = h->updated_time > 0;
@@ -458,6 +499,15 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
     }
     if (h->timing) for (float& x : h->stage_ms) x = 0.0f;
     fill_frame_consts(h, h->h_fc, t, m);
+    for (int g = 0; g < h->ngroups; ++g) {
+        // a LiDAR's pass: the sensor at its origin, its own points
+        FrameConsts& c = h->h_gfc[g];
+        c = *h->h_fc;
+        c.g.sx = (h->sensor[0] + h->gorig[g][0]) - h->center[0];
+        c.g.sy = (h->sensor[1] + h->gorig[g][1]) - h->center[1];
+        c.g.sz = (h->sensor[2] + h->gorig[g][2]) - h->center[2];
+        c.m = h->gcount[g];
+    }
     const bool use_graph = frame_uses_graph(h);
     bool done = false;
     if (use_graph) {
@@ -854,6 +904,8 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
     CREATE_TRY(hipHostMalloc((void**)&h->h_sc, sizeof(DevScalars), hipHostMallocDefault));
     CREATE_TRY(hipHostMalloc((void**)&h->h_fc, sizeof(FrameConsts), hipHostMallocDefault));
     CREATE_TRY(dalloc(h, &h->fc, 1));
+    CREATE_TRY(hipHostMalloc((void**)&h->h_gfc, sizeof(FrameConsts) * K3DOM_MAX_LIDARS, hipHostMallocDefault));
+    CREATE_TRY(dalloc(h, &h->gfc, K3DOM_MAX_LIDARS));
     h->gather_blocks = h->num_cus * 4;
     for (int e = 0; e < EV_COUNT; ++e) CREATE_TRY(hipEventCreate(&h->ev[e]));
     CREATE_TRY(dalloc(h, &h->pair_counts, 2));
@@ -885,6 +937,7 @@ k3dom_status k3dom_destroy(k3dom_handle* h) {
     if (h->h_pts) (void)hipHostFree(h->h_pts);
     if (h->h_sc) (void)hipHostFree(h->h_sc);
     if (h->h_fc) (void)hipHostFree(h->h_fc);
+    if (h->h_gfc) (void)hipHostFree(h->h_gfc);
     drop_graph(h);
     for (int e = 0; e < EV_COUNT; ++e)
         if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
@@ -939,6 +992,70 @@ k3dom_status k3dom_update_grid_device(k3dom_handle* h, double t, const float* d_
     k3dom_status s = ensure_points(h, std::max(n, 1));
     if (s != K3DOM_OK) return s;
     return frame(h, t, d_xs, d_ys, d_zs, n);
+}
+
+// The node's PCL transform (src/k3dom/k3dom_node.cpp:73-83) on the GPU, for
+// several LiDARs kept apart by their origins.
+k3dom_status k3dom_update_grid_lidars(k3dom_handle* h, double t, const k3dom_lidar* lidars, int32_t n_lidars,
+                                      int32_t points_on_device) {
+    if (!h) return K3DOM_ERR_INVALID_ARG;
+    if (!lidars || n_lidars < 1 || n_lidars > K3DOM_MAX_LIDARS)
+        return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: 1 to K3DOM_MAX_LIDARS LiDARs");
+    if (h->method == K3DOM_METHOD_DSPHDMIB && n_lidars > 1)
+        return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: DS-PHD/MIB maps take one LiDAR");
+    long long total = 0;
+    for (int k = 0; k < n_lidars; ++k) {
+        const k3dom_lidar& L = lidars[k];
+        if (L.n < 0 || (L.n > 0 && (!L.xs || !L.ys || !L.zs)))
+            return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: bad point arrays");
+        for (float v : L.rot)
+            if (!(v - v == 0.0f)) return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: non-finite rotation");
+        for (float v : L.origin)
+            if (!(v - v == 0.0f)) return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: non-finite origin");
+        total += L.n;
+    }
+    if (total > 0x7fffffff) return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: too many points");
+    const int m = (int)total;
+    k3dom_status s = ensure_points(h, std::max(m, 1));
+    if (s != K3DOM_OK) return s;
+    // the frame's points, LiDAR after LiDAR, rotated into map axes in the
+    // handle's buffer (host points go through the pinned staging buffer)
+    const size_t cap = (size_t)h->m_cap;
+    float* dx = h->d_pts;
+    float* dy = h->d_pts + cap;
+    float* dz = h->d_pts + 2 * cap;
+    int at = 0;
+    for (int k = 0; k < n_lidars; ++k) {
+        const k3dom_lidar& L = lidars[k];
+        if (!points_on_device && L.n > 0) {
+            std::memcpy(h->h_pts + at, L.xs, sizeof(float) * L.n);
+            std::memcpy(h->h_pts + cap + at, L.ys, sizeof(float) * L.n);
+            std::memcpy(h->h_pts + 2 * cap + at, L.zs, sizeof(float) * L.n);
+        }
+        h->gstart[k] = at;
+        h->gcount[k] = L.n;
+        for (int i = 0; i < 3; ++i) h->gorig[k][i] = L.origin[i];
+        at += L.n;
+    }
+    if (!points_on_device && m > 0)
+        for (int c = 0; c < 3; ++c)
+            HIP_TRY(h, hipMemcpyAsync(h->d_pts + c * cap, h->h_pts + c * cap, sizeof(float) * m,
+                                      hipMemcpyHostToDevice, h->stream));
+    for (int k = 0; k < n_lidars; ++k) {
+        const k3dom_lidar& L = lidars[k];
+        const size_t o = (size_t)h->gstart[k];
+        const float *sx = points_on_device ? L.xs : dx + o, *sy = points_on_device ? L.ys : dy + o,
+                    *sz = points_on_device ? L.zs : dz + o;
+        HIP_TRY(h, launch_lidar_rotate(h->stream, L.rot, sx, sy, sz, dx + o, dy + o, dz + o, L.n));
+    }
+    if (n_lidars > 1 && !h->gdel_occ) {
+        HIP_TRY(h, dalloc(h, &h->gdel_occ, (size_t)h->V));
+        HIP_TRY(h, dalloc(h, &h->gdel_free, (size_t)h->V));
+    }
+    h->ngroups = n_lidars;
+    s = frame(h, t, dx, dy, dz, m);
+    h->ngroups = 0;
+    return s;
 }
 
 k3dom_status k3dom_get_info(const k3dom_handle* h, k3dom_info* out) {
@@ -1187,6 +1304,7 @@ k3dom_status k3dom_count_bgki_pairs(k3dom_handle* h, uint64_t* out2) {
     if (!h || !out2) return K3DOM_ERR_INVALID_ARG;
     if (h->method != K3DOM_METHOD_K3DOM) return fail(h, K3DOM_ERR_STATE, "pairs are counted for K3DOM maps");
     if (h->frames_here == 0) return fail(h, K3DOM_ERR_STATE, "no frame processed yet");
+    if (h->last_grouped) return fail(h, K3DOM_ERR_STATE, "the pair counter covers single-sensor frames");
     HIP_TRY(h, hipMemsetAsync(h->pair_counts, 0, 2 * sizeof(unsigned long long), h->stream));
     const uint32_t n_items = h->h_sc->n_light + h->h_sc->n_heavy;
     HIP_TRY(h, launch_bgki_count(h->stream, h->num_cus * 8, h->last_geom, h->tg, h->p.ls, h->ibuf, n_items,

@@ -58,6 +58,15 @@ struct k3dom_handle {
     // MapPoint export (allocated at the first export): 3 float4 per voxel,
     // visibility flags, their scan and its scratch
     float4* mp_out = nullptr;
+    // LiDAR groups of the current frame (k3dom_update_grid_lidars; 0: one
+    // sensor at the pose): point ranges, origins, per-group frame constants
+    // (device array and pinned staging), and the evidence of groups after the first
+    int ngroups = 0;
+    int gstart[K3DOM_MAX_LIDARS] = {}, gcount[K3DOM_MAX_LIDARS] = {};
+    float gorig[K3DOM_MAX_LIDARS][3] = {};
+    FrameConsts *gfc = nullptr, *h_gfc = nullptr;
+    float *gdel_occ = nullptr, *gdel_free = nullptr;
+    bool last_grouped = false;  // the last frame ran per-LiDAR kernel inference
     uint32_t *mp_flags = nullptr, *mp_off = nullptr, *mp_scratch = nullptr;
     const uint32_t* last_perm = nullptr;
     const uint32_t* last_skeys = nullptr;

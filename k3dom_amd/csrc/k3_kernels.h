@@ -57,6 +57,15 @@ struct FrameConsts {
     float pshift[3];     // map shift (voxels) still due on the particle positions, applied by predict
 };
 
+// ------------------------------------------------------------------ LiDAR ingest
+// dst = rot * src for n points (k3_lidar_rotate; src may equal dst).
+hipError_t launch_lidar_rotate(hipStream_t st, const float* rot9, const float* sx, const float* sy, const float* sz,
+                               float* dx, float* dy, float* dz, int n);
+// Evidence of a further LiDAR added to the running sums (acc += add, per
+// voxel); returns at once when the abort flag is set.
+hipError_t launch_add_evidence(hipStream_t st, const uint32_t* abort_flag, float* acc_occ, float* acc_free,
+                               const float* add_occ, const float* add_free, int V);
+
 // ------------------------------------------------------------------ MapPoint export
 // Geometry and field rules of one export (k3dom_export_mappoints).
 struct MapPointArgs {

@@ -649,6 +649,46 @@ hipError_t launch_mass_update(hipStream_t st, const uint32_t* abort_flag, VoxelA
     return hipGetLastError();
 }
 
+// ============================================================== LiDAR ingest
+struct Rot9 {
+    float r[9];
+};
+__global__ __launch_bounds__(TPB) void k_lidar_rotate(Rot9 R, const float* sx, const float* sy, const float* sz,
+                                                      float* dx, float* dy, float* dz, int n) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    float x, y, z;
+    k3_lidar_rotate(R.r, sx[i], sy[i], sz[i], &x, &y, &z);
+    dx[i] = x;
+    dy[i] = y;
+    dz[i] = z;
+}
+hipError_t launch_lidar_rotate(hipStream_t st, const float* rot9, const float* sx, const float* sy, const float* sz,
+                               float* dx, float* dy, float* dz, int n) {
+    if (n <= 0) return hipSuccess;
+    Rot9 R;
+    for (int k = 0; k < 9; ++k) R.r[k] = rot9[k];
+    hipLaunchKernelGGL(k_lidar_rotate, dim3(blocks_for(n)), dim3(TPB), 0, st, R, sx, sy, sz, dx, dy, dz, n);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(TPB) void k_add_evidence(const uint32_t* __restrict__ abort_flag,
+                                                      float* __restrict__ acc_occ, float* __restrict__ acc_free,
+                                                      const float* __restrict__ add_occ,
+                                                      const float* __restrict__ add_free, int V) {
+    if (*abort_flag) return;
+    const int c = blockIdx.x * TPB + threadIdx.x;
+    if (c >= V) return;
+    acc_occ[c] = acc_occ[c] + add_occ[c];
+    acc_free[c] = acc_free[c] + add_free[c];
+}
+hipError_t launch_add_evidence(hipStream_t st, const uint32_t* abort_flag, float* acc_occ, float* acc_free,
+                               const float* add_occ, const float* add_free, int V) {
+    hipLaunchKernelGGL(k_add_evidence, dim3(blocks_for(V)), dim3(TPB), 0, st, abort_flag, acc_occ, acc_free,
+                       add_occ, add_free, V);
+    return hipGetLastError();
+}
+
 // ============================================================== MapPoint export
 // The node's evaluation records (publish_mapdata, src/k3dom/k3dom_node.cpp:161-184,
 // src/dsphdmib/dsphdmib_node.cpp:153-177) with the GridCell accessors of

@@ -321,6 +321,19 @@ K3_HD void k3_voxel_center(int cell, int gs, int gsz, float res, float sx, float
     *z = (zi - (float)gsz / 2.0f) * res - sz;
 }
 
+// ---------------------------------------------------------------- LiDAR ingest
+// A point of a LiDAR (its own frame) in map-aligned axes relative to that
+// LiDAR: R p with R row-major (LiDAR axes to map axes), each row as
+// (r0 x + r1 y) + r2 z.  The reference's node does this with PCL on the host
+// for its one LiDAR (src/k3dom/k3dom_node.cpp:73-76); several LiDARs keep
+// their own origins (each one's evidence is computed with the sensor at its
+// origin), so no translation enters the point.
+K3_HD void k3_lidar_rotate(const float* R, float x, float y, float z, float* ox, float* oy, float* oz) {
+    *ox = R[0] * x + R[1] * y + R[2] * z;
+    *oy = R[3] * x + R[4] * y + R[5] * z;
+    *oz = R[6] * x + R[7] * y + R[8] * z;
+}
+
 // ---------------------------------------------------------------- mass rules
 // calc_credit, src/k3dom/kernel/mass_update.cu:12-14 (= tanh(mass/scale))
 K3_HD float k3_credit(float mass, float mass_scale) {

@@ -128,9 +128,20 @@ struct Scene {
     }
 
     int frame(int f, float* xs, float* ys, float* zs, int cap, double* pose_out) const {
-        double ex, ey, ez, t;
-        pose(f, &ex, &ey, &ez, &t);
-        pose_out[0] = ex; pose_out[1] = ey; pose_out[2] = ez; pose_out[3] = t;
+        const double mount[3] = {0.0, 0.0, 0.0};
+        return frame_from(f, mount, 0.0, xs, ys, zs, cap, pose_out);
+    }
+
+    // A LiDAR mounted at `mount` (metres, world axes, from the ego position)
+    // and turned by yaw (radians) about z: its points in its own frame
+    // (azimuth 0 along its x axis); pose_out = the ego position and time.
+    int frame_from(int f, const double* mount, double yaw, float* xs, float* ys, float* zs, int cap,
+                   double* pose_out) const {
+        double px, py, pz, t;
+        pose(f, &px, &py, &pz, &t);
+        pose_out[0] = px; pose_out[1] = py; pose_out[2] = pz; pose_out[3] = t;
+        const double ex = px + mount[0], ey = py + mount[1], ez = pz + mount[2];
+        const double cyaw = std::cos(yaw), syaw = std::sin(yaw);
         const double tr = f * (double)d.frame_dt;
         // boxes at this time, with the azimuth interval each one covers
         struct Live { double lx, ly, lz, ux, uy, uz; int a0, a1; bool all; };
@@ -150,8 +161,8 @@ struct Scene {
             L.a0 = 0; L.a1 = A - 1;
             if (!L.all) {
                 // angular interval of the four footprint corners (it spans < pi)
-                double ang[4] = {std::atan2(L.ly - ey, L.lx - ex), std::atan2(L.ly - ey, L.ux - ex),
-                                 std::atan2(L.uy - ey, L.lx - ex), std::atan2(L.uy - ey, L.ux - ex)};
+                double ang[4] = {std::atan2(L.ly - ey, L.lx - ex) - yaw, std::atan2(L.ly - ey, L.ux - ex) - yaw,
+                                 std::atan2(L.uy - ey, L.lx - ex) - yaw, std::atan2(L.uy - ey, L.ux - ex) - yaw};
                 double ref = ang[0], lo = 0.0, hi = 0.0;
                 for (int k = 1; k < 4; ++k) {
                     double dlt = std::remainder(ang[k] - ref, 2.0 * M_PI);
@@ -187,8 +198,10 @@ struct Scene {
 #pragma omp parallel for schedule(static)
         for (int a = 0; a < A; ++a) {
             double az = 2.0 * M_PI * a / A, ca = std::cos(az), sa = std::sin(az);
+            // world direction of the LiDAR's azimuth a (a yaw of 0 leaves it unchanged)
+            const double wca = yaw == 0.0 ? ca : cyaw * ca - syaw * sa, wsa = yaw == 0.0 ? sa : syaw * ca + cyaw * sa;
             for (int b = 0; b < NB; ++b) {
-                double dx = ce[b] * ca, dy = ce[b] * sa, dz = se[b];
+                double dx = ce[b] * wca, dy = ce[b] * wsa, dz = se[b];
                 double best = -1.0;
                 for (int i : cols[a]) {
                     const Live& L = live[i];
@@ -259,6 +272,12 @@ void k3scene_destroy(void* h) { delete (Scene*)h; }
 // frame (which may exceed cap).  pose_out = {x, y, z, t}.
 int k3scene_frame(void* h, int f, float* xs, float* ys, float* zs, int cap, double* pose_out) {
     return ((Scene*)h)->frame(f, xs, ys, zs, cap, pose_out);
+}
+// The same for a LiDAR mounted at mount[3] (from the ego, world axes) and
+// turned by yaw: points in the LiDAR's own frame.
+int k3scene_frame_lidar(void* h, int f, const double* mount, double yaw, float* xs, float* ys, float* zs, int cap,
+                        double* pose_out) {
+    return ((Scene*)h)->frame_from(f, mount, yaw, xs, ys, zs, cap, pose_out);
 }
 void k3scene_occupancy(void* h, int f, int gs, int gsz, float res, float cx, float cy, float cz, float size,
                        float size_z, uint8_t* out) {

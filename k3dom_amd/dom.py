@@ -114,6 +114,29 @@ class DOM:
         check(self._lib.k3dom_update_grid(self._h, float(t), _fptr(xs), _fptr(ys), _fptr(zs), len(xs)),
               self._h, "updateGrid")
 
+    def updateGridLidars(self, t, lidars, on_device=False):
+        """One frame from several LiDARs: lidars = [(rot 3x3, origin 3, xs,
+        ys, zs), ...] with points in each LiDAR's frame (numpy arrays, or
+        device pointers with their count when on_device: (rot, origin, dx,
+        dy, dz, n)) (k3dom_update_grid_lidars)."""
+        K = len(lidars)
+        arr = (_lib.Lidar * K)()
+        keep = []
+        for k, l in enumerate(lidars):
+            rot = np.asarray(l[0], np.float32).reshape(9)
+            org = np.asarray(l[1], np.float32).reshape(3)
+            arr[k].rot[:] = [float(v) for v in rot]
+            arr[k].origin[:] = [float(v) for v in org]
+            if on_device:
+                arr[k].xs, arr[k].ys, arr[k].zs, arr[k].n = l[2], l[3], l[4], int(l[5])
+            else:
+                pts = [np.ascontiguousarray(a, np.float32) for a in l[2:5]]
+                keep.append(pts)
+                arr[k].xs, arr[k].ys, arr[k].zs = (a.ctypes.data for a in pts)
+                arr[k].n = len(pts[0])
+        check(self._lib.k3dom_update_grid_lidars(self._h, float(t), arr, K, 1 if on_device else 0), self._h,
+              "update_grid_lidars")
+
     def updateGridDevice(self, t, d_xs, d_ys, d_zs, n):
         """Points already in device memory (integer device addresses)."""
         check(self._lib.k3dom_update_grid_device(self._h, float(t), C.c_void_p(d_xs), C.c_void_p(d_ys),

@@ -49,6 +49,9 @@ def _scene_lib():
         lib.k3scene_frame.restype = C.c_int
         lib.k3scene_frame.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                       C.POINTER(C.c_double)]
+        lib.k3scene_frame_lidar.restype = C.c_int
+        lib.k3scene_frame_lidar.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_double, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         lib.k3scene_occupancy.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                           C.c_float, C.c_float, C.c_float, C.c_void_p]
         _slib = lib
@@ -70,6 +73,25 @@ class Scene:
         pose = (C.c_double * 4)()
         n = self._lib.k3scene_frame(self._h, f, xs.ctypes.data, ys.ctypes.data, zs.ctypes.data, self.cap, pose)
         return pose[3], (pose[0], pose[1], pose[2]), xs[:n].copy(), ys[:n].copy(), zs[:n].copy()
+
+    def frame_lidars(self, f, rig):
+        """Frame f seen by a rig of LiDARs, rig = [(mount xyz from the ego in
+        metres, yaw in radians), ...]: returns (t, ego position, lidars) with
+        lidars = [(rot 3x3 float32 LiDAR-to-map axes, origin float32 = mount,
+        xs, ys, zs in the LiDAR's frame), ...]."""
+        out = []
+        pose = (C.c_double * 4)()
+        for mount, yaw in rig:
+            xs = np.empty(self.cap, np.float32)
+            ys = np.empty(self.cap, np.float32)
+            zs = np.empty(self.cap, np.float32)
+            m = (C.c_double * 3)(*mount)
+            n = self._lib.k3scene_frame_lidar(self._h, f, m, float(yaw), xs.ctypes.data, ys.ctypes.data,
+                                              zs.ctypes.data, self.cap, pose)
+            c, sn = np.cos(yaw), np.sin(yaw)
+            rot = np.array([[c, -sn, 0.0], [sn, c, 0.0], [0.0, 0.0, 1.0]], np.float32)
+            out.append((rot, np.asarray(mount, np.float32), xs[:n].copy(), ys[:n].copy(), zs[:n].copy()))
+        return pose[3], (pose[0], pose[1], pose[2]), out
 
     def occupancy(self, f, gs, gsz, res, center, size, size_z):
         out = np.empty(gs * gs * gsz, np.uint8)
@@ -170,11 +192,24 @@ def workload(cfg):
         s = workload(1).scene
         return Workload("differentiated config-1 scene, ls0=0.3", p, s, 8)
     if cfg == 5:
+        # 512x512x64 @0.1 m, 8,000,000 particles (+10% newborn), four 128x2048 LiDARs
+        # at the vehicle's corners at 90-degree yaw spacing, ego 15 m/s
         p = k3dom_params(0.1, 512, 64, 8_000_000, 800_000)
         s = _desc(seed=5, n_static=1000, n_moving=200, speed_max=15.0, xmin=-60, xmax=300, ymin=-60, ymax=60,
                   ground_z=-3.2, ego_vx=15.0, beams=128, az_bins=2048, max_range=40.0, range_noise=0.03)
-        return Workload("stress 512x512x64@0.1m N=8000000", p, s, 50)
+        return Workload("stress 512x512x64@0.1m N=8000000, 4 LiDARs", p, s, 50, extra={"rig": CFG5_RIG})
     raise ValueError(f"unknown configuration {cfg}")
+
+
+# four LiDARs at the vehicle's corners, 90 degrees apart in yaw (mount from the ego, yaw)
+CFG5_RIG = [((1.0, 0.0, 0.2), 0.0), ((0.0, 0.8, 0.2), np.pi / 2), ((-1.0, 0.0, 0.2), np.pi),
+            ((0.0, -0.8, 0.2), 1.5 * np.pi)]
+
+
+def small_rig(spread=0.5, n=3):
+    """n LiDARs on a circle of radius `spread` around the ego, yaws 360/n apart."""
+    return [((spread * np.cos(2 * np.pi * k / n), spread * np.sin(2 * np.pi * k / n), 0.1 * k), 2 * np.pi * k / n)
+            for k in range(n)]
 
 
 def batch_workloads(n_tiles=32, seed=4, edge_min=32, edge_max=128, particles_min=16_384, particles_max=1_048_576,

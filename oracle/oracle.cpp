@@ -13,6 +13,12 @@
 //   include/common/dom_types.h       GridCell fields and accessors
 //   src/dsphdmib/dom.cu, src/dsphdmib/kernel/{measurement_grid,mass_update}.cu
 //                                    the DS-PHD/MIB method (measurement grid and cell update)
+//   src/k3dom/k3dom_node.cpp:73-83   the LiDAR-to-map point transform, generalised to
+//                                    several LiDARs (no counterpart in the reference:
+//                                    each LiDAR's evidence is computed as the reference
+//                                    computes its one sensor's, with the sensor at that
+//                                    LiDAR's origin, and the LiDARs' evidence is added
+//                                    in LiDAR order)
 //
 // Deliberate, documented deviations from the reference (each also in the
 // repository design notes):
@@ -273,6 +279,14 @@ struct Oracle {
     double updated_time = -1.0;
     bool brute = false;
     int method = K3DOM_METHOD_K3DOM;
+    // LiDAR groups of the current frame (empty: one sensor at the pose):
+    // points [start, start + n) with their ray origin o (relative to the
+    // pose, map-aligned axes)
+    struct Group {
+        int start, n;
+        float o[3];
+    };
+    std::vector<Group> groups;
 
     explicit Oracle(const k3dom_params& prm) : p(prm) {
         gs = (int)(p.size / p.resolution);
@@ -366,11 +380,30 @@ struct Oracle {
     void mass_update(float decay, const float* xs, const float* ys, const float* zs, int n,
                      const std::vector<float>& wsorted) {
         auto t0 = clk::now();
-        Geom g{gs, gsz, p.resolution, sensor[0] - center[0], sensor[1] - center[1], sensor[2] - center[2]};
-        g.build();
-        del_occ.assign(V, 0.0f);
-        del_free.assign(V, 0.0f);
-        bgki_stats = bgki_all(g, xs, ys, zs, n, p.ls, p.sigma, brute, del_occ.data(), del_free.data());
+        if (groups.empty()) {
+            Geom g{gs, gsz, p.resolution, sensor[0] - center[0], sensor[1] - center[1], sensor[2] - center[2]};
+            g.build();
+            del_occ.assign(V, 0.0f);
+            del_free.assign(V, 0.0f);
+            bgki_stats = bgki_all(g, xs, ys, zs, n, p.ls, p.sigma, brute, del_occ.data(), del_free.data());
+        } else {
+            // one BGKI per LiDAR with the sensor at its origin, added in LiDAR order
+            std::vector<float> go(V), gf(V);
+            for (size_t k = 0; k < groups.size(); ++k) {
+                const Group& G = groups[k];
+                Geom g{gs, gsz, p.resolution, (sensor[0] + G.o[0]) - center[0], (sensor[1] + G.o[1]) - center[1],
+                       (sensor[2] + G.o[2]) - center[2]};
+                g.build();
+                float* to = k == 0 ? (del_occ.assign(V, 0.0f), del_occ.data()) : go.data();
+                float* tf = k == 0 ? (del_free.assign(V, 0.0f), del_free.data()) : gf.data();
+                bgki_all(g, xs + G.start, ys + G.start, zs + G.start, G.n, p.ls, p.sigma, brute, to, tf);
+                if (k > 0)
+                    for (int i = 0; i < V; ++i) {
+                        del_occ[i] = del_occ[i] + go[i];
+                        del_free[i] = del_free[i] + gf[i];
+                    }
+            }
+        }
         stage_ms[3] = ms_since(t0);
         t0 = clk::now();
         for (int i = 0; i < V; ++i) {
@@ -392,7 +425,11 @@ struct Oracle {
     // to del_occ / del_free, the arrays that hold the evidence in the K3DOM path.
     void ds_meas_grid(const float* xs, const float* ys, const float* zs, int n) {
         auto t0 = clk::now();
-        Geom g{gs, gsz, p.resolution, sensor[0] - center[0], sensor[1] - center[1], sensor[2] - center[2]};
+        // one LiDAR at most: with the sensor at its origin
+        const float o[3] = {groups.empty() ? 0.0f : groups[0].o[0], groups.empty() ? 0.0f : groups[0].o[1],
+                            groups.empty() ? 0.0f : groups[0].o[2]};
+        Geom g{gs, gsz, p.resolution, (sensor[0] + o[0]) - center[0], (sensor[1] + o[1]) - center[1],
+               (sensor[2] + o[2]) - center[2]};
         g.build();
         const float wr = p.resolution * 0.866f;
         const float w2 = wr * wr;
@@ -900,6 +937,34 @@ void orc_update_pose(void* h, float x, float y, float z) { ((Oracle*)h)->update_
 int orc_update_grid(void* h, double t, const float* xs, const float* ys, const float* zs, int n) {
     if (!h || n < 0 || (n > 0 && (!xs || !ys || !zs))) return K3DOM_ERR_INVALID_ARG;
     ((Oracle*)h)->update_grid(t, xs, ys, zs, n);
+    return K3DOM_OK;
+}
+
+// Several LiDARs: counts[k] points per LiDAR, concatenated in xs/ys/zs in
+// their own frames; rot: 9 floats per LiDAR (row-major, LiDAR to map axes);
+// origin: 3 floats per LiDAR (relative to the pose, map axes).
+int orc_update_grid_lidars(void* hh, double t, int n_lidars, const float* rot, const float* origin,
+                           const int* counts, const float* xs, const float* ys, const float* zs) {
+    Oracle* h = (Oracle*)hh;
+    if (!h || n_lidars < 1 || !rot || !origin || !counts) return K3DOM_ERR_INVALID_ARG;
+    if (h->method == K3DOM_METHOD_DSPHDMIB && n_lidars > 1) return K3DOM_ERR_INVALID_ARG;
+    int total = 0;
+    for (int k = 0; k < n_lidars; ++k) {
+        if (counts[k] < 0) return K3DOM_ERR_INVALID_ARG;
+        total += counts[k];
+    }
+    if (total > 0 && (!xs || !ys || !zs)) return K3DOM_ERR_INVALID_ARG;
+    std::vector<float> rx(total), ry(total), rz(total);
+    h->groups.clear();
+    int at = 0;
+    for (int k = 0; k < n_lidars; ++k) {
+        Oracle::Group G{at, counts[k], {origin[3 * k], origin[3 * k + 1], origin[3 * k + 2]}};
+        for (int q = at; q < at + counts[k]; ++q) k3_lidar_rotate(rot + 9 * k, xs[q], ys[q], zs[q], &rx[q], &ry[q], &rz[q]);
+        h->groups.push_back(G);
+        at += counts[k];
+    }
+    h->update_grid(t, rx.data(), ry.data(), rz.data(), total);
+    h->groups.clear();
     return K3DOM_OK;
 }
 

@@ -36,6 +36,8 @@ def lib():
             "orc_set_brute": (None, [C.c_void_p, C.c_int]),
             "orc_update_pose": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
             "orc_update_grid": (C.c_int, [C.c_void_p, C.c_double, fp, fp, fp, C.c_int]),
+            "orc_update_grid_lidars": (C.c_int, [C.c_void_p, C.c_double, C.c_int, fp, fp, C.POINTER(C.c_int),
+                                                 fp, fp, fp]),
             "orc_get_info": (None, [C.c_void_p, C.POINTER(Info)]),
             "orc_read_cells": (None, [C.c_void_p, C.POINTER(Cell)]),
             "orc_read_array": (C.c_int64, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
@@ -116,6 +118,19 @@ class OracleDOM:
         st = self._L.orc_update_grid(self._h, float(t), _fp(xs), _fp(ys), _fp(zs), len(xs))
         if st != 0:
             raise ValueError(f"orc_update_grid status {st}")
+
+    def updateGridLidars(self, t, lidars):
+        """lidars: sequence of (rot 3x3, origin 3, xs, ys, zs), points in each LiDAR's frame."""
+        K = len(lidars)
+        rot = np.ascontiguousarray(np.stack([np.asarray(l[0], np.float32).reshape(9) for l in lidars]), np.float32)
+        org = np.ascontiguousarray(np.stack([np.asarray(l[1], np.float32).reshape(3) for l in lidars]), np.float32)
+        counts = np.array([len(l[2]) for l in lidars], np.int32)
+        xs, ys, zs = (np.ascontiguousarray(np.concatenate([np.asarray(l[k], np.float32) for l in lidars]), np.float32)
+                      for k in (2, 3, 4))
+        st = self._L.orc_update_grid_lidars(self._h, float(t), K, _fp(rot), _fp(org),
+                                            counts.ctypes.data_as(C.POINTER(C.c_int)), _fp(xs), _fp(ys), _fp(zs))
+        if st != 0:
+            raise ValueError(f"orc_update_grid_lidars status {st}")
 
     def info(self):
         from k3dom_amd._lib import Info

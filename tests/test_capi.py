@@ -42,10 +42,10 @@ def test_exports_every_declared_function(lib):
 
 
 def test_struct_layout_matches_header(tmp_path, lib):
-    from k3dom_amd._lib import Params, Cell, Info
+    from k3dom_amd._lib import Params, Cell, Info, Lidar
     src = tmp_path / "lay.c"
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "k3dom.h"', "int main(void){"]
-    for cname, py in [("k3dom_params", Params), ("k3dom_cell", Cell), ("k3dom_info", Info)]:
+    for cname, py in [("k3dom_params", Params), ("k3dom_cell", Cell), ("k3dom_info", Info), ("k3dom_lidar", Lidar)]:
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
         for f, _ in py._fields_:
             lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
@@ -54,7 +54,7 @@ def test_struct_layout_matches_header(tmp_path, lib):
     exe = tmp_path / "lay"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
-    for cname, py in [("k3dom_params", Params), ("k3dom_cell", Cell), ("k3dom_info", Info)]:
+    for cname, py in [("k3dom_params", Params), ("k3dom_cell", Cell), ("k3dom_info", Info), ("k3dom_lidar", Lidar)]:
         assert int(got[cname]) == C.sizeof(py)
         for f, _ in py._fields_:
             assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
