@@ -13,7 +13,8 @@ kernel-inference gather, whose algorithmic work is its contributing
 against its time; `adjoint` times the differentiated window (forward +
 backward ms/frame against the plain forward, config-1 scene, ls 0.3, 8-frame
 windows); `batch` times 32 independent uneven map tiles advanced together;
-`dsphdmib` times the DS-PHD/MIB method on the same frames; `cpu_baseline`
+`dsphdmib` times the DS-PHD/MIB method on the same frames; `lidars` times the
+four-LiDAR stress configuration (512x512x64, 8M particles); `cpu_baseline`
 times the serial restatement on a bounded sample of the same workload on the
 host.
 """
@@ -216,6 +217,52 @@ def dsphdmib_leg(torch, w, frames, dpts, warmup=3, steps=20, breakdown=5):
     return res
 
 
+def lidars_leg(torch, warmup=3, steps=5):
+    """The stress configuration: 512x512x64 voxels at 0.1 m, 8,000,000
+    particles (+800,000 newborn), four 128x2048 LiDARs at the vehicle's
+    corners, 90 degrees apart in yaw, ego at 15 m/s; points in each LiDAR's
+    frame already in device memory, transformed and kept by LiDAR on the GPU
+    (k3dom_update_grid_lidars)."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, workload
+    w = workload(5)
+    sc = Scene(w.scene)
+    rig = w.extra["rig"]
+    F = warmup + steps
+    frames = [sc.frame_lidars(f, rig) for f in range(F + 2)]
+    dev = [[tuple(torch.from_numpy(a).cuda() for a in l[2:5]) for l in fr[2]] for fr in frames]
+    torch.cuda.synchronize()
+    d = DOM(w.params)
+
+    def step(f):
+        t, pose, lidars = frames[f]
+        d.updatePose(*pose)
+        d.updateGridLidars(t, [(l[0], l[1], x.data_ptr(), y.data_ptr(), z.data_ptr(), len(l[2]))
+                               for l, (x, y, z) in zip(lidars, dev[f])], on_device=True)
+
+    for f in range(warmup):
+        step(f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, F):
+        step(f)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    d.set_timing(True)
+    st = np.zeros(len(DOM.STAGES))
+    for f in range(F, F + 2):
+        step(f)
+        st += d.stage_times()
+    st /= 2
+    res = {"workload": w.name, "lidars": len(rig), "warmup": warmup, "steps": steps,
+           "points_per_frame": int(np.mean([sum(len(l[2]) for l in fr[2]) for fr in frames[warmup:F]])),
+           "frames_per_s": round(steps / el, 3), "ms_per_frame": round(el / steps * 1e3, 3),
+           "kernel_inference_ms": round(float(st[DOM.STAGES.index("lists")] + st[DOM.STAGES.index("bgki")]), 3),
+           "frame_gpu_ms": round(float(st[DOM.STAGES.index("frame")]), 3)}
+    d.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +274,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiated-window leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-tiles leg")
     ap.add_argument("--no-dsphdmib", action="store_true", help="skip the DS-PHD/MIB method leg")
+    ap.add_argument("--no-lidars", action="store_true", help="skip the four-LiDAR stress leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -356,6 +404,11 @@ def main():
         ds = dsphdmib_leg(torch, w, frames, dpts)
         log(f"DS-PHD/MIB: {ds}")
 
+    lid = None
+    if not args.no_lidars:
+        lid = lidars_leg(torch)
+        log(f"four LiDARs: {lid}")
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(w, frames)
@@ -384,6 +437,7 @@ def main():
         "adjoint": adj,
         "batch": bat,
         "dsphdmib": ds,
+        "lidars": lid,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -168,6 +168,14 @@ public:
         check(k3dom_read_cells(handle_, reinterpret_cast<k3dom_cell*>(grid_cell_array_host)), "copyCellsToHost");
     }
 
+    // One frame from several LiDARs (k3dom_update_grid_lidars): each with its
+    // points in its own frame, its rotation to the map's axes and its origin
+    // relative to the pose of updatePose; the transform runs on the GPU.
+    void updateGridLidars(double t, const std::vector<k3dom_lidar>& lidars, bool points_on_device = false) {
+        check(k3dom_update_grid_lidars(handle_, t, lidars.data(), (int32_t)lidars.size(), points_on_device ? 1 : 0),
+              "updateGridLidars");
+    }
+
     // The node's /map_data records (src/k3dom/k3dom_node.cpp:161-184,
     // src/dsphdmib/dsphdmib_node.cpp:153-177) built on the GPU: every voxel
     // in index order, or with visible = true only the voxels passing the

@@ -37,7 +37,8 @@ int main() {
     bool (dom::DOM_c::*save)(std::vector<unsigned char>&) = &dom::DOM_c::saveCheckpoint;
     bool (dom::DOM_c::*load)(const void*, size_t) = &dom::DOM_c::loadCheckpoint;
     void (dom::DOM_c::*mp)(std::vector<MapPoint>&, bool, float, float) = &dom::DOM_c::copyMapPointsToHost;
-    (void)upd; (void)pose; (void)p; (void)save; (void)load; (void)mp;
+    void (dom::DOM_c::*lid)(double, const std::vector<k3dom_lidar>&, bool) = &dom::DOM_c::updateGridLidars;
+    (void)upd; (void)pose; (void)p; (void)save; (void)load; (void)mp; (void)lid;
     return 0;
 }
 ''')
