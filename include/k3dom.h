@@ -382,6 +382,13 @@ k3dom_status k3dom_window_set_frame(k3dom_window* w, int32_t f, double t, const 
  * selects every voxel. */
 k3dom_status k3dom_window_set_mask(k3dom_window* w, const uint8_t* mask);
 
+/* Frame f of the window from several LiDARs (host points in each LiDAR's
+ * frame, as k3dom_update_grid_lidars): the frame's evidence and its
+ * length-scale derivative are the LiDARs' (each with the sensor at its
+ * origin), added in LiDAR order. */
+k3dom_status k3dom_window_set_frame_lidars(k3dom_window* w, int32_t f, double t, const float pose[3],
+                                          const k3dom_lidar* lidars, int32_t n_lidars, const uint8_t* gt_occ);
+
 /* Runs the frames from a fresh map with parameters theta[6]; writes the loss. */
 k3dom_status k3dom_window_forward(k3dom_window* w, const float theta[6], double* loss);
 

@@ -108,6 +108,8 @@ SIGNATURES = {
     "k3dom_window_set_frame": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _fp, _fp, _fp, _fp, C.c_int32,
                                          C.POINTER(C.c_uint8)]),
     "k3dom_window_set_mask": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8)]),
+    "k3dom_window_set_frame_lidars": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _fp, C.POINTER(Lidar), C.c_int32,
+                                                C.POINTER(C.c_uint8)]),
     "k3dom_window_forward": (C.c_int, [C.c_void_p, _fp, C.POINTER(C.c_double)]),
     "k3dom_window_backward": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), _fp]),
     "k3dom_window_read_record": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]),

@@ -20,6 +20,7 @@ import os
 
 import numpy as np
 
+from . import _lib
 from ._lib import Params, check, load
 
 THETA = ("ls", "sigma_occ", "sigma_free", "prior_free", "prior_static", "prior_dynamic")
@@ -100,6 +101,37 @@ class Window:
         for f, (t, pose, xs, ys, zs) in enumerate(recs):
             gt = scene.occupancy(first_frame + f, gs, gsz, p.resolution, centers[f], p.size, p.size_z)
             self.set_frame(f, t, pose, xs, ys, zs, gt)
+        return recs
+
+    def set_frame_lidars(self, f, t, pose, lidars, gt):
+        """Frame f from several LiDARs: lidars = [(rot 3x3, origin 3, xs, ys, zs), ...]
+        with points in each LiDAR's frame (k3dom_window_set_frame_lidars)."""
+        K = len(lidars)
+        arr = (_lib.Lidar * K)()
+        keep = []
+        for k, l in enumerate(lidars):
+            arr[k].rot[:] = [float(v) for v in np.asarray(l[0], np.float32).reshape(9)]
+            arr[k].origin[:] = [float(v) for v in np.asarray(l[1], np.float32).reshape(3)]
+            pts = [np.ascontiguousarray(a, np.float32) for a in l[2:5]]
+            keep.append(pts)
+            arr[k].xs, arr[k].ys, arr[k].zs = (a.ctypes.data for a in pts)
+            arr[k].n = len(pts[0])
+        pz = np.ascontiguousarray(pose, np.float32)
+        g = np.ascontiguousarray(gt, np.uint8)
+        assert g.size == self.V
+        self._check(self._lib.k3dom_window_set_frame_lidars(
+            self._h, int(f), float(t), pz.ctypes.data_as(C.POINTER(C.c_float)), arr, K,
+            g.ctypes.data_as(C.POINTER(C.c_uint8))), "set_frame_lidars")
+
+    def set_scene_rig(self, scene, rig, first_frame=0):
+        """set_scene for a rig of LiDARs (Scene.frame_lidars)."""
+        p = self.params
+        recs = [scene.frame_lidars(first_frame + f, rig) for f in range(self.frames)]
+        centers = map_centers(p, [r[1] for r in recs])
+        gs, _, gsz = self.grid
+        for f, (t, pose, lidars) in enumerate(recs):
+            gt = scene.occupancy(first_frame + f, gs, gsz, p.resolution, centers[f], p.size, p.size_z)
+            self.set_frame_lidars(f, t, pose, lidars, gt)
         return recs
 
     def set_mask(self, mask=None):

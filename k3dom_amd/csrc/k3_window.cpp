@@ -21,6 +21,11 @@ struct FrameIn {
     int n = 0;
     size_t off = 0;  // offset of the frame's x array in the point store (floats)
     bool set = false;
+    // several LiDARs (0: one sensor): points per LiDAR, consecutive in the
+    // frame's arrays (already rotated into map axes), and their origins
+    int ngroups = 0;
+    int gcount[K3DOM_MAX_LIDARS] = {};
+    float gorig[K3DOM_MAX_LIDARS][3] = {};
 };
 
 struct FrameOut {
@@ -53,6 +58,10 @@ struct k3dom_window {
     double* adj_partial = nullptr;
     FrameConsts* d_fc = nullptr;   // T: the forward frames' constants (device)
     FrameConsts* h_fc = nullptr;   // T: pinned host copies
+    FrameConsts* d_gfc = nullptr;  // T * K3DOM_MAX_LIDARS: per-LiDAR constants of grouped frames
+    FrameConsts* h_gfc = nullptr;
+    // evidence of LiDARs after the first (and its length-scale derivative) in the backward pass
+    float *g_o = nullptr, *g_f = nullptr, *g_lo = nullptr, *g_lf = nullptr;
     float theta[6] = {0, 0, 0, 0, 0, 0};
     bool forward_done = false;
     float ms[2] = {0, 0};
@@ -146,6 +155,8 @@ k3dom_status k3dom_window_create(const k3dom_params* params, int32_t frames, k3d
     WC(walloc(w, &w->adj_partial, (size_t)window_blocks(w->V) * 8));
     WC(walloc(w, &w->d_fc, (size_t)frames));
     WC(hipHostMalloc((void**)&w->h_fc, sizeof(FrameConsts) * frames, hipHostMallocDefault));
+    WC(walloc(w, &w->d_gfc, (size_t)frames * K3DOM_MAX_LIDARS));
+    WC(hipHostMalloc((void**)&w->h_gfc, sizeof(FrameConsts) * frames * K3DOM_MAX_LIDARS, hipHostMallocDefault));
 #undef WC
     *out = w;
     return K3DOM_OK;
@@ -155,6 +166,7 @@ k3dom_status k3dom_window_destroy(k3dom_window* w) {
     if (!w) return K3DOM_OK;
     if (w->h) (void)hipStreamSynchronize(w->h->stream);
     if (w->h_fc) (void)hipHostFree(w->h_fc);
+    if (w->h_gfc) (void)hipHostFree(w->h_gfc);
     k3dom_destroy(w->h);  // frees every device buffer of the window too
     delete w;
     return K3DOM_OK;
@@ -180,6 +192,7 @@ k3dom_status k3dom_window_set_frame(k3dom_window* w, int32_t f, double t, const 
             fi.t = t;
             for (int k = 0; k < 3; ++k) fi.pose[k] = pose[k];
             fi.set = true;
+            fi.ngroups = 0;
         } else if (fi.set) {
             pts.insert(pts.end(), w->host_pts.begin() + fi.off, w->host_pts.begin() + fi.off + 3 * (size_t)fi.n);
         }
@@ -189,6 +202,48 @@ k3dom_status k3dom_window_set_frame(k3dom_window* w, int32_t f, double t, const 
     w->pts_dirty = true;
     W_HIP(w, hipMemcpy(w->d_gt + (size_t)f * w->V, gt_occ, (size_t)w->V, hipMemcpyHostToDevice));
     w->forward_done = false;
+    return K3DOM_OK;
+}
+
+k3dom_status k3dom_window_set_frame_lidars(k3dom_window* w, int32_t f, double t, const float pose[3],
+                                          const k3dom_lidar* lidars, int32_t n_lidars, const uint8_t* gt_occ) {
+    if (!w) return K3DOM_ERR_INVALID_ARG;
+    if (!lidars || n_lidars < 1 || n_lidars > K3DOM_MAX_LIDARS)
+        return wfail(w, K3DOM_ERR_INVALID_ARG, "1 to K3DOM_MAX_LIDARS LiDARs");
+    long long total = 0;
+    for (int k = 0; k < n_lidars; ++k) {
+        if (lidars[k].n < 0 || (lidars[k].n > 0 && (!lidars[k].xs || !lidars[k].ys || !lidars[k].zs)))
+            return wfail(w, K3DOM_ERR_INVALID_ARG, "bad point arrays");
+        total += lidars[k].n;
+    }
+    if (total > 0x7fffffff) return wfail(w, K3DOM_ERR_INVALID_ARG, "too many points");
+    // the frame's points rotated into map axes, LiDAR after LiDAR (the same
+    // float expression as the ingest kernel)
+    const int n = (int)total;
+    std::vector<float> x(n), y(n), z(n);
+    int at = 0;
+    for (int k = 0; k < n_lidars; ++k) {
+        const k3dom_lidar& L = lidars[k];
+        for (int q = 0; q < L.n; ++q) k3_lidar_rotate(L.rot, L.xs[q], L.ys[q], L.zs[q], &x[at + q], &y[at + q], &z[at + q]);
+        at += L.n;
+    }
+    k3dom_status s = k3dom_window_set_frame(w, f, t, pose, x.data(), y.data(), z.data(), n, gt_occ);
+    if (s != K3DOM_OK) return s;
+    FrameIn& fi = w->in[f];
+    fi.ngroups = n_lidars;
+    for (int k = 0; k < n_lidars; ++k) {
+        fi.gcount[k] = lidars[k].n;
+        for (int i = 0; i < 3; ++i) fi.gorig[k][i] = lidars[k].origin[i];
+    }
+    if (n_lidars > 1 && !w->g_o) {
+        const size_t V = (size_t)w->V;
+        W_HIP(w, walloc(w, &w->g_o, V));
+        W_HIP(w, walloc(w, &w->g_f, V));
+        W_HIP(w, walloc(w, &w->g_lo, V));
+        W_HIP(w, walloc(w, &w->g_lf, V));
+        W_HIP(w, walloc(w, &w->h->gdel_occ, V));
+        W_HIP(w, walloc(w, &w->h->gdel_free, V));
+    }
     return K3DOM_OK;
 }
 
@@ -241,8 +296,17 @@ k3dom_status k3dom_window_forward(k3dom_window* w, const float theta[6], double*
         }
         if ((s = from_handle(w, ensure_points(h, std::max(fi.n, 1)))) != K3DOM_OK) return s;
         const float* px = w->d_pts + fi.off;
-        if ((s = from_handle(w, frame(h, fi.t, px, px + fi.n, px + 2 * fi.n, fi.n))) != K3DOM_OK) return s;
+        h->ngroups = fi.ngroups;
+        for (int g = 0, at = 0; g < fi.ngroups; at += fi.gcount[g], ++g) {
+            h->gstart[g] = at;
+            h->gcount[g] = fi.gcount[g];
+            for (int k = 0; k < 3; ++k) h->gorig[g][k] = fi.gorig[g][k];
+        }
+        s = from_handle(w, frame(h, fi.t, px, px + fi.n, px + 2 * fi.n, fi.n));
+        h->ngroups = 0;
+        if (s != K3DOM_OK) return s;
         w->h_fc[f] = *h->h_fc;
+        for (int g = 0; g < fi.ngroups; ++g) w->h_gfc[(size_t)f * K3DOM_MAX_LIDARS + g] = h->h_gfc[g];
         W_HIP(w, launch_window_record(h->stream, w->V, h->gs, h->gsz, fo.normal, h->p.particle_min_vel,
                                       h->vox[h->vcur], h->P[1], h->NB, h->bnd, w->rec[f], w->d_gt + (size_t)f * w->V,
                                       w->use_mask ? w->d_mask : nullptr, w->loss_partial + (size_t)f * nb));
@@ -275,11 +339,38 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
         W_HIP(w, hipMemcpyAsync(w->d_fc + f, w->h_fc + f, sizeof(FrameConsts), hipMemcpyHostToDevice, h->stream));
         W_HIP(w, launch_reset_scalars(h->stream, h->sc));
         const float* px = w->d_pts + fi.off;
-        if ((s = from_handle(w, build_lists(h, fc, px, px + fi.n, px + 2 * fi.n))) != K3DOM_OK) return s;
-        W_HIP(w, launch_bgki_grad(h->stream, h->gather_blocks, fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
-                                  h->ibuf, h->lbuf[2], h->rec, w->del_o, w->del_f, w->dls_o, w->dls_f, h->sc));
-        W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_fc[f].g, h->tg, h->ibuf, w->del_o, w->del_f,
-                                          w->dls_o, w->dls_f, h->sc));
+        if (fi.ngroups == 0) {
+            if ((s = from_handle(w, build_lists(h, fc, px, px + fi.n, px + 2 * fi.n))) != K3DOM_OK) return s;
+            W_HIP(w, launch_bgki_grad(h->stream, h->gather_blocks, fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
+                                      h->ibuf, h->lbuf[2], h->rec, w->del_o, w->del_f, w->dls_o, w->dls_f, h->sc));
+            W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_fc[f].g, h->tg, h->ibuf, w->del_o,
+                                              w->del_f, w->dls_o, w->dls_f, h->sc));
+        } else {
+            // one pass per LiDAR with the sensor at its origin, added in LiDAR order
+            const size_t g0 = (size_t)f * K3DOM_MAX_LIDARS;
+            W_HIP(w, hipMemcpyAsync(w->d_gfc + g0, w->h_gfc + g0, sizeof(FrameConsts) * fi.ngroups,
+                                    hipMemcpyHostToDevice, h->stream));
+            for (int g = 0, at = 0; g < fi.ngroups; at += fi.gcount[g], ++g) {
+                const FrameConsts* gfc = w->d_gfc + g0 + g;
+                const float* gx = px + at;
+                if ((s = from_handle(w, build_lists(h, gfc, gx, gx + fi.n, gx + 2 * fi.n))) != K3DOM_OK) return s;
+                if (g > 0) W_HIP(w, launch_reset_item_counter(h->stream, h->sc));
+                float* to = g == 0 ? w->del_o : w->g_o;
+                float* tf = g == 0 ? w->del_f : w->g_f;
+                float* tlo = g == 0 ? w->dls_o : w->g_lo;
+                float* tlf = g == 0 ? w->dls_f : w->g_lf;
+                W_HIP(w, launch_bgki_grad(h->stream, h->gather_blocks, gfc, h->tg, h->p.ls, h->sigma_occ,
+                                          h->sigma_free, h->ibuf, h->lbuf[2], h->rec, to, tf, tlo, tlf, h->sc));
+                W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_gfc[g0 + g].g, h->tg, h->ibuf, to, tf,
+                                                  tlo, tlf, h->sc));
+                if (g > 0) {
+                    W_HIP(w, launch_add_evidence(h->stream, &h->sc->overflow, w->del_o, w->del_f, w->g_o, w->g_f,
+                                                 w->V));
+                    W_HIP(w, launch_add_evidence(h->stream, &h->sc->overflow, w->dls_o, w->dls_f, w->g_lo, w->g_lf,
+                                                 w->V));
+                }
+            }
+        }
         AdjointFrameArgs a{};
         a.V = w->V;
         a.gs = h->gs;

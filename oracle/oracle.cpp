@@ -793,6 +793,12 @@ struct WindowInput {
     const int32_t* counts;    // T * 4 * V: n, m, nb, mb
     const uint8_t* gt;        // T * V
     const uint8_t* mask;      // V or null
+    // several LiDARs per frame (null: one sensor): group count per frame, the
+    // points per group (T * K3DOM_MAX_LIDARS, consecutive within the frame)
+    // and each group's sensor position relative to the map centre (3 per group)
+    const int32_t* ngroups = nullptr;
+    const int32_t* gcount = nullptr;
+    const float* gsrel = nullptr;
 };
 
 template <class X>
@@ -805,17 +811,42 @@ static X kern_x(double dd, const X& ls, const X& sigma) {
 }
 
 template <class X>
+static void bgki_x_points(const WindowInput& in, const Geom& g, const float* xs, const float* ys, const float* zs,
+                          int n, const X& ls, const X& so, const X& sf, std::vector<X>& dO, std::vector<X>& dF);
+
+// Evidence of frame t (per LiDAR group with its sensor position, added)
+template <class X>
 static void bgki_x(const WindowInput& in, int t, const X& ls, const X& so, const X& sf, std::vector<X>& dO,
                    std::vector<X>& dF) {
-    const int gs = in.gs, gsz = in.gsz;
-    Geom g{gs, gsz, in.p->resolution, in.sensor_rel[3 * t], in.sensor_rel[3 * t + 1], in.sensor_rel[3 * t + 2]};
-    g.build();
-    const double lsv = k3g_val(ls);
-    const double R = lsv * (1.0 + 1e-5) + 1e-4;
     size_t off = 0;
     for (int f = 0; f < t; ++f) off += 3 * (size_t)in.npts[f];
     const int n = in.npts[t];
     const float *xs = in.pts + off, *ys = xs + n, *zs = ys + n;
+    const int K = in.ngroups ? in.ngroups[t] : 0;
+    if (K == 0) {
+        Geom g{in.gs, in.gsz, in.p->resolution, in.sensor_rel[3 * t], in.sensor_rel[3 * t + 1],
+               in.sensor_rel[3 * t + 2]};
+        g.build();
+        bgki_x_points(in, g, xs, ys, zs, n, ls, so, sf, dO, dF);
+        return;
+    }
+    int at = 0;
+    for (int k = 0; k < K; ++k) {
+        const float* sr = in.gsrel + 3 * ((size_t)t * K3DOM_MAX_LIDARS + k);
+        Geom g{in.gs, in.gsz, in.p->resolution, sr[0], sr[1], sr[2]};
+        g.build();
+        const int c = in.gcount[(size_t)t * K3DOM_MAX_LIDARS + k];
+        bgki_x_points(in, g, xs + at, ys + at, zs + at, c, ls, so, sf, dO, dF);
+        at += c;
+    }
+}
+
+template <class X>
+static void bgki_x_points(const WindowInput& in, const Geom& g, const float* xs, const float* ys, const float* zs,
+                          int n, const X& ls, const X& so, const X& sf, std::vector<X>& dO, std::vector<X>& dF) {
+    const int gs = in.gs, gsz = in.gsz;
+    const double lsv = k3g_val(ls);
+    const double R = lsv * (1.0 + 1e-5) + 1e-4;
     const double res = in.p->resolution;
     for (int pi = 0; pi < n; ++pi) {
         const K3PointRec pr = k3_point_prep(xs[pi], ys[pi], zs[pi], in.p->resolution);
@@ -1064,13 +1095,15 @@ int64_t orc_read_counts(void* hh, int32_t* dst, int64_t capacity) {
 
 // Loss of a window in double (grad == NULL), or loss and its gradient by
 // forward-mode differentiation (grad[6]).
-int orc_window_chain64(const k3dom_params* p, int32_t T, const int32_t* npts, const float* pts,
-                       const float* sensor_rel, const int32_t* move, const float* decay, const int32_t* counts,
-                       const uint8_t* gt, const uint8_t* mask, const double* theta, double* loss, double* grad) {
+int orc_window_chain64_groups(const k3dom_params* p, int32_t T, const int32_t* npts, const float* pts,
+                              const float* sensor_rel, const int32_t* move, const float* decay, const int32_t* counts,
+                              const uint8_t* gt, const uint8_t* mask, const int32_t* ngroups, const int32_t* gcount,
+                              const float* gsrel, const double* theta, double* loss, double* grad) {
     if (!p || T <= 0 || !npts || !pts || !sensor_rel || !move || !decay || !counts || !gt || !theta || !loss)
         return K3DOM_ERR_INVALID_ARG;
+    if (ngroups && (!gcount || !gsrel)) return K3DOM_ERR_INVALID_ARG;
     WindowInput in{p, T, (int)(p->size / p->resolution), (int)(p->size_z / p->resolution), npts, pts, sensor_rel,
-                   move, decay, counts, gt, mask};
+                   move, decay, counts, gt, mask, ngroups, gcount, gsrel};
     if (!grad) {
         double th[6];
         for (int k = 0; k < 6; ++k) th[k] = theta[k];
@@ -1084,6 +1117,12 @@ int orc_window_chain64(const k3dom_params* p, int32_t T, const int32_t* npts, co
     *loss = L.v;
     for (int k = 0; k < 6; ++k) grad[k] = L.d[k];
     return K3DOM_OK;
+}
+int orc_window_chain64(const k3dom_params* p, int32_t T, const int32_t* npts, const float* pts,
+                       const float* sensor_rel, const int32_t* move, const float* decay, const int32_t* counts,
+                       const uint8_t* gt, const uint8_t* mask, const double* theta, double* loss, double* grad) {
+    return orc_window_chain64_groups(p, T, npts, pts, sensor_rel, move, decay, counts, gt, mask, nullptr, nullptr,
+                                     nullptr, theta, loss, grad);
 }
 // kernel at distance d (evaluated from the float32 square d * d)
 float orc_kernel(float d, float ls, float sigma) { return k3_kernel_value_sq(d * d, k3_kernel_scale(ls), sigma); }

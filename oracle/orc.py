@@ -54,6 +54,12 @@ def lib():
                                              C.POINTER(C.c_int32), fp, C.POINTER(C.c_int32),
                                              C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(C.c_double),
                                              C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "orc_window_chain64_groups": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_int32), fp, fp,
+                                                    C.POINTER(C.c_int32), fp, C.POINTER(C.c_int32),
+                                                    C.POINTER(C.c_uint8), C.POINTER(C.c_uint8),
+                                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32), fp,
+                                                    C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                    C.POINTER(C.c_double)]),
             "orc_credit": (C.c_float, [C.c_float, C.c_float]),
             "orc_box_muller": (None, [C.c_uint32, C.c_uint32, fp, fp]),
             "orc_update_mass": (None, [fp, C.c_float, C.c_float, C.c_float]),
@@ -215,7 +221,9 @@ class WindowChain64:
     the frozen decisions of a forward run: loss(theta) and, by forward-mode
     differentiation, grad(theta)."""
 
-    def __init__(self, params, frames, sensor_rel, moves, decays, counts, gts, mask=None):
+    def __init__(self, params, frames, sensor_rel, moves, decays, counts, gts, mask=None, groups=None):
+        """groups (several LiDARs): (ngroups[T], gcount[T, 16], gsrel[T, 16, 3])
+        with each group's points consecutive within its frame."""
         from k3dom_amd._lib import Params
         self.params = params if isinstance(params, Params) else Params.from_buffer_copy(bytes(params))
         self.T = len(frames)
@@ -228,17 +236,26 @@ class WindowChain64:
         self.counts = np.ascontiguousarray(np.asarray(counts, np.int32).reshape(-1))
         self.gts = np.ascontiguousarray(np.asarray(gts, np.uint8).reshape(-1))
         self.mask = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+        self.groups = None
+        if groups is not None:
+            ng, gc, gr = groups
+            self.groups = (np.ascontiguousarray(ng, np.int32), np.ascontiguousarray(gc, np.int32).reshape(-1),
+                           np.ascontiguousarray(gr, np.float32).reshape(-1))
 
     def _call(self, theta, grad):
         th = np.ascontiguousarray(theta, np.float64)
         loss = C.c_double()
         g = (C.c_double * 6)() if grad else None
         u8 = C.POINTER(C.c_uint8)
-        st = lib().orc_window_chain64(
-            C.byref(self.params), self.T, self.npts.ctypes.data_as(C.POINTER(C.c_int32)), _fp(self.pts),
-            _fp(self.sensor_rel), self.moves.ctypes.data_as(C.POINTER(C.c_int32)), _fp(self.decays),
-            self.counts.ctypes.data_as(C.POINTER(C.c_int32)), self.gts.ctypes.data_as(u8),
+        i32 = C.POINTER(C.c_int32)
+        gr = self.groups
+        st = lib().orc_window_chain64_groups(
+            C.byref(self.params), self.T, self.npts.ctypes.data_as(i32), _fp(self.pts),
+            _fp(self.sensor_rel), self.moves.ctypes.data_as(i32), _fp(self.decays),
+            self.counts.ctypes.data_as(i32), self.gts.ctypes.data_as(u8),
             None if self.mask is None else self.mask.ctypes.data_as(u8),
+            None if gr is None else gr[0].ctypes.data_as(i32), None if gr is None else gr[1].ctypes.data_as(i32),
+            None if gr is None else _fp(gr[2]),
             th.ctypes.data_as(C.POINTER(C.c_double)), C.byref(loss), g)
         if st != 0:
             raise ValueError(f"orc_window_chain64 status {st}")

@@ -181,3 +181,103 @@ def test_window_torch_autograd_and_per_voxel_priors(gpu):
     assert np.allclose(th.grad.numpy().astype(np.float64), g.astype(np.float32), rtol=1e-6, atol=0)
     for k in range(3):
         assert abs(pv[k].astype(np.float64).sum() - g[3 + k]) <= 1e-4 * abs(g[3 + k]) + 1e-6
+
+
+# ------------------------------------------------------------ several LiDARs
+
+def _rotate32(rot, xs, ys, zs):
+    """k3_lidar_rotate in float32: each row as (r0 x + r1 y) + r2 z."""
+    r = np.asarray(rot, np.float32)
+    xs, ys, zs = (np.asarray(a, np.float32) for a in (xs, ys, zs))
+    return tuple((r[i, 0] * xs + r[i, 1] * ys) + r[i, 2] * zs for i in range(3))
+
+
+def _chain_from_gpu_rig(win, w, rig, frames, mask=None):
+    """The double chain of a rig window: points rotated as the GPU does,
+    each LiDAR's sensor position (pose + origin) - centre as the frame uses."""
+    from k3dom_amd.adjoint import map_centers
+    from k3dom_amd.scene import Scene
+    from oracle.orc import WindowChain64
+    p = w.params
+    sc = Scene(w.scene)
+    recs = [sc.frame_lidars(f, rig) for f in range(frames)]
+    centers = map_centers(p, [r[1] for r in recs])
+    K = len(rig)
+    pts, srel, decays, prev = [], [], [], None
+    ng = np.full(frames, K, np.int32)
+    gc = np.zeros((frames, 16), np.int32)
+    gr = np.zeros((frames, 16, 3), np.float32)
+    for f, (t, pose, lidars) in enumerate(recs):
+        rot = [_rotate32(l[0], *l[2:5]) for l in lidars]
+        pts.append(tuple(np.concatenate([r[k] for r in rot]) for k in range(3)))
+        s, c = np.float32(pose), np.float32(centers[f])
+        srel.append(s - c)
+        for k, l in enumerate(lidars):
+            gc[f, k] = len(l[2])
+            gr[f, k] = (s + np.asarray(l[1], np.float32)) - c
+        decays.append(1.0 if prev is None else float(np.float32(float(np.float32(p.gamma)) ** float(np.float32(t - prev)))))
+        prev = t
+    gs = win.grid[0]
+    gts = [sc.occupancy(f, gs, win.grid[2], p.resolution, centers[f], p.size, p.size_z) for f in range(frames)]
+    moves = [win.record(f, "move") for f in range(frames)]
+    counts = [win.record(f, "counts") for f in range(frames)]
+    return WindowChain64(p, pts, srel, moves, decays, counts, gts, mask, groups=(ng, gc, gr))
+
+
+def test_chain_single_identity_group_is_the_plain_chain(oracle_lib):
+    """One LiDAR at the pose with the identity: the grouped chain is the plain one."""
+    from k3dom_amd.adjoint import theta_of
+    from oracle.orc import WindowChain64
+    w = _small(frames=4, seed=7, ego_vx=4.0)
+    chain, _ = oracle_window(w, 4)
+    T = chain.T
+    ng = np.ones(T, np.int32)
+    gc = np.zeros((T, 16), np.int32)
+    gc[:, 0] = chain.npts
+    gr = np.zeros((T, 16, 3), np.float32)
+    gr[:, 0] = chain.sensor_rel.reshape(T, 3)
+    g = WindowChain64(chain.params, [], [], [], [], [], [])
+    g.__dict__.update({k: v for k, v in chain.__dict__.items()})
+    g.groups = (ng, gc.reshape(-1), gr.reshape(-1))
+    th = theta_of(w.params).astype(np.float64)
+    assert g.loss(th) == chain.loss(th)
+
+
+@pytest.mark.gpu
+def test_window_rig_record_and_gradient(gpu):
+    """A window over three LiDARs: its forward frames equal the serial
+    restatement's multi-LiDAR frames bit for bit, and its gradient matches the
+    FP64 chain with the LiDAR groups within 1e-3."""
+    from k3dom_amd.adjoint import Window, theta_of
+    from k3dom_amd.scene import Scene, small_rig
+    from oracle.orc import OracleDOM
+    from tests._parity import bits_equal
+    w = _small(frames=4, seed=11, ego_vx=4.0)
+    rig = small_rig(0.4, 3)
+    print("window gradient: small scene, 3 LiDARs, 4 frames", flush=True)
+    win = Window(w.params, 4)
+    recs = win.set_scene_rig(Scene(w.scene), rig)
+    th = theta_of(w.params)
+    th[1], th[2] = 0.12, 0.08
+    L = win.forward(th)
+    # the forward frames at the map's own parameters against the restatement
+    od = OracleDOM(w.params)
+    win0 = Window(w.params, 4)
+    win0.set_scene_rig(Scene(w.scene), rig)
+    win0.forward(theta_of(w.params))
+    for f, (t, pose, lidars) in enumerate(recs):
+        od.updatePose(*pose)
+        od.updateGridLidars(t, lidars)
+        cells = od.copyCellsToHost()
+        m = win0.record(f, "masses")
+        for k, name in enumerate(("free_mass", "static_mass", "dynamic_mass")):
+            assert bits_equal(m[k], cells[name]), (f, name)
+    win0.close()
+    g = win.backward()
+    chain = _chain_from_gpu_rig(win, w, rig, 4)
+    L64, g64 = chain.grad(th.astype(np.float64))
+    fd = chain.central_differences(th.astype(np.float64), rel_step=1e-6)
+    print("grad", g, "fp64", g64, "fd", fd, flush=True)
+    assert abs(L - L64) <= 1e-5 * abs(L64)
+    assert np.all(_rel_err(g, g64) <= 1e-3), (g, g64)
+    assert np.all(_rel_err(g, fd) <= 1e-3), (g, fd)
