@@ -14,7 +14,8 @@ against its time; `adjoint` times the differentiated window (forward +
 backward ms/frame against the plain forward, config-1 scene, ls 0.3, 8-frame
 windows); `batch` times 32 independent uneven map tiles advanced together;
 `dsphdmib` times the DS-PHD/MIB method on the same frames; `lidars` times the
-four-LiDAR stress configuration (512x512x64, 8M particles); `cpu_baseline`
+four-LiDAR stress configuration (512x512x64, 8M particles) and its
+differentiated window (fwd+bwd against plain forward); `cpu_baseline`
 times the serial restatement on a bounded sample of the same workload on the
 host.
 """
@@ -217,6 +218,56 @@ def dsphdmib_leg(torch, w, frames, dpts, warmup=3, steps=20, breakdown=5):
     return res
 
 
+def adjoint_lidars_leg(torch, T=8, reps=2):
+    """fwd+bwd ms/frame of the differentiated window against the plain
+    forward on the stress configuration: 512x512x64 voxels, 8M particles,
+    four 128x2048 LiDARs (their evidence added per frame), 8-frame windows,
+    loss over every voxel.  Plain forward: the same 8 frames from a fresh map
+    (points in device memory)."""
+    from k3dom_amd.adjoint import Window, theta_of
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, workload
+    w = workload(5)
+    rig = w.extra["rig"]
+    win = Window(w.params, T)
+    recs = win.set_scene_rig(Scene(w.scene), rig)
+    dev = [[tuple(torch.from_numpy(a).cuda() for a in l[2:5]) for l in r[2]] for r in recs]
+    th = theta_of(w.params)
+
+    def plain():
+        d = DOM(w.params)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for (t, pose, lidars), dl in zip(recs, dev):
+            d.updatePose(*pose)
+            d.updateGridLidars(t, [(l[0], l[1], x.data_ptr(), y.data_ptr(), z.data_ptr(), len(l[2]))
+                                   for l, (x, y, z) in zip(lidars, dl)], on_device=True)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        d.close()
+        return el
+
+    plain()
+    win.forward(th)
+    win.backward()
+    tp, tf, tb = [], [], []
+    for _ in range(reps):
+        tp.append(plain())
+        t0 = time.perf_counter()
+        win.forward(th)
+        t1 = time.perf_counter()
+        g = win.backward()
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+    win.close()
+    p_ms, f_ms, b_ms = (np.median(x) / T * 1e3 for x in (tp, tf, tb))
+    return {"workload": w.name, "lidars": len(rig), "frames": T, "reps": reps,
+            "plain_fwd_ms_per_frame": round(p_ms, 3), "fwd_ms_per_frame": round(f_ms, 3),
+            "bwd_ms_per_frame": round(b_ms, 3), "fwd_bwd_vs_plain": round((f_ms + b_ms) / p_ms, 3),
+            "grad": [float(x) for x in g]}
+
+
 def lidars_leg(torch, warmup=3, steps=5):
     """The stress configuration: 512x512x64 voxels at 0.1 m, 8,000,000
     particles (+800,000 newborn), four 128x2048 LiDARs at the vehicle's
@@ -408,6 +459,9 @@ def main():
     if not args.no_lidars:
         lid = lidars_leg(torch)
         log(f"four LiDARs: {lid}")
+        if not args.no_adjoint:
+            lid["adjoint"] = adjoint_lidars_leg(torch)
+            log(f"four LiDARs, differentiated window: fwd+bwd / plain = {lid['adjoint']['fwd_bwd_vs_plain']}")
 
     cpu = None
     if not args.no_cpu_baseline:

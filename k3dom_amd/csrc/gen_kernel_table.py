@@ -11,6 +11,8 @@ with f = N u - i in [0, 1)
 interpolates kbar(sqrt(u)) at f = 0, 1/2, 1 (values in double, coefficients
 rounded to float32).  Entry N is zero, so every u >= 1 evaluates to exactly 0.
 The float32 evaluation (two fmaf) stays within 8e-8 of kbar over [0, 1).
+A second table holds G(u) = -q kbar'(q) (the length-scale derivative factor of
+the differentiated window) with the same intervals and interpolation.
 Run: python3 k3dom_amd/csrc/gen_kernel_table.py > k3dom_amd/csrc/k3_kernel_table.h
 """
 import numpy as np
@@ -30,6 +32,32 @@ def table(n=N):
     ym = kbar(np.sqrt(u0 + h / 2))
     y1 = kbar(np.sqrt(u0 + h))
     y1[-1] = 0.0  # kbar(1) = 0
+    c = 2 * y0 - 4 * ym + 2 * y1
+    b = y1 - y0 - c
+    t = np.zeros((n + 1, 4), np.float32)
+    t[:n, 0] = y0
+    t[:n, 1] = b
+    t[:n, 2] = c
+    return t
+
+
+def dkbar(q):
+    """kbar'(q) in closed form."""
+    return (-(2 * np.pi / 3) * np.sin(2 * np.pi * q) * (1 - q) - (2 + np.cos(2 * np.pi * q)) / 3
+            + np.cos(2 * np.pi * q) * np.pi / PI_T)
+
+
+def gls(u):
+    """G(u) = -q kbar'(q), q = sqrt(u): d/dls [sigma kbar(d / ls)] = sigma G(d^2 / ls^2) / ls."""
+    q = np.sqrt(u)
+    return -q * dkbar(q)
+
+
+def table_dls(n=N):
+    """G(u) tabulated like kbar: same intervals, same interpolation, entry n zero."""
+    h = 1.0 / n
+    u0 = np.arange(n) * h
+    y0, ym, y1 = gls(u0), gls(u0 + h / 2), gls(u0 + h)
     c = 2 * y0 - 4 * ym + 2 * y1
     b = y1 - y0 - c
     t = np.zeros((n + 1, 4), np.float32)
@@ -61,5 +89,13 @@ if __name__ == "__main__":
     print(f"#define K3_KTAB_N {N}")
     print(f"static constexpr float k3_ktab[{(N + 1) * 4}] = {{")
     for row in t:
+        print("    " + ", ".join(f"{float(v)!r}f" for v in row) + ",")
+    print("};")
+    g = table_dls()
+    gerr = np.abs(evaluate(g, u).astype(np.float64) - gls(u.astype(np.float64)))
+    print(f"// Length-scale derivative factor G(u) = -q kbar'(q), q = sqrt(u), tabulated the")
+    print(f"// same way (d/dls [sigma kbar(d/ls)] = sigma G(u) / ls).  Max |float32 evaluation - G|: {gerr.max():.2e}")
+    print(f"static constexpr float k3_ktab_dls[{(N + 1) * 4}] = {{")
+    for row in g:
         print("    " + ", ".join(f"{float(v)!r}f" for v in row) + ",")
     print("};")

@@ -180,19 +180,40 @@ __global__ __launch_bounds__(TPB) void k_window_adjoint(AdjointFrameArgs a, cons
 
 // Sums the block partials in block order: acc[0] becomes lambda_W of the
 // previous frame, acc[1..6] accumulate the parameter gradients.
-__global__ __launch_bounds__(64) void k_window_reduce(int nblocks, const double* __restrict__ partial, double* acc) {
-    const int k = threadIdx.x;
-    if (k >= ADJ_K) return;
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += partial[(size_t)b * ADJ_K + k];
-    if (k == 0) acc[0] = s;
-    else acc[k] += s;
+// Sum of the per-block partials in a fixed order (deterministic): thread t
+// adds blocks t, t + RED_T, ... in turn, then a tree over the threads.
+constexpr int RED_T = 256;
+__global__ __launch_bounds__(RED_T) void k_window_reduce(int nblocks, const double* __restrict__ partial,
+                                                         double* acc) {
+    __shared__ double sh[ADJ_K][RED_T];
+    const int t = threadIdx.x;
+    double s[ADJ_K];
+#pragma unroll
+    for (int k = 0; k < ADJ_K; ++k) s[k] = 0.0;
+    for (int b = t; b < nblocks; b += RED_T)
+#pragma unroll
+        for (int k = 0; k < ADJ_K; ++k) s[k] += partial[(size_t)b * ADJ_K + k];
+#pragma unroll
+    for (int k = 0; k < ADJ_K; ++k) sh[k][t] = s[k];
+    __syncthreads();
+    for (int w = RED_T / 2; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int k = 0; k < ADJ_K; ++k) sh[k][t] += sh[k][t + w];
+        __syncthreads();
+    }
+    if (t < ADJ_K) {
+        if (t == 0)
+            acc[0] = sh[0][0];
+        else
+            acc[t] += sh[t][0];
+    }
 }
 
 hipError_t launch_window_adjoint(hipStream_t st, const AdjointFrameArgs& a, double* acc, double* partial) {
     const unsigned nb = blocks_for(a.V);
     hipLaunchKernelGGL(k_window_adjoint, dim3(nb), dim3(TPB), 0, st, a, acc, partial);
-    hipLaunchKernelGGL(k_window_reduce, dim3(1), dim3(64), 0, st, (int)nb, partial, acc);
+    hipLaunchKernelGGL(k_window_reduce, dim3(1), dim3(RED_T), 0, st, (int)nb, partial, acc);
     return hipGetLastError();
 }
 

@@ -492,18 +492,30 @@ struct BgTerm {
 // The table in LDS as three arrays (a, b, c coefficients): three 4-byte
 // reads per lookup, lanes in different banks unless their indices differ by a
 // multiple of the bank count.
-struct BgTable {
+// With GRAD the derivative-factor table (k3_ktab_dls) rides along the same way.
+struct BgTableV {
     float2 ab[K3_KTAB_N + 1];
     float c[K3_KTAB_N + 1];
+};
+template <bool GRAD>
+struct BgTable : BgTableV {};
+template <>
+struct BgTable<true> : BgTableV {
+    float2 gab[K3_KTAB_N + 1];
+    float gc[K3_KTAB_N + 1];
 };
 // a2 may be slightly negative (a rounded difference of squares standing for
 // 0): the table argument is clamp(a2 * kscale, 0, N) in one med3, which for
 // every non-NaN a2 equals min(max(a2, 0) * kscale, N) of k3_kernel_index, and
 // the fraction is taken with fract, which equals s - (float)(int)s exactly
 // for 0 <= s < 2^24, so the result is that of k3_kernel_value_sq bit for bit.
+// The length-scale derivative (GRAD) is sigma G(u) / ls from the second
+// table at the same index and fraction (k3_kernel_dls_sq), where the value is
+// positive.
 template <bool GRAD>
 __device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float sigma, float ls, float inv_ls,
-                                                const BgTable* stab) {
+                                                const BgTable<GRAD>* stab) {
+    (void)ls;
     BgTerm<GRAD> t;
     const float s = __builtin_amdgcn_fmed3f(a2 * kscale, 0.0f, (float)K3_KTAB_N);
     const int i = (int)s;
@@ -511,15 +523,23 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float si
     const float2 ab = stab->ab[i];
     t.v = k3_kernel_poly(ab.x, ab.y, stab->c[i], f, sigma);
     t.g = 0.0f;
-    if (GRAD) t.g = t.v > 0.0f ? k3_kernel_dls(sqrtf(fmaxf(0.0f, a2)), ls, inv_ls, sigma) : 0.0f;
+    if constexpr (GRAD) {
+        const float2 g = stab->gab[i];
+        t.g = t.v > 0.0f ? sigma * k3_kernel_poly_raw(g.x, g.y, stab->gc[i], f) * inv_ls : 0.0f;
+    }
     return t;
 }
 
-// Copies the kernel table into LDS (every thread of the block takes part).
-__device__ __forceinline__ void bg_load_table(BgTable* stab) {
+// Copies the kernel table(s) into LDS (every thread of the block takes part).
+template <bool GRAD>
+__device__ __forceinline__ void bg_load_table(BgTable<GRAD>* stab) {
     for (int i = threadIdx.x; i <= K3_KTAB_N; i += GA_TPB) {
         stab->ab[i] = make_float2(k3_ktab[4 * i], k3_ktab[4 * i + 1]);
         stab->c[i] = k3_ktab[4 * i + 2];
+        if constexpr (GRAD) {
+            stab->gab[i] = make_float2(k3_ktab_dls[4 * i], k3_ktab_dls[4 * i + 1]);
+            stab->gc[i] = k3_ktab_dls[4 * i + 2];
+        }
     }
     __syncthreads();
 }
@@ -534,7 +554,7 @@ constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 template <bool GRAD>
 __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, int lane,
                                           float4 (&srec)[GA_WAVES][64 * 3], uint32_t (&smeta)[GA_WAVES][64],
-                                          const BgTable* stab) {
+                                          const BgTable<GRAD>* stab) {
     const GridGeom g = a.fc->g;
     const uint4 item = a.items[it];
     const uint32_t tile = uniform(item.x), ib = uniform(item.y), ie = uniform(item.z), slot = uniform(item.w);
@@ -694,11 +714,11 @@ template <bool GRAD>
 __global__ __launch_bounds__(GA_TPB) void k_bgki(BgArgs a) {
     __shared__ float4 srec[GA_WAVES][64 * 3];
     __shared__ uint32_t smeta[GA_WAVES][64];
-    __shared__ BgTable stab_s;
-    BgTable* stab = &stab_s;
+    __shared__ BgTable<GRAD> stab_s;
+    BgTable<GRAD>* stab = &stab_s;
     DevScalars* sc = a.sc;
     if (sc->overflow) return;
-    bg_load_table(stab);
+    bg_load_table<GRAD>(stab);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n_items = sc->n_light + sc->n_heavy;
     while (true) {
@@ -771,9 +791,9 @@ __global__ __launch_bounds__(GA_TPB) void k_bgki_batch(const BgArgs* __restrict_
                                                     const uint32_t* __restrict__ prefix, uint32_t* counter) {
     __shared__ float4 srec[GA_WAVES][64 * 3];
     __shared__ uint32_t smeta[GA_WAVES][64];
-    __shared__ BgTable stab_s;
-    BgTable* stab = &stab_s;
-    bg_load_table(stab);
+    __shared__ BgTable<false> stab_s;
+    BgTable<false>* stab = &stab_s;
+    bg_load_table<false>(stab);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t total = prefix[nmaps];
     while (true) {

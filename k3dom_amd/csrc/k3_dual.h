@@ -178,6 +178,18 @@ template <class X, class T>
 K3_HDT X k3g_credit(const X& mass, T mass_scale) {
     return T(1) - T(2) / (T(1) + k3g_exp(T(2) * mass / mass_scale));
 }
+// Dual numbers: the same value; the derivative in closed form,
+// d/dm (1 - 2 / (1 + e^(2m/s))) = (1 - c^2) / s with c = tanh(m/s) the value,
+// which stays finite where e^(2m/s) overflows (the quotient rule would give
+// inf / inf there).
+template <class T, int N, class S>
+K3_HDT K3Dual<T, N> k3g_credit(const K3Dual<T, N>& mass, S mass_scale) {
+    K3Dual<T, N> r;
+    r.v = T(1) - T(2) / (T(1) + k3g_exp(T(2) * mass.v / T(mass_scale)));
+    const T dc = (T(1) - r.v * r.v) / T(mass_scale);
+    for (int i = 0; i < N; ++i) r.d[i] = dc * mass.d[i];
+    return r;
+}
 template <class X>
 K3_HDT X k3g_positive_diff(const X& a, const X& b) {
     const X d = a - b;

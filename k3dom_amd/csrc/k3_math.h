@@ -207,8 +207,9 @@ K3_HD float k3_kernel_scale(float ls) {
 
 // One table entry (a, b, c): a + f (b + f c), f in [0, 1).  Entries at
 // i >= K3_KTAB_N are zero, so a2 * scale >= N (d >= ls) gives exactly 0.
+K3_HD float k3_kernel_poly_raw(float a, float b, float c, float f) { return fmaf(f, fmaf(f, c, b), a); }
 K3_HD float k3_kernel_poly(float a, float b, float c, float f, float sigma) {
-    const float v = fmaf(f, fmaf(f, c, b), a);
+    const float v = k3_kernel_poly_raw(a, b, c, f);
     return fmaxf(0.0f, sigma * v);
 }
 K3_HD int k3_kernel_index(float a2, float scale, float* f) {
@@ -238,6 +239,16 @@ K3_HD float k3_kernel_dls(float d, float ls, float inv_ls, float sigma) {
     const float two_pi = 6.28318530717958647692f;
     const float dk = -(two_pi / 3.0f) * sn * (1.0f - q) - (2.0f + cs) / 3.0f + cs * (3.14159265358979323846f / 3.141592f);
     return sigma * dk * (-q * inv_ls);
+}
+
+// The same derivative from the second table: sigma G(a2 scale / N) / ls with
+// G(u) = -q kbar'(q), q = sqrt(u), where the kernel value is positive (the
+// gather's derivative evaluation; no square root or sine per pair).
+K3_HD float k3_kernel_dls_sq(float a2, float scale, float inv_ls, float sigma) {
+    float f;
+    const int i = k3_kernel_index(fmaxf(a2, 0.0f), scale, &f);
+    if (!(k3_kernel_poly(k3_ktab[4 * i], k3_ktab[4 * i + 1], k3_ktab[4 * i + 2], f, sigma) > 0.0f)) return 0.0f;
+    return sigma * k3_kernel_poly_raw(k3_ktab_dls[4 * i], k3_ktab_dls[4 * i + 1], k3_ktab_dls[4 * i + 2], f) * inv_ls;
 }
 
 // distance_with_margin, src/k3dom/kernel/mass_update.cu:29-31

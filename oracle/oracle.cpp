@@ -1124,6 +1124,17 @@ int orc_window_chain64(const k3dom_params* p, int32_t T, const int32_t* npts, co
     return orc_window_chain64_groups(p, T, npts, pts, sensor_rel, move, decay, counts, gt, mask, nullptr, nullptr,
                                      nullptr, theta, loss, grad);
 }
+// length-scale derivative of sigma * kbar(d / ls): closed form, and from the table
+float orc_kernel_dls(float d, float ls, float sigma) { return k3_kernel_dls(d, ls, 1.0f / ls, sigma); }
+float orc_kernel_dls_sq(float a2, float ls, float sigma) {
+    return k3_kernel_dls_sq(a2, k3_kernel_scale(ls), 1.0f / ls, sigma);
+}
+// credit in forward-mode dual numbers: value and derivative by the mass
+void orc_dual_credit(double mass, double scale, double* value, double* deriv) {
+    const K3Dual<double, 6> c = k3g_credit(k3d_var<double, 6>(mass, 0), scale);
+    *value = c.v;
+    *deriv = c.d[0];
+}
 // kernel at distance d (evaluated from the float32 square d * d)
 float orc_kernel(float d, float ls, float sigma) { return k3_kernel_value_sq(d * d, k3_kernel_scale(ls), sigma); }
 float orc_credit(float m, float scale) { return k3_credit(m, scale); }

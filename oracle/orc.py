@@ -61,6 +61,9 @@ def lib():
                                                     C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                     C.POINTER(C.c_double)]),
             "orc_credit": (C.c_float, [C.c_float, C.c_float]),
+            "orc_kernel_dls": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+            "orc_kernel_dls_sq": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+            "orc_dual_credit": (None, [C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "orc_box_muller": (None, [C.c_uint32, C.c_uint32, fp, fp]),
             "orc_update_mass": (None, [fp, C.c_float, C.c_float, C.c_float]),
             "orc_predict_mass": (None, [fp, C.c_float, C.c_float, C.c_float]),

@@ -193,3 +193,35 @@ def test_kernel_is_exactly_zero_from_the_length_scale_on(oracle_lib, ls):
     below = np.nextafter(ls32, np.float32(0))
     v = oracle_lib.orc_kernel(float(below), float(ls32), 1.0)
     assert 0.0 <= v < 1e-6
+
+
+def test_kernel_length_scale_derivative_table(oracle_lib):
+    """The gather's length-scale derivative comes from a second table,
+    G(u) = -q kbar'(q) with u = q^2 (k3_ktab_dls): it is what the generator
+    prints, the closed form kbar'(q) matches central differences of kbar in
+    double, and the table evaluation sigma G(d^2/ls^2) / ls stays within 1e-6
+    (relative to sigma / ls) of the closed form over the support, with exact
+    zeros from ls on."""
+    import importlib.util
+    import os
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("gkt", os.path.join(root, "k3dom_amd", "csrc", "gen_kernel_table.py"))
+    gkt = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gkt)
+    header = open(os.path.join(root, "k3dom_amd", "csrc", "k3_kernel_table.h")).read()
+    body = header[header.index("k3_ktab_dls"):]
+    body = body[body.index("{") + 1:body.index("}")]
+    got = np.array([np.float32(float(v)) for v in re.findall(r"([-0-9.e]+)f", body)], np.float32)
+    want = gkt.table_dls().reshape(-1)
+    assert got.shape == want.shape and np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    q = np.linspace(0.01, 0.99, 99)
+    h = 1e-6
+    fd = (gkt.kbar(q + h) - gkt.kbar(q - h)) / (2 * h)
+    assert np.max(np.abs(fd - gkt.dkbar(q))) < 1e-8
+    for ls, sigma in ((0.5, 1.0), (0.3, 0.1), (1.7, 2.5)):
+        ds = np.linspace(0, ls, 4001, endpoint=False).astype(np.float32)
+        tab = np.array([oracle_lib.orc_kernel_dls_sq(float(d) * float(d), ls, sigma) for d in ds])
+        ref = np.array([oracle_lib.orc_kernel_dls(float(d), ls, sigma) for d in ds])
+        assert np.max(np.abs(tab - ref)) <= 1e-6 * sigma / ls
+        assert oracle_lib.orc_kernel_dls_sq(float(np.float32(ls) * np.float32(1.0001)) ** 2, ls, sigma) == 0.0

@@ -281,3 +281,19 @@ def test_window_rig_record_and_gradient(gpu):
     assert abs(L - L64) <= 1e-5 * abs(L64)
     assert np.all(_rel_err(g, g64) <= 1e-3), (g, g64)
     assert np.all(_rel_err(g, fd) <= 1e-3), (g, fd)
+
+
+def test_dual_credit_derivative_is_finite_at_large_masses(oracle_lib):
+    """credit(m) = 1 - 2 / (1 + e^(2m/s)) = tanh(m/s); its forward-mode
+    derivative is (1 - tanh^2) / s, finite and 0 where e^(2m/s) overflows
+    (voxels near a sensor reach masses of 1e4)."""
+    import ctypes as C
+    import math
+    v, d = C.c_double(), C.c_double()
+    for m, s in ((0.0, 2.5), (1.0, 2.5), (3.0, 0.7), (-2.0, 2.5)):
+        oracle_lib.orc_dual_credit(m, s, C.byref(v), C.byref(d))
+        assert abs(v.value - math.tanh(m / s)) < 1e-14
+        assert abs(d.value - (1 - math.tanh(m / s) ** 2) / s) < 1e-12
+    for m in (400.0, 1e4, 1e30):
+        oracle_lib.orc_dual_credit(m, 2.5, C.byref(v), C.byref(d))
+        assert v.value == 1.0 and d.value == 0.0
