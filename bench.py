@@ -13,7 +13,7 @@ kernel-inference gather, whose algorithmic work is its contributing
 against its time; `adjoint` times the differentiated window (forward +
 backward ms/frame against the plain forward, config-1 scene, ls 0.3, 8-frame
 windows); `batch` times 32 independent uneven map tiles advanced together;
-`dsphdmib` times the DS-PHD/MIB method on the same frames; `lidars` times the
+`dsphdmib` times the DS-PHD/MIB method on the same scene; `lidars` times the
 four-LiDAR stress configuration (512x512x64, 8M particles) and its
 differentiated window (fwd+bwd against plain forward); `cpu_baseline`
 times the serial restatement on a bounded sample of the same workload on the
@@ -175,13 +175,17 @@ def batch_leg(torch, warmup=3, steps=8):
     return res
 
 
-def dsphdmib_leg(torch, w, frames, dpts, warmup=3, steps=20, breakdown=5):
-    """The DS-PHD/MIB method (src/dsphdmib/) on the same scene, grid,
-    particle counts and device-resident point clouds: frames/s, the per-stage
-    breakdown of a few further frames (the measurement grid is the
+def dsphdmib_leg(torch, w, warmup=3, steps=20, breakdown=5):
+    """The DS-PHD/MIB method (src/dsphdmib/) on the same scene, grid and
+    particle counts, point clouds resident in device memory: frames/s, the
+    per-stage breakdown of a few further frames (the measurement grid is the
     kernel-inference stage's slot) and its candidate-list entries."""
     from k3dom_amd.dom import DOM
-    from k3dom_amd.scene import dsphdmib_params
+    from k3dom_amd.scene import Scene, dsphdmib_params
+    sc = Scene(w.scene)
+    frames = [sc.frame(f) for f in range(warmup + steps + breakdown)]
+    dpts = [tuple(torch.from_numpy(a).cuda() for a in fr[2:5]) for fr in frames]
+    torch.cuda.synchronize()
     p0 = w.params
     gs = int(np.float32(p0.size) / np.float32(p0.resolution))
     gsz = int(np.float32(p0.size_z) / np.float32(p0.resolution))
@@ -451,8 +455,8 @@ def main():
         log(f"batched tiles: {bat}")
 
     ds = None
-    if not args.no_dsphdmib and len(frames) >= 28:
-        ds = dsphdmib_leg(torch, w, frames, dpts)
+    if not args.no_dsphdmib:
+        ds = dsphdmib_leg(torch, w)
         log(f"DS-PHD/MIB: {ds}")
 
     lid = None
