@@ -520,12 +520,16 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float si
     const float s = __builtin_amdgcn_fmed3f(a2 * kscale, 0.0f, (float)K3_KTAB_N);
     const int i = (int)s;
     const float f = __builtin_amdgcn_fractf(s);
-    const float2 ab = stab->ab[i];
-    t.v = k3_kernel_poly(ab.x, ab.y, stab->c[i], f, sigma);
+    // both entries addressed from 4 i (byte offsets into the two arrays)
+    const uint32_t i4 = (uint32_t)i << 2;
+    const float2 ab = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(stab->ab) + 2u * i4);
+    const float cc = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(stab->c) + i4);
+    t.v = k3_kernel_poly(ab.x, ab.y, cc, f, sigma);
     t.g = 0.0f;
     if constexpr (GRAD) {
-        const float2 g = stab->gab[i];
-        t.g = t.v > 0.0f ? sigma * k3_kernel_poly_raw(g.x, g.y, stab->gc[i], f) * inv_ls : 0.0f;
+        const float2 g = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(stab->gab) + 2u * i4);
+        const float gc = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(stab->gc) + i4);
+        t.g = t.v > 0.0f ? sigma * k3_kernel_poly_raw(g.x, g.y, gc, f) * inv_ls : 0.0f;
     }
     return t;
 }
