@@ -175,12 +175,14 @@ static inline bool narrow(double m, double c, double R, double* t0, double* t1) 
     return *t0 <= *t1;
 }
 
-static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const float* zs, int n,
-                          float ls, float sigma, bool brute, float* del_occ, float* del_free) {
-    const size_t V = (size_t)g.gs * g.gs * g.gsz;
-    ChunkAcc acc;
-    acc.init(V);
-    BgkiStats st;
+// Threads of the culled kernel inference (orc_set_threads; 1 = serial).
+static int g_bgki_threads = 1;
+
+// Culled kernel inference over the voxels with x index in [xlo, xhi): every
+// point in order, so each voxel's terms arrive in point order whatever the
+// slab, and the sums do not depend on how the x axis is split.
+static void bgki_culled_slab(const Geom& g, const float* xs, const float* ys, const float* zs, int n, float ls,
+                             float sigma, int xlo, int xhi, ChunkAcc& acc, BgkiStats& st) {
     const float ls2g = k3_ls2_guard(ls);
     const float kscale = k3_kernel_scale(ls);
     const double R = (double)ls * (1.0 + 1e-5) + 1e-4;
@@ -188,16 +190,11 @@ static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const
         K3PointRec pr = k3_point_prep(xs[p], ys[p], zs[p], g.res);
         if (pr.valid == 0.0f) continue;
         int32_t chunk = p >> K3_BGKI_CHUNK_LOG2;
-        if (brute) {
-            for (int k = 0; k < g.gsz; ++k)
-                for (int j = 0; j < g.gs; ++j)
-                    for (int i = 0; i < g.gs; ++i)
-                        bgki_pair(g, i, j, k, pr, ls, ls2g, kscale, sigma, chunk, acc, st);
-            continue;
-        }
         double mx = pr.x, my = pr.y, mz = pr.z;
         int i0, i1;
         if (!axis_range(std::min(0.0, mx) - R, std::max(0.0, mx) + R, g.res, g.sx, g.gs, &i0, &i1)) continue;
+        i0 = std::max(i0, xlo);
+        i1 = std::min(i1, xhi - 1);
         for (int i = i0; i <= i1; ++i) {
             double ta = 0.0, tb = 1.0;
             if (!narrow(mx, g.cx[i], R, &ta, &tb)) continue;
@@ -213,6 +210,41 @@ static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const
                 for (int k = k0; k <= k1; ++k)
                     bgki_pair(g, i, j, k, pr, ls, ls2g, kscale, sigma, chunk, acc, st);
             }
+        }
+    }
+}
+
+static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const float* zs, int n,
+                          float ls, float sigma, bool brute, float* del_occ, float* del_free) {
+    const size_t V = (size_t)g.gs * g.gs * g.gsz;
+    ChunkAcc acc;
+    acc.init(V);
+    BgkiStats st;
+    if (brute) {
+        const float ls2g = k3_ls2_guard(ls);
+        const float kscale = k3_kernel_scale(ls);
+        for (int p = 0; p < n; ++p) {
+            K3PointRec pr = k3_point_prep(xs[p], ys[p], zs[p], g.res);
+            if (pr.valid == 0.0f) continue;
+            int32_t chunk = p >> K3_BGKI_CHUNK_LOG2;
+            for (int k = 0; k < g.gsz; ++k)
+                for (int j = 0; j < g.gs; ++j)
+                    for (int i = 0; i < g.gs; ++i)
+                        bgki_pair(g, i, j, k, pr, ls, ls2g, kscale, sigma, chunk, acc, st);
+        }
+    } else if (g_bgki_threads <= 1) {
+        bgki_culled_slab(g, xs, ys, zs, n, ls, sigma, 0, g.gs, acc, st);
+    } else {
+        // x slabs of at most 4 voxels, taken dynamically: voxels of different
+        // slabs are disjoint, so the threads never write the same element
+        const int slabs = (g.gs + 3) / 4;
+        std::vector<BgkiStats> sst(slabs);
+#pragma omp parallel for num_threads(g_bgki_threads) schedule(dynamic, 1)
+        for (int s = 0; s < slabs; ++s)
+            bgki_culled_slab(g, xs, ys, zs, n, ls, sigma, 4 * s, std::min(g.gs, 4 * s + 4), acc, sst[s]);
+        for (const BgkiStats& x : sst) {
+            st.occ_pairs += x.occ_pairs;
+            st.free_pairs += x.free_pairs;
         }
     }
     acc.finish(del_occ, del_free);
@@ -964,6 +996,9 @@ void* orc_create_method(const k3dom_params* p, int method) {
 }
 void orc_destroy(void* h) { delete (Oracle*)h; }
 void orc_set_brute(void* h, int brute) { ((Oracle*)h)->brute = brute != 0; }
+// Threads of the culled kernel inference of every oracle map (1 = serial;
+// the results do not depend on it).
+void orc_set_threads(int n) { g_bgki_threads = n < 1 ? 1 : n; }
 void orc_update_pose(void* h, float x, float y, float z) { ((Oracle*)h)->update_pose(x, y, z); }
 int orc_update_grid(void* h, double t, const float* xs, const float* ys, const float* zs, int n) {
     if (!h || n < 0 || (n > 0 && (!xs || !ys || !zs))) return K3DOM_ERR_INVALID_ARG;

@@ -34,6 +34,7 @@ def lib():
             "orc_ds_newborn_part": (C.c_float, [C.c_float] * 3),
             "orc_destroy": (None, [C.c_void_p]),
             "orc_set_brute": (None, [C.c_void_p, C.c_int]),
+            "orc_set_threads": (None, [C.c_int]),
             "orc_update_pose": (None, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
             "orc_update_grid": (C.c_int, [C.c_void_p, C.c_double, fp, fp, fp, C.c_int]),
             "orc_update_grid_lidars": (C.c_int, [C.c_void_p, C.c_double, C.c_int, fp, fp, C.POINTER(C.c_int),
@@ -83,6 +84,12 @@ def lib():
             f.argtypes = a
         _orc = L
     return _orc
+
+
+def set_threads(n):
+    """Threads of the culled kernel inference in every oracle map (1 = serial;
+    the results are the same bits for any count)."""
+    lib().orc_set_threads(int(n))
 
 
 def _fp(a):

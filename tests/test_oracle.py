@@ -55,6 +55,26 @@ def test_bgki_culled_equals_brute_force_config1_subset(oracle_lib):
     assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
 
 
+def test_bgki_threaded_slabs_equal_serial(oracle_lib):
+    """The culled kernel inference split into x slabs over threads gives the
+    serial sums bit for bit (each voxel still takes its terms in point order)."""
+    from k3dom_amd.scene import workload, Scene
+    w = workload(1)
+    t, pose, xs, ys, zs = Scene(w.scene).frame(0)
+    p = w.params
+    rel = (0.37, -0.21, 0.05)
+    try:
+        orc.set_threads(1)
+        a = orc.bgki(64, 16, p.resolution, rel, p.ls, p.sigma, xs, ys, zs, brute=False)
+        orc.set_threads(4)
+        b = orc.bgki(64, 16, p.resolution, rel, p.ls, p.sigma, xs, ys, zs, brute=False)
+    finally:
+        orc.set_threads(1)
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert a[2] == b[2] and a[2][1] > 0
+
+
 def _ref_terms(vx, vy, vz, pts, res, ls, sigma):
     """Reference BGKI of one voxel in double (src/k3dom/kernel/mass_update.cu:41-72)."""
     occ = fre = 0.0
