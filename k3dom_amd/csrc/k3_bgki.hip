@@ -47,6 +47,10 @@ __device__ __forceinline__ void bg_wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// The word of lane q (q the same in every lane) in every lane.
+__device__ __forceinline__ uint32_t lane_word(uint32_t v, uint32_t q) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q);
+}
 
 // ============================================================== points
 __global__ __launch_bounds__(TPB) void k_point_prep(const FrameConsts* __restrict__ fc,
@@ -557,7 +561,7 @@ constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 // tile list) by one wavefront: lane = voxel of the 4x4x4 tile.
 template <bool GRAD>
 __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, int lane,
-                                          float4 (&srec)[GA_WAVES][64 * 3], uint32_t (&smeta)[GA_WAVES][64],
+                                          float4 (&srec)[GA_WAVES][64 * 3],
                                           const BgTable<GRAD>* stab) {
     const GridGeom g = a.fc->g;
     const uint4 item = a.items[it];
@@ -584,6 +588,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
     float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
     float gsum_o = 0.0f, gsum_f = 0.0f;  // length-scale derivatives (GRAD)
     uint32_t cur_chunk = 0xffffffffu;
+    uint32_t meta_l = 0;  // lane l: point index and class of candidate b + l
     for (uint32_t b = ib; b < ie; b += 64) {
         const uint32_t e = b + lane;
         if (e < ie) {
@@ -600,12 +605,12 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
             else if (proj - rr >= r.dm) cls = C_NOFREE;
             const float ox = tcx - r.x, oy = tcy - r.y, oz = tcz - r.z;
             if (!(ox * ox + oy * oy + oz * oz >= occ_r * occ_r)) cls |= F_OCC;
-            smeta[w][lane] = (p << 3) | cls;  // point index and class
+            meta_l = (p << 3) | cls;  // point index and class
         }
         bg_wave_sync();
         const uint32_t cnt = uniform(min(64u, ie - b));
         for (uint32_t q = 0; q < cnt; ++q) {
-            const uint32_t meta = uniform(smeta[w][q]);
+            const uint32_t meta = lane_word(meta_l, q);
             const float4 r0 = srec[w][q * 3];
             const uint32_t chunk = meta >> (3 + K3_BGKI_CHUNK_LOG2);
             const uint32_t cls = meta & 7u;
@@ -622,7 +627,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 // together (independent table reads), then added in order
                 const float dpa = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
                 const float a2a = fmaf(-dpa, dpa, dc2);  // clamped at 0 by bg_term
-                const uint32_t meta2 = q + 1 < cnt ? uniform(smeta[w][q + 1]) : 0u;
+                const uint32_t meta2 = q + 1 < cnt ? lane_word(meta_l, q + 1) : 0u;
                 if (q + 1 < cnt && (meta2 & 7u) == C_MAIN && (meta2 >> (3 + K3_BGKI_CHUNK_LOG2)) == chunk) {
                     const float4 r0b = srec[w][(q + 1) * 3];
                     const float dpb = fmaf(vz, r0b.z, fmaf(vy, r0b.y, vx * r0b.x)) * r0b.w;
@@ -717,7 +722,6 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
 template <bool GRAD>
 __global__ __launch_bounds__(GA_TPB) void k_bgki(BgArgs a) {
     __shared__ float4 srec[GA_WAVES][64 * 3];
-    __shared__ uint32_t smeta[GA_WAVES][64];
     __shared__ BgTable<GRAD> stab_s;
     BgTable<GRAD>* stab = &stab_s;
     DevScalars* sc = a.sc;
@@ -730,7 +734,7 @@ __global__ __launch_bounds__(GA_TPB) void k_bgki(BgArgs a) {
         if (lane == 0) it = atomicAdd(&sc->item_counter, 1u);
         it = uniform(__shfl(it, 0));
         if (it >= n_items) break;
-        bgki_item<GRAD>(a, it, w, lane, srec, smeta, stab);
+        bgki_item<GRAD>(a, it, w, lane, srec, stab);
     }
 }
 
@@ -794,7 +798,6 @@ __global__ void k_bgki_batch_prefix(const BgArgs* __restrict__ maps, int nmaps, 
 __global__ __launch_bounds__(GA_TPB) void k_bgki_batch(const BgArgs* __restrict__ maps, int nmaps,
                                                     const uint32_t* __restrict__ prefix, uint32_t* counter) {
     __shared__ float4 srec[GA_WAVES][64 * 3];
-    __shared__ uint32_t smeta[GA_WAVES][64];
     __shared__ BgTable<false> stab_s;
     BgTable<false>* stab = &stab_s;
     bg_load_table<false>(stab);
@@ -812,7 +815,7 @@ __global__ __launch_bounds__(GA_TPB) void k_bgki_batch(const BgArgs* __restrict_
             else hi = mid - 1;
         }
         const BgArgs a = maps[lo];
-        bgki_item<false>(a, it - prefix[lo], w, lane, srec, smeta, stab);
+        bgki_item<false>(a, it - prefix[lo], w, lane, srec, stab);
     }
 }
 
