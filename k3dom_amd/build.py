@@ -86,8 +86,12 @@ def build_all():
     build_torch_ext()
 
 
+TARGETS = {"all": build_all, "lib": build_lib, "scene": build_scene, "replay": build_replay, "torch": build_torch_ext}
+
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
+    bad = [w for w in what if w not in TARGETS]
+    if bad:
+        sys.exit(f"usage: python -m k3dom_amd.build [{'|'.join(TARGETS)}]...  (unknown: {' '.join(bad)})")
     for w in what:
-        {"all": build_all, "lib": build_lib, "scene": build_scene, "replay": build_replay,
-         "torch": build_torch_ext}[w]()
+        TARGETS[w]()

@@ -134,7 +134,7 @@ hipError_t launch_fill_voxels(hipStream_t st, VoxelArrays v, int V, float pf, fl
 // Stable LSD radix sort, 8-bit digits.  Per pass: per-tile LDS histograms
 // (digit-major in global memory), one exclusive scan, then a scatter that
 // ranks equal digits in input order with wavefront ballots.
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = TPB * RS_ITEMS;
 
 int sort_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
