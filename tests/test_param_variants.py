@@ -8,7 +8,8 @@ parameter sets that steer the update through different code paths:
   split into 1024-point chunks and recombined in chunk order);
 - a sensor offset from the pose, another kernel amplitude and mass scale;
 - particle counts that fill no block evenly;
-- the DS-PHD/MIB method on the odd grid with a sensor offset.
+- the DS-PHD/MIB method on the odd grid with a sensor offset;
+- an initialisation frame with no points, for both methods.
 
 Every GridCell field, particle array and per-frame array is compared after
 each frame (tests/_parity.py)."""
@@ -36,6 +37,18 @@ def test_parameter_variant_bit_exact(gpu, name):
     gd, _ = run_pair(w, w.frames)
     if name == "ls_half_map":
         assert gd.stats()["gather_split_items"] > 0, "expected tile lists longer than one chunk"
+
+
+@pytest.mark.parametrize("method", ["k3dom", "dsphdmib"])
+def test_first_frame_without_points_bit_exact(gpu, method):
+    """The initialisation frame sees no points (particles placed from the
+    priors alone); the frames after it are normal."""
+    from k3dom_amd.scene import small_workload
+
+    def mutate(f, t, pose, xs, ys, zs):
+        return (t, pose, xs[:0], ys[:0], zs[:0]) if f == 0 else (t, pose, xs, ys, zs)
+    w = small_workload(frames=4, seed=27, ego_vx=2.0, method=method)
+    run_pair(w, w.frames, mutate=mutate, method=method)
 
 
 def test_dsphdmib_odd_grid_sensor_offset_bit_exact(gpu):
