@@ -1048,10 +1048,9 @@ k3dom_status k3dom_update_grid_lidars(k3dom_handle* h, double t, const k3dom_lid
                     *sz = points_on_device ? L.zs : dz + o;
         HIP_TRY(h, launch_lidar_rotate(h->stream, L.rot, sx, sy, sz, dx + o, dy + o, dz + o, L.n));
     }
-    if (n_lidars > 1 && !h->gdel_occ) {
-        HIP_TRY(h, dalloc(h, &h->gdel_occ, (size_t)h->V));
-        HIP_TRY(h, dalloc(h, &h->gdel_free, (size_t)h->V));
-    }
+    // (each checked on its own: a failed allocation leaves the other usable)
+    if (n_lidars > 1 && !h->gdel_occ) HIP_TRY(h, dalloc(h, &h->gdel_occ, (size_t)h->V));
+    if (n_lidars > 1 && !h->gdel_free) HIP_TRY(h, dalloc(h, &h->gdel_free, (size_t)h->V));
     h->ngroups = n_lidars;
     s = frame(h, t, dx, dy, dz, m);
     h->ngroups = 0;

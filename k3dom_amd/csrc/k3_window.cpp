@@ -235,14 +235,11 @@ k3dom_status k3dom_window_set_frame_lidars(k3dom_window* w, int32_t f, double t,
         fi.gcount[k] = lidars[k].n;
         for (int i = 0; i < 3; ++i) fi.gorig[k][i] = lidars[k].origin[i];
     }
-    if (n_lidars > 1 && !w->g_o) {
+    if (n_lidars > 1) {
         const size_t V = (size_t)w->V;
-        W_HIP(w, walloc(w, &w->g_o, V));
-        W_HIP(w, walloc(w, &w->g_f, V));
-        W_HIP(w, walloc(w, &w->g_lo, V));
-        W_HIP(w, walloc(w, &w->g_lf, V));
-        W_HIP(w, walloc(w, &w->h->gdel_occ, V));
-        W_HIP(w, walloc(w, &w->h->gdel_free, V));
+        // each allocated while missing: a failed allocation leaves the others in place
+        for (float** p : {&w->g_o, &w->g_f, &w->g_lo, &w->g_lf, &w->h->gdel_occ, &w->h->gdel_free})
+            if (!*p) W_HIP(w, walloc(w, p, V));
     }
     return K3DOM_OK;
 }
