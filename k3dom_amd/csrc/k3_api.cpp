@@ -104,8 +104,17 @@ k3dom_status ensure_ds_list(k3dom_handle* h, long long want) {
     return K3DOM_OK;
 }
 
+// True when a buffer sized from the point capacity is missing: a previous
+// frame's allocation failed part way (that frame returned the error).
+bool point_buffers_missing(const k3dom_handle* h) {
+    if (!h->d_pts || !h->rec || !h->pxyz || !h->h_pts || !h->ibuf.items || !h->ibuf.partial) return true;
+    for (const LevelBufs& B : h->lbuf)
+        if (!B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask) return true;
+    return h->method == K3DOM_METHOD_DSPHDMIB && (!h->ds.list || !h->ds.segcnt || !h->ds.part);
+}
+
 k3dom_status ensure_points(k3dom_handle* h, int m) {
-    if (m <= h->m_cap) return K3DOM_OK;
+    if (m <= h->m_cap && !point_buffers_missing(h)) return K3DOM_OK;
     int cap = std::max(m, h->m_cap * 3 / 2);
     dfree(h, h->d_pts); dfree(h, h->rec); dfree(h, h->pxyz);
     if (h->h_pts) { (void)hipHostFree(h->h_pts); h->h_pts = nullptr; }
@@ -172,13 +181,16 @@ k3dom_status ensure_caps(k3dom_handle* h, const long long want[3], long long wan
     for (int l = 0; l < 3; ++l) {
         const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : h->lbuf[l - 1].list_cap;
         const long long items_need = h->ldesc[l].nparents + (parent_cap + h->ldesc[l].seg - 1) / h->ldesc[l].seg;
-        if (want[l] > h->lbuf[l].list_cap || items_need > h->lbuf[l].items_cap || parent_cap > h->lbuf[l].mask_stride ||
-            !h->lbuf[l].list) {
+        const LevelBufs& B = h->lbuf[l];
+        // (a buffer still missing after a failed allocation is allocated again)
+        const bool missing = !B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask;
+        if (want[l] > B.list_cap || items_need > B.items_cap || parent_cap > B.mask_stride || missing) {
             k3dom_status s = alloc_level(h, l, std::max(want[l], h->lbuf[l].list_cap));
             if (s != K3DOM_OK) return s;
         }
     }
-    if (want_heavy > h->ibuf.heavy_cap || !h->ibuf.items) return alloc_heavy(h, std::max(want_heavy, h->ibuf.heavy_cap));
+    if (want_heavy > h->ibuf.heavy_cap || !h->ibuf.items || !h->ibuf.partial)
+        return alloc_heavy(h, std::max(want_heavy, h->ibuf.heavy_cap));
     return K3DOM_OK;
 }
 
