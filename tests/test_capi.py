@@ -116,6 +116,7 @@ def test_params_yaml_missing_file(lib):
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF, "config")), reason="the reference's config files are not on this machine")
 def test_params_from_reference_yaml(lib):
     from k3dom_amd.dom import params_from_yaml
     for ds, res, size, size_z, off in [("complex", 0.2, 40.0, 5.0, (5.0, -4.7, 0.0)),
