@@ -1,0 +1,59 @@
+"""The headline configuration on the GPU: 256x256x32 voxels at 0.1 m,
+2,000,000 particles + 200,000 newborn per frame, a 64x2048 LiDAR (40 m) and
+an ego at 10 m/s, so that the map shifts by 10 voxels every frame.
+
+- The initialisation frame and three normal frames, each after a map shift,
+  are compared bit for bit with the serial restatement on every array the
+  smaller parity tests compare (every GridCell field, particles, sort
+  permutation and keys, newborn set, born masses, evidence, resampled
+  indices).  The restatement's kernel inference runs on host threads; it
+  gives the same bits for any thread count.
+- Twelve further frames are checked against the size-independent properties
+  of tests/_invariants.py.
+
+Reference path: src/k3dom/dom.cu:29-50 (updateGrid), src/common/dom.cu:111-297
+(updatePose and the particle stages)."""
+import os
+
+import pytest
+
+from tests._invariants import check_frame_invariants
+from tests._parity import run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def _host_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+@pytest.mark.timeout(900)
+def test_config2_bit_exact_with_map_shifts(gpu):
+    from k3dom_amd.scene import workload
+    from oracle import orc
+    orc.set_threads(_host_threads())
+    try:
+        w = workload(2)
+        gd, od = run_pair(w, 4)
+    finally:
+        orc.set_threads(1)
+    # ego at 10 m/s, 10 Hz, shift threshold 1 m: the map moved every frame
+    assert gd.info().center_pos[0] == od.info().center_pos[0] == pytest.approx(3.0)
+    assert gd.stats()["points"] > 60000
+
+
+@pytest.mark.timeout(600)
+def test_config2_invariants_over_frames(gpu):
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, workload
+    w = workload(2)
+    sc = Scene(w.scene)
+    gd = DOM(w.params)
+    for f in range(14):
+        t, pose, xs, ys, zs = sc.frame(f)
+        normal = gd.info().updated_time > 0
+        gd.updatePose(*pose)
+        gd.updateGrid(t, xs, ys, zs)
+        if f == 0 or f >= 2:
+            print(f"  frame {f}: {len(xs)} points", flush=True)
+            check_frame_invariants(gd, normal, what=f"cfg2 frame {f}:")
