@@ -229,6 +229,23 @@ enum {
 k3dom_status k3dom_export_mappoints(k3dom_handle* h, int32_t mode, float min_occ, float min_mass,
                                    k3dom_mappoint* out, int64_t capacity, int64_t* count);
 
+/* Allocation-free variant of k3dom_export_mappoints.  No counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The records go to DEVICE memory d_out (capacity records; d_out == NULL with
+ * capacity == 0 returns only the count), and the visible-voxel compaction
+ * runs in one pass over the library's fixed device scratch (about 0.5 MiB per
+ * GPU, held until the process ends) instead of buffers the handle allocates:
+ * the call allocates nothing.  Maps of more than K3DOM_NOALLOC_MAX_VOXELS
+ * voxels are rejected with K3DOM_ERR_INVALID_ARG.  With capacity below the
+ * count, the first capacity records are written, *count receives the count
+ * and K3DOM_ERR_INVALID_ARG is returned.  Every call on one GPU shares that
+ * scratch: calls that overlap on one GPU (handles used from several threads
+ * or streams at once) race. */
+#define K3DOM_NOALLOC_MAX_VOXELS (65536ll * 4096ll)
+k3dom_status k3dom_export_mappoints_noalloc(k3dom_handle* h, int32_t mode, float min_occ, float min_mass,
+                                           k3dom_mappoint* d_out, int64_t capacity, int64_t* count);
+
 /* Checkpoint and resume, no counterpart in the reference (whose filter state
  * exists only in device memory for the life of the process).  A checkpoint
  * holds the parameters and method, the frame counter (which keys every random

@@ -87,6 +87,8 @@ SIGNATURES = {
     "k3dom_checkpoint_load": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]),
     "k3dom_export_mappoints": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_int64,
                                          C.POINTER(C.c_int64)]),
+    "k3dom_export_mappoints_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_void_p,
+                                                 C.c_int64, C.POINTER(C.c_int64)]),
     "k3dom_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "k3dom_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "k3dom_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),

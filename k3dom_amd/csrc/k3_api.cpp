@@ -835,6 +835,7 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
     hipDeviceProp_t prop;
     CREATE_TRY(hipGetDeviceProperties(&prop, h->device));
     h->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    (void)noalloc_prepare(h->device);
     CREATE_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     CREATE_TRY(hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
     CREATE_TRY(hipEventCreateWithFlags(&h->pts_ready, hipEventDisableTiming));

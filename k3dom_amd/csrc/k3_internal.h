@@ -185,5 +185,8 @@ k3dom_status settle_particles(k3dom_handle* h);
 k3dom_status begin_frame_voxels(k3dom_handle* h, bool normal);
 // Whether the next frame runs as a captured graph.
 bool frame_uses_graph(const k3dom_handle* h);
+// Looks up the allocation-free export's device scratch on a GPU (once per
+// device; false when the lookup failed, which the export then reports).
+bool noalloc_prepare(int device);
 
 }  // namespace k3api

@@ -26,6 +26,7 @@
 //   k_resample           src/common/kernel/resampling.cu:38-89
 //   k_shift_*            src/common/kernel/ego_motion_compensation.cu:12-41
 #include "k3_kernels.h"
+#include "k3_mappoint.h"
 
 #include <algorithm>
 
@@ -690,36 +691,7 @@ hipError_t launch_add_evidence(hipStream_t st, const uint32_t* abort_flag, float
 }
 
 // ============================================================== MapPoint export
-// The node's evaluation records (publish_mapdata, src/k3dom/k3dom_node.cpp:161-184,
-// src/dsphdmib/dsphdmib_node.cpp:153-177) with the GridCell accessors of
-// include/common/dom_types.h:47-64 and DOM_c::indexToCoord of
-// include/common/dom.h:70-81, same float expressions.
-struct MpCell {
-    float f, s, d, occ, vx, vy, vz;
-};
-__device__ __forceinline__ MpCell mp_cell(const VoxelArrays& v, int c, const MapPointArgs& a) {
-    MpCell m;
-    m.f = v.f[c];
-    if (a.ds) {
-        m.s = a.prior_s;
-        m.d = a.prior_d;
-        m.occ = v.s[c];
-    } else {
-        m.s = v.s[c];
-        m.d = v.d[c];
-        m.occ = a.prior_occ;
-    }
-    m.vx = v.mvx[c];
-    m.vy = v.mvy[c];
-    m.vz = v.mvz[c];
-    return m;
-}
-__device__ __forceinline__ bool mp_visible(const MpCell& m, const MapPointArgs& a) {
-    if (a.ds) return m.occ + (1.0f - m.occ - m.f) / 2.0f > a.min_occ && m.occ + m.f > a.min_mass;
-    const float total = m.f + m.s + m.d;
-    return (m.s + m.d) / total > a.min_occ && total > a.min_mass;
-}
-
+// Records and filter: k3_mappoint.h.
 __global__ __launch_bounds__(TPB) void k_mappoint_flags(VoxelArrays v, int V, MapPointArgs a,
                                                         uint32_t* __restrict__ flags) {
     const int c = blockIdx.x * TPB + threadIdx.x;
@@ -731,28 +703,7 @@ __global__ __launch_bounds__(TPB) void k_mappoint_fill(VoxelArrays v, int V, Map
                                                        const uint32_t* __restrict__ off, float4* __restrict__ out) {
     const int c = blockIdx.x * TPB + threadIdx.x;
     if (c >= V || (flags && !flags[c])) return;
-    const MpCell m = mp_cell(v, c, a);
-    const int gs2 = a.gs * a.gs;
-    const float x = ((float)(c % a.gs) + 0.5f) * a.res - a.size / 2.0f + a.cx;
-    const float y = ((float)((c % gs2) / a.gs) + 0.5f) * a.res - a.size / 2.0f + a.cy;
-    const float z = ((float)(c / gs2) + 0.5f) * a.res - a.size_z / 2.0f + a.cz;
-    float occ_val, dyn_val, eval_aug, dyn_aug;
-    if (a.ds) {
-        occ_val = m.occ + (1.0f - m.occ - m.f) / 2.0f;
-        dyn_val = (m.vx * m.vx + m.vy * m.vy + m.vz * m.vz) * a.res * a.res;
-        eval_aug = -(1.0f - m.occ - m.f);
-        dyn_aug = m.occ;
-    } else {
-        const float total = m.f + m.s + m.d;
-        occ_val = (m.s + m.d) / total;
-        dyn_val = m.d / total;
-        eval_aug = total;
-        dyn_aug = 1.1f;
-    }
-    const size_t o = 3 * (size_t)(off ? off[c] : (uint32_t)c);
-    out[o] = make_float4(x, y, z, 1.0f);
-    out[o + 1] = make_float4(m.vx * a.res, m.vy * a.res, m.vz * a.res, occ_val);
-    out[o + 2] = make_float4(dyn_val, eval_aug, dyn_aug, 0.0f);
+    mp_store(out, off ? off[c] : (uint32_t)c, c, mp_cell(v, c, a), a);
 }
 
 hipError_t launch_export_mappoints(hipStream_t st, const VoxelArrays& v, int V, const MapPointArgs& a, int mode,
