@@ -64,6 +64,80 @@ static double ms_since(clk::time_point t0) {
     return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
 }
 
+// predictKernel, src/common/kernel/predict.cu:20-56: every particle i
+// through k3_predict_particle with its Philox counters (i, frame, PREDICT).
+static void predict_particles(const K3PredictConst& c, int n, float* px, float* py, float* pz, float* vx,
+                              float* vy, float* vz, float* w, int32_t* key) {
+    for (int i = 0; i < n; ++i)
+        k3_predict_particle((uint32_t)i, c, &px[i], &py[i], &pz[i], &vx[i], &vy[i], &vz[i], &w[i], &key[i]);
+}
+
+// statisticalMomentsKernel1/2, src/common/kernel/statistical_moments.cu:12-63:
+// per voxel with particles [start, end], sum(w v) / pers when pers > 0, else
+// 0; 0 for voxels without particles.  The sums take the two-level
+// association (left-to-right sums of K3_SEG_BLOCK-particle blocks from the
+// voxel's first particle, added left to right).
+static void voxel_moments(int V, const int32_t* start, const int32_t* end, const float* w, const float* vx,
+                          const float* vy, const float* vz, const float* pers, float* mvx, float* mvy, float* mvz) {
+    for (int c = 0; c < V; ++c) {
+        if (start[c] != -1) {
+            float sx = 0, sy = 0, sz = 0;
+            for (int b = start[c]; b <= end[c]; b += K3_SEG_BLOCK) {
+                float bx = 0, by = 0, bz = 0;
+                for (int j = b; j <= std::min(end[c], b + K3_SEG_BLOCK - 1); ++j) {
+                    bx += w[j] * vx[j];
+                    by += w[j] * vy[j];
+                    bz += w[j] * vz[j];
+                }
+                sx += bx;
+                sy += by;
+                sz += bz;
+            }
+            const float rho = pers[c];
+            mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
+            mvy[c] = rho > 0.0f ? sy / rho : 0.0f;
+            mvz[c] = rho > 0.0f ? sz / rho : 0.0f;
+        } else {
+            mvx[c] = 0; mvy[c] = 0; mvz[c] = 0;
+        }
+    }
+}
+
+// FP64 restatement of one voxel's predict_mass, update_mass and newborn split
+// (src/k3dom/kernel/mass_update.cu:12-150,161-181) in double with libm: from
+// (f, s, d) before prediction to out4 = (f, s, d) after the update and born.
+static void mass_update64(double f, double s, double d, double dyn_pred, double decay, double del_occ,
+                          double del_free, double mass_scale, double p_B, double* out4) {
+    double pred_total = (f + d) * decay;
+    s *= decay;
+    d = std::min(dyn_pred, std::max(0.0, pred_total - s));
+    f = pred_total - d;
+    auto credit = [&](double m) { return std::tanh(m / mass_scale); };
+    if (del_free > 0 || del_occ > 0) {
+        double ds = del_free + del_occ, ps = f + s + d;
+        double c = credit(std::sqrt(ds * ps));
+        double l_DtoF = c * std::max(0.0, del_free - del_occ) / ds * d / ps;
+        double l_StoFD = c * std::max(0.0, del_free - del_occ) / ds * std::max(0.0, s + d - f) / ps;
+        double l_FtoD = c * std::max(0.0, del_occ - del_free) / ds * std::max(0.0, f - s - d) / ps;
+        double df = l_StoFD / 2 * s + l_DtoF * d - l_FtoD * f;
+        double dss = -l_StoFD * s;
+        double dd = l_FtoD * f + l_StoFD / 2 * s - l_DtoF * d;
+        f += df; s += dss; d += dd;
+        c = credit(s + d);
+        double ratio = (s <= 0 || d < 0) ? 0.0 : 1.0 / (1.0 + d / s);
+        double beta = (1 - c) + c * ratio;
+        s = s + beta * del_occ;
+        d = d + (1 - beta) * del_occ;
+        f = f + del_free;
+    }
+    double born;
+    if (dyn_pred <= 0) born = d;
+    else if (pred_total <= dyn_pred) born = 0;
+    else born = d * p_B * (pred_total - dyn_pred) / (dyn_pred + p_B * (pred_total - dyn_pred));
+    out4[0] = f; out4[1] = s; out4[2] = d; out4[3] = born;
+}
+
+
 struct BgkiStats {
     uint64_t occ_pairs = 0;
     uint64_t free_pairs = 0;
@@ -343,6 +417,14 @@ struct Oracle {
     uint32_t key0() const { return (uint32_t)p.seed; }
     uint32_t key1() const { return (uint32_t)(p.seed >> 32); }
 
+    // FP64 shadow of the last frame's floating-point stages (keep64): the masses
+    // before the mass update, then in double from the frame's own FP32 inputs:
+    // the predicted dynamic mass (segment sum of the sorted weights), the
+    // mass rules (f, s, d, born) and the mean velocities.
+    bool keep64 = false;
+    enum { E_PRE_F, E_PRE_S, E_PRE_D, E_DYN, E_F, E_S, E_D, E_BORN, E_MVX, E_MVY, E_MVZ, E_COUNT };
+    std::vector<double> e64[E_COUNT];
+
     // DOM_c::updatePose, src/common/dom.cu:111-158
     void update_pose(float nx, float ny, float nz) {
         sensor[0] = nx; sensor[1] = ny; sensor[2] = nz;
@@ -438,7 +520,21 @@ struct Oracle {
         }
         stage_ms[3] = ms_since(t0);
         t0 = clk::now();
+        if (keep64)
+            for (auto& a : e64) a.assign(V, 0.0);
         for (int i = 0; i < V; ++i) {
+            if (keep64) {
+                double dyn64 = 0.0;
+                if (start[i] != -1)
+                    for (int j = start[i]; j <= end[i]; ++j) dyn64 += (double)wsorted[j];
+                double o4[4];
+                const float dyn32 = start[i] != -1 ? seg_sum(wsorted.data(), start[i], end[i]) : 0.0f;
+                mass_update64(f[i], s[i], d[i], dyn32, decay, del_occ[i], del_free[i], p.mass_scale, p.birth_prob,
+                              o4);
+                e64[E_PRE_F][i] = f[i]; e64[E_PRE_S][i] = s[i]; e64[E_PRE_D][i] = d[i];
+                e64[E_DYN][i] = dyn64;
+                e64[E_F][i] = o4[0]; e64[E_S][i] = o4[1]; e64[E_D][i] = o4[2]; e64[E_BORN][i] = o4[3];
+            }
             float pred_total = (f[i] + d[i]) * decay;
             const float dyn_pred = start[i] != -1 ? seg_sum(wsorted.data(), start[i], end[i]) : 0.0f;
             k3_predict_mass(&f[i], &s[i], &d[i], dyn_pred, pred_total, decay);
@@ -565,8 +661,8 @@ struct Oracle {
         c.dt = dt; c.p_S = p.persistence_prob; c.sigma_pos = p.stddev_process_noise_position;
         c.sigma_vel = p.stddev_process_noise_velocity; c.vclamp = 5 * p.init_max_velocity;
         c.gs = gs; c.gsz = gsz; c.frame = frame; c.key0 = key0(); c.key1 = key1();
-        for (int i = 0; i < N; ++i)
-            k3_predict_particle((uint32_t)i, c, &px[i], &py[i], &pz[i], &vx[i], &vy[i], &vz[i], &w[i], &key[i]);
+        predict_particles(c, N, px.data(), py.data(), pz.data(), vx.data(), vy.data(), vz.data(), w.data(),
+                          key.data());
     }
 
     // stable counting sort by key + particleToGridKernel
@@ -639,28 +735,22 @@ struct Oracle {
 
     // statisticalMomentsKernel1/2, src/common/kernel/statistical_moments.cu:22-63
     void moments() {
-        for (int c = 0; c < V; ++c) {
-            if (start[c] != -1) {
-                float sx = 0, sy = 0, sz = 0;  // two-level segment sums (K3_SEG_BLOCK)
-                for (int b = start[c]; b <= end[c]; b += K3_SEG_BLOCK) {
-                    float bx = 0, by = 0, bz = 0;
-                    for (int j = b; j <= std::min(end[c], b + K3_SEG_BLOCK - 1); ++j) {
-                        bx += w[j] * vx[j];
-                        by += w[j] * vy[j];
-                        bz += w[j] * vz[j];
+        voxel_moments(V, start.data(), end.data(), w.data(), vx.data(), vy.data(), vz.data(), pers.data(),
+                      mvx.data(), mvy.data(), mvz.data());
+        if (keep64 && (int)e64[E_MVX].size() == V)
+            for (int c = 0; c < V; ++c) {
+                double sx = 0, sy = 0, sz = 0;
+                if (start[c] != -1)
+                    for (int j = start[c]; j <= end[c]; ++j) {
+                        sx += (double)w[j] * vx[j];
+                        sy += (double)w[j] * vy[j];
+                        sz += (double)w[j] * vz[j];
                     }
-                    sx += bx;
-                    sy += by;
-                    sz += bz;
-                }
-                float rho = pers[c];
-                mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
-                mvy[c] = rho > 0.0f ? sy / rho : 0.0f;
-                mvz[c] = rho > 0.0f ? sz / rho : 0.0f;
-            } else {
-                mvx[c] = 0; mvy[c] = 0; mvz[c] = 0;
+                const double rho = pers[c];
+                e64[E_MVX][c] = rho > 0 ? sx / rho : 0.0;
+                e64[E_MVY][c] = rho > 0 ? sy / rho : 0.0;
+                e64[E_MVZ][c] = rho > 0 ? sz / rho : 0.0;
             }
-        }
     }
 
     // resampling, src/common/dom.cu:259-297 and src/common/kernel/resampling.cu:13-89,
@@ -1193,6 +1283,61 @@ float orc_newborn_part(float dyn_pred, float total_pred, float dyn_up, float p_B
 }
 int orc_key(float x, float y, float z, int gs, int gsz) { return k3_key(x, y, z, gs, gsz); }
 
+// FP64 shadow of the floating-point stages of later frames (on != 0).
+void orc_set_keep64(void* hh, int on) { ((Oracle*)hh)->keep64 = on != 0; }
+// Array `which` of the FP64 shadow of the last normal frame (V doubles; order
+// of Oracle::E_*: pre-update f, s, d; dynamic mass; f, s, d, born; mean vx,
+// vy, vz); -1 when not kept.
+int64_t orc_read64(void* hh, int which, double* dst, int64_t capacity) {
+    Oracle* h = (Oracle*)hh;
+    if (which < 0 || which >= Oracle::E_COUNT || (int64_t)h->e64[which].size() != h->V) return -1;
+    const int64_t bytes = 8 * (int64_t)h->V;
+    if (capacity < bytes) return -1;
+    std::memcpy(dst, h->e64[which].data(), (size_t)bytes);
+    return bytes;
+}
+// The motion step of the frame (predict_particles) on caller arrays.
+void orc_predict(int n, float dt, float p_S, float sigma_pos, float sigma_vel, float vclamp, int gs, int gsz,
+                 uint32_t frame, uint64_t seed, float* px, float* py, float* pz, float* vx, float* vy, float* vz,
+                 float* w, int32_t* key) {
+    K3PredictConst c;
+    c.dt = dt; c.p_S = p_S; c.sigma_pos = sigma_pos; c.sigma_vel = sigma_vel; c.vclamp = vclamp;
+    c.gs = gs; c.gsz = gsz; c.frame = frame; c.key0 = (uint32_t)seed; c.key1 = (uint32_t)(seed >> 32);
+    predict_particles(c, n, px, py, pz, vx, vy, vz, w, key);
+}
+// The velocity draw of new particles (k3_sample_velocity) from three Philox words.
+void orc_sample_velocity(uint32_t r0, uint32_t r1, uint32_t r2, float vmax, float min_vel, float* out3) {
+    k3_sample_velocity(r0, r1, r2, vmax, min_vel, &out3[0], &out3[1], &out3[2]);
+}
+// The moments of the frame (voxel_moments) on caller arrays.
+void orc_moments(int V, const int32_t* start, const int32_t* end, const float* w, const float* vx, const float* vy,
+                 const float* vz, const float* pers, float* mvx, float* mvy, float* mvz) {
+    voxel_moments(V, start, end, w, vx, vy, vz, pers, mvx, mvy, mvz);
+}
+// Overwrites every cell of a map (GridCell records, index order), for
+// hand-built states.
+void orc_set_cells(void* hh, const k3dom_cell* in) {
+    Oracle* h = (Oracle*)hh;
+    for (int c = 0; c < h->V; ++c) {
+        const k3dom_cell& o = in[c];
+        h->start[c] = o.start_idx; h->end[c] = o.end_idx;
+        h->muA[c] = o.mu_A; h->muUA[c] = o.mu_UA; h->wA[c] = o.w_A; h->wUA[c] = o.w_UA;
+        h->mvx[c] = o.mean_x_vel; h->mvy[c] = o.mean_y_vel; h->mvz[c] = o.mean_z_vel;
+        h->lik[c] = o.likelihood; h->pA[c] = o.p_A;
+        h->f[c] = o.free_mass; h->s[c] = o.static_mass; h->d[c] = o.dynamic_mass;
+        h->occ[c] = o.occ_mass; h->pers[c] = o.pers_mass; h->pred[c] = o.pred_mass;
+    }
+}
+// Overwrites the particle positions (3 * N floats: x[N], y[N], z[N]).
+void orc_set_particle_pos(void* hh, const float* pos) {
+    Oracle* h = (Oracle*)hh;
+    for (int i = 0; i < h->N; ++i) {
+        h->px[i] = pos[i];
+        h->py[i] = pos[h->N + i];
+        h->pz[i] = pos[2 * h->N + i];
+    }
+}
+
 // BGKI of one frame on a grid, brute force or culled, returning the
 // contributing pair counts.
 void orc_bgki(int gs, int gsz, float res, float sx, float sy, float sz, float ls, float sigma,
@@ -1219,34 +1364,7 @@ void orc_bgki64(int gs, int gsz, float res, float sx, float sy, float sz, float 
 // prediction; out fsd after update, plus born.
 void orc_mass_update64(const float* fsd_in, double dyn_pred, double decay, double del_occ, double del_free,
                        double mass_scale, double p_B, double* out4) {
-    double f = fsd_in[0], s = fsd_in[1], d = fsd_in[2];
-    double pred_total = (f + d) * decay;
-    s *= decay;
-    d = std::min(dyn_pred, std::max(0.0, pred_total - s));
-    f = pred_total - d;
-    auto credit = [&](double m) { return std::tanh(m / mass_scale); };
-    if (del_free > 0 || del_occ > 0) {
-        double ds = del_free + del_occ, ps = f + s + d;
-        double c = credit(std::sqrt(ds * ps));
-        double l_DtoF = c * std::max(0.0, del_free - del_occ) / ds * d / ps;
-        double l_StoFD = c * std::max(0.0, del_free - del_occ) / ds * std::max(0.0, s + d - f) / ps;
-        double l_FtoD = c * std::max(0.0, del_occ - del_free) / ds * std::max(0.0, f - s - d) / ps;
-        double df = l_StoFD / 2 * s + l_DtoF * d - l_FtoD * f;
-        double dss = -l_StoFD * s;
-        double dd = l_FtoD * f + l_StoFD / 2 * s - l_DtoF * d;
-        f += df; s += dss; d += dd;
-        c = credit(s + d);
-        double ratio = (s <= 0 || d < 0) ? 0.0 : 1.0 / (1.0 + d / s);
-        double beta = (1 - c) + c * ratio;
-        s = s + beta * del_occ;
-        d = d + (1 - beta) * del_occ;
-        f = f + del_free;
-    }
-    double born;
-    if (dyn_pred <= 0) born = d;
-    else if (pred_total <= dyn_pred) born = 0;
-    else born = d * p_B * (pred_total - dyn_pred) / (dyn_pred + p_B * (pred_total - dyn_pred));
-    out4[0] = f; out4[1] = s; out4[2] = d; out4[3] = born;
+    mass_update64(fsd_in[0], fsd_in[1], fsd_in[2], dyn_pred, decay, del_occ, del_free, mass_scale, p_B, out4);
 }
 
 }  // extern "C"

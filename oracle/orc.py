@@ -70,6 +70,14 @@ def lib():
             "orc_predict_mass": (None, [fp, C.c_float, C.c_float, C.c_float]),
             "orc_newborn_part": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_float]),
             "orc_key": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_int, C.c_int]),
+            "orc_predict": (None, [C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                   C.c_int, C.c_uint32, C.c_uint64] + [fp] * 7 + [C.POINTER(C.c_int32)]),
+            "orc_sample_velocity": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_float, fp]),
+            "orc_moments": (None, [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32)] + [fp] * 8),
+            "orc_set_cells": (None, [C.c_void_p, C.c_void_p]),
+            "orc_set_particle_pos": (None, [C.c_void_p, fp]),
+            "orc_set_keep64": (None, [C.c_void_p, C.c_int]),
+            "orc_read64": (C.c_int64, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int64]),
             "orc_bgki": (None, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                 fp, fp, fp, C.c_int, C.c_int, fp, fp, C.POINTER(C.c_uint64)]),
             "orc_bgki64": (None, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
@@ -198,6 +206,35 @@ class OracleDOM:
         self._L.orc_bgki_pairs(self._h, out)
         return int(out[0]), int(out[1])
 
+    def set_cells(self, cells):
+        """Overwrites every cell with a GridCell-compatible structured array
+        (hand-built states)."""
+        from k3dom_amd.dom import CELL_DTYPE
+        a = np.ascontiguousarray(cells, dtype=CELL_DTYPE)
+        assert a.shape == (self.grid_cell_count,)
+        self._L.orc_set_cells(self._h, a.ctypes.data_as(C.c_void_p))
+
+    SHADOW64 = ["pre_f", "pre_s", "pre_d", "dyn", "f", "s", "d", "born", "mvx", "mvy", "mvz"]
+
+    def keep64(self, on=True):
+        """Keep an FP64 shadow of the floating-point stages of later frames."""
+        self._L.orc_set_keep64(self._h, 1 if on else 0)
+
+    def read64(self, name):
+        """FP64 shadow array of the last normal frame (SHADOW64 names)."""
+        out = np.empty(self.grid_cell_count, np.float64)
+        n = self._L.orc_read64(self._h, self.SHADOW64.index(name), out.ctypes.data_as(C.POINTER(C.c_double)),
+                               out.nbytes)
+        if n < 0:
+            raise ValueError(f"read64({name}): not kept")
+        return out
+
+    def set_particle_pos(self, pos):
+        """Overwrites the particle positions with a (3, N) float32 array."""
+        a = np.ascontiguousarray(pos, dtype=np.float32)
+        assert a.shape == (3, self.particle_count)
+        self._L.orc_set_particle_pos(self._h, _fp(a))
+
 
 def bgki(gs, gsz, res, sensor_rel, ls, sigma, xs, ys, zs, brute=False):
     """(del_occ, del_free, (occ_pairs, free_pairs)) of one point set on a grid."""
@@ -287,3 +324,35 @@ class WindowChain64:
             tm[k] -= h
             g[k] = (self.loss(tp) - self.loss(tm)) / (tp[k] - tm[k])
         return g
+
+
+def predict(px, py, pz, vx, vy, vz, w, dt, p_S, sigma_pos, sigma_vel, vclamp, gs, gsz, frame=0, seed=123456):
+    """The frame's motion step on copies of the given particle arrays; returns
+    (px, py, pz, vx, vy, vz, w, key)."""
+    arrs = [np.array(a, dtype=np.float32, copy=True) for a in (px, py, pz, vx, vy, vz, w)]
+    n = len(arrs[0])
+    key = np.empty(n, np.int32)
+    lib().orc_predict(n, dt, p_S, sigma_pos, sigma_vel, vclamp, gs, gsz, frame, seed, *[_fp(a) for a in arrs],
+                      key.ctypes.data_as(C.POINTER(C.c_int32)))
+    return (*arrs, key)
+
+
+def sample_velocity(r0, r1, r2, vmax, min_vel):
+    """The new-particle velocity of three Philox words."""
+    out = np.empty(3, np.float32)
+    lib().orc_sample_velocity(int(r0) & 0xFFFFFFFF, int(r1) & 0xFFFFFFFF, int(r2) & 0xFFFFFFFF, vmax, min_vel,
+                              _fp(out))
+    return out
+
+
+def moments(start, end, w, vx, vy, vz, pers):
+    """The frame's per-voxel mean velocities; returns (mvx, mvy, mvz)."""
+    start = np.ascontiguousarray(start, np.int32)
+    end = np.ascontiguousarray(end, np.int32)
+    w, vx, vy, vz, pers = (np.ascontiguousarray(a, np.float32) for a in (w, vx, vy, vz, pers))
+    V = len(start)
+    out = [np.empty(V, np.float32) for _ in range(3)]
+    i32 = C.POINTER(C.c_int32)
+    lib().orc_moments(V, start.ctypes.data_as(i32), end.ctypes.data_as(i32), _fp(w), _fp(vx), _fp(vy), _fp(vz),
+                      _fp(pers), *[_fp(o) for o in out])
+    return tuple(out)
