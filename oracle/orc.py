@@ -356,3 +356,13 @@ def moments(start, end, w, vx, vy, vz, pers):
     lib().orc_moments(V, start.ctypes.data_as(i32), end.ctypes.data_as(i32), _fp(w), _fp(vx), _fp(vy), _fp(vz),
                       _fp(pers), *[_fp(o) for o in out])
     return tuple(out)
+
+
+def mass_update64(fsd, dyn_pred, decay, del_occ, del_free, mass_scale, p_B):
+    """One voxel's mass rules in double from FP32 masses (f, s, d) before
+    prediction; returns (f, s, d, born) after the update."""
+    fsd = np.ascontiguousarray(fsd, np.float32)
+    out = (C.c_double * 4)()
+    lib().orc_mass_update64(_fp(fsd), float(dyn_pred), float(decay), float(del_occ), float(del_free),
+                            float(mass_scale), float(p_B), out)
+    return tuple(out)
