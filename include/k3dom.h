@@ -409,6 +409,10 @@ k3dom_status k3dom_window_set_frame_lidars(k3dom_window* w, int32_t f, double t,
 /* Runs the frames from a fresh map with parameters theta[6]; writes the loss. */
 k3dom_status k3dom_window_forward(k3dom_window* w, const float theta[6], double* loss);
 
+/* Number of forward passes run on w so far (a backward pass belongs to the
+ * last one: a caller that keeps several losses checks this number). */
+k3dom_status k3dom_window_forward_id(const k3dom_window* w, uint64_t* id);
+
 /* Gradient of dloss * loss with respect to theta after a forward.  When
  * grad_prior_voxels is not NULL it receives 3 * grid_cell_count floats, the
  * gradient with respect to each voxel's initial (free, static, dynamic) mass. */

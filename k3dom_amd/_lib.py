@@ -113,6 +113,7 @@ SIGNATURES = {
     "k3dom_window_set_frame_lidars": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _fp, C.POINTER(Lidar), C.c_int32,
                                                 C.POINTER(C.c_uint8)]),
     "k3dom_window_forward": (C.c_int, [C.c_void_p, _fp, C.POINTER(C.c_double)]),
+    "k3dom_window_forward_id": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "k3dom_window_backward": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_double), _fp]),
     "k3dom_window_read_record": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_int64]),
     "k3dom_window_times": (C.c_int, [C.c_void_p, _fp]),
@@ -149,4 +150,6 @@ def check(status, handle=None, what=""):
     if status != 0:
         lib = load()
         msg = lib.k3dom_last_error(handle)
-        raise K3domError(f"{what} failed with status {status}: {msg.decode() if msg else ''}")
+        err = K3domError(f"{what} failed with status {status}: {msg.decode() if msg else ''}")
+        err.status = status
+        raise err

@@ -226,4 +226,21 @@ hipError_t launch_window_clear(hipStream_t st, double* acc, int n) {
     return hipGetLastError();
 }
 
+// sticky |= *flag (one thread): keeps a per-frame flag that the next frame's
+// reset clears
+__global__ void k_flag_or(uint32_t* sticky, const uint32_t* flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *flag) *sticky = 1u;
+}
+__global__ void k_flag_clear(uint32_t* sticky) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *sticky = 0u;
+}
+hipError_t launch_flag_or(hipStream_t st, uint32_t* sticky, const uint32_t* flag) {
+    hipLaunchKernelGGL(k_flag_or, dim3(1), dim3(64), 0, st, sticky, flag);
+    return hipGetLastError();
+}
+hipError_t launch_flag_clear(hipStream_t st, uint32_t* sticky) {
+    hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(64), 0, st, sticky);
+    return hipGetLastError();
+}
+
 }  // namespace k3

@@ -495,6 +495,7 @@ void finish_frame(k3dom_handle* h, double t, int m) {
     if (h->updated_time > 0) h->normal_here++;
     h->frames_here++;
     h->last_grouped = h->ngroups > 0;
+    h->last_frame_ok = true;
     h->updated_time = t;
     h->frame++;
 }
@@ -503,7 +504,30 @@ bool frame_uses_graph(const k3dom_handle* h) {
     return h->updated_time > 0 && h->use_graph && !h->timing && h->ngroups == 0;
 }
 
+k3dom_status frame_body(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m);
+
+// A frame that fails (an allocation for a grown capacity, a HIP error) leaves
+// the state a later frame reads as it was: the particle set, the masses, the
+// pose, the time and the frame counter are only written by a frame that
+// completes, and the map shift still due on the particles and on the deferred
+// voxel fields stays due.  (The ranges, keys and evidence of the failed
+// frame's first stages may be left in the per-frame arrays until the next frame.)
 k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
+    const bool pshift_pending = h->pshift_pending, shift_pending = h->shift_pending;
+    const int pshift[3] = {h->pshift[0], h->pshift[1], h->pshift[2]};
+    const k3dom_status s = frame_body(h, t, dx, dy, dz, m);
+    if (s != K3DOM_OK) {
+        (void)hipStreamSynchronize(h->stream);
+        h->last_frame_ok = false;
+        h->pshift_pending = pshift_pending;
+        for (int k = 0; k < 3; ++k) h->pshift[k] = pshift[k];
+        // (re-applying a deferred shift that the failed frame settled is idempotent)
+        h->shift_pending = shift_pending;
+    }
+    return s;
+}
+
+k3dom_status frame_body(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
     const bool normal = h->updated_time > 0;
     {
         k3dom_status st = begin_frame_voxels(h, normal);
@@ -938,6 +962,12 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
     CREATE_TRY(hipMemsetAsync(h->ridx, 0, N * sizeof(int32_t), h->stream));
     CREATE_TRY(hipStreamSynchronize(h->stream));
 #undef CREATE_TRY
+    {
+        // test hook: the n-th device allocation after creation fails once
+        const char* fa = std::getenv("K3DOM_TEST_FAIL_ALLOC");
+        h->fail_alloc_at = fa ? std::atoi(fa) : 0;
+        h->alloc_count = 0;
+    }
     *out = h;
     return K3DOM_OK;
 }
@@ -1161,12 +1191,14 @@ k3dom_status k3dom_export_mappoints(k3dom_handle* h, int32_t mode, float min_occ
         if (st != K3DOM_OK) return st;
     }
     const size_t V = (size_t)h->V;
-    if (!h->mp_out) {
-        HIP_TRY(h, dalloc(h, &h->mp_out, 3 * V));
-        HIP_TRY(h, dalloc(h, &h->mp_flags, V));
-        HIP_TRY(h, dalloc(h, &h->mp_off, V + 1));
-        HIP_TRY(h, dalloc_zero(h, &h->mp_scratch, (size_t)scan_scratch_words((long long)V)));
-    }
+    // each buffer on its own while missing: an allocation that failed in an
+    // earlier call leaves the others in place and is tried again here
+    if (!h->mp_out) HIP_TRY(h, dalloc(h, &h->mp_out, 3 * V));
+    if (!h->mp_flags) HIP_TRY(h, dalloc(h, &h->mp_flags, V));
+    if (!h->mp_off) HIP_TRY(h, dalloc(h, &h->mp_off, V + 1));
+    if (!h->mp_scratch) HIP_TRY(h, dalloc_zero(h, &h->mp_scratch, (size_t)scan_scratch_words((long long)V)));
+    if (!h->mp_out || !h->mp_flags || !h->mp_off || !h->mp_scratch)
+        return fail(h, K3DOM_ERR_STATE, "export_mappoints: a buffer is missing");
     MapPointArgs a;
     a.gs = h->gs;
     a.gsz = h->gsz;
@@ -1260,9 +1292,11 @@ k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
     if (h->method != K3DOM_METHOD_K3DOM) return fail(h, K3DOM_ERR_STATE, "autotune tunes the K3DOM gather");
     if (h->normal_here < 1) return fail(h, K3DOM_ERR_STATE, "autotune needs a normal frame first");
-    hipEvent_t e0, e1;
-    HIP_TRY(h, hipEventCreate(&e0));
-    HIP_TRY(h, hipEventCreate(&e1));
+    if (!h->last_frame_ok || point_buffers_missing(h))
+        return fail(h, K3DOM_ERR_STATE, "autotune needs the last frame to have completed");
+    EventPair ev;  // destroyed on every return below
+    HIP_TRY(h, ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     const int cands[] = {1, 2, 3, 4, 5, 6, 8};
     float best = 1e30f;
     int best_c = h->gather_blocks / h->num_cus;
@@ -1285,8 +1319,6 @@ k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
             best_c = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     h->gather_blocks = best_c * h->num_cus;
     drop_graph(h);
     if (blocks_per_cu) *blocks_per_cu = best_c;
@@ -1317,6 +1349,8 @@ k3dom_status k3dom_count_bgki_pairs(k3dom_handle* h, uint64_t* out2) {
     if (h->method != K3DOM_METHOD_K3DOM) return fail(h, K3DOM_ERR_STATE, "pairs are counted for K3DOM maps");
     if (h->frames_here == 0) return fail(h, K3DOM_ERR_STATE, "no frame processed yet");
     if (h->last_grouped) return fail(h, K3DOM_ERR_STATE, "the pair counter covers single-sensor frames");
+    if (!h->last_frame_ok || point_buffers_missing(h))
+        return fail(h, K3DOM_ERR_STATE, "the pair counter needs the last frame to have completed");
     HIP_TRY(h, hipMemsetAsync(h->pair_counts, 0, 2 * sizeof(unsigned long long), h->stream));
     const uint32_t n_items = h->h_sc->n_light + h->h_sc->n_heavy;
     HIP_TRY(h, launch_bgki_count(h->stream, h->num_cus * 8, h->last_geom, h->tg, h->p.ls, h->ibuf, n_items,

@@ -279,14 +279,15 @@ k3dom_status k3dom_batch_update(k3dom_batch* b, double t, const float* poses, co
 k3dom_status k3dom_batch_autotune(k3dom_batch* b, int32_t* blocks_per_cu) {
     if (!b) return K3DOM_ERR_INVALID_ARG;
     for (k3dom_handle* h : b->maps)
-        if (h->normal_here < 1) return bfail(b, K3DOM_ERR_STATE, "autotune needs a normal frame on every map first");
+        if (h->normal_here < 1 || !h->last_frame_ok || point_buffers_missing(h))
+            return bfail(b, K3DOM_ERR_STATE, "autotune needs a completed normal frame on every map first");
     const int n = (int)b->maps.size();
     // the last frame's lists are still in place: re-run the joint gather over
     // them for each launch shape (the gather only rewrites the same evidence)
     B_HIP(b, hipMemcpyAsync(b->d_args, b->h_args, sizeof(BgArgs) * n, hipMemcpyHostToDevice, b->main));
-    hipEvent_t e0, e1;
-    B_HIP(b, hipEventCreate(&e0));
-    B_HIP(b, hipEventCreate(&e1));
+    EventPair ev;  // destroyed on every return below
+    B_HIP(b, ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     const int cands[] = {1, 2, 3, 4, 5, 6, 8};
     float best = 1e30f;
     int best_c = b->gather_blocks / b->num_cus;
@@ -306,8 +307,6 @@ k3dom_status k3dom_batch_autotune(k3dom_batch* b, int32_t* blocks_per_cu) {
             best_c = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     b->gather_blocks = best_c * b->num_cus;
     drop_batch_graph(b);
     if (blocks_per_cu) *blocks_per_cu = best_c;

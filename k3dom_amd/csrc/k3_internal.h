@@ -86,6 +86,9 @@ struct k3dom_handle {
     LevelBufs lbuf[3]{};
     ItemBufs ibuf{};
     bool tiny_caps = false;  // test hook: start every capacity small to exercise the regrowth path
+    // test hook (K3DOM_TEST_FAIL_ALLOC=n at creation): the n-th device
+    // allocation after creation reports out-of-memory
+    int fail_alloc_at = 0, alloc_count = 0;
     int overflow_retries = 0;
     DevScalars* h_sc = nullptr;  // pinned mirror
 
@@ -94,6 +97,9 @@ struct k3dom_handle {
     // created, reset or restored from a checkpoint: the lists and per-frame
     // arrays are its own only after one
     uint32_t frames_here = 0, normal_here = 0;
+    // the last frame completed (its lists and work items are whole); false
+    // after a frame that returned an error
+    bool last_frame_ok = false;
     bool first_pose = false;
     float sensor[3] = {0, 0, 0}, center[3] = {0, 0, 0};
     double updated_time = -1.0;
@@ -135,6 +141,7 @@ template <class T>
 hipError_t dalloc(k3dom_handle* h, T** p, size_t count) {
     void* q = nullptr;
     const size_t bytes = sizeof(T) * (count ? count : 1);
+    if (h->fail_alloc_at > 0 && ++h->alloc_count == h->fail_alloc_at) return hipErrorOutOfMemory;
     hipError_t e = hipMalloc(&q, bytes);
     if (e == hipSuccess) {
         h->allocs.push_back(q);
@@ -150,7 +157,22 @@ hipError_t dalloc_zero(k3dom_handle* h, T** p, size_t count) {
     return e;
 }
 void dfree(k3dom_handle* h, void* p);
+// Two timing events destroyed on every exit path of their scope.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t create() {
+        hipError_t e = hipEventCreate(&e0);
+        return e == hipSuccess ? hipEventCreate(&e1) : e;
+    }
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
 k3dom_status ensure_points(k3dom_handle* h, int m);
+// True when a buffer sized from the point capacity is missing (an allocation
+// of an earlier frame failed part way).
+bool point_buffers_missing(const k3dom_handle* h);
 GridGeom grid_geom(const k3dom_handle* h);
 // One filter frame on device-resident points (DOM::updateGrid).
 k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m);

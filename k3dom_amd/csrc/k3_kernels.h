@@ -318,6 +318,9 @@ hipError_t launch_window_record(hipStream_t st, int V, int gs, int gsz, bool nor
 // acc[0]: lambda_W (in: this frame's, out: the previous frame's); acc[1..6] += gradients
 hipError_t launch_window_adjoint(hipStream_t st, const AdjointFrameArgs& a, double* acc, double* partial);
 hipError_t launch_window_clear(hipStream_t st, double* acc, int n);
+// *sticky |= (*flag != 0), and *sticky = 0 (one-thread kernels, capture-safe)
+hipError_t launch_flag_or(hipStream_t st, uint32_t* sticky, const uint32_t* flag);
+hipError_t launch_flag_clear(hipStream_t st, uint32_t* sticky);
 
 hipError_t launch_reset_scalars(hipStream_t st, DevScalars* sc);
 hipError_t launch_clear_overflow(hipStream_t st, DevScalars* sc);

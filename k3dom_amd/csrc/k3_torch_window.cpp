@@ -32,7 +32,11 @@ struct K3WindowLoss : public torch::autograd::Function<K3WindowLoss> {
         torch::Tensor th = theta.detach().to(torch::kCPU, torch::kFloat32).contiguous();
         double loss = 0.0;
         check(k3dom_window_forward(w, th.data_ptr<float>(), &loss), w, "k3dom_window_forward");
+        uint64_t id = 0;
+        check(k3dom_window_forward_id(w, &id), w, "k3dom_window_forward_id");
         ctx->saved_data["handle"] = handle;
+        ctx->saved_data["theta"] = th;
+        ctx->saved_data["forward_id"] = static_cast<int64_t>(id);
         ctx->saved_data["dtype"] = theta.scalar_type();
         ctx->saved_data["device"] = theta.device();
         return torch::tensor(loss, torch::dtype(torch::kFloat64));
@@ -42,6 +46,15 @@ struct K3WindowLoss : public torch::autograd::Function<K3WindowLoss> {
                                                    torch::autograd::variable_list grad_out) {
         k3dom_window* w = as_window(ctx->saved_data["handle"].toInt());
         const double g = grad_out[0].defined() ? grad_out[0].to(torch::kCPU, torch::kFloat64).item<double>() : 0.0;
+        // the window's records are those of its last forward: when another
+        // forward ran since this node's, run this node's forward again first
+        uint64_t id = 0;
+        check(k3dom_window_forward_id(w, &id), w, "k3dom_window_forward_id");
+        if (static_cast<int64_t>(id) != ctx->saved_data["forward_id"].toInt()) {
+            torch::Tensor th = ctx->saved_data["theta"].toTensor();
+            double loss = 0.0;
+            check(k3dom_window_forward(w, th.data_ptr<float>(), &loss), w, "k3dom_window_forward");
+        }
         double grad[6];
         check(k3dom_window_backward(w, g, grad, nullptr), w, "k3dom_window_backward");
         torch::Tensor gt = torch::from_blob(grad, {6}, torch::kFloat64).clone();

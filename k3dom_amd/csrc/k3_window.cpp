@@ -64,6 +64,8 @@ struct k3dom_window {
     float *g_o = nullptr, *g_f = nullptr, *g_lo = nullptr, *g_lf = nullptr;
     float theta[6] = {0, 0, 0, 0, 0, 0};
     bool forward_done = false;
+    uint64_t forward_id = 0;       // forward passes run so far (k3dom_window_forward_id)
+    uint32_t* overflow_any = nullptr;  // device: a frame of the backward pass overflowed
     float ms[2] = {0, 0};
     std::string err;
 };
@@ -152,6 +154,7 @@ k3dom_status k3dom_window_create(const k3dom_params* params, int32_t frames, k3d
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < 3; ++k) WC(walloc(w, &w->lam[b][k], V));
     WC(walloc(w, &w->acc, 8));
+    WC(walloc(w, &w->overflow_any, 1));
     WC(walloc(w, &w->adj_partial, (size_t)window_blocks(w->V) * 8));
     WC(walloc(w, &w->d_fc, (size_t)frames));
     WC(hipHostMalloc((void**)&w->h_fc, sizeof(FrameConsts) * frames, hipHostMallocDefault));
@@ -315,6 +318,7 @@ k3dom_status k3dom_window_forward(k3dom_window* w, const float theta[6], double*
     for (double x : lp) total += x;
     *loss = total / ((double)w->T * (double)w->mask_count);
     w->forward_done = true;
+    w->forward_id++;
     w->ms[0] = (float)(now_ms() - t0);
     return K3DOM_OK;
 }
@@ -326,6 +330,7 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
     const double t0 = now_ms();
     k3dom_status s;
     W_HIP(w, launch_window_clear(h->stream, w->acc, 8));
+    W_HIP(w, launch_flag_clear(h->stream, w->overflow_any));
     const double loss_scale = dloss / ((double)w->T * (double)w->mask_count);
     int cur = 0;  // lam[cur] receives this frame's pre-state adjoint; lam[1 - cur] holds the next frame's
     for (int f = w->T - 1; f >= 0; --f) {
@@ -368,6 +373,8 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
                 }
             }
         }
+        // the next frame's scalar reset clears the flag: keep it
+        W_HIP(w, launch_flag_or(h->stream, w->overflow_any, &h->sc->overflow));
         AdjointFrameArgs a{};
         a.V = w->V;
         a.gs = h->gs;
@@ -410,7 +417,7 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
     double acc[8];
     uint32_t overflow = 0;
     W_HIP(w, hipMemcpyAsync(acc, w->acc, sizeof(acc), hipMemcpyDeviceToHost, h->stream));
-    W_HIP(w, hipMemcpyAsync(&overflow, &h->sc->overflow, sizeof(overflow), hipMemcpyDeviceToHost, h->stream));
+    W_HIP(w, hipMemcpyAsync(&overflow, w->overflow_any, sizeof(overflow), hipMemcpyDeviceToHost, h->stream));
     std::vector<double> pv;
     if (grad_prior_voxels) {
         // the adjoint of frame 0's pre-state sits in lam[1 - cur] after the last swap
@@ -424,6 +431,12 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
     if (overflow) return wfail(w, K3DOM_ERR_STATE, "candidate lists overflowed in the backward pass");
     for (int k = 0; k < 6; ++k) grad[k] = acc[1 + k];
     w->ms[1] = (float)(now_ms() - t0);
+    return K3DOM_OK;
+}
+
+k3dom_status k3dom_window_forward_id(const k3dom_window* w, uint64_t* id) {
+    if (!w || !id) return K3DOM_ERR_INVALID_ARG;
+    *id = w->forward_id;
     return K3DOM_OK;
 }
 

@@ -41,10 +41,13 @@
 //     reference formula, no square root), exactly zero from d^2 = ls^2 on;
 //   * gamma^dt is computed once per frame on the host.
 //
-// Parity pins for this file live in tests/test_oracle.py: brute-force BGKI ==
-// culled BGKI bit for bit, closed-form mass rules, invariants (stable sort
+// Parity pins for this file live in tests/test_oracle.py (brute-force BGKI ==
+// culled BGKI bit for bit, closed-form mass rules, invariants: stable sort
 // permutation, ranges tile [0,N), sum of newborn counts == B, uniform
-// resampled weights), and the FP64 stage evaluators further below.
+// resampled weights), tests/test_oracle_pins.py (motion step, velocity draw,
+// moments and map shift against closed forms, statistics and hand-built
+// cases), tests/test_invariants.py (frame properties) and
+// tests/test_fp64_errors.py (the FP64 stage evaluators and FP64 shadow below).
 
 #include "../include/k3dom.h"
 #include "../k3dom_amd/csrc/k3_math.h"

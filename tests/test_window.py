@@ -183,6 +183,30 @@ def test_window_torch_autograd_and_per_voxel_priors(gpu):
         assert abs(pv[k].astype(np.float64).sum() - g[3 + k]) <= 1e-4 * abs(g[3 + k]) + 1e-6
 
 
+@pytest.mark.gpu
+def test_window_two_losses_before_backward(gpu):
+    """Two forward passes on one window before the first loss's backward
+    (a second loss, or a validation loss): each loss's gradient is the one
+    at its own theta, as computed alone."""
+    import torch
+    from k3dom_amd.adjoint import theta_of
+    w = _small(frames=3, seed=14)
+    win = _gpu_window(w, 3)
+    th0 = np.array(theta_of(w.params), np.float32)
+    th1 = th0 * np.array([1.3, 0.7, 1.2, 2.0, 0.5, 1.5], np.float32)
+    a = torch.tensor(th0, requires_grad=True)
+    b = torch.tensor(th1, requires_grad=True)
+    la = win.loss(a)
+    lb = win.loss(b)
+    la.backward()
+    lb.backward()
+    for th, t in ((th0, a), (th1, b)):
+        win.forward(th)
+        g = win.backward()
+        assert np.array_equal(t.grad.numpy(), g.astype(np.float32)), (t.grad.numpy(), g)
+    assert not np.allclose(a.grad.numpy(), b.grad.numpy())
+
+
 # ------------------------------------------------------------ several LiDARs
 
 def _rotate32(rot, xs, ys, zs):
