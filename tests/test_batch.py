@@ -1,6 +1,7 @@
 """Batched maps: every map of a batch (joint gather in one persistent launch,
 maps' stages on concurrent streams, optionally one captured graph per frame)
-gives exactly the results of the same map updated alone."""
+gives exactly the results of the same map updated alone, and of the serial
+restatement of that map."""
 import numpy as np
 import pytest
 
@@ -34,28 +35,39 @@ def _compare_maps(batch, solos, normal, what):
             assert bits_equal(a, b), f"{what} map {i} {name}: {first_diff(a, b)}"
 
 
-def _run(graph, tiny, monkeypatch, autotune=False):
+def _run(graph, tiny, monkeypatch, autotune=False, ws=None, mutate=None):
+    """Runs a batch, each map alone and each map's serial restatement over the
+    same frames and compares them after every frame.  mutate(i, f, rec)
+    may change map i's frame f (time, pose, points)."""
     from k3dom_amd.batch import Batch
     from k3dom_amd.dom import DOM
     from k3dom_amd.scene import Scene
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_state
     if tiny:
         monkeypatch.setenv("K3DOM_TEST_TINY_CAPACITY", "1")
-    ws = _tiles()
+    ws = ws or _tiles()
     scenes = [Scene(w.scene) for w in ws]
     batch = Batch([w.params for w in ws])
     if graph:
         batch.set_graph(True)
     solos = [DOM(w.params) for w in ws]
+    oracles = [OracleDOM(w.params) for w in ws]
     for f in range(ws[0].frames):
         recs = [s.frame(f) for s in scenes]
+        if mutate is not None:
+            recs = [mutate(i, f, r) for i, r in enumerate(recs)]
         t = recs[0][0]
         print(f"  batch frame {f}: points {[len(r[2]) for r in recs]}", flush=True)
         normal = batch.maps[0].info().updated_time > 0
         batch.update(t, [r[1] for r in recs], [r[2:] for r in recs])
-        for sm, r in zip(solos, recs):
-            sm.updatePose(*r[1])
-            sm.updateGrid(t, *r[2:])
+        for sm, om, r in zip(solos, oracles, recs):
+            for m in (sm, om):
+                m.updatePose(*r[1])
+                m.updateGrid(t, *r[2:])
         _compare_maps(batch, solos, normal, f"frame {f}")
+        for i, (bm, om) in enumerate(zip(batch.maps, oracles)):
+            compare_state(bm, om, normal_frame=normal, what=f"frame {f} map {i} vs restatement:")
         if autotune and f == 1:
             bpc = batch.autotune()
             assert bpc in (1, 2, 3, 4, 5, 6, 8)
@@ -76,6 +88,54 @@ def test_batch_graph_equals_solo_maps(gpu, monkeypatch):
 def test_batch_capacity_regrowth_equals_solo_maps(gpu, monkeypatch):
     batch, _ = _run(graph=True, tiny=True, monkeypatch=monkeypatch)
     assert any(m.stats()["capacity_regrowths"] > 0 for m in batch.maps)
+
+
+@pytest.mark.gpu
+def test_batch_with_an_empty_map_and_regrowth(gpu, monkeypatch):
+    """One map gets no points in any frame (its gather items are all empty)
+    while the others overflow their tiny starting capacities, grow and re-run:
+    every map still equals its solo run and its serial restatement."""
+    def mutate(i, f, rec):
+        t, pose, xs, ys, zs = rec
+        return (t, pose, xs[:0], ys[:0], zs[:0]) if i == 1 else rec
+    batch, _ = _run(graph=True, tiny=True, monkeypatch=monkeypatch, ws=_tiles(n=3, seed=11), mutate=mutate)
+    st1 = batch.maps[1].stats()
+    assert st1["points"] == 0 and st1["level_entries"] == [0, 0, 0] and st1["gather_split_items"] == 0
+    assert any(m.stats()["capacity_regrowths"] > 0 for i, m in enumerate(batch.maps) if i != 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_batch_of_32_uneven_tiles_full_size(gpu):
+    """The batched-tiles configuration itself: 32 tiles, edges 32-128 voxels
+    at 0.1 m, 16k-1M particles, 32x1024 LiDAR each, one persistent gather
+    and a captured graph per frame.  Every map passes the size-independent
+    frame properties; the six smallest maps also equal their serial
+    restatements bit for bit."""
+    from k3dom_amd.batch import Batch
+    from k3dom_amd.scene import Scene, batch_workloads
+    from oracle.orc import OracleDOM
+    from tests._invariants import check_frame_invariants
+    from tests._parity import compare_state
+    ws = batch_workloads()
+    scenes = [Scene(w.scene) for w in ws]
+    batch = Batch([w.params for w in ws])
+    batch.set_graph(True)
+    cost = [w.params.particle_count + int(w.params.size / w.params.resolution) ** 3 for w in ws]
+    small = sorted(range(len(ws)), key=lambda i: cost[i])[:6]
+    oracles = {i: OracleDOM(ws[i].params) for i in small}
+    for f in range(3):
+        recs = [s.frame(f) for s in scenes]
+        normal = batch.maps[0].info().updated_time > 0
+        batch.update(recs[0][0], [r[1] for r in recs], [r[2:] for r in recs])
+        print(f"  32-tile batch frame {f}: {sum(len(r[2]) for r in recs)} points, {batch.last_ms():.1f} ms",
+              flush=True)
+        for i, om in oracles.items():
+            om.updatePose(*recs[i][1])
+            om.updateGrid(recs[i][0], *recs[i][2:])
+            compare_state(batch.maps[i], om, normal_frame=normal, what=f"frame {f} tile {i} vs restatement:")
+        for i, m in enumerate(batch.maps):
+            check_frame_invariants(m, normal, what=f"frame {f} tile {i}:")
 
 
 @pytest.mark.gpu

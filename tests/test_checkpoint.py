@@ -84,22 +84,28 @@ def _compare(a, b, what, normal=True):
 def test_resume_continues_bit_exact(gpu, method, tmp_path):
     """Checkpoint after a shifting updatePose (shift still pending), restore
     from a file, continue both maps over the same frames (the restored one
-    graph-replayed from its second frame on)."""
+    graph-replayed from its second frame on); the restored map also equals
+    the serial restatement run through every frame without interruption."""
     from k3dom_amd.dom import DOM
     from k3dom_amd.scene import Scene, small_workload
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_state
     w = small_workload(ego_vx=4.0, frames=11, seed=3, method=method)
     sc = Scene(w.scene)
     frames = [sc.frame(f) for f in range(w.frames)]
     a = DOM(w.params, method=method)
+    o = OracleDOM(w.params, method=method)  # the serial restatement, never interrupted
     k = None
     for f in range(w.frames):
         t, pose, xs, ys, zs = frames[f]
         c0 = a.info().center_pos[0]
         a.updatePose(*pose)
+        o.updatePose(*pose)
         if f >= 3 and a.info().center_pos[0] != c0:
             k = f
             break
         a.updateGrid(t, xs, ys, zs)
+        o.updateGrid(t, xs, ys, zs)
     assert k is not None, "no map shift in the first frames"
     print(f"  {method}: checkpoint after the shifting pose of frame {k}", flush=True)
     path = str(tmp_path / "map.k3ck")
@@ -107,13 +113,15 @@ def test_resume_continues_bit_exact(gpu, method, tmp_path):
     b = DOM.load(path)
     assert b.method == method and b.params.particle_count == w.params.particle_count
     t, pose, xs, ys, zs = frames[k]
-    for m in (a, b):
+    for m in (a, b, o):
         m.updateGrid(t, xs, ys, zs)
     _compare(a, b, f"frame {k} after resume:")
+    compare_state(b, o, normal_frame=True, what=f"frame {k} after resume vs restatement:")
     b.set_graph(True)
     for f in range(k + 1, w.frames):
-        _run((a, b), frames[f])
+        _run((a, b, o), frames[f])
         _compare(a, b, f"frame {f} after resume:")
+        compare_state(b, o, normal_frame=True, what=f"frame {f} after resume vs restatement:")
 
 
 @pytest.mark.gpu

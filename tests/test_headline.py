@@ -57,3 +57,33 @@ def test_config2_invariants_over_frames(gpu):
         if f == 0 or f >= 2:
             print(f"  frame {f}: {len(xs)} points", flush=True)
             check_frame_invariants(gd, normal, what=f"cfg2 frame {f}:")
+
+
+@pytest.mark.timeout(1200)
+def test_config5_four_lidars_bit_exact(gpu):
+    """The stress configuration: 512x512x64 voxels at 0.1 m, 8,000,000
+    particles + 800,000 newborn, four 128x2048 LiDARs at the vehicle's
+    corners 90 degrees apart (about 415k points per frame), ego at 15 m/s.
+    The initialisation frame and one normal frame (after a map shift) equal
+    the serial restatement's multi-LiDAR frames bit for bit."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, workload
+    from oracle import orc
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_state
+    orc.set_threads(_host_threads())
+    try:
+        w = workload(5)
+        sc = Scene(w.scene)
+        gd, od = DOM(w.params), OracleDOM(w.params)
+        for f in range(2):
+            t, pose, lidars = sc.frame_lidars(f, w.extra["rig"])
+            print(f"  frame {f}: {sum(len(l[2]) for l in lidars)} points", flush=True)
+            normal = gd.info().updated_time > 0
+            for m in (gd, od):
+                m.updatePose(*pose)
+                m.updateGridLidars(t, lidars)
+            compare_state(gd, od, normal_frame=normal, what=f"cfg5 frame {f}:")
+        assert gd.info().center_pos[0] == od.info().center_pos[0] > 0
+    finally:
+        orc.set_threads(1)
