@@ -33,7 +33,8 @@ def _run(cmd):
 def build_lib(verbose_asm=False):
     out = os.path.join(HERE, "libk3dom.so")
     srcs = [os.path.join(CSRC, f) for f in ("k3_kernels.hip", "k3_bgki.hip", "k3_adjoint.hip", "k3_noalloc.hip", "k3_api.cpp",
-                                             "k3_window.cpp", "k3_batch.cpp", "k3_checkpoint.cpp")]
+                                             "k3_params.cpp", "k3_window.cpp", "k3_batch.cpp", "k3_checkpoint.cpp",
+                                             "k3_ckformat.cpp")]
     cmd = ["hipcc", *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", out + ".tmp", *srcs]
     _run(cmd)
     os.replace(out + ".tmp", out)

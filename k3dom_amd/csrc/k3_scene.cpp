@@ -15,26 +15,7 @@
 #include <cstring>
 #include <vector>
 
-extern "C" {
-typedef struct k3scene_desc {
-    uint64_t seed;
-    int32_t n_static, n_moving;
-    float side_min, side_max;      // box edge lengths ~ U[side_min, side_max] per axis
-    float speed_min, speed_max;    // moving boxes: speed ~ U[.,.], heading ~ U[0, 2pi)
-    float xmin, xmax, ymin, ymax;  // box centres ~ U over this rectangle (global frame)
-    float ground_z;                // boxes stand on this plane
-    float clearance;               // box footprints keep this distance from the ego path
-    float ego_x0, ego_y0, ego_z0;  // ego (sensor) position at frame 0
-    float ego_vx, ego_vy;          // ego velocity (m/s)
-    int32_t beams, az_bins;
-    float elev_min_deg, elev_max_deg;
-    float max_range, range_noise;
-    float frame_dt;                // seconds between frames
-    double t0;                     // time stamp of frame 0
-    int32_t ground;                // 1: the plane z = ground_z also returns points
-    int32_t pad;
-} k3scene_desc;
-}
+#include "k3_scene.h"
 
 namespace {
 

@@ -1159,6 +1159,7 @@ int64_t orc_read_array(void* hh, int which, void* dst, int64_t capacity) {
     auto put3 = [&](const std::vector<float>& a, const std::vector<float>& b, const std::vector<float>& c) -> int64_t {
         int64_t bytes = (int64_t)(a.size() * 3 * 4);
         if (capacity < bytes) return -1;
+        if (bytes == 0) return 0;  // (no memcpy from an empty vector's null data)
         float* o = (float*)dst;
         memcpy(o, a.data(), a.size() * 4);
         memcpy(o + a.size(), b.data(), b.size() * 4);
@@ -1167,6 +1168,7 @@ int64_t orc_read_array(void* hh, int which, void* dst, int64_t capacity) {
     };
     auto put = [&](const void* src, size_t bytes) -> int64_t {
         if (capacity < (int64_t)bytes) return -1;
+        if (bytes == 0) return 0;
         memcpy(dst, src, bytes);
         return (int64_t)bytes;
     };
