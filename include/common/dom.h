@@ -159,6 +159,17 @@ public:
                getCenterPositionZ();
     }
 
+    // The reference's stage methods (include/common/dom.h:89-94), called in
+    // the order of DOM::updateGrid (k3dom.h: k3dom_particle_prediction ...).
+    void particlePrediction(float dt) { check(k3dom_particle_prediction(handle_, dt), "particlePrediction"); }
+    void particleAssignment() { check(k3dom_particle_assignment(handle_), "particleAssignment"); }
+    void updatePersistentParticles() {
+        check(k3dom_update_persistent_particles(handle_), "updatePersistentParticles");
+    }
+    void initializeNewParticles() { check(k3dom_initialize_new_particles(handle_), "initializeNewParticles"); }
+    void statisticalMoments() { check(k3dom_statistical_moments(handle_), "statisticalMoments"); }
+    void resampling() { check(k3dom_resampling(handle_), "resampling"); }
+
     // The reference never initialises or advances its iteration counter
     // (src/common/dom.cu:29-34); this one counts the frames processed.
     int getIteration() const { return info().iteration; }

@@ -156,6 +156,40 @@ k3dom_status k3dom_update_grid(k3dom_handle* h, double t, const float* xs, const
 k3dom_status k3dom_update_grid_device(k3dom_handle* h, double t, const float* d_xs,
                                       const float* d_ys, const float* d_zs, int32_t n);
 
+/* The reference's public stage methods of DOM_c (include/common/dom.h:89-94)
+ * and DOM (include/k3dom/dom.h:24-28), for callers that drive a frame stage by
+ * stage.  A normal frame calls them in the order of DOM::updateGrid
+ * (src/k3dom/dom.cu:33-44):
+ *   k3dom_particle_prediction(h, dt)        particlePrediction(dt)
+ *   k3dom_particle_assignment(h)            particleAssignment()
+ *   k3dom_grid_cell_occupancy_update(h, dt, xs, ys, zs, n)
+ *                                           gridCellOccupancyUpdate(dt, points)
+ *   k3dom_update_persistent_particles(h)    updatePersistentParticles()
+ *   k3dom_initialize_new_particles(h)       initializeNewParticles()
+ *   k3dom_statistical_moments(h)            statisticalMoments()
+ *   k3dom_resampling(h)                     resampling() and the copy-back
+ * and a first frame calls k3dom_initialize_particles (initializeParticles,
+ * src/k3dom/dom.cu:52-77).  Each runs its part of the frame's kernels and
+ * returns when it is done; the frame's results equal those of the same
+ * frame given to k3dom_update_grid (dt = t - the previous frame's time), bit
+ * for bit.  As in the reference, the stages leave the stored time of the last
+ * frame to updateGrid.  The persistent update also computes the moments (the
+ * same weights and sums), so statisticalMoments may come before or after
+ * initializeNewParticles and launches nothing.  A stage called out of this
+ * order, a whole-frame call in the middle of a staged frame, or a
+ * DS-PHD/MIB map returns K3DOM_ERR_STATE; a stage that fails abandons the
+ * frame.  Points are host arrays relative to the sensor in map-aligned axes. */
+k3dom_status k3dom_particle_prediction(k3dom_handle* h, float dt);
+k3dom_status k3dom_particle_assignment(k3dom_handle* h);
+k3dom_status k3dom_grid_cell_occupancy_update(k3dom_handle* h, float dt, const float* xs, const float* ys,
+                                              const float* zs, int32_t n);
+k3dom_status k3dom_update_persistent_particles(k3dom_handle* h);
+k3dom_status k3dom_initialize_new_particles(k3dom_handle* h);
+k3dom_status k3dom_statistical_moments(k3dom_handle* h);
+k3dom_status k3dom_resampling(k3dom_handle* h);
+k3dom_status k3dom_initialize_particles(k3dom_handle* h, const float* xs, const float* ys, const float* zs,
+                                        int32_t n);
+
 /* Geometry, pose, time and frame count (include/common/dom.h:59-83). */
 k3dom_status k3dom_get_info(const k3dom_handle* h, k3dom_info* out);
 

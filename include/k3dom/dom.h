@@ -22,6 +22,22 @@ public:
         updateGrid((double)t, measurements_x.data(), measurements_y.data(), measurements_z.data(),
                    (int)measurements_x.size());
     }
+    // src/k3dom/dom.cu:52-77 and :79-106, the first-frame and evidence stages
+    // of updateGrid for callers that drive the stages themselves (k3dom.h).
+    void initializeParticles(std::vector<float>& measurements_x, std::vector<float>& measurements_y,
+                             std::vector<float>& measurements_z) {
+        assert(measurements_x.size() == measurements_y.size() && measurements_y.size() == measurements_z.size());
+        check(k3dom_initialize_particles(handle_, measurements_x.data(), measurements_y.data(),
+                                         measurements_z.data(), (int32_t)measurements_x.size()),
+              "initializeParticles");
+    }
+    void gridCellOccupancyUpdate(float dt, std::vector<float>& measurements_x, std::vector<float>& measurements_y,
+                                 std::vector<float>& measurements_z) {
+        assert(measurements_x.size() == measurements_y.size() && measurements_y.size() == measurements_z.size());
+        check(k3dom_grid_cell_occupancy_update(handle_, dt, measurements_x.data(), measurements_y.data(),
+                                               measurements_z.data(), (int32_t)measurements_x.size()),
+              "gridCellOccupancyUpdate");
+    }
     // Same with a double time stamp (seconds) and plain host arrays of n points.
     void updateGrid(double t, const float* xs, const float* ys, const float* zs, int n) {
         check(k3dom_update_grid(handle_, t, xs, ys, zs, (int32_t)n), "updateGrid");

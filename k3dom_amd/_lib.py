@@ -89,6 +89,14 @@ SIGNATURES = {
                                          C.POINTER(C.c_int64)]),
     "k3dom_export_mappoints_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_void_p,
                                                  C.c_int64, C.POINTER(C.c_int64)]),
+    "k3dom_particle_prediction": (C.c_int, [C.c_void_p, C.c_float]),
+    "k3dom_particle_assignment": (C.c_int, [C.c_void_p]),
+    "k3dom_grid_cell_occupancy_update": (C.c_int, [C.c_void_p, C.c_float, _fp, _fp, _fp, C.c_int32]),
+    "k3dom_update_persistent_particles": (C.c_int, [C.c_void_p]),
+    "k3dom_initialize_new_particles": (C.c_int, [C.c_void_p]),
+    "k3dom_statistical_moments": (C.c_int, [C.c_void_p]),
+    "k3dom_resampling": (C.c_int, [C.c_void_p]),
+    "k3dom_initialize_particles": (C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_int32]),
     "k3dom_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "k3dom_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "k3dom_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),

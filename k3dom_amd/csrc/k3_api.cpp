@@ -230,7 +230,10 @@ k3dom_status build_lists(k3dom_handle* h, const FrameConsts* fc, const float* xs
 
 void fill_frame_consts(k3dom_handle* h, FrameConsts* c, double t, int m) {
     const bool normal = h->updated_time > 0;
-    const float dt = normal ? (float)(t - h->updated_time) : 0.0f;
+    fill_frame_consts_dt(h, c, normal, normal ? (float)(t - h->updated_time) : 0.0f, m);
+}
+
+void fill_frame_consts_dt(k3dom_handle* h, FrameConsts* c, bool normal, float dt, int m) {
     c->g = grid_geom(h);
     c->decay = normal ? (float)std::pow((double)h->p.gamma, (double)dt) : 1.0f;
     c->frame = h->frame;
@@ -253,67 +256,98 @@ void fill_frame_consts(k3dom_handle* h, FrameConsts* c, double t, int m) {
     pc.key1 = (uint32_t)(h->p.seed >> 32);
 }
 
-// Everything after the gather: combination of split items, mass update,
-// persistent update, newborns, resampling, scalar readback.  Tail kernels
-// return at once when the overflow flag is set, so the frame can be re-run.
-k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
+// The frame's tail in the stage order of DOM::updateGrid (src/k3dom/dom.cu:
+// 38-43): tail_mass (the part of gridCellOccupancyUpdate after the
+// evidence), tail_persistent (updatePersistentParticles, with
+// statisticalMoments and the static penalty of resampling fused into it: the
+// moments read the weights after the persistent update and before the
+// penalty, and the newborn stage between them changes neither), tail_newborn
+// (initializeNewParticles, or the first frame's particle placement) and
+// tail_resample (resampling).  Tail kernels return at once when the overflow
+// flag is set, so the frame can be re-run.
+namespace {
+// per-block maxima: born masses first, then the joint particle weights
+uint32_t* born_parts(k3dom_handle* h) { return h->maxparts; }
+uint32_t* joint_parts(k3dom_handle* h) { return h->maxparts + mass_update_parts(h->V); }
+}  // namespace
+
+k3dom_status tail_mass(k3dom_handle* h, bool normal) {
     VoxelArrays& v = h->vox[h->vcur];
-    const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
     const uint32_t* ab = &h->sc->overflow;
     const bool ds = h->method == K3DOM_METHOD_DSPHDMIB;
     if (!ds && h->ngroups == 0)
         HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, grid_geom(h), h->tg, h->ibuf, h->del_occ, h->del_free,
                                        h->sc));
     mark(h, ST_MASS);
-    // per-block maxima: born masses first, then the joint particle weights
-    uint32_t* born_parts = h->maxparts;
-    uint32_t* joint_parts = h->maxparts + mass_update_parts(h->V);
-    const int n_born_parts = mass_update_parts(h->V);
     if (normal) {
-        const int n_persist_parts = voxel_particles_parts(h->N);
         if (ds)
             HIP_TRY(h, launch_ds_mass_update(h->stream, ab, v, h->V, h->wsort, h->P[1].key, h->N, h->usum,
                                              h->del_occ, h->del_free, h->p.birth_prob, h->born, h->ds_norm,
-                                             born_parts));
+                                             born_parts(h)));
         else
             HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, h->wsort, h->P[1].key, h->N, h->usum, h->del_occ,
                                           h->del_free, h->fc, h->p.mass_scale, h->p.birth_prob, h->born,
-                                          born_parts));
-        mark(h, ST_PERSIST);
-        HIP_TRY(h, launch_voxel_particles(h->stream, ab, v, h->V, h->P[1], h->wsort, h->N, h->p.particle_min_vel,
-                                          h->usum, joint_parts));
-        mark(h, ST_NEWBORN);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, born_parts, n_born_parts, &h->sc->E_born,
-                                     h->cdf_v, h->blocksums, &h->sc->total_born));
-        HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->B, h->born, h->bnd,
-                                 v.wua));
-        HIP_TRY(h, launch_newborn(h->stream, ab, h->bnd, h->V, h->B, h->gs, v.wua, h->NB, h->p.init_max_velocity,
-                                  h->p.particle_min_vel, h->fc, k0, k1, joint_parts + n_persist_parts));
-        mark(h, ST_MOMENTS);
-        mark(h, ST_RESAMPLE);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->P[1].w, h->N, h->NB.w, h->B, joint_parts,
-                                     n_persist_parts + newborn_parts(h->B), &h->sc->E_joint, h->cdf_j, h->blocksums,
-                                     &h->sc->total_joint));
-        HIP_TRY(h, launch_resample(h->stream, ab, h->cdf_j, h->N + h->B, &h->sc->total_joint, &h->sc->E_joint,
-                                   h->P[1], h->NB, h->P[0], h->N, h->fc, k0, k1, h->ridx));
+                                          born_parts(h)));
     } else {
         if (ds)
-            HIP_TRY(h, launch_ds_born_from_meas(h->stream, h->del_occ, h->born, h->V, born_parts));
+            HIP_TRY(h, launch_ds_born_from_meas(h->stream, h->del_occ, h->born, h->V, born_parts(h)));
         else
             HIP_TRY(h, launch_mass_update(h->stream, ab, v, h->V, nullptr, nullptr, 0, h->usum, h->del_occ,
                                           h->del_free, nullptr, h->p.mass_scale, h->p.birth_prob, h->born,
-                                          born_parts));
-        mark(h, ST_PERSIST);
-        mark(h, ST_NEWBORN);
-        HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, born_parts, n_born_parts, &h->sc->E_born,
-                                     h->cdf_v, h->blocksums, &h->sc->total_born));
+                                          born_parts(h)));
+    }
+    return K3DOM_OK;
+}
+
+k3dom_status tail_persistent(k3dom_handle* h) {
+    mark(h, ST_PERSIST);
+    HIP_TRY(h, launch_voxel_particles(h->stream, &h->sc->overflow, h->vox[h->vcur], h->V, h->P[1], h->wsort, h->N,
+                                      h->p.particle_min_vel, h->usum, joint_parts(h)));
+    return K3DOM_OK;
+}
+
+k3dom_status tail_newborn(k3dom_handle* h, bool normal) {
+    VoxelArrays& v = h->vox[h->vcur];
+    const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
+    const uint32_t* ab = &h->sc->overflow;
+    if (!normal) mark(h, ST_PERSIST);
+    mark(h, ST_NEWBORN);
+    HIP_TRY(h, launch_fixed_scan(h->stream, h->born, h->V, nullptr, 0, born_parts(h), mass_update_parts(h->V),
+                                 &h->sc->E_born, h->cdf_v, h->blocksums, &h->sc->total_born));
+    if (normal) {
+        HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->B, h->born, h->bnd,
+                                 v.wua));
+        HIP_TRY(h, launch_newborn(h->stream, ab, h->bnd, h->V, h->B, h->gs, v.wua, h->NB, h->p.init_max_velocity,
+                                  h->p.particle_min_vel, h->fc, k0, k1, joint_parts(h) + voxel_particles_parts(h->N)));
+    } else {
         HIP_TRY(h, launch_bounds(h->stream, ab, h->cdf_v, &h->sc->total_born, h->V, (uint32_t)h->N, h->born, h->bnd,
                                  nullptr));
         HIP_TRY(h, launch_init_particles(h->stream, ab, h->bnd, h->V, h->N, h->gs, h->P[0], h->p.init_max_velocity,
                                          h->p.particle_min_vel, h->fc, k0, k1));
-        mark(h, ST_MOMENTS);
-        mark(h, ST_RESAMPLE);
     }
+    mark(h, ST_MOMENTS);
+    return K3DOM_OK;
+}
+
+k3dom_status tail_resample(k3dom_handle* h) {
+    const uint32_t k0 = (uint32_t)h->p.seed, k1 = (uint32_t)(h->p.seed >> 32);
+    mark(h, ST_RESAMPLE);
+    HIP_TRY(h, launch_fixed_scan(h->stream, h->P[1].w, h->N, h->NB.w, h->B, joint_parts(h),
+                                 voxel_particles_parts(h->N) + newborn_parts(h->B), &h->sc->E_joint, h->cdf_j,
+                                 h->blocksums, &h->sc->total_joint));
+    HIP_TRY(h, launch_resample(h->stream, &h->sc->overflow, h->cdf_j, h->N + h->B, &h->sc->total_joint,
+                               &h->sc->E_joint, h->P[1], h->NB, h->P[0], h->N, h->fc, k0, k1, h->ridx));
+    return K3DOM_OK;
+}
+
+// Everything after the gather, then the frame's scalars to the host mirror.
+k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal) {
+    k3dom_status s = tail_mass(h, normal);
+    if (s == K3DOM_OK && normal) s = tail_persistent(h);
+    if (s == K3DOM_OK) s = tail_newborn(h, normal);
+    if (s == K3DOM_OK && normal) s = tail_resample(h);
+    if (s != K3DOM_OK) return s;
+    if (!normal) mark(h, ST_RESAMPLE);
     mark(h, ST_FRAME);
     HIP_TRY(h, hipMemcpyAsync(h->h_sc, h->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->stream));
     return K3DOM_OK;
@@ -446,6 +480,30 @@ k3dom_status capture_graph(k3dom_handle* h) {
     return K3DOM_OK;
 }
 
+// Grows what the last run of the kernel-inference stage overflowed (its need
+// is known up to that stage, from the scalars read back) and clears the flag
+// for a re-run.
+k3dom_status grow_after_overflow(k3dom_handle* h) {
+    long long want[3], want_heavy = h->ibuf.heavy_cap;
+    for (int l = 0; l < 3; ++l) {
+        want[l] = h->lbuf[l].list_cap;
+        if (h->h_sc->overflow_mask & (1u << l))
+            want[l] = std::max(want[l] * 2, (long long)h->h_sc->level_total[l] * 3 / 2 + 1024);
+    }
+    if (h->h_sc->overflow_mask & (1u << 4))
+        want_heavy = std::max(want_heavy * 2, (long long)h->h_sc->n_heavy * 3 / 2 + 256);
+    if (h->h_sc->overflow_mask & (1u << 5)) {
+        const long long need = (long long)h->h_sc->ds_total * 5 / 4 + 1024;
+        k3dom_status s = ensure_ds_list(h, std::max(h->ds.cap * 2, need));
+        if (s != K3DOM_OK) return s;
+    }
+    k3dom_status s = ensure_caps(h, want, want_heavy);
+    if (s != K3DOM_OK) return s;
+    HIP_TRY(h, launch_clear_overflow(h->stream, h->sc));
+    h->overflow_retries++;
+    return K3DOM_OK;
+}
+
 // Runs kernel inference and the frame tail until no capacity overflows.
 // checked: the frame already ran once and `done` is its outcome (graph path);
 // otherwise the first iteration enqueues the kernel-inference stage itself.
@@ -454,24 +512,8 @@ k3dom_status resolve_overflow(k3dom_handle* h, bool normal, const float* dx, con
     for (int attempt = 0; !done; ++attempt) {
         if (checked || attempt > 0) {
             if (attempt >= 8) return fail(h, K3DOM_ERR_OUT_OF_MEMORY, "kernel-inference capacities keep overflowing");
-            // grow what overflowed (the frame's need is known up to that stage) and re-run
-            long long want[3], want_heavy = h->ibuf.heavy_cap;
-            for (int l = 0; l < 3; ++l) {
-                want[l] = h->lbuf[l].list_cap;
-                if (h->h_sc->overflow_mask & (1u << l))
-                    want[l] = std::max(want[l] * 2, (long long)h->h_sc->level_total[l] * 3 / 2 + 1024);
-            }
-            if (h->h_sc->overflow_mask & (1u << 4))
-                want_heavy = std::max(want_heavy * 2, (long long)h->h_sc->n_heavy * 3 / 2 + 256);
-            if (h->h_sc->overflow_mask & (1u << 5)) {
-                const long long need = (long long)h->h_sc->ds_total * 5 / 4 + 1024;
-                k3dom_status s = ensure_ds_list(h, std::max(h->ds.cap * 2, need));
-                if (s != K3DOM_OK) return s;
-            }
-            k3dom_status s = ensure_caps(h, want, want_heavy);
+            k3dom_status s = grow_after_overflow(h);
             if (s != K3DOM_OK) return s;
-            HIP_TRY(h, launch_clear_overflow(h->stream, h->sc));
-            h->overflow_retries++;
         }
         k3dom_status s = enqueue_rest(h, normal, dx, dy, dz);
         if (s != K3DOM_OK) return s;
@@ -482,6 +524,11 @@ k3dom_status resolve_overflow(k3dom_handle* h, bool normal, const float* dx, con
 }
 
 void finish_frame(k3dom_handle* h, double t, int m) {
+    count_frame(h, h->updated_time > 0, m);
+    h->updated_time = t;
+}
+
+void count_frame(k3dom_handle* h, bool normal, int m) {
     h->last_geom = h->h_fc->g;
     h->stats.points = (uint64_t)m;
     for (int l = 0; l < 3; ++l) h->stats.level_entries[l] = h->h_sc->level_total[l];
@@ -489,11 +536,10 @@ void finish_frame(k3dom_handle* h, double t, int m) {
     h->stats.gather_items = (uint64_t)h->h_sc->n_light + h->h_sc->n_heavy;
     h->stats.gather_split_items = h->h_sc->n_heavy;
     h->stats.capacity_regrowths = (uint64_t)h->overflow_retries;
-    if (h->updated_time > 0) h->normal_here++;
+    if (normal) h->normal_here++;
     h->frames_here++;
     h->last_grouped = h->ngroups > 0;
     h->last_frame_ok = true;
-    h->updated_time = t;
     h->frame++;
 }
 
@@ -510,6 +556,8 @@ k3dom_status frame_body(k3dom_handle* h, double t, const float* dx, const float*
 // voxel fields stays due.  (The ranges, keys and evidence of the failed
 // frame's first stages may be left in the per-frame arrays until the next frame.)
 k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
+    if (h->stage != 0)
+        return fail(h, K3DOM_ERR_STATE, "a frame driven by the stage methods is unfinished (resampling ends it)");
     const bool pshift_pending = h->pshift_pending, shift_pending = h->shift_pending;
     const int pshift[3] = {h->pshift[0], h->pshift[1], h->pshift[2]};
     const k3dom_status s = frame_body(h, t, dx, dy, dz, m);
@@ -874,6 +922,8 @@ k3dom_status k3dom_update_pose(k3dom_handle* h, float x, float y, float z) {
 k3dom_status k3dom_update_grid(k3dom_handle* h, double t, const float* xs, const float* ys, const float* zs,
                                int32_t n) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
+    if (h->stage != 0)
+        return fail(h, K3DOM_ERR_STATE, "a frame driven by the stage methods is unfinished (resampling ends it)");
     if (n < 0 || (n > 0 && (!xs || !ys || !zs))) return fail(h, K3DOM_ERR_INVALID_ARG, "bad point arrays");
     k3dom_status s = ensure_points(h, std::max(n, 1));
     if (s != K3DOM_OK) return s;
@@ -904,6 +954,8 @@ k3dom_status k3dom_update_grid(k3dom_handle* h, double t, const float* xs, const
 k3dom_status k3dom_update_grid_device(k3dom_handle* h, double t, const float* d_xs, const float* d_ys,
                                       const float* d_zs, int32_t n) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
+    if (h->stage != 0)
+        return fail(h, K3DOM_ERR_STATE, "a frame driven by the stage methods is unfinished (resampling ends it)");
     if (n < 0 || (n > 0 && (!d_xs || !d_ys || !d_zs))) return fail(h, K3DOM_ERR_INVALID_ARG, "bad point arrays");
     k3dom_status s = ensure_points(h, std::max(n, 1));
     if (s != K3DOM_OK) return s;
@@ -915,6 +967,8 @@ k3dom_status k3dom_update_grid_device(k3dom_handle* h, double t, const float* d_
 k3dom_status k3dom_update_grid_lidars(k3dom_handle* h, double t, const k3dom_lidar* lidars, int32_t n_lidars,
                                       int32_t points_on_device) {
     if (!h) return K3DOM_ERR_INVALID_ARG;
+    if (h->stage != 0)
+        return fail(h, K3DOM_ERR_STATE, "a frame driven by the stage methods is unfinished (resampling ends it)");
     if (!lidars || n_lidars < 1 || n_lidars > K3DOM_MAX_LIDARS)
         return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: 1 to K3DOM_MAX_LIDARS LiDARs");
     if (h->method == K3DOM_METHOD_DSPHDMIB && n_lidars > 1)
@@ -970,6 +1024,175 @@ k3dom_status k3dom_update_grid_lidars(k3dom_handle* h, double t, const k3dom_lid
     h->ngroups = n_lidars;
     s = frame(h, t, dx, dy, dz, m);
     h->ngroups = 0;
+    return s;
+}
+
+// ---------------------------------------------------------------- stage methods
+// The reference's public stage methods (include/common/dom.h:89-94,
+// include/k3dom/dom.h:24-28), in the order DOM::updateGrid calls them
+// (src/k3dom/dom.cu:33-44).  Each runs its part of the frame's kernels and
+// returns when it is done; a normal frame driven through them equals the
+// same frame of k3dom_update_grid bit for bit.  Like the reference's stages
+// they leave the stored time of the last frame to updateGrid.
+namespace {
+
+enum { STG_IDLE = 0, STG_PREDICTED, STG_ASSIGNED, STG_OCCUPANCY, STG_PERSISTENT, STG_NEWBORN };
+
+k3dom_status stage_entry(k3dom_handle* h, int need, const char* what) {
+    if (!h) return K3DOM_ERR_INVALID_ARG;
+    if (h->method != K3DOM_METHOD_K3DOM)
+        return fail(h, K3DOM_ERR_STATE, std::string(what) + ": the stage methods drive K3DOM maps");
+    if (h->stage != need)
+        return fail(h, K3DOM_ERR_STATE, std::string(what) + ": called out of the order of DOM::updateGrid "
+                                        "(particlePrediction, particleAssignment, gridCellOccupancyUpdate, "
+                                        "updatePersistentParticles, initializeNewParticles, statisticalMoments, "
+                                        "resampling)");
+    return K3DOM_OK;
+}
+
+// A failed stage abandons the frame: the next call starts a new one.
+k3dom_status stage_done(k3dom_handle* h, k3dom_status s, int next) {
+    if (s == K3DOM_OK) s = hipStreamSynchronize(h->stream) == hipSuccess
+                               ? K3DOM_OK : fail(h, K3DOM_ERR_HIP, "stage: stream synchronisation failed");
+    if (s != K3DOM_OK) {
+        h->stage = STG_IDLE;
+        h->last_frame_ok = false;
+        return s;
+    }
+    h->stage = next;
+    return K3DOM_OK;
+}
+
+// Points of a stage (host arrays) into the handle's point buffer; their count
+// into the frame constants.
+k3dom_status stage_points(k3dom_handle* h, const float* xs, const float* ys, const float* zs, int n) {
+    if (n < 0 || (n > 0 && (!xs || !ys || !zs))) return fail(h, K3DOM_ERR_INVALID_ARG, "bad point arrays");
+    k3dom_status s = ensure_points(h, std::max(n, 1));
+    if (s != K3DOM_OK) return s;
+    const size_t cap = (size_t)h->m_cap;
+    for (int k = 0; k < 3 && n > 0; ++k)
+        HIP_TRY(h, hipMemcpyAsync(h->d_pts + k * cap, k == 0 ? xs : (k == 1 ? ys : zs), sizeof(float) * n,
+                                  hipMemcpyHostToDevice, h->stream));
+    h->h_fc->m = n;
+    return K3DOM_OK;
+}
+
+// Kernel inference over the handle's points and the mass update (normal) or
+// the first frame's born masses and particle placement, re-run after growing
+// what overflowed.
+k3dom_status stage_evidence(k3dom_handle* h, bool normal) {
+    const size_t cap = (size_t)h->m_cap;
+    HIP_TRY(h, hipMemcpyAsync(h->fc, h->h_fc, sizeof(FrameConsts), hipMemcpyHostToDevice, h->stream));
+    for (int attempt = 0;; ++attempt) {
+        if (attempt > 0) {
+            if (attempt >= 8) return fail(h, K3DOM_ERR_OUT_OF_MEMORY, "kernel-inference capacities keep overflowing");
+            k3dom_status s = grow_after_overflow(h);
+            if (s != K3DOM_OK) return s;
+        }
+        k3dom_status s = build_lists(h, h->fc, h->d_pts, h->d_pts + cap, h->d_pts + 2 * cap);
+        if (s != K3DOM_OK) return s;
+        HIP_TRY(h, launch_bgki(h->stream, h->gather_blocks, h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
+                               h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc));
+        s = tail_mass(h, normal);
+        if (s == K3DOM_OK && !normal) s = tail_newborn(h, false);
+        if (s != K3DOM_OK) return s;
+        HIP_TRY(h, hipMemcpyAsync(h->h_sc, h->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
+        if (!h->h_sc->overflow) return K3DOM_OK;
+    }
+}
+
+}  // namespace
+
+k3dom_status k3dom_particle_prediction(k3dom_handle* h, float dt) {
+    k3dom_status s = stage_entry(h, STG_IDLE, "particlePrediction");
+    if (s != K3DOM_OK) return s;
+    const bool pshift_pending = h->pshift_pending;
+    if ((s = begin_frame_voxels(h, true)) != K3DOM_OK) return s;
+    fill_frame_consts_dt(h, h->h_fc, true, dt, 0);
+    if (hipMemcpyAsync(h->fc, h->h_fc, sizeof(FrameConsts), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        launch_reset_scalars(h->stream, h->sc) != hipSuccess ||
+        launch_predict(h->stream, h->fc, h->P[0], h->prec, h->N) != hipSuccess) {
+        h->pshift_pending = pshift_pending;
+        return stage_done(h, fail(h, K3DOM_ERR_HIP, "particlePrediction: launch failed"), STG_IDLE);
+    }
+    return stage_done(h, K3DOM_OK, STG_PREDICTED);
+}
+
+k3dom_status k3dom_particle_assignment(k3dom_handle* h) {
+    k3dom_status s = stage_entry(h, STG_PREDICTED, "particleAssignment");
+    if (s != K3DOM_OK) return s;
+    VoxelArrays& v = h->vox[h->vcur];
+    if (launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys, &h->last_perm) !=
+            hipSuccess ||
+        launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N) != hipSuccess ||
+        launch_ranges(h->stream, h->P[1].key, h->N, v.start, v.end, h->V) != hipSuccess)
+        s = fail(h, K3DOM_ERR_HIP, "particleAssignment: launch failed");
+    return stage_done(h, s, STG_ASSIGNED);
+}
+
+k3dom_status k3dom_grid_cell_occupancy_update(k3dom_handle* h, float dt, const float* xs, const float* ys,
+                                              const float* zs, int32_t n) {
+    k3dom_status s = stage_entry(h, STG_ASSIGNED, "gridCellOccupancyUpdate");
+    if (s != K3DOM_OK) return s;
+    s = stage_points(h, xs, ys, zs, n);
+    if (s == K3DOM_OK) {
+        h->h_fc->decay = (float)std::pow((double)h->p.gamma, (double)dt);
+        s = stage_evidence(h, true);
+    }
+    return stage_done(h, s, STG_OCCUPANCY);
+}
+
+k3dom_status k3dom_update_persistent_particles(k3dom_handle* h) {
+    k3dom_status s = stage_entry(h, STG_OCCUPANCY, "updatePersistentParticles");
+    if (s != K3DOM_OK) return s;
+    return stage_done(h, tail_persistent(h), STG_PERSISTENT);
+}
+
+k3dom_status k3dom_initialize_new_particles(k3dom_handle* h) {
+    k3dom_status s = stage_entry(h, STG_PERSISTENT, "initializeNewParticles");
+    if (s != K3DOM_OK) return s;
+    return stage_done(h, tail_newborn(h, true), STG_NEWBORN);
+}
+
+k3dom_status k3dom_statistical_moments(k3dom_handle* h) {
+    // computed with the persistent update (same weights, same sums)
+    k3dom_status s = stage_entry(h, h && h->stage == STG_PERSISTENT ? STG_PERSISTENT : STG_NEWBORN,
+                                 "statisticalMoments");
+    return s;
+}
+
+k3dom_status k3dom_resampling(k3dom_handle* h) {
+    k3dom_status s = stage_entry(h, STG_NEWBORN, "resampling");
+    if (s != K3DOM_OK) return s;
+    s = tail_resample(h);
+    if (s == K3DOM_OK && hipMemcpyAsync(h->h_sc, h->sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->stream) !=
+                             hipSuccess)
+        s = fail(h, K3DOM_ERR_HIP, "resampling: scalar readback failed");
+    s = stage_done(h, s, STG_IDLE);
+    if (s == K3DOM_OK) count_frame(h, true, h->h_fc->m);
+    return s;
+}
+
+k3dom_status k3dom_initialize_particles(k3dom_handle* h, const float* xs, const float* ys, const float* zs,
+                                        int32_t n) {
+    k3dom_status s = stage_entry(h, STG_IDLE, "initializeParticles");
+    if (s != K3DOM_OK) return s;
+    if ((s = begin_frame_voxels(h, false)) != K3DOM_OK) return s;
+    const bool pshift_pending = h->pshift_pending;
+    const int pshift[3] = {h->pshift[0], h->pshift[1], h->pshift[2]};
+    fill_frame_consts_dt(h, h->h_fc, false, 0.0f, 0);
+    s = stage_points(h, xs, ys, zs, n);
+    if (s == K3DOM_OK && launch_reset_scalars(h->stream, h->sc) != hipSuccess)
+        s = fail(h, K3DOM_ERR_HIP, "initializeParticles: launch failed");
+    if (s == K3DOM_OK) s = stage_evidence(h, false);
+    s = stage_done(h, s, STG_IDLE);
+    if (s == K3DOM_OK) {
+        count_frame(h, false, n);
+    } else {
+        h->pshift_pending = pshift_pending;
+        for (int k = 0; k < 3; ++k) h->pshift[k] = pshift[k];
+    }
     return s;
 }
 

@@ -100,6 +100,7 @@ struct k3dom_handle {
     // the last frame completed (its lists and work items are whole); false
     // after a frame that returned an error
     bool last_frame_ok = false;
+    int stage = 0;  // the stage methods' position in the frame (0: between frames)
     bool first_pose = false;
     float sensor[3] = {0, 0, 0}, center[3] = {0, 0, 0};
     double updated_time = -1.0;
@@ -187,8 +188,19 @@ k3dom_status enqueue_after_gather(k3dom_handle* h, bool normal);
 k3dom_status resolve_overflow(k3dom_handle* h, bool normal, const float* dx, const float* dy, const float* dz,
                               bool done, bool checked);
 void finish_frame(k3dom_handle* h, double t, int m);
-// Host copy of the frame constants of a frame at time t with m points.
+// The counters and statistics of a completed frame (finish_frame without the time).
+void count_frame(k3dom_handle* h, bool normal, int m);
+// Host copy of the frame constants of a frame at time t with m points, or of
+// a frame of time step dt (normal) or an initialisation frame (dt 0).
 void fill_frame_consts(k3dom_handle* h, FrameConsts* c, double t, int m);
+void fill_frame_consts_dt(k3dom_handle* h, FrameConsts* c, bool normal, float dt, int m);
+// Stages of the frame tail (stage order of DOM::updateGrid) and the growth
+// of overflowed kernel-inference capacities before a re-run.
+k3dom_status tail_mass(k3dom_handle* h, bool normal);
+k3dom_status tail_persistent(k3dom_handle* h);
+k3dom_status tail_newborn(k3dom_handle* h, bool normal);
+k3dom_status tail_resample(k3dom_handle* h);
+k3dom_status grow_after_overflow(k3dom_handle* h);
 void drop_graph(k3dom_handle* h);
 // State after construction (priors, no pose, frame count 0).
 k3dom_status reset_map(k3dom_handle* h);

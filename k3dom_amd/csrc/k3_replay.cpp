@@ -5,7 +5,13 @@
 // Built twice: k3dom_replay over include/k3dom/dom.h and, with
 // K3_REPLAY_DSPHDMIB defined, k3dom_replay_dsphdmib over include/dsphdmib/dom.h.
 //
-//   k3dom_replay PARAMS FRAMES OUT
+//   k3dom_replay PARAMS FRAMES OUT [stages]
+//     stages  (K3DOM only) drive each frame through the stage methods in the
+//             order DOM::updateGrid calls them (src/k3dom/dom.cu:29-50):
+//             initializeParticles for a first frame, else particlePrediction,
+//             particleAssignment, gridCellOccupancyUpdate,
+//             updatePersistentParticles, initializeNewParticles,
+//             statisticalMoments, resampling
 //     PARAMS  raw k3dom_params record (sizeof(k3dom_params) bytes)
 //     FRAMES  "K3FR", int32 frame count, then per frame: double t,
 //             float pose[3], int32 n, float x[n], float y[n], float z[n]
@@ -29,10 +35,16 @@ bool read_exact(FILE* f, void* dst, size_t n) { return std::fread(dst, 1, n, f) 
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: %s PARAMS FRAMES OUT\n", argv[0]);
+    if (argc != 4 && argc != 5) {
+        std::fprintf(stderr, "usage: %s PARAMS FRAMES OUT [stages]\n", argv[0]);
         return 2;
     }
+#ifdef K3_REPLAY_DSPHDMIB
+    const bool stages = false;
+#else
+    const bool stages = argc == 5 && std::strcmp(argv[4], "stages") == 0;
+#endif
+    double prev_t = -1.0;  // the updateGrid time bookkeeping of src/k3dom/dom.cu:31,46
     k3dom_params cp;
     FILE* pf = std::fopen(argv[1], "rb");
     if (!pf || !read_exact(pf, &cp, sizeof(cp))) {
@@ -97,8 +109,27 @@ int main(int argc, char** argv) {
         }
         grid_map.updatePose(pose[0], pose[1], pose[2]);
         if (grid_map.lastStatus() != K3DOM_OK) return 1;
-        grid_map.updateGrid(t, xs.data(), ys.data(), zs.data(), n);
+        if (!stages) {
+            grid_map.updateGrid(t, xs.data(), ys.data(), zs.data(), n);
+            if (grid_map.lastStatus() != K3DOM_OK) return 1;
+            continue;
+        }
+#ifndef K3_REPLAY_DSPHDMIB
+        if (prev_t > 0) {
+            const float dt = (float)(t - prev_t);
+            grid_map.particlePrediction(dt);
+            grid_map.particleAssignment();
+            grid_map.gridCellOccupancyUpdate(dt, xs, ys, zs);
+            grid_map.updatePersistentParticles();
+            grid_map.initializeNewParticles();
+            grid_map.statisticalMoments();
+            grid_map.resampling();
+        } else {
+            grid_map.initializeParticles(xs, ys, zs);
+        }
         if (grid_map.lastStatus() != K3DOM_OK) return 1;
+        prev_t = t;
+#endif
     }
     std::fclose(ff);
     grid_map.copyCellsToHost();

@@ -142,6 +142,41 @@ class DOM:
         check(self._lib.k3dom_update_grid_device(self._h, float(t), C.c_void_p(d_xs), C.c_void_p(d_ys),
                                                  C.c_void_p(d_zs), int(n)), self._h, "updateGridDevice")
 
+    # ---------------------------------------------------------------- stage methods
+    # include/common/dom.h:89-94, include/k3dom/dom.h:24-28, in the order of
+    # DOM::updateGrid (src/k3dom/dom.cu:33-44); see k3dom.h.
+    def particlePrediction(self, dt):
+        check(self._lib.k3dom_particle_prediction(self._h, float(dt)), self._h, "particlePrediction")
+
+    def particleAssignment(self):
+        check(self._lib.k3dom_particle_assignment(self._h), self._h, "particleAssignment")
+
+    def gridCellOccupancyUpdate(self, dt, xs, ys, zs):
+        xs, ys, zs = (np.ascontiguousarray(a, dtype=np.float32) for a in (xs, ys, zs))
+        if not (len(xs) == len(ys) == len(zs)):
+            raise ValueError("point arrays differ in length")
+        check(self._lib.k3dom_grid_cell_occupancy_update(self._h, float(dt), _fptr(xs), _fptr(ys), _fptr(zs),
+                                                          len(xs)), self._h, "gridCellOccupancyUpdate")
+
+    def updatePersistentParticles(self):
+        check(self._lib.k3dom_update_persistent_particles(self._h), self._h, "updatePersistentParticles")
+
+    def initializeNewParticles(self):
+        check(self._lib.k3dom_initialize_new_particles(self._h), self._h, "initializeNewParticles")
+
+    def statisticalMoments(self):
+        check(self._lib.k3dom_statistical_moments(self._h), self._h, "statisticalMoments")
+
+    def resampling(self):
+        check(self._lib.k3dom_resampling(self._h), self._h, "resampling")
+
+    def initializeParticles(self, xs, ys, zs):
+        xs, ys, zs = (np.ascontiguousarray(a, dtype=np.float32) for a in (xs, ys, zs))
+        if not (len(xs) == len(ys) == len(zs)):
+            raise ValueError("point arrays differ in length")
+        check(self._lib.k3dom_initialize_particles(self._h, _fptr(xs), _fptr(ys), _fptr(zs), len(xs)), self._h,
+              "initializeParticles")
+
     def info(self):
         out = Info()
         check(self._lib.k3dom_get_info(self._h, C.byref(out)), self._h, "get_info")

@@ -91,6 +91,55 @@ def test_replay_through_cpp_header_matches_oracle(gpu, tmp_path):
     compare_cells(cells, od.copyCellsToHost(), "replay")
 
 
+@pytest.mark.gpu
+def test_replay_through_cpp_stage_methods_matches_oracle(gpu, tmp_path):
+    """The same replay driving every frame through the reference's stage
+    methods (initializeParticles, then particlePrediction ... resampling in
+    the order of src/k3dom/dom.cu:29-50) gives the oracle's cells bit for bit."""
+    from k3dom_amd.dom import CELL_DTYPE
+    from k3dom_amd.scene import small_workload
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_cells
+    w = small_workload(ego_vx=4.0, frames=6, seed=4)
+    recs = _write_inputs(tmp_path, w, w.frames)
+    out = tmp_path / "cells.bin"
+    r = subprocess.run([REPLAY, str(tmp_path / "params.bin"), str(tmp_path / "frames.bin"), str(out), "stages"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cells = np.frombuffer(out.read_bytes()[4:], CELL_DTYPE)
+    od = OracleDOM(w.params)
+    for t, pose, xs, ys, zs in recs:
+        od.updatePose(*pose)
+        od.updateGrid(t, xs, ys, zs)
+    compare_cells(cells, od.copyCellsToHost(), "stage replay")
+
+
+def test_k3dom_header_declares_the_reference_stage_methods(tmp_path):
+    """A caller of the reference's stage methods (include/common/dom.h:89-94,
+    include/k3dom/dom.h:24-28) compiles unchanged against the drop-in header."""
+    src = tmp_path / "caller_stages.cpp"
+    src.write_text(r'''
+#include "k3dom/dom.h"
+int main() {
+    void (dom::DOM_c::*pred)(float) = &dom::DOM_c::particlePrediction;
+    void (dom::DOM_c::*stages[])() = {&dom::DOM_c::particleAssignment, &dom::DOM_c::updatePersistentParticles,
+                                       &dom::DOM_c::initializeNewParticles, &dom::DOM_c::statisticalMoments,
+                                       &dom::DOM_c::resampling};
+    void (dom::DOM::*init)(std::vector<float>&, std::vector<float>&, std::vector<float>&) =
+        &dom::DOM::initializeParticles;
+    void (dom::DOM::*occ)(float, std::vector<float>&, std::vector<float>&, std::vector<float>&) =
+        &dom::DOM::gridCellOccupancyUpdate;
+    (void)pred; (void)stages; (void)init; (void)occ;
+    return 0;
+}
+''')
+    exe = tmp_path / "caller_stages"
+    lib = os.path.join(ROOT, "k3dom_amd")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-L", lib, "-lk3dom",
+                    "-Wl,-rpath," + lib, "-o", str(exe)], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
+
+
 def test_dsphdmib_header_compiles_with_reference_names(tmp_path):
     """A caller of the reference's DS-PHD/MIB class (include/dsphdmib/dom.h:17-41)
     compiles unchanged: same class and member names, MeasurementCell records,
