@@ -12,7 +12,8 @@ kernel-inference gather, whose algorithmic work is its contributing
 (voxel, point) pairs; `frame_hbm` states the frame's compulsory HBM traffic
 against its time; `adjoint` times the differentiated window (forward +
 backward ms/frame against the plain forward, config-1 scene, ls 0.3, 8-frame
-windows); `batch` times 32 independent uneven map tiles advanced together;
+windows); `batch` (only with --batch) times 32 independent uneven map tiles advanced
+together;
 `dsphdmib` times the DS-PHD/MIB method on the same scene; `lidars` times the
 four-LiDAR stress configuration (512x512x64, 8M particles) and its
 differentiated window (fwd+bwd against plain forward); `cpu_baseline`
@@ -346,7 +347,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true", help="skip the pair-count pass")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiated-window leg")
-    ap.add_argument("--no-batch", action="store_true", help="skip the batched-tiles leg")
+    ap.add_argument("--batch", action="store_true",
+                    help="run the batched-tiles leg (off by default: a host fault in the 32-tile batch is unexplained)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-tiles leg (the default)")
     ap.add_argument("--no-dsphdmib", action="store_true", help="skip the DS-PHD/MIB method leg")
     ap.add_argument("--no-lidars", action="store_true", help="skip the four-LiDAR stress leg")
     ap.add_argument("--copy-probe", action="store_true",
@@ -468,8 +471,9 @@ def main():
         adj = adjoint_leg(torch)
         log(f"differentiated window: fwd+bwd / plain = {adj['fwd_bwd_vs_plain']}")
 
-    bat = None
-    if not args.no_batch:
+    bat = {"skipped": "off by default: a host segmentation fault inside k3dom_batch_update on the 32-tile "
+                      "workload is unexplained; run with --batch"}
+    if args.batch and not args.no_batch:
         bat = batch_leg(torch)
         log(f"batched tiles: {bat}")
 
