@@ -6,6 +6,10 @@ full-size plain configuration: 256x256x32 voxels at 0.1 m, 2,000,000
 particles + 200,000 newborn per frame, a 64x2048 ray-cast LiDAR on an ego
 moving at 10 m/s through 300 static + 50 moving boxes.  Point clouds of every
 frame are uploaded to device memory before timing (inputs resident in HBM).
+`host_array` times the same frames through the drop-in's host-array
+updateGrid (points copied from host memory every frame); `walls` times the
+same configuration with enclosing walls so that every ray returns (131,072
+points per frame).
 
 Prints one JSON line (rank 0).  `roofline` is for the dominant kernel, the
 kernel-inference gather, whose algorithmic work is its contributing
@@ -91,6 +95,76 @@ def cpu_baseline(workload, frames):
                                  f"(other stages serial); {dt_mt:.2f} s"}}
 
 
+WALLS_M = 27.0  # half-size of the enclosing square: every ray of the +-15 degree fan returns within 40 m
+
+
+def host_array_leg(torch, w, frames, warmup, steps):
+    """The drop-in's host-array updateGrid (k3dom_update_grid, the reference's
+    own call): the same frames with the point clouds in host memory; every
+    frame copies them into the handle's pinned buffer and uploads them beside
+    prediction and sort on a second stream."""
+    from k3dom_amd.dom import DOM
+    d = DOM(w.params)
+
+    def step(f):
+        t, pose, xs, ys, zs = frames[f]
+        d.updatePose(*pose)
+        d.updateGrid(t, xs, ys, zs)
+
+    for f in range(warmup):
+        step(f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, warmup + steps):
+        step(f)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    d.close()
+    return {"api": "updateGrid(t, xs, ys, zs), numpy float32 host arrays", "warmup": warmup, "steps": steps,
+            "frames_per_s": round(steps / el, 3), "ms_per_frame": round(el / steps * 1e3, 4)}
+
+
+def walls_leg(torch, w, warmup=5, steps=30, breakdown=5):
+    """The same configuration with enclosing walls (a square of half-size
+    WALLS_M m around the sensor, unbounded in z): every ray of the 64x2048 fan
+    returns, 131,072 points per frame.  Points resident in device memory."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, with_walls
+    ww = with_walls(w, WALLS_M)
+    sc = Scene(ww.scene)
+    F = warmup + steps
+    frames = [sc.frame(f) for f in range(F + breakdown)]
+    dev = [tuple(torch.from_numpy(a).cuda() for a in fr[2:5]) for fr in frames]
+    torch.cuda.synchronize()
+    d = DOM(w.params)
+
+    def step(f):
+        t, pose, xs, _, _ = frames[f]
+        dx, dy, dz = dev[f]
+        d.updatePose(*pose)
+        d.updateGridDevice(t, dx.data_ptr(), dy.data_ptr(), dz.data_ptr(), len(xs))
+
+    for f in range(warmup):
+        step(f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(warmup, F):
+        step(f)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    d.set_timing(True)
+    st = np.zeros(len(DOM.STAGES))
+    for f in range(F, F + breakdown):
+        step(f)
+        st += d.stage_times()
+    st /= breakdown
+    d.close()
+    return {"workload": ww.name + " (every ray returns)",
+            "points_per_frame": int(np.mean([len(fr[2]) for fr in frames[warmup:F]])), "warmup": warmup,
+            "steps": steps, "frames_per_s": round(steps / el, 3), "ms_per_frame": round(el / steps * 1e3, 4),
+            "stage_ms": {n: round(float(v), 4) for n, v in zip(DOM.STAGES, st)}}
+
+
 def adjoint_leg(torch, reps=5):
     """fwd+bwd ms/frame of the differentiated window against the plain forward
     ms/frame, on the differentiated configuration (config-1 scene, ls 0.3,
@@ -137,6 +211,7 @@ def adjoint_leg(torch, reps=5):
     return {"workload": w.name, "frames": T, "reps": reps, "plain_fwd_ms_per_frame": round(p_ms, 4),
             "fwd_ms_per_frame": round(f_ms, 4), "bwd_ms_per_frame": round(b_ms, 4),
             "fwd_bwd_vs_plain": round((f_ms + b_ms) / p_ms, 3),
+            "fwd_bwd_vs_faster_fwd": round((f_ms + b_ms) / min(p_ms, f_ms), 3),
             "grad": [float(x) for x in g], "theta": [float(x) for x in th]}
 
 
@@ -289,6 +364,7 @@ def adjoint_lidars_leg(torch, T=8, reps=2):
     return {"workload": w.name, "lidars": len(rig), "frames": T, "reps": reps,
             "plain_fwd_ms_per_frame": round(p_ms, 3), "fwd_ms_per_frame": round(f_ms, 3),
             "bwd_ms_per_frame": round(b_ms, 3), "fwd_bwd_vs_plain": round((f_ms + b_ms) / p_ms, 3),
+            "fwd_bwd_vs_faster_fwd": round((f_ms + b_ms) / min(p_ms, f_ms), 3),
             "grad": [float(x) for x in g]}
 
 
@@ -352,6 +428,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-tiles leg (the default)")
     ap.add_argument("--no-dsphdmib", action="store_true", help="skip the DS-PHD/MIB method leg")
     ap.add_argument("--no-lidars", action="store_true", help="skip the four-LiDAR stress leg")
+    ap.add_argument("--no-host-array", action="store_true", help="skip the host-array updateGrid leg")
+    ap.add_argument("--no-walls", action="store_true", help="skip the every-ray-returns (walls) leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -466,6 +544,16 @@ def main():
         log("copy probe: one neg kernel, 256 MiB read + 256 MiB written")
         del a, b
 
+    host = None
+    if not args.no_host_array:
+        host = host_array_leg(torch, w, frames, args.warmup, args.steps)
+        log(f"host-array updateGrid: {host}")
+
+    walls = None
+    if not args.no_walls and args.config == 2:
+        walls = walls_leg(torch, w)
+        log(f"walls: {walls}")
+
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg(torch)
@@ -515,6 +603,8 @@ def main():
         "stage_ms_frames": f"{BREAKDOWN} frames after the timed ones, every stage timed with HIP events",
         "roofline": roof,
         "frame_hbm": frame_hbm,
+        "host_array": host,
+        "walls": walls,
         "adjoint": adj,
         "batch": bat,
         "dsphdmib": ds,

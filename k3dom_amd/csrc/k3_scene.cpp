@@ -193,6 +193,13 @@ struct Scene {
                     double tg = ((double)d.ground_z - ez) / dz;
                     if (tg > 0.0 && tg <= R && (best < 0.0 || tg < best)) best = tg;
                 }
+                if (d.walls > 0.0f) {
+                    const double hx = std::max(std::fabs(dx), std::fabs(dy));
+                    if (hx > 1e-12) {
+                        const double tw = (double)d.walls / hx;
+                        if (tw <= R && (best < 0.0 || tw < best)) best = tw;
+                    }
+                }
                 if (best > 0.0) {
                     uint64_t h = splitmix(d.seed ^ splitmix(((uint64_t)f << 32) ^ (uint64_t)(a * NB + b)));
                     double u1 = (double)((h >> 40) + 1) * 0x1p-24, u2 = (double)((h >> 16) & 0xffffff) * 0x1p-24;

@@ -24,7 +24,8 @@ typedef struct k3scene_desc {
     float frame_dt;                // seconds between frames
     double t0;                     // time stamp of frame 0
     int32_t ground;                // 1: the plane z = ground_z also returns points
-    int32_t pad;
+    float walls;                   // > 0: every ray also hits the walls of a square of this
+                                   // half-size (metres) centred on the sensor, unbounded in z
 } k3scene_desc;
 #ifdef __cplusplus
 }

@@ -30,7 +30,7 @@ class SceneDesc(C.Structure):
         ("ego_vy", C.c_float), ("beams", C.c_int32), ("az_bins", C.c_int32),
         ("elev_min_deg", C.c_float), ("elev_max_deg", C.c_float), ("max_range", C.c_float),
         ("range_noise", C.c_float), ("frame_dt", C.c_float), ("t0", C.c_double),
-        ("ground", C.c_int32), ("pad", C.c_int32),
+        ("ground", C.c_int32), ("walls", C.c_float),
     ]
 
 
@@ -199,6 +199,16 @@ def workload(cfg):
                   ground_z=-3.2, ego_vx=15.0, beams=128, az_bins=2048, max_range=40.0, range_noise=0.03)
         return Workload("stress 512x512x64@0.1m N=8000000, 4 LiDARs", p, s, 50, extra={"rig": CFG5_RIG})
     raise ValueError(f"unknown configuration {cfg}")
+
+
+def with_walls(w, half=27.0):
+    """Workload w with enclosing walls: a square of half-size `half` metres
+    around the sensor, unbounded in z, that every ray hits if nothing nearer
+    does (at cfg2, 27 m keeps the farthest wall return of the +-15 degree fan
+    within the 40 m range, so all 64x2048 rays return)."""
+    d = SceneDesc.from_buffer_copy(bytes(w.scene))
+    d.walls = half
+    return Workload(f"{w.name}, enclosing walls at {half:g} m", w.params, d, w.frames, dict(w.extra))
 
 
 # four LiDARs at the vehicle's corners, 90 degrees apart in yaw (mount from the ego, yaw)

@@ -8,6 +8,8 @@ an ego at 10 m/s, so that the map shifts by 10 voxels every frame.
   permutation and keys, newborn set, born masses, evidence, resampled
   indices).  The restatement's kernel inference runs on host threads; it
   gives the same bits for any thread count.
+- The same with enclosing walls, so that every ray of the fan returns
+  (131,072 points per frame): the initialisation frame and two normal frames.
 - Twelve further frames are checked against the size-independent properties
   of tests/_invariants.py.
 
@@ -40,6 +42,21 @@ def test_config2_bit_exact_with_map_shifts(gpu):
     # ego at 10 m/s, 10 Hz, shift threshold 1 m: the map moved every frame
     assert gd.info().center_pos[0] == od.info().center_pos[0] == pytest.approx(3.0)
     assert gd.stats()["points"] > 60000
+
+
+@pytest.mark.timeout(900)
+def test_config2_every_ray_returns_bit_exact(gpu):
+    """The headline configuration with enclosing walls, so that all 131,072
+    rays of the fan return: the initialisation frame and two normal frames
+    (each after a map shift) equal the serial restatement bit for bit."""
+    from k3dom_amd.scene import with_walls, workload
+    from oracle import orc
+    orc.set_threads(_host_threads())
+    try:
+        gd, od = run_pair(with_walls(workload(2)), 3)
+    finally:
+        orc.set_threads(1)
+    assert gd.stats()["points"] == 131072
 
 
 @pytest.mark.timeout(600)
