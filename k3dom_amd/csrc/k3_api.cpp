@@ -101,12 +101,25 @@ k3dom_status ensure_ds_list(k3dom_handle* h, long long want) {
     return K3DOM_OK;
 }
 
+// Kernel-inference buffers and scalars of LiDAR group g (group 0: the handle's own).
+LevelBufs* group_lbuf(k3dom_handle* h, int g) { return g == 0 ? h->lbuf : h->gk[g].lbuf; }
+const LevelBufs* group_lbuf(const k3dom_handle* h, int g) { return g == 0 ? h->lbuf : h->gk[g].lbuf; }
+ItemBufs& group_ibuf(k3dom_handle* h, int g) { return g == 0 ? h->ibuf : h->gk[g].ibuf; }
+const ItemBufs& group_ibuf(const k3dom_handle* h, int g) { return g == 0 ? h->ibuf : h->gk[g].ibuf; }
+DevScalars* group_sc(k3dom_handle* h, int g) { return g == 0 ? h->sc : h->gsc + g; }
+
 // True when a buffer sized from the point capacity is missing: a previous
 // frame's allocation failed part way (that frame returned the error).
 bool point_buffers_missing(const k3dom_handle* h) {
-    if (!h->d_pts || !h->rec || !h->pxyz || !h->h_pts || !h->ibuf.items || !h->ibuf.partial) return true;
-    for (const LevelBufs& B : h->lbuf)
-        if (!B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask) return true;
+    if (!h->d_pts || !h->rec || !h->pxyz || !h->h_pts) return true;
+    for (int g = 0; g < h->ngk; ++g) {
+        const ItemBufs& I = group_ibuf(h, g);
+        if (!I.items || !I.partial) return true;
+        for (int l = 0; l < 3; ++l) {
+            const LevelBufs& B = group_lbuf(h, g)[l];
+            if (!B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask) return true;
+        }
+    }
     return h->method == K3DOM_METHOD_DSPHDMIB && (!h->ds.list || !h->ds.segcnt || !h->ds.part);
 }
 
@@ -140,37 +153,97 @@ k3dom_status ensure_points(k3dom_handle* h, int m) {
 
 // (Re)allocates the buffers of one candidate level whose sizes follow from
 // its list capacity and its parent's.
+// (every group with buffers gets the same capacities)
 k3dom_status alloc_level(k3dom_handle* h, int l, long long list_cap) {
-    LevelBufs& B = h->lbuf[l];
     const LevelDesc& L = h->ldesc[l];
-    for (void* p : {(void*)B.counts, (void*)B.slot_off, (void*)B.list, (void*)B.bsum, (void*)B.mask}) dfree(h, p);
-    B.counts = B.slot_off = B.list = B.bsum = B.mask = nullptr;
-    const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : h->lbuf[l - 1].list_cap;
-    B.items_cap = L.nparents + (parent_cap + L.seg - 1) / L.seg;
-    B.slot_cap = B.items_cap * L.nch;
-    B.list_cap = list_cap;
-    B.mask_stride = parent_cap;
     h->caps_version++;
-    HIP_TRY(h, dalloc(h, &B.mask, (size_t)parent_cap * (size_t)level_child_groups(L)));
-    HIP_TRY(h, dalloc(h, &B.counts, (size_t)B.slot_cap));
-    HIP_TRY(h, dalloc(h, &B.slot_off, (size_t)B.slot_cap + 1));
-    HIP_TRY(h, dalloc(h, &B.list, (size_t)std::max(1ll, list_cap)));
-    HIP_TRY(h, dalloc_zero(h, &B.bsum, (size_t)scan_scratch_words(std::max<long long>(B.slot_cap, L.nparents + 1))));
+    for (int g = 0; g < h->ngk; ++g) {
+        LevelBufs& B = group_lbuf(h, g)[l];
+        for (void* p : {(void*)B.counts, (void*)B.slot_off, (void*)B.list, (void*)B.bsum, (void*)B.mask}) dfree(h, p);
+        B.counts = B.slot_off = B.list = B.bsum = B.mask = nullptr;
+        const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : group_lbuf(h, g)[l - 1].list_cap;
+        B.items_cap = L.nparents + (parent_cap + L.seg - 1) / L.seg;
+        B.slot_cap = B.items_cap * L.nch;
+        B.list_cap = list_cap;
+        B.mask_stride = parent_cap;
+        HIP_TRY(h, dalloc(h, &B.mask, (size_t)parent_cap * (size_t)level_child_groups(L)));
+        HIP_TRY(h, dalloc(h, &B.counts, (size_t)B.slot_cap));
+        HIP_TRY(h, dalloc(h, &B.slot_off, (size_t)B.slot_cap + 1));
+        HIP_TRY(h, dalloc(h, &B.list, (size_t)std::max(1ll, list_cap)));
+        HIP_TRY(h, dalloc_zero(h, &B.bsum,
+                               (size_t)scan_scratch_words(std::max<long long>(B.slot_cap, L.nparents + 1))));
+    }
     return K3DOM_OK;
 }
 
 k3dom_status alloc_heavy(k3dom_handle* h, long long heavy_cap) {
-    ItemBufs& I = h->ibuf;
-    dfree(h, I.items);
-    dfree(h, I.partial);
-    I.items = nullptr;
-    I.partial = nullptr;
-    I.heavy_cap = heavy_cap;
     h->caps_version++;
-    HIP_TRY(h, dalloc(h, &I.items, (size_t)(h->ntiles + heavy_cap)));
-    HIP_TRY(h, dalloc(h, &I.partial, (size_t)std::max(1ll, heavy_cap) * I.partial_stride));
+    for (int g = 0; g < h->ngk; ++g) {
+        ItemBufs& I = group_ibuf(h, g);
+        dfree(h, I.items);
+        dfree(h, I.partial);
+        I.items = nullptr;
+        I.partial = nullptr;
+        I.heavy_cap = heavy_cap;
+        HIP_TRY(h, dalloc(h, &I.items, (size_t)(h->ntiles + heavy_cap)));
+        HIP_TRY(h, dalloc(h, &I.partial, (size_t)std::max(1ll, heavy_cap) * I.partial_stride));
+    }
     return K3DOM_OK;
 }
+
+// The capacity-independent buffers of a group's lists and work items (as the
+// handle's own, allocated at creation).
+k3dom_status alloc_group_fixed(k3dom_handle* h, k3dom_handle::GroupKi& G) {
+    for (int l = 0; l < 3; ++l) {
+        LevelBufs& LB = G.lbuf[l];
+        const LevelDesc& L = h->ldesc[l];
+        const size_t nboxes = (size_t)L.nbx * L.nby * L.nbz;
+        HIP_TRY(h, dalloc(h, &LB.nseg, (size_t)L.nparents));
+        HIP_TRY(h, dalloc(h, &LB.item_base, (size_t)L.nparents + 1));
+        HIP_TRY(h, dalloc(h, &LB.box_off, nboxes));
+        HIP_TRY(h, dalloc(h, &LB.box_len, nboxes));
+    }
+    const size_t T = (size_t)h->ntiles;
+    ItemBufs& I = G.ibuf;
+    HIP_TRY(h, dalloc(h, &I.heavy_cnt, T));
+    HIP_TRY(h, dalloc(h, &I.light_cnt, T));
+    HIP_TRY(h, dalloc(h, &I.heavy_off, T + 1));
+    HIP_TRY(h, dalloc(h, &I.light_off, T + 1));
+    HIP_TRY(h, dalloc_zero(h, &I.bsum, (size_t)scan_scratch_words((long long)T)));
+    HIP_TRY(h, dalloc(h, &I.tile_partial, 2 * T));
+    HIP_TRY(h, dalloc(h, &G.del_occ, (size_t)h->V));
+    HIP_TRY(h, dalloc(h, &G.del_free, (size_t)h->V));
+    return K3DOM_OK;
+}
+
+// Buffers for n LiDAR groups: groups 1.. get their own lists, work items and
+// evidence with the handle's current capacities, and the joint gather its
+// argument sets, once.
+k3dom_status ensure_groups(k3dom_handle* h, int n) {
+    // (each allocated while missing: a failed allocation leaves the others in place)
+    if (!h->gsc) HIP_TRY(h, dalloc_zero(h, &h->gsc, (size_t)K3DOM_MAX_LIDARS));
+    if (!h->d_gargs) HIP_TRY(h, dalloc(h, &h->d_gargs, (size_t)K3DOM_MAX_LIDARS));
+    if (!h->d_gq) HIP_TRY(h, dalloc(h, &h->d_gq, (size_t)K3DOM_MAX_LIDARS + 2));
+    if (!h->h_gargs)
+        HIP_TRY(h, hipHostMalloc((void**)&h->h_gargs, sizeof(BgArgs) * K3DOM_MAX_LIDARS, hipHostMallocDefault));
+    if (n <= h->ngk) return K3DOM_OK;
+    for (int g = h->ngk; g < n; ++g) {
+        k3dom_handle::GroupKi& G = h->gk[g];
+        G = k3dom_handle::GroupKi{};
+        G.ibuf.partial_stride = 128;
+        k3dom_status s = alloc_group_fixed(h, G);
+        if (s != K3DOM_OK) return s;
+    }
+    // capacities as group 0's: the new groups' missing buffers make
+    // ensure_caps allocate every level (alloc_level / alloc_heavy cover
+    // every group)
+    h->ngk = n;
+    h->caps_version++;
+    long long want[3];
+    for (int l = 0; l < 3; ++l) want[l] = h->lbuf[l].list_cap;
+    return ensure_caps(h, want, h->ibuf.heavy_cap);
+}
+
 
 // Capacities for a frame of m points: grown for larger point sets, and by the
 // overflow path when a frame needs more than estimated.
@@ -178,16 +251,21 @@ k3dom_status ensure_caps(k3dom_handle* h, const long long want[3], long long wan
     for (int l = 0; l < 3; ++l) {
         const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : h->lbuf[l - 1].list_cap;
         const long long items_need = h->ldesc[l].nparents + (parent_cap + h->ldesc[l].seg - 1) / h->ldesc[l].seg;
-        const LevelBufs& B = h->lbuf[l];
-        // (a buffer still missing after a failed allocation is allocated again)
-        const bool missing = !B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask;
-        if (want[l] > B.list_cap || items_need > B.items_cap || parent_cap > B.mask_stride || missing) {
+        bool grow = false;
+        for (int g = 0; g < h->ngk; ++g) {
+            const LevelBufs& B = group_lbuf(h, g)[l];
+            // (a buffer still missing after a failed allocation is allocated again)
+            const bool missing = !B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask;
+            grow = grow || want[l] > B.list_cap || items_need > B.items_cap || parent_cap > B.mask_stride || missing;
+        }
+        if (grow) {
             k3dom_status s = alloc_level(h, l, std::max(want[l], h->lbuf[l].list_cap));
             if (s != K3DOM_OK) return s;
         }
     }
-    if (want_heavy > h->ibuf.heavy_cap || !h->ibuf.items || !h->ibuf.partial)
-        return alloc_heavy(h, std::max(want_heavy, h->ibuf.heavy_cap));
+    bool grow = want_heavy > h->ibuf.heavy_cap;
+    for (int g = 0; g < h->ngk; ++g) grow = grow || !group_ibuf(h, g).items || !group_ibuf(h, g).partial;
+    if (grow) return alloc_heavy(h, std::max(want_heavy, h->ibuf.heavy_cap));
     return K3DOM_OK;
 }
 
@@ -215,17 +293,27 @@ GridGeom grid_geom(const k3dom_handle* h) {
     return g;
 }
 
-k3dom_status build_lists(k3dom_handle* h, const FrameConsts* fc, const float* xs, const float* ys, const float* zs) {
+// Point records and candidate lists of group g's points (fc->m of them at
+// xs, ys, zs), into the group's buffers; the records go to rec + off.
+k3dom_status build_lists_group(k3dom_handle* h, int g, size_t off, const FrameConsts* fc, const float* xs,
+                               const float* ys, const float* zs) {
     if (h->pts_pending) {
         HIP_TRY(h, hipStreamWaitEvent(h->stream, h->pts_ready, 0));
         h->pts_pending = false;
     }
-    HIP_TRY(h, launch_point_prep(h->stream, fc, xs, ys, zs, h->m_cap, h->p.resolution, h->rec, h->pxyz));
+    LevelBufs* LB = group_lbuf(h, g);
+    DevScalars* sc = group_sc(h, g);
+    float4* pxyz = h->pxyz + off;
+    HIP_TRY(h, launch_point_prep(h->stream, fc, xs, ys, zs, h->m_cap, h->p.resolution, h->rec + off, pxyz));
     for (int l = 0; l < 3; ++l)
-        HIP_TRY(h, launch_build_level(h->stream, l, fc, h->tg, h->ldesc[l], h->lbuf[l], l ? &h->lbuf[l - 1] : nullptr,
-                                      h->p.ls, h->pxyz, h->sc));
-    HIP_TRY(h, launch_build_items(h->stream, h->ntiles, h->lbuf[2], h->ibuf, h->sc));
+        HIP_TRY(h, launch_build_level(h->stream, l, fc, h->tg, h->ldesc[l], LB[l], l ? &LB[l - 1] : nullptr, h->p.ls,
+                                      pxyz, sc));
+    HIP_TRY(h, launch_build_items(h->stream, h->ntiles, LB[2], group_ibuf(h, g), sc));
     return K3DOM_OK;
+}
+
+k3dom_status build_lists(k3dom_handle* h, const FrameConsts* fc, const float* xs, const float* ys, const float* zs) {
+    return build_lists_group(h, 0, 0, fc, xs, ys, zs);
 }
 
 void fill_frame_consts(k3dom_handle* h, FrameConsts* c, double t, int m) {
@@ -373,35 +461,41 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const f
         return enqueue_after_gather(h, normal);
     }
     if (h->ngroups > 0) {
-        // one kernel-inference pass per LiDAR (sensor at its origin, its own
-        // points), each combined into its evidence; later LiDARs' evidence is
-        // added to the first's in LiDAR order
+        // every LiDAR's lists and work items (sensor at its origin, its own
+        // points, its own buffers), then one persistent gather over all
+        // LiDARs' items, each into its own evidence; later LiDARs' evidence
+        // is added to the first's in LiDAR order
         mark(h, ST_LISTS);
         for (int g = 0; g < h->ngroups; ++g) {
-            const FrameConsts* gfc = h->gfc + g;
+            if (g > 0) HIP_TRY(h, launch_reset_scalars(h->stream, group_sc(h, g)));
             const size_t o = (size_t)h->gstart[g];
-            k3dom_status s = build_lists(h, gfc, dx + o, dy + o, dz + o);
+            k3dom_status s = build_lists_group(h, g, o, h->gfc + g, dx + o, dy + o, dz + o);
             if (s != K3DOM_OK) return s;
-            if (g == 0) {
-                mark(h, ST_BGKI);
-                mark(h, ST_BGKI_KERNEL);
-            }
-            float* to = g == 0 ? h->del_occ : h->gdel_occ;
-            float* tf = g == 0 ? h->del_free : h->gdel_free;
-            // the gather's work queue restarts for every pass (the frame start
-            // and the overflow re-run clear it before the first)
-            if (g > 0) HIP_TRY(h, launch_reset_item_counter(h->stream, h->sc));
+        }
+        mark(h, ST_BGKI);
+        mark(h, ST_BGKI_KERNEL);
+        for (int g = 0; g < h->ngroups; ++g)
+            h->h_gargs[g] = make_bg_args(h->gfc + g, h->tg, h->p.ls, h->sigma_occ, h->sigma_free, group_ibuf(h, g),
+                                         group_lbuf(h, g)[2], h->rec + h->gstart[g],
+                                         g == 0 ? h->del_occ : h->gk[g].del_occ,
+                                         g == 0 ? h->del_free : h->gk[g].del_free, nullptr, nullptr, group_sc(h, g));
+        HIP_TRY(h, hipMemcpyAsync(h->d_gargs, h->h_gargs, sizeof(BgArgs) * h->ngroups, hipMemcpyHostToDevice,
+                                  h->stream));
+        HIP_TRY(h, launch_bgki_batch(h->stream, h->gather_blocks, h->d_gargs, h->ngroups, h->d_gq,
+                                     h->d_gq + K3DOM_MAX_LIDARS + 1));
+        for (int g = 0; g < h->ngroups; ++g) {
             GridGeom gg = grid_geom(h);
             gg.sx = h->h_gfc[g].g.sx;
             gg.sy = h->h_gfc[g].g.sy;
             gg.sz = h->h_gfc[g].g.sz;
-            HIP_TRY(h, launch_bgki(h->stream, h->gather_blocks, gfc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
-                                   h->ibuf, h->lbuf[2], h->rec, to, tf, h->sc));
-            HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, gg, h->tg, h->ibuf, to, tf, h->sc));
-            if (g > 0)
-                HIP_TRY(h, launch_add_evidence(h->stream, &h->sc->overflow, h->del_occ, h->del_free, h->gdel_occ,
-                                               h->gdel_free, h->V));
+            HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, gg, h->tg, group_ibuf(h, g),
+                                           g == 0 ? h->del_occ : h->gk[g].del_occ,
+                                           g == 0 ? h->del_free : h->gk[g].del_free, group_sc(h, g)));
         }
+        for (int g = 1; g < h->ngroups; ++g)
+            HIP_TRY(h, launch_add_evidence(h->stream, &h->sc->overflow, h->del_occ, h->del_free, h->gk[g].del_occ,
+                                           h->gk[g].del_free, h->V));
+        if (h->ngroups > 1) HIP_TRY(h, launch_merge_group_scalars(h->stream, h->sc, h->gsc, h->ngroups));
         mark(h, EV_BGKI_END);
         return enqueue_after_gather(h, normal);
     }
@@ -477,7 +571,18 @@ k3dom_status capture_graph(k3dom_handle* h) {
     HIP_TRY(h, e);
     HIP_TRY(h, hipGraphInstantiate(&h->gexec[p], h->graph[p], nullptr, nullptr, 0));
     h->graph_caps[p] = h->caps_version;
+    h->graph_ngroups[p] = h->ngroups;
+    for (int g = 0; g < K3DOM_MAX_LIDARS; ++g) h->graph_gstart[p][g] = g < h->ngroups ? h->gstart[g] : 0;
     return K3DOM_OK;
+}
+
+// The graph of voxel buffer p was captured for this frame's LiDAR layout
+// (group count and point offsets, which its kernels take as arguments).
+bool graph_layout_matches(const k3dom_handle* h, int p) {
+    if (h->graph_ngroups[p] != h->ngroups) return false;
+    for (int g = 0; g < h->ngroups; ++g)
+        if (h->graph_gstart[p][g] != h->gstart[g]) return false;
+    return true;
 }
 
 // Grows what the last run of the kernel-inference stage overflowed (its need
@@ -544,7 +649,7 @@ void count_frame(k3dom_handle* h, bool normal, int m) {
 }
 
 bool frame_uses_graph(const k3dom_handle* h) {
-    return h->updated_time > 0 && h->use_graph && !h->timing && h->ngroups == 0;
+    return h->updated_time > 0 && h->use_graph && !h->timing;
 }
 
 k3dom_status frame_body(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m);
@@ -574,6 +679,10 @@ k3dom_status frame(k3dom_handle* h, double t, const float* dx, const float* dy, 
 
 k3dom_status frame_body(k3dom_handle* h, double t, const float* dx, const float* dy, const float* dz, int m) {
     const bool normal = h->updated_time > 0;
+    if (h->ngroups > 0) {
+        k3dom_status st = ensure_groups(h, h->ngroups);
+        if (st != K3DOM_OK) return st;
+    }
     {
         k3dom_status st = begin_frame_voxels(h, normal);
         if (st != K3DOM_OK) return st;
@@ -602,7 +711,7 @@ k3dom_status frame_body(k3dom_handle* h, double t, const float* dx, const float*
             }
         }
         const int p = h->vcur;
-        if (!h->gexec[p] || h->graph_caps[p] != h->caps_version) {
+        if (!h->gexec[p] || h->graph_caps[p] != h->caps_version || !graph_layout_matches(h, p)) {
             k3dom_status s = capture_graph(h);
             if (s != K3DOM_OK) return s;
         }
@@ -902,6 +1011,7 @@ k3dom_status k3dom_destroy(k3dom_handle* h) {
     if (h->h_sc) (void)hipHostFree(h->h_sc);
     if (h->h_fc) (void)hipHostFree(h->h_fc);
     if (h->h_gfc) (void)hipHostFree(h->h_gfc);
+    if (h->h_gargs) (void)hipHostFree(h->h_gargs);
     drop_graph(h);
     for (int e = 0; e < EV_COUNT; ++e)
         if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
@@ -986,31 +1096,37 @@ k3dom_status k3dom_update_grid_lidars(k3dom_handle* h, double t, const k3dom_lid
     }
     if (total > 0x7fffffff) return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: too many points");
     const int m = (int)total;
-    k3dom_status s = ensure_points(h, std::max(m, 1));
+    // LiDAR k's points at k * gstride of the handle's buffer: the slot only
+    // grows, so frames of one rig keep their offsets (and a captured graph)
+    int most = 1;
+    for (int k = 0; k < n_lidars; ++k) most = std::max(most, lidars[k].n);
+    if ((long long)most * n_lidars > 0x7fffffff)
+        return fail(h, K3DOM_ERR_INVALID_ARG, "update_grid_lidars: too many points");
+    const int stride = std::max(h->gstride, most);
+    k3dom_status s = ensure_points(h, stride * n_lidars);
     if (s != K3DOM_OK) return s;
-    // the frame's points, LiDAR after LiDAR, rotated into map axes in the
-    // handle's buffer (host points go through the pinned staging buffer)
+    h->gstride = stride;
+    // the frame's points, rotated into map axes in the handle's buffer (host
+    // points go through the pinned staging buffer)
     const size_t cap = (size_t)h->m_cap;
     float* dx = h->d_pts;
     float* dy = h->d_pts + cap;
     float* dz = h->d_pts + 2 * cap;
-    int at = 0;
     for (int k = 0; k < n_lidars; ++k) {
         const k3dom_lidar& L = lidars[k];
+        const int at = k * stride;
         if (!points_on_device && L.n > 0) {
             std::memcpy(h->h_pts + at, L.xs, sizeof(float) * L.n);
             std::memcpy(h->h_pts + cap + at, L.ys, sizeof(float) * L.n);
             std::memcpy(h->h_pts + 2 * cap + at, L.zs, sizeof(float) * L.n);
+            for (int c = 0; c < 3; ++c)
+                HIP_TRY(h, hipMemcpyAsync(h->d_pts + c * cap + at, h->h_pts + c * cap + at, sizeof(float) * L.n,
+                                          hipMemcpyHostToDevice, h->stream));
         }
         h->gstart[k] = at;
         h->gcount[k] = L.n;
         for (int i = 0; i < 3; ++i) h->gorig[k][i] = L.origin[i];
-        at += L.n;
     }
-    if (!points_on_device && m > 0)
-        for (int c = 0; c < 3; ++c)
-            HIP_TRY(h, hipMemcpyAsync(h->d_pts + c * cap, h->h_pts + c * cap, sizeof(float) * m,
-                                      hipMemcpyHostToDevice, h->stream));
     for (int k = 0; k < n_lidars; ++k) {
         const k3dom_lidar& L = lidars[k];
         const size_t o = (size_t)h->gstart[k];
@@ -1018,9 +1134,6 @@ k3dom_status k3dom_update_grid_lidars(k3dom_handle* h, double t, const k3dom_lid
                     *sz = points_on_device ? L.zs : dz + o;
         HIP_TRY(h, launch_lidar_rotate(h->stream, L.rot, sx, sy, sz, dx + o, dy + o, dz + o, L.n));
     }
-    // (each checked on its own: a failed allocation leaves the other usable)
-    if (n_lidars > 1 && !h->gdel_occ) HIP_TRY(h, dalloc(h, &h->gdel_occ, (size_t)h->V));
-    if (n_lidars > 1 && !h->gdel_free) HIP_TRY(h, dalloc(h, &h->gdel_free, (size_t)h->V));
     h->ngroups = n_lidars;
     s = frame(h, t, dx, dy, dz, m);
     h->ngroups = 0;

@@ -826,6 +826,32 @@ hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps
     return hipGetLastError();
 }
 
+// The frame's scalars after the joint gather of LiDAR groups: group 0's
+// scalars (sc) take every other group's (gsc[1 .. ng - 1]) overflow flag and
+// bits, so that an overflow anywhere stops the frame tail; the largest list
+// totals and item counts per level, which size every group's regrowth; and
+// the groups' summed items.
+__global__ void k_merge_group_scalars(DevScalars* sc, const DevScalars* __restrict__ gsc, int ng) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    DevScalars m = *sc;
+    for (int g = 1; g < ng; ++g) {
+        const DevScalars& o = gsc[g];
+        m.overflow |= o.overflow;
+        m.overflow_mask |= o.overflow_mask;
+        m.n_light += o.n_light;
+        m.n_heavy += o.n_heavy;
+        for (int l = 0; l < 3; ++l) {
+            m.level_total[l] = max(m.level_total[l], o.level_total[l]);
+            m.level_items[l] = max(m.level_items[l], o.level_items[l]);
+        }
+    }
+    *sc = m;
+}
+hipError_t launch_merge_group_scalars(hipStream_t st, DevScalars* sc, const DevScalars* gsc, int ng) {
+    hipLaunchKernelGGL(k_merge_group_scalars, dim3(1), dim3(64), 0, st, sc, gsc, ng);
+    return hipGetLastError();
+}
+
 // Resets the work-queue head so that the gather can be launched again over
 // the same items (launch-shape tuning).
 __global__ void k_reset_item_counter(DevScalars* sc) {

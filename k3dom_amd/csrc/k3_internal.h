@@ -65,7 +65,21 @@ struct k3dom_handle {
     int gstart[K3DOM_MAX_LIDARS] = {}, gcount[K3DOM_MAX_LIDARS] = {};
     float gorig[K3DOM_MAX_LIDARS][3] = {};
     FrameConsts *gfc = nullptr, *h_gfc = nullptr;
-    float *gdel_occ = nullptr, *gdel_free = nullptr;
+    float *gdel_occ = nullptr, *gdel_free = nullptr;  // (the differentiated window's per-LiDAR passes)
+    // Kernel-inference buffers of LiDAR groups 1.. (group 0 uses lbuf, ibuf
+    // and sc): every group builds its lists and work items into its own
+    // buffers, and one persistent gather launch covers all groups' items.
+    struct GroupKi {
+        LevelBufs lbuf[3]{};
+        ItemBufs ibuf{};
+        float *del_occ = nullptr, *del_free = nullptr;
+    };
+    GroupKi gk[K3DOM_MAX_LIDARS]{};
+    int ngk = 1;                                      // groups with buffers (group 0 always)
+    DevScalars* gsc = nullptr;                        // K3DOM_MAX_LIDARS scalar blocks ([0] unused)
+    BgArgs *d_gargs = nullptr, *h_gargs = nullptr;    // the joint gather's argument sets
+    uint32_t* d_gq = nullptr;                         // its item prefix (K3DOM_MAX_LIDARS + 1) and queue head
+    int gstride = 0;                                  // point-buffer slot of each LiDAR (update_grid_lidars)
     bool last_grouped = false;  // the last frame ran per-LiDAR kernel inference
     uint32_t *mp_flags = nullptr, *mp_off = nullptr, *mp_scratch = nullptr;
     const uint32_t* last_perm = nullptr;
@@ -117,6 +131,9 @@ struct k3dom_handle {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int caps_version = 0;
     int graph_caps[2] = {-1, -1};
+    // the LiDAR layout each graph was captured for (0 groups: one sensor)
+    int graph_ngroups[2] = {0, 0};
+    int graph_gstart[2][K3DOM_MAX_LIDARS] = {};
 
     int timing = 0;  // 0 off, 1 every stage, 2 the kernel-inference gather only
     hipEvent_t ev[EV_COUNT]{};

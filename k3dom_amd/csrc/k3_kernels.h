@@ -191,6 +191,9 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
 hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps, int nmaps, uint32_t* prefix,
                              uint32_t* counter);
 hipError_t launch_reset_item_counter(hipStream_t st, DevScalars* sc);
+// Group 0's scalars after a joint gather of ng LiDAR groups (gsc[1 .. ng - 1]):
+// overflow flags and bits, largest per-level needs, summed items.
+hipError_t launch_merge_group_scalars(hipStream_t st, DevScalars* sc, const DevScalars* gsc, int ng);
 
 // sigma_o / sigma_f weight the occupied / free terms (both the reference's
 // single sigma in a plain frame).  Split items need I.partial with 128 floats
