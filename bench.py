@@ -430,6 +430,7 @@ def main():
     ap.add_argument("--no-lidars", action="store_true", help="skip the four-LiDAR stress leg")
     ap.add_argument("--no-host-array", action="store_true", help="skip the host-array updateGrid leg")
     ap.add_argument("--no-walls", action="store_true", help="skip the every-ray-returns (walls) leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one plain 256 MiB device copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -473,6 +474,12 @@ def main():
 
     for f in range(args.warmup):
         step(f)
+    if not args.no_autotune:
+        # launch shapes (list segments, gather block and grid sizes) picked on
+        # the last warm-up frame; results do not depend on them
+        d.autotune()
+    shape = d.launch_shape()
+    log(f"launch shape: {shape}")
     d.set_timing(2)  # HIP events around the gather kernel only
     kernel_idx = DOM.STAGES.index("bgki_kernel")
     gather_ms = 0.0
@@ -598,7 +605,8 @@ def main():
         "config": {"workload": w.name, "grid": [gs, gs, gsz], "resolution_m": float(np.float32(p.resolution)),
                    "particles": p.particle_count, "newborn_per_frame": p.new_born_particle_count,
                    "lidar": f"{w.scene.beams}x{w.scene.az_bins}", "points_per_frame_mean": int(np.mean(npts)),
-                   "ego_speed_mps": float(w.scene.ego_vx), "parallelism": "single GPU"},
+                   "ego_speed_mps": float(w.scene.ego_vx), "parallelism": "single GPU",
+                   "launch_shape": shape, "launch_shape_source": "default" if args.no_autotune else "autotuned"},
         "stage_ms": {n: round(float(v), 4) for n, v in zip(DOM.STAGES, stage_avg)},
         "stage_ms_frames": f"{BREAKDOWN} frames after the timed ones, every stage timed with HIP events",
         "roofline": roof,

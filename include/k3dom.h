@@ -361,12 +361,27 @@ typedef struct k3dom_stats {
 } k3dom_stats;
 k3dom_status k3dom_get_stats(const k3dom_handle* h, k3dom_stats* out);
 
-/* Launch-shape tuning, no counterpart in the reference: after a normal frame,
- * times the kernel-inference gather over that frame's work items for several
- * grid sizes (1 to 8 blocks of 512 threads per compute unit) and keeps the
- * fastest (written to *blocks_per_cu when not NULL).  Results do not depend
- * on the choice. */
+/* Launch-shape tuning, no counterpart in the reference: after a normal
+ * single-sensor frame, times (1) the candidate-list build over that frame's
+ * point records for several list segment lengths (parent-list entries per
+ * work item: 128-512 at the top level, 256-2048 below), then (2) the
+ * kernel-inference gather over that frame's work items for several block
+ * sizes (4, 8 or 16 wavefronts) and grid sizes (1 to 8 blocks per compute
+ * unit, at most 32 wavefronts per unit), and keeps the fastest of each.
+ * Writes the chosen blocks per compute unit to *blocks_per_cu when not NULL.
+ * Results do not depend on the choices. */
 k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu);
+
+/* The launch shapes in use: shape[0] gather blocks per compute unit,
+ * shape[1] wavefronts per gather block, shape[2] and shape[3] the list
+ * segment lengths of the top level and of the two finer levels.  No
+ * counterpart in the reference. */
+k3dom_status k3dom_launch_shape(const k3dom_handle* h, int32_t shape[4]);
+/* Sets the launch shapes (the layout of k3dom_launch_shape: blocks per
+ * compute unit 1-16, wavefronts per gather block 4, 8 or 16, list segment
+ * lengths 32-8192), e.g. those an earlier k3dom_autotune chose.  Between
+ * frames only.  Results do not depend on the shapes. */
+k3dom_status k3dom_set_launch_shape(k3dom_handle* h, const int32_t shape[4]);
 
 /* ---------------------------------------------------------------------------
  * Batched maps.  No counterpart in the reference (one map per process): many
@@ -395,7 +410,8 @@ k3dom_status k3dom_batch_map(k3dom_batch* b, int32_t i, k3dom_handle** out);
  * over the map streams); captured again when a buffer is reallocated. */
 k3dom_status k3dom_batch_set_graph(k3dom_batch* b, int32_t enable);
 
-/* As k3dom_autotune for the joint gather of the batch. */
+/* As k3dom_autotune's gather sweep (block and grid sizes) for the joint
+ * gather of the batch. */
 k3dom_status k3dom_batch_autotune(k3dom_batch* b, int32_t* blocks_per_cu);
 
 /* Host wall time of the last k3dom_batch_update, milliseconds. */

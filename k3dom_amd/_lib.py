@@ -100,6 +100,8 @@ SIGNATURES = {
     "k3dom_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "k3dom_set_graph": (C.c_int, [C.c_void_p, C.c_int32]),
     "k3dom_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "k3dom_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "k3dom_set_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     "k3dom_batch_create": (C.c_int, [C.POINTER(Params), C.c_int32, C.POINTER(C.c_void_p)]),
     "k3dom_batch_destroy": (C.c_int, [C.c_void_p]),
     "k3dom_batch_update": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(C.c_float), C.POINTER(C.c_void_p),

@@ -482,7 +482,7 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const f
         HIP_TRY(h, hipMemcpyAsync(h->d_gargs, h->h_gargs, sizeof(BgArgs) * h->ngroups, hipMemcpyHostToDevice,
                                   h->stream));
         HIP_TRY(h, launch_bgki_batch(h->stream, h->gather_blocks, h->d_gargs, h->ngroups, h->d_gq,
-                                     h->d_gq + K3DOM_MAX_LIDARS + 1));
+                                     h->d_gq + K3DOM_MAX_LIDARS + 1, h->gather_waves));
         for (int g = 0; g < h->ngroups; ++g) {
             GridGeom gg = grid_geom(h);
             gg.sx = h->h_gfc[g].g.sx;
@@ -505,7 +505,7 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const f
     mark(h, ST_BGKI);
     mark(h, ST_BGKI_KERNEL);
     HIP_TRY(h, launch_bgki(h->stream, h->gather_blocks, h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free, h->ibuf,
-                           h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc));
+                           h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc, h->gather_waves));
     mark(h, EV_BGKI_END);
     return enqueue_after_gather(h, normal);
 }
@@ -1205,7 +1205,7 @@ k3dom_status stage_evidence(k3dom_handle* h, bool normal) {
         k3dom_status s = build_lists(h, h->fc, h->d_pts, h->d_pts + cap, h->d_pts + 2 * cap);
         if (s != K3DOM_OK) return s;
         HIP_TRY(h, launch_bgki(h->stream, h->gather_blocks, h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
-                               h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc));
+                               h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc, h->gather_waves));
         s = tail_mass(h, normal);
         if (s == K3DOM_OK && !normal) s = tail_newborn(h, false);
         if (s != K3DOM_OK) return s;
@@ -1503,34 +1503,120 @@ k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
     if (h->normal_here < 1) return fail(h, K3DOM_ERR_STATE, "autotune needs a normal frame first");
     if (!h->last_frame_ok || point_buffers_missing(h))
         return fail(h, K3DOM_ERR_STATE, "autotune needs the last frame to have completed");
+    if (h->last_grouped) return fail(h, K3DOM_ERR_STATE, "autotune needs a single-sensor frame last");
     EventPair ev;  // destroyed on every return below
     HIP_TRY(h, ev.create());
     hipEvent_t e0 = ev.e0, e1 = ev.e1;
-    const int cands[] = {1, 2, 3, 4, 5, 6, 8};
-    float best = 1e30f;
-    int best_c = h->gather_blocks / h->num_cus;
-    for (int c : cands) {
+    // Median-free timing: 3 runs after one warm-up, summed.
+    auto timed = [&](auto&& enqueue, float* out) -> k3dom_status {
         float tot = 0.0f;
         for (int rep = 0; rep < 4; ++rep) {
-            // the last frame's items are still in place: re-run the gather over them
-            HIP_TRY(h, launch_reset_item_counter(h->stream, h->sc));
             HIP_TRY(h, hipEventRecord(e0, h->stream));
-            HIP_TRY(h, launch_bgki(h->stream, c * h->num_cus, h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
-                                   h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc));
+            k3dom_status s = enqueue();
+            if (s != K3DOM_OK) return s;
             HIP_TRY(h, hipEventRecord(e1, h->stream));
             HIP_TRY(h, hipEventSynchronize(e1));
             float ms = 0.0f;
             HIP_TRY(h, hipEventElapsedTime(&ms, e0, e1));
             if (rep > 0) tot += ms;
         }
-        if (tot < best) {
-            best = tot;
-            best_c = c;
+        *out = tot;
+        return K3DOM_OK;
+    };
+    // 1. list segment lengths: the last frame's candidate lists are rebuilt
+    // from its point records (h->pxyz, h->fc) for each choice; the lists
+    // (ascending per box) and the work items do not depend on it
+    auto rebuild_lists = [&]() -> k3dom_status {
+        for (int l = 0; l < 3; ++l)
+            HIP_TRY(h, launch_build_level(h->stream, l, h->fc, h->tg, h->ldesc[l], h->lbuf[l],
+                                          l ? &h->lbuf[l - 1] : nullptr, h->p.ls, h->pxyz, h->sc));
+        HIP_TRY(h, launch_build_items(h->stream, h->ntiles, h->lbuf[2], h->ibuf, h->sc));
+        return K3DOM_OK;
+    };
+    auto set_segs = [&](int s0, int s12) -> k3dom_status {
+        h->ldesc[0].seg = s0;
+        h->ldesc[1].seg = h->ldesc[2].seg = s12;
+        long long want[3];
+        for (int l = 0; l < 3; ++l) want[l] = h->lbuf[l].list_cap;
+        return ensure_caps(h, want, h->ibuf.heavy_cap);
+    };
+    const int segs0[] = {128, 256, 512};
+    const int segs12[] = {256, 512, 1024, 2048};
+    float best = 1e30f;
+    int b0 = h->ldesc[0].seg, b12 = h->ldesc[2].seg;
+    for (int s0 : segs0)
+        for (int s12 : segs12) {
+            k3dom_status s = set_segs(s0, s12);
+            if (s != K3DOM_OK) return s;
+            float ms = 0.0f;
+            if ((s = timed(rebuild_lists, &ms)) != K3DOM_OK) return s;
+            if (ms < best) {
+                best = ms;
+                b0 = s0;
+                b12 = s12;
+            }
         }
+    {
+        k3dom_status s = set_segs(b0, b12);
+        if (s == K3DOM_OK) s = rebuild_lists();
+        if (s != K3DOM_OK) return s;
+        HIP_TRY(h, hipStreamSynchronize(h->stream));
     }
+    // 2. the gather's block size (wavefronts) and grid size, over the same
+    // items (the gather only rewrites the same evidence); at most 32
+    // wavefronts per compute unit
+    best = 1e30f;
+    int best_w = h->gather_waves, best_c = h->gather_blocks / h->num_cus;
+    for (int wv : {4, 8, 16})
+        for (int c : {1, 2, 3, 4, 5, 6, 8}) {
+            if (wv * c > 32) continue;
+            auto gather = [&]() -> k3dom_status {
+                HIP_TRY(h, launch_reset_item_counter(h->stream, h->sc));
+                HIP_TRY(h, launch_bgki(h->stream, c * h->num_cus, h->fc, h->tg, h->p.ls, h->sigma_occ,
+                                       h->sigma_free, h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc,
+                                       wv));
+                return K3DOM_OK;
+            };
+            float ms = 0.0f;
+            k3dom_status s = timed(gather, &ms);
+            if (s != K3DOM_OK) return s;
+            if (ms < best) {
+                best = ms;
+                best_w = wv;
+                best_c = c;
+            }
+        }
+    h->gather_waves = best_w;
     h->gather_blocks = best_c * h->num_cus;
     drop_graph(h);
     if (blocks_per_cu) *blocks_per_cu = best_c;
+    return K3DOM_OK;
+}
+
+k3dom_status k3dom_set_launch_shape(k3dom_handle* h, const int32_t shape[4]) {
+    if (!h || !shape) return K3DOM_ERR_INVALID_ARG;
+    if (shape[0] < 1 || shape[0] > 16 || (shape[1] != 4 && shape[1] != 8 && shape[1] != 16) || shape[2] < 32 ||
+        shape[2] > 8192 || shape[3] < 32 || shape[3] > 8192)
+        return fail(h, K3DOM_ERR_INVALID_ARG,
+                    "set_launch_shape: blocks per CU 1-16, 4/8/16 wavefronts, list segments 32-8192");
+    if (h->stage != 0) return fail(h, K3DOM_ERR_STATE, "set_launch_shape between frames only");
+    h->gather_blocks = shape[0] * h->num_cus;
+    h->gather_waves = shape[1];
+    h->ldesc[0].seg = shape[2];
+    h->ldesc[1].seg = h->ldesc[2].seg = shape[3];
+    drop_graph(h);
+    if (h->m_cap == 0) return K3DOM_OK;  // sized with the first frame's points
+    long long want[3];
+    for (int l = 0; l < 3; ++l) want[l] = h->lbuf[l].list_cap;
+    return ensure_caps(h, want, h->ibuf.heavy_cap);
+}
+
+k3dom_status k3dom_launch_shape(const k3dom_handle* h, int32_t shape[4]) {
+    if (!h || !shape) return K3DOM_ERR_INVALID_ARG;
+    shape[0] = h->gather_blocks / std::max(1, h->num_cus);
+    shape[1] = h->gather_waves;
+    shape[2] = h->ldesc[0].seg;
+    shape[3] = h->ldesc[2].seg;
     return K3DOM_OK;
 }
 

@@ -28,6 +28,7 @@ struct k3dom_batch {
     uint32_t* d_counter = nullptr;
     void* d_mem = nullptr;
     int gather_blocks = 0;
+    int gather_waves = 8;
     int num_cus = 256;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -82,7 +83,7 @@ k3dom_status enqueue_batch(k3dom_batch* b) {
         B_HIP(b, hipStreamWaitEvent(b->main, b->head_done[i], 0));
     }
     B_HIP(b, hipMemcpyAsync(b->d_args, b->h_args, sizeof(BgArgs) * n, hipMemcpyHostToDevice, b->main));
-    B_HIP(b, launch_bgki_batch(b->main, b->gather_blocks, b->d_args, n, b->d_prefix, b->d_counter));
+    B_HIP(b, launch_bgki_batch(b->main, b->gather_blocks, b->d_args, n, b->d_prefix, b->d_counter, b->gather_waves));
     B_HIP(b, hipEventRecord(b->gathered, b->main));
     for (int i = 0; i < n; ++i) {
         k3dom_handle* h = b->maps[i];
@@ -288,26 +289,30 @@ k3dom_status k3dom_batch_autotune(k3dom_batch* b, int32_t* blocks_per_cu) {
     EventPair ev;  // destroyed on every return below
     B_HIP(b, ev.create());
     hipEvent_t e0 = ev.e0, e1 = ev.e1;
-    const int cands[] = {1, 2, 3, 4, 5, 6, 8};
+    // block size (wavefronts) and grid size, at most 32 wavefronts per compute unit
     float best = 1e30f;
-    int best_c = b->gather_blocks / b->num_cus;
-    for (int c : cands) {
-        float tot = 0.0f;
-        for (int rep = 0; rep < 4; ++rep) {
-            B_HIP(b, hipEventRecord(e0, b->main));
-            B_HIP(b, launch_bgki_batch(b->main, c * b->num_cus, b->d_args, n, b->d_prefix, b->d_counter));
-            B_HIP(b, hipEventRecord(e1, b->main));
-            B_HIP(b, hipEventSynchronize(e1));
-            float ms = 0.0f;
-            B_HIP(b, hipEventElapsedTime(&ms, e0, e1));
-            if (rep > 0) tot += ms;
+    int best_c = b->gather_blocks / b->num_cus, best_w = b->gather_waves;
+    for (int wv : {4, 8, 16})
+        for (int c : {1, 2, 3, 4, 5, 6, 8}) {
+            if (wv * c > 32) continue;
+            float tot = 0.0f;
+            for (int rep = 0; rep < 4; ++rep) {
+                B_HIP(b, hipEventRecord(e0, b->main));
+                B_HIP(b, launch_bgki_batch(b->main, c * b->num_cus, b->d_args, n, b->d_prefix, b->d_counter, wv));
+                B_HIP(b, hipEventRecord(e1, b->main));
+                B_HIP(b, hipEventSynchronize(e1));
+                float ms = 0.0f;
+                B_HIP(b, hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) tot += ms;
+            }
+            if (tot < best) {
+                best = tot;
+                best_c = c;
+                best_w = wv;
+            }
         }
-        if (tot < best) {
-            best = tot;
-            best_c = c;
-        }
-    }
     b->gather_blocks = best_c * b->num_cus;
+    b->gather_waves = best_w;
     drop_batch_graph(b);
     if (blocks_per_cu) *blocks_per_cu = best_c;
     return K3DOM_OK;

@@ -27,9 +27,8 @@ namespace k3 {
 namespace {
 constexpr int TPB = 256;
 constexpr int BG_WAVES = TPB / 64;
-// the gather runs blocks of GA_TPB threads: one kernel table in LDS per 8 wavefronts
-constexpr int GA_TPB = 512;
-constexpr int GA_WAVES = GA_TPB / 64;
+// the gather runs blocks of W wavefronts (4, 8 or 16; picked by the
+// autotuner, 8 by default): one kernel table in LDS per block
 constexpr uint32_t BG_HEAVY_LEN = 1u << K3_BGKI_CHUNK_LOG2;
 constexpr uint32_t BG_DIRECT = 0xffffffffu;
 
@@ -541,7 +540,7 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float si
 // Copies the kernel table(s) into LDS (every thread of the block takes part).
 template <bool GRAD>
 __device__ __forceinline__ void bg_load_table(BgTable<GRAD>* stab) {
-    for (int i = threadIdx.x; i <= K3_KTAB_N; i += GA_TPB) {
+    for (int i = threadIdx.x; i <= K3_KTAB_N; i += blockDim.x) {
         stab->ab[i] = make_float2(k3_ktab[4 * i], k3_ktab[4 * i + 1]);
         stab->c[i] = k3_ktab[4 * i + 2];
         if constexpr (GRAD) {
@@ -560,8 +559,7 @@ constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 // Gather of one work item (a whole tile, or one 1024-point chunk of a long
 // tile list) by one wavefront: lane = voxel of the 4x4x4 tile.
 template <bool GRAD>
-__device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, int lane,
-                                          float4 (&srec)[GA_WAVES][64 * 3],
+__device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane, float4* srec,
                                           const BgTable<GRAD>* stab) {
     const GridGeom g = a.fc->g;
     const uint4 item = a.items[it];
@@ -595,9 +593,9 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
             const uint32_t p = a.tlist[e];
             const K3PointRec r = a.rec[p];
             const float4* src = reinterpret_cast<const float4*>(a.rec + p);
-            srec[w][lane * 3 + 0] = src[0];
-            srec[w][lane * 3 + 1] = src[1];
-            srec[w][lane * 3 + 2] = src[2];
+            srec[lane * 3 + 0] = src[0];
+            srec[lane * 3 + 1] = src[1];
+            srec[lane * 3 + 2] = src[2];
             const float proj = fmaf(tcz, r.z, fmaf(tcy, r.y, tcx * r.x)) * r.inv_dm;
             uint32_t cls = C_GENERAL;
             if (proj + rr <= 0.0f) cls = C_BEHIND;
@@ -611,7 +609,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
         const uint32_t cnt = uniform(min(64u, ie - b));
         for (uint32_t q = 0; q < cnt; ++q) {
             const uint32_t meta = lane_word(meta_l, q);
-            const float4 r0 = srec[w][q * 3];
+            const float4 r0 = srec[q * 3];
             const uint32_t chunk = meta >> (3 + K3_BGKI_CHUNK_LOG2);
             const uint32_t cls = meta & 7u;
             if (chunk != cur_chunk) {
@@ -629,7 +627,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 const float a2a = fmaf(-dpa, dpa, dc2);  // clamped at 0 by bg_term
                 const uint32_t meta2 = q + 1 < cnt ? lane_word(meta_l, q + 1) : 0u;
                 if (q + 1 < cnt && (meta2 & 7u) == C_MAIN && (meta2 >> (3 + K3_BGKI_CHUNK_LOG2)) == chunk) {
-                    const float4 r0b = srec[w][(q + 1) * 3];
+                    const float4 r0b = srec[(q + 1) * 3];
                     const float dpb = fmaf(vz, r0b.z, fmaf(vy, r0b.y, vx * r0b.x)) * r0b.w;
                     const float a2b = fmaf(-dpb, dpb, dc2);
                     const BgTerm<GRAD> ta = bg_term<GRAD>(a2a, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
@@ -673,8 +671,8 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
                 cur_f += kc.v;
                 if (GRAD) gsum_f += kc.g;
             } else if (fc == C_GENERAL) {  // k3_free_distance, the three cases as selects
-                const float4 r1 = srec[w][q * 3 + 1];
-                const float4 r2 = srec[w][q * 3 + 2];
+                const float4 r1 = srec[q * 3 + 1];
+                const float4 r2 = srec[q * 3 + 2];
                 const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
                 const bool c1 = dp <= 0.0f;
                 const bool c2 = !c1 && dp < r1.y;
@@ -719,9 +717,9 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int w, i
     }
 }
 
-template <bool GRAD>
-__global__ __launch_bounds__(GA_TPB) void k_bgki(BgArgs a) {
-    __shared__ float4 srec[GA_WAVES][64 * 3];
+template <bool GRAD, int W>
+__global__ __launch_bounds__(W * 64) void k_bgki(BgArgs a) {
+    __shared__ float4 srec[W][64 * 3];
     __shared__ BgTable<GRAD> stab_s;
     BgTable<GRAD>* stab = &stab_s;
     DevScalars* sc = a.sc;
@@ -734,7 +732,7 @@ __global__ __launch_bounds__(GA_TPB) void k_bgki(BgArgs a) {
         if (lane == 0) it = atomicAdd(&sc->item_counter, 1u);
         it = uniform(__shfl(it, 0));
         if (it >= n_items) break;
-        bgki_item<GRAD>(a, it, w, lane, srec, stab);
+        bgki_item<GRAD>(a, it, lane, srec[w], stab);
     }
 }
 
@@ -762,20 +760,28 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
     return a;
 }
 
+template <bool GRAD>
+static hipError_t launch_bgki_w(hipStream_t st, int grid_blocks, int waves, const BgArgs& a) {
+    if (waves == 4) hipLaunchKernelGGL((k_bgki<GRAD, 4>), dim3(grid_blocks), dim3(256), 0, st, a);
+    else if (waves == 16) hipLaunchKernelGGL((k_bgki<GRAD, 16>), dim3(grid_blocks), dim3(1024), 0, st, a);
+    else hipLaunchKernelGGL((k_bgki<GRAD, 8>), dim3(grid_blocks), dim3(512), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                        float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec,
-                       float* del_occ, float* del_free, DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki<false>, dim3(grid_blocks), dim3(GA_TPB), 0, st,
-                       make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, nullptr, nullptr, sc));
-    return hipGetLastError();
+                       float* del_occ, float* del_free, DevScalars* sc, int waves) {
+    return launch_bgki_w<false>(st, grid_blocks, waves,
+                                make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, nullptr,
+                                             nullptr, sc));
 }
 hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T,
                             const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
                             DevScalars* sc) {
-    hipLaunchKernelGGL(k_bgki<true>, dim3(grid_blocks), dim3(GA_TPB), 0, st,
-                       make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, dls_occ, dls_free, sc));
-    return hipGetLastError();
+    return launch_bgki_w<true>(st, grid_blocks, 8,
+                               make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, dls_occ,
+                                            dls_free, sc));
 }
 
 // ---------------------------------------------------------------- many maps
@@ -795,9 +801,10 @@ __global__ void k_bgki_batch_prefix(const BgArgs* __restrict__ maps, int nmaps, 
 }
 
 // One persistent launch over the items of all maps.
-__global__ __launch_bounds__(GA_TPB) void k_bgki_batch(const BgArgs* __restrict__ maps, int nmaps,
-                                                    const uint32_t* __restrict__ prefix, uint32_t* counter) {
-    __shared__ float4 srec[GA_WAVES][64 * 3];
+template <int W>
+__global__ __launch_bounds__(W * 64) void k_bgki_batch(const BgArgs* __restrict__ maps, int nmaps,
+                                                       const uint32_t* __restrict__ prefix, uint32_t* counter) {
+    __shared__ float4 srec[W][64 * 3];
     __shared__ BgTable<false> stab_s;
     BgTable<false>* stab = &stab_s;
     bg_load_table<false>(stab);
@@ -815,14 +822,19 @@ __global__ __launch_bounds__(GA_TPB) void k_bgki_batch(const BgArgs* __restrict_
             else hi = mid - 1;
         }
         const BgArgs a = maps[lo];
-        bgki_item<false>(a, it - prefix[lo], w, lane, srec, stab);
+        bgki_item<false>(a, it - prefix[lo], lane, srec[w], stab);
     }
 }
 
 hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps, int nmaps, uint32_t* prefix,
-                             uint32_t* counter) {
+                             uint32_t* counter, int waves) {
     hipLaunchKernelGGL(k_bgki_batch_prefix, dim3(1), dim3(64), 0, st, maps, nmaps, prefix, counter);
-    hipLaunchKernelGGL(k_bgki_batch, dim3(grid_blocks), dim3(GA_TPB), 0, st, maps, nmaps, prefix, counter);
+    if (waves == 4)
+        hipLaunchKernelGGL(k_bgki_batch<4>, dim3(grid_blocks), dim3(256), 0, st, maps, nmaps, prefix, counter);
+    else if (waves == 16)
+        hipLaunchKernelGGL(k_bgki_batch<16>, dim3(grid_blocks), dim3(1024), 0, st, maps, nmaps, prefix, counter);
+    else
+        hipLaunchKernelGGL(k_bgki_batch<8>, dim3(grid_blocks), dim3(512), 0, st, maps, nmaps, prefix, counter);
     return hipGetLastError();
 }
 

@@ -189,7 +189,7 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
 // The gather over the items of several maps in one persistent launch
 // (maps: device array of nmaps argument sets; prefix: nmaps + 1 entries).
 hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps, int nmaps, uint32_t* prefix,
-                             uint32_t* counter);
+                             uint32_t* counter, int waves = 8);
 hipError_t launch_reset_item_counter(hipStream_t st, DevScalars* sc);
 // Group 0's scalars after a joint gather of ng LiDAR groups (gsc[1 .. ng - 1]):
 // overflow flags and bits, largest per-level needs, summed items.
@@ -198,9 +198,10 @@ hipError_t launch_merge_group_scalars(hipStream_t st, DevScalars* sc, const DevS
 // sigma_o / sigma_f weight the occupied / free terms (both the reference's
 // single sigma in a plain frame).  Split items need I.partial with 128 floats
 // per item (256 for the _grad variants).
+// waves: wavefronts per block of the gather (4, 8 or 16)
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                        float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles, const K3PointRec* rec,
-                       float* del_occ, float* del_free, DevScalars* sc);
+                       float* del_occ, float* del_free, DevScalars* sc, int waves = 8);
 hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg, const ItemBufs& I,
                                float* del_occ, float* del_free, DevScalars* sc);
 // The same evidence plus its derivative in the length scale (dls_*).

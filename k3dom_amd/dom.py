@@ -295,11 +295,24 @@ class DOM:
         check(self._lib.k3dom_set_timing(self._h, level), self._h, "set_timing")
 
     def autotune(self):
-        """Times the gather over the last frame's items for several grid sizes and
-        keeps the fastest; returns the chosen blocks per compute unit."""
+        """Times the candidate-list build for several list segment lengths and
+        the gather over the last frame's items for several block and grid
+        sizes, keeps the fastest; returns the chosen blocks per compute unit."""
         c = C.c_int32()
         check(self._lib.k3dom_autotune(self._h, C.byref(c)), self._h, "autotune")
         return c.value
+
+    def launch_shape(self):
+        """The launch shapes in use: gather blocks per compute unit, wavefronts
+        per gather block, and the list segment lengths (top level, finer levels)."""
+        s = (C.c_int32 * 4)()
+        check(self._lib.k3dom_launch_shape(self._h, s), self._h, "launch_shape")
+        return {"blocks_per_cu": s[0], "gather_waves": s[1], "list_seg_top": s[2], "list_seg_fine": s[3]}
+
+    def set_launch_shape(self, blocks_per_cu, gather_waves, list_seg_top, list_seg_fine):
+        """Sets the launch shapes (between frames; results do not depend on them)."""
+        s = (C.c_int32 * 4)(blocks_per_cu, gather_waves, list_seg_top, list_seg_fine)
+        check(self._lib.k3dom_set_launch_shape(self._h, s), self._h, "set_launch_shape")
 
     def set_graph(self, on=True):
         """Replay each normal frame as one captured HIP graph (same results)."""

@@ -161,4 +161,33 @@ def test_single_map_autotune_keeps_results(gpu):
             m.updateGrid(t, xs, ys, zs)
         if f == 1:
             assert gd.autotune() in (1, 2, 3, 4, 5, 6, 8)
+            shape = gd.launch_shape()
+            assert shape["gather_waves"] in (4, 8, 16) and shape["blocks_per_cu"] * shape["gather_waves"] <= 32
+            assert shape["list_seg_top"] in (128, 256, 512) and shape["list_seg_fine"] in (256, 512, 1024, 2048)
         compare_state(gd, od, normal_frame=normal, what=f"autotuned frame {f}:")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(1, 4, 128, 256), (8, 4, 512, 2048), (2, 16, 64, 512), (3, 8, 256, 1024)])
+def test_launch_shapes_keep_results(gpu, shape, monkeypatch):
+    """Every gather block size and list segment length the autotuner may
+    pick (and shorter segments, with tiny list capacities that overflow and
+    regrow) gives the restatement's frames bit for bit."""
+    from k3dom_amd.dom import DOM
+    from k3dom_amd.scene import Scene, small_workload
+    from oracle.orc import OracleDOM
+    from tests._parity import compare_state
+    if shape[2] == 64:
+        monkeypatch.setenv("K3DOM_TEST_TINY_CAPACITY", "1")
+    w = small_workload(frames=4, seed=17, ego_vx=3.0)
+    sc = Scene(w.scene)
+    gd, od = DOM(w.params), OracleDOM(w.params)
+    gd.set_launch_shape(*shape)
+    assert tuple(gd.launch_shape().values()) == shape
+    for f in range(w.frames):
+        t, pose, xs, ys, zs = sc.frame(f)
+        normal = gd.info().updated_time > 0
+        for m in (gd, od):
+            m.updatePose(*pose)
+            m.updateGrid(t, xs, ys, zs)
+        compare_state(gd, od, normal_frame=normal, what=f"shape {shape} frame {f}:")
