@@ -44,9 +44,9 @@ __device__ void block_sum_out(double (&v)[K], double* out) {
 }  // namespace
 
 // ------------------------------------------------------------------ record
-__global__ __launch_bounds__(TPB) void k_window_record(int V, int gs, int gsz, int normal, float min2, VoxelArrays v,
-                                                       ParticleArrays Ps, ParticleArrays NB,
-                                                       const uint32_t* __restrict__ bnd, WindowFrame rec,
+// Voxel pass: masses, dynamic mass and loss of every voxel; the particle
+// counts are zeroed here and added by the particle-parallel counts below.
+__global__ __launch_bounds__(TPB) void k_window_record(int V, VoxelArrays v, WindowFrame rec,
                                                        const uint8_t* __restrict__ gt,
                                                        const uint8_t* __restrict__ mask, double* loss_partial) {
     const int c = blockIdx.x * TPB + threadIdx.x;
@@ -57,44 +57,66 @@ __global__ __launch_bounds__(TPB) void k_window_record(int V, int gs, int gsz, i
         rec.s[c] = s;
         rec.d[c] = d;
         rec.dyn[c] = v.pred[c];
-        int n = 0, m = 0, nb = 0, mb = 0;
-        if (normal) {
-            const int st = v.start[c];
-            if (st != -1) {
-                const int en = v.end[c];
-                for (int j = st; j <= en; ++j) {
-                    // inside test of k3_predict_particle and penalty test of the persistent update
-                    const float x = Ps.px[j], y = Ps.py[j], z = Ps.pz[j];
-                    const bool inb = !((x >= (float)gs || x < 0.0f) || (y >= (float)gs || y < 0.0f) ||
-                                       (z >= (float)gsz || z < 0.0f));
-                    const float vx = Ps.vx[j], vy = Ps.vy[j], vz = Ps.vz[j];
-                    const bool pen = vx * vx + vy * vy + vz * vz < min2;
-                    n += inb;
-                    m += inb && !pen;
-                }
-            }
-            const uint32_t lo = c ? bnd[c - 1] : 0u, hi = bnd[c];
-            for (uint32_t k = lo; k < hi; ++k) {
-                const float vx = NB.vx[k], vy = NB.vy[k], vz = NB.vz[k];
-                nb += 1;
-                mb += !(vx * vx + vy * vy + vz * vz < min2);
-            }
-        }
-        rec.n[c] = n;
-        rec.m[c] = m;
-        rec.nb[c] = nb;
-        rec.mb[c] = mb;
+        rec.n[c] = 0;
+        rec.m[c] = 0;
+        rec.nb[c] = 0;
+        rec.mb[c] = 0;
         if (!mask || mask[c]) loss[0] = (double)k3g_bce_occ<float, float>(f, s, d, gt[c]);
     }
     block_sum_out<1>(loss, loss_partial);
 }
 
+// Particle-parallel counts per voxel over a key-sorted particle set (the
+// sorted persistent set, or the newborn set in voxel order): n[key] counts
+// the particles inside the grid (persistent set only: the inside test of
+// k3_predict_particle), m[key] those of them above the minimum speed (the
+// penalty test of the persistent update).  Each wavefront adds, per run of
+// equal keys among its 64 particles, the run's counts with one atomic add.
+__global__ __launch_bounds__(TPB) void k_window_count(int n, int gs, int gsz, int test_inside, float min2,
+                                                      ParticleArrays P, int32_t* cnt_n, int32_t* cnt_m) {
+    const int j = blockIdx.x * TPB + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = j < n;
+    int key = -1;
+    bool in = false, fast = false;
+    if (valid) {
+        key = P.key[j];
+        const float x = P.px[j], y = P.py[j], z = P.pz[j];
+        in = !test_inside || !((x >= (float)gs || x < 0.0f) || (y >= (float)gs || y < 0.0f) ||
+                               (z >= (float)gsz || z < 0.0f));
+        const float vx = P.vx[j], vy = P.vy[j], vz = P.vz[j];
+        fast = !(vx * vx + vy * vy + vz * vz < min2);
+    }
+    // (the permutes run in every lane)
+    const int prev = __builtin_amdgcn_ds_bpermute(((lane + 63) & 63) << 2, key);
+    const int next = __builtin_amdgcn_ds_bpermute(((lane + 1) & 63) << 2, key);
+    const uint64_t heads = __ballot(valid && (lane == 0 || prev != key));
+    const uint64_t inm = __ballot(valid && in), okm = __ballot(valid && in && fast);
+    const bool last = valid && (lane == 63 || next != key);
+    if (last) {
+        const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const int start = 63 - __clzll(heads & upto);
+        const uint64_t run = upto & ~((1ull << start) - 1ull);
+        const int rn = __popcll(inm & run), rm = __popcll(okm & run);
+        if (rn) atomicAdd(&cnt_n[key], rn);
+        if (rm) atomicAdd(&cnt_m[key], rm);
+    }
+}
+
 hipError_t launch_window_record(hipStream_t st, int V, int gs, int gsz, bool normal, float min_vel,
-                                const VoxelArrays& v, const ParticleArrays& Ps, const ParticleArrays& NB,
-                                const uint32_t* bnd, const WindowFrame& rec, const uint8_t* gt, const uint8_t* mask,
+                                const VoxelArrays& v, const ParticleArrays& Ps, int N, const ParticleArrays& NB,
+                                int B, const WindowFrame& rec, const uint8_t* gt, const uint8_t* mask,
                                 double* loss_partial) {
-    hipLaunchKernelGGL(k_window_record, dim3(blocks_for(V)), dim3(TPB), 0, st, V, gs, gsz, normal ? 1 : 0,
-                       min_vel * min_vel, v, Ps, NB, bnd, rec, gt, mask, loss_partial);
+    hipLaunchKernelGGL(k_window_record, dim3(blocks_for(V)), dim3(TPB), 0, st, V, v, rec, gt, mask, loss_partial);
+    if (normal) {
+        const float min2 = min_vel * min_vel;
+        if (N > 0)
+            hipLaunchKernelGGL(k_window_count, dim3(blocks_for(N)), dim3(TPB), 0, st, N, gs, gsz, 1, min2, Ps,
+                               rec.n, rec.m);
+        if (B > 0)
+            hipLaunchKernelGGL(k_window_count, dim3(blocks_for(B)), dim3(TPB), 0, st, B, gs, gsz, 0, min2, NB,
+                               rec.nb, rec.mb);
+    }
     return hipGetLastError();
 }
 int window_blocks(int V) { return (int)blocks_for(V); }

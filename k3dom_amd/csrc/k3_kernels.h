@@ -315,9 +315,10 @@ struct AdjointFrameArgs {
     int has_w, has_w_prev;     // W of this / of the previous frame depends on the parameters
 };
 int window_blocks(int V);
+// Ps: the N key-sorted persistent particles, NB: the B newborn ones (in voxel order)
 hipError_t launch_window_record(hipStream_t st, int V, int gs, int gsz, bool normal, float min_vel,
-                                const VoxelArrays& v, const ParticleArrays& Ps, const ParticleArrays& NB,
-                                const uint32_t* bnd, const WindowFrame& rec, const uint8_t* gt, const uint8_t* mask,
+                                const VoxelArrays& v, const ParticleArrays& Ps, int N, const ParticleArrays& NB,
+                                int B, const WindowFrame& rec, const uint8_t* gt, const uint8_t* mask,
                                 double* loss_partial);
 // acc[0]: lambda_W (in: this frame's, out: the previous frame's); acc[1..6] += gradients
 hipError_t launch_window_adjoint(hipStream_t st, const AdjointFrameArgs& a, double* acc, double* partial);

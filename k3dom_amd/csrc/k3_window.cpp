@@ -308,8 +308,9 @@ k3dom_status k3dom_window_forward(k3dom_window* w, const float theta[6], double*
         w->h_fc[f] = *h->h_fc;
         for (int g = 0; g < fi.ngroups; ++g) w->h_gfc[(size_t)f * K3DOM_MAX_LIDARS + g] = h->h_gfc[g];
         W_HIP(w, launch_window_record(h->stream, w->V, h->gs, h->gsz, fo.normal, h->p.particle_min_vel,
-                                      h->vox[h->vcur], h->P[1], h->NB, h->bnd, w->rec[f], w->d_gt + (size_t)f * w->V,
-                                      w->use_mask ? w->d_mask : nullptr, w->loss_partial + (size_t)f * nb));
+                                      h->vox[h->vcur], h->P[1], h->N, h->NB, h->B, w->rec[f],
+                                      w->d_gt + (size_t)f * w->V, w->use_mask ? w->d_mask : nullptr,
+                                      w->loss_partial + (size_t)f * nb));
     }
     std::vector<double> lp((size_t)w->T * nb);
     W_HIP(w, hipMemcpyAsync(lp.data(), w->loss_partial, sizeof(double) * lp.size(), hipMemcpyDeviceToHost, h->stream));
