@@ -211,6 +211,7 @@ k3dom_status alloc_group_fixed(k3dom_handle* h, k3dom_handle::GroupKi& G) {
     HIP_TRY(h, dalloc(h, &I.light_off, T + 1));
     HIP_TRY(h, dalloc_zero(h, &I.bsum, (size_t)scan_scratch_words((long long)T)));
     HIP_TRY(h, dalloc(h, &I.tile_partial, 2 * T));
+    HIP_TRY(h, dalloc(h, &I.qheads, BG_QWORDS));
     HIP_TRY(h, dalloc(h, &G.del_occ, (size_t)h->V));
     HIP_TRY(h, dalloc(h, &G.del_free, (size_t)h->V));
     return K3DOM_OK;
@@ -223,7 +224,7 @@ k3dom_status ensure_groups(k3dom_handle* h, int n) {
     // (each allocated while missing: a failed allocation leaves the others in place)
     if (!h->gsc) HIP_TRY(h, dalloc_zero(h, &h->gsc, (size_t)K3DOM_MAX_LIDARS));
     if (!h->d_gargs) HIP_TRY(h, dalloc(h, &h->d_gargs, (size_t)K3DOM_MAX_LIDARS));
-    if (!h->d_gq) HIP_TRY(h, dalloc(h, &h->d_gq, (size_t)K3DOM_MAX_LIDARS + 2));
+    if (!h->d_gq) HIP_TRY(h, dalloc(h, &h->d_gq, (size_t)K3DOM_MAX_LIDARS + 1 + BG_QWORDS));
     if (!h->h_gargs)
         HIP_TRY(h, hipHostMalloc((void**)&h->h_gargs, sizeof(BgArgs) * K3DOM_MAX_LIDARS, hipHostMallocDefault));
     if (n <= h->ngk) return K3DOM_OK;
@@ -304,7 +305,7 @@ k3dom_status build_lists_group(k3dom_handle* h, int g, size_t off, const FrameCo
     LevelBufs* LB = group_lbuf(h, g);
     DevScalars* sc = group_sc(h, g);
     float4* pxyz = h->pxyz + off;
-    HIP_TRY(h, launch_point_prep(h->stream, fc, xs, ys, zs, h->m_cap, h->p.resolution, h->rec + off, pxyz));
+    HIP_TRY(h, launch_point_prep(h->stream, fc, xs, ys, zs, h->m_cap, h->p.resolution, h->p.ls, h->rec + off, pxyz));
     for (int l = 0; l < 3; ++l)
         HIP_TRY(h, launch_build_level(h->stream, l, fc, h->tg, h->ldesc[l], LB[l], l ? &LB[l - 1] : nullptr, h->p.ls,
                                       pxyz, sc));
@@ -957,6 +958,7 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
     CREATE_TRY(dalloc(h, &I.light_off, T + 1));
     CREATE_TRY(dalloc_zero(h, &I.bsum, (size_t)scan_scratch_words((long long)T)));
     CREATE_TRY(dalloc(h, &I.tile_partial, 2 * T));
+    CREATE_TRY(dalloc(h, &I.qheads, BG_QWORDS));
     {
         const char* tiny = std::getenv("K3DOM_TEST_TINY_CAPACITY");
         h->tiny_caps = tiny && tiny[0] == '1';
@@ -1571,7 +1573,6 @@ k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
         for (int c : {1, 2, 3, 4, 5, 6, 8}) {
             if (wv * c > 32) continue;
             auto gather = [&]() -> k3dom_status {
-                HIP_TRY(h, launch_reset_item_counter(h->stream, h->sc));
                 HIP_TRY(h, launch_bgki(h->stream, c * h->num_cus, h->fc, h->tg, h->p.ls, h->sigma_occ,
                                        h->sigma_free, h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc,
                                        wv));

@@ -25,7 +25,7 @@ struct k3dom_batch {
     BgArgs* d_args = nullptr;   // device argument sets of the joint gather
     BgArgs* h_args = nullptr;   // pinned staging
     uint32_t* d_prefix = nullptr;
-    uint32_t* d_counter = nullptr;
+    uint32_t* d_qheads = nullptr;  // work-queue heads of the joint gather (BG_QWORDS)
     void* d_mem = nullptr;
     int gather_blocks = 0;
     int gather_waves = 8;
@@ -83,7 +83,7 @@ k3dom_status enqueue_batch(k3dom_batch* b) {
         B_HIP(b, hipStreamWaitEvent(b->main, b->head_done[i], 0));
     }
     B_HIP(b, hipMemcpyAsync(b->d_args, b->h_args, sizeof(BgArgs) * n, hipMemcpyHostToDevice, b->main));
-    B_HIP(b, launch_bgki_batch(b->main, b->gather_blocks, b->d_args, n, b->d_prefix, b->d_counter, b->gather_waves));
+    B_HIP(b, launch_bgki_batch(b->main, b->gather_blocks, b->d_args, n, b->d_prefix, b->d_qheads, b->gather_waves));
     B_HIP(b, hipEventRecord(b->gathered, b->main));
     for (int i = 0; i < n; ++i) {
         k3dom_handle* h = b->maps[i];
@@ -166,11 +166,11 @@ k3dom_status k3dom_batch_create(const k3dom_params* params, int32_t n_maps, k3do
         if ((e = hipEventCreateWithFlags(&b->head_done[i], hipEventDisableTiming)) != hipSuccess) return bail(e);
         if ((e = hipEventCreateWithFlags(&b->tail_done[i], hipEventDisableTiming)) != hipSuccess) return bail(e);
     }
-    const size_t bytes = sizeof(BgArgs) * n_maps + sizeof(uint32_t) * (n_maps + 2);
+    const size_t bytes = sizeof(BgArgs) * n_maps + sizeof(uint32_t) * (n_maps + 1 + BG_QWORDS);
     if ((e = hipMalloc(&b->d_mem, bytes)) != hipSuccess) return bail(e);
     b->d_args = (BgArgs*)b->d_mem;
     b->d_prefix = (uint32_t*)((char*)b->d_mem + sizeof(BgArgs) * n_maps);
-    b->d_counter = b->d_prefix + n_maps + 1;
+    b->d_qheads = b->d_prefix + n_maps + 1;
     if ((e = hipHostMalloc((void**)&b->h_args, sizeof(BgArgs) * n_maps, hipHostMallocDefault)) != hipSuccess)
         return bail(e);
     *out = b;
@@ -298,7 +298,7 @@ k3dom_status k3dom_batch_autotune(k3dom_batch* b, int32_t* blocks_per_cu) {
             float tot = 0.0f;
             for (int rep = 0; rep < 4; ++rep) {
                 B_HIP(b, hipEventRecord(e0, b->main));
-                B_HIP(b, launch_bgki_batch(b->main, c * b->num_cus, b->d_args, n, b->d_prefix, b->d_counter, wv));
+                B_HIP(b, launch_bgki_batch(b->main, c * b->num_cus, b->d_args, n, b->d_prefix, b->d_qheads, wv));
                 B_HIP(b, hipEventRecord(e1, b->main));
                 B_HIP(b, hipEventSynchronize(e1));
                 float ms = 0.0f;

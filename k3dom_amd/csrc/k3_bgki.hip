@@ -46,6 +46,10 @@ __device__ __forceinline__ void bg_wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// A value equal in every lane, held in a scalar register.
+__device__ __forceinline__ float uniform_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 // The word of lane q (q the same in every lane) in every lane.
 __device__ __forceinline__ uint32_t lane_word(uint32_t v, uint32_t q) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q);
@@ -54,18 +58,20 @@ __device__ __forceinline__ uint32_t lane_word(uint32_t v, uint32_t q) {
 // ============================================================== points
 __global__ __launch_bounds__(TPB) void k_point_prep(const FrameConsts* __restrict__ fc,
                                                     const float* __restrict__ xs, const float* __restrict__ ys,
-                                                    const float* __restrict__ zs, float res,
+                                                    const float* __restrict__ zs, float res, float sk,
                                                     K3PointRec* __restrict__ rec, float4* __restrict__ pxyz) {
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= fc->m) return;
-    K3PointRec r = k3_point_prep(xs[i], ys[i], zs[i], res);
+    const float x = xs[i], y = ys[i], z = zs[i];
+    K3PointRec r = k3_point_prep(x, y, z, res, sk);
     rec[i] = r;
-    pxyz[i] = make_float4(r.x, r.y, r.z, r.valid != 0.0f ? r.dm : -1.0f);
+    pxyz[i] = make_float4(x, y, z, r.valid != 0.0f ? sqrtf(fmaf(z, z, fmaf(y, y, x * x))) : -1.0f);
 }
 hipError_t launch_point_prep(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
-                             const float* zs, int m_cap, float res, K3PointRec* rec, float4* pxyz) {
+                             const float* zs, int m_cap, float res, float ls, K3PointRec* rec, float4* pxyz) {
     if (m_cap <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(m_cap)), dim3(TPB), 0, st, fc, xs, ys, zs, res, rec, pxyz);
+    hipLaunchKernelGGL(k_point_prep, dim3(blocks_for(m_cap)), dim3(TPB), 0, st, fc, xs, ys, zs, res, k3_kernel_sk(ls),
+                       rec, pxyz);
     return hipGetLastError();
 }
 
@@ -507,20 +513,18 @@ struct BgTable<true> : BgTableV {
     float2 gab[K3_KTAB_N + 1];
     float gc[K3_KTAB_N + 1];
 };
-// a2 may be slightly negative (a rounded difference of squares standing for
-// 0): the table argument is clamp(a2 * kscale, 0, N) in one med3, which for
-// every non-NaN a2 equals min(max(a2, 0) * kscale, N) of k3_kernel_index, and
-// the fraction is taken with fract, which equals s - (float)(int)s exactly
-// for 0 <= s < 2^24, so the result is that of k3_kernel_value_sq bit for bit.
-// The length-scale derivative (GRAD) is sigma G(u) / ls from the second
-// table at the same index and fraction (k3_kernel_dls_sq), where the value is
-// positive.
+// The table argument s (in table units, see K3PointRec) may be slightly
+// negative (a rounded difference of squares standing for 0) or beyond N: it
+// is clamped with one med3, equal to min(max(s, 0), N) of k3_kernel_index_s
+// for every non-NaN s, and the fraction is taken with fract, which equals
+// c - (float)(int)c exactly for 0 <= c < 2^24, so the result is that of
+// k3_kernel_value_s bit for bit.  The length-scale derivative (GRAD) is
+// sigma G(u) / ls from the second table at the same index and fraction
+// (k3_kernel_dls_s), where the value is positive.
 template <bool GRAD>
-__device__ __forceinline__ BgTerm<GRAD> bg_term(float a2, float kscale, float sigma, float ls, float inv_ls,
-                                                const BgTable<GRAD>* stab) {
-    (void)ls;
+__device__ __forceinline__ BgTerm<GRAD> bg_term(float s_raw, float sigma, float inv_ls, const BgTable<GRAD>* stab) {
     BgTerm<GRAD> t;
-    const float s = __builtin_amdgcn_fmed3f(a2 * kscale, 0.0f, (float)K3_KTAB_N);
+    const float s = __builtin_amdgcn_fmed3f(s_raw, 0.0f, (float)K3_KTAB_N);
     const int i = (int)s;
     const float f = __builtin_amdgcn_fractf(s);
     // both entries addressed from 4 i (byte offsets into the two arrays)
@@ -556,11 +560,23 @@ __device__ __forceinline__ void bg_load_table(BgTable<GRAD>* stab) {
 template <bool GRAD>
 constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 
+// Read-only device memory in the constant address space: loads at a uniform
+// address become scalar loads (the point records do not change during the
+// gather).
+typedef const __attribute__((address_space(4))) float bg_const_f;
+// First four words of point record p (ux, uy, uz, mgs).
+__device__ __forceinline__ float4 bg_dir(const bg_const_f* rec, uint32_t p) {
+    const bg_const_f* r = rec + 12u * p;
+    return make_float4(r[0], r[1], r[2], r[3]);
+}
+
 // Gather of one work item (a whole tile, or one 1024-point chunk of a long
-// tile list) by one wavefront: lane = voxel of the 4x4x4 tile.
+// tile list) by one wavefront: lane = voxel of the 4x4x4 tile.  Distances
+// are in table units (K3PointRec): per pair, the table argument is s =
+// |v - m|^2 kscale (occupied) or a free distance of k3_free_s.
 template <bool GRAD>
 __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane, float4* srec,
-                                          const BgTable<GRAD>* stab) {
+                                          uint32_t* smeta, const BgTable<GRAD>* stab) {
     const GridGeom g = a.fc->g;
     const uint4 item = a.items[it];
     const uint32_t tile = uniform(item.x), ib = uniform(item.y), ie = uniform(item.z), slot = uniform(item.w);
@@ -570,120 +586,198 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
     const float vx = axis_center(i, g.gs, g.res, g.sx);
     const float vy = axis_center(j, g.gs, g.res, g.sy);
     const float vz = axis_center(k, g.gsz, g.res, g.sz);
-    const float dc2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
+    const float dc2s = fmaf(vz, vz, fmaf(vy, vy, vx * vx)) * a.kscale;
     // free term of a voxel for every point in front of which it lies
     // (the kernel is an exact zero from distance ls on)
-    const BgTerm<GRAD> kc = bg_term<GRAD>(dc2, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
-    // bounding box of the tile's 64 voxel centres (lanes outside the grid included)
-    const float tcx = 0.5f * (axis_center(tx * 4, g.gs, g.res, g.sx) + axis_center(tx * 4 + 3, g.gs, g.res, g.sx));
-    const float tcy = 0.5f * (axis_center(ty * 4, g.gs, g.res, g.sy) + axis_center(ty * 4 + 3, g.gs, g.res, g.sy));
+    const BgTerm<GRAD> kc = bg_term<GRAD>(dc2s, a.sigma_f, a.inv_ls, stab);
+    // bounding box of the tile's 64 voxel centres (lanes outside the grid
+    // included); uniform values are kept in scalar registers
+    const float tcx =
+        uniform_f(0.5f * (axis_center(tx * 4, g.gs, g.res, g.sx) + axis_center(tx * 4 + 3, g.gs, g.res, g.sx)));
+    const float tcy =
+        uniform_f(0.5f * (axis_center(ty * 4, g.gs, g.res, g.sy) + axis_center(ty * 4 + 3, g.gs, g.res, g.sy)));
     const float tcz =
-        0.5f * (axis_center(tz * 4, g.gsz, g.res, g.sz) + axis_center(tz * 4 + 3, g.gsz, g.res, g.sz));
-    const float trad = 1.5f * g.res * 1.7320508f;
+        uniform_f(0.5f * (axis_center(tz * 4, g.gsz, g.res, g.sz) + axis_center(tz * 4 + 3, g.gsz, g.res, g.sz)));
+    const float trad = uniform_f(1.5f * g.res * 1.7320508f);
     const float marg = 1e-3f + 1e-5f * (fabsf(tcx) + fabsf(tcy) + fabsf(tcz) + trad);
-    const float rr = trad + marg;
-    const float occ_r = a.ls + rr;
+    const float rr = uniform_f(trad + marg);
+    const float occ_r = uniform_f(a.ls + rr);
+    // the same radius along the ray in table units: a lane's projection v . u
+    // lies within rrs of the tile centre's (|u| = sk up to rounding, which
+    // the margin covers)
+    const float rrs = uniform_f(rr * a.sk * (1.0f + 1e-6f));
+    // A candidate whose every term is exactly zero for the whole tile is
+    // dropped before the loop.  Free and occupied distances of a voxel are at
+    // least its distance to the ray's line, and over the tile's box of voxel
+    // centres (half-width hb) that distance is at least
+    //   B = |c_perp| - hb (|n_x| + |n_y| + |n_z|),  n = c_perp / |c_perp|,
+    // c_perp the tile centre's offset from the line.  The lanes' rounded
+    // table arguments are within 16 2^-24 |v|^2 kscale of the exact ones, so
+    // B^2 >= ls^2 (1 + 1e-5) + lerr with lerr = 1e-5 (|c| + trad)^2 + 1e-6
+    // leaves every lane at or beyond the kernel's support.  A behind-sensor
+    // candidate without an occupied term is dropped when no lane of the tile
+    // has a free term (kc is zero in every lane), and one lying wholly beyond
+    // its point likewise.
+    const float hb = uniform_f(1.5f * g.res * (1.0f + 1e-6f));
+    const float tc_n = sqrtf(fmaf(tcz, tcz, fmaf(tcy, tcy, tcx * tcx))) + trad;
+    const float lim2 = uniform_f(fmaf(a.ls * a.ls, 1.0f + 1e-5f, fmaf(1e-5f * tc_n, tc_n, 1e-6f)));
+    const float inv_sk = uniform_f(1.0f / a.sk);
+    // the point records (12 floats each), read with scalar loads
+    const bg_const_f* srow = (const bg_const_f*)(uintptr_t)a.rec;
+    const bool kc_live = __any(kc.v != 0.0f);
     float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
     float gsum_o = 0.0f, gsum_f = 0.0f;  // length-scale derivatives (GRAD)
-    uint32_t cur_chunk = 0xffffffffu;
-    uint32_t meta_l = 0;  // lane l: point index and class of candidate b + l
+    uint32_t last_chunk = 0xffffffffu;  // chunk of the last candidate kept so far
     for (uint32_t b = ib; b < ie; b += 64) {
-        const uint32_t e = b + lane;
-        if (e < ie) {
-            const uint32_t p = a.tlist[e];
-            const K3PointRec r = a.rec[p];
+        const uint32_t cnt = uniform(min(64u, ie - b));
+        bool keep = false;
+        uint32_t meta = 0;
+        float4 q0, q1, q2;
+        if ((uint32_t)lane < cnt) {
+            const uint32_t p = a.tlist[b + lane];
             const float4* src = reinterpret_cast<const float4*>(a.rec + p);
-            srec[lane * 3 + 0] = src[0];
-            srec[lane * 3 + 1] = src[1];
-            srec[lane * 3 + 2] = src[2];
-            const float proj = fmaf(tcz, r.z, fmaf(tcy, r.y, tcx * r.x)) * r.inv_dm;
+            // record: q0 = (ux, uy, uz, mgs), q1 = (x, y, z, dms), q2 = (ex, ey, ez, valid)
+            q0 = src[0];
+            q1 = src[1];
+            q2 = src[2];
+            const float proj = fmaf(tcz, q0.z, fmaf(tcy, q0.y, tcx * q0.x));
             uint32_t cls = C_GENERAL;
-            if (proj + rr <= 0.0f) cls = C_BEHIND;
-            else if (proj - rr > 0.0f && proj + rr < r.mg) cls = C_MAIN;
-            else if (proj - rr >= r.dm) cls = C_NOFREE;
-            const float ox = tcx - r.x, oy = tcy - r.y, oz = tcz - r.z;
-            if (!(ox * ox + oy * oy + oz * oz >= occ_r * occ_r)) cls |= F_OCC;
-            meta_l = (p << 3) | cls;  // point index and class
+            if (proj + rrs <= 0.0f) cls = C_BEHIND;
+            else if (proj - rrs > 0.0f && proj + rrs < q0.w) cls = C_MAIN;
+            else if (proj - rrs >= q1.w) cls = C_NOFREE;
+            const float ox = tcx - q1.x, oy = tcy - q1.y, oz = tcz - q1.z;
+            const bool occ = !(ox * ox + oy * oy + oz * oz >= occ_r * occ_r);
+            if (occ) cls |= F_OCC;
+            // the tile centre's offset from the ray's line (unit direction u / sk)
+            const float pl = proj * inv_sk * inv_sk;
+            const float px = fmaf(-pl, q0.x, tcx);
+            const float py = fmaf(-pl, q0.y, tcy);
+            const float pz = fmaf(-pl, q0.z, tcz);
+            const float dl = sqrtf(fmaf(pz, pz, fmaf(py, py, px * px)));
+            const float bnd = dl - hb * (fabsf(px) + fabsf(py) + fabsf(pz)) / dl;
+            const bool far = dl > 0.0f && bnd > 0.0f && bnd * bnd >= lim2;
+            const bool free_dead = cls == C_NOFREE || (cls == C_BEHIND && !kc_live);
+            keep = !far && (occ || !free_dead);
+            meta = (p << 3) | cls;
+        }
+        // kept candidates, in list order, to the front of the staging area
+        const uint64_t km = __ballot(keep);
+        const uint32_t nk = (uint32_t)__popcll(km);
+        if (keep) {
+            const uint32_t at = bg_lanes_below(km);
+            srec[at * 3 + 0] = q0;
+            srec[at * 3 + 1] = q1;
+            srec[at * 3 + 2] = q2;
+            smeta[at] = meta;
         }
         bg_wave_sync();
-        const uint32_t cnt = uniform(min(64u, ie - b));
-        for (uint32_t q = 0; q < cnt; ++q) {
-            const uint32_t meta = lane_word(meta_l, q);
-            const float4 r0 = srec[q * 3];
-            const uint32_t chunk = meta >> (3 + K3_BGKI_CHUNK_LOG2);
-            const uint32_t cls = meta & 7u;
-            if (chunk != cur_chunk) {
+        // Per-candidate bits as wave masks (bit q = kept candidate q), read on
+        // the scalar unit: a chunk starts at q, q is a pure perpendicular-free
+        // candidate, q has an occupied term, and the two free-class bits.
+        const bool live = (uint32_t)lane < nk;
+        const uint32_t mt = live ? smeta[lane] : 0u;
+        const uint32_t cls = mt & 7u;
+        const uint32_t chunk = mt >> (3 + K3_BGKI_CHUNK_LOG2);
+        uint32_t prev = (uint32_t)__shfl_up((int)chunk, 1);
+        if (lane == 0) prev = last_chunk;
+        const uint64_t m_start = __ballot(live && chunk != prev);
+        const uint64_t m_main = __ballot(live && cls == C_MAIN);
+        const uint64_t m_occ = __ballot(live && (cls & F_OCC) != 0u);
+        const uint64_t m_b0 = __ballot(live && (cls & 1u) != 0u);
+        const uint64_t m_b1 = __ballot(live && (cls & 2u) != 0u);
+        if (nk) last_chunk = lane_word(chunk, nk - 1u);
+        const uint32_t cnt_k = uniform(nk);
+        uint32_t q = 0;
+        while (q < cnt_k) {
+            if ((m_start >> q) & 1ull) {
                 tot_o += cur_o;
                 tot_f += cur_f;
                 cur_o = 0.0f;
                 cur_f = 0.0f;
-                cur_chunk = chunk;
             }
-            if (cls == C_MAIN) {
-                // the common case: free term only, perpendicular distance for
-                // every lane; two such candidates of one chunk are evaluated
-                // together (independent table reads), then added in order
-                const float dpa = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
-                const float a2a = fmaf(-dpa, dpa, dc2);  // clamped at 0 by bg_term
-                const uint32_t meta2 = q + 1 < cnt ? lane_word(meta_l, q + 1) : 0u;
-                if (q + 1 < cnt && (meta2 & 7u) == C_MAIN && (meta2 >> (3 + K3_BGKI_CHUNK_LOG2)) == chunk) {
-                    const float4 r0b = srec[(q + 1) * 3];
-                    const float dpb = fmaf(vz, r0b.z, fmaf(vy, r0b.y, vx * r0b.x)) * r0b.w;
-                    const float a2b = fmaf(-dpb, dpb, dc2);
-                    const BgTerm<GRAD> ta = bg_term<GRAD>(a2a, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
-                    const BgTerm<GRAD> tb = bg_term<GRAD>(a2b, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
-                    cur_f += ta.v;
-                    cur_f += tb.v;
+            if ((m_main >> q) & 1ull) {
+                // A run of candidates whose free term is the perpendicular
+                // distance for every lane and that have no occupied term, up
+                // to the next other candidate or chunk start: two terms are
+                // evaluated together (independent table reads), then added
+                // in order.  Only the direction u is read.
+                const uint64_t stop = q < 63u ? ((~m_main | m_start) >> (q + 1u)) : 0ull;
+                const uint32_t r = min(cnt_k, stop ? q + 1u + (uint32_t)__builtin_ctzll(stop) : 64u);
+                // The directions come through the scalar cache (the records
+                // are read-only during the launch), two pairs ahead: the only
+                // LDS reads of the run are the table lookups.
+                const uint32_t rl = r - 1u;
+                float4 u0 = bg_dir(srow, lane_word(mt, q) >> 3);
+                float4 u1 = bg_dir(srow, lane_word(mt, min(q + 1u, rl)) >> 3);
+                float4 u2 = bg_dir(srow, lane_word(mt, min(q + 2u, rl)) >> 3);
+                float4 u3 = bg_dir(srow, lane_word(mt, min(q + 3u, rl)) >> 3);
+                for (; q + 2u <= r; q += 2u) {
+                    const float4 n0 = bg_dir(srow, lane_word(mt, min(q + 4u, rl)) >> 3);
+                    const float4 n1 = bg_dir(srow, lane_word(mt, min(q + 5u, rl)) >> 3);
+                    const float d0 = fmaf(vz, u0.z, fmaf(vy, u0.y, vx * u0.x));
+                    const float d1 = fmaf(vz, u1.z, fmaf(vy, u1.y, vx * u1.x));
+                    // squared distances clamped at 0 by bg_term
+                    const BgTerm<GRAD> t0 = bg_term<GRAD>(fmaf(-d0, d0, dc2s), a.sigma_f, a.inv_ls, stab);
+                    const BgTerm<GRAD> t1 = bg_term<GRAD>(fmaf(-d1, d1, dc2s), a.sigma_f, a.inv_ls, stab);
+                    cur_f += t0.v;
+                    cur_f += t1.v;
                     if (GRAD) {
-                        gsum_f += ta.g;
-                        gsum_f += tb.g;
+                        gsum_f += t0.g;
+                        gsum_f += t1.g;
                     }
+                    u0 = u2;
+                    u1 = u3;
+                    u2 = n0;
+                    u3 = n1;
+                }
+                if (q < r) {
+                    const float d0 = fmaf(vz, u0.z, fmaf(vy, u0.y, vx * u0.x));
+                    const BgTerm<GRAD> t0 = bg_term<GRAD>(fmaf(-d0, d0, dc2s), a.sigma_f, a.inv_ls, stab);
+                    cur_f += t0.v;
+                    if (GRAD) gsum_f += t0.g;
                     ++q;
-                } else {
-                    const BgTerm<GRAD> ta = bg_term<GRAD>(a2a, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
-                    cur_f += ta.v;
-                    if (GRAD) gsum_f += ta.g;
                 }
                 continue;
             }
-            // record: r0 = (x, y, z, inv_dm), r1 = (dm, mg, ex, ey), r2 = (ez, valid, -, -)
-            if (cls & F_OCC) {  // occupied term (k3_occ_distance)
-                const float ox = vx - r0.x, oy = vy - r0.y, oz = vz - r0.z;
-                const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox));
-                const bool co = so < a.ls2g;
+            // record: r0 = (ux, uy, uz, mgs), r1 = (x, y, z, dms), r2 = (ex, ey, ez, valid)
+            const float4 r0 = srec[q * 3];
+            const float4 r1 = srec[q * 3 + 1];
+            if ((m_occ >> q) & 1ull) {  // occupied term (k3_occ_s)
+                const float ox = vx - r1.x, oy = vy - r1.y, oz = vz - r1.z;
+                const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox)) * a.kscale;
+                const bool co = so < (float)K3_KTAB_N;
                 if (__any(co)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(so, a.kscale, a.sigma_o, a.ls, a.inv_ls, stab);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(so, a.sigma_o, a.inv_ls, stab);
                     cur_o += t.v;
                     if (GRAD) gsum_o += t.g;
                 }
             }
-            const uint32_t fc = cls & 3u;
-            if (fc == C_MAIN) {  // 0 < dp < mg for every lane
-                const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
-                const float a2 = fmaf(-dp, dp, dc2);  // clamped at 0 by bg_term
-                const bool cf = a2 < a.ls2g;
+            const uint32_t fc = (uint32_t)((m_b0 >> q) & 1ull) | ((uint32_t)((m_b1 >> q) & 1ull) << 1);
+            if (fc == C_MAIN) {  // 0 < dp < mgs for every lane
+                const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x));
+                const float sf = fmaf(-dp, dp, dc2s);  // clamped at 0 by bg_term
+                const bool cf = sf < (float)K3_KTAB_N;
                 if (__any(cf)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(a2, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(sf, a.sigma_f, a.inv_ls, stab);
                     cur_f += t.v;
                     if (GRAD) gsum_f += t.g;
                 }
             } else if (fc == C_BEHIND) {  // dp <= 0 for every lane
                 cur_f += kc.v;
                 if (GRAD) gsum_f += kc.g;
-            } else if (fc == C_GENERAL) {  // k3_free_distance, the three cases as selects
-                const float4 r1 = srec[q * 3 + 1];
+            } else if (fc == C_GENERAL) {  // k3_free_s, the three cases as selects
                 const float4 r2 = srec[q * 3 + 2];
-                const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x)) * r0.w;
+                const float dp = fmaf(vz, r0.z, fmaf(vy, r0.y, vx * r0.x));
                 const bool c1 = dp <= 0.0f;
-                const bool c2 = !c1 && dp < r1.y;
-                const bool c3 = !c1 && !c2 && dp < r1.x;
-                const float a2 = fmaf(-dp, dp, dc2);  // clamped at 0 by bg_term
-                const float ex = vx - r1.z, ey = vy - r1.w, ez = vz - r2.x;
-                const float a3 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-                const float aa = c2 ? a2 : a3;
-                const bool cs = (c2 || c3) && aa < a.ls2g;
+                const bool c2 = !c1 && dp < r0.w;
+                const bool c3 = !c1 && !c2 && dp < r1.w;
+                const float sp = fmaf(-dp, dp, dc2s);  // clamped at 0 by bg_term
+                const float ex = vx - r2.x, ey = vy - r2.y, ez = vz - r2.z;
+                const float se = fmaf(ez, ez, fmaf(ey, ey, ex * ex)) * a.kscale;
+                const float sa = c2 ? sp : se;
+                const bool cs = (c2 || c3) && sa < (float)K3_KTAB_N;
                 if (__any(cs)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(aa, a.kscale, a.sigma_f, a.ls, a.inv_ls, stab);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(sa, a.sigma_f, a.inv_ls, stab);
                     cur_f += cs ? t.v : (c1 ? kc.v : 0.0f);
                     if (GRAD) gsum_f += cs ? t.g : (c1 ? kc.g : 0.0f);
                 } else {
@@ -691,6 +785,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
                     if (GRAD) gsum_f += c1 ? kc.g : 0.0f;
                 }
             }
+            ++q;
         }
         bg_wave_sync();
     }
@@ -717,9 +812,39 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
     }
 }
 
+// Work distribution of the gather.  One work counter in device memory that
+// every wavefront increments is slow (device-scope atomics from all XCDs on
+// one address are serialised), and a static deal leaves wavefronts idle
+// (items range from a few to 1024 candidates).  So the items are spread over
+// BG_NQ queues, item i being position i / BG_NQ of queue i % BG_NQ, each with
+// its own head (one 128-byte line each).  A wavefront claims the items of its
+// home queue one at a time with one atomic each; once the home queue is
+// empty it moves on to the following queues, reading a queue's head before
+// claiming from it so that an emptied queue costs no atomic.  The heads are
+// zeroed by the gather's launcher (k_reset_queues) before every launch.
+__device__ __forceinline__ uint32_t bg_queue_len(uint32_t n_items, uint32_t q) {
+    return n_items > q ? (n_items - q + BG_NQ - 1u) / BG_NQ : 0u;
+}
+// Claims a position of queue q (home: claim without looking first); returns
+// len (or more) when the queue is empty.
+__device__ __forceinline__ uint32_t bg_claim(uint32_t* qheads, uint32_t q, uint32_t len, bool home, int lane) {
+    uint32_t pos = len;
+    if (lane == 0) {
+        uint32_t* head = qheads + q * BG_QSTRIDE;
+        if (home || __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < len)
+            pos = atomicAdd(head, 1u);
+    }
+    return uniform(__shfl(pos, 0));
+}
+
+__global__ void k_reset_queues(uint32_t* qheads) {
+    if (threadIdx.x < BG_NQ) qheads[threadIdx.x * BG_QSTRIDE] = 0u;
+}
+
 template <bool GRAD, int W>
 __global__ __launch_bounds__(W * 64) void k_bgki(BgArgs a) {
     __shared__ float4 srec[W][64 * 3];
+    __shared__ uint32_t smeta[W][64];
     __shared__ BgTable<GRAD> stab_s;
     BgTable<GRAD>* stab = &stab_s;
     DevScalars* sc = a.sc;
@@ -727,12 +852,14 @@ __global__ __launch_bounds__(W * 64) void k_bgki(BgArgs a) {
     bg_load_table<GRAD>(stab);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n_items = sc->n_light + sc->n_heavy;
-    while (true) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(&sc->item_counter, 1u);
-        it = uniform(__shfl(it, 0));
-        if (it >= n_items) break;
-        bgki_item<GRAD>(a, it, lane, srec[w], stab);
+    const uint32_t home = (blockIdx.x * W + (uint32_t)w) % BG_NQ;
+    for (uint32_t k = 0; k < BG_NQ; ++k) {
+        const uint32_t q = (home + k) % BG_NQ, len = bg_queue_len(n_items, q);
+        while (true) {
+            const uint32_t pos = bg_claim(a.qheads, q, len, k == 0, lane);
+            if (pos >= len) break;
+            bgki_item<GRAD>(a, q + pos * BG_NQ, lane, srec[w], smeta[w], stab);
+        }
     }
 }
 
@@ -743,12 +870,13 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
     a.fc = fc;
     a.tg = tg;
     a.ls = ls;
-    a.ls2g = k3_ls2_guard(ls);
     a.kscale = k3_kernel_scale(ls);
+    a.sk = k3_kernel_sk(ls);
     a.inv_ls = 1.0f / ls;
     a.sigma_o = sigma_o;
     a.sigma_f = sigma_f;
     a.items = I.items;
+    a.qheads = I.qheads;
     a.tlist = T.list;
     a.rec = rec;
     a.del_occ = del_occ;
@@ -762,6 +890,7 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
 
 template <bool GRAD>
 static hipError_t launch_bgki_w(hipStream_t st, int grid_blocks, int waves, const BgArgs& a) {
+    hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(BG_NQ), 0, st, a.qheads);
     if (waves == 4) hipLaunchKernelGGL((k_bgki<GRAD, 4>), dim3(grid_blocks), dim3(256), 0, st, a);
     else if (waves == 16) hipLaunchKernelGGL((k_bgki<GRAD, 16>), dim3(grid_blocks), dim3(1024), 0, st, a);
     else hipLaunchKernelGGL((k_bgki<GRAD, 8>), dim3(grid_blocks), dim3(512), 0, st, a);
@@ -785,11 +914,15 @@ hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* 
 }
 
 // ---------------------------------------------------------------- many maps
-// Work items of every map of a batch in one queue: prefix[m] is the first
-// item of map m (a map whose lists overflowed contributes none).
+// Work items of every map of a batch in one joint numbering: prefix[m] is the
+// first item of map m (a map whose lists overflowed contributes none).  The
+// joint items are spread over the work queues as in the single-map gather;
+// their heads (qheads, BG_QWORDS words) are zeroed here.
 __global__ void k_bgki_batch_prefix(const BgArgs* __restrict__ maps, int nmaps, uint32_t* __restrict__ prefix,
-                                    uint32_t* __restrict__ counter) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+                                    uint32_t* __restrict__ qheads) {
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x < BG_NQ) qheads[threadIdx.x * BG_QSTRIDE] = 0u;
+    if (threadIdx.x != 0) return;
     uint32_t acc = 0;
     for (int m = 0; m < nmaps; ++m) {
         prefix[m] = acc;
@@ -797,44 +930,47 @@ __global__ void k_bgki_batch_prefix(const BgArgs* __restrict__ maps, int nmaps, 
         if (!sc->overflow) acc += sc->n_light + sc->n_heavy;
     }
     prefix[nmaps] = acc;
-    *counter = 0;
 }
 
 // One persistent launch over the items of all maps.
 template <int W>
 __global__ __launch_bounds__(W * 64) void k_bgki_batch(const BgArgs* __restrict__ maps, int nmaps,
-                                                       const uint32_t* __restrict__ prefix, uint32_t* counter) {
+                                                       const uint32_t* __restrict__ prefix, uint32_t* qheads) {
     __shared__ float4 srec[W][64 * 3];
+    __shared__ uint32_t smeta[W][64];
     __shared__ BgTable<false> stab_s;
     BgTable<false>* stab = &stab_s;
     bg_load_table<false>(stab);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t total = prefix[nmaps];
-    while (true) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd(counter, 1u);
-        it = uniform(__shfl(it, 0));
-        if (it >= total) break;
-        int lo = 0, hi = nmaps - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (prefix[mid] <= it) lo = mid;
-            else hi = mid - 1;
+    const uint32_t home = (blockIdx.x * W + (uint32_t)w) % BG_NQ;
+    for (uint32_t k = 0; k < BG_NQ; ++k) {
+        const uint32_t q = (home + k) % BG_NQ, len = bg_queue_len(total, q);
+        while (true) {
+            const uint32_t pos = bg_claim(qheads, q, len, k == 0, lane);
+            if (pos >= len) break;
+            const uint32_t it = q + pos * BG_NQ;
+            int lo = 0, hi = nmaps - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (prefix[mid] <= it) lo = mid;
+                else hi = mid - 1;
+            }
+            const BgArgs a = maps[lo];
+            bgki_item<false>(a, it - prefix[lo], lane, srec[w], smeta[w], stab);
         }
-        const BgArgs a = maps[lo];
-        bgki_item<false>(a, it - prefix[lo], lane, srec[w], stab);
     }
 }
 
 hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps, int nmaps, uint32_t* prefix,
-                             uint32_t* counter, int waves) {
-    hipLaunchKernelGGL(k_bgki_batch_prefix, dim3(1), dim3(64), 0, st, maps, nmaps, prefix, counter);
+                             uint32_t* qheads, int waves) {
+    hipLaunchKernelGGL(k_bgki_batch_prefix, dim3(1), dim3(BG_NQ), 0, st, maps, nmaps, prefix, qheads);
     if (waves == 4)
-        hipLaunchKernelGGL(k_bgki_batch<4>, dim3(grid_blocks), dim3(256), 0, st, maps, nmaps, prefix, counter);
+        hipLaunchKernelGGL(k_bgki_batch<4>, dim3(grid_blocks), dim3(256), 0, st, maps, nmaps, prefix, qheads);
     else if (waves == 16)
-        hipLaunchKernelGGL(k_bgki_batch<16>, dim3(grid_blocks), dim3(1024), 0, st, maps, nmaps, prefix, counter);
+        hipLaunchKernelGGL(k_bgki_batch<16>, dim3(grid_blocks), dim3(1024), 0, st, maps, nmaps, prefix, qheads);
     else
-        hipLaunchKernelGGL(k_bgki_batch<8>, dim3(grid_blocks), dim3(512), 0, st, maps, nmaps, prefix, counter);
+        hipLaunchKernelGGL(k_bgki_batch<8>, dim3(grid_blocks), dim3(512), 0, st, maps, nmaps, prefix, qheads);
     return hipGetLastError();
 }
 
@@ -861,16 +997,6 @@ __global__ void k_merge_group_scalars(DevScalars* sc, const DevScalars* __restri
 }
 hipError_t launch_merge_group_scalars(hipStream_t st, DevScalars* sc, const DevScalars* gsc, int ng) {
     hipLaunchKernelGGL(k_merge_group_scalars, dim3(1), dim3(64), 0, st, sc, gsc, ng);
-    return hipGetLastError();
-}
-
-// Resets the work-queue head so that the gather can be launched again over
-// the same items (launch-shape tuning).
-__global__ void k_reset_item_counter(DevScalars* sc) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) sc->item_counter = 0;
-}
-hipError_t launch_reset_item_counter(hipStream_t st, DevScalars* sc) {
-    hipLaunchKernelGGL(k_reset_item_counter, dim3(1), dim3(64), 0, st, sc);
     return hipGetLastError();
 }
 
@@ -958,11 +1084,11 @@ __global__ __launch_bounds__(TPB) void k_bgki_count(GridGeom g, TileGeom tg, flo
         const float vx = axis_center(i, g.gs, g.res, g.sx);
         const float vy = axis_center(j, g.gs, g.res, g.sy);
         const float vz = axis_center(k, g.gsz, g.res, g.sz);
-        const float dc2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
+        const float dc2s = fmaf(vz, vz, fmaf(vy, vy, vx * vx)) * kscale;
         for (uint32_t e = item.y; e < item.z; ++e) {
             const K3PointRec r = rec[tlist[e]];
-            co += k3_occ_dist2(vx, vy, vz, r, kscale) >= 0.0f;
-            cf += k3_free_dist2(vx, vy, vz, dc2, r, kscale) >= 0.0f;
+            co += k3_occ_s(vx, vy, vz, r, kscale) >= 0.0f;
+            cf += k3_free_s(vx, vy, vz, dc2s, r, kscale) >= 0.0f;
         }
     }
     for (int o = 32; o > 0; o >>= 1) {

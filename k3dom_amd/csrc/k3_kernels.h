@@ -14,7 +14,6 @@ struct DevScalars {
     int32_t E_born, E_joint;  // fixed-point exponents
     uint64_t total_born;      // u64 fixed-point totals
     uint64_t total_joint;
-    uint32_t item_counter;    // kernel-inference work queue head
     uint32_t n_light;         // items of tiles handled whole
     uint32_t n_heavy;         // items of tiles split into point chunks
     uint32_t overflow;        // non-zero: a capacity was exceeded, the frame tail is skipped
@@ -153,7 +152,7 @@ constexpr int MAX_CHILDREN = 1024;
 
 // m_cap: capacity of the point arrays (the launch covers it; fc->m points are live)
 hipError_t launch_point_prep(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
-                             const float* zs, int m_cap, float res, K3PointRec* rec, float4* pxyz);
+                             const float* zs, int m_cap, float res, float ls, K3PointRec* rec, float4* pxyz);
 // Blocks share a box's children in groups; masks need level_child_groups(L)
 // words per parent-list position.
 int level_child_groups(const LevelDesc& L);
@@ -161,12 +160,18 @@ hipError_t launch_build_level(hipStream_t st, int level, const FrameConsts* fc, 
                               const LevelBufs& B, const LevelBufs* parent, float ls, const float4* pxyz,
                               DevScalars* sc);
 
+// Work queues of the gather: BG_NQ heads, one per BG_QSTRIDE words (128 bytes).
+constexpr uint32_t BG_NQ = 64;
+constexpr uint32_t BG_QSTRIDE = 32;
+constexpr size_t BG_QWORDS = (size_t)BG_NQ * BG_QSTRIDE;
+
 struct ItemBufs {
     uint32_t *heavy_cnt, *light_cnt, *heavy_off, *light_off;  // ntiles (+1)
     uint32_t* bsum;
     uint4* items;             // ntiles + heavy_cap
     uint32_t* tile_partial;   // 2 * ntiles
     float* partial;           // heavy_cap * partial_stride
+    uint32_t* qheads;         // BG_QWORDS: the gather's work-queue heads
     int partial_stride = 128; // floats per split item (256 with length-scale derivatives)
     long long heavy_cap;
 };
@@ -176,8 +181,9 @@ hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& tiles
 struct BgArgs {
     const FrameConsts* fc;
     TileGeom tg;
-    float ls, ls2g, kscale, inv_ls, sigma_o, sigma_f;
+    float ls, kscale, sk, inv_ls, sigma_o, sigma_f;
     const uint4* items;
+    uint32_t* qheads;  // the work-queue heads of the single-map gather
     const uint32_t* tlist;
     const K3PointRec* rec;
     float *del_occ, *del_free, *dls_occ, *dls_free, *partial;
@@ -187,10 +193,10 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
                     const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec, float* del_occ, float* del_free,
                     float* dls_occ, float* dls_free, DevScalars* sc);
 // The gather over the items of several maps in one persistent launch
-// (maps: device array of nmaps argument sets; prefix: nmaps + 1 entries).
+// (maps: device array of nmaps argument sets; prefix: nmaps + 1 entries;
+// qheads: BG_QWORDS words of work-queue heads, zeroed by the launch).
 hipError_t launch_bgki_batch(hipStream_t st, int grid_blocks, const BgArgs* maps, int nmaps, uint32_t* prefix,
-                             uint32_t* counter, int waves = 8);
-hipError_t launch_reset_item_counter(hipStream_t st, DevScalars* sc);
+                             uint32_t* qheads, int waves = 8);
 // Group 0's scalars after a joint gather of ng LiDAR groups (gsc[1 .. ng - 1]):
 // overflow flags and bits, largest per-level needs, summed items.
 hipError_t launch_merge_group_scalars(hipStream_t st, DevScalars* sc, const DevScalars* gsc, int ng);

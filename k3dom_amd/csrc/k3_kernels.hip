@@ -1666,13 +1666,12 @@ hipError_t launch_reset_scalars(hipStream_t st, DevScalars* sc) {
     return hipGetLastError();
 }
 
-// Clears the overflow state and the work-queue head before a re-run of the
+// Clears the overflow state before a re-run of the
 // kernel-inference stage (the other scalars are untouched by the aborted tail).
 __global__ void k_clear_overflow(DevScalars* sc) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         sc->overflow = 0;
         sc->overflow_mask = 0;
-        sc->item_counter = 0;
     }
 }
 hipError_t launch_clear_overflow(hipStream_t st, DevScalars* sc) {

@@ -204,26 +204,35 @@ K3_HD void k3_box_muller(uint32_t a, uint32_t b, float* n0, float* n1) {
 K3_HD float k3_kernel_scale(float ls) {
     return (float)((double)K3_KTAB_N / ((double)ls * (double)ls));
 }
+// Its square root, sqrt(N) / ls: lengths multiplied by it are in table units
+// (the point record's direction and segment bounds).
+K3_HD float k3_kernel_sk(float ls) {
+    return (float)(sqrt((double)K3_KTAB_N) / (double)ls);
+}
 
 // One table entry (a, b, c): a + f (b + f c), f in [0, 1).  Entries at
-// i >= K3_KTAB_N are zero, so a2 * scale >= N (d >= ls) gives exactly 0.
+// i >= K3_KTAB_N are zero, so s >= N (d >= ls) gives exactly 0.
 K3_HD float k3_kernel_poly_raw(float a, float b, float c, float f) { return fmaf(f, fmaf(f, c, b), a); }
 K3_HD float k3_kernel_poly(float a, float b, float c, float f, float sigma) {
     const float v = k3_kernel_poly_raw(a, b, c, f);
     return fmaxf(0.0f, sigma * v);
 }
-K3_HD int k3_kernel_index(float a2, float scale, float* f) {
-    const float s = fminf(a2 * scale, (float)K3_KTAB_N);
-    const int i = (int)s;
-    *f = s - (float)i;
+// Table index and fraction of the argument s (clamped to [0, N]; a rounded
+// difference of squares may be slightly negative and stands for 0).
+K3_HD int k3_kernel_index_s(float s, float* f) {
+    const float c = fminf(fmaxf(s, 0.0f), (float)K3_KTAB_N);
+    const int i = (int)c;
+    *f = c - (float)i;
     return i;
 }
-// k(sqrt(a2)) for a2 >= 0 with scale = k3_kernel_scale(ls).
-K3_HD float k3_kernel_value_sq(float a2, float scale, float sigma) {
+// k(d) for the table argument s = d^2 * k3_kernel_scale(ls).
+K3_HD float k3_kernel_value_s(float s, float sigma) {
     float f;
-    const int i = k3_kernel_index(a2, scale, &f);
+    const int i = k3_kernel_index_s(s, &f);
     return k3_kernel_poly(k3_ktab[4 * i], k3_ktab[4 * i + 1], k3_ktab[4 * i + 2], f, sigma);
 }
+// k(sqrt(a2)) for a2 >= 0 with scale = k3_kernel_scale(ls).
+K3_HD float k3_kernel_value_sq(float a2, float scale, float sigma) { return k3_kernel_value_s(a2 * scale, sigma); }
 
 // Derivative in ls of sigma * kbar(d / ls), kbar the bracket of the
 // reference's kernel (src/k3dom/kernel/mass_update.cu:24-26) differentiated in
@@ -241,14 +250,17 @@ K3_HD float k3_kernel_dls(float d, float ls, float inv_ls, float sigma) {
     return sigma * dk * (-q * inv_ls);
 }
 
-// The same derivative from the second table: sigma G(a2 scale / N) / ls with
+// The same derivative from the second table: sigma G(s / N) / ls with
 // G(u) = -q kbar'(q), q = sqrt(u), where the kernel value is positive (the
 // gather's derivative evaluation; no square root or sine per pair).
-K3_HD float k3_kernel_dls_sq(float a2, float scale, float inv_ls, float sigma) {
+K3_HD float k3_kernel_dls_s(float s, float inv_ls, float sigma) {
     float f;
-    const int i = k3_kernel_index(fmaxf(a2, 0.0f), scale, &f);
+    const int i = k3_kernel_index_s(s, &f);
     if (!(k3_kernel_poly(k3_ktab[4 * i], k3_ktab[4 * i + 1], k3_ktab[4 * i + 2], f, sigma) > 0.0f)) return 0.0f;
     return sigma * k3_kernel_poly_raw(k3_ktab_dls[4 * i], k3_ktab_dls[4 * i + 1], k3_ktab_dls[4 * i + 2], f) * inv_ls;
+}
+K3_HD float k3_kernel_dls_sq(float a2, float scale, float inv_ls, float sigma) {
+    return k3_kernel_dls_s(a2 * scale, inv_ls, sigma);
 }
 
 // distance_with_margin, src/k3dom/kernel/mass_update.cu:29-31
@@ -259,66 +271,66 @@ K3_HD float k3_margin(float distance, float resolution) {
 // Per-point values of one measurement that every voxel reuses (one 48-byte
 // record per point).  p = (x, y, z) is the point relative to the sensor in
 // map-aligned axes; dm = |p|; mg = margin(dm); e = (mg/dm) p is the end of the
-// free segment; inv_dm = 1/dm (NaN when dm == 0, which disables the free term
-// exactly as the reference's 0/0 does).
+// free segment.  Distances along the ray are kept in table units: with
+// sk = k3_kernel_sk(ls), u = p (sk / dm) is the ray's direction scaled by sk,
+// mgs = mg sk and dms = dm sk, so that for a voxel v the projection v . u and
+// the squared perpendicular distance |v|^2 sk^2 - (v . u)^2 come out in the
+// units of the table argument.  u is NaN when dm == 0, which disables the
+// free term exactly as the reference's 0/0 does.
 struct K3PointRec {
-    float x, y, z, inv_dm;
-    float dm, mg, ex, ey;
-    float ez, valid, pad0, pad1;
+    float ux, uy, uz, mgs;
+    float x, y, z, dms;
+    float ex, ey, ez, valid;
 };
 
 // valid = 0 for points with a non-finite coordinate or |p|^2 overflow: such a
 // point contributes nothing.
-K3_HD K3PointRec k3_point_prep(float x, float y, float z, float resolution) {
+K3_HD K3PointRec k3_point_prep(float x, float y, float z, float resolution, float sk) {
     K3PointRec r;
     float ss = fmaf(z, z, fmaf(y, y, x * x));
     bool ok = (x - x == 0.0f) && (y - y == 0.0f) && (z - z == 0.0f) && (ss - ss == 0.0f);
     float dm = sqrtf(ss);
     float mg = k3_margin(dm, resolution);
     float resize = mg / dm;
+    float us = (dm > 0.0f) ? sk / dm : k3_u2f(0x7fc00000u);
+    r.ux = x * us; r.uy = y * us; r.uz = z * us;
+    r.mgs = mg * sk;
     r.x = x; r.y = y; r.z = z;
-    r.inv_dm = (dm > 0.0f) ? 1.0f / dm : k3_u2f(0x7fc00000u);
-    r.dm = dm; r.mg = mg;
+    r.dms = dm * sk;
     r.ex = resize * x; r.ey = resize * y; r.ez = resize * z;
     r.valid = ok ? 1.0f : 0.0f;
-    r.pad0 = 0.0f; r.pad1 = 0.0f;
     return r;
 }
 
-// Conservative bound: s >= ls2g implies s * k3_kernel_scale(ls) >= K3_KTAB_N,
-// i.e. a zero kernel term.
-K3_HD float k3_ls2_guard(float ls) {
-    return (float)((double)ls * (double)ls * (1.0 + 0x1p-20));
-}
-
-// Squared occupied-evidence distance of BGKI (src/k3dom/kernel/mass_update.cu:49):
-// |v - m|^2 when its kernel term can be non-zero (a2 * scale < N), else -1.
-K3_HD float k3_occ_dist2(float vx, float vy, float vz, const K3PointRec& p, float scale) {
+// Table argument of the occupied evidence of BGKI (src/k3dom/kernel/mass_update.cu:49),
+// s = |v - m|^2 * scale, when its kernel term can be non-zero (s < N), else -1.
+K3_HD float k3_occ_s(float vx, float vy, float vz, const K3PointRec& p, float scale) {
     float dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
-    float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    return (s * scale < (float)K3_KTAB_N) ? s : -1.0f;
+    float s = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) * scale;
+    return (s < (float)K3_KTAB_N) ? s : -1.0f;
 }
 
-// Squared free-evidence distance of BGKI (src/k3dom/kernel/mass_update.cu:54-68):
-// squared distance from the voxel centre to the free segment [0, e] of the
-// ray, with the region beyond the measured point excluded; -1 when there is
-// no free term or its kernel term is zero.  dc2 = |v|^2.  The perpendicular
-// distance squared is clamped at 0 (the reference can take sqrtf of a
-// rounding-negative value and drop the term).
-K3_HD float k3_free_dist2(float vx, float vy, float vz, float dc2, const K3PointRec& p, float scale) {
-    float dp = fmaf(vz, p.z, fmaf(vy, p.y, vx * p.x)) * p.inv_dm;
+// Table argument of the free evidence of BGKI (src/k3dom/kernel/mass_update.cu:54-68):
+// the squared distance from the voxel centre to the free segment [0, e] of
+// the ray in table units, with the region beyond the measured point
+// excluded; -1 when there is no free term or its kernel term is zero.
+// dc2s = |v|^2 * scale.  The perpendicular distance squared is clamped at 0
+// (the reference can take sqrtf of a rounding-negative value and drop the
+// term).
+K3_HD float k3_free_s(float vx, float vy, float vz, float dc2s, const K3PointRec& p, float scale) {
+    float dp = fmaf(vz, p.uz, fmaf(vy, p.uy, vx * p.ux));
     float a;
     if (dp <= 0.0f) {
-        a = dc2;
-    } else if (dp < p.mg) {
-        a = fmaxf(0.0f, fmaf(-dp, dp, dc2));
-    } else if (dp < p.dm) {
+        a = dc2s;
+    } else if (dp < p.mgs) {
+        a = fmaxf(0.0f, fmaf(-dp, dp, dc2s));
+    } else if (dp < p.dms) {
         float dx = vx - p.ex, dy = vy - p.ey, dz = vz - p.ez;
-        a = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        a = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) * scale;
     } else {
         return -1.0f;
     }
-    return (a * scale < (float)K3_KTAB_N) ? a : -1.0f;
+    return (a < (float)K3_KTAB_N) ? a : -1.0f;
 }
 
 // Voxel centre relative to the sensor, src/k3dom/kernel/mass_update.cu:92-99.

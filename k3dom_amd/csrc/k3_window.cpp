@@ -357,7 +357,6 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
                 const FrameConsts* gfc = w->d_gfc + g0 + g;
                 const float* gx = px + at;
                 if ((s = from_handle(w, build_lists(h, gfc, gx, gx + fi.n, gx + 2 * fi.n))) != K3DOM_OK) return s;
-                if (g > 0) W_HIP(w, launch_reset_item_counter(h->stream, h->sc));
                 float* to = g == 0 ? w->del_o : w->g_o;
                 float* tf = g == 0 ? w->del_f : w->g_f;
                 float* tlo = g == 0 ? w->dls_o : w->g_lo;

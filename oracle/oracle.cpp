@@ -211,21 +211,19 @@ static float seg_sum(const float* x, int st, int en) {
     return s;
 }
 
-static void bgki_pair(const Geom& g, int i, int j, int k, const K3PointRec& pr, float ls, float ls2g,
-                      float kscale, float sigma, int32_t chunk, ChunkAcc& acc, BgkiStats& st) {
-    (void)ls;
-    (void)ls2g;
+static void bgki_pair(const Geom& g, int i, int j, int k, const K3PointRec& pr, float kscale, float sigma,
+                      int32_t chunk, ChunkAcc& acc, BgkiStats& st) {
     float vx = g.cx[i], vy = g.cy[j], vz = g.cz[k];
     size_t v = (size_t)i + (size_t)g.gs * j + (size_t)g.gs * g.gs * k;
-    float dc2 = fmaf(vz, vz, fmaf(vy, vy, vx * vx));
-    float a_o = k3_occ_dist2(vx, vy, vz, pr, kscale);
-    if (a_o >= 0.0f) {
-        acc.add(acc.tot_o, acc.cur_o, acc.chunk_o, v, chunk, k3_kernel_value_sq(a_o, kscale, sigma));
+    float dc2s = fmaf(vz, vz, fmaf(vy, vy, vx * vx)) * kscale;
+    float s_o = k3_occ_s(vx, vy, vz, pr, kscale);
+    if (s_o >= 0.0f) {
+        acc.add(acc.tot_o, acc.cur_o, acc.chunk_o, v, chunk, k3_kernel_value_s(s_o, sigma));
         st.occ_pairs++;
     }
-    float a_f = k3_free_dist2(vx, vy, vz, dc2, pr, kscale);
-    if (a_f >= 0.0f) {
-        acc.add(acc.tot_f, acc.cur_f, acc.chunk_f, v, chunk, k3_kernel_value_sq(a_f, kscale, sigma));
+    float s_f = k3_free_s(vx, vy, vz, dc2s, pr, kscale);
+    if (s_f >= 0.0f) {
+        acc.add(acc.tot_f, acc.cur_f, acc.chunk_f, v, chunk, k3_kernel_value_s(s_f, sigma));
         st.free_pairs++;
     }
 }
@@ -260,11 +258,11 @@ static int g_bgki_threads = 1;
 // slab, and the sums do not depend on how the x axis is split.
 static void bgki_culled_slab(const Geom& g, const float* xs, const float* ys, const float* zs, int n, float ls,
                              float sigma, int xlo, int xhi, ChunkAcc& acc, BgkiStats& st) {
-    const float ls2g = k3_ls2_guard(ls);
     const float kscale = k3_kernel_scale(ls);
+    const float sk = k3_kernel_sk(ls);
     const double R = (double)ls * (1.0 + 1e-5) + 1e-4;
     for (int p = 0; p < n; ++p) {
-        K3PointRec pr = k3_point_prep(xs[p], ys[p], zs[p], g.res);
+        K3PointRec pr = k3_point_prep(xs[p], ys[p], zs[p], g.res, sk);
         if (pr.valid == 0.0f) continue;
         int32_t chunk = p >> K3_BGKI_CHUNK_LOG2;
         double mx = pr.x, my = pr.y, mz = pr.z;
@@ -285,7 +283,7 @@ static void bgki_culled_slab(const Geom& g, const float* xs, const float* ys, co
                 double zlo = std::min(tc * mz, td * mz) - R, zhi = std::max(tc * mz, td * mz) + R;
                 if (!axis_range(zlo, zhi, g.res, g.sz, g.gsz, &k0, &k1)) continue;
                 for (int k = k0; k <= k1; ++k)
-                    bgki_pair(g, i, j, k, pr, ls, ls2g, kscale, sigma, chunk, acc, st);
+                    bgki_pair(g, i, j, k, pr, kscale, sigma, chunk, acc, st);
             }
         }
     }
@@ -298,16 +296,16 @@ static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const
     acc.init(V);
     BgkiStats st;
     if (brute) {
-        const float ls2g = k3_ls2_guard(ls);
         const float kscale = k3_kernel_scale(ls);
+        const float sk = k3_kernel_sk(ls);
         for (int p = 0; p < n; ++p) {
-            K3PointRec pr = k3_point_prep(xs[p], ys[p], zs[p], g.res);
+            K3PointRec pr = k3_point_prep(xs[p], ys[p], zs[p], g.res, sk);
             if (pr.valid == 0.0f) continue;
             int32_t chunk = p >> K3_BGKI_CHUNK_LOG2;
             for (int k = 0; k < g.gsz; ++k)
                 for (int j = 0; j < g.gs; ++j)
                     for (int i = 0; i < g.gs; ++i)
-                        bgki_pair(g, i, j, k, pr, ls, ls2g, kscale, sigma, chunk, acc, st);
+                        bgki_pair(g, i, j, k, pr, kscale, sigma, chunk, acc, st);
         }
     } else if (g_bgki_threads <= 1) {
         bgki_culled_slab(g, xs, ys, zs, n, ls, sigma, 0, g.gs, acc, st);
@@ -974,7 +972,8 @@ static void bgki_x_points(const WindowInput& in, const Geom& g, const float* xs,
     const double R = lsv * (1.0 + 1e-5) + 1e-4;
     const double res = in.p->resolution;
     for (int pi = 0; pi < n; ++pi) {
-        const K3PointRec pr = k3_point_prep(xs[pi], ys[pi], zs[pi], in.p->resolution);
+        // only the validity flag of the record is used here
+        const K3PointRec pr = k3_point_prep(xs[pi], ys[pi], zs[pi], in.p->resolution, 1.0f);
         if (pr.valid == 0.0f) continue;
         const double xm = xs[pi], ym = ys[pi], zm = zs[pi];
         const double dm = std::sqrt(xm * xm + ym * ym + zm * zm);
