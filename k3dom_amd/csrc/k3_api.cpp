@@ -365,8 +365,8 @@ k3dom_status tail_mass(k3dom_handle* h, bool normal) {
     const uint32_t* ab = &h->sc->overflow;
     const bool ds = h->method == K3DOM_METHOD_DSPHDMIB;
     if (!ds && h->ngroups == 0)
-        HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, grid_geom(h), h->tg, h->ibuf, h->del_occ, h->del_free,
-                                       h->sc));
+        HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, grid_geom(h), h->tg, h->ibuf, h->sigma_occ,
+                                       h->sigma_free, h->del_occ, h->del_free, h->sc));
     mark(h, ST_MASS);
     if (normal) {
         if (ds)
@@ -489,8 +489,8 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const f
             gg.sx = h->h_gfc[g].g.sx;
             gg.sy = h->h_gfc[g].g.sy;
             gg.sz = h->h_gfc[g].g.sz;
-            HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, gg, h->tg, group_ibuf(h, g),
-                                           g == 0 ? h->del_occ : h->gk[g].del_occ,
+            HIP_TRY(h, launch_bgki_combine(h->stream, h->ntiles, gg, h->tg, group_ibuf(h, g), h->sigma_occ,
+                                           h->sigma_free, g == 0 ? h->del_occ : h->gk[g].del_occ,
                                            g == 0 ? h->del_free : h->gk[g].del_free, group_sc(h, g)));
         }
         for (int g = 1; g < h->ngroups; ++g)

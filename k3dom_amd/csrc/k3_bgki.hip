@@ -513,16 +513,18 @@ struct BgTable<true> : BgTableV {
     float2 gab[K3_KTAB_N + 1];
     float gc[K3_KTAB_N + 1];
 };
-// The table argument s (in table units, see K3PointRec) may be slightly
-// negative (a rounded difference of squares standing for 0) or beyond N: it
-// is clamped with one med3, equal to min(max(s, 0), N) of k3_kernel_index_s
-// for every non-NaN s, and the fraction is taken with fract, which equals
-// c - (float)(int)c exactly for 0 <= c < 2^24, so the result is that of
-// k3_kernel_value_s bit for bit.  The length-scale derivative (GRAD) is
-// sigma G(u) / ls from the second table at the same index and fraction
-// (k3_kernel_dls_s), where the value is positive.
+// The kernel bracket (sigma = 1) of table argument s (in table units, see
+// K3PointRec); the sums are multiplied by the sigma factor once per voxel.
+// s may be slightly negative (a rounded difference of squares standing for
+// 0) or beyond N: it is clamped with one med3, equal to min(max(s, 0), N) of
+// k3_kernel_index_s for every non-NaN s, and the fraction is taken with
+// fract, which equals c - (float)(int)c exactly for 0 <= c < 2^24, so the
+// result is that of k3_kernel_bracket_s bit for bit.  The length-scale
+// derivative (GRAD) is G(u) from the second table at the same index and
+// fraction where the bracket is positive (k3_kernel_dls_s without its
+// sigma / ls factor).
 template <bool GRAD>
-__device__ __forceinline__ BgTerm<GRAD> bg_term(float s_raw, float sigma, float inv_ls, const BgTable<GRAD>* stab) {
+__device__ __forceinline__ BgTerm<GRAD> bg_term(float s_raw, const BgTable<GRAD>* stab) {
     BgTerm<GRAD> t;
     const float s = __builtin_amdgcn_fmed3f(s_raw, 0.0f, (float)K3_KTAB_N);
     const int i = (int)s;
@@ -531,12 +533,12 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float s_raw, float sigma, float 
     const uint32_t i4 = (uint32_t)i << 2;
     const float2 ab = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(stab->ab) + 2u * i4);
     const float cc = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(stab->c) + i4);
-    t.v = k3_kernel_poly(ab.x, ab.y, cc, f, sigma);
+    t.v = fmaxf(0.0f, k3_kernel_poly_raw(ab.x, ab.y, cc, f));
     t.g = 0.0f;
     if constexpr (GRAD) {
         const float2 g = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(stab->gab) + 2u * i4);
         const float gc = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(stab->gc) + i4);
-        t.g = t.v > 0.0f ? sigma * k3_kernel_poly_raw(g.x, g.y, gc, f) * inv_ls : 0.0f;
+        t.g = t.v > 0.0f ? k3_kernel_poly_raw(g.x, g.y, gc, f) : 0.0f;
     }
     return t;
 }
@@ -560,14 +562,13 @@ __device__ __forceinline__ void bg_load_table(BgTable<GRAD>* stab) {
 template <bool GRAD>
 constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 
-// Read-only device memory in the constant address space: loads at a uniform
-// address become scalar loads (the point records do not change during the
-// gather).
-typedef const __attribute__((address_space(4))) float bg_const_f;
-// First four words of point record p (ux, uy, uz, mgs).
-__device__ __forceinline__ float4 bg_dir(const bg_const_f* rec, uint32_t p) {
-    const bg_const_f* r = rec + 12u * p;
-    return make_float4(r[0], r[1], r[2], r[3]);
+// The first 16 bytes of a staged record (direction u and mgs) as one
+// ds_read_b128: the empty asm statement keeps the fourth word live, since a
+// 12-byte read of the three words used measured slower.
+__device__ __forceinline__ float4 bg_lds_dir(const float4* p) {
+    float4 r = *p;
+    asm volatile("" : "+v"(r.w));
+    return r;
 }
 
 // Gather of one work item (a whole tile, or one 1024-point chunk of a long
@@ -589,7 +590,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
     const float dc2s = fmaf(vz, vz, fmaf(vy, vy, vx * vx)) * a.kscale;
     // free term of a voxel for every point in front of which it lies
     // (the kernel is an exact zero from distance ls on)
-    const BgTerm<GRAD> kc = bg_term<GRAD>(dc2s, a.sigma_f, a.inv_ls, stab);
+    const BgTerm<GRAD> kc = bg_term<GRAD>(dc2s, stab);
     // bounding box of the tile's 64 voxel centres (lanes outside the grid
     // included); uniform values are kept in scalar registers
     const float tcx =
@@ -622,8 +623,6 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
     const float tc_n = sqrtf(fmaf(tcz, tcz, fmaf(tcy, tcy, tcx * tcx))) + trad;
     const float lim2 = uniform_f(fmaf(a.ls * a.ls, 1.0f + 1e-5f, fmaf(1e-5f * tc_n, tc_n, 1e-6f)));
     const float inv_sk = uniform_f(1.0f / a.sk);
-    // the point records (12 floats each), read with scalar loads
-    const bg_const_f* srow = (const bg_const_f*)(uintptr_t)a.rec;
     const bool kc_live = __any(kc.v != 0.0f);
     float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
     float gsum_o = 0.0f, gsum_f = 0.0f;  // length-scale derivatives (GRAD)
@@ -700,39 +699,27 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
                 // distance for every lane and that have no occupied term, up
                 // to the next other candidate or chunk start: two terms are
                 // evaluated together (independent table reads), then added
-                // in order.  Only the direction u is read.
+                // in order.  Only the direction u is used.
                 const uint64_t stop = q < 63u ? ((~m_main | m_start) >> (q + 1u)) : 0ull;
                 const uint32_t r = min(cnt_k, stop ? q + 1u + (uint32_t)__builtin_ctzll(stop) : 64u);
-                // The directions come through the scalar cache (the records
-                // are read-only during the launch), two pairs ahead: the only
-                // LDS reads of the run are the table lookups.
-                const uint32_t rl = r - 1u;
-                float4 u0 = bg_dir(srow, lane_word(mt, q) >> 3);
-                float4 u1 = bg_dir(srow, lane_word(mt, min(q + 1u, rl)) >> 3);
-                float4 u2 = bg_dir(srow, lane_word(mt, min(q + 2u, rl)) >> 3);
-                float4 u3 = bg_dir(srow, lane_word(mt, min(q + 3u, rl)) >> 3);
                 for (; q + 2u <= r; q += 2u) {
-                    const float4 n0 = bg_dir(srow, lane_word(mt, min(q + 4u, rl)) >> 3);
-                    const float4 n1 = bg_dir(srow, lane_word(mt, min(q + 5u, rl)) >> 3);
+                    const float4 u0 = bg_lds_dir(srec + 3u * q), u1 = bg_lds_dir(srec + 3u * q + 3u);
                     const float d0 = fmaf(vz, u0.z, fmaf(vy, u0.y, vx * u0.x));
                     const float d1 = fmaf(vz, u1.z, fmaf(vy, u1.y, vx * u1.x));
                     // squared distances clamped at 0 by bg_term
-                    const BgTerm<GRAD> t0 = bg_term<GRAD>(fmaf(-d0, d0, dc2s), a.sigma_f, a.inv_ls, stab);
-                    const BgTerm<GRAD> t1 = bg_term<GRAD>(fmaf(-d1, d1, dc2s), a.sigma_f, a.inv_ls, stab);
+                    const BgTerm<GRAD> t0 = bg_term<GRAD>(fmaf(-d0, d0, dc2s), stab);
+                    const BgTerm<GRAD> t1 = bg_term<GRAD>(fmaf(-d1, d1, dc2s), stab);
                     cur_f += t0.v;
                     cur_f += t1.v;
                     if (GRAD) {
                         gsum_f += t0.g;
                         gsum_f += t1.g;
                     }
-                    u0 = u2;
-                    u1 = u3;
-                    u2 = n0;
-                    u3 = n1;
                 }
                 if (q < r) {
+                    const float4 u0 = bg_lds_dir(srec + 3u * q);
                     const float d0 = fmaf(vz, u0.z, fmaf(vy, u0.y, vx * u0.x));
-                    const BgTerm<GRAD> t0 = bg_term<GRAD>(fmaf(-d0, d0, dc2s), a.sigma_f, a.inv_ls, stab);
+                    const BgTerm<GRAD> t0 = bg_term<GRAD>(fmaf(-d0, d0, dc2s), stab);
                     cur_f += t0.v;
                     if (GRAD) gsum_f += t0.g;
                     ++q;
@@ -747,7 +734,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
                 const float so = fmaf(oz, oz, fmaf(oy, oy, ox * ox)) * a.kscale;
                 const bool co = so < (float)K3_KTAB_N;
                 if (__any(co)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(so, a.sigma_o, a.inv_ls, stab);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(so, stab);
                     cur_o += t.v;
                     if (GRAD) gsum_o += t.g;
                 }
@@ -758,7 +745,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
                 const float sf = fmaf(-dp, dp, dc2s);  // clamped at 0 by bg_term
                 const bool cf = sf < (float)K3_KTAB_N;
                 if (__any(cf)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(sf, a.sigma_f, a.inv_ls, stab);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(sf, stab);
                     cur_f += t.v;
                     if (GRAD) gsum_f += t.g;
                 }
@@ -777,7 +764,7 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
                 const float sa = c2 ? sp : se;
                 const bool cs = (c2 || c3) && sa < (float)K3_KTAB_N;
                 if (__any(cs)) {
-                    const BgTerm<GRAD> t = bg_term<GRAD>(sa, a.sigma_f, a.inv_ls, stab);
+                    const BgTerm<GRAD> t = bg_term<GRAD>(sa, stab);
                     cur_f += cs ? t.v : (c1 ? kc.v : 0.0f);
                     if (GRAD) gsum_f += cs ? t.g : (c1 ? kc.g : 0.0f);
                 } else {
@@ -794,11 +781,12 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
     if (slot == BG_DIRECT) {
         if (i < g.gs && j < g.gs && k < g.gsz) {
             const int cell = i + g.gs * j + g.gs * g.gs * k;
-            a.del_occ[cell] = tot_o;
-            a.del_free[cell] = tot_f;
+            const float fo = k3_sigma_factor(a.sigma_o), ff = k3_sigma_factor(a.sigma_f);
+            a.del_occ[cell] = fo * tot_o;
+            a.del_free[cell] = ff * tot_f;
             if (GRAD) {
-                a.dls_occ[cell] = gsum_o;
-                a.dls_free[cell] = gsum_f;
+                a.dls_occ[cell] = (fo * a.inv_ls) * gsum_o;
+                a.dls_free[cell] = (ff * a.inv_ls) * gsum_f;
             }
         }
     } else {
@@ -841,8 +829,10 @@ __global__ void k_reset_queues(uint32_t* qheads) {
     if (threadIdx.x < BG_NQ) qheads[threadIdx.x * BG_QSTRIDE] = 0u;
 }
 
+// The forward gather is held to 64 VGPRs, so that a SIMD runs 8 of its
+// wavefronts (a few per-item values spill; the candidate loops do not).
 template <bool GRAD, int W>
-__global__ __launch_bounds__(W * 64) void k_bgki(BgArgs a) {
+__global__ __launch_bounds__(W * 64, GRAD ? 1 : 8) void k_bgki(BgArgs a) {
     __shared__ float4 srec[W][64 * 3];
     __shared__ uint32_t smeta[W][64];
     __shared__ BgTable<GRAD> stab_s;
@@ -1005,7 +995,7 @@ __global__ __launch_bounds__(TPB) void k_bgki_combine(int ntiles, GridGeom g, Ti
                                                       const uint32_t* __restrict__ tile_partial,
                                                       const float* __restrict__ partial, float* del_occ,
                                                       float* del_free, float* dls_occ, float* dls_free,
-                                                      const DevScalars* sc) {
+                                                      float fo, float ff, float inv_ls, const DevScalars* sc) {
     if (sc->overflow) return;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tile = blockIdx.x * BG_WAVES + w;
@@ -1046,24 +1036,26 @@ __global__ __launch_bounds__(TPB) void k_bgki_combine(int ntiles, GridGeom g, Ti
         }
     }
     const int cell = i + g.gs * j + g.gs * g.gs * k;
-    del_occ[cell] = ro;
-    del_free[cell] = rf;
+    del_occ[cell] = fo * ro;
+    del_free[cell] = ff * rf;
     if (GRAD) {
-        dls_occ[cell] = go;
-        dls_free[cell] = gf;
+        dls_occ[cell] = (fo * inv_ls) * go;
+        dls_free[cell] = (ff * inv_ls) * gf;
     }
 }
 hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg, const ItemBufs& I,
-                               float* del_occ, float* del_free, DevScalars* sc) {
+                               float sigma_o, float sigma_f, float* del_occ, float* del_free, DevScalars* sc) {
     hipLaunchKernelGGL(k_bgki_combine<false>, dim3(blocks_for(ntiles, BG_WAVES)), dim3(TPB), 0, st, ntiles, g, tg,
-                       I.tile_partial, I.partial, del_occ, del_free, (float*)nullptr, (float*)nullptr, sc);
+                       I.tile_partial, I.partial, del_occ, del_free, (float*)nullptr, (float*)nullptr,
+                       k3_sigma_factor(sigma_o), k3_sigma_factor(sigma_f), 0.0f, sc);
     return hipGetLastError();
 }
 hipError_t launch_bgki_combine_grad(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg,
-                                    const ItemBufs& I, float* del_occ, float* del_free, float* dls_occ,
-                                    float* dls_free, DevScalars* sc) {
+                                    const ItemBufs& I, float ls, float sigma_o, float sigma_f, float* del_occ,
+                                    float* del_free, float* dls_occ, float* dls_free, DevScalars* sc) {
     hipLaunchKernelGGL(k_bgki_combine<true>, dim3(blocks_for(ntiles, BG_WAVES)), dim3(TPB), 0, st, ntiles, g, tg,
-                       I.tile_partial, I.partial, del_occ, del_free, dls_occ, dls_free, sc);
+                       I.tile_partial, I.partial, del_occ, del_free, dls_occ, dls_free, k3_sigma_factor(sigma_o),
+                       k3_sigma_factor(sigma_f), 1.0f / ls, sc);
     return hipGetLastError();
 }
 

@@ -208,16 +208,18 @@ hipError_t launch_merge_group_scalars(hipStream_t st, DevScalars* sc, const DevS
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                        float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles, const K3PointRec* rec,
                        float* del_occ, float* del_free, DevScalars* sc, int waves = 8);
+// Evidence of the split tiles: chunk sums added in chunk order, times the
+// sigma factors (direct items are written by the gather itself).
 hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg, const ItemBufs& I,
-                               float* del_occ, float* del_free, DevScalars* sc);
+                               float sigma_o, float sigma_f, float* del_occ, float* del_free, DevScalars* sc);
 // The same evidence plus its derivative in the length scale (dls_*).
 hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles,
                             const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
                             DevScalars* sc);
 hipError_t launch_bgki_combine_grad(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg,
-                                    const ItemBufs& I, float* del_occ, float* del_free, float* dls_occ,
-                                    float* dls_free, DevScalars* sc);
+                                    const ItemBufs& I, float ls, float sigma_o, float sigma_f, float* del_occ,
+                                    float* del_free, float* dls_occ, float* dls_free, DevScalars* sc);
 hipError_t launch_bgki_count(hipStream_t st, int grid_blocks, const GridGeom& g, const TileGeom& tg, float ls,
                              const ItemBufs& I, uint32_t n_items, const LevelBufs& tiles, const K3PointRec* rec,
                              unsigned long long* counts);

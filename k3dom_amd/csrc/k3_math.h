@@ -225,12 +225,20 @@ K3_HD int k3_kernel_index_s(float s, float* f) {
     *f = c - (float)i;
     return i;
 }
-// k(d) for the table argument s = d^2 * k3_kernel_scale(ls).
-K3_HD float k3_kernel_value_s(float s, float sigma) {
+// The bracket of the kernel (sigma = 1), clamped at 0, for the table argument
+// s = d^2 * k3_kernel_scale(ls).  Kernel inference sums these per voxel (in
+// the chunked order above) and multiplies the sum by k3_sigma_factor(sigma)
+// once: max(0, sigma k) = sigma max(0, k) for sigma >= 0, and the bracket is
+// non-negative on [0, ls), so a negative sigma gives no evidence, as the
+// reference's max(0, sigma k) does there.
+K3_HD float k3_kernel_bracket_s(float s) {
     float f;
     const int i = k3_kernel_index_s(s, &f);
-    return k3_kernel_poly(k3_ktab[4 * i], k3_ktab[4 * i + 1], k3_ktab[4 * i + 2], f, sigma);
+    return fmaxf(0.0f, k3_kernel_poly_raw(k3_ktab[4 * i], k3_ktab[4 * i + 1], k3_ktab[4 * i + 2], f));
 }
+K3_HD float k3_sigma_factor(float sigma) { return fmaxf(sigma, 0.0f); }
+// One term, sigma k(d), for the table argument s.
+K3_HD float k3_kernel_value_s(float s, float sigma) { return k3_sigma_factor(sigma) * k3_kernel_bracket_s(s); }
 // k(sqrt(a2)) for a2 >= 0 with scale = k3_kernel_scale(ls).
 K3_HD float k3_kernel_value_sq(float a2, float scale, float sigma) { return k3_kernel_value_s(a2 * scale, sigma); }
 
@@ -256,7 +264,7 @@ K3_HD float k3_kernel_dls(float d, float ls, float inv_ls, float sigma) {
 K3_HD float k3_kernel_dls_s(float s, float inv_ls, float sigma) {
     float f;
     const int i = k3_kernel_index_s(s, &f);
-    if (!(k3_kernel_poly(k3_ktab[4 * i], k3_ktab[4 * i + 1], k3_ktab[4 * i + 2], f, sigma) > 0.0f)) return 0.0f;
+    if (!(k3_kernel_poly_raw(k3_ktab[4 * i], k3_ktab[4 * i + 1], k3_ktab[4 * i + 2], f) > 0.0f)) return 0.0f;
     return sigma * k3_kernel_poly_raw(k3_ktab_dls[4 * i], k3_ktab_dls[4 * i + 1], k3_ktab_dls[4 * i + 2], f) * inv_ls;
 }
 K3_HD float k3_kernel_dls_sq(float a2, float scale, float inv_ls, float sigma) {

@@ -346,8 +346,9 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
             if ((s = from_handle(w, build_lists(h, fc, px, px + fi.n, px + 2 * fi.n))) != K3DOM_OK) return s;
             W_HIP(w, launch_bgki_grad(h->stream, h->gather_blocks, fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
                                       h->ibuf, h->lbuf[2], h->rec, w->del_o, w->del_f, w->dls_o, w->dls_f, h->sc));
-            W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_fc[f].g, h->tg, h->ibuf, w->del_o,
-                                              w->del_f, w->dls_o, w->dls_f, h->sc));
+            W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_fc[f].g, h->tg, h->ibuf, h->p.ls,
+                                              h->sigma_occ, h->sigma_free, w->del_o, w->del_f, w->dls_o, w->dls_f,
+                                              h->sc));
         } else {
             // one pass per LiDAR with the sensor at its origin, added in LiDAR order
             const size_t g0 = (size_t)f * K3DOM_MAX_LIDARS;
@@ -363,8 +364,8 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
                 float* tlf = g == 0 ? w->dls_f : w->g_lf;
                 W_HIP(w, launch_bgki_grad(h->stream, h->gather_blocks, gfc, h->tg, h->p.ls, h->sigma_occ,
                                           h->sigma_free, h->ibuf, h->lbuf[2], h->rec, to, tf, tlo, tlf, h->sc));
-                W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_gfc[g0 + g].g, h->tg, h->ibuf, to, tf,
-                                                  tlo, tlf, h->sc));
+                W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_gfc[g0 + g].g, h->tg, h->ibuf, h->p.ls,
+                                                  h->sigma_occ, h->sigma_free, to, tf, tlo, tlf, h->sc));
                 if (g > 0) {
                     W_HIP(w, launch_add_evidence(h->stream, &h->sc->overflow, w->del_o, w->del_f, w->g_o, w->g_f,
                                                  w->V));

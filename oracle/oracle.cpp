@@ -170,10 +170,12 @@ struct ChunkAcc {
         }
         cur[v] += term;
     }
-    void finish(float* del_occ, float* del_free) {
+    // the sums of kernel brackets times the sigma factor (k3_kernel_bracket_s)
+    void finish(float* del_occ, float* del_free, float sigma) {
+        const float sf = k3_sigma_factor(sigma);
         for (size_t v = 0; v < tot_o.size(); ++v) {
-            del_occ[v] = tot_o[v] + cur_o[v];
-            del_free[v] = tot_f[v] + cur_f[v];
+            del_occ[v] = sf * (tot_o[v] + cur_o[v]);
+            del_free[v] = sf * (tot_f[v] + cur_f[v]);
         }
     }
 };
@@ -211,19 +213,19 @@ static float seg_sum(const float* x, int st, int en) {
     return s;
 }
 
-static void bgki_pair(const Geom& g, int i, int j, int k, const K3PointRec& pr, float kscale, float sigma,
-                      int32_t chunk, ChunkAcc& acc, BgkiStats& st) {
+static void bgki_pair(const Geom& g, int i, int j, int k, const K3PointRec& pr, float kscale, int32_t chunk,
+                      ChunkAcc& acc, BgkiStats& st) {
     float vx = g.cx[i], vy = g.cy[j], vz = g.cz[k];
     size_t v = (size_t)i + (size_t)g.gs * j + (size_t)g.gs * g.gs * k;
     float dc2s = fmaf(vz, vz, fmaf(vy, vy, vx * vx)) * kscale;
     float s_o = k3_occ_s(vx, vy, vz, pr, kscale);
     if (s_o >= 0.0f) {
-        acc.add(acc.tot_o, acc.cur_o, acc.chunk_o, v, chunk, k3_kernel_value_s(s_o, sigma));
+        acc.add(acc.tot_o, acc.cur_o, acc.chunk_o, v, chunk, k3_kernel_bracket_s(s_o));
         st.occ_pairs++;
     }
     float s_f = k3_free_s(vx, vy, vz, dc2s, pr, kscale);
     if (s_f >= 0.0f) {
-        acc.add(acc.tot_f, acc.cur_f, acc.chunk_f, v, chunk, k3_kernel_value_s(s_f, sigma));
+        acc.add(acc.tot_f, acc.cur_f, acc.chunk_f, v, chunk, k3_kernel_bracket_s(s_f));
         st.free_pairs++;
     }
 }
@@ -257,7 +259,7 @@ static int g_bgki_threads = 1;
 // point in order, so each voxel's terms arrive in point order whatever the
 // slab, and the sums do not depend on how the x axis is split.
 static void bgki_culled_slab(const Geom& g, const float* xs, const float* ys, const float* zs, int n, float ls,
-                             float sigma, int xlo, int xhi, ChunkAcc& acc, BgkiStats& st) {
+                             int xlo, int xhi, ChunkAcc& acc, BgkiStats& st) {
     const float kscale = k3_kernel_scale(ls);
     const float sk = k3_kernel_sk(ls);
     const double R = (double)ls * (1.0 + 1e-5) + 1e-4;
@@ -283,7 +285,7 @@ static void bgki_culled_slab(const Geom& g, const float* xs, const float* ys, co
                 double zlo = std::min(tc * mz, td * mz) - R, zhi = std::max(tc * mz, td * mz) + R;
                 if (!axis_range(zlo, zhi, g.res, g.sz, g.gsz, &k0, &k1)) continue;
                 for (int k = k0; k <= k1; ++k)
-                    bgki_pair(g, i, j, k, pr, kscale, sigma, chunk, acc, st);
+                    bgki_pair(g, i, j, k, pr, kscale, chunk, acc, st);
             }
         }
     }
@@ -305,10 +307,10 @@ static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const
             for (int k = 0; k < g.gsz; ++k)
                 for (int j = 0; j < g.gs; ++j)
                     for (int i = 0; i < g.gs; ++i)
-                        bgki_pair(g, i, j, k, pr, kscale, sigma, chunk, acc, st);
+                        bgki_pair(g, i, j, k, pr, kscale, chunk, acc, st);
         }
     } else if (g_bgki_threads <= 1) {
-        bgki_culled_slab(g, xs, ys, zs, n, ls, sigma, 0, g.gs, acc, st);
+        bgki_culled_slab(g, xs, ys, zs, n, ls, 0, g.gs, acc, st);
     } else {
         // x slabs of at most 4 voxels, taken dynamically: voxels of different
         // slabs are disjoint, so the threads never write the same element
@@ -316,13 +318,13 @@ static BgkiStats bgki_all(const Geom& g, const float* xs, const float* ys, const
         std::vector<BgkiStats> sst(slabs);
 #pragma omp parallel for num_threads(g_bgki_threads) schedule(dynamic, 1)
         for (int s = 0; s < slabs; ++s)
-            bgki_culled_slab(g, xs, ys, zs, n, ls, sigma, 4 * s, std::min(g.gs, 4 * s + 4), acc, sst[s]);
+            bgki_culled_slab(g, xs, ys, zs, n, ls, 4 * s, std::min(g.gs, 4 * s + 4), acc, sst[s]);
         for (const BgkiStats& x : sst) {
             st.occ_pairs += x.occ_pairs;
             st.free_pairs += x.free_pairs;
         }
     }
-    acc.finish(del_occ, del_free);
+    acc.finish(del_occ, del_free, sigma);
     return st;
 }
 

@@ -125,8 +125,9 @@ def test_bgki_hand_case(oracle_lib):
 
 
 def test_bgki_chunked_association(oracle_lib):
-    # 3000 points all within ls of the voxel next to the sensor: the sum is
-    # sum over 1024-point chunks (in order) of in-order chunk sums
+    # 3000 points all within ls of the voxel next to the sensor: the evidence
+    # is sigma times the sum over 1024-point chunks (in order) of in-order
+    # chunk sums of the kernel brackets
     rng = np.random.default_rng(7)
     n = 3000
     dirs = rng.normal(size=(n, 3))
@@ -134,8 +135,8 @@ def test_bgki_chunked_association(oracle_lib):
     pts = (dirs * rng.uniform(0.5, 3.0, size=(n, 1))).astype(np.float32)
     gs, gsz, res, ls, sigma = 4, 4, 0.1, 0.5, 0.1
     o, f, _ = orc.bgki(gs, gsz, res, (0.0, 0.0, 0.0), ls, sigma, pts[:, 0], pts[:, 1], pts[:, 2])
-    # per-point free terms of voxel 0 from single-point evaluations
-    terms = np.array([orc.bgki(gs, gsz, res, (0.0, 0.0, 0.0), ls, sigma, pts[i:i + 1, 0], pts[i:i + 1, 1],
+    # per-point free brackets of voxel 0 from single-point evaluations with sigma = 1
+    terms = np.array([orc.bgki(gs, gsz, res, (0.0, 0.0, 0.0), ls, 1.0, pts[i:i + 1, 0], pts[i:i + 1, 1],
                                pts[i:i + 1, 2])[1][0] for i in range(n)], np.float32)
     tot = np.float32(0)
     for c0 in range(0, n, 1024):
@@ -143,10 +144,12 @@ def test_bgki_chunked_association(oracle_lib):
         for v in terms[c0:c0 + 1024]:
             part = np.float32(part + v)
         tot = np.float32(tot + part)
+    tot = np.float32(np.float32(sigma) * tot)
     assert f[0].view(np.uint32) == tot.view(np.uint32)
     seq = np.float32(0)
     for v in terms:
         seq = np.float32(seq + v)
+    seq = np.float32(np.float32(sigma) * seq)
     assert abs(float(seq) - float(tot)) < 1e-3 * float(tot)
 
 
