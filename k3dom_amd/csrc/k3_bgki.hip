@@ -620,9 +620,10 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
     // has a free term (kc is zero in every lane), and one lying wholly beyond
     // its point likewise.
     const float hb = uniform_f(1.5f * g.res * (1.0f + 1e-6f));
-    const float tc_n = sqrtf(fmaf(tcz, tcz, fmaf(tcy, tcy, tcx * tcx))) + trad;
+    // (these bounds carry relative margins of 1e-5, far above the 2^-22 of the
+    // hardware square root and reciprocal used for them)
+    const float tc_n = __builtin_amdgcn_sqrtf(fmaf(tcz, tcz, fmaf(tcy, tcy, tcx * tcx))) + trad;
     const float lim2 = uniform_f(fmaf(a.ls * a.ls, 1.0f + 1e-5f, fmaf(1e-5f * tc_n, tc_n, 1e-6f)));
-    const float inv_sk = uniform_f(1.0f / a.sk);
     const bool kc_live = __any(kc.v != 0.0f);
     float tot_o = 0.0f, tot_f = 0.0f, cur_o = 0.0f, cur_f = 0.0f;
     float gsum_o = 0.0f, gsum_f = 0.0f;  // length-scale derivatives (GRAD)
@@ -648,12 +649,12 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
             const bool occ = !(ox * ox + oy * oy + oz * oz >= occ_r * occ_r);
             if (occ) cls |= F_OCC;
             // the tile centre's offset from the ray's line (unit direction u / sk)
-            const float pl = proj * inv_sk * inv_sk;
+            const float pl = proj * a.inv_sk2;
             const float px = fmaf(-pl, q0.x, tcx);
             const float py = fmaf(-pl, q0.y, tcy);
             const float pz = fmaf(-pl, q0.z, tcz);
-            const float dl = sqrtf(fmaf(pz, pz, fmaf(py, py, px * px)));
-            const float bnd = dl - hb * (fabsf(px) + fabsf(py) + fabsf(pz)) / dl;
+            const float dl = __builtin_amdgcn_sqrtf(fmaf(pz, pz, fmaf(py, py, px * px)));
+            const float bnd = dl - hb * (fabsf(px) + fabsf(py) + fabsf(pz)) * __builtin_amdgcn_rcpf(dl);
             const bool far = dl > 0.0f && bnd > 0.0f && bnd * bnd >= lim2;
             const bool free_dead = cls == C_NOFREE || (cls == C_BEHIND && !kc_live);
             keep = !far && (occ || !free_dead);
@@ -862,6 +863,7 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
     a.ls = ls;
     a.kscale = k3_kernel_scale(ls);
     a.sk = k3_kernel_sk(ls);
+    a.inv_sk2 = (float)(1.0 / ((double)a.sk * (double)a.sk));
     a.inv_ls = 1.0f / ls;
     a.sigma_o = sigma_o;
     a.sigma_f = sigma_f;

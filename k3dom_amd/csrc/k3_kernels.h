@@ -181,7 +181,7 @@ hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& tiles
 struct BgArgs {
     const FrameConsts* fc;
     TileGeom tg;
-    float ls, kscale, sk, inv_ls, sigma_o, sigma_f;
+    float ls, kscale, sk, inv_sk2, inv_ls, sigma_o, sigma_f;
     const uint4* items;
     uint32_t* qheads;  // the work-queue heads of the single-map gather
     const uint32_t* tlist;
