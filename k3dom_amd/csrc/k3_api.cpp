@@ -60,7 +60,7 @@ void setup_levels(k3dom_handle* h) {
         sizes[0][0] *= 2;
         sizes[0][1] *= 2;
     }
-    const int segs[3] = {256, 1024, 1024};
+    const int segs[3] = {256, 512, 512};
     for (int l = 0; l < 3; ++l) {
         LevelDesc& L = h->ldesc[l];
         L.bx = sizes[l][0]; L.by = sizes[l][1]; L.bz = sizes[l][2];
@@ -975,7 +975,7 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
     CREATE_TRY(dalloc(h, &h->fc, 1));
     CREATE_TRY(hipHostMalloc((void**)&h->h_gfc, sizeof(FrameConsts) * K3DOM_MAX_LIDARS, hipHostMallocDefault));
     CREATE_TRY(dalloc(h, &h->gfc, K3DOM_MAX_LIDARS));
-    h->gather_blocks = h->num_cus * 4;
+    h->gather_blocks = h->num_cus * 8;
     for (int e = 0; e < EV_COUNT; ++e) CREATE_TRY(hipEventCreate(&h->ev[e]));
     CREATE_TRY(dalloc(h, &h->pair_counts, 2));
     // prior fill and zeroed particle state (src/common/kernel/init.cu:20-35)

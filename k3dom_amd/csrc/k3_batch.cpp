@@ -28,7 +28,7 @@ struct k3dom_batch {
     uint32_t* d_qheads = nullptr;  // work-queue heads of the joint gather (BG_QWORDS)
     void* d_mem = nullptr;
     int gather_blocks = 0;
-    int gather_waves = 8;
+    int gather_waves = 4;
     int num_cus = 256;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -149,7 +149,7 @@ k3dom_status k3dom_batch_create(const k3dom_params* params, int32_t n_maps, k3do
         b->maps.push_back(h);
     }
     b->num_cus = b->maps[0]->num_cus;
-    b->gather_blocks = b->num_cus * 4;
+    b->gather_blocks = b->num_cus * 8;
     b->head_done.assign(n_maps, nullptr);
     b->tail_done.assign(n_maps, nullptr);
     b->m_now.assign(n_maps, 0);

@@ -899,8 +899,8 @@ hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, c
 hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T,
                             const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
-                            DevScalars* sc) {
-    return launch_bgki_w<true>(st, grid_blocks, 8,
+                            DevScalars* sc, int waves) {
+    return launch_bgki_w<true>(st, grid_blocks, waves,
                                make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, dls_occ,
                                             dls_free, sc));
 }

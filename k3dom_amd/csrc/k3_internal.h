@@ -124,7 +124,7 @@ struct k3dom_handle {
     FrameConsts* fc = nullptr;
     FrameConsts* h_fc = nullptr;
     int gather_blocks = 0;     // blocks of the kernel-inference gather (autotuned)
-    int gather_waves = 8;      // wavefronts per gather block (autotuned: 4, 8 or 16)
+    int gather_waves = 4;      // wavefronts per gather block (autotuned: 4, 8 or 16)
     // captured normal-frame graphs, one per voxel buffer (vcur), each
     // re-captured when buffers move
     bool use_graph = false;

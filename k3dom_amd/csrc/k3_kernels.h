@@ -216,7 +216,7 @@ hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, co
 hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles,
                             const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
-                            DevScalars* sc);
+                            DevScalars* sc, int waves = 4);
 hipError_t launch_bgki_combine_grad(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg,
                                     const ItemBufs& I, float ls, float sigma_o, float sigma_f, float* del_occ,
                                     float* del_free, float* dls_occ, float* dls_free, DevScalars* sc);

@@ -345,7 +345,8 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
         if (fi.ngroups == 0) {
             if ((s = from_handle(w, build_lists(h, fc, px, px + fi.n, px + 2 * fi.n))) != K3DOM_OK) return s;
             W_HIP(w, launch_bgki_grad(h->stream, h->gather_blocks, fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
-                                      h->ibuf, h->lbuf[2], h->rec, w->del_o, w->del_f, w->dls_o, w->dls_f, h->sc));
+                                      h->ibuf, h->lbuf[2], h->rec, w->del_o, w->del_f, w->dls_o, w->dls_f, h->sc,
+                                      h->gather_waves));
             W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_fc[f].g, h->tg, h->ibuf, h->p.ls,
                                               h->sigma_occ, h->sigma_free, w->del_o, w->del_f, w->dls_o, w->dls_f,
                                               h->sc));
@@ -363,7 +364,8 @@ k3dom_status k3dom_window_backward(k3dom_window* w, double dloss, double grad[6]
                 float* tlo = g == 0 ? w->dls_o : w->g_lo;
                 float* tlf = g == 0 ? w->dls_f : w->g_lf;
                 W_HIP(w, launch_bgki_grad(h->stream, h->gather_blocks, gfc, h->tg, h->p.ls, h->sigma_occ,
-                                          h->sigma_free, h->ibuf, h->lbuf[2], h->rec, to, tf, tlo, tlf, h->sc));
+                                          h->sigma_free, h->ibuf, h->lbuf[2], h->rec, to, tf, tlo, tlf, h->sc,
+                                          h->gather_waves));
                 W_HIP(w, launch_bgki_combine_grad(h->stream, h->ntiles, w->h_gfc[g0 + g].g, h->tg, h->ibuf, h->p.ls,
                                                   h->sigma_occ, h->sigma_free, to, tf, tlo, tlf, h->sc));
                 if (g > 0) {
