@@ -306,9 +306,7 @@ k3dom_status build_lists_group(k3dom_handle* h, int g, size_t off, const FrameCo
     DevScalars* sc = group_sc(h, g);
     float4* pxyz = h->pxyz + off;
     HIP_TRY(h, launch_point_prep(h->stream, fc, xs, ys, zs, h->m_cap, h->p.resolution, h->p.ls, h->rec + off, pxyz));
-    for (int l = 0; l < 3; ++l)
-        HIP_TRY(h, launch_build_level(h->stream, l, fc, h->tg, h->ldesc[l], LB[l], l ? &LB[l - 1] : nullptr, h->p.ls,
-                                      pxyz, sc));
+    HIP_TRY(h, launch_build_levels(h->stream, fc, h->tg, h->ldesc, LB, group_ibuf(h, g), h->p.ls, pxyz, sc));
     HIP_TRY(h, launch_build_items(h->stream, h->ntiles, LB[2], group_ibuf(h, g), sc));
     return K3DOM_OK;
 }
@@ -1529,9 +1527,7 @@ k3dom_status k3dom_autotune(k3dom_handle* h, int32_t* blocks_per_cu) {
     // from its point records (h->pxyz, h->fc) for each choice; the lists
     // (ascending per box) and the work items do not depend on it
     auto rebuild_lists = [&]() -> k3dom_status {
-        for (int l = 0; l < 3; ++l)
-            HIP_TRY(h, launch_build_level(h->stream, l, h->fc, h->tg, h->ldesc[l], h->lbuf[l],
-                                          l ? &h->lbuf[l - 1] : nullptr, h->p.ls, h->pxyz, h->sc));
+        HIP_TRY(h, launch_build_levels(h->stream, h->fc, h->tg, h->ldesc, h->lbuf, h->ibuf, h->p.ls, h->pxyz, h->sc));
         HIP_TRY(h, launch_build_items(h->stream, h->ntiles, h->lbuf[2], h->ibuf, h->sc));
         return K3DOM_OK;
     };

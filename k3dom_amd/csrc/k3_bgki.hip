@@ -153,27 +153,28 @@ __device__ __forceinline__ int find_parent(const uint32_t* __restrict__ item_bas
 }
 
 // ============================================================== candidate lists
-__global__ __launch_bounds__(TPB) void k_level_nseg(int level, LevelDesc L, const FrameConsts* __restrict__ fc,
-                                                    const uint32_t* __restrict__ parent_len,
-                                                    uint32_t* __restrict__ nseg, const DevScalars* sc) {
-    int P = blockIdx.x * TPB + threadIdx.x;
-    if (P >= L.nparents) return;
-    uint32_t len = level == 0 ? (uint32_t)fc->m : (sc->overflow ? 0u : parent_len[P]);
-    nseg[P] = (len + (uint32_t)L.seg - 1) / (uint32_t)L.seg;
-}
-
 struct ItemCtx {
     int P;
     uint32_t seg, nsegP, ib, begin, end, poff;
 };
+// Work items of level 0 (one parent: every point): ceil(m / seg).
+__device__ __forceinline__ uint32_t level0_items(const LevelDesc& L, int m) {
+    return ((uint32_t)m + (uint32_t)L.seg - 1u) / (uint32_t)L.seg;
+}
 __device__ __forceinline__ ItemCtx item_ctx(int level, const LevelDesc& L, uint32_t item,
                                             const uint32_t* __restrict__ item_base,
                                             const uint32_t* __restrict__ parent_off,
                                             const uint32_t* __restrict__ parent_len, int m) {
     ItemCtx x;
-    x.P = find_parent(item_base, L.nparents, item);
-    x.ib = item_base[x.P];
-    x.nsegP = item_base[x.P + 1] - x.ib;
+    if (level == 0) {
+        x.P = 0;
+        x.ib = 0;
+        x.nsegP = level0_items(L, m);
+    } else {
+        x.P = find_parent(item_base, L.nparents, item);
+        x.ib = item_base[x.P];
+        x.nsegP = item_base[x.P + 1] - x.ib;
+    }
     x.seg = item - x.ib;
     uint32_t plen = level ? parent_len[x.P] : (uint32_t)m;
     x.poff = level ? parent_off[x.P] : 0u;
@@ -212,18 +213,25 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
                                                      const uint32_t* __restrict__ parent_off,
                                                      const uint32_t* __restrict__ parent_len,
                                                      const uint32_t* __restrict__ parent_list,
-                                                     const uint32_t* __restrict__ item_base,
+                                                     uint32_t* __restrict__ item_base,
                                                      uint32_t* __restrict__ counts, uint32_t* __restrict__ masks,
-                                                     long long mask_stride, const DevScalars* sc) {
+                                                     long long mask_stride, DevScalars* sc) {
     __shared__ float4 cb[LV_CG];
     __shared__ uint32_t wcnt[TPB / 64][LV_CG];
+    const int m = fc->m;
+    if (level == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the single parent's items, for the slot scan, the fill and the boxes
+        const uint32_t n0 = level0_items(L, m);
+        item_base[0] = 0;
+        item_base[1] = n0;
+        sc->level_items[0] = n0;
+    }
     if (sc->overflow) return;
     const GridGeom g = fc->g;
-    const int m = fc->m;
     const int ncg = child_groups(L);
     const uint32_t item = blockIdx.x / ncg;
     const int c0 = (blockIdx.x % ncg) * LV_CG, nc = min(LV_CG, L.nch - c0);
-    if (item >= item_base[L.nparents]) return;
+    if (item >= (level ? item_base[L.nparents] : level0_items(L, m))) return;
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
     if (t < nc) {
@@ -333,77 +341,82 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
     if (item == 0 && t == 0) sc->level_total[level] = total;
 }
 
+// Each box's list (offset and length in the level's list).  A box of level
+// l is a parent of level l + 1, so the kernel also writes that level's work
+// items per parent (next_nseg: ceil(len / next_seg)); at the tile level it
+// writes each tile's gather items instead: one for a list of at most
+// BG_HEAVY_LEN candidates, else one per point chunk its list spans.
 __global__ __launch_bounds__(TPB) void k_level_boxes(int level, LevelDesc L, const uint32_t* __restrict__ item_base,
                                                      const uint32_t* __restrict__ slot_off,
+                                                     const uint32_t* __restrict__ list,
                                                      uint32_t* __restrict__ box_off, uint32_t* __restrict__ box_len,
+                                                     uint32_t* __restrict__ next_nseg, int next_seg,
+                                                     uint32_t* __restrict__ heavy_cnt, uint32_t* __restrict__ light_cnt,
                                                      const DevScalars* sc) {
     const int box = blockIdx.x * TPB + threadIdx.x;
     if (box >= L.nbx * L.nby * L.nbz) return;
-    if (sc->overflow) {
-        box_off[box] = 0;
-        box_len[box] = 0;
-        return;
-    }
-    int P, c;
-    if (level == 0) {
-        P = 0;
-        c = box;
-    } else {
-        int bxi = box % L.nbx, byi = (box / L.nbx) % L.nby, bzi = box / (L.nbx * L.nby);
-        P = (bxi / L.ncx) + L.pnbx * ((byi / L.ncy) + L.pnby * (bzi / L.ncz));
-        c = (bxi % L.ncx) + L.ncx * ((byi % L.ncy) + L.ncy * (bzi % L.ncz));
-    }
-    const uint32_t ib = item_base[P], ns = item_base[P + 1] - ib;
     uint32_t off = 0, len = 0;
-    if (ns > 0) {
-        uint32_t s0 = ib * L.nch + c * ns;
-        off = slot_off[s0];
-        len = slot_off[s0 + ns] - off;
+    if (!sc->overflow) {
+        int P, c;
+        if (level == 0) {
+            P = 0;
+            c = box;
+        } else {
+            int bxi = box % L.nbx, byi = (box / L.nbx) % L.nby, bzi = box / (L.nbx * L.nby);
+            P = (bxi / L.ncx) + L.pnbx * ((byi / L.ncy) + L.pnby * (bzi / L.ncz));
+            c = (bxi % L.ncx) + L.ncx * ((byi % L.ncy) + L.ncy * (bzi % L.ncz));
+        }
+        const uint32_t ib = item_base[P], ns = item_base[P + 1] - ib;
+        if (ns > 0) {
+            uint32_t s0 = ib * L.nch + c * ns;
+            off = slot_off[s0];
+            len = slot_off[s0 + ns] - off;
+        }
     }
     box_off[box] = off;
     box_len[box] = len;
+    if (next_nseg) next_nseg[box] = (len + (uint32_t)next_seg - 1u) / (uint32_t)next_seg;
+    if (heavy_cnt) {
+        if (len > BG_HEAVY_LEN) {
+            heavy_cnt[box] = (list[off + len - 1u] >> K3_BGKI_CHUNK_LOG2) - (list[off] >> K3_BGKI_CHUNK_LOG2) + 1u;
+            light_cnt[box] = 0;
+        } else {
+            heavy_cnt[box] = 0;
+            light_cnt[box] = 1;
+        }
+    }
 }
 
-hipError_t launch_build_level(hipStream_t st, int level, const FrameConsts* fc, const TileGeom& tg, const LevelDesc& L,
-                              const LevelBufs& B, const LevelBufs* parent, float ls, const float4* pxyz,
-                              DevScalars* sc) {
-    const uint32_t* poff = parent ? parent->box_off : nullptr;
-    const uint32_t* plen = parent ? parent->box_len : nullptr;
-    const uint32_t* plist = parent ? parent->list : nullptr;
-    hipLaunchKernelGGL(k_level_nseg, dim3(blocks_for(L.nparents)), dim3(TPB), 0, st, level, L, fc, plen, B.nseg, sc);
-    (void)launch_scan_u32(st, B.nseg, B.item_base, B.bsum, L.nparents, nullptr, 1, &sc->level_items[level]);
-    const unsigned grid = (unsigned)std::max(1ll, B.items_cap * child_groups(L));
-    hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(TPB), 0, st, level, fc, tg, L, ls, pxyz, poff, plen, plist,
-                       B.item_base, B.counts, B.mask, B.mask_stride, sc);
-    (void)launch_scan_u32(st, B.counts, B.slot_off, B.bsum, B.slot_cap, B.item_base + L.nparents, (uint32_t)L.nch,
-                    nullptr);
-    hipLaunchKernelGGL(k_level_fill, dim3(grid), dim3(TPB), 0, st, level, fc, tg, L, ls, pxyz, poff, plen, plist,
-                       B.item_base, B.slot_off, B.mask, B.mask_stride, B.list, B.list_cap, sc);
-    hipLaunchKernelGGL(k_level_boxes, dim3(blocks_for((long long)L.nbx * L.nby * L.nbz)), dim3(TPB), 0, st, level, L,
-                       B.item_base, B.slot_off, B.box_off, B.box_len, sc);
+hipError_t launch_build_levels(hipStream_t st, const FrameConsts* fc, const TileGeom& tg, const LevelDesc* L,
+                               LevelBufs* B, const ItemBufs& I, float ls, const float4* pxyz, DevScalars* sc) {
+    for (int level = 0; level < 3; ++level) {
+        const LevelDesc& Ll = L[level];
+        const LevelBufs& Bl = B[level];
+        const LevelBufs* parent = level ? &B[level - 1] : nullptr;
+        const uint32_t* poff = parent ? parent->box_off : nullptr;
+        const uint32_t* plen = parent ? parent->box_len : nullptr;
+        const uint32_t* plist = parent ? parent->list : nullptr;
+        // level 0's single parent: its items come from the point count (k_level_count);
+        // the other levels' per-parent items were written by the previous level's boxes
+        if (level > 0)
+            (void)launch_scan_u32(st, Bl.nseg, Bl.item_base, Bl.bsum, Ll.nparents, nullptr, 1, &sc->level_items[level]);
+        const unsigned grid = (unsigned)std::max(1ll, Bl.items_cap * child_groups(Ll));
+        hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(TPB), 0, st, level, fc, tg, Ll, ls, pxyz, poff, plen, plist,
+                           Bl.item_base, Bl.counts, Bl.mask, Bl.mask_stride, sc);
+        (void)launch_scan_u32(st, Bl.counts, Bl.slot_off, Bl.bsum, Bl.slot_cap, Bl.item_base + Ll.nparents,
+                              (uint32_t)Ll.nch, nullptr);
+        hipLaunchKernelGGL(k_level_fill, dim3(grid), dim3(TPB), 0, st, level, fc, tg, Ll, ls, pxyz, poff, plen, plist,
+                           Bl.item_base, Bl.slot_off, Bl.mask, Bl.mask_stride, Bl.list, Bl.list_cap, sc);
+        const bool last = level == 2;
+        hipLaunchKernelGGL(k_level_boxes, dim3(blocks_for((long long)Ll.nbx * Ll.nby * Ll.nbz)), dim3(TPB), 0, st, level,
+                           Ll, Bl.item_base, Bl.slot_off, Bl.list, Bl.box_off, Bl.box_len,
+                           last ? nullptr : B[level + 1].nseg, last ? 1 : L[level + 1].seg,
+                           last ? I.heavy_cnt : nullptr, last ? I.light_cnt : nullptr, sc);
+    }
     return hipGetLastError();
 }
 
 // ============================================================== work items
-__global__ __launch_bounds__(TPB) void k_count_items(int ntiles, const uint32_t* __restrict__ toff,
-                                                     const uint32_t* __restrict__ tlen,
-                                                     const uint32_t* __restrict__ tlist,
-                                                     uint32_t* __restrict__ heavy_cnt,
-                                                     uint32_t* __restrict__ light_cnt, const DevScalars* sc) {
-    int t = blockIdx.x * TPB + threadIdx.x;
-    if (t >= ntiles) return;
-    uint32_t len = sc->overflow ? 0u : tlen[t];
-    if (len > BG_HEAVY_LEN) {
-        uint32_t c0 = tlist[toff[t]] >> K3_BGKI_CHUNK_LOG2;
-        uint32_t c1 = tlist[toff[t] + len - 1] >> K3_BGKI_CHUNK_LOG2;
-        heavy_cnt[t] = c1 - c0 + 1;
-        light_cnt[t] = 0;
-    } else {
-        heavy_cnt[t] = 0;
-        light_cnt[t] = 1;
-    }
-}
-
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t v) {
     while (lo < hi) {
         uint32_t mid = (lo + hi) >> 1;
@@ -469,8 +482,7 @@ __global__ __launch_bounds__(TPB) void k_emit_heavy(int ntiles, const uint32_t* 
 }
 
 hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& T, const ItemBufs& I, DevScalars* sc) {
-    hipLaunchKernelGGL(k_count_items, dim3(blocks_for(ntiles)), dim3(TPB), 0, st, ntiles, T.box_off, T.box_len,
-                       T.list, I.heavy_cnt, I.light_cnt, sc);
+    // (the tiles' item counts come from the last level's boxes)
     (void)launch_scan_u32(st, I.heavy_cnt, I.heavy_off, I.bsum, ntiles, nullptr, 1, &sc->n_heavy);
     (void)launch_scan_u32(st, I.light_cnt, I.light_off, I.bsum, ntiles, nullptr, 1, &sc->n_light);
     hipLaunchKernelGGL(k_emit_light, dim3(blocks_for(ntiles)), dim3(TPB), 0, st, ntiles, T.box_off, T.box_len,

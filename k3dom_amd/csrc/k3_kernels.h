@@ -156,9 +156,6 @@ hipError_t launch_point_prep(hipStream_t st, const FrameConsts* fc, const float*
 // Blocks share a box's children in groups; masks need level_child_groups(L)
 // words per parent-list position.
 int level_child_groups(const LevelDesc& L);
-hipError_t launch_build_level(hipStream_t st, int level, const FrameConsts* fc, const TileGeom& tg, const LevelDesc& L,
-                              const LevelBufs& B, const LevelBufs* parent, float ls, const float4* pxyz,
-                              DevScalars* sc);
 
 // Work queues of the gather: BG_NQ heads, one per BG_QSTRIDE words (128 bytes).
 constexpr uint32_t BG_NQ = 64;
@@ -175,6 +172,11 @@ struct ItemBufs {
     int partial_stride = 128; // floats per split item (256 with length-scale derivatives)
     long long heavy_cap;
 };
+// The candidate lists of the three levels (L, B: three each), and the
+// tiles' gather-item counts into I.heavy_cnt / I.light_cnt.
+hipError_t launch_build_levels(hipStream_t st, const FrameConsts* fc, const TileGeom& tg, const LevelDesc* L,
+                               LevelBufs* B, const ItemBufs& I, float ls, const float4* pxyz, DevScalars* sc);
+// The gather's work items from the tiles' lists and item counts.
 hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& tiles, const ItemBufs& I,
                               DevScalars* sc);
 // Arguments of the kernel-inference gather over one map.

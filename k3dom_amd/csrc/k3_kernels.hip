@@ -199,8 +199,9 @@ __global__ __launch_bounds__(TPB) void k_radix_scatter(const uint32_t* __restric
 // n_max.  The total is also written to out[n] and, if given, to *total.  in
 // may equal out.
 
+// (a count read from device memory never exceeds the capacity the launch covers)
 __device__ __forceinline__ uint32_t scan_count(const uint32_t* n_dev, uint32_t n_mul, uint32_t n_host) {
-    return n_dev ? (*n_dev) * n_mul : n_host;
+    return n_dev ? min((*n_dev) * n_mul, n_host) : n_host;
 }
 
 // exclusive block scan of one value per thread (256 threads)
