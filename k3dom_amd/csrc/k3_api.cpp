@@ -160,7 +160,8 @@ k3dom_status alloc_level(k3dom_handle* h, int l, long long list_cap) {
     for (int g = 0; g < h->ngk; ++g) {
         LevelBufs& B = group_lbuf(h, g)[l];
         for (void* p : {(void*)B.counts, (void*)B.slot_off, (void*)B.list, (void*)B.bsum, (void*)B.mask}) dfree(h, p);
-        B.counts = B.slot_off = B.list = B.bsum = B.mask = nullptr;
+        B.counts = B.slot_off = B.list = B.bsum = nullptr;
+        B.mask = nullptr;
         const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : group_lbuf(h, g)[l - 1].list_cap;
         B.items_cap = L.nparents + (parent_cap + L.seg - 1) / L.seg;
         B.slot_cap = B.items_cap * L.nch;

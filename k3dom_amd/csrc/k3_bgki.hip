@@ -28,7 +28,7 @@ namespace {
 constexpr int TPB = 256;
 constexpr int BG_WAVES = TPB / 64;
 // the gather runs blocks of W wavefronts (4, 8 or 16; picked by the
-// autotuner, 8 by default): one kernel table in LDS per block
+// autotuner, 4 by default): one kernel table in LDS per block
 constexpr uint32_t BG_HEAVY_LEN = 1u << K3_BGKI_CHUNK_LOG2;
 constexpr uint32_t BG_DIRECT = 0xffffffffu;
 
@@ -99,26 +99,54 @@ __device__ float4 box_sphere(const GridGeom& g, float ls, int tx0, int ty0, int 
     return make_float4(cx, cy, cz, ls + R + 2e-3f + 1e-5f * cn);
 }
 
+// A candidate point of a list level: the segment [sensor, p] and its slack.
+// An invalid candidate (or a lane past the segment's end) gets NaN
+// coordinates, so every distance test on it is false without a branch.
 struct Cand {
-    float x, y, z, inv_mm, slack;
-    bool valid;
+    float x, y, z, mm, inv_mm, slack;
 };
 __device__ __forceinline__ Cand make_cand(float4 p, bool in_range) {
     Cand c;
-    c.x = p.x; c.y = p.y; c.z = p.z;
-    float mm = p.x * p.x + p.y * p.y + p.z * p.z;
-    c.inv_mm = mm > 0.0f ? 1.0f / mm : 0.0f;
+    const bool valid = in_range && p.w >= 0.0f;
+    const float nan = __builtin_nanf("");
+    c.x = valid ? p.x : nan;
+    c.y = valid ? p.y : nan;
+    c.z = valid ? p.z : nan;
+    c.mm = c.x * c.x + c.y * c.y + c.z * c.z;
+    c.inv_mm = c.mm > 0.0f ? __builtin_amdgcn_rcpf(c.mm) : 0.0f;
     c.slack = 1e-5f * p.w;
-    c.valid = in_range && p.w >= 0.0f;
     return c;
 }
-// Distance from the box centre to the segment [0, p] below the threshold:
-// true for every box holding a voxel with a non-zero term for the point.
-__device__ __forceinline__ bool cand_hits(const float4& b, const Cand& c) {
-    float t = fminf(fmaxf((b.x * c.x + b.y * c.y + b.z * c.z) * c.inv_mm, 0.0f), 1.0f);
-    float dx = b.x - t * c.x, dy = b.y - t * c.y, dz = b.z - t * c.z;
-    float thr = b.w + c.slack;
-    return c.valid && b.w >= 0.0f && (dx * dx + dy * dy + dz * dz < thr * thr);
+// Per-child constants of the distance test: the sphere centre b and radius
+// w of the box, |b|^2, and w^2 plus the rounding margin below.
+struct ChildBox {
+    float bx, by, bz, bb, w2x, wsq;
+};
+__device__ __forceinline__ ChildBox make_child(float4 b, bool real) {
+    ChildBox k;
+    k.bx = real ? b.x : 0.0f;
+    k.by = real ? b.y : 0.0f;
+    k.bz = real ? b.z : 0.0f;
+    k.bb = real ? fmaf(b.z, b.z, fmaf(b.y, b.y, b.x * b.x)) : __builtin_inff();
+    k.w2x = real ? 2.0f * b.w : 0.0f;
+    k.wsq = real ? fmaf(k.bb, 0x1p-18f, b.w * b.w) : 0.0f;
+    return k;
+}
+// Squared distance from a box's sphere centre b to the segment [0, p] of
+// candidate c: with t = clamp(b . p / |p|^2, 0, 1) the nearest point is t p
+// and |b - t p|^2 = |b|^2 - t (2 b . p - t |p|^2).  The expansion cancels: its
+// rounding error stays below about 40 * 2^-24 |b|^2 (the dot product's, the
+// reciprocal's and the final sums'), so the threshold (w + slack)^2 =
+// w^2 + slack (2 w + slack) gets 2^-18 |b|^2 on top (in wsq), and the test
+// stays a superset of the pairs with a non-zero term.  A child outside the
+// tile grid has |b|^2 = +inf and never passes.
+__device__ __forceinline__ bool cand_hits(const ChildBox& k, const Cand& c) {
+    const float dp = fmaf(k.bz, c.z, fmaf(k.by, c.y, k.bx * c.x));
+    const float t = __builtin_amdgcn_fmed3f(dp * c.inv_mm, 0.0f, 1.0f);
+    const float r = fmaf(-t, c.mm, dp);  // b . p - t |p|^2
+    const float d2 = fmaf(-t, dp + r, k.bb);
+    const float thr2 = fmaf(c.slack, k.w2x + c.slack, k.wsq);
+    return d2 < thr2;
 }
 
 // Child c of parent P at this level (level 0: the box with id c).  Returns the
@@ -142,16 +170,6 @@ __device__ int child_box(int level, const LevelDesc& L, const TileGeom& tg, int 
     return bxi + L.nbx * (byi + L.nby * bzi);
 }
 
-__device__ __forceinline__ int find_parent(const uint32_t* __restrict__ item_base, int nparents, uint32_t item) {
-    int lo = 0, hi = nparents - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (item_base[mid] <= item) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 // ============================================================== candidate lists
 struct ItemCtx {
     int P;
@@ -161,6 +179,27 @@ struct ItemCtx {
 __device__ __forceinline__ uint32_t level0_items(const LevelDesc& L, int m) {
     return ((uint32_t)m + (uint32_t)L.seg - 1u) / (uint32_t)L.seg;
 }
+// The parent of work item `item` (the same in every lane): the largest P
+// with item_base[P] <= item.  Each round the wavefront samples 64 evenly
+// spaced entries of the bracket [lo, lo + span) and keeps the last sample not
+// above item, so nparents <= 4096 takes two dependent loads instead of the
+// twelve of a binary search.  item_base[lo] <= item holds throughout (lane
+// 0 samples lo), so the ballot is never empty.
+__device__ __forceinline__ int find_parent(const uint32_t* __restrict__ item_base, int nparents, uint32_t item) {
+    const int lane = (int)(threadIdx.x & 63);
+    int lo = 0, span = nparents;
+    while (span > 1) {
+        const int step = (span + 63) >> 6;
+        const int j = lo + lane * step;
+        const bool ok = lane * step < span && item_base[j] <= item;
+        const uint64_t bal = __ballot(ok);
+        const int hi = 63 - __builtin_clzll(bal);
+        lo += hi * step;
+        span = min(step, span - hi * step);
+    }
+    return lo;
+}
+
 __device__ __forceinline__ ItemCtx item_ctx(int level, const LevelDesc& L, uint32_t item,
                                             const uint32_t* __restrict__ item_base,
                                             const uint32_t* __restrict__ parent_off,
@@ -183,41 +222,48 @@ __device__ __forceinline__ ItemCtx item_ctx(int level, const LevelDesc& L, uint3
     return x;
 }
 
-// Children are handled in groups of LV_CG per block, so a box with many
-// children (level 0) is spread over several blocks: block b takes item
-// b / ncg and children [LV_CG * (b % ncg), ...) of it.
-constexpr int LV_CG = 32;
+// Children are handled in groups of LV_CG, one wavefront per group, so the
+// children of a box with many (level 0: all boxes) are spread over several
+// wavefronts: task t takes item t / ncg and children [LV_CG * (t % ncg), ...)
+// of it.  Eight children per wavefront keep each wavefront's serial work
+// short and give every level thousands of wavefronts to hide load latency.
+constexpr int LV_CG = 8;
+static_assert(LV_CG <= 8, "a candidate's hits per child group are one mask byte");
 __host__ __device__ inline int child_groups(const LevelDesc& L) { return (L.nch + LV_CG - 1) / LV_CG; }
 int level_child_groups(const LevelDesc& L) { return child_groups(L); }
-
-// Candidate e of an item's segment: its point index and packed position.
-struct CandLoad {
-    uint32_t idx;
-    float4 p;
-    bool in;
-};
-__device__ __forceinline__ CandLoad load_cand(int level, const uint32_t* __restrict__ parent_list, const ItemCtx& x,
-                                              const float4* __restrict__ pxyz, uint32_t e) {
-    CandLoad r;
-    r.in = e < x.end;
-    r.idx = r.in ? (level ? parent_list[x.poff + e] : e) : 0u;
-    r.p = pxyz[r.idx];
-    return r;
+// blocks of the count and fill launches: one wavefront per (item, child group)
+unsigned level_task_blocks(long long items_cap, const LevelDesc& L) {
+    return blocks_for(std::max(1ll, items_cap * child_groups(L)), BG_WAVES);
 }
 
-// Per-item, per-child counts of the candidates of the item's segment that
-// pass the child's test.  Lane c of each wavefront keeps child c's count of
-// that wavefront's candidates; the four are added at the end.
+// Point index of candidate e of an item's segment (0 past its end).
+__device__ __forceinline__ uint32_t cand_index(int level, const uint32_t* __restrict__ parent_list, const ItemCtx& x,
+                                               uint32_t e) {
+    return e < x.end ? (level ? parent_list[x.poff + e] : e) : 0u;
+}
+
+// One wavefront per task: a work item and one group of at most LV_CG of its
+// children (task = item * ncg + group).  Without block-wide barriers the
+// four wavefronts of a block work on four tasks independently, so many items
+// are in flight per CU while each waits on its chain of dependent loads
+// (parent, bounds, candidate indices, positions).
+__device__ __forceinline__ uint32_t lv_task() { return blockIdx.x * (uint32_t)BG_WAVES + (threadIdx.x >> 6); }
+
+// Per-task, per-child counts of the candidates of the item's segment that
+// pass the child's test, and per candidate the children it passes (for the
+// fill).  NC (= LV_CG) bounds the group's children, nc of them real.  Lane
+// c sets up child c's test constants in the wavefront's LDS slice; the
+// per-child counts are wavefront-uniform, in scalar registers.
+template <int NC>
 __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConsts* __restrict__ fc, TileGeom tg,
                                                      LevelDesc L, float ls, const float4* __restrict__ pxyz,
                                                      const uint32_t* __restrict__ parent_off,
                                                      const uint32_t* __restrict__ parent_len,
                                                      const uint32_t* __restrict__ parent_list,
                                                      uint32_t* __restrict__ item_base,
-                                                     uint32_t* __restrict__ counts, uint32_t* __restrict__ masks,
+                                                     uint32_t* __restrict__ counts, uint8_t* __restrict__ masks,
                                                      long long mask_stride, DevScalars* sc) {
-    __shared__ float4 cb[LV_CG];
-    __shared__ uint32_t wcnt[TPB / 64][LV_CG];
+    __shared__ ChildBox cbs[BG_WAVES][NC];
     const int m = fc->m;
     if (level == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         // the single parent's items, for the slot scan, the fill and the boxes
@@ -227,51 +273,65 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
         sc->level_items[0] = n0;
     }
     if (sc->overflow) return;
-    const GridGeom g = fc->g;
     const int ncg = child_groups(L);
-    const uint32_t item = blockIdx.x / ncg;
-    const int c0 = (blockIdx.x % ncg) * LV_CG, nc = min(LV_CG, L.nch - c0);
+    const uint32_t task = lv_task(), item = task / (uint32_t)ncg, grp = task % (uint32_t)ncg;
     if (item >= (level ? item_base[L.nparents] : level0_items(L, m))) return;
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int lane = threadIdx.x & 63;
+    const int c0 = (int)grp * LV_CG, nc = min(NC, L.nch - c0);
     const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
-    if (t < nc) {
-        int t0[3], t1[3];
-        const int box = child_box(level, L, tg, x.P, c0 + t, t0, t1);
-        cb[t] = box < 0 ? make_float4(0.0f, 0.0f, 0.0f, -1.0f)
-                        : box_sphere(g, ls, t0[0], t0[1], t0[2], t1[0], t1[1], t1[2]);
-    }
-    __syncthreads();
-    uint32_t mine = 0;  // lane c: candidates of this wavefront passing child c
-    uint32_t* gmask = masks + (size_t)(blockIdx.x % ncg) * (size_t)mask_stride + (level ? x.poff : 0u);
-    // the next chunk's candidate is loaded while this one is tested
-    CandLoad nx = load_cand(level, parent_list, x, pxyz, x.begin + t);
-    for (uint32_t base = x.begin; base < x.end; base += TPB) {
-        const CandLoad cur = nx;
-        if (base + TPB < x.end) nx = load_cand(level, parent_list, x, pxyz, base + TPB + t);
-        const Cand cd = make_cand(cur.p, cur.in);
-        uint32_t hits = 0;  // bit c: passes child c (kept for the fill)
-        for (int c = 0; c < nc; ++c) {
-            const bool hit = cand_hits(cb[c], cd);
-            hits |= (uint32_t)hit << c;
-            const uint32_t n = (uint32_t)__popcll(__ballot(hit));
-            mine += lane == c ? n : 0u;
+    // the first groups' point indices and positions are in flight during the
+    // child set-up; afterwards each group's positions load while the one
+    // before is tested
+    uint32_t i_next = cand_index(level, parent_list, x, x.begin + 64u + lane);
+    float4 p_next = pxyz[cand_index(level, parent_list, x, x.begin + lane)];
+    ChildBox* cb = cbs[threadIdx.x >> 6];
+    if (lane < NC) {
+        float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool real = false;
+        if (lane < nc) {
+            int t0[3], t1[3];
+            const int box = child_box(level, L, tg, x.P, c0 + lane, t0, t1);
+            if (box >= 0) {
+                b = box_sphere(fc->g, ls, t0[0], t0[1], t0[2], t1[0], t1[1], t1[2]);
+                real = true;
+            }
         }
-        if (cur.in) gmask[base + t] = hits;
+        cb[lane] = make_child(b, real);
     }
-    if (lane < nc) wcnt[w][lane] = mine;
-    __syncthreads();
-    if (t < nc) counts[x.ib * L.nch + (c0 + t) * x.nsegP + x.seg] = wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+    bg_wave_sync();
+    uint32_t cnt[NC];  // candidates passing child c (wavefront-uniform)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) cnt[c] = 0;
+    uint8_t* gmask = masks + (size_t)grp * (size_t)mask_stride + (level ? x.poff : 0u);
+    for (uint32_t base = x.begin; base < x.end; base += 64u) {
+        const float4 p = p_next;
+        const bool in = base + lane < x.end;
+        if (base + 64u < x.end) {
+            p_next = pxyz[i_next];
+            i_next = cand_index(level, parent_list, x, base + 128u + lane);
+        }
+        const Cand cd = make_cand(p, in);
+        uint32_t hits = 0;  // bit c: passes child c (kept for the fill)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const bool hit = cand_hits(cb[c], cd);
+            hits |= hit ? (1u << c) : 0u;
+            cnt[c] += (uint32_t)__popcll(__ballot(hit));
+        }
+        if (in) gmask[base + lane] = (uint8_t)hits;
+    }
+    uint32_t mine = 0;  // lane c: the count of child c
+#pragma unroll
+    for (int c = 0; c < NC; ++c) mine = lane == c ? cnt[c] : mine;
+    if (lane < nc) counts[x.ib * L.nch + (c0 + lane) * x.nsegP + x.seg] = mine;
 }
 
 // Ordered fill: each candidate passing child c goes to that child's list at
 // the scanned offset of (item, c) plus the number of earlier passing
-// candidates of the item.  Per chunk of TPB candidates, every wavefront
-// records its ballot mask per child in LDS; after one barrier each wavefront
-// writes its candidates behind those of the lower wavefronts and of earlier
-// chunks.  Masks and running counts are double-buffered by chunk parity, so
-// one barrier per chunk suffices (a wavefront reaching chunk k + 2 has passed
-// the barrier of chunk k + 1, which every wavefront reaches only after
-// finishing chunk k).
+// candidates of the item.  The task's wavefront walks the segment in order,
+// 64 candidates at a time; a candidate's position is the running offset of
+// child c (wavefront-uniform) plus its rank in the group's ballot.
+template <int NC>
 __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts* __restrict__ fc, TileGeom tg,
                                                     LevelDesc L, float ls, const float4* __restrict__ pxyz,
                                                     const uint32_t* __restrict__ parent_off,
@@ -279,14 +339,10 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
                                                     const uint32_t* __restrict__ parent_list,
                                                     const uint32_t* __restrict__ item_base,
                                                     const uint32_t* __restrict__ slot_off,
-                                                    const uint32_t* __restrict__ masks, long long mask_stride,
+                                                    const uint8_t* __restrict__ masks, long long mask_stride,
                                                     uint32_t* __restrict__ list, long long list_cap,
                                                     DevScalars* sc) {
-    __shared__ uint32_t ob[LV_CG];
-    __shared__ uint32_t run[2][LV_CG];
-    __shared__ uint64_t wm[2][TPB / 64][LV_CG];
     if (sc->overflow) return;
-    const GridGeom g = fc->g;
     const int m = fc->m;
     const uint32_t n_items = item_base[L.nparents];
     const uint32_t total = slot_off[n_items * (uint32_t)L.nch];
@@ -299,46 +355,36 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
         return;
     }
     const int ncg = child_groups(L);
-    const uint32_t item = blockIdx.x / ncg;
-    const int c0 = (blockIdx.x % ncg) * LV_CG, nc = min(LV_CG, L.nch - c0);
+    const uint32_t task = lv_task(), item = task / (uint32_t)ncg, grp = task % (uint32_t)ncg;
     if (item >= n_items) return;
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int lane = threadIdx.x & 63;
+    const int c0 = (int)grp * LV_CG, nc = min(NC, L.nch - c0);
     const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
-    if (t < nc) {
-        ob[t] = slot_off[x.ib * L.nch + (c0 + t) * x.nsegP + x.seg];
-        run[0][t] = 0;
-    }
-    __syncthreads();
-    // the children each candidate passes, as the count kernel found them
-    const uint32_t* gmask = masks + (size_t)(blockIdx.x % ncg) * (size_t)mask_stride + (level ? x.poff : 0u);
-    int buf = 0;
-    for (uint32_t base = x.begin; base < x.end; base += TPB, buf ^= 1) {
-        const uint32_t e = base + t;
-        const bool in = e < x.end;
-        const uint32_t idx = in ? (level ? parent_list[x.poff + e] : e) : 0u;
-        const uint32_t hits = in ? gmask[e] : 0u;
-        uint64_t mymask = 0;  // lane c: this wavefront's mask for child c
-        for (int c = 0; c < nc; ++c) {
-            const uint64_t bal = __ballot((hits >> c) & 1u);
-            mymask = lane == c ? bal : mymask;
+    // the children each candidate passes, as the count kernel found them; the
+    // next group's indices and masks load while this one is placed
+    const uint8_t* gmask = masks + (size_t)grp * (size_t)mask_stride + (level ? x.poff : 0u);
+    uint32_t idx_next = cand_index(level, parent_list, x, x.begin + lane);
+    uint32_t hits_next = x.begin + lane < x.end ? gmask[x.begin + lane] : 0u;
+    const uint32_t at0 = lane < nc ? slot_off[x.ib * L.nch + (c0 + lane) * x.nsegP + x.seg] : 0u;
+    uint32_t at[NC];  // where child c's next candidate goes (wavefront-uniform)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) at[c] = lane_word(at0, (uint32_t)c);
+    for (uint32_t base = x.begin; base < x.end; base += 64u) {
+        const uint32_t idx = idx_next, hits = hits_next;
+        const uint32_t en = base + 64u + lane;
+        if (base + 64u < x.end) {
+            idx_next = cand_index(level, parent_list, x, en);
+            hits_next = en < x.end ? gmask[en] : 0u;
         }
-        if (lane < nc) wm[buf][w][lane] = mymask;
-        __syncthreads();
-        for (int c = 0; c < nc; ++c) {
-            const uint64_t bal = wm[buf][w][c];
-            if ((bal >> lane) & 1ull) {
-                uint32_t pre = run[buf][c];
-                for (int ww = 0; ww < w; ++ww) pre += (uint32_t)__popcll(wm[buf][ww][c]);
-                list[ob[c] + pre + bg_lanes_below(bal)] = idx;
-            }
-        }
-        if (t < nc) {
-            uint32_t r = run[buf][t];
-            for (int ww = 0; ww < TPB / 64; ++ww) r += (uint32_t)__popcll(wm[buf][ww][t]);
-            run[buf ^ 1][t] = r;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const bool hit = (hits >> c) & 1u;
+            const uint64_t bal = __ballot(hit);
+            if (hit) list[at[c] + bg_lanes_below(bal)] = idx;
+            at[c] += (uint32_t)__popcll(bal);
         }
     }
-    if (item == 0 && t == 0) sc->level_total[level] = total;
+    if (task == 0 && lane == 0) sc->level_total[level] = total;
 }
 
 // Each box's list (offset and length in the level's list).  A box of level
@@ -400,13 +446,13 @@ hipError_t launch_build_levels(hipStream_t st, const FrameConsts* fc, const Tile
         // the other levels' per-parent items were written by the previous level's boxes
         if (level > 0)
             (void)launch_scan_u32(st, Bl.nseg, Bl.item_base, Bl.bsum, Ll.nparents, nullptr, 1, &sc->level_items[level]);
-        const unsigned grid = (unsigned)std::max(1ll, Bl.items_cap * child_groups(Ll));
-        hipLaunchKernelGGL(k_level_count, dim3(grid), dim3(TPB), 0, st, level, fc, tg, Ll, ls, pxyz, poff, plen, plist,
-                           Bl.item_base, Bl.counts, Bl.mask, Bl.mask_stride, sc);
+        const unsigned grid = level_task_blocks(Bl.items_cap, Ll);
+        hipLaunchKernelGGL(k_level_count<LV_CG>, dim3(grid), dim3(TPB), 0, st, level, fc, tg, Ll, ls, pxyz, poff, plen,
+                           plist, Bl.item_base, Bl.counts, Bl.mask, Bl.mask_stride, sc);
         (void)launch_scan_u32(st, Bl.counts, Bl.slot_off, Bl.bsum, Bl.slot_cap, Bl.item_base + Ll.nparents,
                               (uint32_t)Ll.nch, nullptr);
-        hipLaunchKernelGGL(k_level_fill, dim3(grid), dim3(TPB), 0, st, level, fc, tg, Ll, ls, pxyz, poff, plen, plist,
-                           Bl.item_base, Bl.slot_off, Bl.mask, Bl.mask_stride, Bl.list, Bl.list_cap, sc);
+        hipLaunchKernelGGL(k_level_fill<LV_CG>, dim3(grid), dim3(TPB), 0, st, level, fc, tg, Ll, ls, pxyz, poff, plen,
+                           plist, Bl.item_base, Bl.slot_off, Bl.mask, Bl.mask_stride, Bl.list, Bl.list_cap, sc);
         const bool last = level == 2;
         hipLaunchKernelGGL(k_level_boxes, dim3(blocks_for((long long)Ll.nbx * Ll.nby * Ll.nbz)), dim3(TPB), 0, st, level,
                            Ll, Bl.item_base, Bl.slot_off, Bl.list, Bl.box_off, Bl.box_len,

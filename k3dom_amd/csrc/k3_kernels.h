@@ -141,9 +141,9 @@ struct LevelBufs {
     uint32_t *box_off, *box_len;   // nboxes
     uint32_t* list;                // list_cap
     uint32_t* bsum;                // scan scratch
-    uint32_t* mask;                // per child group and parent-list position: the children a candidate passes
+    uint8_t* mask;                 // per child group and parent-list position: the children a candidate passes
     long long items_cap, slot_cap, list_cap;
-    long long mask_stride;         // parent-list capacity (one mask word per position per child group)
+    long long mask_stride;         // parent-list capacity (one mask byte per position per child group)
 };
 struct TileGeom {
     int ntx, nty, ntz;
@@ -153,8 +153,8 @@ constexpr int MAX_CHILDREN = 1024;
 // m_cap: capacity of the point arrays (the launch covers it; fc->m points are live)
 hipError_t launch_point_prep(hipStream_t st, const FrameConsts* fc, const float* xs, const float* ys,
                              const float* zs, int m_cap, float res, float ls, K3PointRec* rec, float4* pxyz);
-// Blocks share a box's children in groups; masks need level_child_groups(L)
-// words per parent-list position.
+// Wavefronts share a box's children in groups; masks need
+// level_child_groups(L) bytes per parent-list position.
 int level_child_groups(const LevelDesc& L);
 
 // Work queues of the gather: BG_NQ heads, one per BG_QSTRIDE words (128 bytes).
