@@ -117,7 +117,7 @@ bool point_buffers_missing(const k3dom_handle* h) {
         if (!I.items || !I.partial) return true;
         for (int l = 0; l < 3; ++l) {
             const LevelBufs& B = group_lbuf(h, g)[l];
-            if (!B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask) return true;
+            if (!B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask || !B.ictx) return true;
         }
     }
     return h->method == K3DOM_METHOD_DSPHDMIB && (!h->ds.list || !h->ds.segcnt || !h->ds.part);
@@ -159,15 +159,18 @@ k3dom_status alloc_level(k3dom_handle* h, int l, long long list_cap) {
     h->caps_version++;
     for (int g = 0; g < h->ngk; ++g) {
         LevelBufs& B = group_lbuf(h, g)[l];
-        for (void* p : {(void*)B.counts, (void*)B.slot_off, (void*)B.list, (void*)B.bsum, (void*)B.mask}) dfree(h, p);
+        for (void* p : {(void*)B.counts, (void*)B.slot_off, (void*)B.list, (void*)B.bsum, (void*)B.mask, (void*)B.ictx})
+            dfree(h, p);
         B.counts = B.slot_off = B.list = B.bsum = nullptr;
         B.mask = nullptr;
+        B.ictx = nullptr;
         const long long parent_cap = l == 0 ? (long long)std::max(h->m_cap, 1) : group_lbuf(h, g)[l - 1].list_cap;
         B.items_cap = L.nparents + (parent_cap + L.seg - 1) / L.seg;
         B.slot_cap = B.items_cap * L.nch;
         B.list_cap = list_cap;
         B.mask_stride = parent_cap;
         HIP_TRY(h, dalloc(h, &B.mask, (size_t)parent_cap * (size_t)level_child_groups(L)));
+        HIP_TRY(h, dalloc(h, &B.ictx, (size_t)B.items_cap));
         HIP_TRY(h, dalloc(h, &B.counts, (size_t)B.slot_cap));
         HIP_TRY(h, dalloc(h, &B.slot_off, (size_t)B.slot_cap + 1));
         HIP_TRY(h, dalloc(h, &B.list, (size_t)std::max(1ll, list_cap)));
@@ -203,6 +206,7 @@ k3dom_status alloc_group_fixed(k3dom_handle* h, k3dom_handle::GroupKi& G) {
         HIP_TRY(h, dalloc(h, &LB.item_base, (size_t)L.nparents + 1));
         HIP_TRY(h, dalloc(h, &LB.box_off, nboxes));
         HIP_TRY(h, dalloc(h, &LB.box_len, nboxes));
+        if (l > 0) HIP_TRY(h, dalloc(h, &LB.ctab, 2 * (size_t)L.nparents * (size_t)L.nch));
     }
     const size_t T = (size_t)h->ntiles;
     ItemBufs& I = G.ibuf;
@@ -257,7 +261,7 @@ k3dom_status ensure_caps(k3dom_handle* h, const long long want[3], long long wan
         for (int g = 0; g < h->ngk; ++g) {
             const LevelBufs& B = group_lbuf(h, g)[l];
             // (a buffer still missing after a failed allocation is allocated again)
-            const bool missing = !B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask;
+            const bool missing = !B.list || !B.counts || !B.slot_off || !B.bsum || !B.mask || !B.ictx;
             grow = grow || want[l] > B.list_cap || items_need > B.items_cap || parent_cap > B.mask_stride || missing;
         }
         if (grow) {
@@ -948,6 +952,7 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
         CREATE_TRY(dalloc(h, &LB.item_base, (size_t)L.nparents + 1));
         CREATE_TRY(dalloc(h, &LB.box_off, nboxes));
         CREATE_TRY(dalloc(h, &LB.box_len, nboxes));
+        if (l > 0) CREATE_TRY(dalloc(h, &LB.ctab, 2 * (size_t)L.nparents * (size_t)L.nch));
     }
     const size_t T = (size_t)h->ntiles;
     ItemBufs& I = h->ibuf;

@@ -54,6 +54,9 @@ __device__ __forceinline__ float uniform_f(float v) {
 __device__ __forceinline__ uint32_t lane_word(uint32_t v, uint32_t q) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q);
 }
+__device__ __forceinline__ float lane_float(float v, int q) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
+}
 
 // ============================================================== points
 __global__ __launch_bounds__(TPB) void k_point_prep(const FrameConsts* __restrict__ fc,
@@ -103,7 +106,7 @@ __device__ float4 box_sphere(const GridGeom& g, float ls, int tx0, int ty0, int 
 // An invalid candidate (or a lane past the segment's end) gets NaN
 // coordinates, so every distance test on it is false without a branch.
 struct Cand {
-    float x, y, z, mm, inv_mm, slack;
+    float x, y, z, mm, half_inv_mm, slack;
 };
 __device__ __forceinline__ Cand make_cand(float4 p, bool in_range) {
     Cand c;
@@ -113,40 +116,48 @@ __device__ __forceinline__ Cand make_cand(float4 p, bool in_range) {
     c.y = valid ? p.y : nan;
     c.z = valid ? p.z : nan;
     c.mm = c.x * c.x + c.y * c.y + c.z * c.z;
-    c.inv_mm = c.mm > 0.0f ? __builtin_amdgcn_rcpf(c.mm) : 0.0f;
+    c.half_inv_mm = c.mm > 0.0f ? 0.5f * __builtin_amdgcn_rcpf(c.mm) : 0.0f;
     c.slack = 1e-5f * p.w;
     return c;
 }
-// Per-child constants of the distance test: the sphere centre b and radius
-// w of the box, |b|^2, and w^2 plus the rounding margin below.
-struct ChildBox {
-    float bx, by, bz, bb, w2x, wsq;
+// Per-child constants of the distance test: twice the sphere centre b of
+// the box, |b|^2, the radius w, and w^2 plus the rounding margin below (two
+// float4: the children of a parent sit contiguously in the per-level table,
+// read with scalar loads).
+struct __align__(16) ChildBox {
+    float bx2, by2, bz2, bb, w, wsq, pad0, pad1;
 };
 __device__ __forceinline__ ChildBox make_child(float4 b, bool real) {
     ChildBox k;
-    k.bx = real ? b.x : 0.0f;
-    k.by = real ? b.y : 0.0f;
-    k.bz = real ? b.z : 0.0f;
+    k.bx2 = real ? 2.0f * b.x : 0.0f;
+    k.by2 = real ? 2.0f * b.y : 0.0f;
+    k.bz2 = real ? 2.0f * b.z : 0.0f;
     k.bb = real ? fmaf(b.z, b.z, fmaf(b.y, b.y, b.x * b.x)) : __builtin_inff();
-    k.w2x = real ? 2.0f * b.w : 0.0f;
+    k.w = real ? b.w : 0.0f;
     k.wsq = real ? fmaf(k.bb, 0x1p-18f, b.w * b.w) : 0.0f;
+    k.pad0 = k.pad1 = 0.0f;
     return k;
 }
 // Squared distance from a box's sphere centre b to the segment [0, p] of
-// candidate c: with t = clamp(b . p / |p|^2, 0, 1) the nearest point is t p
-// and |b - t p|^2 = |b|^2 - t (2 b . p - t |p|^2).  The expansion cancels: its
+// candidate c against the candidate's threshold thr2: with
+// t = clamp(b . p / |p|^2, 0, 1) the nearest point is t p and
+// |b - t p|^2 = |b|^2 - t (2 b . p - t |p|^2).  The expansion cancels: its
 // rounding error stays below about 40 * 2^-24 |b|^2 (the dot product's, the
-// reciprocal's and the final sums'), so the threshold (w + slack)^2 =
-// w^2 + slack (2 w + slack) gets 2^-18 |b|^2 on top (in wsq), and the test
-// stays a superset of the pairs with a non-zero term.  A child outside the
-// tile grid has |b|^2 = +inf and never passes.
-__device__ __forceinline__ bool cand_hits(const ChildBox& k, const Cand& c) {
-    const float dp = fmaf(k.bz, c.z, fmaf(k.by, c.y, k.bx * c.x));
-    const float t = __builtin_amdgcn_fmed3f(dp * c.inv_mm, 0.0f, 1.0f);
-    const float r = fmaf(-t, c.mm, dp);  // b . p - t |p|^2
-    const float d2 = fmaf(-t, dp + r, k.bb);
-    const float thr2 = fmaf(c.slack, k.w2x + c.slack, k.wsq);
+// reciprocal's and the final sums'), so the threshold carries 2^-18 |b|^2 on
+// top of (w + slack)^2 (in wsq), and the test stays a superset of the pairs
+// with a non-zero term.  A child outside the tile grid has |b|^2 = +inf and
+// never passes.
+__device__ __forceinline__ bool cand_hits(const ChildBox& k, const Cand& c, float thr2) {
+    const float dp2 = fmaf(k.bz2, c.z, fmaf(k.by2, c.y, k.bx2 * c.x));  // 2 b . p
+    const float t = __builtin_amdgcn_fmed3f(dp2 * c.half_inv_mm, 0.0f, 1.0f);
+    const float d2 = fmaf(-t, fmaf(-t, c.mm, dp2), k.bb);
     return d2 < thr2;
+}
+// The threshold of candidate c shared by the children of a group: with the
+// group's largest radius and largest w^2 + margin it is at least each
+// child's own (w + slack)^2 + 2^-18 |b|^2, so the shared test only widens.
+__device__ __forceinline__ float cand_thr2(const Cand& c, float wmax, float wsqmax) {
+    return fmaf(c.slack, fmaf(2.0f, wmax, c.slack), wsqmax);
 }
 
 // Child c of parent P at this level (level 0: the box with id c).  Returns the
@@ -236,10 +247,13 @@ unsigned level_task_blocks(long long items_cap, const LevelDesc& L) {
     return blocks_for(std::max(1ll, items_cap * child_groups(L)), BG_WAVES);
 }
 
-// Point index of candidate e of an item's segment (0 past its end).
+// Point index of candidate e of an item's segment, e clamped to the last
+// one (an item holds at least one candidate).  The load is unconditional,
+// so a load of the next group stays in flight until its value is used.
 __device__ __forceinline__ uint32_t cand_index(int level, const uint32_t* __restrict__ parent_list, const ItemCtx& x,
                                                uint32_t e) {
-    return e < x.end ? (level ? parent_list[x.poff + e] : e) : 0u;
+    const uint32_t ec = min(e, x.end - 1u);
+    return level ? parent_list[x.poff + ec] : ec;
 }
 
 // One wavefront per task: a work item and one group of at most LV_CG of its
@@ -252,8 +266,9 @@ __device__ __forceinline__ uint32_t lv_task() { return blockIdx.x * (uint32_t)BG
 // Per-task, per-child counts of the candidates of the item's segment that
 // pass the child's test, and per candidate the children it passes (for the
 // fill).  NC (= LV_CG) bounds the group's children, nc of them real.  Lane
-// c sets up child c's test constants in the wavefront's LDS slice; the
-// per-child counts are wavefront-uniform, in scalar registers.
+// c sets up child c's test constants, which every lane then reads from lane
+// c: wavefront-uniform, they stay in scalar registers (as do the per-child
+// counts), which keeps the kernel's vector registers low for occupancy.
 template <int NC>
 __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConsts* __restrict__ fc, TileGeom tg,
                                                      LevelDesc L, float ls, const float4* __restrict__ pxyz,
@@ -262,8 +277,8 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
                                                      const uint32_t* __restrict__ parent_list,
                                                      uint32_t* __restrict__ item_base,
                                                      uint32_t* __restrict__ counts, uint8_t* __restrict__ masks,
-                                                     long long mask_stride, DevScalars* sc) {
-    __shared__ ChildBox cbs[BG_WAVES][NC];
+                                                     long long mask_stride, uint4* __restrict__ ictx,
+                                                     const ChildBox* __restrict__ ctab, DevScalars* sc) {
     const int m = fc->m;
     if (level == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         // the single parent's items, for the slot scan, the fill and the boxes
@@ -274,7 +289,7 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
     }
     if (sc->overflow) return;
     const int ncg = child_groups(L);
-    const uint32_t task = lv_task(), item = task / (uint32_t)ncg, grp = task % (uint32_t)ncg;
+    const uint32_t task = uniform(lv_task()), item = task / (uint32_t)ncg, grp = task % (uint32_t)ncg;
     if (item >= (level ? item_base[L.nparents] : level0_items(L, m))) return;
     const int lane = threadIdx.x & 63;
     const int c0 = (int)grp * LV_CG, nc = min(NC, L.nch - c0);
@@ -284,8 +299,9 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
     // before is tested
     uint32_t i_next = cand_index(level, parent_list, x, x.begin + 64u + lane);
     float4 p_next = pxyz[cand_index(level, parent_list, x, x.begin + lane)];
-    ChildBox* cb = cbs[threadIdx.x >> 6];
-    if (lane < NC) {
+    ChildBox cb[NC];
+    if (level == 0) {
+        // one parent: lane c sets up child c, every lane reads it from lane c
         float4 b = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         bool real = false;
         if (lane < nc) {
@@ -296,9 +312,28 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
                 real = true;
             }
         }
-        cb[lane] = make_child(b, real);
+        const ChildBox k = make_child(b, real);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            cb[c].bx2 = lane_float(k.bx2, c);
+            cb[c].by2 = lane_float(k.by2, c);
+            cb[c].bz2 = lane_float(k.bz2, c);
+            cb[c].bb = lane_float(k.bb, c);
+            cb[c].w = lane_float(k.w, c);
+            cb[c].wsq = lane_float(k.wsq, c);
+        }
+    } else {
+        // the previous level's boxes kernel wrote the parent's children in order
+        const ChildBox* tp = ctab + (size_t)x.P * (size_t)L.nch + (size_t)c0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) cb[c] = tp[c];
     }
-    bg_wave_sync();
+    float wmax = cb[0].w, wsqmax = cb[0].wsq;
+#pragma unroll
+    for (int c = 1; c < NC; ++c) {
+        wmax = fmaxf(wmax, cb[c].w);
+        wsqmax = fmaxf(wsqmax, cb[c].wsq);
+    }
     uint32_t cnt[NC];  // candidates passing child c (wavefront-uniform)
 #pragma unroll
     for (int c = 0; c < NC; ++c) cnt[c] = 0;
@@ -306,15 +341,14 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
     for (uint32_t base = x.begin; base < x.end; base += 64u) {
         const float4 p = p_next;
         const bool in = base + lane < x.end;
-        if (base + 64u < x.end) {
-            p_next = pxyz[i_next];
-            i_next = cand_index(level, parent_list, x, base + 128u + lane);
-        }
+        p_next = pxyz[i_next];
+        i_next = cand_index(level, parent_list, x, base + 128u + lane);
         const Cand cd = make_cand(p, in);
+        const float thr2 = cand_thr2(cd, wmax, wsqmax);
         uint32_t hits = 0;  // bit c: passes child c (kept for the fill)
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-            const bool hit = cand_hits(cb[c], cd);
+            const bool hit = cand_hits(cb[c], cd, thr2);
             hits |= hit ? (1u << c) : 0u;
             cnt[c] += (uint32_t)__popcll(__ballot(hit));
         }
@@ -324,58 +358,72 @@ __global__ __launch_bounds__(TPB) void k_level_count(int level, const FrameConst
 #pragma unroll
     for (int c = 0; c < NC; ++c) mine = lane == c ? cnt[c] : mine;
     if (lane < nc) counts[x.ib * L.nch + (c0 + lane) * x.nsegP + x.seg] = mine;
+    // the item's context for the fill (slot of child 0, segments of the
+    // parent, first parent-list position, candidates)
+    if (grp == 0 && lane == 0)
+        ictx[item] = make_uint4(x.ib * L.nch + x.seg, x.nsegP, (level ? x.poff : 0u) + x.begin, x.end - x.begin);
 }
 
 // Ordered fill: each candidate passing child c goes to that child's list at
 // the scanned offset of (item, c) plus the number of earlier passing
 // candidates of the item.  The task's wavefront walks the segment in order,
 // 64 candidates at a time; a candidate's position is the running offset of
-// child c (wavefront-uniform) plus its rank in the group's ballot.
-template <int NC>
-__global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts* __restrict__ fc, TileGeom tg,
-                                                    LevelDesc L, float ls, const float4* __restrict__ pxyz,
-                                                    const uint32_t* __restrict__ parent_off,
-                                                    const uint32_t* __restrict__ parent_len,
-                                                    const uint32_t* __restrict__ parent_list,
+// child c (wavefront-uniform) plus its rank in the group's ballot.  The
+// item's bounds come from the context the count kernel recorded, loaded
+// together with the item count and the overflow flag, so the candidates'
+// loads start after two round trips to memory.
+// LISTED: the candidates come from a parent list (levels 1, 2), else they
+// are the points themselves (level 0).
+template <int NC, bool LISTED>
+__global__ __launch_bounds__(TPB) void k_level_fill(int level, LevelDesc L, const uint32_t* __restrict__ parent_list,
                                                     const uint32_t* __restrict__ item_base,
+                                                    const uint4* __restrict__ ictx, long long items_cap,
                                                     const uint32_t* __restrict__ slot_off,
                                                     const uint8_t* __restrict__ masks, long long mask_stride,
                                                     uint32_t* __restrict__ list, long long list_cap,
                                                     DevScalars* sc) {
-    if (sc->overflow) return;
-    const int m = fc->m;
+    const int ncg = child_groups(L);
+    const uint32_t task = lv_task(), item = task / (uint32_t)ncg, grp = task % (uint32_t)ncg;
+    const uint32_t overflow = sc->overflow;
     const uint32_t n_items = item_base[L.nparents];
+    const uint4 cx = (long long)item < items_cap ? ictx[item] : make_uint4(0u, 0u, 0u, 0u);
+    if (overflow) return;
     const uint32_t total = slot_off[n_items * (uint32_t)L.nch];
+    const bool live = item < n_items;
+    const int lane = threadIdx.x & 63;
+    const int c0 = (int)grp * LV_CG, nc = min(NC, L.nch - c0);
+    // cx: slot of (item, child 0), segments of the parent, first parent-list
+    // position, candidates
+    // (a task past the last item reads position 0, in bounds, and stores nothing)
+    const uint32_t s0 = cx.x, nsegP = cx.y, abs0 = live ? cx.z : 0u, len = live ? cx.w : 0u;
+    // the children each candidate passes, as the count kernel found them; the
+    // next group's indices and masks load while this one is placed
+    const uint8_t* gmask = masks + (size_t)grp * (size_t)mask_stride + abs0;
+    const uint32_t* gidx = parent_list + abs0;
+    // candidate i of the item, i clamped to the last (a live item holds one
+    // at least); unconditional loads stay in flight until their use
+    const uint32_t last = len ? len - 1u : 0u;
+    uint32_t idx_next = LISTED ? gidx[min((uint32_t)lane, last)] : abs0 + (uint32_t)lane;
+    uint32_t hits_next = gmask[min((uint32_t)lane, last)];
+    const uint32_t at0 = live && lane < nc ? slot_off[s0 + (uint32_t)(c0 + lane) * nsegP] : 0u;
     if ((long long)total > list_cap) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (task == 0 && lane == 0) {
             sc->level_total[level] = total;
             atomicOr(&sc->overflow_mask, 1u << level);
             atomicExch(&sc->overflow, 1u);
         }
         return;
     }
-    const int ncg = child_groups(L);
-    const uint32_t task = lv_task(), item = task / (uint32_t)ncg, grp = task % (uint32_t)ncg;
-    if (item >= n_items) return;
-    const int lane = threadIdx.x & 63;
-    const int c0 = (int)grp * LV_CG, nc = min(NC, L.nch - c0);
-    const ItemCtx x = item_ctx(level, L, item, item_base, parent_off, parent_len, m);
-    // the children each candidate passes, as the count kernel found them; the
-    // next group's indices and masks load while this one is placed
-    const uint8_t* gmask = masks + (size_t)grp * (size_t)mask_stride + (level ? x.poff : 0u);
-    uint32_t idx_next = cand_index(level, parent_list, x, x.begin + lane);
-    uint32_t hits_next = x.begin + lane < x.end ? gmask[x.begin + lane] : 0u;
-    const uint32_t at0 = lane < nc ? slot_off[x.ib * L.nch + (c0 + lane) * x.nsegP + x.seg] : 0u;
+    if (task == 0 && lane == 0) sc->level_total[level] = total;
+    if (!live) return;
     uint32_t at[NC];  // where child c's next candidate goes (wavefront-uniform)
 #pragma unroll
     for (int c = 0; c < NC; ++c) at[c] = lane_word(at0, (uint32_t)c);
-    for (uint32_t base = x.begin; base < x.end; base += 64u) {
-        const uint32_t idx = idx_next, hits = hits_next;
-        const uint32_t en = base + 64u + lane;
-        if (base + 64u < x.end) {
-            idx_next = cand_index(level, parent_list, x, en);
-            hits_next = en < x.end ? gmask[en] : 0u;
-        }
+    for (uint32_t i = 0; i < len; i += 64u) {
+        const uint32_t idx = idx_next, hits = i + lane < len ? hits_next : 0u;
+        const uint32_t en = min(i + 64u + lane, last);
+        idx_next = LISTED ? gidx[en] : abs0 + en;
+        hits_next = gmask[en];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const bool hit = (hits >> c) & 1u;
@@ -384,7 +432,6 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
             at[c] += (uint32_t)__popcll(bal);
         }
     }
-    if (task == 0 && lane == 0) sc->level_total[level] = total;
 }
 
 // Each box's list (offset and length in the level's list).  A box of level
@@ -392,15 +439,30 @@ __global__ __launch_bounds__(TPB) void k_level_fill(int level, const FrameConsts
 // items per parent (next_nseg: ceil(len / next_seg)); at the tile level it
 // writes each tile's gather items instead: one for a list of at most
 // BG_HEAVY_LEN candidates, else one per point chunk its list spans.
+// Below the tile level, thread (box, c) also writes the test constants of
+// child c of the box at the next level (next_tab, parent-major) for that
+// level's count kernel; the box's own work is done by its thread c = 0.
 __global__ __launch_bounds__(TPB) void k_level_boxes(int level, LevelDesc L, const uint32_t* __restrict__ item_base,
                                                      const uint32_t* __restrict__ slot_off,
                                                      const uint32_t* __restrict__ list,
                                                      uint32_t* __restrict__ box_off, uint32_t* __restrict__ box_len,
                                                      uint32_t* __restrict__ next_nseg, int next_seg,
                                                      uint32_t* __restrict__ heavy_cnt, uint32_t* __restrict__ light_cnt,
+                                                     const FrameConsts* __restrict__ fc, TileGeom tg, LevelDesc Ln,
+                                                     float ls, ChildBox* __restrict__ next_tab,
                                                      const DevScalars* sc) {
-    const int box = blockIdx.x * TPB + threadIdx.x;
+    const int nchn = next_tab ? Ln.nch : 1;
+    const long long gid = (long long)blockIdx.x * TPB + threadIdx.x;
+    const int box = (int)(gid / nchn), cn = (int)(gid % nchn);
     if (box >= L.nbx * L.nby * L.nbz) return;
+    if (next_tab) {
+        int t0[3], t1[3];
+        const int cb = child_box(level + 1, Ln, tg, box, cn, t0, t1);
+        const float4 b = cb >= 0 ? box_sphere(fc->g, ls, t0[0], t0[1], t0[2], t1[0], t1[1], t1[2])
+                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        next_tab[gid] = make_child(b, cb >= 0);
+        if (cn != 0) return;
+    }
     uint32_t off = 0, len = 0;
     if (!sc->overflow) {
         int P, c;
@@ -448,16 +510,20 @@ hipError_t launch_build_levels(hipStream_t st, const FrameConsts* fc, const Tile
             (void)launch_scan_u32(st, Bl.nseg, Bl.item_base, Bl.bsum, Ll.nparents, nullptr, 1, &sc->level_items[level]);
         const unsigned grid = level_task_blocks(Bl.items_cap, Ll);
         hipLaunchKernelGGL(k_level_count<LV_CG>, dim3(grid), dim3(TPB), 0, st, level, fc, tg, Ll, ls, pxyz, poff, plen,
-                           plist, Bl.item_base, Bl.counts, Bl.mask, Bl.mask_stride, sc);
+                           plist, Bl.item_base, Bl.counts, Bl.mask, Bl.mask_stride, Bl.ictx,
+                           reinterpret_cast<const ChildBox*>(Bl.ctab), sc);
         (void)launch_scan_u32(st, Bl.counts, Bl.slot_off, Bl.bsum, Bl.slot_cap, Bl.item_base + Ll.nparents,
                               (uint32_t)Ll.nch, nullptr);
-        hipLaunchKernelGGL(k_level_fill<LV_CG>, dim3(grid), dim3(TPB), 0, st, level, fc, tg, Ll, ls, pxyz, poff, plen,
-                           plist, Bl.item_base, Bl.slot_off, Bl.mask, Bl.mask_stride, Bl.list, Bl.list_cap, sc);
+        auto fill = level ? k_level_fill<LV_CG, true> : k_level_fill<LV_CG, false>;
+        hipLaunchKernelGGL(fill, dim3(grid), dim3(TPB), 0, st, level, Ll, plist, Bl.item_base, Bl.ictx, Bl.items_cap,
+                           Bl.slot_off, Bl.mask, Bl.mask_stride, Bl.list, Bl.list_cap, sc);
         const bool last = level == 2;
-        hipLaunchKernelGGL(k_level_boxes, dim3(blocks_for((long long)Ll.nbx * Ll.nby * Ll.nbz)), dim3(TPB), 0, st, level,
-                           Ll, Bl.item_base, Bl.slot_off, Bl.list, Bl.box_off, Bl.box_len,
-                           last ? nullptr : B[level + 1].nseg, last ? 1 : L[level + 1].seg,
-                           last ? I.heavy_cnt : nullptr, last ? I.light_cnt : nullptr, sc);
+        const LevelDesc& Ln = L[last ? level : level + 1];
+        const long long nthreads = (long long)Ll.nbx * Ll.nby * Ll.nbz * (last ? 1 : Ln.nch);
+        hipLaunchKernelGGL(k_level_boxes, dim3(blocks_for(nthreads)), dim3(TPB), 0, st, level, Ll, Bl.item_base,
+                           Bl.slot_off, Bl.list, Bl.box_off, Bl.box_len, last ? nullptr : B[level + 1].nseg,
+                           last ? 1 : Ln.seg, last ? I.heavy_cnt : nullptr, last ? I.light_cnt : nullptr, fc, tg, Ln,
+                           ls, last ? nullptr : reinterpret_cast<ChildBox*>(B[level + 1].ctab), sc);
     }
     return hipGetLastError();
 }

@@ -139,9 +139,11 @@ struct LevelBufs {
     uint32_t *nseg, *item_base;    // nparents, nparents + 1
     uint32_t *counts, *slot_off;   // slot_cap, slot_cap + 1
     uint32_t *box_off, *box_len;   // nboxes
+    float4* ctab;                  // levels 1, 2: nparents * nch children's test constants (two float4 each)
     uint32_t* list;                // list_cap
     uint32_t* bsum;                // scan scratch
     uint8_t* mask;                 // per child group and parent-list position: the children a candidate passes
+    uint4* ictx;                   // items_cap: each work item's context, from the count kernel to the fill
     long long items_cap, slot_cap, list_cap;
     long long mask_stride;         // parent-list capacity (one mask byte per position per child group)
 };
