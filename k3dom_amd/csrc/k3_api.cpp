@@ -210,11 +210,11 @@ k3dom_status alloc_group_fixed(k3dom_handle* h, k3dom_handle::GroupKi& G) {
     }
     const size_t T = (size_t)h->ntiles;
     ItemBufs& I = G.ibuf;
-    HIP_TRY(h, dalloc(h, &I.heavy_cnt, T));
-    HIP_TRY(h, dalloc(h, &I.light_cnt, T));
-    HIP_TRY(h, dalloc(h, &I.heavy_off, T + 1));
-    HIP_TRY(h, dalloc(h, &I.light_off, T + 1));
-    HIP_TRY(h, dalloc_zero(h, &I.bsum, (size_t)scan_scratch_words((long long)T)));
+    HIP_TRY(h, dalloc(h, &I.heavy_cnt, 2 * T));
+    I.light_cnt = I.heavy_cnt + T;
+    HIP_TRY(h, dalloc(h, &I.heavy_off, 2 * T + 1));
+    I.light_off = I.heavy_off + T;
+    HIP_TRY(h, dalloc_zero(h, &I.bsum, (size_t)scan_scratch_words(2 * (long long)T)));
     HIP_TRY(h, dalloc(h, &I.tile_partial, 2 * T));
     HIP_TRY(h, dalloc(h, &I.qheads, BG_QWORDS));
     HIP_TRY(h, dalloc(h, &G.del_occ, (size_t)h->V));
@@ -508,8 +508,9 @@ k3dom_status enqueue_rest(k3dom_handle* h, bool normal, const float* dx, const f
     if (s != K3DOM_OK) return s;
     mark(h, ST_BGKI);
     mark(h, ST_BGKI_KERNEL);
+    // (build_lists left the work-queue heads zeroed)
     HIP_TRY(h, launch_bgki(h->stream, h->gather_blocks, h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free, h->ibuf,
-                           h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc, h->gather_waves));
+                           h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc, h->gather_waves, true));
     mark(h, EV_BGKI_END);
     return enqueue_after_gather(h, normal);
 }
@@ -956,11 +957,11 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
     }
     const size_t T = (size_t)h->ntiles;
     ItemBufs& I = h->ibuf;
-    CREATE_TRY(dalloc(h, &I.heavy_cnt, T));
-    CREATE_TRY(dalloc(h, &I.light_cnt, T));
-    CREATE_TRY(dalloc(h, &I.heavy_off, T + 1));
-    CREATE_TRY(dalloc(h, &I.light_off, T + 1));
-    CREATE_TRY(dalloc_zero(h, &I.bsum, (size_t)scan_scratch_words((long long)T)));
+    CREATE_TRY(dalloc(h, &I.heavy_cnt, 2 * T));
+    I.light_cnt = I.heavy_cnt + T;
+    CREATE_TRY(dalloc(h, &I.heavy_off, 2 * T + 1));
+    I.light_off = I.heavy_off + T;
+    CREATE_TRY(dalloc_zero(h, &I.bsum, (size_t)scan_scratch_words(2 * (long long)T)));
     CREATE_TRY(dalloc(h, &I.tile_partial, 2 * T));
     CREATE_TRY(dalloc(h, &I.qheads, BG_QWORDS));
     {
@@ -1211,7 +1212,7 @@ k3dom_status stage_evidence(k3dom_handle* h, bool normal) {
         k3dom_status s = build_lists(h, h->fc, h->d_pts, h->d_pts + cap, h->d_pts + 2 * cap);
         if (s != K3DOM_OK) return s;
         HIP_TRY(h, launch_bgki(h->stream, h->gather_blocks, h->fc, h->tg, h->p.ls, h->sigma_occ, h->sigma_free,
-                               h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc, h->gather_waves));
+                               h->ibuf, h->lbuf[2], h->rec, h->del_occ, h->del_free, h->sc, h->gather_waves, true));
         s = tail_mass(h, normal);
         if (s == K3DOM_OK && !normal) s = tail_newborn(h, false);
         if (s != K3DOM_OK) return s;

@@ -538,24 +538,42 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t 
     return lo;
 }
 
+// heavy_off / light_off: one exclusive scan over the tiles' split-item
+// counts followed by their whole-tile counts, so light_off[t] already counts
+// the ntiles' split items before it (they come first in items[]).
+// k_emit_light also publishes both totals and zeroes the gather's queue
+// heads (so a gather launched right after needs no reset launch).
 __global__ __launch_bounds__(TPB) void k_emit_light(int ntiles, const uint32_t* __restrict__ toff,
                                                     const uint32_t* __restrict__ tlen,
                                                     const uint32_t* __restrict__ light_cnt,
+                                                    const uint32_t* __restrict__ heavy_off,
                                                     const uint32_t* __restrict__ light_off, uint4* __restrict__ items,
                                                     uint32_t* __restrict__ tile_partial, long long heavy_cap,
-                                                    const DevScalars* sc) {
-    int t = blockIdx.x * TPB + threadIdx.x;
+                                                    uint32_t* __restrict__ qheads, DevScalars* sc) {
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    const uint32_t H = heavy_off[ntiles];
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            sc->n_heavy = H;
+            sc->n_light = light_off[ntiles] - H;
+        }
+        if (threadIdx.x < BG_NQ) qheads[threadIdx.x * BG_QSTRIDE] = 0u;
+    }
     // items[] holds ntiles + heavy_cap entries: with more split items than
     // that, k_emit_heavy flags the overflow and nothing may be written here
-    if (t >= ntiles || sc->overflow || (long long)sc->n_heavy > heavy_cap) return;
+    if (t >= ntiles || sc->overflow || (long long)H > heavy_cap) return;
     if (light_cnt[t]) {
         uint32_t o = toff[t];
-        items[sc->n_heavy + light_off[t]] = make_uint4((uint32_t)t, o, o + tlen[t], BG_DIRECT);
+        items[light_off[t]] = make_uint4((uint32_t)t, o, o + tlen[t], BG_DIRECT);
         tile_partial[2 * t] = 0;
         tile_partial[2 * t + 1] = 0;
     }
 }
 
+// Split item h = (tile t, its k-th point chunk c): candidates [b, e) of the
+// tile's list, b the first entry of chunk c or later.  Its end e is the next
+// item's b (or the list's end for the tile's last chunk), so each item
+// searches once and writes its predecessor's end; k = 0 starts the list.
 __global__ __launch_bounds__(TPB) void k_emit_heavy(int ntiles, const uint32_t* __restrict__ toff,
                                                     const uint32_t* __restrict__ tlen,
                                                     const uint32_t* __restrict__ tlist,
@@ -564,7 +582,7 @@ __global__ __launch_bounds__(TPB) void k_emit_heavy(int ntiles, const uint32_t* 
                                                     uint32_t* __restrict__ tile_partial, long long heavy_cap,
                                                     DevScalars* sc) {
     if (sc->overflow) return;
-    const uint32_t H = sc->n_heavy;
+    const uint32_t H = heavy_off[ntiles];
     if ((long long)H > heavy_cap) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             atomicOr(&sc->overflow_mask, 1u << 4);
@@ -581,24 +599,27 @@ __global__ __launch_bounds__(TPB) void k_emit_heavy(int ntiles, const uint32_t* 
         else hi = mid - 1;
     }
     const int t = lo;
-    const uint32_t k = h - heavy_off[t];
+    const uint32_t k = h - heavy_off[t], cnt = heavy_cnt[t];
     const uint32_t o = toff[t], len = tlen[t];
     const uint32_t c = (tlist[o] >> K3_BGKI_CHUNK_LOG2) + k;
-    const uint32_t b = lower_bound_u32(tlist, o, o + len, c << K3_BGKI_CHUNK_LOG2);
-    const uint32_t e = lower_bound_u32(tlist, o, o + len, (c + 1) << K3_BGKI_CHUNK_LOG2);
-    items[h] = make_uint4((uint32_t)t, b, e, h);
+    const uint32_t b = k ? lower_bound_u32(tlist, o, o + len, c << K3_BGKI_CHUNK_LOG2) : o;
+    uint32_t* it = reinterpret_cast<uint32_t*>(items + h);
+    *reinterpret_cast<uint2*>(it) = make_uint2((uint32_t)t, b);
+    it[3] = h;
+    if (k) it[-2] = b;  // the previous item's end (items[h - 1].z)
+    if (k + 1 == cnt) it[2] = o + len;
     if (k == 0) {
         tile_partial[2 * t] = h;
-        tile_partial[2 * t + 1] = heavy_cnt[t];
+        tile_partial[2 * t + 1] = cnt;
     }
 }
 
 hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& T, const ItemBufs& I, DevScalars* sc) {
-    // (the tiles' item counts come from the last level's boxes)
-    (void)launch_scan_u32(st, I.heavy_cnt, I.heavy_off, I.bsum, ntiles, nullptr, 1, &sc->n_heavy);
-    (void)launch_scan_u32(st, I.light_cnt, I.light_off, I.bsum, ntiles, nullptr, 1, &sc->n_light);
+    // (the tiles' item counts come from the last level's boxes; light_cnt
+    // follows heavy_cnt and light_off heavy_off, so one scan covers both)
+    (void)launch_scan_u32(st, I.heavy_cnt, I.heavy_off, I.bsum, 2ll * ntiles, nullptr, 1, nullptr);
     hipLaunchKernelGGL(k_emit_light, dim3(blocks_for(ntiles)), dim3(TPB), 0, st, ntiles, T.box_off, T.box_len,
-                       I.light_cnt, I.light_off, I.items, I.tile_partial, I.heavy_cap, sc);
+                       I.light_cnt, I.heavy_off, I.light_off, I.items, I.tile_partial, I.heavy_cap, I.qheads, sc);
     hipLaunchKernelGGL(k_emit_heavy, dim3(blocks_for(std::max(1ll, I.heavy_cap))), dim3(TPB), 0, st, ntiles,
                        T.box_off, T.box_len, T.list, I.heavy_cnt, I.heavy_off, I.items, I.tile_partial, I.heavy_cap,
                        sc);
@@ -934,7 +955,8 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
 // home queue one at a time with one atomic each; once the home queue is
 // empty it moves on to the following queues, reading a queue's head before
 // claiming from it so that an emptied queue costs no atomic.  The heads are
-// zeroed by the gather's launcher (k_reset_queues) before every launch.
+// zeroed before every launch: by the gather's launcher (k_reset_queues), or,
+// right after the work items are emitted, by k_emit_light.
 __device__ __forceinline__ uint32_t bg_queue_len(uint32_t n_items, uint32_t q) {
     return n_items > q ? (n_items - q + BG_NQ - 1u) / BG_NQ : 0u;
 }
@@ -1005,8 +1027,8 @@ BgArgs make_bg_args(const FrameConsts* fc, const TileGeom& tg, float ls, float s
 }
 
 template <bool GRAD>
-static hipError_t launch_bgki_w(hipStream_t st, int grid_blocks, int waves, const BgArgs& a) {
-    hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(BG_NQ), 0, st, a.qheads);
+static hipError_t launch_bgki_w(hipStream_t st, int grid_blocks, int waves, const BgArgs& a, bool queues_zeroed) {
+    if (!queues_zeroed) hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(BG_NQ), 0, st, a.qheads);
     if (waves == 4) hipLaunchKernelGGL((k_bgki<GRAD, 4>), dim3(grid_blocks), dim3(256), 0, st, a);
     else if (waves == 16) hipLaunchKernelGGL((k_bgki<GRAD, 16>), dim3(grid_blocks), dim3(1024), 0, st, a);
     else hipLaunchKernelGGL((k_bgki<GRAD, 8>), dim3(grid_blocks), dim3(512), 0, st, a);
@@ -1015,10 +1037,11 @@ static hipError_t launch_bgki_w(hipStream_t st, int grid_blocks, int waves, cons
 
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                        float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T, const K3PointRec* rec,
-                       float* del_occ, float* del_free, DevScalars* sc, int waves) {
+                       float* del_occ, float* del_free, DevScalars* sc, int waves, bool queues_zeroed) {
     return launch_bgki_w<false>(st, grid_blocks, waves,
                                 make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, nullptr,
-                                             nullptr, sc));
+                                             nullptr, sc),
+                                queues_zeroed);
 }
 hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T,
@@ -1026,7 +1049,8 @@ hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* 
                             DevScalars* sc, int waves) {
     return launch_bgki_w<true>(st, grid_blocks, waves,
                                make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, dls_occ,
-                                            dls_free, sc));
+                                            dls_free, sc),
+                               false);
 }
 
 // ---------------------------------------------------------------- many maps

@@ -165,7 +165,10 @@ constexpr uint32_t BG_QSTRIDE = 32;
 constexpr size_t BG_QWORDS = (size_t)BG_NQ * BG_QSTRIDE;
 
 struct ItemBufs {
-    uint32_t *heavy_cnt, *light_cnt, *heavy_off, *light_off;  // ntiles (+1)
+    // per tile: split-item and whole-tile counts, one array of 2 ntiles
+    // (light_cnt = heavy_cnt + ntiles), and their exclusive scan (2 ntiles + 1,
+    // light_off = heavy_off + ntiles)
+    uint32_t *heavy_cnt, *light_cnt, *heavy_off, *light_off;
     uint32_t* bsum;
     uint4* items;             // ntiles + heavy_cap
     uint32_t* tile_partial;   // 2 * ntiles
@@ -178,7 +181,8 @@ struct ItemBufs {
 // tiles' gather-item counts into I.heavy_cnt / I.light_cnt.
 hipError_t launch_build_levels(hipStream_t st, const FrameConsts* fc, const TileGeom& tg, const LevelDesc* L,
                                LevelBufs* B, const ItemBufs& I, float ls, const float4* pxyz, DevScalars* sc);
-// The gather's work items from the tiles' lists and item counts.
+// The gather's work items from the tiles' lists and item counts; also
+// zeroes the gather's work-queue heads.
 hipError_t launch_build_items(hipStream_t st, int ntiles, const LevelBufs& tiles, const ItemBufs& I,
                               DevScalars* sc);
 // Arguments of the kernel-inference gather over one map.
@@ -209,9 +213,11 @@ hipError_t launch_merge_group_scalars(hipStream_t st, DevScalars* sc, const DevS
 // single sigma in a plain frame).  Split items need I.partial with 128 floats
 // per item (256 for the _grad variants).
 // waves: wavefronts per block of the gather (4, 8 or 16)
+// queues_zeroed: the work-queue heads are already zero (launch_build_items
+// was told to zero them and nothing has claimed since).
 hipError_t launch_bgki(hipStream_t st, int grid_blocks, const FrameConsts* fc, const TileGeom& tg, float ls,
                        float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& tiles, const K3PointRec* rec,
-                       float* del_occ, float* del_free, DevScalars* sc, int waves = 8);
+                       float* del_occ, float* del_free, DevScalars* sc, int waves = 8, bool queues_zeroed = false);
 // Evidence of the split tiles: chunk sums added in chunk order, times the
 // sigma factors (direct items are written by the gather itself).
 hipError_t launch_bgki_combine(hipStream_t st, int ntiles, const GridGeom& g, const TileGeom& tg, const ItemBufs& I,
