@@ -270,14 +270,19 @@ __global__ __launch_bounds__(1024) void k_scan_small_u32(const uint32_t* in, con
 // publishes its tile sum at once, then wavefront 0 reads the status words of
 // the preceding tiles, 64 at a time, adding tile sums back to the nearest
 // tile that has published its inclusive prefix, and publishes its own.  A
-// status word is (flag << 32) | value, flag 1 = tile sum, 2 = inclusive
-// prefix, 0 = not yet written.  The last block to finish clears the status
-// words and both counters, so the scratch is ready for the next launch (it
-// must be zero before the first one).
-// Scratch (u32 words): [0] ticket, [1] finished blocks, [2..] u64 statuses.
+// status word is (epoch << 34) | (flag << 32) | value: flag 1 = tile sum,
+// 2 = inclusive prefix; a word of another launch's epoch counts as not yet
+// written.  The 64-bit ticket counter holds (epoch << 32) | next tile; the
+// block taking a launch's last ticket moves it to the next epoch, tile 0
+// (every other block of the launch has taken its ticket by then), so nothing
+// needs clearing between launches (the scratch must be zero before the
+// first one).
+// Scratch (u32 words): [0, 1] the ticket counter, [2..] u64 statuses.
 constexpr int LB_ITEMS = 16;
 constexpr int LB_TILE = TPB * LB_ITEMS;
 constexpr uint64_t LB_SUM = 1ull << 32, LB_PREFIX = 2ull << 32;
+constexpr int LB_EPOCH_SHIFT = 34;
+constexpr uint64_t LB_EPOCH_MASK = (1ull << 30) - 1ull;
 
 static inline long long lb_tiles(long long n_max) { return std::max(1ll, (n_max + LB_TILE - 1) / LB_TILE); }
 
@@ -285,15 +290,23 @@ __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, c
                                                            uint32_t n_mul, uint32_t n_host, uint32_t* out,
                                                            uint32_t* __restrict__ total, uint32_t* __restrict__ scr,
                                                            uint32_t nblocks) {
-    __shared__ uint32_t s_tile, s_excl, s_last;
+    __shared__ uint32_t s_tile, s_excl;
+    __shared__ uint64_t s_tag;
     __shared__ uint32_t lds4[4];
-    uint32_t* ticket = scr;
-    uint32_t* finished = scr + 1;
+    unsigned long long* ticket = reinterpret_cast<unsigned long long*>(scr);
     uint64_t* status = reinterpret_cast<uint64_t*>(scr + 2);
     const int t = threadIdx.x, lane = t & 63;
-    if (t == 0) s_tile = atomicAdd(ticket, 1u);
+    if (t == 0) {
+        const unsigned long long tk = atomicAdd(ticket, 1ull);
+        const uint64_t epoch = tk >> 32;
+        const uint32_t tile = (uint32_t)tk;
+        if (tile == nblocks - 1u) atomicExch(ticket, (unsigned long long)((epoch + 1ull) << 32));
+        s_tile = tile;
+        s_tag = (epoch & LB_EPOCH_MASK) << LB_EPOCH_SHIFT;
+    }
     __syncthreads();
     const uint32_t tile = s_tile;
+    const uint64_t tag = s_tag;
     const uint32_t n = scan_count(n_dev, n_mul, n_host);
     const uint32_t ntiles = (n + LB_TILE - 1) / LB_TILE;
     if (tile < ntiles) {
@@ -308,22 +321,23 @@ __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, c
         uint32_t tot;
         const uint32_t x = block_excl_scan_u32(sum, lds4, &tot);
         if (t == 0)
-            __hip_atomic_store(&status[tile], (tile == 0 ? LB_PREFIX : LB_SUM) | tot, __ATOMIC_RELAXED,
+            __hip_atomic_store(&status[tile], tag | (tile == 0 ? LB_PREFIX : LB_SUM) | tot, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         if (t < 64) {
             uint32_t excl = 0;
             for (int j = (int)tile - 1; j >= 0; j -= 64) {
                 const int jj = j - lane;
-                uint64_t st = LB_PREFIX;  // before tile 0: an empty prefix
+                uint64_t st = tag | LB_PREFIX;  // before tile 0: an empty prefix
                 if (jj >= 0) {
                     st = __hip_atomic_load(&status[jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     // the tile of jj was taken before this one, so its sum is on its way
-                    while ((st >> 32) == 0) {
+                    while ((st & ~((1ull << 32) - 1ull)) != (tag | LB_SUM) &&
+                           (st & ~((1ull << 32) - 1ull)) != (tag | LB_PREFIX)) {
                         __builtin_amdgcn_s_sleep(1);
                         st = __hip_atomic_load(&status[jj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
-                const uint64_t pm = __ballot((st >> 32) == 2);
+                const uint64_t pm = __ballot((st & (3ull << 32)) == LB_PREFIX);
                 const int stop = pm ? __builtin_ctzll(pm) : 64;
                 uint32_t add = lane <= stop ? (uint32_t)st : 0u;
                 for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o);
@@ -332,7 +346,7 @@ __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, c
             }
             if (t == 0) {
                 if (tile > 0)
-                    __hip_atomic_store(&status[tile], LB_PREFIX | (excl + tot), __ATOMIC_RELAXED,
+                    __hip_atomic_store(&status[tile], tag | LB_PREFIX | (excl + tot), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                 s_excl = excl;
             }
@@ -351,20 +365,6 @@ __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, c
     } else if (ntiles == 0 && tile == 0 && t == 0) {
         out[0] = 0;
         if (total) *total = 0;
-    }
-    // the last block to finish resets the scratch for the next launch
-    if (t == 0) {
-        __threadfence();
-        s_last = atomicAdd(finished, 1u) == nblocks - 1;
-    }
-    __syncthreads();
-    if (s_last) {
-        for (uint32_t i = t; i < nblocks; i += TPB)
-            __hip_atomic_store(&status[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (t == 0) {
-            __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(finished, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
     }
 }
 
