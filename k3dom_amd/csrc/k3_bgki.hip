@@ -976,10 +976,12 @@ __global__ void k_reset_queues(uint32_t* qheads) {
     if (threadIdx.x < BG_NQ) qheads[threadIdx.x * BG_QSTRIDE] = 0u;
 }
 
-// The forward gather is held to 64 VGPRs, so that a SIMD runs 8 of its
-// wavefronts (a few per-item values spill; the candidate loops do not).
+// The gather is held to 64 VGPRs, so that a SIMD runs 8 of its wavefronts
+// (a few per-item values spill; the candidate loops do not).  With the
+// length-scale derivative (GRAD) the bound holds in blocks of 8 or 16
+// wavefronts, which is how launch_bgki_grad runs it.
 template <bool GRAD, int W>
-__global__ __launch_bounds__(W * 64, GRAD ? 1 : 8) void k_bgki(BgArgs a) {
+__global__ __launch_bounds__(W * 64, 8) void k_bgki(BgArgs a) {
     __shared__ float4 srec[W][64 * 3];
     __shared__ uint32_t smeta[W][64];
     __shared__ BgTable<GRAD> stab_s;
@@ -1047,6 +1049,14 @@ hipError_t launch_bgki_grad(hipStream_t st, int grid_blocks, const FrameConsts* 
                             float sigma_o, float sigma_f, const ItemBufs& I, const LevelBufs& T,
                             const K3PointRec* rec, float* del_occ, float* del_free, float* dls_occ, float* dls_free,
                             DevScalars* sc, int waves) {
+    // the same wavefronts per CU in blocks of at least 8: at 4 wavefronts per
+    // block the derivative gather keeps 75 VGPRs (6 wavefronts per SIMD), and
+    // the four-LiDAR window backward at 512x512x64 took 16.7 ms per frame
+    // against 14.4 ms in blocks of 8
+    if (waves < 8) {
+        grid_blocks = std::max(1, grid_blocks * waves / 8);
+        waves = 8;
+    }
     return launch_bgki_w<true>(st, grid_blocks, waves,
                                make_bg_args(fc, tg, ls, sigma_o, sigma_f, I, T, rec, del_occ, del_free, dls_occ,
                                             dls_free, sc),
