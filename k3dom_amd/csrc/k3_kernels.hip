@@ -1611,30 +1611,85 @@ __global__ __launch_bounds__(TPB) void k_resample(const uint32_t* __restrict__ a
                                                   int32_t* __restrict__ idx_out) {
     if (*abort_flag) return;
     const uint32_t frame = fc->frame;
-    int i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= N) return;
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    // a wavefront past the last output has nothing to do; the others keep
+    // every lane for the wavefront-wide steps (lanes past N repeat output N - 1)
+    if (i - lane >= N) return;
+    const int ic = min(i, N - 1);
     const uint64_t Q = *total;
     const int E = *Ep;
     const float W = (float)((double)Q * k3_pow2d(-E));
     const float nw = W / (float)N;
     int idx;
     if (Q == 0) {
-        idx = i;
+        idx = ic;
     } else {
         uint32_t r[4];
         k3_philox4x32_10(0u, frame, K3_STREAM_RESAMPLE, 0u, k0, k1, r);
         const double u = (double)r[0] * 0x1p-32;
         const double step = (double)Q / (double)N;
-        uint64_t T = (uint64_t)(((double)i + u) * step);
+        uint64_t T = (uint64_t)(((double)ic + u) * step);
         if (T > Q - 1) T = Q - 1;
-        int lo = 0, hi = L;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (cdf[mid] <= T) lo = mid + 1;
-            else hi = mid;
+        // The ancestor is the number of CDF entries <= T (the first entry
+        // above T).  T grows with the output, so lane 0's target is the
+        // wavefront's smallest: its count a0 comes from a 64-way sampled
+        // search (each round narrows [lo, hi] to one sample interval: four
+        // rounds for 2.2M entries instead of 21 dependent loads), and every
+        // lane's count from windows of 64 entries read from a0 on, each lane
+        // searching the window's sorted values by shuffles.
+        // (readfirstlane returns int: each word goes through uint32_t, not sign-extended)
+        const uint64_t T0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(T >> 32)) << 32) |
+                            (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)T);
+        uint32_t lo = 0, hi = (uint32_t)L;
+        while (hi - lo > 64u) {
+            const uint32_t st = (hi - lo + 63u) >> 6;
+            const uint32_t j = lo + (uint32_t)lane * st;
+            const uint64_t b = __ballot(j < hi && cdf[j] <= T0);
+            if (!b) {  // the entry at lo is already above T0
+                hi = lo;
+                break;
+            }
+            const uint32_t jk = lo + (uint32_t)(63 - __builtin_clzll(b)) * st;
+            lo = jk + 1u;
+            hi = min(jk + st, hi);
         }
-        idx = lo;
+        if (hi > lo) {
+            const uint32_t j = lo + (uint32_t)lane;
+            lo += (uint32_t)__popcll(__ballot(j < hi && cdf[j] <= T0));
+        }
+        uint32_t base = lo;
+        bool done = false;
+        uint32_t ans = 0;
+        for (int win = 0; win < 8; ++win, base += 64u) {
+            const uint32_t j = base + (uint32_t)lane;
+            const uint64_t v = j < (uint32_t)L ? cdf[j] : ~0ull;
+            const int vlo = (int)(uint32_t)v, vhi = (int)(uint32_t)(v >> 32);
+            uint32_t c = 0;  // window entries <= T (they increase with the lane)
+            for (uint32_t sz = 64; sz > 0; sz >>= 1) {
+                const uint32_t at = c + sz - 1u;
+                const int src = (int)min(at, 63u);
+                const uint64_t x = ((uint64_t)(uint32_t)__shfl(vhi, src) << 32) | (uint64_t)(uint32_t)__shfl(vlo, src);
+                if (at < 64u && x <= T) c += sz;
+            }
+            if (!done && c < 64u) {
+                ans = base + c;
+                done = true;
+            }
+            if (!__ballot(!done)) break;
+        }
+        if (!done) {  // a wavefront whose targets span more than 8 windows
+            uint32_t a = base, b = (uint32_t)L;
+            while (a < b) {
+                const uint32_t mid = (a + b) >> 1;
+                if (cdf[mid] <= T) a = mid + 1u;
+                else b = mid;
+            }
+            ans = a;
+        }
+        idx = (int)ans;
     }
+    if (i >= N) return;
     idx_out[i] = idx;
     if (idx < N) {
         O.px[i] = S.px[idx]; O.py[i] = S.py[idx]; O.pz[i] = S.pz[idx];
