@@ -678,7 +678,11 @@ __device__ __forceinline__ BgTerm<GRAD> bg_term(float s_raw, const BgTable<GRAD>
     const uint32_t i4 = (uint32_t)i << 2;
     const float2 ab = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(stab->ab) + 2u * i4);
     const float cc = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(stab->c) + i4);
-    t.v = fmaxf(0.0f, k3_kernel_poly_raw(ab.x, ab.y, cc, f));
+    // max(0, poly) as a clamp to [0, 1], which folds into the last FMA's
+    // clamp bit (one instruction fewer per term): equal, since no entry's
+    // quadratic exceeds 1 on [0, 1] (the largest value is the bracket's k(0) = 1,
+    // entry 0 at f = 0; pinned by tests/test_math.py)
+    t.v = __builtin_amdgcn_fmed3f(k3_kernel_poly_raw(ab.x, ab.y, cc, f), 0.0f, 1.0f);
     t.g = 0.0f;
     if constexpr (GRAD) {
         const float2 g = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(stab->gab) + 2u * i4);
@@ -707,14 +711,13 @@ __device__ __forceinline__ void bg_load_table(BgTable<GRAD>* stab) {
 template <bool GRAD>
 constexpr int bg_partial_stride() { return GRAD ? 256 : 128; }
 
-// The first 16 bytes of a staged record (direction u and mgs) as one
-// ds_read_b128: the empty asm statement keeps the fourth word live, since a
-// 12-byte read of the three words used measured slower.
-__device__ __forceinline__ float4 bg_lds_dir(const float4* p) {
-    float4 r = *p;
-    asm volatile("" : "+v"(r.w));
-    return r;
-}
+// The first 16 bytes of a staged record (direction u and mgs) are read as
+// one ds_read_b128: the empty asm statements of bg_keep_w keep the fourth
+// word live, since a 12-byte read of the three words used measured slower.
+// They stand after the values' first use, so that every read of a step is in
+// flight before the one wait for them (an asm statement right after a read
+// makes the compiler wait for that read there).
+__device__ __forceinline__ void bg_keep_w(float w) { asm volatile("" ::"v"(w)); }
 
 // Gather of one work item (a whole tile, or one 1024-point chunk of a long
 // tile list) by one wavefront: lane = voxel of the 4x4x4 tile.  Distances
@@ -849,9 +852,11 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
                 const uint64_t stop = q < 63u ? ((~m_main | m_start) >> (q + 1u)) : 0ull;
                 const uint32_t r = min(cnt_k, stop ? q + 1u + (uint32_t)__builtin_ctzll(stop) : 64u);
                 for (; q + 2u <= r; q += 2u) {
-                    const float4 u0 = bg_lds_dir(srec + 3u * q), u1 = bg_lds_dir(srec + 3u * q + 3u);
+                    const float4 u0 = srec[3u * q], u1 = srec[3u * q + 3u];
                     const float d0 = fmaf(vz, u0.z, fmaf(vy, u0.y, vx * u0.x));
                     const float d1 = fmaf(vz, u1.z, fmaf(vy, u1.y, vx * u1.x));
+                    bg_keep_w(u0.w);
+                    bg_keep_w(u1.w);
                     // squared distances clamped at 0 by bg_term
                     const BgTerm<GRAD> t0 = bg_term<GRAD>(fmaf(-d0, d0, dc2s), stab);
                     const BgTerm<GRAD> t1 = bg_term<GRAD>(fmaf(-d1, d1, dc2s), stab);
@@ -863,8 +868,9 @@ __device__ __forceinline__ void bgki_item(const BgArgs& a, uint32_t it, int lane
                     }
                 }
                 if (q < r) {
-                    const float4 u0 = bg_lds_dir(srec + 3u * q);
+                    const float4 u0 = srec[3u * q];
                     const float d0 = fmaf(vz, u0.z, fmaf(vy, u0.y, vx * u0.x));
+                    bg_keep_w(u0.w);
                     const BgTerm<GRAD> t0 = bg_term<GRAD>(fmaf(-d0, d0, dc2s), stab);
                     cur_f += t0.v;
                     if (GRAD) gsum_f += t0.g;

@@ -179,6 +179,35 @@ def test_kernel_table_matches_generator_and_formula(oracle_lib):
     assert gkt.evaluate(gkt.table(), np.float32(0.3)) == gkt.evaluate(want.reshape(-1, 4), np.float32(0.3))
 
 
+def test_kernel_table_never_exceeds_one():
+    """The gather clamps each table value to [0, 1] (one clamp bit on its last
+    FMA) where the restatement takes max(0, value): the same only if no
+    interval's quadratic a + f (b + f c) exceeds 1 for f in [0, 1].  Checked
+    exactly at the interval ends and the vertex, and through the float32 FMA
+    evaluation on a dense grid of the first intervals (the largest values)."""
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("gkt", os.path.join(root, "k3dom_amd", "csrc", "gen_kernel_table.py"))
+    gkt = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gkt)
+    t = gkt.table().astype(np.float64)
+    a, b, c = t[:, 0], t[:, 1], t[:, 2]
+    top = np.maximum(a, a + b + c)
+    inner = (c != 0) & (-b / np.where(c != 0, 2 * c, 1) > 0) & (-b / np.where(c != 0, 2 * c, 1) < 1)
+    fv = np.where(inner, -b / np.where(c != 0, 2 * c, 1), 0.0)
+    top = np.maximum(top, np.where(inner, a + fv * (b + fv * c), -np.inf))
+    assert np.max(top) <= 1.0
+    assert np.max(top[1:]) < 1.0 - 1e-3  # only entry 0 reaches 1, at f = 0
+    f = np.linspace(0.0, 1.0, 100001).astype(np.float32)
+    t32 = gkt.table()
+    for i in range(4):
+        # float32 FMA: the product is exact in double, one rounding to float32
+        inner32 = (f.astype(np.float64) * np.float64(t32[i, 2]) + np.float64(t32[i, 1])).astype(np.float32)
+        v = (f.astype(np.float64) * inner32.astype(np.float64) + np.float64(t32[i, 0])).astype(np.float32)
+        assert np.max(v) <= np.float32(1.0)
+
+
 @pytest.mark.parametrize("ls", [0.5, 0.3, 0.1, 0.7, 1.3, 0.05, 2.0, 0.123456])
 def test_kernel_is_exactly_zero_from_the_length_scale_on(oracle_lib, ls):
     """Every d beyond ls (past the rounding of d^2 / ls^2) gives an exact zero,
