@@ -531,8 +531,8 @@ k3dom_status enqueue_head(k3dom_handle* h, bool normal) {
         mark(h, ST_SORT);
         HIP_TRY(h, launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys,
                                      &h->last_perm));
-        HIP_TRY(h, launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N));
-        HIP_TRY(h, launch_ranges(h->stream, h->P[1].key, h->N, v.start, v.end, h->V));
+        HIP_TRY(h, launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N,
+                                        v.start, v.end, h->V));
     }
     return K3DOM_OK;
 }
@@ -1245,8 +1245,8 @@ k3dom_status k3dom_particle_assignment(k3dom_handle* h) {
     VoxelArrays& v = h->vox[h->vcur];
     if (launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys, &h->last_perm) !=
             hipSuccess ||
-        launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N) != hipSuccess ||
-        launch_ranges(h->stream, h->P[1].key, h->N, v.start, v.end, h->V) != hipSuccess)
+        launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N, v.start, v.end,
+                             h->V) != hipSuccess)
         s = fail(h, K3DOM_ERR_HIP, "particleAssignment: launch failed");
     return stage_done(h, s, STG_ASSIGNED);
 }

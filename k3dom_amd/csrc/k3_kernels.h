@@ -117,10 +117,11 @@ int sort_blocks(int n);
 hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int key_bits, SortScratch s,
                              const uint32_t** out_keys, const uint32_t** out_idx);
 // Sorted particles from the predicted records to dst, except the weights,
-// which go to wsorted.
+// which go to wsorted; and the voxels' particle ranges (start / end, reset
+// for all V voxels first).
 hipError_t launch_gather_sorted(hipStream_t st, const float4* prec, ParticleArrays dst, float* wsorted,
-                                const uint32_t* perm, const uint32_t* sorted_keys, int n);
-hipError_t launch_ranges(hipStream_t st, const int32_t* sorted_keys, int n, int32_t* start, int32_t* end, int V);
+                                const uint32_t* perm, const uint32_t* sorted_keys, int n, int32_t* start,
+                                int32_t* end, int V);
 
 // ------------------------------------------------------------------ kernel inference (k3_bgki.hip)
 // Candidate lists are built over a hierarchy of boxes of 4x4x4-voxel tiles:

@@ -11,7 +11,7 @@
 // Reference kernels these replace (youngjae-min/k3dom):
 //   k_predict            src/common/kernel/predict.cu:12-59
 //   k_radix_* / k_gather src/common/dom.cu:169-183 (thrust::sort_by_key)
-//   k_mark_ranges        src/common/kernel/particle_to_grid.cu:13-39
+//   k_gather (ranges)    src/common/kernel/particle_to_grid.cu:13-39
 //   k_point_prep, k_list_level, k_bgki, k_bgki_combine
 //                        src/k3dom/kernel/mass_update.cu:32-77 (BGKI)
 //   k_unit_weight_sums, k_mass_update
@@ -406,10 +406,20 @@ hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int 
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(TPB) void k_reset_ranges(int32_t* start, int32_t* end, int V) {
+    int c = blockIdx.x * TPB + threadIdx.x;
+    if (c >= V) return;
+    start[c] = -1;
+    end[c] = -1;
+}
+
+// The sorted particles from their records, and each voxel's range of sorted
+// positions (start / end of its run of equal keys, after k_reset_ranges).
 __global__ __launch_bounds__(TPB) void k_gather(const float4* __restrict__ prec, ParticleArrays dst,
                                                 float* __restrict__ wsorted,
                                                 const uint32_t* __restrict__ perm,
-                                                const uint32_t* __restrict__ skeys, int n) {
+                                                const uint32_t* __restrict__ skeys, int n, int32_t* __restrict__ start,
+                                                int32_t* __restrict__ end) {
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
     const uint32_t j = perm[i];
@@ -417,31 +427,17 @@ __global__ __launch_bounds__(TPB) void k_gather(const float4* __restrict__ prec,
     dst.px[i] = a.x; dst.py[i] = a.y; dst.pz[i] = a.z;
     dst.vx[i] = a.w; dst.vy[i] = b.x; dst.vz[i] = b.y;
     wsorted[i] = b.z;
-    dst.key[i] = (int32_t)skeys[i];
+    const uint32_t k = skeys[i];
+    dst.key[i] = (int32_t)k;
+    if (i == 0 || skeys[i - 1] != k) start[k] = i;
+    if (i == n - 1 || skeys[i + 1] != k) end[k] = i;
 }
 hipError_t launch_gather_sorted(hipStream_t st, const float4* prec, ParticleArrays dst, float* wsorted,
-                                const uint32_t* perm, const uint32_t* sorted_keys, int n) {
-    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(TPB), 0, st, prec, dst, wsorted, perm, sorted_keys, n);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(TPB) void k_reset_ranges(int32_t* start, int32_t* end, int V) {
-    int c = blockIdx.x * TPB + threadIdx.x;
-    if (c >= V) return;
-    start[c] = -1;
-    end[c] = -1;
-}
-__global__ __launch_bounds__(TPB) void k_mark_ranges(const int32_t* __restrict__ key, int n, int32_t* start,
-                                                     int32_t* end) {
-    int i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= n) return;
-    int k = key[i];
-    if (i == 0 || key[i - 1] != k) start[k] = i;
-    if (i == n - 1 || key[i + 1] != k) end[k] = i;
-}
-hipError_t launch_ranges(hipStream_t st, const int32_t* sorted_keys, int n, int32_t* start, int32_t* end, int V) {
+                                const uint32_t* perm, const uint32_t* sorted_keys, int n, int32_t* start,
+                                int32_t* end, int V) {
     hipLaunchKernelGGL(k_reset_ranges, dim3(blocks_for(V)), dim3(TPB), 0, st, start, end, V);
-    hipLaunchKernelGGL(k_mark_ranges, dim3(blocks_for(n)), dim3(TPB), 0, st, sorted_keys, n, start, end);
+    hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(TPB), 0, st, prec, dst, wsorted, perm, sorted_keys, n,
+                       start, end);
     return hipGetLastError();
 }
 
