@@ -131,79 +131,6 @@ hipError_t launch_fill_voxels(hipStream_t st, VoxelArrays v, int V, float pf, fl
     return hipGetLastError();
 }
 
-// ============================================================== radix sort
-// Stable LSD radix sort, 8-bit digits.  Per pass: per-tile LDS histograms
-// (digit-major in global memory), one exclusive scan, then a scatter that
-// ranks equal digits in input order with wavefront ballots.
-constexpr int RS_ITEMS = 8;
-constexpr int RS_TILE = TPB * RS_ITEMS;
-
-int sort_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
-
-__global__ __launch_bounds__(TPB) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
-                                                    uint32_t* __restrict__ hist, int nblk) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    int base = blockIdx.x * RS_TILE;
-#pragma unroll 4
-    for (int r = 0; r < RS_ITEMS; ++r) {
-        int e = base + r * TPB + threadIdx.x;
-        if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
-}
-
-__global__ __launch_bounds__(TPB) void k_radix_scatter(const uint32_t* __restrict__ kin,
-                                                       const uint32_t* __restrict__ iin, int n, int shift,
-                                                       const uint32_t* __restrict__ offs, int nblk,
-                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ iout) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wc[4][256];
-    const int t = threadIdx.x, w = t >> 6;
-    run[t] = offs[t * nblk + blockIdx.x];
-    const int base = blockIdx.x * RS_TILE;
-    for (int r = 0; r < RS_ITEMS; ++r) {
-        wc[0][t] = 0; wc[1][t] = 0; wc[2][t] = 0; wc[3][t] = 0;
-        __syncthreads();
-        int e = base + r * TPB + t;
-        bool valid = e < n;
-        uint32_t k = valid ? kin[e] : 0u;
-        uint32_t id = valid ? (iin ? iin[e] : (uint32_t)e) : 0u;
-        uint32_t dg = (k >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            bool bit = (dg >> b) & 1u;
-            uint64_t bal = __ballot(valid && bit);
-            peers &= bit ? bal : ~bal;
-        }
-        uint32_t rank = lanes_below(peers);
-        if (valid && rank == 0) wc[w][dg] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[dg] + rank;
-            for (int ww = 0; ww < w; ++ww) pos += wc[ww][dg];
-            kout[pos] = k;
-            iout[pos] = id;
-        }
-        __syncthreads();
-        run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
-        __syncthreads();
-    }
-}
-
-// Exclusive scans of u32.  The element count is n_host, or (*n_dev) * n_mul
-// when n_dev is given (a count produced on the device); the grid is sized for
-// n_max.  The total is also written to out[n] and, if given, to *total.  in
-// may equal out.
-
-// (a count read from device memory never exceeds the capacity the launch covers)
-__device__ __forceinline__ uint32_t scan_count(const uint32_t* n_dev, uint32_t n_mul, uint32_t n_host) {
-    return n_dev ? min((*n_dev) * n_mul, n_host) : n_host;
-}
-
 // exclusive block scan of one value per thread (256 threads)
 __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* lds4, uint32_t* total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -220,6 +147,124 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* ld
     *total = lds4[0] + lds4[1] + lds4[2] + lds4[3];
     __syncthreads();
     return x - v + add;
+}
+
+// ============================================================== radix sort
+// Stable LSD radix sort with digits of db <= 8 bits (the key bits split evenly
+// over the passes).  Per pass: per-tile LDS histograms (digit-major in global
+// memory), one exclusive scan, then a scatter that ranks equal digits in input
+// order with wavefront ballots.
+#ifndef K3_RS_ITEMS
+#define K3_RS_ITEMS 8
+#endif
+constexpr int RS_ITEMS = K3_RS_ITEMS;
+constexpr int RS_TILE = TPB * RS_ITEMS;
+
+int sort_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+__global__ __launch_bounds__(TPB) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift, int db,
+                                                    uint32_t* __restrict__ hist, int nblk) {
+    __shared__ uint32_t h[256];
+    const uint32_t dmask = (1u << db) - 1u;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        int e = base + r * TPB + threadIdx.x;
+        if (e < n) atomicAdd(&h[(keys[e] >> shift) & dmask], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x <= dmask) hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Each wavefront ranks a contiguous quarter of the tile (RS_ITEMS rounds of 64
+// keys, digit counters private to the wavefront, so no block barrier per
+// round); the tile is then staged in LDS in (digit, input order) order and
+// written out in runs of equal digits, so consecutive lanes store to
+// consecutive addresses.
+__global__ __launch_bounds__(TPB) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                       const uint32_t* __restrict__ iin, int n, int shift, int db,
+                                                       const uint32_t* __restrict__ offs, int nblk,
+                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ iout) {
+    __shared__ uint32_t wc[4][256];
+    __shared__ uint32_t gbase[256], lbase[256];
+    __shared__ uint32_t lds4[4];
+    __shared__ uint32_t sk[RS_TILE], si[RS_TILE];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const uint32_t dmask = (1u << db) - 1u;
+    wc[0][t] = 0; wc[1][t] = 0; wc[2][t] = 0; wc[3][t] = 0;
+    const uint32_t gb = t <= (int)dmask ? offs[t * nblk + blockIdx.x] : 0u;
+    const int base = blockIdx.x * RS_TILE;
+    const int wbase = base + w * 64 * RS_ITEMS + lane;
+    uint32_t k[RS_ITEMS], id[RS_ITEMS], rank[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int e = wbase + r * 64;
+        k[r] = e < n ? kin[e] : 0u;
+        id[r] = e < n ? (iin ? iin[e] : (uint32_t)e) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const bool valid = wbase + r * 64 < n;
+        const uint32_t dg = (k[r] >> shift) & dmask;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < db; ++b) {
+            const bool bit = (dg >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t below = lanes_below(peers);
+        uint32_t old = 0;
+        if (valid) old = wc[w][dg];
+        rank[r] = old + below;
+        if (valid && below == 0) wc[w][dg] = old + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // per digit: the wavefronts' offsets within the digit, and the tile's
+    // exclusive digit offsets
+    uint32_t c0 = wc[0][t], c1 = wc[1][t], c2 = wc[2][t], c3 = wc[3][t];
+    uint32_t tot;
+    const uint32_t lb = block_excl_scan_u32(c0 + c1 + c2 + c3, lds4, &tot);
+    wc[0][t] = lb;
+    wc[1][t] = lb + c0;
+    wc[2][t] = lb + c0 + c1;
+    wc[3][t] = lb + c0 + c1 + c2;
+    gbase[t] = gb;
+    lbase[t] = lb;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        if (wbase + r * 64 < n) {
+            const uint32_t p = wc[w][(k[r] >> shift) & dmask] + rank[r];
+            sk[p] = k[r];
+            si[p] = id[r];
+        }
+    }
+    __syncthreads();
+    const int cnt = min(RS_TILE, n - base);
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int i = r * TPB + t;
+        if (i < cnt) {
+            const uint32_t kk = sk[i];
+            const uint32_t d = (kk >> shift) & dmask;
+            const uint32_t p = gbase[d] + (uint32_t)i - lbase[d];
+            kout[p] = kk;
+            iout[p] = si[i];
+        }
+    }
+}
+
+// Exclusive scans of u32.  The element count is n_host, or (*n_dev) * n_mul
+// when n_dev is given (a count produced on the device); the grid is sized for
+// n_max.  The total is also written to out[n] and, if given, to *total.  in
+// may equal out.
+
+// (a count read from device memory never exceeds the capacity the launch covers)
+__device__ __forceinline__ uint32_t scan_count(const uint32_t* n_dev, uint32_t n_mul, uint32_t n_host) {
+    return n_dev ? min((*n_dev) * n_mul, n_host) : n_host;
 }
 
 // One-workgroup scan for up to SC_SMALL elements (one launch instead of three).
@@ -386,18 +431,19 @@ hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int 
                              const uint32_t** out_keys, const uint32_t** out_idx) {
     const int nblk = sort_blocks(n);
     const int passes = key_bits <= 8 ? 1 : (key_bits + 7) / 8;
+    const int db = std::max(1, (key_bits + passes - 1) / passes);
     const uint32_t* kin = (const uint32_t*)in_keys;
     const uint32_t* iin = nullptr;
     uint32_t* kb[2] = {s.k0, s.k1};
     uint32_t* ib[2] = {s.i0, s.i1};
     for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(TPB), 0, st, kin, n, shift, s.hist, nblk);
-        (void)launch_scan_u32(st, s.hist, s.hist, s.bsum, 256ll * nblk, nullptr, 1, nullptr);
+        int shift = db * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(TPB), 0, st, kin, n, shift, db, s.hist, nblk);
+        (void)launch_scan_u32(st, s.hist, s.hist, s.bsum, (1ll << db) * nblk, nullptr, 1, nullptr);
         uint32_t* ko = kb[p & 1];
         uint32_t* io = ib[p & 1];
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(TPB), 0, st, kin, iin, n, shift, s.hist, nblk, ko,
-                           io);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(TPB), 0, st, kin, iin, n, shift, db, s.hist, nblk,
+                           ko, io);
         kin = ko;
         iin = io;
     }
