@@ -936,8 +936,7 @@ k3dom_status k3dom_create_method(const k3dom_params* params, int32_t method, k3d
     CREATE_TRY(dalloc(h, &h->sort.k1, N));
     CREATE_TRY(dalloc(h, &h->sort.i0, N));
     CREATE_TRY(dalloc(h, &h->sort.i1, N));
-    CREATE_TRY(dalloc(h, &h->sort.hist, (size_t)256 * sort_blocks(h->N) + 1));
-    CREATE_TRY(dalloc_zero(h, &h->sort.bsum, (size_t)scan_scratch_words(256ll * sort_blocks(h->N))));
+    CREATE_TRY(dalloc_zero(h, &h->sort.hist, (size_t)sort_hist_words(h->N)));
     CREATE_TRY(dalloc(h, &h->cdf_v, V));
     CREATE_TRY(dalloc(h, &h->cdf_j, N + B));
     CREATE_TRY(dalloc(h, &h->blocksums, (size_t)fixed_scan_scratch_words((long long)std::max(V, N + B))));

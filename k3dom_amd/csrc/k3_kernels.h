@@ -110,10 +110,10 @@ hipError_t launch_fill_voxels(hipStream_t st, VoxelArrays v, int V, float prior_
 // modified.  On return *out_keys / *out_idx point into the scratch buffers.
 struct SortScratch {
     uint32_t *k0, *k1, *i0, *i1;
-    uint32_t* hist;  // 256 * nblocks + 1
-    uint32_t* bsum;  // scan scratch for 256 * nblocks elements
+    uint32_t* hist;  // sort_hist_words(n): digit counts per tile and per group of tiles
 };
 int sort_blocks(int n);
+long long sort_hist_words(int n);
 hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int key_bits, SortScratch s,
                              const uint32_t** out_keys, const uint32_t** out_idx);
 // Sorted particles from the predicted records to dst, except the weights,

@@ -151,31 +151,70 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* ld
 
 // ============================================================== radix sort
 // Stable LSD radix sort with digits of db <= 8 bits (the key bits split evenly
-// over the passes).  Per pass: per-tile LDS histograms (digit-major in global
-// memory), one exclusive scan, then a scatter that ranks equal digits in input
-// order with wavefront ballots.
-#ifndef K3_RS_ITEMS
-#define K3_RS_ITEMS 8
-#endif
-constexpr int RS_ITEMS = K3_RS_ITEMS;
+// over the passes).  Per pass two kernels: the digit counts of every tile of
+// RS_TILE keys (and, by atomics, of every group of S consecutive tiles), then
+// a scatter in which each tile adds up its own digit offsets from those counts
+// (ceil(ntiles / S) + S - 1 rows of 256 words at most, read from L2) and ranks
+// equal digits in input order with wavefront ballots.  Each pass has its own
+// group counts, zero on entry: the histogram kernel of pass p clears those of
+// pass p + 1 (of pass 0, for the last pass: read by a finished scatter), so a
+// sort leaves them as it found them (at least two passes, so a pass never
+// clears its own).
+constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = TPB * RS_ITEMS;
+constexpr int RS_BATCH = 16;
 
 int sort_blocks(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+// tiles per group, about sqrt(ntiles), so that both sums of the scatter's
+// prologue stay short
+static int sort_group(int nblk) {
+    int S = 1;
+    while ((long long)S * S < nblk) ++S;
+    return S;
+}
+constexpr int RS_MAX_PASSES = 4;  // 32-bit keys
+// counts per tile, and per group for each pass
+long long sort_hist_words(int n) {
+    const int nblk = sort_blocks(n);
+    const int S = sort_group(nblk);
+    return 256ll * nblk + RS_MAX_PASSES * 256ll * ((nblk + S - 1) / S);
+}
 
+// The peers of each lane: the valid lanes whose digit (db bits) equals its own.
+__device__ __forceinline__ uint64_t digit_peers(uint32_t dg, int db, bool valid) {
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < db; ++b) {
+        const bool bit = (dg >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return peers;
+}
+
+// cnt[b * 256 + d]: keys of digit d in tile b; grp[g * 256 + d] += the same
+// over the tiles of group g (zero on entry); clear[0, nclear) = 0.
 __global__ __launch_bounds__(TPB) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift, int db,
-                                                    uint32_t* __restrict__ hist, int nblk) {
+                                                    int S, uint32_t* __restrict__ cnt, uint32_t* __restrict__ grp,
+                                                    uint32_t* __restrict__ clear, int nclear) {
     __shared__ uint32_t h[256];
     const uint32_t dmask = (1u << db) - 1u;
-    h[threadIdx.x] = 0;
+    const int t = threadIdx.x;
+    if (blockIdx.x * 256 + t < nclear) clear[blockIdx.x * 256 + t] = 0u;
+    h[t] = 0;
     __syncthreads();
-    int base = blockIdx.x * RS_TILE;
-#pragma unroll 4
-    for (int r = 0; r < RS_ITEMS; ++r) {
-        int e = base + r * TPB + threadIdx.x;
-        if (e < n) atomicAdd(&h[(keys[e] >> shift) & dmask], 1u);
+    const int base = blockIdx.x * RS_TILE;
+    uint32_t k[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) k[r] = keys[min(base + r * TPB + t, n - 1)];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r)
+        if (base + r * TPB + t < n) atomicAdd(&h[(k[r] >> shift) & dmask], 1u);
+    __syncthreads();
+    if (t <= (int)dmask) {
+        const uint32_t v = h[t];
+        cnt[(size_t)blockIdx.x * 256 + t] = v;
+        if (v) atomicAdd(&grp[(size_t)(blockIdx.x / S) * 256 + t], v);
     }
-    __syncthreads();
-    if (threadIdx.x <= dmask) hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
 // Each wavefront ranks a contiguous quarter of the tile (RS_ITEMS rounds of 64
@@ -185,36 +224,67 @@ __global__ __launch_bounds__(TPB) void k_radix_hist(const uint32_t* __restrict__
 // consecutive addresses.
 __global__ __launch_bounds__(TPB) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                        const uint32_t* __restrict__ iin, int n, int shift, int db,
-                                                       const uint32_t* __restrict__ offs, int nblk,
+                                                       const uint32_t* __restrict__ cnt,
+                                                       const uint32_t* __restrict__ grp, int nblk, int S, int ngrp,
                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ iout) {
     __shared__ uint32_t wc[4][256];
     __shared__ uint32_t gbase[256], lbase[256];
     __shared__ uint32_t lds4[4];
     __shared__ uint32_t sk[RS_TILE], si[RS_TILE];
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    const uint32_t dmask = (1u << db) - 1u;
-    wc[0][t] = 0; wc[1][t] = 0; wc[2][t] = 0; wc[3][t] = 0;
-    const uint32_t gb = t <= (int)dmask ? offs[t * nblk + blockIdx.x] : 0u;
-    const int base = blockIdx.x * RS_TILE;
+    const int R = 1 << db;
+    const uint32_t dmask = (uint32_t)R - 1u;
+    const int tile = blockIdx.x, g = tile / S;
+    const int base = tile * RS_TILE;
     const int wbase = base + w * 64 * RS_ITEMS + lane;
     uint32_t k[RS_ITEMS], id[RS_ITEMS], rank[RS_ITEMS];
 #pragma unroll
+    // (clamped, not predicated, loads: all in flight at once)
     for (int r = 0; r < RS_ITEMS; ++r) {
-        const int e = wbase + r * 64;
-        k[r] = e < n ? kin[e] : 0u;
-        id[r] = e < n ? (iin ? iin[e] : (uint32_t)e) : 0u;
+        const int e = min(wbase + r * 64, n - 1);
+        k[r] = kin[e];
+        id[r] = iin ? iin[e] : (uint32_t)e;
     }
-    __syncthreads();
+    // The tile's start for digit d: the keys of digits below d, plus the keys
+    // of digit d in earlier groups (threads d < R) and in earlier tiles of
+    // this group (threads R + d, or thread d itself when 2R > TPB).
+    // (loads in batches of RS_BATCH, all in flight before the first add)
+    uint32_t total = 0, before = 0, inside = 0;
+    if (t < R) {
+        const uint32_t* gp = grp + t;
+        for (int j0 = 0; j0 < ngrp; j0 += RS_BATCH) {
+            uint32_t v[RS_BATCH];
+#pragma unroll
+            for (int q = 0; q < RS_BATCH; ++q) v[q] = gp[(size_t)min(j0 + q, ngrp - 1) * 256];
+#pragma unroll
+            for (int q = 0; q < RS_BATCH; ++q) {
+                total += j0 + q < ngrp ? v[q] : 0u;
+                before += j0 + q < g ? v[q] : 0u;
+            }
+        }
+    }
+    const int di = 2 * R <= TPB ? t - R : t;
+    if (di >= 0 && di < R) {
+        const uint32_t* cp = cnt + di;
+        for (int b0 = g * S; b0 < tile; b0 += RS_BATCH) {
+            uint32_t v[RS_BATCH];
+#pragma unroll
+            for (int q = 0; q < RS_BATCH; ++q) v[q] = cp[(size_t)min(b0 + q, tile - 1) * 256];
+#pragma unroll
+            for (int q = 0; q < RS_BATCH; ++q) inside += b0 + q < tile ? v[q] : 0u;
+        }
+    }
+    if (2 * R <= TPB && di >= 0 && di < R) gbase[di] = inside;
+    wc[0][t] = 0; wc[1][t] = 0; wc[2][t] = 0; wc[3][t] = 0;
+    uint32_t all;
+    // (its barriers also order the gbase and wc stores above before the reads below)
+    const uint32_t below_d = block_excl_scan_u32(total, lds4, &all);
+    const uint32_t gb = t < R ? below_d + before + (2 * R <= TPB ? gbase[t] : inside) : 0u;
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
         const bool valid = wbase + r * 64 < n;
         const uint32_t dg = (k[r] >> shift) & dmask;
-        uint64_t peers = __ballot(valid);
-        for (int b = 0; b < db; ++b) {
-            const bool bit = (dg >> b) & 1u;
-            const uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
-        }
+        const uint64_t peers = digit_peers(dg, db, valid);
         const uint32_t below = lanes_below(peers);
         uint32_t old = 0;
         if (valid) old = wc[w][dg];
@@ -243,11 +313,11 @@ __global__ __launch_bounds__(TPB) void k_radix_scatter(const uint32_t* __restric
         }
     }
     __syncthreads();
-    const int cnt = min(RS_TILE, n - base);
+    const int ntile = min(RS_TILE, n - base);
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; ++r) {
         const int i = r * TPB + t;
-        if (i < cnt) {
+        if (i < ntile) {
             const uint32_t kk = sk[i];
             const uint32_t d = (kk >> shift) & dmask;
             const uint32_t p = gbase[d] + (uint32_t)i - lbase[d];
@@ -429,21 +499,33 @@ hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 
 hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int key_bits, SortScratch s,
                              const uint32_t** out_keys, const uint32_t** out_idx) {
+    if (n <= 0) {
+        *out_keys = (const uint32_t*)in_keys;
+        *out_idx = s.i0;
+        return hipSuccess;
+    }
     const int nblk = sort_blocks(n);
-    const int passes = key_bits <= 8 ? 1 : (key_bits + 7) / 8;
+    const int passes = std::max(2, (key_bits + 7) / 8);
+    if (passes > RS_MAX_PASSES) return hipErrorInvalidValue;
     const int db = std::max(1, (key_bits + passes - 1) / passes);
+    const int S = sort_group(nblk);
+    const int ngrp = (nblk + S - 1) / S;
+    uint32_t* cnt = s.hist;
+    uint32_t* grp0 = cnt + 256ll * nblk;
     const uint32_t* kin = (const uint32_t*)in_keys;
     const uint32_t* iin = nullptr;
     uint32_t* kb[2] = {s.k0, s.k1};
     uint32_t* ib[2] = {s.i0, s.i1};
     for (int p = 0; p < passes; ++p) {
         int shift = db * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(TPB), 0, st, kin, n, shift, db, s.hist, nblk);
-        (void)launch_scan_u32(st, s.hist, s.hist, s.bsum, (1ll << db) * nblk, nullptr, 1, nullptr);
+        uint32_t* grp = grp0 + 256ll * ngrp * p;
+        uint32_t* clear = grp0 + 256ll * ngrp * ((p + 1) % passes);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(TPB), 0, st, kin, n, shift, db, S, cnt, grp, clear,
+                           256 * ngrp);
         uint32_t* ko = kb[p & 1];
         uint32_t* io = ib[p & 1];
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(TPB), 0, st, kin, iin, n, shift, db, s.hist, nblk,
-                           ko, io);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(TPB), 0, st, kin, iin, n, shift, db, cnt, grp, nblk,
+                           S, ngrp, ko, io);
         kin = ko;
         iin = io;
     }
