@@ -530,9 +530,9 @@ k3dom_status enqueue_head(k3dom_handle* h, bool normal) {
         HIP_TRY(h, launch_predict(h->stream, h->fc, h->P[0], h->prec, h->N));
         mark(h, ST_SORT);
         HIP_TRY(h, launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys,
-                                     &h->last_perm));
+                                     &h->last_perm, v.start, v.end, h->V));
         HIP_TRY(h, launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N,
-                                        v.start, v.end, h->V));
+                                        v.start, v.end));
     }
     return K3DOM_OK;
 }
@@ -1242,10 +1242,10 @@ k3dom_status k3dom_particle_assignment(k3dom_handle* h) {
     k3dom_status s = stage_entry(h, STG_PREDICTED, "particleAssignment");
     if (s != K3DOM_OK) return s;
     VoxelArrays& v = h->vox[h->vcur];
-    if (launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys, &h->last_perm) !=
-            hipSuccess ||
-        launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N, v.start, v.end,
-                             h->V) != hipSuccess)
+    if (launch_radix_sort(h->stream, h->P[0].key, h->N, h->key_bits, h->sort, &h->last_skeys, &h->last_perm,
+                          v.start, v.end, h->V) != hipSuccess ||
+        launch_gather_sorted(h->stream, h->prec, h->P[1], h->wsort, h->last_perm, h->last_skeys, h->N, v.start,
+                             v.end) != hipSuccess)
         s = fail(h, K3DOM_ERR_HIP, "particleAssignment: launch failed");
     return stage_done(h, s, STG_ASSIGNED);
 }

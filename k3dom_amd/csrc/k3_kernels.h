@@ -114,14 +114,17 @@ struct SortScratch {
 };
 int sort_blocks(int n);
 long long sort_hist_words(int n);
+// Stable sort of in_keys (key_bits bits); also sets fill_a and fill_b[0, nfill)
+// to -1 (the voxel ranges, before launch_gather_sorted marks them).
 hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int key_bits, SortScratch s,
-                             const uint32_t** out_keys, const uint32_t** out_idx);
+                             const uint32_t** out_keys, const uint32_t** out_idx, int32_t* fill_a, int32_t* fill_b,
+                             int nfill);
 // Sorted particles from the predicted records to dst, except the weights,
-// which go to wsorted; and the voxels' particle ranges (start / end, reset
-// for all V voxels first).
+// which go to wsorted; and the voxels' particle ranges (start / end, all -1
+// on entry: launch_radix_sort's fill).
 hipError_t launch_gather_sorted(hipStream_t st, const float4* prec, ParticleArrays dst, float* wsorted,
                                 const uint32_t* perm, const uint32_t* sorted_keys, int n, int32_t* start,
-                                int32_t* end, int V);
+                                int32_t* end);
 
 // ------------------------------------------------------------------ kernel inference (k3_bgki.hip)
 // Candidate lists are built over a hierarchy of boxes of 4x4x4-voxel tiles:

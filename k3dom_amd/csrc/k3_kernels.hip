@@ -131,6 +131,13 @@ hipError_t launch_fill_voxels(hipStream_t st, VoxelArrays v, int V, float pf, fl
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(TPB) void k_reset_ranges(int32_t* start, int32_t* end, int V) {
+    int c = blockIdx.x * TPB + threadIdx.x;
+    if (c >= V) return;
+    start[c] = -1;
+    end[c] = -1;
+}
+
 // exclusive block scan of one value per thread (256 threads)
 __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t* lds4, uint32_t* total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -192,14 +199,20 @@ __device__ __forceinline__ uint64_t digit_peers(uint32_t dg, int db, bool valid)
 }
 
 // cnt[b * 256 + d]: keys of digit d in tile b; grp[g * 256 + d] += the same
-// over the tiles of group g (zero on entry); clear[0, nclear) = 0.
+// over the tiles of group g (zero on entry); clear[0, nclear) = 0; and
+// fa, fb[0, nfill) = -1.
 __global__ __launch_bounds__(TPB) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift, int db,
                                                     int S, uint32_t* __restrict__ cnt, uint32_t* __restrict__ grp,
-                                                    uint32_t* __restrict__ clear, int nclear) {
+                                                    uint32_t* __restrict__ clear, int nclear, int32_t* __restrict__ fa,
+                                                    int32_t* __restrict__ fb, int nfill) {
     __shared__ uint32_t h[256];
     const uint32_t dmask = (1u << db) - 1u;
     const int t = threadIdx.x;
     if (blockIdx.x * 256 + t < nclear) clear[blockIdx.x * 256 + t] = 0u;
+    for (long long i = (long long)blockIdx.x * 256 + t; i < nfill; i += (long long)gridDim.x * 256) {
+        fa[i] = -1;
+        fb[i] = -1;
+    }
     h[t] = 0;
     __syncthreads();
     const int base = blockIdx.x * RS_TILE;
@@ -498,10 +511,15 @@ hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 }
 
 hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int key_bits, SortScratch s,
-                             const uint32_t** out_keys, const uint32_t** out_idx) {
+                             const uint32_t** out_keys, const uint32_t** out_idx, int32_t* fill_a, int32_t* fill_b,
+                             int nfill) {
     if (n <= 0) {
         *out_keys = (const uint32_t*)in_keys;
         *out_idx = s.i0;
+        if (nfill > 0) {
+            hipLaunchKernelGGL(k_reset_ranges, dim3(blocks_for(nfill)), dim3(TPB), 0, st, fill_a, fill_b, nfill);
+            return hipGetLastError();
+        }
         return hipSuccess;
     }
     const int nblk = sort_blocks(n);
@@ -521,7 +539,7 @@ hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int 
         uint32_t* grp = grp0 + 256ll * ngrp * p;
         uint32_t* clear = grp0 + 256ll * ngrp * ((p + 1) % passes);
         hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(TPB), 0, st, kin, n, shift, db, S, cnt, grp, clear,
-                           256 * ngrp);
+                           256 * ngrp, fill_a, fill_b, p == 0 ? nfill : 0);
         uint32_t* ko = kb[p & 1];
         uint32_t* io = ib[p & 1];
         hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(TPB), 0, st, kin, iin, n, shift, db, cnt, grp, nblk,
@@ -534,15 +552,8 @@ hipError_t launch_radix_sort(hipStream_t st, const int32_t* in_keys, int n, int 
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(TPB) void k_reset_ranges(int32_t* start, int32_t* end, int V) {
-    int c = blockIdx.x * TPB + threadIdx.x;
-    if (c >= V) return;
-    start[c] = -1;
-    end[c] = -1;
-}
-
 // The sorted particles from their records, and each voxel's range of sorted
-// positions (start / end of its run of equal keys, after k_reset_ranges).
+// positions (start / end of its run of equal keys; -1 on entry).
 __global__ __launch_bounds__(TPB) void k_gather(const float4* __restrict__ prec, ParticleArrays dst,
                                                 float* __restrict__ wsorted,
                                                 const uint32_t* __restrict__ perm,
@@ -562,8 +573,7 @@ __global__ __launch_bounds__(TPB) void k_gather(const float4* __restrict__ prec,
 }
 hipError_t launch_gather_sorted(hipStream_t st, const float4* prec, ParticleArrays dst, float* wsorted,
                                 const uint32_t* perm, const uint32_t* sorted_keys, int n, int32_t* start,
-                                int32_t* end, int V) {
-    hipLaunchKernelGGL(k_reset_ranges, dim3(blocks_for(V)), dim3(TPB), 0, st, start, end, V);
+                                int32_t* end) {
     hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(TPB), 0, st, prec, dst, wsorted, perm, sorted_keys, n,
                        start, end);
     return hipGetLastError();

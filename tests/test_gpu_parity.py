@@ -120,6 +120,14 @@ def test_crowded_voxels_bit_exact(gpu):
     assert counts.max() > 64
 
 
+def test_partial_single_sort_tile_bit_exact(gpu):
+    """1,000 particles: fewer than one 2,048-key sort tile, not a multiple of
+    the 64-lane wavefront, so every sort pass runs one partly filled tile."""
+    from k3dom_amd.scene import small_workload
+    w = small_workload(particles=1000, newborn=100, frames=3, seed=29)
+    run_pair(w, w.frames)
+
+
 def test_very_crowded_voxels_bit_exact(gpu):
     """200,000 particles on an 8x8x4 grid: voxels with more than 64 blocks of
     64 particles (several rounds of the wavefront's block loop)."""
