@@ -414,16 +414,38 @@ constexpr uint64_t LB_EPOCH_MASK = (1ull << 30) - 1ull;
 
 static inline long long lb_tiles(long long n_max) { return std::max(1ll, (n_max + LB_TILE - 1) / LB_TILE); }
 
+// This thread's LB_ITEMS consecutive elements of a tile, unconditionally (as
+// 16-byte loads when in is 16-byte aligned), clamped to the capacity cap.
+__device__ __forceinline__ void lb_load(const uint32_t* in, uint32_t b0, uint32_t cap, bool vec,
+                                        uint32_t (&v)[LB_ITEMS]) {
+    if (vec && b0 + LB_ITEMS <= cap) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + b0);
+#pragma unroll
+        for (int k = 0; k < LB_ITEMS / 4; ++k) {
+            const uint4 q = p[k];
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < LB_ITEMS; ++k) v[k] = in[min(b0 + k, cap - 1u)];
+    }
+}
+
 __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, const uint32_t* __restrict__ n_dev,
                                                            uint32_t n_mul, uint32_t n_host, uint32_t* out,
                                                            uint32_t* __restrict__ total, uint32_t* __restrict__ scr,
-                                                           uint32_t nblocks) {
+                                                           uint32_t nblocks, bool vec) {
     __shared__ uint32_t s_tile, s_excl;
     __shared__ uint64_t s_tag;
     __shared__ uint32_t lds4[4];
     unsigned long long* ticket = reinterpret_cast<unsigned long long*>(scr);
     uint64_t* status = reinterpret_cast<uint64_t*>(scr + 2);
     const int t = threadIdx.x, lane = t & 63;
+    // The device count and the data of tile blockIdx.x (what the ticket
+    // almost always hands out) are loaded while the ticket is taken.
+    const uint32_t n = scan_count(n_dev, n_mul, n_host);
+    uint32_t v[LB_ITEMS];
+    lb_load(in, blockIdx.x * LB_TILE + t * LB_ITEMS, n_host, vec, v);
     if (t == 0) {
         const unsigned long long tk = atomicAdd(ticket, 1ull);
         const uint64_t epoch = tk >> 32;
@@ -435,15 +457,14 @@ __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, c
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint64_t tag = s_tag;
-    const uint32_t n = scan_count(n_dev, n_mul, n_host);
     const uint32_t ntiles = (n + LB_TILE - 1) / LB_TILE;
     if (tile < ntiles) {
         const uint32_t b0 = tile * LB_TILE + t * LB_ITEMS;
-        uint32_t v[LB_ITEMS];
+        if (tile != blockIdx.x) lb_load(in, b0, n_host, vec, v);  // (block-uniform)
         uint32_t sum = 0;
 #pragma unroll
         for (int k = 0; k < LB_ITEMS; ++k) {
-            v[k] = (b0 + k < n) ? in[b0 + k] : 0u;
+            v[k] = b0 + k < n ? v[k] : 0u;
             sum += v[k];
         }
         uint32_t tot;
@@ -505,8 +526,9 @@ hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, ui
         return hipGetLastError();
     }
     const uint32_t nb = (uint32_t)lb_tiles(n_max);
+    const bool vec = (reinterpret_cast<uintptr_t>(in) & 15u) == 0;
     hipLaunchKernelGGL(k_scan_lookback_u32, dim3(nb), dim3(TPB), 0, st, in, n_dev, n_mul, (uint32_t)n_max, out,
-                       total, scratch, nb);
+                       total, scratch, nb, vec);
     return hipGetLastError();
 }
 
