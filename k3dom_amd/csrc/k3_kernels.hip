@@ -352,19 +352,39 @@ __device__ __forceinline__ uint32_t scan_count(const uint32_t* n_dev, uint32_t n
 
 // One-workgroup scan for up to SC_SMALL elements (one launch instead of three).
 constexpr int SC_SMALL_ITEMS = 16;
+constexpr int SC_ITEMS = 16;  // elements per thread of both scan kernels
 constexpr int SC_SMALL = 1024 * SC_SMALL_ITEMS;
+// SC_ITEMS consecutive elements from b0, loaded unconditionally (as 16-byte
+// loads when in is 16-byte aligned) with indices clamped to the capacity cap.
+__device__ __forceinline__ void sc_load(const uint32_t* in, uint32_t b0, uint32_t cap, bool vec,
+                                        uint32_t (&v)[SC_ITEMS]) {
+    if (vec && b0 + SC_ITEMS <= cap) {
+        const uint4* p = reinterpret_cast<const uint4*>(in + b0);
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS / 4; ++k) {
+            const uint4 q = p[k];
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SC_ITEMS; ++k) v[k] = in[min(b0 + k, cap - 1u)];
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_scan_small_u32(const uint32_t* in, const uint32_t* __restrict__ n_dev,
                                                          uint32_t n_mul, uint32_t n_host, uint32_t* out,
-                                                         uint32_t* __restrict__ total) {
+                                                         uint32_t* __restrict__ total, bool vec) {
     __shared__ uint32_t wsum[16];
+    // (the count and the data in flight together)
     const uint32_t n = scan_count(n_dev, n_mul, n_host);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t b0 = (uint32_t)t * SC_SMALL_ITEMS;
-    uint32_t v[SC_SMALL_ITEMS];
+    uint32_t v[SC_SMALL_ITEMS] = {};
+    if (n_host) sc_load(in, b0, n_host, vec, v);
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < SC_SMALL_ITEMS; ++k) {
-        v[k] = (b0 + k < n) ? in[b0 + k] : 0u;
+        v[k] = b0 + k < n ? v[k] : 0u;
         s += v[k];
     }
     uint32_t x = s;
@@ -408,28 +428,12 @@ __global__ __launch_bounds__(1024) void k_scan_small_u32(const uint32_t* in, con
 // Scratch (u32 words): [0, 1] the ticket counter, [2..] u64 statuses.
 constexpr int LB_ITEMS = 16;
 constexpr int LB_TILE = TPB * LB_ITEMS;
+static_assert(LB_ITEMS == SC_ITEMS && SC_SMALL_ITEMS == SC_ITEMS, "both scans load with sc_load");
 constexpr uint64_t LB_SUM = 1ull << 32, LB_PREFIX = 2ull << 32;
 constexpr int LB_EPOCH_SHIFT = 34;
 constexpr uint64_t LB_EPOCH_MASK = (1ull << 30) - 1ull;
 
 static inline long long lb_tiles(long long n_max) { return std::max(1ll, (n_max + LB_TILE - 1) / LB_TILE); }
-
-// This thread's LB_ITEMS consecutive elements of a tile, unconditionally (as
-// 16-byte loads when in is 16-byte aligned), clamped to the capacity cap.
-__device__ __forceinline__ void lb_load(const uint32_t* in, uint32_t b0, uint32_t cap, bool vec,
-                                        uint32_t (&v)[LB_ITEMS]) {
-    if (vec && b0 + LB_ITEMS <= cap) {
-        const uint4* p = reinterpret_cast<const uint4*>(in + b0);
-#pragma unroll
-        for (int k = 0; k < LB_ITEMS / 4; ++k) {
-            const uint4 q = p[k];
-            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < LB_ITEMS; ++k) v[k] = in[min(b0 + k, cap - 1u)];
-    }
-}
 
 __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, const uint32_t* __restrict__ n_dev,
                                                            uint32_t n_mul, uint32_t n_host, uint32_t* out,
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, c
     // almost always hands out) are loaded while the ticket is taken.
     const uint32_t n = scan_count(n_dev, n_mul, n_host);
     uint32_t v[LB_ITEMS];
-    lb_load(in, blockIdx.x * LB_TILE + t * LB_ITEMS, n_host, vec, v);
+    sc_load(in, blockIdx.x * LB_TILE + t * LB_ITEMS, n_host, vec, v);
     if (t == 0) {
         const unsigned long long tk = atomicAdd(ticket, 1ull);
         const uint64_t epoch = tk >> 32;
@@ -460,7 +464,7 @@ __global__ __launch_bounds__(TPB) void k_scan_lookback_u32(const uint32_t* in, c
     const uint32_t ntiles = (n + LB_TILE - 1) / LB_TILE;
     if (tile < ntiles) {
         const uint32_t b0 = tile * LB_TILE + t * LB_ITEMS;
-        if (tile != blockIdx.x) lb_load(in, b0, n_host, vec, v);  // (block-uniform)
+        if (tile != blockIdx.x) sc_load(in, b0, n_host, vec, v);  // (block-uniform)
         uint32_t sum = 0;
 #pragma unroll
         for (int k = 0; k < LB_ITEMS; ++k) {
@@ -521,12 +525,13 @@ long long scan_scratch_words(long long n_max) { return 2 + 2 * lb_tiles(n_max); 
 
 hipError_t launch_scan_u32(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* scratch, long long n_max,
                            const uint32_t* n_dev, uint32_t n_mul, uint32_t* total) {
+    const bool vec = (reinterpret_cast<uintptr_t>(in) & 15u) == 0;
     if (n_max <= SC_SMALL) {
-        hipLaunchKernelGGL(k_scan_small_u32, dim3(1), dim3(1024), 0, st, in, n_dev, n_mul, (uint32_t)n_max, out, total);
+        hipLaunchKernelGGL(k_scan_small_u32, dim3(1), dim3(1024), 0, st, in, n_dev, n_mul, (uint32_t)n_max, out, total,
+                           vec);
         return hipGetLastError();
     }
     const uint32_t nb = (uint32_t)lb_tiles(n_max);
-    const bool vec = (reinterpret_cast<uintptr_t>(in) & 15u) == 0;
     hipLaunchKernelGGL(k_scan_lookback_u32, dim3(nb), dim3(TPB), 0, st, in, n_dev, n_mul, (uint32_t)n_max, out,
                        total, scratch, nb, vec);
     return hipGetLastError();
