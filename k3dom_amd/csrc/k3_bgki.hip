@@ -529,13 +529,36 @@ hipError_t launch_build_levels(hipStream_t st, const FrameConsts* fc, const Tile
 }
 
 // ============================================================== work items
-__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t v) {
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1;
-        else hi = mid;
+// Number of leading elements of the sorted a[lo, hi) that satisfy below(x)
+// (a prefix of the range), found with 8-way rounds: 7 independent probes per
+// round, so the dependent loads are log8 of the length instead of log2.
+template <class Below>
+__device__ __forceinline__ uint32_t partition_point_u32(const uint32_t* a, uint32_t lo, uint32_t hi, Below below) {
+    while (hi - lo > 8) {
+        const uint32_t step = (hi - lo) / 8;
+        uint32_t x[7];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) x[j] = a[lo + (j + 1) * step];
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) m += below(x[j]) ? 1u : 0u;
+        // the answer is past probe m (if any) and at most probe m + 1 (if any)
+        const uint32_t nlo = m ? lo + m * step + 1 : lo;
+        const uint32_t nhi = m < 7 ? lo + (m + 1) * step : hi;
+        lo = nlo;
+        hi = nhi;
     }
-    return lo;
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = a[min(lo + j, hi - 1)];  // (hi > lo here unless the range is empty)
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) m += (lo + j < hi && below(x[j])) ? 1u : 0u;
+    return lo + m;
+}
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t v) {
+    if (lo >= hi) return lo;
+    return partition_point_u32(a, lo, hi, [v](uint32_t x) { return x < v; });
 }
 
 // heavy_off / light_off: one exclusive scan over the tiles' split-item
@@ -592,13 +615,8 @@ __global__ __launch_bounds__(TPB) void k_emit_heavy(int ntiles, const uint32_t* 
     }
     const uint32_t h = blockIdx.x * TPB + threadIdx.x;
     if (h >= H) return;
-    int lo = 0, hi = ntiles - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (heavy_off[mid] <= h) lo = mid;
-        else hi = mid - 1;
-    }
-    const int t = lo;
+    // the last tile whose offset is <= h (heavy_off[0] = 0 <= h)
+    const int t = (int)partition_point_u32(heavy_off, 0u, (uint32_t)ntiles, [h](uint32_t x) { return x <= h; }) - 1;
     const uint32_t k = h - heavy_off[t], cnt = heavy_cnt[t];
     const uint32_t o = toff[t], len = tlen[t];
     const uint32_t c = (tlist[o] >> K3_BGKI_CHUNK_LOG2) + k;
