@@ -725,12 +725,22 @@ __global__ __launch_bounds__(UNIT_TPB) void k_unit_weight_sums(const uint32_t* _
     float* lw = lw_all[wv];
     int32_t* lk = lk_all[wv];
     const int m = min(n - r0, UNIT_SPAN);
+    // (all loads in flight together: unconditional, clamped)
+    constexpr int NK = UNIT_SPAN / 64;
+    float xw[NK];
+    int32_t xk[NK];
 #pragma unroll
-    for (int k = 0; k < UNIT_SPAN / 64; ++k) {
+    for (int k = 0; k < NK; ++k) {
+        const int i = min(r0 + k * 64 + lane, n - 1);
+        xw[k] = w[i];
+        xk[k] = key[i];
+    }
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
         const int j = k * 64 + lane;
         if (j < m) {
-            lw[j] = w[r0 + j];
-            lk[j] = key[r0 + j];
+            lw[j] = xw[k];
+            lk[j] = xk[k];
         }
     }
     wave_lds_sync();
@@ -781,10 +791,14 @@ __global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict_
     const float decay = fc ? fc->decay : 1.0f;
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * TPB + threadIdx.x;
+    // (the voxel's four words in one round trip: unconditional, clamped)
+    const int cc = min(c, V - 1);
+    const int st0 = v.start[cc], en0 = v.end[cc];
+    const float rho = v.pers[cc], pred = v.pred[cc];
     int st = -1, nb = 0;
     if (c < V) {
-        st = v.start[c];
-        if (st != -1) nb = unit_count(st, v.end[c]);
+        st = st0;
+        if (st != -1) nb = unit_count(st, en0);
     }
     const bool wide = nb > WAVE_CHAIN_UNITS;
     float dyn = (st != -1 && !wide) ? chain_sum(usum, st, nb) : 0.0f;
@@ -1282,10 +1296,13 @@ __global__ __launch_bounds__(TPB) void k_ds_mass_update(const uint32_t* __restri
     if (*abort_flag) return;
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * TPB + threadIdx.x;
+    // (the range's two words in one round trip: unconditional, clamped)
+    const int cc = min(c, V - 1);
+    const int st0 = v.start[cc], en0 = v.end[cc];
     int st = -1, nb = 0;
     if (c < V) {
-        st = v.start[c];
-        if (st != -1) nb = unit_count(st, v.end[c]);
+        st = st0;
+        if (st != -1) nb = unit_count(st, en0);
     }
     const bool wide = nb > WAVE_CHAIN_UNITS;
     float sum = (st != -1 && !wide) ? chain_sum(usum, st, nb) : 0.0f;
@@ -1401,16 +1418,36 @@ __global__ __launch_bounds__(UNIT_TPB) void k_unit_persist(const uint32_t* __res
     const int m = max(0, min(n - r0, UNIT_SPAN));
     const float min2 = min_vel * min_vel;
     float wmax = 0.0f;
-#pragma unroll 4
-    for (int k = 0; k < UNIT_SPAN / 64; ++k) {
+    // All loads first, in two rounds (the particles, then their voxels),
+    // unconditional with clamped indices, instead of two dependent round
+    // trips per 64 positions.
+    constexpr int NK = UNIT_SPAN / 64;
+    int kk[NK];
+    float px_[NK], py_[NK], pz_[NK], pw_[NK], pe_[NK], pr_[NK];
+    if (n > 0) {
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            const int i = min(r0 + k * 64 + lane, n - 1);
+            kk[k] = P.key[i];
+            px_[k] = P.vx[i]; py_[k] = P.vy[i]; pz_[k] = P.vz[i];
+            pw_[k] = wsrc[i];
+        }
+#pragma unroll
+        for (int k = 0; k < NK; ++k) {
+            pe_[k] = v.pers[kk[k]];
+            pr_[k] = v.pred[kk[k]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
         const int j = k * 64 + lane;
         if (j < m) {
             const int i = r0 + j;
-            const int kj = P.key[i];
-            const float x = P.vx[i], y = P.vy[i], z = P.vz[i];
-            const float mu = voxel_mu(v.pers[kj], v.pred[kj]);
+            const int kj = kk[k];
+            const float x = px_[k], y = py_[k], z = pz_[k];
+            const float mu = voxel_mu(pe_[k], pr_[k]);
             float pen;
-            const float wn = persist_weight(wsrc[i], mu, x, y, z, min2, &pen);
+            const float wn = persist_weight(pw_[k], mu, x, y, z, min2, &pen);
             lx[j] = wn * x;
             ly[j] = wn * y;
             lz[j] = wn * z;
@@ -1453,10 +1490,14 @@ __global__ __launch_bounds__(TPB) void k_voxel_particles(const uint32_t* __restr
     if (*abort_flag) return;
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * TPB + threadIdx.x;
+    // (the voxel's four words in one round trip: unconditional, clamped)
+    const int cc = min(c, V - 1);
+    const int st0 = v.start[cc], en0 = v.end[cc];
+    const float rho = v.pers[cc], pred = v.pred[cc];
     int st = -1, nb = 0;
     if (c < V) {
-        st = v.start[c];
-        if (st != -1) nb = unit_count(st, v.end[c]);
+        st = st0;
+        if (st != -1) nb = unit_count(st, en0);
     }
     const bool wide = nb > WAVE_CHAIN_UNITS;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
@@ -1483,8 +1524,7 @@ __global__ __launch_bounds__(TPB) void k_voxel_particles(const uint32_t* __restr
         v.mvy[c] = 0.0f;
         v.mvz[c] = 0.0f;
     } else {
-        const float rho = v.pers[c];
-        v.mu[c] = voxel_mu(rho, v.pred[c]);
+        v.mu[c] = voxel_mu(rho, pred);
         v.mvx[c] = rho > 0.0f ? sx / rho : 0.0f;
         v.mvy[c] = rho > 0.0f ? sy / rho : 0.0f;
         v.mvz[c] = rho > 0.0f ? sz / rho : 0.0f;
@@ -1507,8 +1547,15 @@ hipError_t launch_voxel_particles(hipStream_t st, const uint32_t* abort_flag, Vo
 }
 
 // ============================================================== fixed-point CDF
+// Position i of a[0, na) followed by b[0, nb) (na + nb > 0), 0 past the end:
+// one unconditional load from a selected, clamped address, so that a thread's
+// loads are all in flight together.
 __device__ __forceinline__ float cat_load(const float* a, int na, const float* b, int nb, int i) {
-    return i < na ? a[i] : (i < na + nb ? b[i - na] : 0.0f);
+    const int n = na + nb;
+    const int c = min(i, n - 1);
+    const float* p = c < na ? a + c : b + (c - na);
+    const float x = *p;
+    return i < n ? x : 0.0f;
 }
 
 // Exponent of the fixed-point CDF from the largest of nparts per-block maxima
@@ -1565,11 +1612,13 @@ __global__ __launch_bounds__(TPB) void k_fixed_reduce(const float* __restrict__ 
     const int n = na + nb;
     const int base = blockIdx.x * FL_TILE + threadIdx.x;
     uint64_t s = 0;
+    float x[FL_ITEMS] = {};
+    if (n > 0) {
 #pragma unroll
-    for (int k = 0; k < FL_ITEMS; ++k) {
-        const int e = base + k * TPB;
-        if (e < n) s += k3_to_fixed(cat_load(a, na, b, nb, e), E);
+        for (int k = 0; k < FL_ITEMS; ++k) x[k] = cat_load(a, na, b, nb, base + k * TPB);  // (0 past the end)
     }
+#pragma unroll
+    for (int k = 0; k < FL_ITEMS; ++k) s += k3_to_fixed(x[k], E);
     uint64_t tot;
     block_scan_incl_u64(s, lds4, &tot);
     if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
@@ -1611,11 +1660,13 @@ __global__ __launch_bounds__(TPB) void k_fixed_down(const float* __restrict__ a,
     const int n = na + nb;
     const int t = threadIdx.x;
     const int base = blockIdx.x * FL_TILE;
+    float x[FL_ITEMS] = {};
+    if (n > 0) {
 #pragma unroll
-    for (int k = 0; k < FL_ITEMS; ++k) {
-        const int e = base + k * TPB + t;
-        tile_v[fl_pad(k * TPB + t)] = e < n ? k3_to_fixed(cat_load(a, na, b, nb, e), E) : 0ull;
+        for (int k = 0; k < FL_ITEMS; ++k) x[k] = cat_load(a, na, b, nb, base + k * TPB + t);  // (0 past the end)
     }
+#pragma unroll
+    for (int k = 0; k < FL_ITEMS; ++k) tile_v[fl_pad(k * TPB + t)] = k3_to_fixed(x[k], E);
     __syncthreads();
     uint64_t q[FL_ITEMS];
     uint64_t s = 0;
