@@ -94,21 +94,27 @@ __global__ __launch_bounds__(TPB) void k_shift_voxels(VoxelArrays dst, VoxelArra
     int nx = x + mx, ny = y + my, nz = z + mz;
     bool in = nx >= 0 && nx < gs && ny >= 0 && ny < gs && nz >= 0 && nz < gsz;
     int sc = in ? nx + gs * ny + gs * gs * nz : c;
+    // every field loaded (unconditionally: sc is always a voxel) before the
+    // first store, so that the loads are all in flight together
     if (fields & SHIFT_KEPT) {
-        dst.f[c] = in ? src.f[sc] : pf;
-        dst.s[c] = in ? src.s[sc] : ps;
-        dst.d[c] = in ? src.d[sc] : pd;
-        dst.mu[c] = src.mu[sc];
+        const float f = src.f[sc], s = src.s[sc], d = src.d[sc], mu = src.mu[sc];
+        dst.f[c] = in ? f : pf;
+        dst.s[c] = in ? s : ps;
+        dst.d[c] = in ? d : pd;
+        dst.mu[c] = mu;
     }
     if (fields & SHIFT_REWRITTEN) {
-        dst.start[c] = in ? src.start[sc] : -1;
-        dst.end[c] = in ? src.end[sc] : -1;
-        dst.pers[c] = src.pers[sc];
-        dst.pred[c] = src.pred[sc];
-        dst.wua[c] = src.wua[sc];
-        dst.mvx[c] = src.mvx[sc];
-        dst.mvy[c] = src.mvy[sc];
-        dst.mvz[c] = src.mvz[sc];
+        const int32_t st = src.start[sc], en = src.end[sc];
+        const float pers = src.pers[sc], pred = src.pred[sc], wua = src.wua[sc];
+        const float mvx = src.mvx[sc], mvy = src.mvy[sc], mvz = src.mvz[sc];
+        dst.start[c] = in ? st : -1;
+        dst.end[c] = in ? en : -1;
+        dst.pers[c] = pers;
+        dst.pred[c] = pred;
+        dst.wua[c] = wua;
+        dst.mvx[c] = mvx;
+        dst.mvy[c] = mvy;
+        dst.mvz[c] = mvz;
     }
 }
 hipError_t launch_shift_voxels(hipStream_t st, VoxelArrays dst, VoxelArrays src, int gs, int gsz, int mx, int my,
@@ -762,13 +768,13 @@ __global__ __launch_bounds__(UNIT_TPB) void k_unit_weight_sums(const uint32_t* _
 
 // prediction, evidence update and newborn split of one voxel
 // (src/k3dom/kernel/mass_update.cu:163-181); returns the born mass
-__device__ __forceinline__ float mass_voxel(const VoxelArrays& v, int c, float dyn_pred,
-                                            const float* __restrict__ del_occ, const float* __restrict__ del_free,
-                                            float decay, float mass_scale, float p_B, float* __restrict__ born) {
-    float f = v.f[c], s = v.s[c], d = v.d[c];
+// (f, s, d and the two evidence values: the voxel's words, loaded by the caller)
+__device__ __forceinline__ float mass_voxel(const VoxelArrays& v, int c, float dyn_pred, float f, float s, float d,
+                                            float docc, float dfree, float decay, float mass_scale, float p_B,
+                                            float* __restrict__ born) {
     const float pred_total = (f + d) * decay;
     k3_predict_mass(&f, &s, &d, dyn_pred, pred_total, decay);
-    k3_update_mass(&f, &s, &d, del_occ[c], del_free[c], mass_scale);
+    k3_update_mass(&f, &s, &d, docc, dfree, mass_scale);
     const float b = k3_newborn_part(dyn_pred, pred_total, d, p_B);
     v.f[c] = f;
     v.s[c] = s;
@@ -791,10 +797,10 @@ __global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict_
     const float decay = fc ? fc->decay : 1.0f;
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * TPB + threadIdx.x;
-    // (the voxel's four words in one round trip: unconditional, clamped)
+    // (the voxel's seven words in one round trip: unconditional, clamped)
     const int cc = min(c, V - 1);
     const int st0 = v.start[cc], en0 = v.end[cc];
-    const float rho = v.pers[cc], pred = v.pred[cc];
+    const float f0 = v.f[cc], s0 = v.s[cc], d0 = v.d[cc], docc = del_occ[cc], dfree = del_free[cc];
     int st = -1, nb = 0;
     if (c < V) {
         st = st0;
@@ -808,7 +814,7 @@ __global__ __launch_bounds__(TPB) void k_mass_update(const uint32_t* __restrict_
         if (lane == h) dyn = s;
     }
     float b = 0.0f;
-    if (c < V) b = mass_voxel(v, c, dyn, del_occ, del_free, decay, mass_scale, p_B, born);
+    if (c < V) b = mass_voxel(v, c, dyn, f0, s0, d0, docc, dfree, decay, mass_scale, p_B, born);
     block_max_store(b, born_parts + blockIdx.x);
 }
 
@@ -1903,16 +1909,16 @@ __global__ __launch_bounds__(TPB) void k_resample(const uint32_t* __restrict__ a
     }
     if (i >= N) return;
     idx_out[i] = idx;
-    if (idx < N) {
-        O.px[i] = S.px[idx]; O.py[i] = S.py[idx]; O.pz[i] = S.pz[idx];
-        O.vx[i] = S.vx[idx]; O.vy[i] = S.vy[idx]; O.vz[i] = S.vz[idx];
-        O.key[i] = S.key[idx];
-    } else {
-        int b = idx - N;
-        O.px[i] = NB.px[b]; O.py[i] = NB.py[b]; O.pz[i] = NB.pz[b];
-        O.vx[i] = NB.vx[b]; O.vy[i] = NB.vy[b]; O.vz[i] = NB.vz[b];
-        O.key[i] = NB.key[b];
-    }
+    // the ancestor's seven words from the sorted or the newborn set, all
+    // loaded (through selected pointers) before the first store
+    const bool pers = idx < N;
+    const int j = pers ? idx : idx - N;
+    const float px = (pers ? S.px : NB.px)[j], py = (pers ? S.py : NB.py)[j], pz = (pers ? S.pz : NB.pz)[j];
+    const float vx = (pers ? S.vx : NB.vx)[j], vy = (pers ? S.vy : NB.vy)[j], vz = (pers ? S.vz : NB.vz)[j];
+    const int32_t key = (pers ? S.key : NB.key)[j];
+    O.px[i] = px; O.py[i] = py; O.pz[i] = pz;
+    O.vx[i] = vx; O.vy[i] = vy; O.vz[i] = vz;
+    O.key[i] = key;
     O.w[i] = nw;
 }
 hipError_t launch_resample(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, int total_len, const uint64_t* total,
