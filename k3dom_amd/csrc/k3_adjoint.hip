@@ -135,37 +135,49 @@ __global__ __launch_bounds__(TPB) void k_window_adjoint(AdjointFrameArgs a, cons
         const int i = c % gs, j = (c / gs) % gs, k = c / (gs * gs);
         // adjoint of the masses after this frame: loss term + next frame's pre-state adjoint
         // (the adjoint arithmetic runs in double; the forward values are the frame's floats)
+        // (the cell's words, its next-frame adjoint and its previous masses are
+        // loaded unconditionally, from a valid cell, before any is used, so
+        // that they are all in flight together)
+        const bool use = !a.mask || a.mask[c];
+        const float post_f = a.post.f[c], post_s = a.post.s[c], post_d = a.post.d[c];
+        const auto gt = a.gt[c];
+        // new cell c' = c - move of the next frame
+        const int ni = i - a.move_next[0], nj = j - a.move_next[1], nk = k - a.move_next[2];
+        const bool next_in = a.lam_next_f && ni >= 0 && ni < gs && nj >= 0 && nj < gs && nk >= 0 && nk < gsz;
+        const int cn = next_in ? ni + gs * nj + gs * gs * nk : c;
+        double nf = 0.0, ns = 0.0, nd = 0.0;
+        if (a.lam_next_f) {
+            nf = a.lam_next_f[cn];
+            ns = a.lam_next_s[cn];
+            nd = a.lam_next_d[cn];
+        }
+        // masses before this frame: the previous frame's, moved by this frame's shift, or priors
+        const int si = i + a.move[0], sj = j + a.move[1], sk = k + a.move[2];
+        const bool prev_in = a.prev.f && si >= 0 && si < gs && sj >= 0 && sj < gs && sk >= 0 && sk < gsz;
+        const int cs = prev_in ? si + gs * sj + gs * gs * sk : c;
+        double pf = a.prior_f, ps = a.prior_s, pd = a.prior_d;
+        if (a.prev.f) {
+            const float xf = a.prev.f[cs], xs = a.prev.s[cs], xd = a.prev.d[cs];
+            if (prev_in) {
+                pf = xf;
+                ps = xs;
+                pd = xd;
+            }
+        }
+        const bool prior_cell = !prev_in;
         double lf = 0.0, ls_ = 0.0, ld = 0.0;
-        if (!a.mask || a.mask[c]) {
+        if (use) {
             typedef K3Dual<double, 3> D3;
-            const D3 bce = k3g_bce_occ<D3, double>(k3d_var<double, 3>(a.post.f[c], 0),
-                                                   k3d_var<double, 3>(a.post.s[c], 1),
-                                                   k3d_var<double, 3>(a.post.d[c], 2), a.gt[c]);
+            const D3 bce = k3g_bce_occ<D3, double>(k3d_var<double, 3>(post_f, 0), k3d_var<double, 3>(post_s, 1),
+                                                   k3d_var<double, 3>(post_d, 2), gt);
             lf = a.loss_scale * bce.d[0];
             ls_ = a.loss_scale * bce.d[1];
             ld = a.loss_scale * bce.d[2];
         }
-        if (a.lam_next_f) {  // new cell c' = c - move of the next frame
-            const int ni = i - a.move_next[0], nj = j - a.move_next[1], nk = k - a.move_next[2];
-            if (ni >= 0 && ni < gs && nj >= 0 && nj < gs && nk >= 0 && nk < gsz) {
-                const int cn = ni + gs * nj + gs * gs * nk;
-                lf += a.lam_next_f[cn];
-                ls_ += a.lam_next_s[cn];
-                ld += a.lam_next_d[cn];
-            }
-        }
-        // masses before this frame: the previous frame's, moved by this frame's shift, or priors
-        double pf = a.prior_f, ps = a.prior_s, pd = a.prior_d;
-        bool prior_cell = true;
-        if (a.prev.f) {
-            const int si = i + a.move[0], sj = j + a.move[1], sk = k + a.move[2];
-            if (si >= 0 && si < gs && sj >= 0 && sj < gs && sk >= 0 && sk < gsz) {
-                const int cs = si + gs * sj + gs * gs * sk;
-                pf = a.prev.f[cs];
-                ps = a.prev.s[cs];
-                pd = a.prev.d[cs];
-                prior_cell = false;
-            }
+        if (next_in) {
+            lf += nf;
+            ls_ += ns;
+            ld += nd;
         }
         typedef K3Dual<double, 6> D6;
         const double del_o = a.del_occ[c], del_f = a.del_free[c];
