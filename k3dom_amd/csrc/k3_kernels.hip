@@ -594,15 +594,17 @@ __global__ __launch_bounds__(TPB) void k_gather(const float4* __restrict__ prec,
                                                 int32_t* __restrict__ end) {
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= n) return;
+    // (the neighbours' keys loaded with the others, before any store: a wait
+    // for a load issued after the stores would also wait for the stores)
     const uint32_t j = perm[i];
+    const uint32_t k = skeys[i], kprev = skeys[max(i - 1, 0)], knext = skeys[min(i + 1, n - 1)];
     const float4 a = prec[2 * j], b = prec[2 * j + 1];
     dst.px[i] = a.x; dst.py[i] = a.y; dst.pz[i] = a.z;
     dst.vx[i] = a.w; dst.vy[i] = b.x; dst.vz[i] = b.y;
     wsorted[i] = b.z;
-    const uint32_t k = skeys[i];
     dst.key[i] = (int32_t)k;
-    if (i == 0 || skeys[i - 1] != k) start[k] = i;
-    if (i == n - 1 || skeys[i + 1] != k) end[k] = i;
+    if (i == 0 || kprev != k) start[k] = i;
+    if (i == n - 1 || knext != k) end[k] = i;
 }
 hipError_t launch_gather_sorted(hipStream_t st, const float4* prec, ParticleArrays dst, float* wsorted,
                                 const uint32_t* perm, const uint32_t* sorted_keys, int n, int32_t* start,
@@ -664,11 +666,13 @@ __device__ __forceinline__ float lane_value(float v, int q) {
 
 // First unit start at or after position q of voxel k (k holds q); one past
 // the voxel's segment when no unit of it starts there.
-__device__ __forceinline__ int first_unit_start(int q, int k, const int32_t* __restrict__ start,
-                                                const int32_t* __restrict__ end) {
-    const int s0 = start[k], e0 = end[k];
+__device__ __forceinline__ int first_unit_start_in(int q, int s0, int e0) {
     const int u = q + (K3_SEG_BLOCK - (q - s0) % K3_SEG_BLOCK) % K3_SEG_BLOCK;
     return u > e0 ? e0 + 1 : u;
+}
+__device__ __forceinline__ int first_unit_start(int q, int k, const int32_t* __restrict__ start,
+                                                const int32_t* __restrict__ end) {
+    return first_unit_start_in(q, start[k], end[k]);
 }
 
 // End (exclusive, local) of the unit starting at local position u: the first
@@ -1427,10 +1431,15 @@ __global__ __launch_bounds__(UNIT_TPB) void k_unit_persist(const uint32_t* __res
     // All loads first, in two rounds (the particles, then their voxels),
     // unconditional with clamped indices, instead of two dependent round
     // trips per 64 positions.
+    // The range of the voxel at this lane's first unit position (r0 + q0) is
+    // loaded in the second round too: loaded after the P.w stores, its wait
+    // would also wait for them.
     constexpr int NK = UNIT_SPAN / 64;
     int kk[NK];
     float px_[NK], py_[NK], pz_[NK], pw_[NK], pe_[NK], pr_[NK];
+    int q_start = 0, q_end = 0;
     if (n > 0) {
+        int kq;
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
             const int i = min(r0 + k * 64 + lane, n - 1);
@@ -1438,11 +1447,14 @@ __global__ __launch_bounds__(UNIT_TPB) void k_unit_persist(const uint32_t* __res
             px_[k] = P.vx[i]; py_[k] = P.vy[i]; pz_[k] = P.vz[i];
             pw_[k] = wsrc[i];
         }
+        kq = P.key[min(r0 + UNIT_LANE * lane, n - 1)];
 #pragma unroll
         for (int k = 0; k < NK; ++k) {
             pe_[k] = v.pers[kk[k]];
             pr_[k] = v.pred[kk[k]];
         }
+        q_start = v.start[kq];
+        q_end = v.end[kq];
     }
 #pragma unroll
     for (int k = 0; k < NK; ++k) {
@@ -1467,7 +1479,7 @@ __global__ __launch_bounds__(UNIT_TPB) void k_unit_persist(const uint32_t* __res
     block_max_store(wmax, joint_parts + blockIdx.x);
     const int q0 = UNIT_LANE * lane, q1 = min(q0 + UNIT_LANE, m);
     if (q0 >= q1) return;
-    for (int u = first_unit_start(r0 + q0, lk[q0], v.start, v.end) - r0; u < q1;) {
+    for (int u = first_unit_start_in(r0 + q0, q_start, q_end) - r0; u < q1;) {
         const int e = unit_end(lk, u, m);
         float sx = 0.0f, sy = 0.0f, sz = 0.0f;
         int i = u;
