@@ -214,11 +214,6 @@ __global__ __launch_bounds__(TPB) void k_radix_hist(const uint32_t* __restrict__
     __shared__ uint32_t h[256];
     const uint32_t dmask = (1u << db) - 1u;
     const int t = threadIdx.x;
-    if (blockIdx.x * 256 + t < nclear) clear[blockIdx.x * 256 + t] = 0u;
-    for (long long i = (long long)blockIdx.x * 256 + t; i < nfill; i += (long long)gridDim.x * 256) {
-        fa[i] = -1;
-        fb[i] = -1;
-    }
     h[t] = 0;
     __syncthreads();
     const int base = blockIdx.x * RS_TILE;
@@ -233,6 +228,12 @@ __global__ __launch_bounds__(TPB) void k_radix_hist(const uint32_t* __restrict__
         const uint32_t v = h[t];
         cnt[(size_t)blockIdx.x * 256 + t] = v;
         if (v) atomicAdd(&grp[(size_t)(blockIdx.x / S) * 256 + t], v);
+    }
+    // (the clears last: a load's wait issued after stores also waits for them)
+    if (blockIdx.x * 256 + t < nclear) clear[blockIdx.x * 256 + t] = 0u;
+    for (long long i = (long long)blockIdx.x * 256 + t; i < nfill; i += (long long)gridDim.x * 256) {
+        fa[i] = -1;
+        fb[i] = -1;
     }
 }
 
@@ -1735,21 +1736,24 @@ __global__ __launch_bounds__(TPB) void k_bounds(const uint32_t* __restrict__ abo
     if (*abort_flag) return;
     int j = blockIdx.x * TPB + threadIdx.x;
     if (j >= V) return;
+    // (every load issued before the first store, unconditionally)
     const uint64_t tot = *total;
+    const uint64_t cj = cdf[j], cp = cdf[max(j - 1, 0)];
+    const float bornj = wua ? born[j] : 0.0f;
     uint32_t bj, bp;
     if (tot == 0) {
         bj = (uint32_t)(((uint64_t)(j + 1) * cnt) / (uint64_t)V);
         bp = (uint32_t)(((uint64_t)j * cnt) / (uint64_t)V);
     } else {
         const double scale = (double)cnt / (double)tot;
-        bj = slot_bound(cdf[j], tot, cnt, scale);
-        bp = j ? slot_bound(cdf[j - 1], tot, cnt, scale) : 0u;
+        bj = slot_bound(cj, tot, cnt, scale);
+        bp = j ? slot_bound(cp, tot, cnt, scale) : 0u;
     }
     bnd[j] = bj;
     if (wua) {
         const float pA = 0.0f;
         int nn = (int)(bj - bp);
-        wua[j] = nn > 0 ? (float)(((1.0 - (double)pA) * (double)born[j]) / (double)nn) : 0.0f;
+        wua[j] = nn > 0 ? (float)(((1.0 - (double)pA) * (double)bornj) / (double)nn) : 0.0f;
     }
 }
 hipError_t launch_bounds(hipStream_t st, const uint32_t* abort_flag, const uint64_t* cdf, const uint64_t* total, int V, uint32_t cnt,
