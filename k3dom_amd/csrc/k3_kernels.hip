@@ -1307,9 +1307,10 @@ __global__ __launch_bounds__(TPB) void k_ds_mass_update(const uint32_t* __restri
     if (*abort_flag) return;
     const int lane = threadIdx.x & 63;
     const int c = blockIdx.x * TPB + threadIdx.x;
-    // (the range's two words in one round trip: unconditional, clamped)
+    // (the voxel's six words in one round trip: unconditional, clamped)
     const int cc = min(c, V - 1);
     const int st0 = v.start[cc], en0 = v.end[cc];
+    const float vf = v.f[cc], vs = v.s[cc], mocc = meas_occ[cc], mfree = meas_free[cc];
     int st = -1, nb = 0;
     if (c < V) {
         st = st0;
@@ -1329,13 +1330,13 @@ __global__ __launch_bounds__(TPB) void k_ds_mass_update(const uint32_t* __restri
             m_occ_pred = sum;
             norm[c] = sum > 1.0f ? sum : 0.0f;
             if (sum > 1.0f) m_occ_pred = 1.0f;
-            const float m_free_pred = k3ds_predict_free(v.f[c], m_occ_pred);
-            k3ds_update_masses(m_occ_pred, m_free_pred, meas_occ[c], meas_free[c], &occ_up, &free_up);
+            const float m_free_pred = k3ds_predict_free(vf, m_occ_pred);
+            k3ds_update_masses(m_occ_pred, m_free_pred, mocc, mfree, &occ_up, &free_up);
             rho = k3ds_newborn_part(m_occ_pred, occ_up, p_B);
         } else {
-            const float m_occ = v.s[c];
-            const float m_free = k3ds_predict_free(v.f[c], m_occ);
-            k3ds_update_masses(m_occ, m_free, meas_occ[c], meas_free[c], &occ_up, &free_up);
+            const float m_occ = vs;
+            const float m_free = k3ds_predict_free(vf, m_occ);
+            k3ds_update_masses(m_occ, m_free, mocc, mfree, &occ_up, &free_up);
         }
         born[c] = rho;
         v.pers[c] = occ_up - rho;
